@@ -1,0 +1,89 @@
+"""ctypes bindings used by the tests to call any library exporting the reference's
+mrrr() prototype (INCLUDE/mrrr.h:62-65): the HIP product, the CPU oracle, and the
+compiled reference.  Also holds the accuracy metrics used across the test-suite.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_SO = os.path.join(REPO, "oracle", "liboracle.so")
+REF_SO = os.path.join(REPO, "oracle", "_ref", "libmrrr_ref.so")
+
+_P = ctypes.c_void_p
+
+
+def _ptr(a):
+    return a.ctypes.data_as(_P) if a is not None else None
+
+
+def load(path, symbol):
+    lib = ctypes.CDLL(path)
+    fn = getattr(lib, symbol)
+    fn.restype = ctypes.c_int
+    fn.argtypes = [ctypes.c_char_p, ctypes.c_char_p] + [_P] * 13
+    return lib, fn
+
+
+def call_mrrr(fn, jobz, rng, D, E, vl=0.0, vu=0.0, il=1, iu=1, tryrac=1,
+              want_z=True, mcols=None):
+    """Calls an mrrr()-compatible function; returns a dict of outputs."""
+    n = len(D)
+    D = np.ascontiguousarray(D, dtype=np.float64).copy()
+    E = np.ascontiguousarray(E, dtype=np.float64).copy()
+    if len(E) < n:
+        E = np.concatenate([E, np.zeros(n - len(E))])
+    W = np.zeros(max(n, 1))
+    if mcols is None:
+        mcols = n
+    Z = np.zeros((max(mcols, 1), n)) if (want_z and jobz.upper() == "V") else None
+    Zs = np.zeros(2 * max(n, 1), dtype=np.int32)
+    ni = ctypes.c_int(n)
+    m = ctypes.c_int(0)
+    ldz = ctypes.c_int(n)
+    cvl = ctypes.c_double(vl)
+    cvu = ctypes.c_double(vu)
+    cil = ctypes.c_int(il)
+    ciu = ctypes.c_int(iu)
+    tr = ctypes.c_int(tryrac)
+    info = fn(jobz.encode(), rng.encode(), ctypes.byref(ni), _ptr(D), _ptr(E),
+              ctypes.byref(cvl), ctypes.byref(cvu), ctypes.byref(cil), ctypes.byref(ciu),
+              ctypes.byref(tr), ctypes.byref(m), _ptr(W), _ptr(Z), ctypes.byref(ldz), _ptr(Zs))
+    mm = m.value
+    out = dict(info=info, m=mm, W=W[:mm].copy(), Zsupp=Zs[:2 * mm].copy(),
+               vl=cvl.value, vu=cvu.value, il=cil.value, iu=ciu.value, tryrac=tr.value)
+    out["Z"] = Z[:mm].T.copy() if Z is not None else None
+    return out
+
+
+def tnorm(D, E):
+    n = len(D)
+    T1 = np.abs(D).copy()
+    T1[:-1] += np.abs(E[: n - 1])
+    T1[1:] += np.abs(E[: n - 1])
+    return float(T1.max()) if n else 0.0
+
+
+def tz(D, E, Z):
+    """T @ Z for a tridiagonal T, Z of shape (n, m)."""
+    n = len(D)
+    out = D[:, None] * Z
+    if n > 1:
+        e = E[: n - 1][:, None]
+        out[:-1] += e * Z[1:]
+        out[1:] += e * Z[:-1]
+    return out
+
+
+def resid_orth(D, E, W, Z):
+    """resid = max|TZ - ZW| / (n eps ||T||_inf), orth = max|Z^T Z - I| / (n eps)."""
+    n = len(D)
+    eps = np.finfo(np.float64).eps
+    tn = max(tnorm(D, E), np.finfo(np.float64).tiny)
+    R = tz(D, E, Z) - Z * W[None, :]
+    resid = float(np.abs(R).max()) / (n * eps * tn) if Z.size else 0.0
+    G = Z.T @ Z
+    G[np.diag_indices_from(G)] -= 1.0
+    orth = float(np.abs(G).max()) / (n * eps) if Z.size else 0.0
+    return resid, orth
