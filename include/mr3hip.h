@@ -1,0 +1,115 @@
+/*
+ * mr3hip.h -- public C ABI of the MI355X MRRR tridiagonal eigensolver (libmr3hip.so).
+ *
+ * Plain pointers and sizes only; every entry point is extern "C".
+ */
+#ifndef MR3HIP_H
+#define MR3HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/*
+ * Drop-in replacement of the reference routine
+ *   int mrrr(char *jobz, char *range, int *n, double *D, double *E, double *vl,
+ *            double *vu, int *il, int *iu, int *tryrac, int *m, double *W,
+ *            double *Z, int *ldz, int *Zsupp);
+ * (HPAC/mr3smp INCLUDE/mrrr.h:62-65, SRC/mrrr.c:87-296).  Same argument meaning:
+ * jobz "N"/"V"/"C", range "A"/"V"/"I" (first character, any case), host arrays,
+ * Z column-major with leading dimension ldz, Zsupp 1-based supports.  Returns 0 on
+ * success, 1 for an invalid argument (mrrr.c:118-126), 2 when the device or the
+ * algorithm fails (the reference only asserts there).  On return D and E hold no
+ * defined values, as in the reference.  Safe to call from several host threads.
+ */
+int mrrr(char *jobz, char *range, int *n, double *D, double *E, double *vl,
+         double *vu, int *il, int *iu, int *tryrac, int *m, double *W, double *Z,
+         int *ldz, int *Zsupp);
+
+/* Fortran binding of the same routine (SRC/mrrr.c:630-637): info receives the
+ * return value. */
+void mrrr_(char *jobz, char *range, int *n, double *D, double *E, double *vl,
+           double *vu, int *il, int *iu, int *tryrac, int *m, double *W, double *Z,
+           int *ldz, int *Zsupp, int *info);
+
+/*
+ * One problem of a batched solve.  All array pointers are DEVICE pointers on the
+ * current HIP device; D and E are read only.  Inputs: n, jobz ('N', 'V' or 'C'),
+ * range ('A', 'V' or 'I'), vl/vu or il/iu as for mrrr(), tryrac, ldz and mcols
+ * (columns available in Z).  Outputs: W (n entries), Z (ldz x m, column-major),
+ * Zsupp (2m ints), and the scalar fields m, info, vl_out, vu_out, il_out,
+ * iu_out, tryrac_out.  Counterpart of one mrrr() call (mrrr.h:62-65); it has no
+ * batched twin in the reference, whose callers loop over mrrr().
+ */
+typedef struct mr3_problem {
+    int n;
+    char jobz, range;
+    char pad0, pad1;
+    double vl, vu;
+    int il, iu;
+    int tryrac;
+    int ldz, mcols;
+    const double *D, *E;
+    double *W, *Z;
+    int *Zsupp;
+    /* outputs */
+    int m, info;
+    double vl_out, vu_out;
+    int il_out, iu_out, tryrac_out;
+} mr3_problem;
+
+/*
+ * Solves nprob independent problems in one pass of the device pipeline on the
+ * given HIP stream (NULL = the library's per-thread stream).  Returns 0 if every
+ * problem succeeded, 1 if any had invalid arguments, 2 on a device failure; the
+ * per-problem result is in probs[i].info.  The call returns when the results are
+ * in device memory and the scalar outputs are filled in.
+ */
+int mr3_solve_batch(mr3_problem *probs, int nprob, void *stream);
+
+/* The single-problem device path: mr3_solve_batch with nprob = 1. */
+int mr3_solve_one(mr3_problem *prob, void *stream);
+
+/*
+ * Serving: a request loop that admits problems as device memory and concurrency
+ * slots free up, runs them on several streams at once and reports completion.
+ * budget_bytes bounds the device memory of admitted problems (8 n m for Z plus the
+ * per-problem workspace); max_inflight bounds the number of batches on the device.
+ * Problems use host arrays; the server copies them in and results out.
+ */
+typedef struct mr3_server mr3_server;
+
+typedef struct mr3_request {
+    int n;
+    char jobz, range;
+    char pad0, pad1;
+    double vl, vu;
+    int il, iu, tryrac;
+    const double *D, *E;   /* host inputs, must stay valid until completion   */
+    double *W;             /* host output, n entries                           */
+    double *Z;             /* host output, n x (columns) with ldz = n, or NULL */
+    int *Zsupp;            /* host output, 2n ints, or NULL                    */
+    int mcols;             /* columns available in Z                           */
+    /* outputs */
+    int m, info;
+    double t_submit, t_start, t_done;   /* seconds, steady clock */
+} mr3_request;
+
+mr3_server *mr3_server_create(long long budget_bytes, int max_inflight, int max_batch);
+/* Queues a request; returns its id (>= 0) or -1 if it can never fit the budget. */
+long long mr3_server_submit(mr3_server *s, mr3_request *req);
+/* Blocks until request id is complete; returns its info. */
+int mr3_server_wait(mr3_server *s, long long id);
+/* Blocks until every submitted request is complete. */
+void mr3_server_drain(mr3_server *s);
+void mr3_server_destroy(mr3_server *s);
+/* Monotonic clock used for the request timestamps. */
+double mr3_now(void);
+
+/* Library version string. */
+const char *mr3_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,224 @@
+"""mr3smp_amd -- MI355X MRRR tridiagonal eigensolver.
+
+The compute path is the HIP library ``libmr3hip.so`` built from ``csrc/`` for gfx950;
+this module is a thin ctypes layer over its C ABI (``include/mr3hip.h``):
+
+* :func:`mrrr` -- the drop-in routine with the reference's argument meaning
+  (HPAC/mr3smp ``INCLUDE/mrrr.h:62-65``), on host numpy arrays;
+* :func:`solve_batch` -- many problems in one pass of the device pipeline, on
+  device tensors that stay in HBM;
+* :class:`Server` -- the continuous-batching request loop.
+
+There is no CPU fallback: if the library is missing or no HIP device is present the
+calls raise.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+__all__ = ["lib", "mrrr", "solve_batch", "Problem", "Server", "Request", "build", "LIB_PATH"]
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmr3hip.so")
+_lib = None
+_P = ctypes.c_void_p
+
+
+class Problem(ctypes.Structure):
+    """Mirror of ``mr3_problem`` (include/mr3hip.h)."""
+    _fields_ = [
+        ("n", ctypes.c_int), ("jobz", ctypes.c_char), ("range", ctypes.c_char),
+        ("pad0", ctypes.c_char), ("pad1", ctypes.c_char),
+        ("vl", ctypes.c_double), ("vu", ctypes.c_double),
+        ("il", ctypes.c_int), ("iu", ctypes.c_int), ("tryrac", ctypes.c_int),
+        ("ldz", ctypes.c_int), ("mcols", ctypes.c_int),
+        ("D", _P), ("E", _P), ("W", _P), ("Z", _P), ("Zsupp", _P),
+        ("m", ctypes.c_int), ("info", ctypes.c_int),
+        ("vl_out", ctypes.c_double), ("vu_out", ctypes.c_double),
+        ("il_out", ctypes.c_int), ("iu_out", ctypes.c_int), ("tryrac_out", ctypes.c_int),
+    ]
+
+
+class Request(ctypes.Structure):
+    """Mirror of ``mr3_request`` (include/mr3hip.h)."""
+    _fields_ = [
+        ("n", ctypes.c_int), ("jobz", ctypes.c_char), ("range", ctypes.c_char),
+        ("pad0", ctypes.c_char), ("pad1", ctypes.c_char),
+        ("vl", ctypes.c_double), ("vu", ctypes.c_double),
+        ("il", ctypes.c_int), ("iu", ctypes.c_int), ("tryrac", ctypes.c_int),
+        ("D", _P), ("E", _P), ("W", _P), ("Z", _P), ("Zsupp", _P),
+        ("mcols", ctypes.c_int),
+        ("m", ctypes.c_int), ("info", ctypes.c_int),
+        ("t_submit", ctypes.c_double), ("t_start", ctypes.c_double), ("t_done", ctypes.c_double),
+    ]
+
+
+def build():
+    """Compiles libmr3hip.so for gfx950 in place (hipcc)."""
+    import subprocess
+    subprocess.run(["bash", os.path.join(HERE, "csrc", "build.sh")], check=True)
+
+
+def lib():
+    """Loads libmr3hip.so; raises if it is not built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built; run mr3smp_amd.build()")
+        L = ctypes.CDLL(LIB_PATH)
+        L.mrrr.restype = ctypes.c_int
+        L.mrrr.argtypes = [ctypes.c_char_p, ctypes.c_char_p] + [_P] * 13
+        L.mr3_solve_batch.restype = ctypes.c_int
+        L.mr3_solve_batch.argtypes = [ctypes.POINTER(Problem), ctypes.c_int, _P]
+        L.mr3_solve_one.restype = ctypes.c_int
+        L.mr3_solve_one.argtypes = [ctypes.POINTER(Problem), _P]
+        L.mr3_server_create.restype = _P
+        L.mr3_server_create.argtypes = [ctypes.c_longlong, ctypes.c_int, ctypes.c_int]
+        L.mr3_server_submit.restype = ctypes.c_longlong
+        L.mr3_server_submit.argtypes = [_P, ctypes.POINTER(Request)]
+        L.mr3_server_wait.restype = ctypes.c_int
+        L.mr3_server_wait.argtypes = [_P, ctypes.c_longlong]
+        L.mr3_server_drain.restype = None
+        L.mr3_server_drain.argtypes = [_P]
+        L.mr3_server_destroy.restype = None
+        L.mr3_server_destroy.argtypes = [_P]
+        L.mr3_now.restype = ctypes.c_double
+        L.mr3_version.restype = ctypes.c_char_p
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    return a.ctypes.data_as(_P) if a is not None else None
+
+
+def mrrr(jobz, rng, D, E, vl=0.0, vu=0.0, il=1, iu=1, tryrac=1, ldz=None, mcols=None):
+    """The drop-in routine on host arrays.
+
+    Returns a dict with info, m, W[m], Z (n x m or None), Zsupp[2m] and the in/out
+    scalars vl, vu, il, iu, tryrac.  D and E are copied, never modified.
+    """
+    L = lib()
+    n = int(len(D))
+    D = np.ascontiguousarray(D, dtype=np.float64).copy()
+    E = np.ascontiguousarray(E, dtype=np.float64).copy()
+    if len(E) < n:
+        E = np.concatenate([E, np.zeros(n - len(E))])
+    ldz = n if ldz is None else int(ldz)
+    if mcols is None:
+        mcols = n
+    wantz = jobz[0].upper() == "V"
+    W = np.zeros(max(n, 1))
+    Z = np.zeros((max(mcols, 1), max(ldz, 1))) if wantz else None
+    Zs = np.zeros(2 * max(n, 1), dtype=np.int32)
+    ni, m, cldz = ctypes.c_int(n), ctypes.c_int(0), ctypes.c_int(ldz)
+    cvl, cvu = ctypes.c_double(vl), ctypes.c_double(vu)
+    cil, ciu, ctr = ctypes.c_int(il), ctypes.c_int(iu), ctypes.c_int(tryrac)
+    info = L.mrrr(jobz.encode(), rng.encode(), ctypes.byref(ni), _ptr(D), _ptr(E),
+                  ctypes.byref(cvl), ctypes.byref(cvu), ctypes.byref(cil), ctypes.byref(ciu),
+                  ctypes.byref(ctr), ctypes.byref(m), _ptr(W), _ptr(Z), ctypes.byref(cldz),
+                  _ptr(Zs))
+    mm = m.value
+    out = dict(info=info, m=mm, W=W[:mm].copy(), Zsupp=Zs[:2 * mm].copy(), vl=cvl.value,
+               vu=cvu.value, il=cil.value, iu=ciu.value, tryrac=ctr.value)
+    out["Z"] = Z[:mm, :n].T.copy() if wantz else None
+    return out
+
+
+def solve_batch(problems, stream=None):
+    """Solves problems given as dicts with device tensors in one pipeline pass.
+
+    Each dict holds ``D``, ``E`` (float64 tensors on the HIP device), ``jobz``,
+    ``range`` and optional ``vl``, ``vu``, ``il``, ``iu``, ``tryrac``, ``W``, ``Z``
+    (float64, shape (mcols, ldz): column j of the math is row j of the tensor),
+    ``Zsupp`` (int32, 2*mcols).  Missing outputs are allocated.  Returns the
+    Problem array (scalar outputs) and the list of dicts with the outputs filled.
+    """
+    import torch
+    L = lib()
+    arr = (Problem * len(problems))()
+    keep = []
+    for i, p in enumerate(problems):
+        D, E = p["D"], p["E"]
+        n = int(D.numel())
+        dev = D.device
+        jobz = p.get("jobz", "V")
+        wantz = jobz.upper() == "V"
+        mcols = int(p.get("mcols", n))
+        if "W" not in p:
+            p["W"] = torch.empty(max(n, 1), dtype=torch.float64, device=dev)
+        if wantz and "Z" not in p:
+            p["Z"] = torch.empty((max(mcols, 1), n), dtype=torch.float64, device=dev)
+        if wantz and "Zsupp" not in p:
+            p["Zsupp"] = torch.empty(2 * max(mcols, 1), dtype=torch.int32, device=dev)
+        q = arr[i]
+        q.n = n
+        q.jobz = jobz[0].encode()
+        q.range = p.get("range", "A")[0].encode()
+        q.vl, q.vu = float(p.get("vl", 0.0)), float(p.get("vu", 0.0))
+        q.il, q.iu = int(p.get("il", 1)), int(p.get("iu", 1))
+        q.tryrac = int(p.get("tryrac", 1))
+        q.ldz = int(p["Z"].shape[1]) if wantz else n
+        q.mcols = mcols if wantz else 0
+        q.D, q.E = D.data_ptr(), E.data_ptr()
+        q.W = p["W"].data_ptr()
+        q.Z = p["Z"].data_ptr() if wantz else None
+        q.Zsupp = p["Zsupp"].data_ptr() if wantz else None
+        keep.append(p)
+    st = None if stream is None else ctypes.c_void_p(stream.cuda_stream)
+    rc = L.mr3_solve_batch(arr, len(problems), st)
+    return rc, arr, keep
+
+
+class Server:
+    """Continuous-batching request loop over host arrays (mr3_server_* in the C ABI)."""
+
+    def __init__(self, budget_bytes=200 << 30, max_inflight=2, max_batch=256):
+        self.L = lib()
+        self.h = self.L.mr3_server_create(int(budget_bytes), int(max_inflight), int(max_batch))
+        self._keep = {}
+
+    def submit(self, jobz, rng, D, E, vl=0.0, vu=0.0, il=1, iu=1, tryrac=1, mcols=None):
+        n = int(len(D))
+        D = np.ascontiguousarray(D, dtype=np.float64)
+        E = np.ascontiguousarray(E, dtype=np.float64)
+        wantz = jobz[0].upper() == "V"
+        mc = n if mcols is None else int(mcols)
+        req = Request()
+        req.n = n
+        req.jobz, req.range = jobz[0].encode(), rng[0].encode()
+        req.vl, req.vu, req.il, req.iu, req.tryrac = vl, vu, il, iu, tryrac
+        W = np.zeros(max(n, 1))
+        Z = np.zeros((max(mc, 1), n)) if wantz else None
+        Zs = np.zeros(2 * max(mc, 1), dtype=np.int32) if wantz else None
+        req.D, req.E, req.W = _ptr(D), _ptr(E), _ptr(W)
+        req.Z, req.Zsupp = (_ptr(Z), _ptr(Zs)) if wantz else (None, None)
+        req.mcols = mc if wantz else 0
+        rid = self.L.mr3_server_submit(self.h, ctypes.byref(req))
+        if rid < 0:
+            raise ValueError("request can never fit the memory budget")
+        self._keep[rid] = (req, D, E, W, Z, Zs)
+        return rid
+
+    def wait(self, rid):
+        info = self.L.mr3_server_wait(self.h, rid)
+        req, D, E, W, Z, Zs = self._keep.pop(rid)
+        m = req.m
+        return dict(info=info, m=m, W=W[:m].copy(), Z=None if Z is None else Z[:m].T.copy(),
+                    Zsupp=None if Zs is None else Zs[:2 * m].copy(),
+                    t_submit=req.t_submit, t_start=req.t_start, t_done=req.t_done)
+
+    def drain(self):
+        self.L.mr3_server_drain(self.h)
+
+    def close(self):
+        if self.h:
+            self.L.mr3_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

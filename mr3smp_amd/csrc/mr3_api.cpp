@@ -1,0 +1,185 @@
+// mr3_api.cpp -- the C-ABI entry points of libmr3hip.so (declared in include/mr3hip.h).
+//
+// mrrr() keeps the reference prototype and argument semantics (HPAC/mr3smp
+// INCLUDE/mrrr.h:62-65, SRC/mrrr.c:87-138): it validates its arguments exactly as the
+// reference does, answers the jobz="C" query for range A/I on the host, and otherwise
+// copies T to the device, runs the batched pipeline on one problem and copies W, Z
+// and Zsupp back.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "../../include/mr3hip.h"
+#include "mr3_internal.h"
+
+namespace mr3 {
+int run_pipeline(std::vector<MatDesc>& md, std::vector<MatState>& ms, hipStream_t st);
+
+static hipStream_t thread_stream() {
+    thread_local hipStream_t s = nullptr;
+    thread_local bool tried = false;
+    if (!tried) {
+        tried = true;
+        if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) s = nullptr;
+    }
+    return s;
+}
+
+static int upper(char c) { return (c >= 'a' && c <= 'z') ? c - 32 : c; }
+
+// Argument checks of mrrr.c:118-126.  Returns 0 if valid.
+static int check_args(int jz, int rg, int n, double vl, double vu, int il, int iu) {
+    if (!(jz == 'N' || jz == 'V' || jz == 'C')) return 1;
+    if (!(rg == 'A' || rg == 'V' || rg == 'I')) return 1;
+    if (n <= 0) return 1;
+    if (rg == 'V') { if (vu <= vl) return 1; }
+    else if (rg == 'I') { if (il < 1 || il > n || iu < il || iu > n) return 1; }
+    return 0;
+}
+
+static MatDesc to_desc(const mr3_problem& p) {
+    MatDesc d;
+    memset(&d, 0, sizeof(d));
+    d.Din = p.D; d.Ein = p.E; d.Wout = p.W; d.Z = p.Z; d.Zsupp = p.Zsupp;
+    d.vl = p.vl; d.vu = p.vu; d.n = p.n;
+    int jz = upper(p.jobz), rg = upper(p.range);
+    d.jobz = jz == 'V' ? kV : (jz == 'N' ? kN : kC);
+    d.range = rg == 'A' ? kAll : (rg == 'V' ? kVal : kInd);
+    d.il = p.il; d.iu = p.iu; d.tryrac = p.tryrac ? 1 : 0;
+    d.ldz = p.ldz > 0 ? p.ldz : p.n;
+    d.mcols = p.mcols;
+    return d;
+}
+
+}  // namespace mr3
+
+using namespace mr3;
+
+extern "C" int mr3_solve_batch(mr3_problem* probs, int nprob, void* stream) {
+    if (nprob <= 0) return 0;
+    hipStream_t st = stream ? (hipStream_t)stream : thread_stream();
+    std::vector<MatDesc> md;
+    std::vector<int> idx;
+    md.reserve(nprob);
+    int worst = 0;
+    for (int i = 0; i < nprob; ++i) {
+        mr3_problem& p = probs[i];
+        p.m = 0;
+        p.info = check_args(upper(p.jobz), upper(p.range), p.n, p.vl, p.vu, p.il, p.iu);
+        p.vl_out = p.vl; p.vu_out = p.vu; p.il_out = p.il; p.iu_out = p.iu;
+        p.tryrac_out = p.tryrac;
+        if (p.info) { worst = std::max(worst, 1); continue; }
+        const int jz = upper(p.jobz), rg = upper(p.range);
+        if (jz == 'C' && rg != 'V') {   // mrrr.c:130-138
+            p.m = (rg == 'A') ? p.n : p.iu - p.il + 1;
+            continue;
+        }
+        if (jz == 'V' && (p.Z == nullptr || p.Zsupp == nullptr || (p.ldz > 0 && p.ldz < p.n))) {
+            p.info = 1; worst = std::max(worst, 1); continue;
+        }
+        md.push_back(to_desc(p));
+        idx.push_back(i);
+    }
+    if (md.empty()) return worst;
+    std::vector<MatState> ms;
+    int rc = run_pipeline(md, ms, st);
+    if (rc != 0) {
+        for (int j : idx) probs[j].info = 2;
+        return 2;
+    }
+    for (size_t j = 0; j < idx.size(); ++j) {
+        mr3_problem& p = probs[idx[j]];
+        const MatState& S = ms[j];
+        p.m = S.m;
+        p.info = (S.status == kOk) ? 0 : 2;
+        p.vl_out = S.vl; p.vu_out = S.vu; p.il_out = S.il; p.iu_out = S.iu;
+        p.tryrac_out = S.tryrac;
+        if (p.jobz == 'V' || p.jobz == 'v') {
+            if (p.m > p.mcols && p.mcols > 0) p.info = 2;
+        }
+        if (p.info) worst = std::max(worst, p.info);
+    }
+    return worst;
+}
+
+extern "C" int mr3_solve_one(mr3_problem* prob, void* stream) {
+    return mr3_solve_batch(prob, 1, stream);
+}
+
+extern "C" int mrrr(char* jobz, char* range, int* np, double* D, double* E, double* vl,
+                    double* vu, int* il, int* iu, int* tryrac, int* m, double* W, double* Z,
+                    int* ldz, int* Zsupp) {
+    const int n = *np;
+    const int jz = upper(jobz[0]), rg = upper(range[0]);
+    if (check_args(jz, rg, n, rg == 'V' ? *vl : 0.0, rg == 'V' ? *vu : 0.0,
+                   rg == 'I' ? *il : 1, rg == 'I' ? *iu : 1))
+        return 1;
+    if (jz == 'C' && rg != 'V') {   // mrrr.c:130-138
+        *m = (rg == 'A') ? n : *iu - *il + 1;
+        return 0;
+    }
+    hipStream_t st = thread_stream();
+    if (!st) return 2;
+    const bool wantz = (jz == 'V');
+    const int mcols = wantz ? ((rg == 'I') ? (*iu - *il + 1) : n) : 0;
+    double *dD = nullptr, *dE = nullptr, *dW = nullptr, *dZ = nullptr;
+    int* dZs = nullptr;
+    int rc = 0;
+    auto fail = [&](int code) {
+        if (dD) (void)hipFreeAsync(dD, st);
+        if (dE) (void)hipFreeAsync(dE, st);
+        if (dW) (void)hipFreeAsync(dW, st);
+        if (dZ) (void)hipFreeAsync(dZ, st);
+        if (dZs) (void)hipFreeAsync(dZs, st);
+        (void)hipStreamSynchronize(st);
+        return code;
+    };
+    if (hipMallocAsync((void**)&dD, sizeof(double) * n, st) != hipSuccess) return fail(2);
+    if (hipMallocAsync((void**)&dE, sizeof(double) * n, st) != hipSuccess) return fail(2);
+    if (hipMallocAsync((void**)&dW, sizeof(double) * n, st) != hipSuccess) return fail(2);
+    if (wantz) {
+        if (hipMallocAsync((void**)&dZ, sizeof(double) * (size_t)n * mcols, st) != hipSuccess) return fail(2);
+        if (hipMallocAsync((void**)&dZs, sizeof(int) * 2 * (size_t)mcols, st) != hipSuccess) return fail(2);
+    }
+    if (hipMemcpyAsync(dD, D, sizeof(double) * n, hipMemcpyHostToDevice, st) != hipSuccess) return fail(2);
+    if (n > 1 && hipMemcpyAsync(dE, E, sizeof(double) * (n - 1), hipMemcpyHostToDevice, st) != hipSuccess)
+        return fail(2);
+    mr3_problem p;
+    memset(&p, 0, sizeof(p));
+    p.n = n; p.jobz = (char)jz; p.range = (char)rg;
+    p.vl = *vl; p.vu = *vu; p.il = *il; p.iu = *iu; p.tryrac = *tryrac;
+    p.ldz = n; p.mcols = mcols;
+    p.D = dD; p.E = dE; p.W = dW; p.Z = dZ; p.Zsupp = dZs;
+    rc = mr3_solve_batch(&p, 1, st);
+    if (rc != 0) return fail(rc == 1 ? 1 : 2);
+    *m = p.m;
+    *vl = p.vl_out; *vu = p.vu_out; *il = p.il_out; *iu = p.iu_out; *tryrac = p.tryrac_out;
+    if (jz != 'C' && p.m > 0) {
+        if (hipMemcpyAsync(W, dW, sizeof(double) * p.m, hipMemcpyDeviceToHost, st) != hipSuccess) return fail(2);
+        if (wantz) {
+            const int lz = *ldz;
+            if (hipMemcpy2DAsync(Z, sizeof(double) * (size_t)lz, dZ, sizeof(double) * (size_t)n,
+                                 sizeof(double) * (size_t)n, p.m, hipMemcpyDeviceToHost, st) != hipSuccess)
+                return fail(2);
+            if (hipMemcpyAsync(Zsupp, dZs, sizeof(int) * 2 * p.m, hipMemcpyDeviceToHost, st) != hipSuccess)
+                return fail(2);
+        }
+    }
+    return fail(0);
+}
+
+extern "C" void mrrr_(char* jobz, char* range, int* n, double* D, double* E, double* vl,
+                      double* vu, int* il, int* iu, int* tryrac, int* m, double* W, double* Z,
+                      int* ldz, int* Zsupp, int* info) {
+    *info = mrrr(jobz, range, n, D, E, vl, vu, il, iu, tryrac, m, W, Z, ldz, Zsupp);
+}
+
+extern "C" const char* mr3_version(void) { return "mr3hip 0.1 (gfx950)"; }
+
+extern "C" double mr3_now(void) {
+    using namespace std::chrono;
+    return duration<double>(steady_clock::now().time_since_epoch()).count();
+}

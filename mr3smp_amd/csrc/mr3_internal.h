@@ -1,0 +1,107 @@
+// mr3_internal.h -- records shared by the host driver and the HIP kernels.
+//
+// Layout in HBM (one launch = one batch of matrices; the drop-in mrrr() is a batch
+// of one):
+//   * per-row arrays are packed: matrix k owns rows [off_k, off_k + n_k) of D, E,
+//     E2, the root representation (D, L = E, DL, DLL), gersch (2 per row), and the
+//     eigenvalue slots W, Werr, Wgap, Wsh, Wcoarse, iblock, Windex, Zidx, smat;
+//   * block records of matrix k live at [off_k, off_k + nsplit_k) of the block array;
+//   * eigenvectors go straight into the caller's Z (column-major, ldz) per matrix.
+#pragma once
+#include <stdint.h>
+
+namespace mr3 {
+
+// Numerical constants of the reference (INCLUDE/mrrr.h:152-200, mrrr_val.c:95-99,
+// process_singleton.c:57, odrrf.f:113-119, odrrd.f:210-212).
+constexpr double kMaxGrowth = 64.0;
+constexpr double kMinRelgap = 1e-3;
+constexpr int kMaxTryRoot = 6;
+constexpr double kFudgeRoot = 2.0;
+constexpr double kRandFactor = 8.0;
+constexpr int kMaxIterRQI = 10;
+constexpr double kMaxGrowth1 = 8.0;
+constexpr double kMaxGrowth2 = 8.0;
+
+enum Range : int { kAll = 1, kVal = 2, kInd = 3 };
+enum Jobz : int { kN = 0, kV = 1, kC = 2 };
+
+// Status codes (per matrix).
+enum Status : int { kOk = 0, kBadInput = 1, kNoRoot = 2, kTooFew = 3 };
+
+struct MatDesc {            // host-filled, one per matrix
+    const double* Din;      // caller's diagonal (device)
+    const double* Ein;      // caller's off-diagonal (device), n-1 used
+    double* Wout;           // caller's eigenvalue output (device), n entries
+    double* Z;              // caller's eigenvectors (device), ldz x mcols, may be null
+    int* Zsupp;             // caller's support output (device), 2*mcols, may be null
+    double vl, vu;
+    int n, off;
+    int jobz, range;
+    int il, iu;
+    int tryrac, ldz;
+    int mcols, pad;
+};
+
+struct MatState {           // device-computed, one per matrix
+    double scale, pivmin, spdiam, tnorm;
+    double gl, gu;          // fudged Gershgorin bounds of T (mrrr_val.c:239-240)
+    double glraw, guraw;    // unfudged bounds
+    double wl, wu, wlu, wul;
+    double vl, vu;          // interval containing the computed eigenvalues (scaled)
+    double rtol1, rtol2, bsrtol, rqtol, split;
+    int il, iu, nvals, mtot, m, nsplit, nwl, nwu;
+    int tryrac, irange, status, done;
+    int maxbl, pad;
+};
+
+struct BlockRec {           // one per unreduced block
+    int bbeg, bend;         // rows, matrix-local
+    int wbeg, wend;         // eigenvalue slots, matrix-local (wend < wbeg: none)
+    int ilow, iupp;         // local index range wanted
+    int mat, pad;
+    double gl, gu;          // mrrr_val block bounds (mrrr_val.c:369-374)
+    double glraw, guraw;    // unfudged Gershgorin bounds of the block
+    double spdiam;          // spdiam returned by compute_root_rrr
+    double sigma;           // root shift
+};
+
+// A cluster of the representation tree at depth >= 1 (process_cluster.c cluster_t).
+struct ClTask {
+    int cb, ce;             // global slot range
+    int mat, depth;
+    int bbeg, bsize;        // block rows (matrix-local begin) and size
+    double bl_spdiam;
+    double lgap;
+    const double* pD;       // parent representation
+    const double* pL;
+    const double* pDL;
+    const double* pDLL;
+    double* cD;             // child representation (arena)
+    double* cL;
+    double* cDL;
+    double* cDLL;
+    double* cwork;          // 2*bsize scratch for the new-RRR search
+    long long arena_off;    // offset of this cluster's slot in the level arena
+    int eig_base, pad;      // first entry of this cluster in the level eigen list
+};
+
+// One eigenvector to compute (process_singleton.c singleton_t with begin == end).
+struct STask {
+    int slot, mat;
+    int bbeg, bsize;
+    double bl_spdiam, lgap;
+    const double* D;
+    const double* L;
+    const double* DL;
+    const double* DLL;
+};
+
+// Counters written by the classify kernels and read back by the host once per level.
+struct LevelCounters {
+    int nS, nC, nEig, maxbl;
+    long long arena;        // doubles needed by the next level's cluster slots
+    int err, pad;
+};
+
+}  // namespace mr3

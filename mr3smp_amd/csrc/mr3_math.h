@@ -1,0 +1,194 @@
+// mr3_math.h -- per-lane numerical kernels of the MRRR algorithm (device code).
+//
+// Every routine is a single-lane restatement of one vendored LAPACK kernel of the
+// reference; the GPU parallelism comes from running one lane per eigenvalue
+// interval or per eigenvector.  All arithmetic is binary64 and the library is built
+// with -ffp-contract=off, so no multiply-add is fused and the Sturm counts carry
+// the same roundings as the reference's x86-64 build.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <float.h>
+#include <math.h>
+
+namespace mr3 {
+
+constexpr double kEps = DBL_EPSILON;   // odmch.f 'P'
+constexpr double kSafmin = DBL_MIN;    // odmch.f 'S' / 'U'
+constexpr int kMaxExpand = 128;        // cap on bracket doublings (2^128 spans any range)
+
+// Sturm count of T - x I, dlaebz IJOB=1 convention (odebz.f:265-277): a pivot with
+// |t| < pivmin becomes -pivmin, a pivot <= 0 counts.
+__device__ __forceinline__ int count_T_ijob1(int n, const double* __restrict__ d,
+                                             const double* __restrict__ e2, double x,
+                                             double pivmin) {
+    double t = d[0] - x;
+    if (fabs(t) < pivmin) t = -pivmin;
+    int c = (t <= 0.0);
+    for (int j = 1; j < n; ++j) {
+        t = d[j] - e2[j - 1] / t - x;
+        if (fabs(t) < pivmin) t = -pivmin;
+        c += (t <= 0.0);
+    }
+    return c;
+}
+
+// Sturm count of T - x I, dlaebz IJOB=2/3 convention (odebz.f:395-415): a pivot
+// <= pivmin counts and is replaced by min(t, -pivmin).
+__device__ __forceinline__ int count_T_ijob2(int n, const double* __restrict__ d,
+                                             const double* __restrict__ e2, double x,
+                                             double pivmin) {
+    double t = d[0] - x;
+    int c = 0;
+    if (t <= pivmin) { c = 1; t = fmin(t, -pivmin); }
+    for (int j = 1; j < n; ++j) {
+        t = d[j] - e2[j - 1] / t - x;
+        if (t <= pivmin) { ++c; t = fmin(t, -pivmin); }
+    }
+    return c;
+}
+
+// dlaneg (odneg.f:97-164): negcount of L D L^T - sigma I at twist r (1-based).
+// The reference runs blocks of 128 steps and re-runs a block with Inf/Inf -> 1
+// when the block ends in NaN.  A NaN in T can only come from the quotient and then
+// persists to the end of the block, and the re-run differs from the fast loop only
+// at quotients that are NaN; substituting per step therefore gives the same count.
+__device__ __forceinline__ int dlaneg(int n, const double* __restrict__ d,
+                                      const double* __restrict__ lld, double sigma, int r) {
+    int neg = 0;
+    double t = -sigma;
+    for (int j = 0; j < r - 1; ++j) {
+        double dplus = d[j] + t;
+        neg += (dplus < 0.0);
+        double tmp = t / dplus;
+        tmp = isnan(tmp) ? 1.0 : tmp;
+        t = tmp * lld[j] - sigma;
+    }
+    double p = d[n - 1] - sigma;
+    for (int j = n - 2; j >= r - 1; --j) {
+        double dminus = lld[j] + p;
+        neg += (dminus < 0.0);
+        double tmp = p / dminus;
+        tmp = isnan(tmp) ? 1.0 : tmp;
+        p = tmp * d[j] - sigma;
+    }
+    double gamma = (t + sigma) + p;
+    neg += (gamma < 0.0);
+    return neg;
+}
+
+// dlarrb (odrrb.f:135-299) for one interval.  The reference keeps a linked list of
+// intervals but never lets them interact: each is bisected against its own index
+// with a gap taken from the (unchanging) WGAP input, so one lane per interval is
+// exact.  `gap` is min(left gap, right gap) as dlarrb forms it.  On return *w and
+// *werr hold the refined midpoint and half width, or are left unchanged when the
+// interval already met the tolerance (odrrb.f:198-205).
+__device__ __forceinline__ void dlarrb1(int n, const double* __restrict__ d,
+                                        const double* __restrict__ lld, int i, double rtol1,
+                                        double rtol2, double gap, double* w, double* werr,
+                                        double pivmin, double spdiam, int twist) {
+    const int maxitr = (int)((log(spdiam + pivmin) - log(pivmin)) / log(2.0)) + 2;
+    const double mnwdth = 2.0 * pivmin;
+    int r = twist;
+    if (r < 1 || r > n) r = n;
+    double left = *w - *werr, right = *w + *werr;
+    // The reference expands without a bound (odrrb.f:176-196); a zero error bound
+    // would never move the end point, so the expansion is capped here.
+    double back = *werr;
+    for (int g = 0; g < kMaxExpand && dlaneg(n, d, lld, left, r) > i - 1; ++g) {
+        left = left - back;
+        back = 2.0 * back;
+    }
+    back = *werr;
+    for (int g = 0; g < kMaxExpand && dlaneg(n, d, lld, right, r) < i; ++g) {
+        right = right + back;
+        back = 2.0 * back;
+    }
+    double width = 0.5 * fabs(left - right);
+    double tmp = fmax(fabs(left), fabs(right));
+    double cvrgd = fmax(rtol1 * gap, rtol2 * tmp);
+    if (width <= cvrgd || width <= mnwdth) return;
+    for (int iter = 0;; ++iter) {
+        double mid = 0.5 * (left + right);
+        width = right - mid;
+        tmp = fmax(fabs(left), fabs(right));
+        cvrgd = fmax(rtol1 * gap, rtol2 * tmp);
+        if (width <= cvrgd || width <= mnwdth || iter == maxitr) break;
+        int negcnt = dlaneg(n, d, lld, mid, r);
+        if (negcnt <= i - 1) left = mid;
+        else right = mid;
+    }
+    *w = 0.5 * (left + right);
+    *werr = right - *w;
+}
+
+// Sturm count on T for dlarrj (odrrj.f:160-166): no pivmin clamp, count if < 0.
+__device__ __forceinline__ int count_T_rrj(int n, const double* __restrict__ d,
+                                           const double* __restrict__ e2, double s) {
+    double dplus = d[0] - s;
+    int cnt = (dplus < 0.0);
+    for (int j = 1; j < n; ++j) {
+        dplus = d[j] - s - e2[j - 1] / dplus;
+        cnt += (dplus < 0.0);
+    }
+    return cnt;
+}
+
+// dlarrj (odrrj.f:100-281) for one interval (intervals are independent).
+__device__ __forceinline__ void dlarrj1(int n, const double* __restrict__ d,
+                                        const double* __restrict__ e2, int i, double rtol,
+                                        double* w, double* werr, double pivmin,
+                                        double spdiam) {
+    const int maxitr = (int)((log(spdiam + pivmin) - log(pivmin)) / log(2.0)) + 2;
+    double left = *w - *werr, mid = *w, right = *w + *werr;
+    double width = right - mid;
+    double tmp = fmax(fabs(left), fabs(right));
+    if (width < rtol * tmp) return;
+    double fac = 1.0;
+    for (int g = 0; g < kMaxExpand && count_T_rrj(n, d, e2, left) > i - 1; ++g) {
+        left = left - *werr * fac;
+        fac = 2.0 * fac;
+    }
+    fac = 1.0;
+    for (int g = 0; g < kMaxExpand && count_T_rrj(n, d, e2, right) < i; ++g) {
+        right = right + *werr * fac;
+        fac = 2.0 * fac;
+    }
+    for (int iter = 0;; ++iter) {
+        mid = 0.5 * (left + right);
+        width = right - mid;
+        tmp = fmax(fabs(left), fabs(right));
+        if (width < rtol * tmp || iter == maxitr) break;
+        int cnt = count_T_rrj(n, d, e2, mid);
+        if (cnt <= i - 1) left = mid;
+        else right = mid;
+    }
+    *w = 0.5 * (left + right);
+    *werr = right - *w;
+}
+
+// 48-bit LCG of dlaruv (odruv.f:330-387): x_i = seed * a^i mod 2^48.  dlarnv with
+// IDIST=2 (odrnv.f:60-116) consumes the stream in chunks of 64 and hands the last
+// value of a chunk on as the next seed, so entry i (0-based) of the vector is
+// 2 * (seed * a^(i+1) mod 2^48) / 2^48 - 1.
+constexpr unsigned long long kLcgA = 33952834046453ULL;
+constexpr unsigned long long kMask48 = (1ULL << 48) - 1ULL;
+
+__host__ __device__ __forceinline__ unsigned long long lcg_pow(unsigned long long e) {
+    unsigned long long r = 1ULL, b = kLcgA;
+    while (e) {
+        if (e & 1ULL) r = (r * b) & kMask48;
+        b = (b * b) & kMask48;
+        e >>= 1;
+    }
+    return r;
+}
+
+__host__ __device__ __forceinline__ double lcg_uniform_pm1(unsigned long long x) {
+    const double R = 1.0 / 4096.0;
+    double it1 = (double)((x >> 36) & 4095ULL), it2 = (double)((x >> 24) & 4095ULL);
+    double it3 = (double)((x >> 12) & 4095ULL), it4 = (double)(x & 4095ULL);
+    double u = R * (it1 + R * (it2 + R * (it3 + R * it4)));
+    return 2.0 * u - 1.0;
+}
+
+}  // namespace mr3

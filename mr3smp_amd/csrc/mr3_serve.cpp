@@ -1,0 +1,220 @@
+// mr3_serve.cpp -- request loop with continuous batching for the MRRR solver.
+//
+// A scheduler thread admits queued requests in arrival order whenever a concurrency
+// slot is free and the device-memory budget (8 n m bytes for Z plus a per-row
+// workspace estimate) has room; each admitted group runs as one batched solve on
+// its own stream, so groups that were admitted at different times overlap on the
+// device.  Requests arriving while groups run wait only for a slot, not for the
+// whole queue to drain.  Results of a problem do not depend on the batch it ran
+// in: every decision of the pipeline is taken per matrix.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/mr3hip.h"
+
+namespace {
+
+constexpr long long kRowBytes = 512;   // per-row workspace estimate of the pipeline
+
+long long footprint(const mr3_request& r) {
+    long long cols = 0;
+    if (r.jobz == 'V' || r.jobz == 'v') cols = r.mcols > 0 ? r.mcols : r.n;
+    return 8LL * r.n * cols + kRowBytes * r.n + 64LL * r.n;
+}
+
+struct Entry {
+    long long id;
+    mr3_request* req;
+    long long bytes;
+};
+
+}  // namespace
+
+struct mr3_server {
+    long long budget = 0, in_use = 0;
+    int max_inflight = 2, max_batch = 256, inflight = 0;
+    bool stop = false;
+    long long next_id = 0;
+    std::deque<Entry> pending;
+    std::unordered_map<long long, mr3_request*> live;   // submitted, not yet complete
+    std::mutex mu;
+    std::condition_variable cv_sched, cv_done;
+    std::thread sched;
+    std::vector<std::thread> workers;
+    std::vector<hipStream_t> streams;
+    std::vector<int> stream_free;
+    int device = 0;
+
+    void run_group(std::vector<Entry> grp, int sidx);
+    void scheduler();
+};
+
+void mr3_server::run_group(std::vector<Entry> grp, int sidx) {
+    (void)hipSetDevice(device);
+    hipStream_t st = streams[sidx];
+    const double t0 = mr3_now();
+    for (auto& e : grp) e.req->t_start = t0;
+    // one device allocation for the whole group
+    size_t bytes = 0;
+    std::vector<size_t> offs(grp.size() * 5);
+    auto take = [&](size_t b) { size_t o = bytes; bytes += (b + 255) & ~(size_t)255; return o; };
+    for (size_t i = 0; i < grp.size(); ++i) {
+        const mr3_request& r = *grp[i].req;
+        const bool wz = (r.jobz == 'V' || r.jobz == 'v');
+        const int mc = wz ? (r.mcols > 0 ? r.mcols : r.n) : 0;
+        offs[5 * i + 0] = take(sizeof(double) * r.n);
+        offs[5 * i + 1] = take(sizeof(double) * r.n);
+        offs[5 * i + 2] = take(sizeof(double) * r.n);
+        offs[5 * i + 3] = take(sizeof(double) * (size_t)r.n * mc);
+        offs[5 * i + 4] = take(sizeof(int) * 2 * (size_t)(mc > 0 ? mc : 1));
+    }
+    char* base = nullptr;
+    int rc = (hipMallocAsync((void**)&base, bytes ? bytes : 1, st) == hipSuccess) ? 0 : 2;
+    std::vector<mr3_problem> probs(grp.size());
+    if (rc == 0) {
+        for (size_t i = 0; i < grp.size(); ++i) {
+            const mr3_request& r = *grp[i].req;
+            const bool wz = (r.jobz == 'V' || r.jobz == 'v');
+            mr3_problem& p = probs[i];
+            memset(&p, 0, sizeof(p));
+            p.n = r.n; p.jobz = r.jobz; p.range = r.range;
+            p.vl = r.vl; p.vu = r.vu; p.il = r.il; p.iu = r.iu; p.tryrac = r.tryrac;
+            p.ldz = r.n;
+            p.mcols = wz ? (r.mcols > 0 ? r.mcols : r.n) : 0;
+            p.D = (const double*)(base + offs[5 * i + 0]);
+            p.E = (const double*)(base + offs[5 * i + 1]);
+            p.W = (double*)(base + offs[5 * i + 2]);
+            p.Z = wz ? (double*)(base + offs[5 * i + 3]) : nullptr;
+            p.Zsupp = wz ? (int*)(base + offs[5 * i + 4]) : nullptr;
+            (void)hipMemcpyAsync((void*)p.D, r.D, sizeof(double) * r.n, hipMemcpyHostToDevice, st);
+            if (r.n > 1)
+                (void)hipMemcpyAsync((void*)p.E, r.E, sizeof(double) * (r.n - 1), hipMemcpyHostToDevice, st);
+        }
+        rc = mr3_solve_batch(probs.data(), (int)probs.size(), st);
+        for (size_t i = 0; i < grp.size(); ++i) {
+            mr3_request& r = *grp[i].req;
+            const mr3_problem& p = probs[i];
+            r.m = p.m;
+            r.info = p.info;
+            if (p.info == 0 && p.m > 0) {
+                if (r.W) (void)hipMemcpyAsync(r.W, p.W, sizeof(double) * p.m, hipMemcpyDeviceToHost, st);
+                if (p.Z && r.Z)
+                    (void)hipMemcpyAsync(r.Z, p.Z, sizeof(double) * (size_t)r.n * p.m, hipMemcpyDeviceToHost, st);
+                if (p.Zsupp && r.Zsupp)
+                    (void)hipMemcpyAsync(r.Zsupp, p.Zsupp, sizeof(int) * 2 * p.m, hipMemcpyDeviceToHost, st);
+            }
+        }
+        (void)hipFreeAsync(base, st);
+        if (hipStreamSynchronize(st) != hipSuccess) rc = 2;
+    }
+    const double t1 = mr3_now();
+    std::lock_guard<std::mutex> lk(mu);
+    for (auto& e : grp) {
+        if (rc == 2 && e.req->info == 0) e.req->info = 2;
+        e.req->t_done = t1;
+        in_use -= e.bytes;
+        live.erase(e.id);
+    }
+    inflight -= 1;
+    stream_free[sidx] = 1;
+    cv_done.notify_all();
+    cv_sched.notify_all();
+}
+
+void mr3_server::scheduler() {
+    (void)hipSetDevice(device);
+    std::unique_lock<std::mutex> lk(mu);
+    for (;;) {
+        cv_sched.wait(lk, [&] {
+            if (stop) return true;
+            if (pending.empty() || inflight >= max_inflight) return false;
+            return in_use + pending.front().bytes <= budget;
+        });
+        if (stop && pending.empty()) break;
+        if (stop) break;
+        // admit in arrival order while the budget and the group size allow
+        std::vector<Entry> grp;
+        while (!pending.empty() && (int)grp.size() < max_batch &&
+               in_use + pending.front().bytes <= budget) {
+            grp.push_back(pending.front());
+            in_use += pending.front().bytes;
+            pending.pop_front();
+        }
+        if (grp.empty()) continue;
+        int sidx = 0;
+        while (!stream_free[sidx]) ++sidx;
+        stream_free[sidx] = 0;
+        inflight += 1;
+        lk.unlock();
+        std::thread(&mr3_server::run_group, this, std::move(grp), sidx).detach();
+        lk.lock();
+    }
+}
+
+extern "C" mr3_server* mr3_server_create(long long budget_bytes, int max_inflight, int max_batch) {
+    mr3_server* s = new mr3_server();
+    s->budget = budget_bytes > 0 ? budget_bytes : (200LL << 30);
+    s->max_inflight = max_inflight > 0 ? max_inflight : 2;
+    s->max_batch = max_batch > 0 ? max_batch : 256;
+    (void)hipGetDevice(&s->device);
+    s->streams.resize(s->max_inflight);
+    s->stream_free.assign(s->max_inflight, 1);
+    for (auto& st : s->streams) {
+        if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) st = nullptr;
+    }
+    s->sched = std::thread(&mr3_server::scheduler, s);
+    return s;
+}
+
+extern "C" long long mr3_server_submit(mr3_server* s, mr3_request* req) {
+    Entry e;
+    e.req = req;
+    e.bytes = footprint(*req);
+    req->t_submit = mr3_now();
+    req->t_start = req->t_done = 0.0;
+    req->m = 0;
+    req->info = -1;
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (e.bytes > s->budget) return -1;
+    e.id = s->next_id++;
+    s->pending.push_back(e);
+    s->live[e.id] = req;
+    s->cv_sched.notify_all();
+    return e.id;
+}
+
+extern "C" int mr3_server_wait(mr3_server* s, long long id) {
+    std::unique_lock<std::mutex> lk(s->mu);
+    mr3_request* r = nullptr;
+    auto it = s->live.find(id);
+    if (it == s->live.end()) return 0;
+    r = it->second;
+    s->cv_done.wait(lk, [&] { return s->live.find(id) == s->live.end(); });
+    return r->info;
+}
+
+extern "C" void mr3_server_drain(mr3_server* s) {
+    std::unique_lock<std::mutex> lk(s->mu);
+    s->cv_done.wait(lk, [&] { return s->live.empty(); });
+}
+
+extern "C" void mr3_server_destroy(mr3_server* s) {
+    mr3_server_drain(s);
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stop = true;
+        s->cv_sched.notify_all();
+    }
+    s->sched.join();
+    for (auto& st : s->streams)
+        if (st) (void)hipStreamDestroy(st);
+    delete s;
+}
