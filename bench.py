@@ -1,0 +1,257 @@
+"""Benchmark: eigenpairs/s of the batched MRRR solve against the plain one-matrix-per-
+solve path on the mixed-size workload, on one MI355X.
+
+Workload (the batched mixed workload): 4096 FP64 tridiagonals, n log-uniform in
+[32, 2048], 50% uniform(-1,1), 25% perturbed 1-2-1 Toeplitz, 25% glued Wilkinson
+(glue 1e-10), jobz = V, range = A, synthetic data from fixed seeds.  Inputs and
+outputs are resident in HBM before the timed region starts.
+
+Prints one JSON line.  Besides the contract fields it reports the plain path's
+rate and the ratio, a short served stream (p50/p99 latency), resid/orth of a
+sample, per-kernel times from HIP events recorded on the library's stream during
+the timed steps, the roofline of the dominant kernel (algorithmic FLOPs counted on
+the device), and the CPU baseline (the compiled reference on the host's cores).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, REPO)
+sys.path.insert(0, os.path.join(REPO, "tests"))
+
+# FP64 operations per counted element, per kernel family (algorithmic work:
+# Sturm step sub/div/sub = 3, dlaneg add/div/mul/sub = 4, dlar1v 16 per element,
+# dlarrf sweep 5 per element).
+FLOPS_PER_STEP = {"coarse": 3, "refine": 4, "endref": 4, "r1v": 16, "r1v_bisect": 4,
+                  "rrf": 5, "rrj": 3, "refine_cl": 4}
+KERNEL_STEPS = {"k_coarse": ["coarse"], "k_refine_root": ["refine"], "k_endref": ["endref"],
+                "k_singleton": ["r1v", "r1v_bisect"], "k_dlarrf": ["rrf"], "k_rrj": ["rrj"],
+                "k_refine_cl": ["refine_cl"]}
+FP64_PEAK_TFLOPS = 78.6
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def build_workload(torch, dev, count):
+    from mr3smp_amd import workloads as WL
+    mats = WL.cfg3(count)
+    probs = []
+    for D, E, kind in mats:
+        n = len(D)
+        probs.append(dict(D=torch.tensor(D, device=dev), E=torch.tensor(E, device=dev),
+                          jobz="V", range="A",
+                          W=torch.empty(n, dtype=torch.float64, device=dev),
+                          Z=torch.empty((n, n), dtype=torch.float64, device=dev),
+                          Zsupp=torch.empty(2 * n, dtype=torch.int32, device=dev)))
+    return mats, probs
+
+
+def accuracy_sample(mats, keep, arr, stride):
+    from mrrr_ctypes import resid_orth
+    worst_r, worst_o = 0.0, 0.0
+    for i in range(0, len(mats), stride):
+        D, E, _ = mats[i]
+        m = arr[i].m
+        W = keep[i]["W"][:m].cpu().numpy()
+        Z = keep[i]["Z"][:m].cpu().numpy().T
+        r, o = resid_orth(D, E, W, Z)
+        worst_r, worst_o = max(worst_r, r), max(worst_o, o)
+    return worst_r, worst_o
+
+
+def cpu_baseline(mats, stride, threads):
+    """The compiled reference (oracle/_ref) when present, else the oracle, on every
+    stride-th matrix, `threads` concurrent callers with PMR_NUM_THREADS=1 (the
+    reference's sequential dstemr route; it is reentrant)."""
+    from concurrent.futures import ThreadPoolExecutor
+    from mrrr_ctypes import ORACLE_SO, REF_SO, call_mrrr, load
+    if os.path.exists(REF_SO):
+        _, fn = load(REF_SO, "mrrr")
+        kind = "reference"
+    elif os.path.exists(ORACLE_SO):
+        _, fn = load(ORACLE_SO, "oracle_mrrr")
+        kind = "port"
+    else:
+        return None
+    os.environ["PMR_NUM_THREADS"] = "1"
+    sample = mats[::stride]
+    pairs = sum(len(D) for D, _, _ in sample)
+
+    def one(t):
+        D, E, _ = t
+        r = call_mrrr(fn, "V", "A", D, E)
+        return r["m"]
+
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(max_workers=threads) as ex:
+        ms = list(ex.map(one, sample))
+    dt = time.perf_counter() - t0
+    assert sum(ms) == pairs
+    return dict(value=pairs / dt, unit="eigenpairs/s", cores=threads, kind=kind,
+                sample=f"every {stride}th matrix of the batched workload ({len(sample)} matrices, "
+                       f"{pairs} eigenpairs, jobz=V range=A), {threads} concurrent callers with "
+                       f"PMR_NUM_THREADS=1, wall {dt:.2f} s")
+
+
+def serve_stream(M, count, rate, budget):
+    from mr3smp_amd import workloads as WL
+    reqs = WL.cfg5(count, rate)
+    srv = M.Server(budget_bytes=budget, max_inflight=3, max_batch=64)
+    t0 = time.perf_counter()
+    ids = []
+    for r in reqs:
+        while time.perf_counter() - t0 < r["t"]:
+            time.sleep(min(0.002, max(0.0, r["t"] - (time.perf_counter() - t0))))
+        kw = {k: r[k] for k in ("vl", "vu", "il", "iu") if k in r}
+        mc = (r["iu"] - r["il"] + 1) if r["range"] == "I" else None
+        ids.append(srv.submit(r["jobz"], r["range"], r["D"], r["E"], mcols=mc, **kw))
+    outs = [srv.wait(i) for i in ids]
+    dt = time.perf_counter() - t0
+    srv.close()
+    lat = np.array([o["t_done"] - o["t_submit"] for o in outs])
+    pairs = sum(o["m"] for o in outs)
+    assert all(o["info"] == 0 for o in outs)
+    return dict(requests=count, rate_per_s=rate, p50_ms=float(np.percentile(lat, 50) * 1e3),
+                p99_ms=float(np.percentile(lat, 99) * 1e3), eigenpairs_per_s=pairs / dt,
+                wall_s=dt)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--count", type=int, default=4096)
+    ap.add_argument("--plain-stride", type=int, default=8)
+    ap.add_argument("--cpu-stride", type=int, default=16)
+    ap.add_argument("--serve-requests", type=int, default=48)
+    ap.add_argument("--serve-rate", type=float, default=20.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    import torch
+    import mr3smp_amd as M
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(0)
+    log("building workload")
+    mats, probs = build_workload(torch, dev, args.count)
+    pairs = sum(len(D) for D, _, _ in mats)
+    stream = torch.cuda.Stream()
+    for _ in range(args.warmup):
+        rc, arr, keep = M.solve_batch(probs, stream=stream)
+        assert rc == 0
+    torch.cuda.synchronize()
+    M.profile_reset()
+    M.profile_enable(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        rc, arr, keep = M.solve_batch(probs, stream=stream)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    M.profile_enable(False)
+    assert rc == 0 and sum(a.m for a in arr) == pairs
+    prof = M.profile_report()
+    value = pairs * args.steps / dt
+    log(f"batched: {dt / args.steps * 1e3:.1f} ms/step, {value:.4g} eigenpairs/s")
+
+    # plain path: one matrix per solve, same device-resident buffers
+    sample = list(range(0, len(probs), args.plain_stride))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ppairs = 0
+    for i in sample:
+        rcp, ap_, _ = M.solve_batch([probs[i]], stream=stream)
+        ppairs += ap_[0].m
+    torch.cuda.synchronize()
+    dtp = time.perf_counter() - t0
+    plain = ppairs / dtp
+    log(f"plain: {plain:.4g} eigenpairs/s on {len(sample)} matrices")
+
+    worst_r, worst_o = accuracy_sample(mats, keep, arr, 64)
+
+    # roofline of the dominant kernel (per launch averages over the timed steps)
+    kern = prof["kernels"]
+    dom = max(kern, key=lambda k: kern[k]["ms"])
+    steps = prof["steps"]
+    flops = sum(steps.get(f, 0) * FLOPS_PER_STEP[f] for f in KERNEL_STEPS.get(dom, []))
+    per_launch_s = kern[dom]["ms"] / 1e3 / kern[dom]["launches"]
+    flops_per_launch = flops / kern[dom]["launches"]
+    achieved = flops_per_launch / per_launch_s / 1e12 if per_launch_s > 0 else 0.0
+    traffic = None
+    tfile = os.path.join(REPO, "profiles", "traffic.json")
+    if os.path.exists(tfile):
+        try:
+            traffic = json.load(open(tfile)).get(dom)
+        except Exception:
+            traffic = None
+    total_flops = sum(steps.get(f, 0) * FLOPS_PER_STEP[f] for f in FLOPS_PER_STEP)
+    gpu_ms = sum(v["ms"] for v in kern.values())
+    z_bytes = 8.0 * sum(len(D) ** 2 for D, _, _ in mats) * args.steps
+    sing_ms = kern.get("k_singleton", {}).get("ms", 0.0)
+
+    serve = None
+    if not args.no_serve:
+        try:
+            serve = serve_stream(M, args.serve_requests, args.serve_rate, 200 << 30)
+            log(f"serve: {serve}")
+        except Exception as e:  # reported, not fatal for the headline number
+            serve = {"error": repr(e)}
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(mats, args.cpu_stride, min(16, os.cpu_count() or 1))
+        log(f"cpu baseline: {cpu}")
+
+    out = {
+        "metric": "eigenpairs/sec batched vs plain on 1 GPU; p50/p99 latency; orth/resid <= ref",
+        "value": value,
+        "unit": "eigenpairs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": dt / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded), inputs and outputs resident in HBM",
+        "config": {"workload": f"batched mixed tridiagonals: {args.count} matrices, n log-uniform "
+                               f"[32,2048], 50% U(-1,1) / 25% perturbed 1-2-1 / 25% glued W21+ "
+                               f"(glue 1e-10), jobz=V range=A",
+                   "matrices": args.count, "eigenpairs_per_step": pairs},
+        "plain_value": plain,
+        "batched_over_plain": value / plain,
+        "plain_sample": f"every {args.plain_stride}th matrix, one solve per matrix",
+        "serve": serve,
+        "accuracy_sample": {"max_resid": worst_r, "max_orth": worst_o,
+                            "units": "n*eps*||T|| and n*eps"},
+        "roofline": {"bound": "mfma", "engine": "fp64 vector ALU (no MFMA on this path)",
+                     "kernel": dom, "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
+                     "traffic": traffic, "flops_per_launch": flops_per_launch,
+                     "ms_per_launch": per_launch_s * 1e3},
+        "z_writeback": {"bytes_per_step": z_bytes / args.steps,
+                        "GBps_over_singleton_kernel": (z_bytes / (sing_ms / 1e3) / 1e9) if sing_ms else None,
+                        "peak_GBps": HBM_PEAK_GBS},
+        "kernels_ms_per_step": {k: v["ms"] / args.steps for k, v in sorted(kern.items(), key=lambda kv: -kv[1]["ms"])},
+        "gpu_fp64_tflops_all_kernels": total_flops / (gpu_ms / 1e3) / 1e12 if gpu_ms else None,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -106,6 +106,18 @@ void mr3_server_destroy(mr3_server *s);
 /* Monotonic clock used for the request timestamps. */
 double mr3_now(void);
 
+/*
+ * Profiling.  While enabled, every kernel launch of the library is bracketed by HIP
+ * events on the stream it runs on and the kernels count their algorithmic work
+ * (sweep elements per kernel family); mr3_profile_report writes a JSON object
+ * {"kernels": {name: {"ms", "launches"}}, "steps": {family: elements}} into buf and
+ * returns its length (or minus the needed size if buf is too small).  No counterpart
+ * in the reference, which has no instrumentation.
+ */
+void mr3_profile_enable(int on);
+void mr3_profile_reset(void);
+int mr3_profile_report(char *buf, int len);
+
 /* Library version string. */
 const char *mr3_version(void);
 

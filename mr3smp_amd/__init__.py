@@ -84,6 +84,11 @@ def lib():
         L.mr3_server_destroy.restype = None
         L.mr3_server_destroy.argtypes = [_P]
         L.mr3_now.restype = ctypes.c_double
+        L.mr3_profile_enable.argtypes = [ctypes.c_int]
+        L.mr3_profile_enable.restype = None
+        L.mr3_profile_reset.restype = None
+        L.mr3_profile_report.argtypes = [ctypes.c_char_p, ctypes.c_int]
+        L.mr3_profile_report.restype = ctypes.c_int
         L.mr3_version.restype = ctypes.c_char_p
         _lib = L
     return _lib
@@ -169,6 +174,27 @@ def solve_batch(problems, stream=None):
     st = None if stream is None else ctypes.c_void_p(stream.cuda_stream)
     rc = L.mr3_solve_batch(arr, len(problems), st)
     return rc, arr, keep
+
+
+def profile_enable(on=True):
+    """Per-kernel HIP-event timing and on-device work counters for later solves."""
+    lib().mr3_profile_enable(1 if on else 0)
+
+
+def profile_reset():
+    lib().mr3_profile_reset()
+
+
+def profile_report():
+    import json
+    L = lib()
+    size = 1 << 16
+    while True:
+        buf = ctypes.create_string_buffer(size)
+        n = L.mr3_profile_report(buf, size)
+        if n >= 0:
+            return json.loads(buf.value.decode())
+        size = -n + 16
 
 
 class Server:

@@ -17,6 +17,9 @@
 
 namespace mr3 {
 int run_pipeline(std::vector<MatDesc>& md, std::vector<MatState>& ms, hipStream_t st);
+void profile_enable(int on);
+void profile_reset();
+int profile_report(char* buf, int len);
 
 static hipStream_t thread_stream() {
     thread_local hipStream_t s = nullptr;
@@ -176,6 +179,10 @@ extern "C" void mrrr_(char* jobz, char* range, int* n, double* D, double* E, dou
                       int* ldz, int* Zsupp, int* info) {
     *info = mrrr(jobz, range, n, D, E, vl, vu, il, iu, tryrac, m, W, Z, ldz, Zsupp);
 }
+
+extern "C" void mr3_profile_enable(int on) { mr3::profile_enable(on); }
+extern "C" void mr3_profile_reset(void) { mr3::profile_reset(); }
+extern "C" int mr3_profile_report(char* buf, int len) { return mr3::profile_report(buf, len); }
 
 extern "C" const char* mr3_version(void) { return "mr3hip 0.1 (gfx950)"; }
 

@@ -13,7 +13,9 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <mutex>
+#include <string>
 #include <vector>
 
 #include "../../include/mr3hip.h"
@@ -35,6 +37,61 @@ namespace mr3 {
     } while (0)
 
 static inline unsigned cdiv(long long a, long long b) { return (unsigned)((a + b - 1) / b); }
+
+// ---- profiling: HIP events around every launch (on the launch stream) and the
+// on-device step counters, accumulated over the solves run while enabled.
+struct ProfEntry {
+    double ms = 0.0;
+    long long launches = 0;
+};
+struct Profiler {
+    std::mutex mu;
+    std::atomic<int> on{0};
+    std::map<std::string, ProfEntry> acc;
+    unsigned long long steps[kStepKinds] = {};
+};
+static Profiler g_prof;
+
+struct CallEvents {
+    bool on = false;
+    std::vector<std::pair<const char*, std::pair<hipEvent_t, hipEvent_t>>> ev;
+    void collect() {
+        if (!on) return;
+        std::lock_guard<std::mutex> lk(g_prof.mu);
+        for (auto& e : ev) {
+            float ms = 0.f;
+            if (hipEventElapsedTime(&ms, e.second.first, e.second.second) == hipSuccess) {
+                ProfEntry& pe = g_prof.acc[e.first];
+                pe.ms += ms;
+                pe.launches += 1;
+            }
+            (void)hipEventDestroy(e.second.first);
+            (void)hipEventDestroy(e.second.second);
+        }
+        ev.clear();
+        unsigned long long st[kStepKinds];
+        if (hipMemcpyFromSymbol(st, HIP_SYMBOL(g_steps), sizeof(st)) == hipSuccess) {
+            for (int i = 0; i < kStepKinds; ++i) g_prof.steps[i] += st[i];
+            unsigned long long z[kStepKinds] = {};
+            (void)hipMemcpyToSymbol(HIP_SYMBOL(g_steps), z, sizeof(z));
+        }
+    }
+};
+
+#define LAUNCH(KERN, GRID, BLOCK, ...)                                              \
+    do {                                                                            \
+        hipEvent_t e0_ = nullptr, e1_ = nullptr;                                    \
+        if (ce.on) {                                                                \
+            (void)hipEventCreate(&e0_);                                             \
+            (void)hipEventCreate(&e1_);                                             \
+            (void)hipEventRecord(e0_, st);                                          \
+        }                                                                           \
+        hipLaunchKernelGGL(KERN, GRID, BLOCK, 0, st, __VA_ARGS__);                   \
+        if (ce.on) {                                                                \
+            (void)hipEventRecord(e1_, st);                                          \
+            ce.ev.push_back({#KERN, {e0_, e1_}});                                   \
+        }                                                                           \
+    } while (0)
 
 // Device memory in one stream-ordered pool: buffers of a solve are taken with
 // hipMallocAsync and returned with hipFreeAsync on the same stream, so that
@@ -105,6 +162,8 @@ int run_pipeline(std::vector<MatDesc>& md, std::vector<MatState>& ms, hipStream_
     if (ntot > 0x7fffffff / 2) return 2;
     const int N = (int)ntot;
     Alloc A(st);
+    CallEvents ce;
+    ce.on = g_prof.on.load() != 0;
     MatDesc* dmd = A.get<MatDesc>(nmat);
     MatState* dms = A.get<MatState>(nmat);
     double* D = A.get<double>(N);
@@ -131,11 +190,11 @@ int run_pipeline(std::vector<MatDesc>& md, std::vector<MatState>& ms, hipStream_
     if (!A.ok) return 2;
     MR3_CHECK(hipMemcpyAsync(dmd, md.data(), sizeof(MatDesc) * nmat, hipMemcpyHostToDevice, st));
 
-    hipLaunchKernelGGL(k_prep, dim3(nmat), dim3(256), 0, st, dmd, dms, D, E, E2, gers, Dcopy,
+    LAUNCH(k_prep, dim3(nmat), dim3(256), dmd, dms, D, E, E2, gers, Dcopy,
                        E2copy, isplit, blk, W, Werr, Wcoarse, iblock, Windex, smat);
-    hipLaunchKernelGGL(k_coarse, dim3(cdiv(N, 256)), dim3(256), 0, st, dmd, dms, D, E2, blk, smat,
+    LAUNCH(k_coarse, dim3(cdiv(N, 256)), dim3(256), dmd, dms, D, E2, blk, smat,
                        iblock, Windex, W, Werr, Wcoarse, N);
-    hipLaunchKernelGGL(k_discard, dim3(cdiv(nmat, 64)), dim3(64), 0, st, dmd, dms, blk, W, Werr,
+    LAUNCH(k_discard, dim3(cdiv(nmat, 64)), dim3(64), dmd, dms, blk, W, Werr,
                        Wcoarse, iblock, Windex, nmat);
     MR3_CHECK(hipGetLastError());
     ms.resize(nmat);
@@ -155,19 +214,19 @@ int run_pipeline(std::vector<MatDesc>& md, std::vector<MatState>& ms, hipStream_
         dbl = A.get<int>(nbl);
         if (!A.ok) return 2;
         MR3_CHECK(hipMemcpyAsync(dbl, blist.data(), sizeof(int) * nbl, hipMemcpyHostToDevice, st));
-        hipLaunchKernelGGL(k_root, dim3(nbl), dim3(64), 0, st, dmd, dms, blk, smat, dbl, D, E, DLr,
+        LAUNCH(k_root, dim3(nbl), dim3(64), dmd, dms, blk, smat, dbl, D, E, DLr,
                            DLLr, W, Werr, Wgap, nbl);
-        hipLaunchKernelGGL(k_refine_root, dim3(cdiv(N, 256)), dim3(256), 0, st, dmd, dms, blk, smat,
+        LAUNCH(k_refine_root, dim3(cdiv(N, 256)), dim3(256), dmd, dms, blk, smat,
                            iblock, Windex, D, DLLr, W, Werr, Wgap, Wtmp, Werrtmp, N);
-        hipLaunchKernelGGL(k_gaps_root, dim3(cdiv(N, 256)), dim3(256), 0, st, dmd, dms, blk, smat,
+        LAUNCH(k_gaps_root, dim3(cdiv(N, 256)), dim3(256), dmd, dms, blk, smat,
                            iblock, Wtmp, Werrtmp, W, Werr, Wgap, N);
         if (any_n)
-            hipLaunchKernelGGL(k_shift_n, dim3(cdiv(N, 256)), dim3(256), 0, st, dmd, dms, blk, smat,
+            LAUNCH(k_shift_n, dim3(cdiv(N, 256)), dim3(256), dmd, dms, blk, smat,
                                iblock, W, N);
         MR3_CHECK(hipGetLastError());
     }
     if (any_v && nbl > 0) {
-        hipLaunchKernelGGL(k_zindex, dim3(nmat), dim3(256), 0, st, dmd, dms, blk, W, iblock, Windex,
+        LAUNCH(k_zindex, dim3(nmat), dim3(256), dmd, dms, blk, W, iblock, Windex,
                            Zidx);
         // task lists: singletons (<= N), clusters (<= N/2 + 1), eigen entries (<= N)
         const int capS = N, capC = N / 2 + 1, capE = N;
@@ -193,7 +252,7 @@ int run_pipeline(std::vector<MatDesc>& md, std::vector<MatState>& ms, hipStream_
         TreeOut out;
         out.S = Sl; out.C = Cl[0]; out.eig_slot = es[0]; out.eig_cl = ec[0]; out.cnt = dcnt;
         out.capS = capS; out.capC = capC; out.capEig = capE;
-        hipLaunchKernelGGL(k_classify_root, dim3(cdiv(nbl, 64)), dim3(64), 0, st, dmd, dms, blk,
+        LAUNCH(k_classify_root, dim3(cdiv(nbl, 64)), dim3(64), dmd, dms, blk,
                            smat, dbl, gers, D, E, DLr, DLLr, W, Werr, Wgap, Wsh, out, nbl);
         MR3_CHECK(hipGetLastError());
         MR3_CHECK(hipMemcpyAsync(&hc, dcnt, sizeof(hc), hipMemcpyDeviceToHost, st));
@@ -206,7 +265,7 @@ int run_pipeline(std::vector<MatDesc>& md, std::vector<MatState>& ms, hipStream_
             const long long arena_need = hc.arena;
             if (nS > 0) {
                 int g = std::min(nwaves, (nS + 63) / 64);
-                hipLaunchKernelGGL(k_singleton, dim3(g), dim3(64), 0, st, Sl, nS, dmd, dms, Windex,
+                LAUNCH(k_singleton, dim3(g), dim3(64), Sl, nS, dmd, dms, Windex,
                                    Zidx, W, Wsh, Werr, Wgap, scratch, maxbl);
                 MR3_CHECK(hipGetLastError());
             }
@@ -215,18 +274,21 @@ int run_pipeline(std::vector<MatDesc>& md, std::vector<MatState>& ms, hipStream_
             double* arena = A.get<double>((size_t)arena_need);
             if (!A.ok) return 2;
             MR3_CHECK(hipMemsetAsync(bump, 0, sizeof(unsigned long long), st));
-            hipLaunchKernelGGL(k_arena_assign, dim3(cdiv(nC, 256)), dim3(256), 0, st, Cl[cur], nC,
+            LAUNCH(k_arena_assign, dim3(cdiv(nC, 256)), dim3(256), Cl[cur], nC,
                                arena, bump);
-            hipLaunchKernelGGL(k_newrrr, dim3(nC), dim3(64), 0, st, Cl[cur], nC, Windex, Wsh, Werr,
-                               Wgap, dms, dmd);
-            hipLaunchKernelGGL(k_refine_cl, dim3(cdiv(nE, 256)), dim3(256), 0, st, Cl[cur], es[cur],
+            LAUNCH(k_endref, dim3(cdiv(2LL * nC, 64)), dim3(64), Cl[cur], nC,
+                               Windex, Wsh, Werr, dms, Wtmp, Werrtmp);
+            LAUNCH(k_dlarrf, dim3(cdiv(nC, 64)), dim3(64), Cl[cur], nC, Wsh, Werr,
+                               Wgap, Wtmp, Werrtmp, dms);
+            LAUNCH(k_newrrr_fin, dim3(nC), dim3(64), Cl[cur], nC, Wsh, Werr);
+            LAUNCH(k_refine_cl, dim3(cdiv(nE, 256)), dim3(256), Cl[cur], es[cur],
                                ec[cur], nE, Windex, Wsh, Werr, Wgap, dms, Wtmp, Werrtmp);
-            hipLaunchKernelGGL(k_gaps_cl, dim3(cdiv(nE, 256)), dim3(256), 0, st, Cl[cur], es[cur],
+            LAUNCH(k_gaps_cl, dim3(cdiv(nE, 256)), dim3(256), Cl[cur], es[cur],
                                ec[cur], nE, Wtmp, Werrtmp, Wsh, Werr, Wgap, W);
             MR3_CHECK(reset());
             const int nxt = cur ^ 1;
             out.C = Cl[nxt]; out.eig_slot = es[nxt]; out.eig_cl = ec[nxt];
-            hipLaunchKernelGGL(k_classify, dim3(cdiv(nC, 64)), dim3(64), 0, st, Cl[cur], nC, Wgap,
+            LAUNCH(k_classify, dim3(cdiv(nC, 64)), dim3(64), Cl[cur], nC, Wgap,
                                Wsh, out);
             MR3_CHECK(hipGetLastError());
             MR3_CHECK(hipMemcpyAsync(&hc, dcnt, sizeof(hc), hipMemcpyDeviceToHost, st));
@@ -236,15 +298,53 @@ int run_pipeline(std::vector<MatDesc>& md, std::vector<MatState>& ms, hipStream_
         }
     }
     if (nbl > 0) {
-        hipLaunchKernelGGL(k_rrj, dim3(cdiv(N, 256)), dim3(256), 0, st, dmd, dms, blk, smat, iblock,
+        LAUNCH(k_rrj, dim3(cdiv(N, 256)), dim3(256), dmd, dms, blk, smat, iblock,
                            Windex, Dcopy, E2copy, W, Werr, N);
-        hipLaunchKernelGGL(k_finalize, dim3(nmat), dim3(256), 0, st, dmd, dms, W, Zidx, blk, iblock,
+        LAUNCH(k_finalize, dim3(nmat), dim3(256), dmd, dms, W, Zidx, blk, iblock,
                            Windex);
         MR3_CHECK(hipGetLastError());
     }
     MR3_CHECK(hipMemcpyAsync(ms.data(), dms, sizeof(MatState) * nmat, hipMemcpyDeviceToHost, st));
     MR3_CHECK(hipStreamSynchronize(st));
+    ce.collect();
     return 0;
+}
+
+void profile_enable(int on) {
+    g_prof.on.store(on ? 1 : 0);
+    int flag = on ? 1 : 0;
+    (void)hipMemcpyToSymbol(HIP_SYMBOL(g_count_steps), &flag, sizeof(int));
+}
+
+void profile_reset() {
+    std::lock_guard<std::mutex> lk(g_prof.mu);
+    g_prof.acc.clear();
+    for (auto& v : g_prof.steps) v = 0;
+}
+
+int profile_report(char* buf, int len) {
+    static const char* kinds[kStepKinds] = {"coarse", "refine", "endref", "r1v",
+                                            "r1v_bisect", "rrf", "rrj", "refine_cl"};
+    std::string js = "{\"kernels\": {";
+    std::lock_guard<std::mutex> lk(g_prof.mu);
+    bool first = true;
+    for (auto& kv : g_prof.acc) {
+        char tmp[256];
+        snprintf(tmp, sizeof(tmp), "%s\"%s\": {\"ms\": %.6f, \"launches\": %lld}", first ? "" : ", ",
+                 kv.first.c_str(), kv.second.ms, kv.second.launches);
+        js += tmp;
+        first = false;
+    }
+    js += "}, \"steps\": {";
+    for (int i = 0; i < kStepKinds; ++i) {
+        char tmp[96];
+        snprintf(tmp, sizeof(tmp), "%s\"%s\": %llu", i ? ", " : "", kinds[i], g_prof.steps[i]);
+        js += tmp;
+    }
+    js += "}}";
+    if ((int)js.size() + 1 > len) return -(int)(js.size() + 1);
+    memcpy(buf, js.c_str(), js.size() + 1);
+    return (int)js.size();
 }
 
 }  // namespace mr3

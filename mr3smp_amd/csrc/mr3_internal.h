@@ -77,13 +77,14 @@ struct ClTask {
     const double* pL;
     const double* pDL;
     const double* pDLL;
-    double* cD;             // child representation (arena)
+    double* cD;             // child representation (arena, 4*bsize doubles)
     double* cL;
-    double* cDL;
+    double* cDL;            // also the trial factorisation of the right shift
     double* cDLL;
-    double* cwork;          // 2*bsize scratch for the new-RRR search
+    double tau;             // shift of the child relative to the parent
     long long arena_off;    // offset of this cluster's slot in the level arena
-    int eig_base, pad;      // first entry of this cluster in the level eigen list
+    int eig_base;           // first entry of this cluster in the level eigen list
+    int shift;              // 1: left end, 2: right end (dlarrf SHIFT)
 };
 
 // One eigenvector to compute (process_singleton.c singleton_t with begin == end).

@@ -21,6 +21,26 @@
 
 namespace mr3 {
 
+// Algorithmic work counters, in sweep elements per kernel family; a lane adds its
+// total once at the end, and only when the host enabled counting.
+enum StepKind : int {
+    kStepCoarse = 0,     // Sturm steps on T, 3 flops (odebz.f:410)
+    kStepRefine = 1,     // dlaneg steps in the root refinement, 4 flops (odneg.f:106-109)
+    kStepEndref = 2,     // dlaneg steps refining cluster ends, 4 flops
+    kStepR1v = 3,        // dlar1v elements, 16 flops (odr1v.f:185-362)
+    kStepR1vBis = 4,     // dlaneg steps of the singleton bisection fallback, 4 flops
+    kStepRrf = 5,        // dlarrf elements per sweep, 5 flops (odrrf.f:212-214)
+    kStepRrj = 6,        // dlarrj Sturm steps, 3 flops
+    kStepRefineCl = 7,   // dlaneg steps in cluster refinement, 4 flops
+    kStepKinds = 8
+};
+__device__ unsigned long long g_steps[kStepKinds];
+__constant__ int g_count_steps;
+
+__device__ __forceinline__ void add_steps(int kind, unsigned long long v) {
+    if (g_count_steps && v) atomicAdd(&g_steps[kind], v);
+}
+
 // ---------------------------------------------------------------------------------
 // Workgroup helpers (256 threads)
 // ---------------------------------------------------------------------------------
@@ -534,16 +554,18 @@ __global__ __launch_bounds__(256) void k_coarse(const MatDesc* __restrict__ md,
     const double atoli = 2.0 * 2.0 * kSafmin + 2.0 * 2.0 * pivmin;
     const double rtoli = S.bsrtol;
     const int itmax = (int)((log(right - left + pivmin) - log(pivmin)) / log(2.0)) + 2;
-    int nl = 0, nr = nbl;
+    int nl = 0, nr = nbl, sweeps = 0;
     for (int it = 0; it < itmax; ++it) {
         double tmp1 = fabs(right - left), tmp2 = fmax(fabs(right), fabs(left));
         if (tmp1 < fmax(fmax(atoli, pivmin), rtoli * tmp2)) break;
         double mid = 0.5 * (left + right);
         int c = count_T_ijob2(nbl, d, e2, mid, pivmin);
+        ++sweeps;
         c = min(nr, max(nl, c));
         if (c >= i) { right = mid; nr = c; }
         else { left = mid; nl = c; }
     }
+    add_steps(kStepCoarse, (unsigned long long)sweeps * nbl);
     const double w = 0.5 * (left + right);
     W[gidx] = w;
     Wcoarse[gidx] = w;
@@ -801,8 +823,9 @@ __global__ __launch_bounds__(256) void k_refine_root(const MatDesc* __restrict__
         double rgap = Wgap[gidx];
         double lgap = (s == br.wbeg) ? rgap : Wgap[gidx - 1];
         double gap = fmin(lgap, rgap);
-        dlarrb1(nbl, D + off + br.bbeg, DLLr + off + br.bbeg, Windex[gidx], S.rtol1, S.rtol2,
-                gap, &w, &we, S.pivmin, br.spdiam, nbl);
+        int sw = dlarrb1(nbl, D + off + br.bbeg, DLLr + off + br.bbeg, Windex[gidx], S.rtol1,
+                         S.rtol2, gap, &w, &we, S.pivmin, br.spdiam, nbl);
+        add_steps(kStepRefine, (unsigned long long)sw * nbl);
     }
     Wtmp[gidx] = w;
     Werrtmp[gidx] = we;
@@ -871,8 +894,9 @@ __global__ __launch_bounds__(256) void k_rrj(const MatDesc* __restrict__ md,
     const BlockRec br = blk[off + iblock[gidx] - 1];
     const int nbl = br.bend - br.bbeg + 1;
     double w = W[gidx], we = Werr[gidx];
-    dlarrj1(nbl, Dcopy + off + br.bbeg, E2copy + off + br.bbeg, Windex[gidx], 4 * DBL_EPSILON,
-            &w, &we, S.pivmin, S.spdiam);
+    int sw = dlarrj1(nbl, Dcopy + off + br.bbeg, E2copy + off + br.bbeg, Windex[gidx],
+                     4 * DBL_EPSILON, &w, &we, S.pivmin, S.spdiam);
+    add_steps(kStepRrj, (unsigned long long)sw * nbl);
     W[gidx] = w;
     Werr[gidx] = we;
 }
@@ -882,11 +906,15 @@ __global__ __launch_bounds__(256) void k_rrj(const MatDesc* __restrict__ md,
 // lambda + sigma across blocks, by local index within a block.  One workgroup per
 // matrix; a rank is formed by binary search in every other block.
 // ---------------------------------------------------------------------------------
-__device__ __forceinline__ bool zkey_before(double la, int ia, double lb, int ib) {
-    // true if (lb, ib) sorts before (la, ia) for entries of different blocks
+__device__ __forceinline__ bool zkey_before(double la, int ia, int ba, double lb, int ib, int bb) {
+    // true if (lb, ib) of block bb sorts before (la, ia) of block ba (bb != ba).  The
+    // reference's comparator (mrrr_vec.c:433-442) orders by lambda, then by local
+    // index, and leaves exact ties of both unordered; the block index settles those
+    // here so that the columns form a permutation.
     if (lb < la) return true;
     if (lb > la) return false;
-    return ib < ia;
+    if (ib != ia) return ib < ia;
+    return bb < ba;
 }
 
 __global__ __launch_bounds__(256) void k_zindex(const MatDesc* __restrict__ md,
@@ -913,7 +941,7 @@ __global__ __launch_bounds__(256) void k_zindex(const MatDesc* __restrict__ md,
             int lo = o.wbeg, hi = o.wend + 1;   // first entry not before (lam, il)
             while (lo < hi) {
                 int mid = (lo + hi) >> 1;
-                if (zkey_before(lam, il, W[off + mid] + o.sigma, Windex[off + mid])) lo = mid + 1;
+                if (zkey_before(lam, il, b, W[off + mid] + o.sigma, Windex[off + mid], ob)) lo = mid + 1;
                 else hi = mid;
             }
             rank += lo - o.wbeg;
@@ -976,7 +1004,7 @@ __device__ void classify_cluster(int cb, int ce, int depth, int mat, int off, in
         return;
     }
     if (nC) {
-        atomicAdd((unsigned long long*)&out.cnt->arena, (unsigned long long)(6LL * bsize * nC));
+        atomicAdd((unsigned long long*)&out.cnt->arena, (unsigned long long)(4LL * bsize * nC));
         atomicMax(&out.cnt->maxbl, bsize);
     }
     if (nS) atomicMax(&out.cnt->maxbl, bsize);
@@ -1000,7 +1028,9 @@ __device__ void classify_cluster(int cb, int ce, int depth, int mat, int off, in
                 c.bbeg = bbeg; c.bsize = bsize; c.bl_spdiam = bl_spdiam;
                 c.lgap = (f == cb) ? cl_lgap : Wgap[f - 1];
                 c.pD = rD; c.pL = rL; c.pDL = rDL; c.pDLL = rDLL;
-                c.cD = c.cL = c.cDL = c.cDLL = c.cwork = nullptr;
+                c.cD = c.cL = c.cDL = c.cDLL = nullptr;
+                c.tau = 0.0;
+                c.shift = 0;
                 c.arena_off = -1;
                 c.eig_base = ebase;
                 for (int j = f; j <= l; ++j) {
@@ -1095,20 +1125,19 @@ __global__ __launch_bounds__(64) void k_classify(const ClTask* __restrict__ C, i
                      t.cL, t.cDL, t.cDLL, Wgap, Wsh, out);
 }
 
-// Assigns each cluster of a level its arena slot (6 * bsize doubles).
+// Assigns each cluster of a level its arena slot (4 * bsize doubles).
 __global__ __launch_bounds__(256) void k_arena_assign(ClTask* C, int nC, double* arena,
                                                       unsigned long long* bump) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= nC) return;
     ClTask& t = C[c];
-    unsigned long long o = atomicAdd(bump, (unsigned long long)(6LL * t.bsize));
+    unsigned long long o = atomicAdd(bump, (unsigned long long)(4LL * t.bsize));
     double* base = arena + o;
     t.arena_off = (long long)o;
     t.cD = base;
     t.cL = base + t.bsize;
     t.cDL = base + 2 * t.bsize;
     t.cDLL = base + 3 * t.bsize;
-    t.cwork = base + 4 * t.bsize;
 }
 
 // ---------------------------------------------------------------------------------
@@ -1144,7 +1173,9 @@ __device__ int dlarrf_lane(int n, const double* __restrict__ d, const double* __
     int ktry = 0;
     const double growthbound = kMaxGrowth1 * spdiam;
     double max1 = 0.0, max2 = 0.0;
+    int sweeps = 0;
     for (;;) {
+        ++sweeps;
         int sawnan1 = 0, sawnan2 = 0;
         ldelta = fmin(ldmax, ldelta);
         rdelta = fmin(rdmax, rdelta);
@@ -1161,6 +1192,7 @@ __device__ int dlarrf_lane(int n, const double* __restrict__ d, const double* __
         }
         sawnan1 = sawnan1 || isnan(max1);
         if (forcer || (max1 <= growthbound && !sawnan1)) { *sigma = lsigma; shift = 1; break; }
+        ++sweeps;
         s = -rsigma;
         work[0] = d[0] + s;
         if (fabs(work[0]) < pivmin) { work[0] = -pivmin; sawnan2 = 1; }
@@ -1224,74 +1256,92 @@ __device__ int dlarrf_lane(int n, const double* __restrict__ d, const double* __
         forcer = 1;
         (void)fail;
     }
+    add_steps(kStepRrf, (unsigned long long)sweeps * n);
     return shift;
 }
 
-__global__ __launch_bounds__(64) void k_newrrr(ClTask* C, int nC, const int* __restrict__ Windex,
-                                               double* Wsh, double* Werr, double* Wgap,
-                                               const MatState* __restrict__ ms,
-                                               const MatDesc* __restrict__ md) {
+// Extremal eigenvalues of every cluster to full accuracy against the parent
+// (process_cluster.c:168-195), one lane per cluster end.  The two dlarrb calls of a
+// cluster run with a zero gap and do not depend on each other; results go to Wtmp
+// and are committed by k_dlarrf.
+__global__ __launch_bounds__(64) void k_endref(const ClTask* __restrict__ C, int nC,
+                                               const int* __restrict__ Windex,
+                                               const double* __restrict__ Wsh,
+                                               const double* __restrict__ Werr,
+                                               const MatState* __restrict__ ms, double* Wtmp,
+                                               double* Werrtmp) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= 2 * nC) return;
+    const ClTask t = C[e >> 1];
+    const int slot = (e & 1) ? t.ce : t.cb;
+    const MatState& S = ms[t.mat];
+    double w = Wsh[slot], we = Werr[slot];
+    int sw = dlarrb1(t.bsize, t.pD, t.pDLL, Windex[slot], S.rqtol, S.rqtol, 0.0, &w, &we,
+                     S.pivmin, t.bl_spdiam, t.bsize);
+    add_steps(kStepEndref, (unsigned long long)sw * t.bsize);
+    Wtmp[slot] = w;
+    Werrtmp[slot] = we;
+}
+
+// New representation of every cluster by dlarrf (odrrf.f:140-384), one lane per
+// cluster.  The left-shift factorisation goes to cD/cL, the right-shift one to the
+// cDL/cDLL space; k_newrrr_fin keeps the chosen one.
+__global__ __launch_bounds__(64) void k_dlarrf(ClTask* C, int nC, double* Wsh, double* Werr,
+                                               double* Wgap, const double* __restrict__ Wtmp,
+                                               const double* __restrict__ Werrtmp,
+                                               const MatState* __restrict__ ms) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nC) return;
+    ClTask& t = C[c];
+    const int cb = t.cb, ce = t.ce, clsize = ce - cb + 1;
+    // Wgap[cb] from the refined left end and the neighbour as it was before the right
+    // end was refined (process_cluster.c:186-189), then commit both ends
+    Wgap[cb] = fmax(0.0, (Wsh[cb + 1] - Werr[cb + 1]) - (Wtmp[cb] + Werrtmp[cb]));
+    Wsh[cb] = Wtmp[cb];
+    Werr[cb] = Werrtmp[cb];
+    Wsh[ce] = Wtmp[ce];
+    Werr[ce] = Werrtmp[ce];
+    double tau;
+    int sh = dlarrf_lane(t.bsize, t.pD, t.pL, t.pDL, Wsh + cb, Wgap + cb, Werr + cb, clsize,
+                         t.bl_spdiam, t.lgap, Wgap[ce], ms[t.mat].pivmin, &tau, t.cD, t.cL,
+                         t.cDL);
+    t.tau = tau;
+    t.shift = sh;
+}
+
+// Keeps the chosen factorisation, accumulates the shift, forms DL and DLL and moves
+// the cluster's eigenvalues to the new representation (process_cluster.c:206-232).
+__global__ __launch_bounds__(64) void k_newrrr_fin(const ClTask* __restrict__ C, int nC,
+                                                   double* Wsh, double* Werr) {
     const int c = blockIdx.x;
     if (c >= nC) return;
     const ClTask t = C[c];
-    const int lane = threadIdx.x;
     const int n = t.bsize;
-    const double pivmin = ms[t.mat].pivmin;
-    const double rqtol = ms[t.mat].rqtol;
-    const int cb = t.cb, ce = t.ce, clsize = ce - cb + 1;
-    const int off = md[t.mat].off;
-    __shared__ double sh_next_before, sh_next_err_before;
-    __shared__ int sh_shift;
-    __shared__ double sh_tau;
-    if (lane == 0) { sh_next_before = Wsh[cb + 1]; sh_next_err_before = Werr[cb + 1]; }
-    __syncthreads();
-    // extremal eigenvalues to full accuracy against the parent (:168-195)
-    if (lane < 2) {
-        const int slot = (lane == 0) ? cb : ce;
-        double w = Wsh[slot], we = Werr[slot];
-        const int p = Windex[slot];
-        dlarrb1(n, t.pD, t.pDLL, p, rqtol, rqtol, 0.0, &w, &we, pivmin, t.bl_spdiam, n);
-        Wsh[slot] = w;
-        Werr[slot] = we;
-    }
-    __syncthreads();
-    if (lane == 0) {
-        // Wgap[cb] from the refined left end and the (not yet refined when the
-        // reference forms it) neighbour (:186-189)
-        double nb = (clsize == 2) ? sh_next_before : Wsh[cb + 1];
-        double nbe = (clsize == 2) ? sh_next_err_before : Werr[cb + 1];
-        Wgap[cb] = fmax(0.0, (nb - nbe) - (Wsh[cb] + Werr[cb]));
-        (void)off;
-    }
-    __syncthreads();
-    if (lane == 0) {
-        double tau;
-        int sh = dlarrf_lane(n, t.pD, t.pL, t.pDL, Wsh + cb, Wgap + cb, Werr + cb, clsize,
-                             t.bl_spdiam, t.lgap, Wgap[ce], pivmin, &tau, t.cD, t.cL, t.cwork);
-        sh_shift = sh;
-        sh_tau = tau;
-    }
-    __syncthreads();
-    const double tau = sh_tau;
-    if (sh_shift == 2) {
-        for (int i = lane; i < n; i += 64) {
-            t.cD[i] = t.cwork[i];
-            if (i < n - 1) t.cL[i] = t.cwork[n + i];
-        }
-    }
-    __syncthreads();
-    if (lane == 0) t.cL[n - 1] = t.pL[n - 1] + tau;
+    const int lane = threadIdx.x;
+    const bool right = (t.shift == 2);
     for (int i = lane; i < n; i += 64) {
-        if (i < n - 1) {
-            double tmp = t.cD[i] * t.cL[i];
-            t.cDL[i] = tmp;
-            t.cDLL[i] = tmp * t.cL[i];
+        double dv, lv;
+        if (right) {
+            dv = t.cDL[i];                               // WORK(i+1)
+            lv = (i < n - 1) ? t.cDL[n + i] : 0.0;       // WORK(N+i+1)
+            t.cD[i] = dv;
+            if (i < n - 1) t.cL[i] = lv;
         } else {
+            dv = t.cD[i];
+            lv = (i < n - 1) ? t.cL[i] : 0.0;
+        }
+        if (i < n - 1) {
+            double tmp = dv * lv;
+            t.cDL[i] = tmp;
+            t.cDLL[i] = tmp * lv;
+        } else {
+            t.cL[n - 1] = t.pL[n - 1] + t.tau;
             t.cDL[i] = 0.0;
             t.cDLL[i] = 0.0;
         }
     }
-    for (int s = cb + lane; s <= ce; s += 64) {
+    const double tau = t.tau;
+    for (int s = t.cb + lane; s <= t.ce; s += 64) {
         double fudge = 3.0 * DBL_EPSILON * fabs(Wsh[s]);
         Wsh[s] -= tau;
         fudge += 4.0 * DBL_EPSILON * fabs(Wsh[s]);
@@ -1318,8 +1368,9 @@ __global__ __launch_bounds__(256) void k_refine_cl(const ClTask* __restrict__ C,
     double w = Wsh[slot], we = Werr[slot];
     double rgap = Wgap[slot];
     double lgap = (slot == t.cb) ? rgap : Wgap[slot - 1];
-    dlarrb1(t.bsize, t.cD, t.cDLL, Windex[slot], S.rtol1, S.rtol2, fmin(lgap, rgap), &w, &we,
-            S.pivmin, t.bl_spdiam, t.bsize);
+    int sw = dlarrb1(t.bsize, t.cD, t.cDLL, Windex[slot], S.rtol1, S.rtol2, fmin(lgap, rgap),
+                     &w, &we, S.pivmin, t.bl_spdiam, t.bsize);
+    add_steps(kStepRefineCl, (unsigned long long)sw * t.bsize);
     Wtmp[slot] = w;
     Werrtmp[slot] = we;
 }
@@ -1375,31 +1426,34 @@ __device__ void dlar1v_lane(int n, double lambda, const double* __restrict__ D,
                             int wantnc, int rin, const Scr& sc, R1v* o) {
     int r1, r2;
     if (rin == 0) { r1 = 1; r2 = n; } else { r1 = rin; r2 = rin; }
-    // stationary transform, odr1v.f:185-229 (S(0) = 0 since B1 = 1)
+    // stationary transform, odr1v.f:185-229 (S(0) = 0 since B1 = 1).  Loops 50 and 51
+    // of the reference run back to back; NaN in either sends both to the slow form.
     int neg1 = 0;
     double s = 0.0 - lambda;
     sc.SG(0) = 0.0;
-    for (int i = 1; i <= r1 - 1; ++i) {
-        double dplus = D[i - 1] + s;
-        double lp = LD[i - 1] / dplus;
-        sc.LP(i) = lp;
-        neg1 += (dplus < 0.0);
-        double si = s * lp * L[i - 1];
-        sc.SG(i) = si;
-        s = si - lambda;
-    }
-    int sawnan1 = isnan(s);
-    if (!sawnan1) {
-        for (int i = r1; i <= r2 - 1; ++i) {
-            double dplus = D[i - 1] + s;
-            double lp = LD[i - 1] / dplus;
-            sc.LP(i) = lp;
-            double si = s * lp * L[i - 1];
-            sc.SG(i) = si;
-            s = si - lambda;
+    const int stop = r2;   // steps i = 1 .. r2-1
+    for (int i0 = 1; i0 < stop; i0 += kU) {
+        double dd[kU], ld[kU], ll[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            int j = min(i0 + u, stop - 1) - 1;
+            dd[u] = D[j]; ld[u] = LD[j]; ll[u] = L[j];
         }
-        sawnan1 = isnan(s);
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            const int i = i0 + u;
+            if (i < stop) {
+                double dplus = dd[u] + s;
+                double lp = ld[u] / dplus;
+                sc.LP(i) = lp;
+                neg1 += (i <= r1 - 1) && (dplus < 0.0);
+                double si = s * lp * ll[u];
+                sc.SG(i) = si;
+                s = si - lambda;
+            }
+        }
     }
+    const int sawnan1 = isnan(s);
     if (sawnan1) {
         neg1 = 0;
         s = 0.0 - lambda;
@@ -1415,74 +1469,63 @@ __device__ void dlar1v_lane(int n, double lambda, const double* __restrict__ D,
             s = si - lambda;
         }
     }
-    // progressive transform, odr1v.f:234-260; gamma(I) = S(I) + P(I) formed on the fly
+    // progressive transform, odr1v.f:234-260.  gamma(I) = S(I) + P(I) is formed as
+    // P(I) appears (with the zero substitution of odr1v.f:277) and stored over S(I)
+    // for the twist search; gamma(R1-1) is kept raw for the count.
     int neg2 = 0;
-    double p = D[n - 1] - lambda;
+    const double pn = D[n - 1] - lambda;   // P(BN-1)
     double gam_r1m1 = 0.0;
-    int sawnan2 = 0;
-    {
-        double pi = p;  // P(BN-1)
-        if (r1 - 1 == n - 1) gam_r1m1 = sc.SG(n - 1) + pi;
-        else if (rin == 0) {
-            double gm = sc.SG(n - 1) + pi;
-            if (gm == 0.0) gm = kEps * sc.SG(n - 1);
-            sc.SG(n - 1) = gm;
-        }
-        for (int i = n - 1; i >= r1; --i) {
-            double dminus = LLD[i - 1] + pi;
-            double tmp = D[i - 1] / dminus;
-            neg2 += (dminus < 0.0);
-            sc.UM(i) = L[i - 1] * tmp;
-            double pim1 = pi * tmp - lambda;   // P(I-1)
-            if (i - 1 == r1 - 1) gam_r1m1 = sc.SG(i - 1) + pim1;
-            else if (rin == 0) {
-                double gm = sc.SG(i - 1) + pim1;
-                if (gm == 0.0) gm = kEps * sc.SG(i - 1);
-                sc.SG(i - 1) = gm;
-            }
-            pi = pim1;
-        }
-        sawnan2 = isnan(pi);
+    double pi = pn;
+    if (r1 - 1 == n - 1) gam_r1m1 = sc.SG(n - 1) + pi;
+    else if (rin == 0) {
+        double gm = sc.SG(n - 1) + pi;
+        if (gm == 0.0) gm = kEps * sc.SG(n - 1);
+        sc.SG(n - 1) = gm;
     }
-    if (sawnan2) {
-        neg2 = 0;
-        double pi = p;
-        if (r1 - 1 == n - 1) gam_r1m1 = sc.SG(n - 1) + pi;
-        // recompute S for the gamma search: the fast sweep overwrote SG below r2
-        for (int i = n - 1; i >= r1; --i) {
-            double dminus = LLD[i - 1] + pi;
-            if (fabs(dminus) < pivmin) dminus = -pivmin;
-            double tmp = D[i - 1] / dminus;
-            neg2 += (dminus < 0.0);
-            sc.UM(i) = L[i - 1] * tmp;
-            double pim1 = pi * tmp - lambda;
-            if (tmp == 0.0) pim1 = D[i - 1] - lambda;
-            pi = pim1;
+    for (int i0 = n - 1; i0 >= r1; i0 -= kU) {
+        double dd[kU], ll[kU], lld[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            int j = max(i0 - u, r1) - 1;
+            dd[u] = D[j]; ll[u] = L[j]; lld[u] = LLD[j];
         }
-        // the gamma values need S again: redo the stationary part into SG, then
-        // combine with a second progressive pass that only forms P.
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            const int i = i0 - u;
+            if (i >= r1) {
+                double dminus = lld[u] + pi;
+                double tmp = dd[u] / dminus;
+                neg2 += (dminus < 0.0);
+                sc.UM(i) = ll[u] * tmp;
+                double pim1 = pi * tmp - lambda;   // P(I-1)
+                if (i - 1 == r1 - 1) gam_r1m1 = sc.SG(i - 1) + pim1;
+                else if (rin == 0) {
+                    double sv = sc.SG(i - 1);
+                    double gm = sv + pim1;
+                    if (gm == 0.0) gm = kEps * sv;
+                    sc.SG(i - 1) = gm;
+                }
+                pi = pim1;
+            }
+        }
+    }
+    const int sawnan2 = isnan(pi);
+    if (sawnan2) {
+        // slow progressive form (odr1v.f:240-258); S is formed again because the fast
+        // sweep overwrote it with gamma values
+        neg2 = 0;
         double ss = 0.0 - lambda;
         sc.SG(0) = 0.0;
-        if (!sawnan1) {
-            for (int i = 1; i <= r2 - 1; ++i) {
-                double dplus = D[i - 1] + ss;
-                double lp = LD[i - 1] / dplus;
-                double si = ss * lp * L[i - 1];
-                sc.SG(i) = si;
-                ss = si - lambda;
-            }
-        } else {
-            for (int i = 1; i <= r2 - 1; ++i) {
-                double dplus = D[i - 1] + ss;
-                if (fabs(dplus) < pivmin) dplus = -pivmin;
-                double lp = LD[i - 1] / dplus;
-                double si = ss * lp * L[i - 1];
-                if (lp == 0.0) si = LLD[i - 1];
-                sc.SG(i) = si;
-                ss = si - lambda;
-            }
+        for (int i = 1; i <= r2 - 1; ++i) {
+            double dplus = D[i - 1] + ss;
+            if (sawnan1 && fabs(dplus) < pivmin) dplus = -pivmin;
+            double lp = LD[i - 1] / dplus;
+            double si = ss * lp * L[i - 1];
+            if (sawnan1 && lp == 0.0) si = LLD[i - 1];
+            sc.SG(i) = si;
+            ss = si - lambda;
         }
-        pi = p;
+        pi = pn;
         if (r1 - 1 == n - 1) gam_r1m1 = sc.SG(n - 1) + pi;
         else if (rin == 0) {
             double gm = sc.SG(n - 1) + pi;
@@ -1493,6 +1536,8 @@ __device__ void dlar1v_lane(int n, double lambda, const double* __restrict__ D,
             double dminus = LLD[i - 1] + pi;
             if (fabs(dminus) < pivmin) dminus = -pivmin;
             double tmp = D[i - 1] / dminus;
+            neg2 += (dminus < 0.0);
+            sc.UM(i) = L[i - 1] * tmp;
             double pim1 = pi * tmp - lambda;
             if (tmp == 0.0) pim1 = D[i - 1] - lambda;
             if (i - 1 == r1 - 1) gam_r1m1 = sc.SG(i - 1) + pim1;
@@ -1519,7 +1564,7 @@ __device__ void dlar1v_lane(int n, double lambda, const double* __restrict__ D,
     o->sawnan = sawnan1 || sawnan2;
 }
 
-// The vector solve of odr1v.f:287-355.  With stage == nullptr it only forms ZTZ and
+// The vector solve of odr1v.f:287-355.  With stage == false it only forms ZTZ and
 // the support; otherwise it also writes z * scale for rows 1..n of the block into
 // the lane's SG region (zero outside the support).
 __device__ void solve_pass(int n, const double* __restrict__ LD, double gaptol, const Scr& sc,
@@ -1531,47 +1576,102 @@ __device__ void solve_pass(int n, const double* __restrict__ LD, double gaptol, 
     // upwards from r
     {
         double zi1 = 1.0, zi2 = 0.0;   // Z(I+1), Z(I+2)
-        int i = r - 1;
-        for (; i >= 1; --i) {
-            double zi;
-            if (!o->sawnan) zi = -(sc.LP(i) * zi1);
-            else if (zi1 == 0.0) zi = -(LD[i] / LD[i - 1]) * zi2;
-            else zi = -(sc.LP(i) * zi1);
-            if ((fabs(zi) + fabs(zi1)) * fabs(LD[i - 1]) < gaptol) {
-                zi = 0.0;
-                is0 = i + 1;
-                if (stage) sc.SG(i - 1) = 0.0;
-                --i;
-                break;
+        int lo = 0;                    // rows below lo+1 are zero
+        bool cut = false;
+        if (!o->sawnan) {
+            for (int i0 = r - 1; i0 >= 1 && !cut; i0 -= kU) {
+                double lp[kU], ld[kU];
+#pragma unroll
+                for (int u = 0; u < kU; ++u) {
+                    int i = max(i0 - u, 1);
+                    lp[u] = sc.LP(i);
+                    ld[u] = LD[i - 1];
+                }
+#pragma unroll
+                for (int u = 0; u < kU; ++u) {
+                    const int i = i0 - u;
+                    if (i >= 1 && !cut) {
+                        double zi = -(lp[u] * zi1);
+                        if ((fabs(zi) + fabs(zi1)) * fabs(ld[u]) < gaptol) {
+                            is0 = i + 1;
+                            lo = i;
+                            cut = true;
+                        } else {
+                            ztz = ztz + zi * zi;
+                            if (stage) sc.SG(i - 1) = zi * scale;
+                            zi1 = zi;
+                        }
+                    }
+                }
             }
-            ztz = ztz + zi * zi;
-            if (stage) sc.SG(i - 1) = zi * scale;
-            zi2 = zi1;
-            zi1 = zi;
+        } else {
+            for (int i = r - 1; i >= 1; --i) {
+                double zi;
+                if (zi1 == 0.0) zi = -(LD[i] / LD[i - 1]) * zi2;
+                else zi = -(sc.LP(i) * zi1);
+                if ((fabs(zi) + fabs(zi1)) * fabs(LD[i - 1]) < gaptol) {
+                    is0 = i + 1;
+                    lo = i;
+                    cut = true;
+                    break;
+                }
+                ztz = ztz + zi * zi;
+                if (stage) sc.SG(i - 1) = zi * scale;
+                zi2 = zi1;
+                zi1 = zi;
+            }
         }
-        if (stage) for (; i >= 1; --i) sc.SG(i - 1) = 0.0;
+        if (stage && cut) for (int i = lo; i >= 1; --i) sc.SG(i - 1) = 0.0;
     }
     // downwards from r
     {
         double zi = 1.0, zim1 = 0.0;   // Z(I), Z(I-1)
-        int i = r;
-        for (; i <= n - 1; ++i) {
-            double zn;
-            if (!o->sawnan) zn = -(sc.UM(i) * zi);
-            else if (zi == 0.0) zn = -(LD[i - 2] / LD[i - 1]) * zim1;
-            else zn = -(sc.UM(i) * zi);
-            if ((fabs(zi) + fabs(zn)) * fabs(LD[i - 1]) < gaptol) {
-                is1 = i;
-                if (stage) sc.SG(i) = 0.0;
-                ++i;
-                break;
+        int hi = n;                    // rows from hi+1 on are zero
+        bool cut = false;
+        if (!o->sawnan) {
+            for (int i0 = r; i0 <= n - 1 && !cut; i0 += kU) {
+                double um[kU], ld[kU];
+#pragma unroll
+                for (int u = 0; u < kU; ++u) {
+                    int i = min(i0 + u, n - 1);
+                    um[u] = sc.UM(i);
+                    ld[u] = LD[i - 1];
+                }
+#pragma unroll
+                for (int u = 0; u < kU; ++u) {
+                    const int i = i0 + u;
+                    if (i <= n - 1 && !cut) {
+                        double zn = -(um[u] * zi);
+                        if ((fabs(zi) + fabs(zn)) * fabs(ld[u]) < gaptol) {
+                            is1 = i;
+                            hi = i;
+                            cut = true;
+                        } else {
+                            ztz = ztz + zn * zn;
+                            if (stage) sc.SG(i) = zn * scale;
+                            zi = zn;
+                        }
+                    }
+                }
             }
-            ztz = ztz + zn * zn;
-            if (stage) sc.SG(i) = zn * scale;
-            zim1 = zi;
-            zi = zn;
+        } else {
+            for (int i = r; i <= n - 1; ++i) {
+                double zn;
+                if (zi == 0.0) zn = -(LD[i - 2] / LD[i - 1]) * zim1;
+                else zn = -(sc.UM(i) * zi);
+                if ((fabs(zi) + fabs(zn)) * fabs(LD[i - 1]) < gaptol) {
+                    is1 = i;
+                    hi = i;
+                    cut = true;
+                    break;
+                }
+                ztz = ztz + zn * zn;
+                if (stage) sc.SG(i) = zn * scale;
+                zim1 = zi;
+                zi = zn;
+            }
         }
-        if (stage) for (; i <= n - 1; ++i) sc.SG(i) = 0.0;
+        if (stage && cut) for (int i = hi; i <= n - 1; ++i) sc.SG(i) = 0.0;
     }
     if (!stage) {
         double tmp = 1.0 / ztz;
@@ -1646,12 +1746,14 @@ __global__ __launch_bounds__(64) void k_singleton(const STask* __restrict__ T, i
             int usedBS = 0, usedRQ = 0, needBS = 0, wantnc = 1, r = 0;
             double bstres = 0.0, bstw = 0.0, wsh = lambda;
             R1v o;
+            unsigned long long r1v_el = 0, bis_el = 0;
             for (int kk = 1; kk <= kMaxIterRQI; ++kk) {
                 if (needBS) {
                     usedBS = 1;
                     double w = wsh, we = werr;
-                    dlarrb1(n, t.D, t.DLL, i_local, 0.0, 2.0 * DBL_EPSILON, 0.0, &w, &we, pivmin,
-                            t.bl_spdiam, r);
+                    bis_el += (unsigned long long)n *
+                              dlarrb1(n, t.D, t.DLL, i_local, 0.0, 2.0 * DBL_EPSILON, 0.0, &w, &we,
+                                      pivmin, t.bl_spdiam, r);
                     wsh = w;
                     werr = we;
                     lambda = wsh;
@@ -1660,6 +1762,7 @@ __global__ __launch_bounds__(64) void k_singleton(const STask* __restrict__ T, i
                 wantnc = usedBS ? 0 : 1;
                 dlar1v_lane(n, lambda, t.D, t.L, t.DL, t.DLL, pivmin, gaptol, wantnc, r, sc, &o);
                 solve_pass(n, t.DL, gaptol, sc, &o, false, 0.0);
+                r1v_el += n;
                 r = o.r;
                 if (kk == 1 || o.resid < bstres) { bstres = o.resid; bstw = lambda; }
                 if (!usedBS && o.resid > tol * gap && fabs(o.rqcorr) > RQtol * fabs(lambda)) {
@@ -1680,7 +1783,10 @@ __global__ __launch_bounds__(64) void k_singleton(const STask* __restrict__ T, i
                 lambda = bstw;
                 dlar1v_lane(n, lambda, t.D, t.L, t.DL, t.DLL, pivmin, gaptol, wantnc, r, sc, &o);
                 solve_pass(n, t.DL, gaptol, sc, &o, false, 0.0);
+                r1v_el += n;
             }
+            add_steps(kStepR1v, r1v_el);
+            add_steps(kStepR1vBis, bis_el);
             solve_pass(n, t.DL, gaptol, sc, &o, true, o.nrminv);
             Zs[0] = o.isupp0 + bl_begin;
             Zs[1] = o.isupp1 + bl_begin;

@@ -16,6 +16,11 @@ constexpr double kEps = DBL_EPSILON;   // odmch.f 'P'
 constexpr double kSafmin = DBL_MIN;    // odmch.f 'S' / 'U'
 constexpr int kMaxExpand = 128;        // cap on bracket doublings (2^128 spans any range)
 
+// Loop chunk: every sweep loads kU consecutive elements before it runs the kU
+// dependent steps, so the loads of a chunk are in flight together instead of each
+// step waiting on its own.
+constexpr int kU = 8;
+
 // Sturm count of T - x I, dlaebz IJOB=1 convention (odebz.f:265-277): a pivot with
 // |t| < pivmin becomes -pivmin, a pivot <= 0 counts.
 __device__ __forceinline__ int count_T_ijob1(int n, const double* __restrict__ d,
@@ -24,10 +29,22 @@ __device__ __forceinline__ int count_T_ijob1(int n, const double* __restrict__ d
     double t = d[0] - x;
     if (fabs(t) < pivmin) t = -pivmin;
     int c = (t <= 0.0);
-    for (int j = 1; j < n; ++j) {
-        t = d[j] - e2[j - 1] / t - x;
-        if (fabs(t) < pivmin) t = -pivmin;
-        c += (t <= 0.0);
+    for (int j0 = 1; j0 < n; j0 += kU) {
+        double dd[kU], ee[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            int j = min(j0 + u, n - 1);
+            dd[u] = d[j];
+            ee[u] = e2[j - 1];
+        }
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            if (j0 + u < n) {
+                t = dd[u] - ee[u] / t - x;
+                if (fabs(t) < pivmin) t = -pivmin;
+                c += (t <= 0.0);
+            }
+        }
     }
     return c;
 }
@@ -40,9 +57,21 @@ __device__ __forceinline__ int count_T_ijob2(int n, const double* __restrict__ d
     double t = d[0] - x;
     int c = 0;
     if (t <= pivmin) { c = 1; t = fmin(t, -pivmin); }
-    for (int j = 1; j < n; ++j) {
-        t = d[j] - e2[j - 1] / t - x;
-        if (t <= pivmin) { ++c; t = fmin(t, -pivmin); }
+    for (int j0 = 1; j0 < n; j0 += kU) {
+        double dd[kU], ee[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            int j = min(j0 + u, n - 1);
+            dd[u] = d[j];
+            ee[u] = e2[j - 1];
+        }
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            if (j0 + u < n) {
+                t = dd[u] - ee[u] / t - x;
+                if (t <= pivmin) { ++c; t = fmin(t, -pivmin); }
+            }
+        }
     }
     return c;
 }
@@ -56,20 +85,45 @@ __device__ __forceinline__ int dlaneg(int n, const double* __restrict__ d,
                                       const double* __restrict__ lld, double sigma, int r) {
     int neg = 0;
     double t = -sigma;
-    for (int j = 0; j < r - 1; ++j) {
-        double dplus = d[j] + t;
-        neg += (dplus < 0.0);
-        double tmp = t / dplus;
-        tmp = isnan(tmp) ? 1.0 : tmp;
-        t = tmp * lld[j] - sigma;
+    const int top = r - 1;   // steps j = 0 .. r-2
+    for (int j0 = 0; j0 < top; j0 += kU) {
+        double dd[kU], ll[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            int j = min(j0 + u, top - 1);
+            dd[u] = d[j];
+            ll[u] = lld[j];
+        }
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            if (j0 + u < top) {
+                double dplus = dd[u] + t;
+                neg += (dplus < 0.0);
+                double tmp = t / dplus;
+                tmp = isnan(tmp) ? 1.0 : tmp;
+                t = tmp * ll[u] - sigma;
+            }
+        }
     }
     double p = d[n - 1] - sigma;
-    for (int j = n - 2; j >= r - 1; --j) {
-        double dminus = lld[j] + p;
-        neg += (dminus < 0.0);
-        double tmp = p / dminus;
-        tmp = isnan(tmp) ? 1.0 : tmp;
-        p = tmp * d[j] - sigma;
+    for (int j0 = n - 2; j0 >= r - 1; j0 -= kU) {
+        double dd[kU], ll[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            int j = max(j0 - u, r - 1);
+            dd[u] = d[j];
+            ll[u] = lld[j];
+        }
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            if (j0 - u >= r - 1) {
+                double dminus = ll[u] + p;
+                neg += (dminus < 0.0);
+                double tmp = p / dminus;
+                tmp = isnan(tmp) ? 1.0 : tmp;
+                p = tmp * dd[u] - sigma;
+            }
+        }
     }
     double gamma = (t + sigma) + p;
     neg += (gamma < 0.0);
@@ -82,7 +136,7 @@ __device__ __forceinline__ int dlaneg(int n, const double* __restrict__ d,
 // exact.  `gap` is min(left gap, right gap) as dlarrb forms it.  On return *w and
 // *werr hold the refined midpoint and half width, or are left unchanged when the
 // interval already met the tolerance (odrrb.f:198-205).
-__device__ __forceinline__ void dlarrb1(int n, const double* __restrict__ d,
+__device__ __forceinline__ int dlarrb1(int n, const double* __restrict__ d,
                                         const double* __restrict__ lld, int i, double rtol1,
                                         double rtol2, double gap, double* w, double* werr,
                                         double pivmin, double spdiam, int twist) {
@@ -91,22 +145,23 @@ __device__ __forceinline__ void dlarrb1(int n, const double* __restrict__ d,
     int r = twist;
     if (r < 1 || r > n) r = n;
     double left = *w - *werr, right = *w + *werr;
+    int sweeps = 0;
     // The reference expands without a bound (odrrb.f:176-196); a zero error bound
     // would never move the end point, so the expansion is capped here.
     double back = *werr;
-    for (int g = 0; g < kMaxExpand && dlaneg(n, d, lld, left, r) > i - 1; ++g) {
+    for (int g = 0; g < kMaxExpand && (++sweeps, dlaneg(n, d, lld, left, r) > i - 1); ++g) {
         left = left - back;
         back = 2.0 * back;
     }
     back = *werr;
-    for (int g = 0; g < kMaxExpand && dlaneg(n, d, lld, right, r) < i; ++g) {
+    for (int g = 0; g < kMaxExpand && (++sweeps, dlaneg(n, d, lld, right, r) < i); ++g) {
         right = right + back;
         back = 2.0 * back;
     }
     double width = 0.5 * fabs(left - right);
     double tmp = fmax(fabs(left), fabs(right));
     double cvrgd = fmax(rtol1 * gap, rtol2 * tmp);
-    if (width <= cvrgd || width <= mnwdth) return;
+    if (width <= cvrgd || width <= mnwdth) return sweeps;
     for (int iter = 0;; ++iter) {
         double mid = 0.5 * (left + right);
         width = right - mid;
@@ -114,11 +169,13 @@ __device__ __forceinline__ void dlarrb1(int n, const double* __restrict__ d,
         cvrgd = fmax(rtol1 * gap, rtol2 * tmp);
         if (width <= cvrgd || width <= mnwdth || iter == maxitr) break;
         int negcnt = dlaneg(n, d, lld, mid, r);
+        ++sweeps;
         if (negcnt <= i - 1) left = mid;
         else right = mid;
     }
     *w = 0.5 * (left + right);
     *werr = right - *w;
+    return sweeps;
 }
 
 // Sturm count on T for dlarrj (odrrj.f:160-166): no pivmin clamp, count if < 0.
@@ -126,15 +183,27 @@ __device__ __forceinline__ int count_T_rrj(int n, const double* __restrict__ d,
                                            const double* __restrict__ e2, double s) {
     double dplus = d[0] - s;
     int cnt = (dplus < 0.0);
-    for (int j = 1; j < n; ++j) {
-        dplus = d[j] - s - e2[j - 1] / dplus;
-        cnt += (dplus < 0.0);
+    for (int j0 = 1; j0 < n; j0 += kU) {
+        double dd[kU], ee[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            int j = min(j0 + u, n - 1);
+            dd[u] = d[j];
+            ee[u] = e2[j - 1];
+        }
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            if (j0 + u < n) {
+                dplus = dd[u] - s - ee[u] / dplus;
+                cnt += (dplus < 0.0);
+            }
+        }
     }
     return cnt;
 }
 
 // dlarrj (odrrj.f:100-281) for one interval (intervals are independent).
-__device__ __forceinline__ void dlarrj1(int n, const double* __restrict__ d,
+__device__ __forceinline__ int dlarrj1(int n, const double* __restrict__ d,
                                         const double* __restrict__ e2, int i, double rtol,
                                         double* w, double* werr, double pivmin,
                                         double spdiam) {
@@ -142,14 +211,15 @@ __device__ __forceinline__ void dlarrj1(int n, const double* __restrict__ d,
     double left = *w - *werr, mid = *w, right = *w + *werr;
     double width = right - mid;
     double tmp = fmax(fabs(left), fabs(right));
-    if (width < rtol * tmp) return;
+    if (width < rtol * tmp) return 0;
+    int sweeps = 0;
     double fac = 1.0;
-    for (int g = 0; g < kMaxExpand && count_T_rrj(n, d, e2, left) > i - 1; ++g) {
+    for (int g = 0; g < kMaxExpand && (++sweeps, count_T_rrj(n, d, e2, left) > i - 1); ++g) {
         left = left - *werr * fac;
         fac = 2.0 * fac;
     }
     fac = 1.0;
-    for (int g = 0; g < kMaxExpand && count_T_rrj(n, d, e2, right) < i; ++g) {
+    for (int g = 0; g < kMaxExpand && (++sweeps, count_T_rrj(n, d, e2, right) < i); ++g) {
         right = right + *werr * fac;
         fac = 2.0 * fac;
     }
@@ -159,11 +229,13 @@ __device__ __forceinline__ void dlarrj1(int n, const double* __restrict__ d,
         tmp = fmax(fabs(left), fabs(right));
         if (width < rtol * tmp || iter == maxitr) break;
         int cnt = count_T_rrj(n, d, e2, mid);
+        ++sweeps;
         if (cnt <= i - 1) left = mid;
         else right = mid;
     }
     *w = 0.5 * (left + right);
     *werr = right - *w;
+    return sweeps;
 }
 
 // 48-bit LCG of dlaruv (odruv.f:330-387): x_i = seed * a^i mod 2^48.  dlarnv with

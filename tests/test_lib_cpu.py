@@ -1,0 +1,55 @@
+"""CPU checks of the product library: it is built, loads, and exports every entry
+point declared in include/mr3hip.h (no compute call: there is no GPU here)."""
+import ctypes
+import os
+import re
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    txt = open(os.path.join(REPO, "include", "mr3hip.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    names = re.findall(r"^[A-Za-z_][\w \*]*?\b(\w+)\s*\(", txt, flags=re.M)
+    return sorted(set(n for n in names if n not in ("if", "while")))
+
+
+def test_header_declares_the_dropin():
+    names = declared_functions()
+    assert "mrrr" in names and "mrrr_" in names and "mr3_solve_batch" in names
+
+
+def test_library_exports_every_declared_symbol():
+    import mr3smp_amd
+    assert os.path.exists(mr3smp_amd.LIB_PATH), "libmr3hip.so not built"
+    lib = ctypes.CDLL(mr3smp_amd.LIB_PATH)
+    missing = [n for n in declared_functions() if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_library_is_gfx950_code_object():
+    import mr3smp_amd
+    blob = open(mr3smp_amd.LIB_PATH, "rb").read()
+    assert b"gfx950" in blob
+
+
+def test_workload_generators():
+    from mr3smp_amd import workloads as WL
+    D, E = WL.cfg2()
+    assert len(D) == 20000 and E[-1] == 0.0
+    assert np.count_nonzero(D[:21] - np.abs(10 - np.arange(21.0))) == 0
+    glue = E[20:952 * 21:21]
+    assert np.all((glue >= 0.5e-8) & (glue <= 1e-8))
+    assert np.all(D[952 * 21:] == 2.0)
+    mats = WL.cfg3(64)
+    ns = [len(m[0]) for m in mats]
+    assert min(ns) >= 32 and max(ns) <= 2048
+    c4 = WL.cfg4(4)
+    for D, E, il, iu in c4:
+        n = len(D)
+        assert 4096 <= n <= 50000 and iu - il + 1 == int(np.ceil(0.1 * n)) and 1 <= il <= iu <= n
+    reqs = WL.cfg5(50, 10.0)
+    assert all(np.diff([r["t"] for r in reqs]) >= 0)
+    assert {r["range"] for r in reqs} <= {"A", "I", "V"}

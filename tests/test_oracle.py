@@ -1,0 +1,294 @@
+"""CPU tests of the oracle (oracle/mr3_oracle.c), the checker for the HIP path.
+
+The oracle is pinned against (1) the reference's own outputs stored in
+tests/golden/ref_cases.npz (written by tests/golden/make_golden.py from the compiled
+reference), bit for bit at the kernel level (dlaneg, dlar1v, dlarrb, dlarrf, dlarnv)
+and to the parity bars at the mrrr() level; (2) the compiled reference itself when
+oracle/_ref is built; (3) what the mathematics fixes: closed-form spectra, dense
+eigensolvers on small inputs, Sylvester inertia for Sturm counts.
+"""
+import ctypes
+import os
+
+import numpy as np
+import pytest
+
+from accuracy import DSTEMR_FLOOR, EPS, check_pair, check_support, w_tol
+from mrrr_ctypes import REF_SO, call_mrrr, load, resid_orth, tnorm
+
+P = ctypes.c_void_p
+
+
+def ptr(a):
+    return a.ctypes.data_as(P)
+
+
+def dense(D, E):
+    n = len(D)
+    return np.diag(D) + np.diag(E[: n - 1], 1) + np.diag(E[: n - 1], -1)
+
+
+# ---------------------------------------------------------------------------------
+# mrrr() level, against the reference's stored outputs
+# ---------------------------------------------------------------------------------
+def golden_names():
+    z = np.load(os.path.join(os.path.dirname(__file__), "golden", "ref_cases.npz"))
+    return [str(x) for x in z["names"]]
+
+
+@pytest.mark.parametrize("name", golden_names())
+def test_oracle_matches_reference_outputs(name, golden, oracle):
+    c = golden[name]
+    _, fn = oracle
+    D, E = c["D"], c["E"]
+    got = call_mrrr(fn, c["jobz"], c["rng"], D, E, **c["kw"])
+    info, m = (int(v) for v in c["info_m"])
+    ref = dict(info=info, m=m, W=c["W"])
+    if c["jobz"] == "C":
+        assert got["info"] == 0 and got["m"] == m
+        return
+    if "resid_orth" in c:
+        ref["resid"], ref["orth"] = (float(v) for v in c["resid_orth"])
+    if "Z" in c:
+        ref["Z"] = c["Z"]
+    dstemr_route = c["threads"] == 1 or len(D) < 500
+    check_pair(D, E, got, ref, name, vectors=(c["jobz"] == "V"),
+               floor=DSTEMR_FLOOR if dstemr_route else 1.0)
+
+
+# ---------------------------------------------------------------------------------
+# kernel level: bit-identical to the reference's Fortran on stored vectors
+# ---------------------------------------------------------------------------------
+def test_dlaneg_bit_exact(golden, oracle_lib):
+    z = np.load(os.path.join(os.path.dirname(__file__), "golden", "ref_cases.npz"))
+    d, lld = z["k/d"], z["k/lld"]
+    n = len(d)
+    f = oracle_lib.oracle_dlaneg
+    f.restype = ctypes.c_int
+    f.argtypes = [ctypes.c_int, P, P, ctypes.c_double, ctypes.c_double, ctypes.c_int]
+    got = [f(n, ptr(d), ptr(lld), float(s), np.finfo(float).tiny, r)
+           for s in z["k/neg_sigmas"] for r in (1, 17, n)]
+    assert np.array_equal(np.array(got), z["k/neg_counts"])
+
+
+def test_dlar1v_bit_exact(oracle_lib):
+    z = np.load(os.path.join(os.path.dirname(__file__), "golden", "ref_cases.npz"))
+    d, l, ld, lld, ev = z["k/d"], z["k/l"], z["k/ld"], z["k/lld"], z["k/ev"]
+    n = len(d)
+    f = oracle_lib.oracle_dlar1v
+    f.restype = None
+    f.argtypes = ([ctypes.c_int] * 3 + [ctypes.c_double] + [P] * 4 + [ctypes.c_double] * 2 +
+                  [P, ctypes.c_int] + [P] * 9)
+    for row in z["k/r1v"]:
+        k, rin, gaptol = int(row[0]), int(row[1]), float(row[2])
+        zz = np.zeros(n)
+        work = np.zeros(4 * n)
+        isz = np.zeros(2, dtype=np.int32)
+        negc, r = ctypes.c_int(), ctypes.c_int(rin)
+        ztz, ming, nrm, res, rq = (ctypes.c_double() for _ in range(5))
+        f(n, 1, n, float(ev[k] * (1 + 1e-10)), ptr(d), ptr(l), ptr(ld), ptr(lld),
+          np.finfo(float).tiny, gaptol, ptr(zz), 1, ctypes.byref(negc), ctypes.byref(ztz),
+          ctypes.byref(ming), ctypes.byref(r), ptr(isz), ctypes.byref(nrm), ctypes.byref(res),
+          ctypes.byref(rq), ptr(work))
+        got = np.concatenate([[k, rin, gaptol, negc.value, ztz.value, ming.value, r.value,
+                               isz[0], isz[1], nrm.value, res.value, rq.value], zz])
+        assert np.array_equal(got, row), f"dlar1v k={k} r={rin}"
+
+
+def test_dlarrb_bit_exact(oracle_lib):
+    z = np.load(os.path.join(os.path.dirname(__file__), "golden", "ref_cases.npz"))
+    d, lld = z["k/d"], z["k/lld"]
+    n = len(d)
+    w, werr, wgap = (a.copy() for a in z["k/rrb_in"])
+    f = oracle_lib.oracle_dlarrb
+    f.restype = None
+    f.argtypes = ([ctypes.c_int, P, P, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                   ctypes.c_double, ctypes.c_int, P, P, P, ctypes.c_double, ctypes.c_double,
+                   ctypes.c_int])
+    f(n, ptr(d), ptr(lld), 1, n, 1e-8, 1e-11, 0, ptr(w), ptr(wgap), ptr(werr),
+      np.finfo(float).tiny, 3.0, n)
+    wo, weo, wgo = z["k/rrb_out"]
+    assert np.array_equal(w, wo) and np.array_equal(werr, weo) and np.array_equal(wgap, wgo)
+
+
+def test_dlarrf_bit_exact(oracle_lib):
+    z = np.load(os.path.join(os.path.dirname(__file__), "golden", "ref_cases.npz"))
+    d, l, ld, ev = z["k/d"], z["k/l"], z["k/ld"], z["k/ev"]
+    n = len(d)
+    wc, gc, ec = ev[10:14].copy(), np.full(4, 1e-3), np.full(4, 1e-12)
+    dplus, lplus, work = np.zeros(n), np.zeros(n), np.zeros(2 * n)
+    sig = ctypes.c_double()
+    f = oracle_lib.oracle_dlarrf
+    f.restype = ctypes.c_int
+    f.argtypes = ([ctypes.c_int, P, P, P, ctypes.c_int, ctypes.c_int, P, P, P] +
+                  [ctypes.c_double] * 4 + [P] * 4)
+    info = f(n, ptr(d), ptr(l), ptr(ld), 1, 4, ptr(wc), ptr(gc), ptr(ec), 3.0, 2e-3, 5e-3,
+             np.finfo(float).tiny, ctypes.byref(sig), ptr(dplus), ptr(lplus), ptr(work))
+    got = np.concatenate([[sig.value, info], dplus, lplus[: n - 1]])
+    assert np.array_equal(got, z["k/rrf_out"])
+
+
+def test_dlarnv_bit_exact(oracle_lib):
+    z = np.load(os.path.join(os.path.dirname(__file__), "golden", "ref_cases.npz"))
+    seed = (ctypes.c_int * 4)(1, 1, 1, 1)
+    x = np.zeros(300)
+    oracle_lib.oracle_dlarnv2(seed, 300, ptr(x))
+    assert np.array_equal(x, z["k/rnv300"])
+    assert list(seed) == list(z["k/rnv_seed_after"])
+
+
+# ---------------------------------------------------------------------------------
+# what the mathematics fixes
+# ---------------------------------------------------------------------------------
+@pytest.mark.parametrize("n", [3, 17, 64, 250])
+def test_toeplitz_closed_form(n, oracle):
+    _, fn = oracle
+    D, E = np.full(n, 2.0), np.ones(n)
+    got = call_mrrr(fn, "V", "A", D, E)
+    k = np.arange(1, n + 1)
+    lam = 2.0 - 2.0 * np.cos(k * np.pi / (n + 1))
+    assert got["m"] == n
+    assert np.abs(got["W"] - lam).max() <= w_tol(D, E)
+    # for off-diagonals +1 the eigenvector of lambda_k is (-1)^j sin(j k pi / (n+1))
+    j = np.arange(1, n + 1)[:, None]
+    V = (-1.0) ** j * np.sin(j * k[None, :] * np.pi / (n + 1))
+    V /= np.linalg.norm(V, axis=0)
+    # eigenvectors are unique up to sign for a simple spectrum
+    dots = np.abs(np.sum(V * got["Z"], axis=0))
+    assert np.all(dots > 1 - 1e3 * n * EPS)
+    check_support(got["Z"], got["Zsupp"])
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_bruteforce_small(seed, oracle):
+    _, fn = oracle
+    rng = np.random.default_rng(100 + seed)
+    n = int(rng.integers(3, 65))
+    D, E = rng.uniform(-1, 1, n), rng.uniform(-1, 1, n)
+    lam = np.linalg.eigvalsh(dense(D, E))
+    got = call_mrrr(fn, "V", "A", D, E)
+    assert got["m"] == n and np.abs(got["W"] - lam).max() <= w_tol(D, E)
+    r, o = resid_orth(D, E, got["W"], got["Z"])
+    assert r < 10 and o < 100   # units of n eps
+    # range V counts equal the dense count when the ends are away from eigenvalues
+    vl, vu = lam[n // 4] + 1e-6, lam[3 * n // 4] + 1e-6
+    if np.min(np.abs(lam - vl)) > 1e-9 and np.min(np.abs(lam - vu)) > 1e-9:
+        gv = call_mrrr(fn, "V", "V", D, E, vl=vl, vu=vu)
+        assert gv["m"] == int(np.sum((lam > vl) & (lam <= vu)))
+        cv = call_mrrr(fn, "C", "V", D, E, vl=vl, vu=vu)
+        assert cv["m"] == gv["m"]
+    # range I returns exactly the requested indices
+    il, iu = 2, n - 1
+    gi = call_mrrr(fn, "V", "I", D, E, il=il, iu=iu)
+    assert gi["m"] == iu - il + 1
+    assert np.abs(gi["W"] - lam[il - 1:iu]).max() <= w_tol(D, E)
+
+
+def test_sturm_count_inertia(oracle_lib):
+    """Sturm count on T equals the number of eigenvalues below x (Sylvester)."""
+    f = oracle_lib.oracle_sturm_count_T
+    f.restype = ctypes.c_int
+    f.argtypes = [ctypes.c_int, P, P, ctypes.c_double, ctypes.c_double]
+    rng = np.random.default_rng(3)
+    for n in (5, 40, 200):
+        D, E = rng.uniform(-1, 1, n), rng.uniform(-1, 1, n)
+        lam = np.linalg.eigvalsh(dense(D, E))
+        e2 = E ** 2
+        for x in rng.uniform(-3, 3, 25):
+            if np.min(np.abs(lam - x)) < 1e-8:
+                continue
+            assert f(n, ptr(D), ptr(e2), float(x), np.finfo(float).tiny) == int(np.sum(lam < x))
+
+
+def test_dlaneg_inertia_all_twists(oracle_lib):
+    """dlaneg gives the inertia of L D L^T - sigma I for every twist index."""
+    f = oracle_lib.oracle_dlaneg
+    f.restype = ctypes.c_int
+    f.argtypes = [ctypes.c_int, P, P, ctypes.c_double, ctypes.c_double, ctypes.c_int]
+    rng = np.random.default_rng(4)
+    n = 30
+    d = rng.uniform(0.5, 2, n) * rng.choice([-1, 1], n)
+    l = rng.uniform(-1, 1, n)
+    Lm = np.eye(n) + np.diag(l[: n - 1], -1)
+    lam = np.linalg.eigvalsh(Lm @ np.diag(d) @ Lm.T)
+    lld = d * l * l
+    for s in rng.uniform(lam[0] - 1, lam[-1] + 1, 20):
+        if np.min(np.abs(lam - s)) < 1e-7:
+            continue
+        want = int(np.sum(lam < s))
+        for r in range(1, n + 1):
+            assert f(n, ptr(d), ptr(lld), float(s), 0.0, r) == want
+
+
+def test_two_by_two_and_one(oracle):
+    _, fn = oracle
+    D, E = np.array([0.7, -0.2]), np.array([0.3, 0.0])
+    got = call_mrrr(fn, "V", "A", D, E)
+    assert got["m"] == 2
+    assert np.abs(got["W"] - np.linalg.eigvalsh(dense(D, E))).max() <= w_tol(D, E)
+    assert np.abs(got["Z"].T @ got["Z"] - np.eye(2)).max() < 4 * EPS
+    one = call_mrrr(fn, "V", "A", np.array([3.0]), np.array([0.0]))
+    assert one["m"] == 1 and one["W"][0] == 3.0 and one["Z"][0, 0] == 1.0
+
+
+def test_split_matrix_blocks(oracle):
+    _, fn = oracle
+    rng = np.random.default_rng(9)
+    n = 50
+    D, E = rng.uniform(-1, 1, n), rng.uniform(-1, 1, n)
+    E[[0, 7, 8, 20, 48]] = 0.0
+    lam = np.linalg.eigvalsh(dense(D, E))
+    got = call_mrrr(fn, "V", "A", D, E)
+    assert got["m"] == n and np.abs(got["W"] - lam).max() <= w_tol(D, E)
+    r, o = resid_orth(D, E, got["W"], got["Z"])
+    assert r < 10 and o < 100   # units of n eps
+
+
+def test_invalid_arguments(oracle):
+    _, fn = oracle
+    D, E = np.ones(4), np.ones(4)
+    assert call_mrrr(fn, "X", "A", D, E)["info"] == 1
+    assert call_mrrr(fn, "V", "Q", D, E)["info"] == 1
+    assert call_mrrr(fn, "V", "V", D, E, vl=1.0, vu=1.0)["info"] == 1
+    assert call_mrrr(fn, "V", "I", D, E, il=0, iu=2)["info"] == 1
+    assert call_mrrr(fn, "V", "I", D, E, il=3, iu=2)["info"] == 1
+    assert call_mrrr(fn, "V", "I", D, E, il=1, iu=5)["info"] == 1
+    assert call_mrrr(fn, "C", "A", D, E)["m"] == 4
+    assert call_mrrr(fn, "C", "I", D, E, il=2, iu=3)["m"] == 2
+
+
+# ---------------------------------------------------------------------------------
+# the compiled reference itself (only where oracle/_ref is built)
+# ---------------------------------------------------------------------------------
+needs_ref = pytest.mark.skipif(not os.path.exists(REF_SO), reason="oracle/_ref not built")
+
+
+@needs_ref
+@pytest.mark.parametrize("kind,n,threads", [("uniform", 150, 1), ("uniform", 700, 16),
+                                            ("toeplitz", 400, 1), ("glued", 630, 16),
+                                            ("normal", 520, 4)])
+def test_oracle_vs_compiled_reference(kind, n, threads, oracle):
+    from mr3smp_amd import workloads as WL
+    rng = np.random.default_rng(n)
+    if kind == "uniform":
+        D, E = WL.uniform(n, rng)
+    elif kind == "toeplitz":
+        D, E = WL.toeplitz_pert(n, rng)
+    elif kind == "glued":
+        D, E = WL.glued_wilkinson(n, rng, 1e-9)
+    else:
+        D, E = WL.normal(n, rng)
+    _, ref = load(REF_SO, "mrrr")
+    old = os.environ.get("PMR_NUM_THREADS")
+    os.environ["PMR_NUM_THREADS"] = str(threads)
+    try:
+        for jobz, rg, kw in (("V", "A", {}), ("N", "A", {}), ("V", "I", dict(il=n // 3, iu=n // 3 + n // 10))):
+            r = call_mrrr(ref, jobz, rg, D, E, **kw)
+            g = call_mrrr(oracle[1], jobz, rg, D, E, **kw)
+            check_pair(D, E, g, r, f"{kind}{n}-{jobz}{rg}", vectors=(jobz == "V"),
+                       floor=DSTEMR_FLOOR if (threads == 1 or n < 500) else 1.0)
+    finally:
+        if old is None:
+            os.environ.pop("PMR_NUM_THREADS", None)
+        else:
+            os.environ["PMR_NUM_THREADS"] = old
