@@ -248,6 +248,7 @@ def main():
                         "peak_GBps": HBM_PEAK_GBS},
         "kernels_ms_per_step": {k: v["ms"] / args.steps for k, v in sorted(kern.items(), key=lambda kv: -kv[1]["ms"])},
         "gpu_fp64_tflops_all_kernels": total_flops / (gpu_ms / 1e3) / 1e12 if gpu_ms else None,
+        "sweep_elements_per_step": {k: v / args.steps for k, v in steps.items()},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -236,13 +236,17 @@ int run_pipeline(std::vector<MatDesc>& md, std::vector<MatState>& ms, hipStream_
         int* ec[2] = {A.get<int>(capE), A.get<int>(capE)};
         LevelCounters* dcnt = A.get<LevelCounters>(1);
         unsigned long long* bump = A.get<unsigned long long>(1);
+        int* bcnt = A.get<int>(N);
+        const int capT = N / 64 + nbl + 1;
+        WaveTask* wtasks = A.get<WaveTask>(capT);
         if (!A.ok) return 2;
+        MR3_CHECK(hipMemsetAsync(bcnt, 0, sizeof(int) * (size_t)N, st));
         // eigenvector scratch: 3 * maxbl doubles per lane, 64 lanes per wave
         const size_t per_wave = (size_t)3 * maxbl * 64 * sizeof(double);
         const size_t budget = singleton_scratch_budget();
         int max_waves = (int)std::max<size_t>(1, std::min<size_t>(budget / per_wave,
                                                                   (size_t)env_int("MR3_SWAVES", 8192)));
-        const int want_waves = (N + 63) / 64;
+        const int want_waves = capT;
         const int nwaves = std::min(max_waves, want_waves);
         double* scratch = A.get<double>((size_t)nwaves * 3 * maxbl * 64);
         if (!A.ok) return 2;
@@ -251,6 +255,7 @@ int run_pipeline(std::vector<MatDesc>& md, std::vector<MatState>& ms, hipStream_
         MR3_CHECK(reset());
         TreeOut out;
         out.S = Sl; out.C = Cl[0]; out.eig_slot = es[0]; out.eig_cl = ec[0]; out.cnt = dcnt;
+        out.bcnt = bcnt;
         out.capS = capS; out.capC = capC; out.capEig = capE;
         LAUNCH(k_classify_root, dim3(cdiv(nbl, 64)), dim3(64), dmd, dms, blk,
                            smat, dbl, gers, D, E, DLr, DLLr, W, Werr, Wgap, Wsh, out, nbl);
@@ -264,9 +269,12 @@ int run_pipeline(std::vector<MatDesc>& md, std::vector<MatState>& ms, hipStream_
             const int nS = hc.nS, nC = hc.nC, nE = hc.nEig;
             const long long arena_need = hc.arena;
             if (nS > 0) {
-                int g = std::min(nwaves, (nS + 63) / 64);
-                LAUNCH(k_singleton, dim3(g), dim3(64), Sl, nS, dmd, dms, Windex,
-                                   Zidx, W, Wsh, Werr, Wgap, scratch, maxbl);
+                // wave tasks: one per 64 singletons of a block
+                LAUNCH(k_plan_s, dim3(cdiv(nbl, 64)), dim3(64), dbl, nbl, smat, dmd, blk, bcnt,
+                       wtasks, capT, dcnt);
+                int g = std::min(nwaves, nS / 64 + nbl + 1);
+                LAUNCH(k_singleton, dim3(g), dim3(64), wtasks, dcnt, Sl, dmd, dms, Windex, Zidx, W,
+                       Wsh, Werr, Wgap, scratch, maxbl);
                 MR3_CHECK(hipGetLastError());
             }
             if (nC == 0) break;
@@ -324,7 +332,8 @@ void profile_reset() {
 
 int profile_report(char* buf, int len) {
     static const char* kinds[kStepKinds] = {"coarse", "refine", "endref", "r1v",
-                                            "r1v_bisect", "rrf", "rrj", "refine_cl"};
+                                            "r1v_bisect", "rrf", "rrj", "refine_cl",
+                                            "r1v_wave", "bisect_wave"};
     std::string js = "{\"kernels\": {";
     std::lock_guard<std::mutex> lk(g_prof.mu);
     bool first = true;

@@ -85,6 +85,14 @@ struct ClTask {
     long long arena_off;    // offset of this cluster's slot in the level arena
     int eig_base;           // first entry of this cluster in the level eigen list
     int shift;              // 1: left end, 2: right end (dlarrf SHIFT)
+    int gblk;               // global index of the block record
+    int wseg;               // first global eigenvalue slot of the block
+};
+
+// A wave's share of a per-block list: entries [start, start + cnt), cnt <= 64, all
+// of one block (so the block order, rows and loop bounds are wave-uniform).
+struct WaveTask {
+    int start, cnt;
 };
 
 // One eigenvector to compute (process_singleton.c singleton_t with begin == end).
@@ -102,7 +110,7 @@ struct STask {
 struct LevelCounters {
     int nS, nC, nEig, maxbl;
     long long arena;        // doubles needed by the next level's cluster slots
-    int err, pad;
+    int err, nT;            // nT: wave tasks planned for the singleton kernel
 };
 
 }  // namespace mr3

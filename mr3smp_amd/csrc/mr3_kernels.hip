@@ -32,7 +32,9 @@ enum StepKind : int {
     kStepRrf = 5,        // dlarrf elements per sweep, 5 flops (odrrf.f:212-214)
     kStepRrj = 6,        // dlarrj Sturm steps, 3 flops
     kStepRefineCl = 7,   // dlaneg steps in cluster refinement, 4 flops
-    kStepKinds = 8
+    kStepR1vWave = 8,    // dlar1v elements executed per wave (lane 0 counts n per call)
+    kStepBisWave = 9,    // bisection-fallback elements of the slowest lane of a wave
+    kStepKinds = 10
 };
 __device__ unsigned long long g_steps[kStepKinds];
 __constant__ int g_count_steps;
@@ -956,10 +958,12 @@ __global__ __launch_bounds__(256) void k_zindex(const MatDesc* __restrict__ md,
 // own representation in a per-level arena.
 // ---------------------------------------------------------------------------------
 struct TreeOut {
-    STask* S;
+    STask* S;               // indexed by global eigenvalue slot space: block b's
+                            // singletons of a level fill [wseg_b, wseg_b + bcnt[b])
     ClTask* C;
     int* eig_slot;          // next level: slot of each eigen entry
     int* eig_cl;            // next level: cluster of each eigen entry
+    int* bcnt;              // singletons emitted per block in this level
     LevelCounters* cnt;
     int capS, capC, capEig;
 };
@@ -967,7 +971,7 @@ struct TreeOut {
 // Walk one cluster's eigenvalues and emit singleton and child-cluster tasks
 // (PMR_create_subtasks, process_cluster.c:438-629).  Every singleton is its own
 // task, the schedule the reference takes when max_size = 1 (:481-482).
-__device__ void classify_cluster(int cb, int ce, int depth, int mat, int off, int bbeg,
+__device__ void classify_cluster(int cb, int ce, int depth, int mat, int gblk, int wseg, int bbeg,
                                  int bsize, double bl_spdiam, double cl_lgap, const double* rD,
                                  const double* rL, const double* rDL, const double* rDLL,
                                  const double* __restrict__ Wgap, const double* __restrict__ Wsh,
@@ -996,7 +1000,8 @@ __device__ void classify_cluster(int cb, int ce, int depth, int mat, int off, in
             new_first = i + 1;
         }
     }
-    int sbase = nS ? atomicAdd(&out.cnt->nS, nS) : 0;
+    if (nS) atomicAdd(&out.cnt->nS, nS);
+    int sbase = nS ? wseg + atomicAdd(&out.bcnt[gblk], nS) : 0;
     int cbase = nC ? atomicAdd(&out.cnt->nC, nC) : 0;
     int ebase = nE ? atomicAdd(&out.cnt->nEig, nE) : 0;
     if (sbase + nS > out.capS || cbase + nC > out.capC || ebase + nE > out.capEig) {
@@ -1033,6 +1038,8 @@ __device__ void classify_cluster(int cb, int ce, int depth, int mat, int off, in
                 c.shift = 0;
                 c.arena_off = -1;
                 c.eig_base = ebase;
+                c.gblk = gblk;
+                c.wseg = wseg;
                 for (int j = f; j <= l; ++j) {
                     out.eig_slot[ebase] = j;
                     out.eig_cl[ebase] = cbase;
@@ -1055,7 +1062,6 @@ __device__ void classify_cluster(int cb, int ce, int depth, int mat, int off, in
         }
         new_first = i + 1;
     }
-    (void)off;
 }
 
 // Level 0: one thread per block slot.  1x1 blocks become singletons of size 1
@@ -1090,7 +1096,8 @@ __global__ __launch_bounds__(64) void k_classify_root(const MatDesc* __restrict_
     const double* rD = D + off + br.bbeg;
     const double* rL = E + off + br.bbeg;
     if (nbl == 1) {
-        int pos = atomicAdd(&out.cnt->nS, 1);
+        atomicAdd(&out.cnt->nS, 1);
+        int pos = off + br.wbeg + atomicAdd(&out.bcnt[gb], 1);
         atomicMax(&out.cnt->maxbl, 1);
         if (pos >= out.capS) { atomicOr(&out.cnt->err, 1); return; }
         STask t;
@@ -1110,7 +1117,7 @@ __global__ __launch_bounds__(64) void k_classify_root(const MatDesc* __restrict_
     for (int s = cb; s <= ce; ++s) { Wsh[s] = W[s]; W[s] += sigma; }
     // lgap of the root task (mrrr_vec.c:357)
     const double lgap = fmax(0.0, (Wsh[cb] + sigma) - Werr[cb] - S.vl);
-    classify_cluster(cb, ce, 0, k, off, br.bbeg, nbl, bl_spdiam, lgap, rD, rL,
+    classify_cluster(cb, ce, 0, k, gb, off + br.wbeg, br.bbeg, nbl, bl_spdiam, lgap, rD, rL,
                      DLr + off + br.bbeg, DLLr + off + br.bbeg, Wgap, Wsh, out);
 }
 
@@ -1121,8 +1128,28 @@ __global__ __launch_bounds__(64) void k_classify(const ClTask* __restrict__ C, i
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= nC) return;
     const ClTask t = C[c];
-    classify_cluster(t.cb, t.ce, t.depth, t.mat, 0, t.bbeg, t.bsize, t.bl_spdiam, t.lgap, t.cD,
-                     t.cL, t.cDL, t.cDLL, Wgap, Wsh, out);
+    classify_cluster(t.cb, t.ce, t.depth, t.mat, t.gblk, t.wseg, t.bbeg, t.bsize, t.bl_spdiam,
+                     t.lgap, t.cD, t.cL, t.cDL, t.cDLL, Wgap, Wsh, out);
+}
+
+// Cuts every block's singletons of this level into wave tasks of at most 64 and
+// clears the block counters for the next level.  One thread per block.
+__global__ __launch_bounds__(64) void k_plan_s(const int* __restrict__ blist, int nblist,
+                                               const int* __restrict__ smat,
+                                               const MatDesc* __restrict__ md,
+                                               const BlockRec* __restrict__ blk, int* bcnt,
+                                               WaveTask* tasks, int capT, LevelCounters* cnt) {
+    const int ib = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ib >= nblist) return;
+    const int gb = blist[ib];
+    const int c = bcnt[gb];
+    if (c == 0) return;
+    bcnt[gb] = 0;
+    const int wseg = md[smat[gb]].off + blk[gb].wbeg;
+    const int k = (c + 63) / 64;
+    const int base = atomicAdd(&cnt->nT, k);
+    if (base + k > capT) { atomicOr(&cnt->err, 2); return; }
+    for (int t = 0; t < k; ++t) tasks[base + t] = WaveTask{wseg + 64 * t, min(64, c - 64 * t)};
 }
 
 // Assigns each cluster of a level its arena slot (4 * bsize doubles).
@@ -1393,123 +1420,153 @@ __global__ __launch_bounds__(256) void k_gaps_cl(const ClTask* __restrict__ C,
 }
 
 // ---------------------------------------------------------------------------------
-// k_singleton: eigenvectors by twisted factorisations with Rayleigh-quotient
-// correction (PMR_process_s_task, process_singleton.c:65-324; dlar1v odr1v.f:156-369),
-// one lane per eigenvector.  Each wave owns a scratch region laid out lane-
-// interleaved (element i of lane l at [i*64 + l]) so every sweep step is one
-// coalesced 512-byte access.  Columns are written once, at the end, through an LDS
-// tile so that each store covers contiguous rows of four columns.
+// Eigenvectors: dlar1v and its vector solve for a wave of singletons of one block,
+// then the singleton kernel (process_singleton.c:65-324; dlar1v odr1v.f:156-369).
 // ---------------------------------------------------------------------------------
 constexpr int SW = 64;          // lanes per wave
 constexpr int TILE_R = 16;      // rows per LDS tile
+constexpr int kUS = 4;          // chunk of the eigenvector sweeps (register budget)
 
-struct Scr {
+__device__ __forceinline__ int wave_max_u(int v) {
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o));
+    return __builtin_amdgcn_readfirstlane(v);
+}
+__device__ __forceinline__ int wave_min_u(int v) {
+    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o));
+    return __builtin_amdgcn_readfirstlane(v);
+}
+
+// Lane-interleaved scratch of one wave with the three arrays interleaved per element:
+// element i of LPLUS, UMINUS and S/gamma of lane l sit at ((3i + a) * 64 + l).  With
+// i wave-uniform every access is one 512-byte row at a scalar base plus the lane.
+struct WScr {
     double* base;
-    int lane, cap;
-    __device__ __forceinline__ double& LP(int i) const { return base[(size_t)i * SW + lane]; }            // 1..n-1
-    __device__ __forceinline__ double& UM(int i) const { return base[((size_t)cap + i) * SW + lane]; }    // 1..n-1
-    __device__ __forceinline__ double& SG(int i) const { return base[((size_t)2 * cap + i) * SW + lane]; } // 0..n-1
+    int lane;
+    __device__ __forceinline__ double& LP(int i) const { return base[(3 * i) * SW + lane]; }
+    __device__ __forceinline__ double& UM(int i) const { return base[(3 * i + 1) * SW + lane]; }
+    __device__ __forceinline__ double& SG(int i) const { return base[(3 * i + 2) * SW + lane]; }
 };
 
-// One dlar1v call (B1 = 1, BN = n) that keeps LPLUS/UMINUS in scratch and returns the
-// scalars; the vector itself is formed later by solve_pass().  SG holds S(I) after
-// the stationary sweep and S(I)+P(I) (with the zero substitution of odr1v.f:277) after
-// the progressive one, which is what the twist-index search reads.
 struct R1v {
     int negcnt, r, isupp0, isupp1, sawnan;
     double ztz, mingma, nrminv, resid, rqcorr;
 };
 
-__device__ void dlar1v_lane(int n, double lambda, const double* __restrict__ D,
-                            const double* __restrict__ L, const double* __restrict__ LD,
-                            const double* __restrict__ LLD, double pivmin, double gaptol,
-                            int wantnc, int rin, const Scr& sc, R1v* o) {
-    int r1, r2;
-    if (rin == 0) { r1 = 1; r2 = n; } else { r1 = rin; r2 = rin; }
-    // stationary transform, odr1v.f:185-229 (S(0) = 0 since B1 = 1).  Loops 50 and 51
-    // of the reference run back to back; NaN in either sends both to the slow form.
+struct Rrr {
+    const double* D;
+    const double* L;
+    const double* LD;
+    const double* LLD;
+};
+
+__device__ void r1v_wave(int n, double lambda, const Rrr& R, double pivmin, int wantnc, int rin,
+                         bool act, const WScr& sc, R1v* o) {
+    const int r1 = rin ? rin : 1, r2 = rin ? rin : n;
+    // stationary transform (odr1v.f:185-229); loops 50 and 51 back to back
     int neg1 = 0;
     double s = 0.0 - lambda;
-    sc.SG(0) = 0.0;
-    const int stop = r2;   // steps i = 1 .. r2-1
-    for (int i0 = 1; i0 < stop; i0 += kU) {
-        double dd[kU], ld[kU], ll[kU];
+    if (act) sc.SG(0) = 0.0;
+    const int stop = act ? r2 : 1;            // this lane's steps: i = 1 .. stop-1
+    const int stopU = wave_max_u(stop);
+    if (stopU > 1) {
+        double dd[kUS], ld[kUS], ll[kUS], dn[kUS], ldn[kUS], lln[kUS];
 #pragma unroll
-        for (int u = 0; u < kU; ++u) {
-            int j = min(i0 + u, stop - 1) - 1;
-            dd[u] = D[j]; ld[u] = LD[j]; ll[u] = L[j];
+        for (int u = 0; u < kUS; ++u) {
+            int j = min(1 + u, stopU - 1) - 1;
+            dd[u] = R.D[j]; ld[u] = R.LD[j]; ll[u] = R.L[j];
         }
+        for (int i0 = 1; i0 < stopU; i0 += kUS) {
 #pragma unroll
-        for (int u = 0; u < kU; ++u) {
-            const int i = i0 + u;
-            if (i < stop) {
-                double dplus = dd[u] + s;
-                double lp = ld[u] / dplus;
-                sc.LP(i) = lp;
-                neg1 += (i <= r1 - 1) && (dplus < 0.0);
-                double si = s * lp * ll[u];
-                sc.SG(i) = si;
-                s = si - lambda;
+            for (int u = 0; u < kUS; ++u) {
+                int j = min(i0 + kUS + u, stopU - 1) - 1;
+                dn[u] = R.D[j]; ldn[u] = R.LD[j]; lln[u] = R.L[j];
             }
+#pragma unroll
+            for (int u = 0; u < kUS; ++u) {
+                const int i = i0 + u;
+                if (i < stop) {
+                    double dplus = dd[u] + s;
+                    double lp = ld[u] / dplus;
+                    sc.LP(i) = lp;
+                    neg1 += (i <= r1 - 1) && (dplus < 0.0);
+                    double si = s * lp * ll[u];
+                    sc.SG(i) = si;
+                    s = si - lambda;
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < kUS; ++u) { dd[u] = dn[u]; ld[u] = ldn[u]; ll[u] = lln[u]; }
         }
     }
-    const int sawnan1 = isnan(s);
+    const int sawnan1 = act && isnan(s);
     if (sawnan1) {
         neg1 = 0;
         s = 0.0 - lambda;
         for (int i = 1; i <= r2 - 1; ++i) {
-            double dplus = D[i - 1] + s;
+            double dplus = R.D[i - 1] + s;
             if (fabs(dplus) < pivmin) dplus = -pivmin;
-            double lp = LD[i - 1] / dplus;
+            double lp = R.LD[i - 1] / dplus;
             sc.LP(i) = lp;
             if (i <= r1 - 1) neg1 += (dplus < 0.0);
-            double si = s * lp * L[i - 1];
-            if (lp == 0.0) si = LLD[i - 1];
+            double si = s * lp * R.L[i - 1];
+            if (lp == 0.0) si = R.LLD[i - 1];
             sc.SG(i) = si;
             s = si - lambda;
         }
     }
-    // progressive transform, odr1v.f:234-260.  gamma(I) = S(I) + P(I) is formed as
-    // P(I) appears (with the zero substitution of odr1v.f:277) and stored over S(I)
-    // for the twist search; gamma(R1-1) is kept raw for the count.
+    // progressive transform (odr1v.f:234-260)
     int neg2 = 0;
-    const double pn = D[n - 1] - lambda;   // P(BN-1)
+    const double pn = R.D[n - 1] - lambda;   // P(BN-1)
     double gam_r1m1 = 0.0;
     double pi = pn;
-    if (r1 - 1 == n - 1) gam_r1m1 = sc.SG(n - 1) + pi;
-    else if (rin == 0) {
-        double gm = sc.SG(n - 1) + pi;
-        if (gm == 0.0) gm = kEps * sc.SG(n - 1);
-        sc.SG(n - 1) = gm;
-    }
-    for (int i0 = n - 1; i0 >= r1; i0 -= kU) {
-        double dd[kU], ll[kU], lld[kU];
-#pragma unroll
-        for (int u = 0; u < kU; ++u) {
-            int j = max(i0 - u, r1) - 1;
-            dd[u] = D[j]; ll[u] = L[j]; lld[u] = LLD[j];
+    if (act) {
+        if (r1 - 1 == n - 1) gam_r1m1 = sc.SG(n - 1) + pi;
+        else if (rin == 0) {
+            double gm = sc.SG(n - 1) + pi;
+            if (gm == 0.0) gm = kEps * sc.SG(n - 1);
+            sc.SG(n - 1) = gm;
         }
+    }
+    const int lo = act ? r1 : n;               // this lane's steps: i = n-1 down to lo
+    const int loU = wave_min_u(lo);
+    if (n - 1 >= loU) {
+        double dd[kUS], ll[kUS], lld[kUS], sv[kUS], dn[kUS], lln[kUS], lldn[kUS], svn[kUS];
 #pragma unroll
-        for (int u = 0; u < kU; ++u) {
-            const int i = i0 - u;
-            if (i >= r1) {
-                double dminus = lld[u] + pi;
-                double tmp = dd[u] / dminus;
-                neg2 += (dminus < 0.0);
-                sc.UM(i) = ll[u] * tmp;
-                double pim1 = pi * tmp - lambda;   // P(I-1)
-                if (i - 1 == r1 - 1) gam_r1m1 = sc.SG(i - 1) + pim1;
-                else if (rin == 0) {
-                    double sv = sc.SG(i - 1);
-                    double gm = sv + pim1;
-                    if (gm == 0.0) gm = kEps * sv;
-                    sc.SG(i - 1) = gm;
+        for (int u = 0; u < kUS; ++u) {
+            int i = max(n - 1 - u, loU);
+            dd[u] = R.D[i - 1]; ll[u] = R.L[i - 1]; lld[u] = R.LLD[i - 1]; sv[u] = sc.SG(i - 1);
+        }
+        for (int i0 = n - 1; i0 >= loU; i0 -= kUS) {
+#pragma unroll
+            for (int u = 0; u < kUS; ++u) {
+                int i = max(i0 - kUS - u, loU);
+                dn[u] = R.D[i - 1]; lln[u] = R.L[i - 1]; lldn[u] = R.LLD[i - 1]; svn[u] = sc.SG(i - 1);
+            }
+#pragma unroll
+            for (int u = 0; u < kUS; ++u) {
+                const int i = i0 - u;
+                if (i >= lo) {
+                    double dminus = lld[u] + pi;
+                    double tmp = dd[u] / dminus;
+                    neg2 += (dminus < 0.0);
+                    sc.UM(i) = ll[u] * tmp;
+                    double pim1 = pi * tmp - lambda;   // P(I-1)
+                    if (i - 1 == r1 - 1) gam_r1m1 = sv[u] + pim1;
+                    else if (rin == 0) {
+                        double gm = sv[u] + pim1;
+                        if (gm == 0.0) gm = kEps * sv[u];
+                        sc.SG(i - 1) = gm;
+                    }
+                    pi = pim1;
                 }
-                pi = pim1;
+            }
+#pragma unroll
+            for (int u = 0; u < kUS; ++u) {
+                dd[u] = dn[u]; ll[u] = lln[u]; lld[u] = lldn[u]; sv[u] = svn[u];
             }
         }
     }
-    const int sawnan2 = isnan(pi);
+    const int sawnan2 = act && isnan(pi);
     if (sawnan2) {
         // slow progressive form (odr1v.f:240-258); S is formed again because the fast
         // sweep overwrote it with gamma values
@@ -1517,11 +1574,11 @@ __device__ void dlar1v_lane(int n, double lambda, const double* __restrict__ D,
         double ss = 0.0 - lambda;
         sc.SG(0) = 0.0;
         for (int i = 1; i <= r2 - 1; ++i) {
-            double dplus = D[i - 1] + ss;
+            double dplus = R.D[i - 1] + ss;
             if (sawnan1 && fabs(dplus) < pivmin) dplus = -pivmin;
-            double lp = LD[i - 1] / dplus;
-            double si = ss * lp * L[i - 1];
-            if (sawnan1 && lp == 0.0) si = LLD[i - 1];
+            double lp = R.LD[i - 1] / dplus;
+            double si = ss * lp * R.L[i - 1];
+            if (sawnan1 && lp == 0.0) si = R.LLD[i - 1];
             sc.SG(i) = si;
             ss = si - lambda;
         }
@@ -1533,13 +1590,13 @@ __device__ void dlar1v_lane(int n, double lambda, const double* __restrict__ D,
             sc.SG(n - 1) = gm;
         }
         for (int i = n - 1; i >= r1; --i) {
-            double dminus = LLD[i - 1] + pi;
+            double dminus = R.LLD[i - 1] + pi;
             if (fabs(dminus) < pivmin) dminus = -pivmin;
-            double tmp = D[i - 1] / dminus;
+            double tmp = R.D[i - 1] / dminus;
             neg2 += (dminus < 0.0);
-            sc.UM(i) = L[i - 1] * tmp;
+            sc.UM(i) = R.L[i - 1] * tmp;
             double pim1 = pi * tmp - lambda;
-            if (tmp == 0.0) pim1 = D[i - 1] - lambda;
+            if (tmp == 0.0) pim1 = R.D[i - 1] - lambda;
             if (i - 1 == r1 - 1) gam_r1m1 = sc.SG(i - 1) + pim1;
             else if (rin == 0) {
                 double gm = sc.SG(i - 1) + pim1;
@@ -1549,71 +1606,104 @@ __device__ void dlar1v_lane(int n, double lambda, const double* __restrict__ D,
             pi = pim1;
         }
     }
-    // twist index (odr1v.f:265-283)
+    // twist index (odr1v.f:265-283): lanes with a free twist scan I = 1 .. n-1
     double mingma = gam_r1m1;
-    if (mingma < 0.0) ++neg1;
-    o->negcnt = wantnc ? neg1 + neg2 : -1;
-    if (fabs(mingma) == 0.0) mingma = kEps * sc.SG(r1 - 1);
+    if (act && mingma < 0.0) ++neg1;
+    if (act && fabs(mingma) == 0.0) mingma = kEps * sc.SG(r1 - 1);
     int r = r1;
-    for (int i = r1; i <= r2 - 1; ++i) {
-        double tmp = sc.SG(i);
-        if (fabs(tmp) <= fabs(mingma)) { mingma = tmp; r = i + 1; }
+    const bool scan = act && rin == 0;
+    if (__any(scan) && n > 1) {
+        double g[kUS], gn[kUS];
+#pragma unroll
+        for (int u = 0; u < kUS; ++u) g[u] = sc.SG(min(1 + u, n - 1));
+        for (int i0 = 1; i0 <= n - 1; i0 += kUS) {
+#pragma unroll
+            for (int u = 0; u < kUS; ++u) gn[u] = sc.SG(min(i0 + kUS + u, n - 1));
+#pragma unroll
+            for (int u = 0; u < kUS; ++u) {
+                if (scan && i0 + u <= n - 1 && fabs(g[u]) <= fabs(mingma)) {
+                    mingma = g[u];
+                    r = i0 + u + 1;
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < kUS; ++u) g[u] = gn[u];
+        }
     }
-    o->r = r;
-    o->mingma = mingma;
-    o->sawnan = sawnan1 || sawnan2;
+    if (act) {
+        o->negcnt = wantnc ? neg1 + neg2 : -1;
+        o->r = r;
+        o->mingma = mingma;
+        o->sawnan = sawnan1 || sawnan2;
+    }
 }
 
-// The vector solve of odr1v.f:287-355.  With stage == false it only forms ZTZ and
-// the support; otherwise it also writes z * scale for rows 1..n of the block into
-// the lane's SG region (zero outside the support).
-__device__ void solve_pass(int n, const double* __restrict__ LD, double gaptol, const Scr& sc,
-                           R1v* o, bool stage, double scale) {
-    const int r = o->r;
+// The vector solve of odr1v.f:287-355 for every lane with act set.  With stage ==
+// false it forms ZTZ and the support; with stage == true it writes z * scale for the
+// block rows 1..n into SG (zero outside the support).  Index ranges are wave-uniform
+// as in r1v_wave.
+__device__ void solve_wave(int n, const double* __restrict__ LD, double gaptol, const WScr& sc,
+                           R1v* o, bool act, bool stage, double scale) {
+    const int r = act ? o->r : 1;
+    const bool nan = act && o->sawnan;
     int is0 = 1, is1 = n;
     double ztz = 1.0;
-    if (stage) sc.SG(r - 1) = scale;   // Z(r) = 1
-    // upwards from r
+    if (act && stage) sc.SG(r - 1) = scale;   // Z(r) = 1
+    // upwards from r: this lane covers I = r-1 down to 1
     {
         double zi1 = 1.0, zi2 = 0.0;   // Z(I+1), Z(I+2)
-        int lo = 0;                    // rows below lo+1 are zero
-        bool cut = false;
-        if (!o->sawnan) {
-            for (int i0 = r - 1; i0 >= 1 && !cut; i0 -= kU) {
-                double lp[kU], ld[kU];
+        bool live = act && !nan && r - 1 >= 1;
+        const int topU = wave_max_u((act && !nan) ? r - 1 : 0);
+        if (topU >= 1) {
+            double lp[kUS], ld[kUS], lpn[kUS], ldn[kUS];
 #pragma unroll
-                for (int u = 0; u < kU; ++u) {
-                    int i = max(i0 - u, 1);
-                    lp[u] = sc.LP(i);
-                    ld[u] = LD[i - 1];
+            for (int u = 0; u < kUS; ++u) {
+                int i = max(topU - u, 1);
+                lp[u] = sc.LP(i); ld[u] = LD[i - 1];
+            }
+            for (int i0 = topU; i0 >= 1; i0 -= kUS) {
+                if (!stage && !__any(live)) break;
+#pragma unroll
+                for (int u = 0; u < kUS; ++u) {
+                    int i = max(i0 - kUS - u, 1);
+                    lpn[u] = sc.LP(i); ldn[u] = LD[i - 1];
                 }
 #pragma unroll
-                for (int u = 0; u < kU; ++u) {
+                for (int u = 0; u < kUS; ++u) {
                     const int i = i0 - u;
-                    if (i >= 1 && !cut) {
-                        double zi = -(lp[u] * zi1);
-                        if ((fabs(zi) + fabs(zi1)) * fabs(ld[u]) < gaptol) {
-                            is0 = i + 1;
-                            lo = i;
-                            cut = true;
-                        } else {
-                            ztz = ztz + zi * zi;
-                            if (stage) sc.SG(i - 1) = zi * scale;
-                            zi1 = zi;
+                    if (act && !nan && i >= 1 && i <= r - 1) {
+                        if (live) {
+                            double zi = -(lp[u] * zi1);
+                            if ((fabs(zi) + fabs(zi1)) * fabs(ld[u]) < gaptol) {
+                                is0 = i + 1;
+                                live = false;
+                                if (stage) sc.SG(i - 1) = 0.0;
+                            } else {
+                                ztz = ztz + zi * zi;
+                                if (stage) sc.SG(i - 1) = zi * scale;
+                                zi1 = zi;
+                            }
+                        } else if (stage) {
+                            sc.SG(i - 1) = 0.0;
                         }
                     }
                 }
+#pragma unroll
+                for (int u = 0; u < kUS; ++u) { lp[u] = lpn[u]; ld[u] = ldn[u]; }
             }
-        } else {
+        }
+        if (nan) {   // slow form after a NaN (odr1v.f:308-322)
+            bool cut = false;
             for (int i = r - 1; i >= 1; --i) {
+                if (cut) { if (stage) sc.SG(i - 1) = 0.0; continue; }
                 double zi;
                 if (zi1 == 0.0) zi = -(LD[i] / LD[i - 1]) * zi2;
                 else zi = -(sc.LP(i) * zi1);
                 if ((fabs(zi) + fabs(zi1)) * fabs(LD[i - 1]) < gaptol) {
                     is0 = i + 1;
-                    lo = i;
                     cut = true;
-                    break;
+                    if (stage) sc.SG(i - 1) = 0.0;
+                    continue;
                 }
                 ztz = ztz + zi * zi;
                 if (stage) sc.SG(i - 1) = zi * scale;
@@ -1621,49 +1711,62 @@ __device__ void solve_pass(int n, const double* __restrict__ LD, double gaptol, 
                 zi1 = zi;
             }
         }
-        if (stage && cut) for (int i = lo; i >= 1; --i) sc.SG(i - 1) = 0.0;
     }
-    // downwards from r
+    // downwards from r: this lane covers I = r .. n-1
     {
         double zi = 1.0, zim1 = 0.0;   // Z(I), Z(I-1)
-        int hi = n;                    // rows from hi+1 on are zero
-        bool cut = false;
-        if (!o->sawnan) {
-            for (int i0 = r; i0 <= n - 1 && !cut; i0 += kU) {
-                double um[kU], ld[kU];
+        bool live = act && !nan && r <= n - 1;
+        const int loU = wave_min_u((act && !nan) ? r : n);
+        if (loU <= n - 1) {
+            double um[kUS], ld[kUS], umn[kUS], ldn[kUS];
 #pragma unroll
-                for (int u = 0; u < kU; ++u) {
-                    int i = min(i0 + u, n - 1);
-                    um[u] = sc.UM(i);
-                    ld[u] = LD[i - 1];
+            for (int u = 0; u < kUS; ++u) {
+                int i = min(loU + u, n - 1);
+                um[u] = sc.UM(i); ld[u] = LD[i - 1];
+            }
+            for (int i0 = loU; i0 <= n - 1; i0 += kUS) {
+                if (!stage && !__any(live)) break;
+#pragma unroll
+                for (int u = 0; u < kUS; ++u) {
+                    int i = min(i0 + kUS + u, n - 1);
+                    umn[u] = sc.UM(i); ldn[u] = LD[i - 1];
                 }
 #pragma unroll
-                for (int u = 0; u < kU; ++u) {
+                for (int u = 0; u < kUS; ++u) {
                     const int i = i0 + u;
-                    if (i <= n - 1 && !cut) {
-                        double zn = -(um[u] * zi);
-                        if ((fabs(zi) + fabs(zn)) * fabs(ld[u]) < gaptol) {
-                            is1 = i;
-                            hi = i;
-                            cut = true;
-                        } else {
-                            ztz = ztz + zn * zn;
-                            if (stage) sc.SG(i) = zn * scale;
-                            zi = zn;
+                    if (act && !nan && i >= r && i <= n - 1) {
+                        if (live) {
+                            double zn = -(um[u] * zi);
+                            if ((fabs(zi) + fabs(zn)) * fabs(ld[u]) < gaptol) {
+                                is1 = i;
+                                live = false;
+                                if (stage) sc.SG(i) = 0.0;
+                            } else {
+                                ztz = ztz + zn * zn;
+                                if (stage) sc.SG(i) = zn * scale;
+                                zi = zn;
+                            }
+                        } else if (stage) {
+                            sc.SG(i) = 0.0;
                         }
                     }
                 }
+#pragma unroll
+                for (int u = 0; u < kUS; ++u) { um[u] = umn[u]; ld[u] = ldn[u]; }
             }
-        } else {
+        }
+        if (nan) {   // slow form after a NaN (odr1v.f:336-350)
+            bool cut = false;
             for (int i = r; i <= n - 1; ++i) {
+                if (cut) { if (stage) sc.SG(i) = 0.0; continue; }
                 double zn;
                 if (zi == 0.0) zn = -(LD[i - 2] / LD[i - 1]) * zim1;
                 else zn = -(sc.UM(i) * zi);
                 if ((fabs(zi) + fabs(zn)) * fabs(LD[i - 1]) < gaptol) {
                     is1 = i;
-                    hi = i;
                     cut = true;
-                    break;
+                    if (stage) sc.SG(i) = 0.0;
+                    continue;
                 }
                 ztz = ztz + zn * zn;
                 if (stage) sc.SG(i) = zn * scale;
@@ -1671,9 +1774,8 @@ __device__ void solve_pass(int n, const double* __restrict__ LD, double gaptol, 
                 zi = zn;
             }
         }
-        if (stage && cut) for (int i = hi; i <= n - 1; ++i) sc.SG(i) = 0.0;
     }
-    if (!stage) {
+    if (act && !stage) {
         double tmp = 1.0 / ztz;
         o->ztz = ztz;
         o->nrminv = sqrt(tmp);
@@ -1684,7 +1786,117 @@ __device__ void solve_pass(int n, const double* __restrict__ LD, double gaptol, 
     }
 }
 
-__global__ __launch_bounds__(64) void k_singleton(const STask* __restrict__ T, int nS,
+// k_singleton: eigenvectors by twisted factorisations with Rayleigh-quotient
+// correction (PMR_process_s_task, process_singleton.c:65-324), one lane per
+// eigenvector.  Every wave takes a task of up to 64 singletons of one block, so the
+// block size, rows and sweep indices are wave-uniform; lanes that share a
+// representation (all of them at the root of a block) hit the same cache lines.  Each
+// wave owns a
+// scratch region of 3 * cap * 64 doubles.  Columns are written once, at the end,
+// through an LDS tile so that each store covers contiguous rows of four columns.
+__device__ void singleton_group(const STask& t, bool valid, int n, int bbeg,
+                                const MatDesc* __restrict__ md, const MatState* __restrict__ ms,
+                                const int* __restrict__ Windex, double* W, double* Wsh,
+                                double* Werr, const double* __restrict__ Wgap, const WScr& sc,
+                                int* supp0, int* supp1) {
+    Rrr R;
+    R.D = t.D; R.L = t.L; R.LD = t.DL; R.LLD = t.DLL;
+    const MatState& S = ms[t.mat];
+    const int off = md[t.mat].off;
+    const int i = t.slot - off;               // matrix-local eigenvalue index
+    const int bl_begin = bbeg, bl_end = bbeg + n - 1;
+    const double pivmin = S.pivmin;
+    const double tol = 4.0 * log((double)n) * DBL_EPSILON;
+    const double RQtol = 2.0 * DBL_EPSILON;
+    double lambda = valid ? Wsh[t.slot] : 0.0;
+    double werr = valid ? Werr[t.slot] : 0.0;
+    double left = lambda - werr, right = lambda + werr;
+    const int i_local = valid ? Windex[t.slot] : 1;
+    double lgap, rgap;
+    if (i == bl_begin) lgap = DBL_EPSILON * fmax(fabs(left), fabs(right));
+    else lgap = t.lgap;
+    if (i == bl_end) rgap = DBL_EPSILON * fmax(fabs(left), fabs(right));
+    else rgap = valid ? Wgap[t.slot] : 0.0;
+    const double gap = fmin(lgap, rgap);
+    const double gaptol = (i == bl_begin || i == bl_end) ? 0.0 : gap * DBL_EPSILON;
+    int usedBS = 0, usedRQ = 0, needBS = 0, wantnc = 1, r = 0;
+    bool done = !valid;
+    double bstres = 0.0, bstw = 0.0, wsh = lambda;
+    R1v o;
+    o.r = 1; o.sawnan = 0; o.resid = 0.0; o.rqcorr = 0.0; o.nrminv = 1.0; o.mingma = 0.0;
+    o.isupp0 = 1; o.isupp1 = n; o.negcnt = 0; o.ztz = 1.0;
+    unsigned long long r1v_el = 0, bis_el = 0, wave_calls = 0;
+    for (int kk = 1; kk <= kMaxIterRQI; ++kk) {
+        const bool act = !done;
+        if (!__any(act)) break;
+        ++wave_calls;
+        if (act && needBS) {
+            usedBS = 1;
+            double w = wsh, we = werr;
+            bis_el += (unsigned long long)n *
+                      dlarrb1(n, t.D, t.DLL, i_local, 0.0, 2.0 * DBL_EPSILON, 0.0, &w, &we, pivmin,
+                              t.bl_spdiam, r);
+            wsh = w;
+            werr = we;
+            lambda = wsh;
+            r = 0;
+        }
+        if (act) wantnc = usedBS ? 0 : 1;
+        r1v_wave(n, lambda, R, pivmin, wantnc, r, act, sc, &o);
+        solve_wave(n, R.LD, gaptol, sc, &o, act, false, 0.0);
+        if (act) {
+            r1v_el += n;
+            r = o.r;
+            if (kk == 1 || o.resid < bstres) { bstres = o.resid; bstw = lambda; }
+            if (!usedBS && o.resid > tol * gap && fabs(o.rqcorr) > RQtol * fabs(lambda)) {
+                int sgndef = (i_local <= o.negcnt) ? -1 : 1;
+                if (o.rqcorr * sgndef >= 0.0 && lambda + o.rqcorr <= right &&
+                    lambda + o.rqcorr >= left) {
+                    usedRQ = 1;
+                    if (sgndef == 1) left = lambda;
+                    else right = lambda;
+                    wsh = 0.5 * (left + right);
+                    lambda += o.rqcorr;
+                } else needBS = 1;
+                if (right - left < RQtol * fabs(lambda)) usedBS = 1;
+                else if (kk == kMaxIterRQI - 1) needBS = 1;
+            } else done = true;
+        }
+    }
+    const bool step2 = valid && usedRQ && usedBS && bstres <= o.resid;
+    if (__any(step2)) {
+        ++wave_calls;
+        if (step2) lambda = bstw;
+        r1v_wave(n, lambda, R, pivmin, wantnc, r, step2, sc, &o);
+        solve_wave(n, R.LD, gaptol, sc, &o, step2, false, 0.0);
+        if (step2) r1v_el += n;
+    }
+    solve_wave(n, R.LD, gaptol, sc, &o, valid, true, o.nrminv);
+    add_steps(kStepR1v, r1v_el);
+    add_steps(kStepR1vBis, bis_el);
+    {
+        unsigned long long bmax = bis_el;
+        for (int o2 = 32; o2 > 0; o2 >>= 1) {
+            unsigned long long ot = __shfl_xor(bmax, o2);
+            bmax = ot > bmax ? ot : bmax;
+        }
+        if (threadIdx.x == 0) {
+            add_steps(kStepR1vWave, wave_calls * (unsigned long long)n);
+            add_steps(kStepBisWave, bmax);
+        }
+    }
+    *supp0 = o.isupp0;
+    *supp1 = o.isupp1;
+    if (valid) {
+        Wsh[t.slot] = lambda;
+        Werr[t.slot] = werr;
+        W[t.slot] = lambda + t.L[n - 1];
+    }
+}
+
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_singleton(const WaveTask* __restrict__ tasks,
+                                                  const LevelCounters* __restrict__ cnt,
+                                                  const STask* __restrict__ T,
                                                   const MatDesc* __restrict__ md,
                                                   const MatState* __restrict__ ms,
                                                   const int* __restrict__ Windex,
@@ -1694,142 +1906,59 @@ __global__ __launch_bounds__(64) void k_singleton(const STask* __restrict__ T, i
                                                   int cap) {
     __shared__ double tile[TILE_R][SW + 1];
     const int lane = threadIdx.x;
-    const int nwaves = gridDim.x;
-    const int ngroups = (nS + SW - 1) / SW;
-    Scr sc;
+    const int nT = __builtin_amdgcn_readfirstlane(cnt->nT);
+    WScr sc;
     sc.base = scratch + (size_t)blockIdx.x * 3 * (size_t)cap * SW;
     sc.lane = lane;
-    sc.cap = cap;
-    for (int grp = blockIdx.x; grp < ngroups; grp += nwaves) {
-        const int ti = grp * SW + lane;
-        const bool active = ti < nS;
-        STask t;
-        if (active) t = T[ti];
-        else { t.bsize = 0; t.mat = 0; t.slot = 0; t.bbeg = 0; }
-        const int n = t.bsize;
-        int col = 0, ldz = 0;
-        double* Zc = nullptr;
-        int* Zs = nullptr;
-        int nfull = 0;
-        if (active) {
-            const MatDesc& dsc = md[t.mat];
-            col = Zidx[t.slot];
-            if (col >= dsc.mcols) col = dsc.mcols - 1;   // never write past the caller's Z
-            ldz = dsc.ldz;
-            nfull = dsc.n;
-            Zc = dsc.Z + (size_t)col * ldz;
-            Zs = dsc.Zsupp + 2 * col;
-        }
-        if (active && n == 1) {
+    for (int ti = blockIdx.x; ti < nT; ti += gridDim.x) {
+        const WaveTask wt = tasks[ti];
+        const int cntw = __builtin_amdgcn_readfirstlane(wt.cnt);
+        const int start = __builtin_amdgcn_readfirstlane(wt.start);
+        const bool valid = lane < cntw;
+        // lanes past the end replay the last task so that every pointer is valid
+        const STask t = T[start + min(lane, cntw - 1)];
+        const int n = __builtin_amdgcn_readfirstlane(t.bsize);
+        const int bbeg = __builtin_amdgcn_readfirstlane(t.bbeg);
+        const MatDesc& dsc = md[t.mat];
+        int col = Zidx[t.slot];
+        if (col >= dsc.mcols) col = dsc.mcols - 1;   // never write past the caller's Z
+        double* Zc = dsc.Z + (size_t)col * dsc.ldz;
+        const int nfull = __builtin_amdgcn_readfirstlane(dsc.n);
+        int s0 = 1, s1 = 1;
+        if (n == 1) {
             sc.SG(0) = 1.0;
-            Zs[0] = t.bbeg + 1;
-            Zs[1] = t.bbeg + 1;
-        } else if (active) {
-            const MatState& S = ms[t.mat];
-            const int off = md[t.mat].off;
-            const int i = t.slot - off;               // matrix-local eigenvalue index
-            const int bl_begin = t.bbeg, bl_end = t.bbeg + n - 1;
-            const double pivmin = S.pivmin;
-            const double tol = 4.0 * log((double)n) * DBL_EPSILON;
-            const double RQtol = 2.0 * DBL_EPSILON;
-            double lambda = Wsh[t.slot];
-            double werr = Werr[t.slot];
-            double left = lambda - werr, right = lambda + werr;
-            const int i_local = Windex[t.slot];
-            double lgap, rgap;
-            if (i == bl_begin) lgap = DBL_EPSILON * fmax(fabs(left), fabs(right));
-            else lgap = t.lgap;
-            if (i == bl_end) rgap = DBL_EPSILON * fmax(fabs(left), fabs(right));
-            else rgap = Wgap[t.slot];
-            const double gap = fmin(lgap, rgap);
-            const double gaptol = (i == bl_begin || i == bl_end) ? 0.0 : gap * DBL_EPSILON;
-            int usedBS = 0, usedRQ = 0, needBS = 0, wantnc = 1, r = 0;
-            double bstres = 0.0, bstw = 0.0, wsh = lambda;
-            R1v o;
-            unsigned long long r1v_el = 0, bis_el = 0;
-            for (int kk = 1; kk <= kMaxIterRQI; ++kk) {
-                if (needBS) {
-                    usedBS = 1;
-                    double w = wsh, we = werr;
-                    bis_el += (unsigned long long)n *
-                              dlarrb1(n, t.D, t.DLL, i_local, 0.0, 2.0 * DBL_EPSILON, 0.0, &w, &we,
-                                      pivmin, t.bl_spdiam, r);
-                    wsh = w;
-                    werr = we;
-                    lambda = wsh;
-                    r = 0;
-                }
-                wantnc = usedBS ? 0 : 1;
-                dlar1v_lane(n, lambda, t.D, t.L, t.DL, t.DLL, pivmin, gaptol, wantnc, r, sc, &o);
-                solve_pass(n, t.DL, gaptol, sc, &o, false, 0.0);
-                r1v_el += n;
-                r = o.r;
-                if (kk == 1 || o.resid < bstres) { bstres = o.resid; bstw = lambda; }
-                if (!usedBS && o.resid > tol * gap && fabs(o.rqcorr) > RQtol * fabs(lambda)) {
-                    int sgndef = (i_local <= o.negcnt) ? -1 : 1;
-                    if (o.rqcorr * sgndef >= 0.0 && lambda + o.rqcorr <= right &&
-                        lambda + o.rqcorr >= left) {
-                        usedRQ = 1;
-                        if (sgndef == 1) left = lambda;
-                        else right = lambda;
-                        wsh = 0.5 * (left + right);
-                        lambda += o.rqcorr;
-                    } else needBS = 1;
-                    if (right - left < RQtol * fabs(lambda)) usedBS = 1;
-                    else if (kk == kMaxIterRQI - 1) needBS = 1;
-                } else break;
-            }
-            if (usedRQ && usedBS && bstres <= o.resid) {
-                lambda = bstw;
-                dlar1v_lane(n, lambda, t.D, t.L, t.DL, t.DLL, pivmin, gaptol, wantnc, r, sc, &o);
-                solve_pass(n, t.DL, gaptol, sc, &o, false, 0.0);
-                r1v_el += n;
-            }
-            add_steps(kStepR1v, r1v_el);
-            add_steps(kStepR1vBis, bis_el);
-            solve_pass(n, t.DL, gaptol, sc, &o, true, o.nrminv);
-            Zs[0] = o.isupp0 + bl_begin;
-            Zs[1] = o.isupp1 + bl_begin;
-            Wsh[t.slot] = lambda;
-            Werr[t.slot] = werr;
-            W[t.slot] = lambda + t.L[n - 1];
+        } else {
+            singleton_group(t, valid, n, bbeg, md, ms, Windex, W, Wsh, Werr, Wgap, sc, &s0, &s1);
         }
-        // ---- cooperative column write: zero rows outside the block, then LDS-tiled
-        // block rows.  Every lane's column info is broadcast with shuffles.
-        int maxn = active ? n : 0;
-        for (int o2 = 32; o2 > 0; o2 >>= 1) maxn = max(maxn, __shfl_xor(maxn, o2));
-        // zero fill outside the block rows, one column at a time
-        for (int c = 0; c < SW; ++c) {
-            const int cact = __shfl((int)active, c);
-            if (!cact) continue;
+        if (valid) {
+            dsc.Zsupp[2 * col] = s0 + bbeg;
+            dsc.Zsupp[2 * col + 1] = s1 + bbeg;
+        }
+        // ---- column write: zero rows outside the block, then the block rows through
+        // an LDS tile (16 rows x 4 columns per store)
+        for (int c = 0; c < cntw; ++c) {
             double* zc = (double*)__shfl((long long)Zc, c);
-            const int cb0 = __shfl(t.bbeg, c);
-            const int cn = __shfl(n, c);
-            const int cfull = __shfl(nfull, c);
-            for (int rr = lane; rr < cb0; rr += SW) zc[rr] = 0.0;
-            for (int rr = cb0 + cn + lane; rr < cfull; rr += SW) zc[rr] = 0.0;
+            for (int rr = lane; rr < bbeg; rr += SW) zc[rr] = 0.0;
+            for (int rr = bbeg + n + lane; rr < nfull; rr += SW) zc[rr] = 0.0;
         }
-        const int sub = lane & 15, cq = lane >> 4;   // 16 rows x 4 columns per store
-        for (int r0 = 0; r0 < maxn; r0 += TILE_R) {
+        const int sub = lane & 15, cq = lane >> 4;
+        for (int r0 = 0; r0 < n; r0 += TILE_R) {
+#pragma unroll
             for (int rr = 0; rr < TILE_R; ++rr) {
-                const int row = r0 + rr;
-                tile[rr][lane] = (active && row < n) ? sc.SG(row) : 0.0;
+                const int row = min(r0 + rr, n - 1);
+                tile[rr][lane] = sc.SG(row);
             }
             __syncthreads();
             for (int cg = 0; cg < SW / 4; ++cg) {
                 const int c = cg * 4 + cq;
-                const int cact = __shfl((int)active, c);
                 double* zc = (double*)__shfl((long long)Zc, c);
-                const int cb0 = __shfl(t.bbeg, c);
-                const int cn = __shfl(n, c);
                 const int row = r0 + sub;
-                if (cact && row < cn) zc[cb0 + row] = tile[sub][c];
+                if (c < cntw && row < n) zc[bbeg + row] = tile[sub][c];
             }
             __syncthreads();
         }
     }
 }
-
 // ---------------------------------------------------------------------------------
 // k_finalize: unscale (mrrr.c:371-386), place eigenvalues in column order
 // (sort_eigenpairs, mrrr.c:525-588) and report the outputs.  One workgroup per

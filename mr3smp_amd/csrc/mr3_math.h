@@ -16,9 +16,10 @@ constexpr double kEps = DBL_EPSILON;   // odmch.f 'P'
 constexpr double kSafmin = DBL_MIN;    // odmch.f 'S' / 'U'
 constexpr int kMaxExpand = 128;        // cap on bracket doublings (2^128 spans any range)
 
-// Loop chunk: every sweep loads kU consecutive elements before it runs the kU
-// dependent steps, so the loads of a chunk are in flight together instead of each
-// step waiting on its own.
+// Loop chunks: every sweep runs in chunks of kU steps and issues the loads of the
+// next chunk before it computes the current one (software pipelining), so a chunk's
+// memory latency is covered by the previous chunk's arithmetic instead of being paid
+// by every dependent step.
 constexpr int kU = 8;
 
 // Sturm count of T - x I, dlaebz IJOB=1 convention (odebz.f:265-277): a pivot with
@@ -29,13 +30,18 @@ __device__ __forceinline__ int count_T_ijob1(int n, const double* __restrict__ d
     double t = d[0] - x;
     if (fabs(t) < pivmin) t = -pivmin;
     int c = (t <= 0.0);
+    if (n < 2) return c;
+    double dd[kU], ee[kU], dn[kU], en[kU];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+        int j = max(min(1 + u, n - 1), 1);
+        dd[u] = d[j]; ee[u] = e2[j - 1];
+    }
     for (int j0 = 1; j0 < n; j0 += kU) {
-        double dd[kU], ee[kU];
 #pragma unroll
         for (int u = 0; u < kU; ++u) {
-            int j = min(j0 + u, n - 1);
-            dd[u] = d[j];
-            ee[u] = e2[j - 1];
+            int j = max(min(j0 + kU + u, n - 1), 1);
+            dn[u] = d[j]; en[u] = e2[j - 1];
         }
 #pragma unroll
         for (int u = 0; u < kU; ++u) {
@@ -45,6 +51,8 @@ __device__ __forceinline__ int count_T_ijob1(int n, const double* __restrict__ d
                 c += (t <= 0.0);
             }
         }
+#pragma unroll
+        for (int u = 0; u < kU; ++u) { dd[u] = dn[u]; ee[u] = en[u]; }
     }
     return c;
 }
@@ -57,13 +65,18 @@ __device__ __forceinline__ int count_T_ijob2(int n, const double* __restrict__ d
     double t = d[0] - x;
     int c = 0;
     if (t <= pivmin) { c = 1; t = fmin(t, -pivmin); }
+    if (n < 2) return c;
+    double dd[kU], ee[kU], dn[kU], en[kU];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+        int j = max(min(1 + u, n - 1), 1);
+        dd[u] = d[j]; ee[u] = e2[j - 1];
+    }
     for (int j0 = 1; j0 < n; j0 += kU) {
-        double dd[kU], ee[kU];
 #pragma unroll
         for (int u = 0; u < kU; ++u) {
-            int j = min(j0 + u, n - 1);
-            dd[u] = d[j];
-            ee[u] = e2[j - 1];
+            int j = max(min(j0 + kU + u, n - 1), 1);
+            dn[u] = d[j]; en[u] = e2[j - 1];
         }
 #pragma unroll
         for (int u = 0; u < kU; ++u) {
@@ -72,6 +85,8 @@ __device__ __forceinline__ int count_T_ijob2(int n, const double* __restrict__ d
                 if (t <= pivmin) { ++c; t = fmin(t, -pivmin); }
             }
         }
+#pragma unroll
+        for (int u = 0; u < kU; ++u) { dd[u] = dn[u]; ee[u] = en[u]; }
     }
     return c;
 }
@@ -86,43 +101,58 @@ __device__ __forceinline__ int dlaneg(int n, const double* __restrict__ d,
     int neg = 0;
     double t = -sigma;
     const int top = r - 1;   // steps j = 0 .. r-2
-    for (int j0 = 0; j0 < top; j0 += kU) {
-        double dd[kU], ll[kU];
+    double dd[kU], ll[kU], dn[kU], ln[kU];
+    if (top > 0) {
 #pragma unroll
         for (int u = 0; u < kU; ++u) {
-            int j = min(j0 + u, top - 1);
-            dd[u] = d[j];
-            ll[u] = lld[j];
+            int j = min(u, top - 1);
+            dd[u] = d[j]; ll[u] = lld[j];
         }
+        for (int j0 = 0; j0 < top; j0 += kU) {
 #pragma unroll
-        for (int u = 0; u < kU; ++u) {
-            if (j0 + u < top) {
-                double dplus = dd[u] + t;
-                neg += (dplus < 0.0);
-                double tmp = t / dplus;
-                tmp = isnan(tmp) ? 1.0 : tmp;
-                t = tmp * ll[u] - sigma;
+            for (int u = 0; u < kU; ++u) {
+                int j = min(j0 + kU + u, top - 1);
+                dn[u] = d[j]; ln[u] = lld[j];
             }
+#pragma unroll
+            for (int u = 0; u < kU; ++u) {
+                if (j0 + u < top) {
+                    double dplus = dd[u] + t;
+                    neg += (dplus < 0.0);
+                    double tmp = t / dplus;
+                    tmp = isnan(tmp) ? 1.0 : tmp;
+                    t = tmp * ll[u] - sigma;
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < kU; ++u) { dd[u] = dn[u]; ll[u] = ln[u]; }
         }
     }
     double p = d[n - 1] - sigma;
-    for (int j0 = n - 2; j0 >= r - 1; j0 -= kU) {
-        double dd[kU], ll[kU];
+    if (n - 2 >= r - 1) {
 #pragma unroll
         for (int u = 0; u < kU; ++u) {
-            int j = max(j0 - u, r - 1);
-            dd[u] = d[j];
-            ll[u] = lld[j];
+            int j = max(n - 2 - u, r - 1);
+            dd[u] = d[j]; ll[u] = lld[j];
         }
+        for (int j0 = n - 2; j0 >= r - 1; j0 -= kU) {
 #pragma unroll
-        for (int u = 0; u < kU; ++u) {
-            if (j0 - u >= r - 1) {
-                double dminus = ll[u] + p;
-                neg += (dminus < 0.0);
-                double tmp = p / dminus;
-                tmp = isnan(tmp) ? 1.0 : tmp;
-                p = tmp * dd[u] - sigma;
+            for (int u = 0; u < kU; ++u) {
+                int j = max(j0 - kU - u, r - 1);
+                dn[u] = d[j]; ln[u] = lld[j];
             }
+#pragma unroll
+            for (int u = 0; u < kU; ++u) {
+                if (j0 - u >= r - 1) {
+                    double dminus = ll[u] + p;
+                    neg += (dminus < 0.0);
+                    double tmp = p / dminus;
+                    tmp = isnan(tmp) ? 1.0 : tmp;
+                    p = tmp * dd[u] - sigma;
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < kU; ++u) { dd[u] = dn[u]; ll[u] = ln[u]; }
         }
     }
     double gamma = (t + sigma) + p;
@@ -183,13 +213,18 @@ __device__ __forceinline__ int count_T_rrj(int n, const double* __restrict__ d,
                                            const double* __restrict__ e2, double s) {
     double dplus = d[0] - s;
     int cnt = (dplus < 0.0);
+    if (n < 2) return cnt;
+    double dd[kU], ee[kU], dn[kU], en[kU];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+        int j = max(min(1 + u, n - 1), 1);
+        dd[u] = d[j]; ee[u] = e2[j - 1];
+    }
     for (int j0 = 1; j0 < n; j0 += kU) {
-        double dd[kU], ee[kU];
 #pragma unroll
         for (int u = 0; u < kU; ++u) {
-            int j = min(j0 + u, n - 1);
-            dd[u] = d[j];
-            ee[u] = e2[j - 1];
+            int j = max(min(j0 + kU + u, n - 1), 1);
+            dn[u] = d[j]; en[u] = e2[j - 1];
         }
 #pragma unroll
         for (int u = 0; u < kU; ++u) {
@@ -198,6 +233,8 @@ __device__ __forceinline__ int count_T_rrj(int n, const double* __restrict__ d,
                 cnt += (dplus < 0.0);
             }
         }
+#pragma unroll
+        for (int u = 0; u < kU; ++u) { dd[u] = dn[u]; ee[u] = en[u]; }
     }
     return cnt;
 }
