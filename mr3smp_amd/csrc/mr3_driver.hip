@@ -139,6 +139,8 @@ static size_t singleton_scratch_budget() {
     return std::min(b, cap);
 }
 
+constexpr int kMaxGen = 4;   // longest singleton list of a wave, in multiples of 64
+
 static int env_int(const char* name, int dflt) {
     const char* v = getenv(name);
     return v ? atoi(v) : dflt;
@@ -241,14 +243,16 @@ int run_pipeline(std::vector<MatDesc>& md, std::vector<MatState>& ms, hipStream_
         WaveTask* wtasks = A.get<WaveTask>(capT);
         if (!A.ok) return 2;
         MR3_CHECK(hipMemsetAsync(bcnt, 0, sizeof(int) * (size_t)N, st));
-        // eigenvector scratch: 3 * maxbl doubles per lane, 64 lanes per wave
-        const size_t per_wave = (size_t)3 * maxbl * 64 * sizeof(double);
+        // eigenvector scratch: two checkpoint arrays of maxbl/kChunk + 2 carries per
+        // lane, 64 lanes per wave
+        const size_t ck_per_lane = (size_t)2 * (maxbl / kChunk + 2);
+        const size_t per_wave = ck_per_lane * 64 * sizeof(double);
         const size_t budget = singleton_scratch_budget();
         int max_waves = (int)std::max<size_t>(1, std::min<size_t>(budget / per_wave,
                                                                   (size_t)env_int("MR3_SWAVES", 8192)));
         const int want_waves = capT;
         const int nwaves = std::min(max_waves, want_waves);
-        double* scratch = A.get<double>((size_t)nwaves * 3 * maxbl * 64);
+        double* scratch = A.get<double>((size_t)nwaves * ck_per_lane * 64);
         if (!A.ok) return 2;
         LevelCounters hc;
         auto reset = [&]() { return hipMemsetAsync(dcnt, 0, sizeof(LevelCounters), st); };
@@ -264,18 +268,81 @@ int run_pipeline(std::vector<MatDesc>& md, std::vector<MatState>& ms, hipStream_
         MR3_CHECK(hipStreamSynchronize(st));
         if (hc.err) return 2;
         int cur = 0;
-        std::vector<double*> arenas;
+        const bool trace = getenv("MR3_TRACE_LEVELS") != nullptr;   // per-level timing to stderr
+        unsigned long long prev_ticks = 0, prev_tasks = 0;
+        unsigned long long* tlog = nullptr;
+        struct LogReset {   // the device pointer never outlives this solve's buffer
+            bool armed = false;
+            ~LogReset() {
+                if (!armed) return;
+                unsigned long long* z = nullptr;
+                (void)hipDeviceSynchronize();
+                (void)hipMemcpyToSymbol(HIP_SYMBOL(g_tasklog), &z, sizeof(z));
+            }
+        } log_reset;
+        if (trace && getenv("MR3_TASKLOG")) {
+            log_reset.armed = true;
+            tlog = A.get<unsigned long long>((size_t)4 * capT);
+            if (!A.ok) return 2;
+            MR3_CHECK(hipMemsetAsync(tlog, 0, (size_t)32 * capT, st));
+            MR3_CHECK(hipMemcpyToSymbolAsync(HIP_SYMBOL(g_tasklog), &tlog, sizeof(tlog), 0,
+                                             hipMemcpyHostToDevice, st));
+            MR3_CHECK(hipMemcpyToSymbolAsync(HIP_SYMBOL(g_tasklog_cap), &capT, sizeof(int), 0,
+                                             hipMemcpyHostToDevice, st));
+        }
         for (int level = 0;; ++level) {
             const int nS = hc.nS, nC = hc.nC, nE = hc.nEig;
             const long long arena_need = hc.arena;
             if (nS > 0) {
-                // wave tasks: one per 64 singletons of a block
+                // wave tasks: lists of up to tsize singletons of a block, longest blocks
+                // first; a wave refills its 64 lanes from its list, so lists longer than 64
+                // keep lanes busy past a slow one, as long as there are waves enough to
+                // fill the device
+                const int gen = std::max(1, std::min(kMaxGen, nS / (64 * 2 * 2048)));
+                const int tsize = 64 * gen;
+                LAUNCH(k_plan_count, dim3(cdiv(nbl, 64)), dim3(64), dbl, nbl, blk, bcnt, tsize, dcnt);
                 LAUNCH(k_plan_s, dim3(cdiv(nbl, 64)), dim3(64), dbl, nbl, smat, dmd, blk, bcnt,
-                       wtasks, capT, dcnt);
-                int g = std::min(nwaves, nS / 64 + nbl + 1);
+                       tsize, wtasks, capT, dcnt);
+                int g = std::min(nwaves, nS / tsize + nbl + 1);
+                hipEvent_t t0 = nullptr, t1 = nullptr;
+                if (trace) {
+                    (void)hipEventCreate(&t0);
+                    (void)hipEventCreate(&t1);
+                    (void)hipEventRecord(t0, st);
+                }
                 LAUNCH(k_singleton, dim3(g), dim3(64), wtasks, dcnt, Sl, dmd, dms, Windex, Zidx, W,
                        Wsh, Werr, Wgap, scratch, maxbl);
                 MR3_CHECK(hipGetLastError());
+                if (trace) {
+                    float ms = 0.f;
+                    (void)hipEventRecord(t1, st);
+                    (void)hipEventSynchronize(t1);
+                    (void)hipEventElapsedTime(&ms, t0, t1);
+                    unsigned long long stv[kStepKinds] = {};
+                    (void)hipMemcpyFromSymbol(stv, HIP_SYMBOL(g_steps), sizeof(stv));
+                    fprintf(stderr,
+                            "mr3 level %d: singletons %d clusters %d waves %d  %.3f ms  "
+                            "tasks %llu  task time %.3f wave-ms\n",
+                            level, nS, nC, g, ms, stv[kStepTasks] - prev_tasks,
+                            (stv[kStepTaskTicks] - prev_ticks) * 1e-5);
+                    prev_ticks = stv[kStepTaskTicks];
+                    prev_tasks = stv[kStepTasks];
+                    (void)hipEventDestroy(t0);
+                    (void)hipEventDestroy(t1);
+                    if (tlog) {   // per-task log of this level to MR3_TASKLOG.<level>
+                        std::vector<unsigned long long> h((size_t)4 * capT);
+                        (void)hipMemcpy(h.data(), tlog, h.size() * 8, hipMemcpyDeviceToHost);
+                        char fn[512];
+                        snprintf(fn, sizeof(fn), "%s.%d", getenv("MR3_TASKLOG"), level);
+                        FILE* f = fopen(fn, "wb");
+                        if (f) {
+                            fwrite(&capT, sizeof(int), 1, f);
+                            fwrite(h.data(), 8, h.size(), f);
+                            fclose(f);
+                        }
+                        (void)hipMemset(tlog, 0, h.size() * 8);
+                    }
+                }
             }
             if (nC == 0) break;
             if (level > 64) return 2;
@@ -333,7 +400,8 @@ void profile_reset() {
 int profile_report(char* buf, int len) {
     static const char* kinds[kStepKinds] = {"coarse", "refine", "endref", "r1v",
                                             "r1v_bisect", "rrf", "rrj", "refine_cl",
-                                            "r1v_wave", "bisect_wave"};
+                                            "r1v_wave", "bisect_wave", "task_ticks",
+                                            "tasks", "t_bisect", "t_r1v", "t_solve"};
     std::string js = "{\"kernels\": {";
     std::lock_guard<std::mutex> lk(g_prof.mu);
     bool first = true;

@@ -20,6 +20,8 @@ constexpr int kMaxTryRoot = 6;
 constexpr double kFudgeRoot = 2.0;
 constexpr double kRandFactor = 8.0;
 constexpr int kMaxIterRQI = 10;
+// steps per chunk of the eigenvector sweeps: checkpoint spacing and LDS buffer rows
+constexpr int kChunk = 16;
 constexpr double kMaxGrowth1 = 8.0;
 constexpr double kMaxGrowth2 = 8.0;
 
@@ -99,6 +101,7 @@ struct WaveTask {
 struct STask {
     int slot, mat;
     int bbeg, bsize;
+    int gblk, wseg;         // global block index, first eigen slot of the block
     double bl_spdiam, lgap;
     const double* D;
     const double* L;
@@ -106,11 +109,17 @@ struct STask {
     const double* DLL;
 };
 
+
 // Counters written by the classify kernels and read back by the host once per level.
+constexpr int kSizeClasses = 32;   // singleton tasks are ordered by floor(log2(block size))
+
 struct LevelCounters {
     int nS, nC, nEig, maxbl;
     long long arena;        // doubles needed by the next level's cluster slots
     int err, nT;            // nT: wave tasks planned for the singleton kernel
+    int next;               // next wave task to hand out
+    int cls[kSizeClasses];  // wave tasks per size class
+    int clsPos[kSizeClasses];
 };
 
 }  // namespace mr3

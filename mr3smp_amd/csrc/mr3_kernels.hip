@@ -34,14 +34,23 @@ enum StepKind : int {
     kStepRefineCl = 7,   // dlaneg steps in cluster refinement, 4 flops
     kStepR1vWave = 8,    // dlar1v elements executed per wave (lane 0 counts n per call)
     kStepBisWave = 9,    // bisection-fallback elements of the slowest lane of a wave
-    kStepKinds = 10
+    kStepTaskTicks = 10, // wave time of the singleton tasks, 100 MHz wall-clock ticks
+    kStepTasks = 11,     // singleton tasks run
+    kStepTBis = 12,      // wave time (ticks) in the bisection fallback
+    kStepTR1v = 13,      // wave time in the twisted factorisations (dlar1v to the twist)
+    kStepTSolve = 14,    // wave time in the vector solves and column writes
+    kStepKinds = 15
 };
 __device__ unsigned long long g_steps[kStepKinds];
 __constant__ int g_count_steps;
+// Optional per-task log of the eigenvector kernel (diagnostics): 4 words per task.
+__device__ unsigned long long* g_tasklog;
+__device__ int g_tasklog_cap;
 
 __device__ __forceinline__ void add_steps(int kind, unsigned long long v) {
     if (g_count_steps && v) atomicAdd(&g_steps[kind], v);
 }
+
 
 // ---------------------------------------------------------------------------------
 // Workgroup helpers (256 threads)
@@ -1022,6 +1031,7 @@ __device__ void classify_cluster(int cb, int ce, int depth, int mat, int gblk, i
         if (new_size == 1) {
             STask t;
             t.slot = new_first; t.mat = mat; t.bbeg = bbeg; t.bsize = bsize;
+            t.gblk = gblk; t.wseg = wseg;
             t.bl_spdiam = bl_spdiam;
             t.lgap = (new_first == cb) ? cl_lgap : Wgap[new_first - 1];
             t.D = rD; t.L = rL; t.DL = rDL; t.DLL = rDLL;
@@ -1102,6 +1112,7 @@ __global__ __launch_bounds__(64) void k_classify_root(const MatDesc* __restrict_
         if (pos >= out.capS) { atomicOr(&out.cnt->err, 1); return; }
         STask t;
         t.slot = cb; t.mat = k; t.bbeg = br.bbeg; t.bsize = 1; t.bl_spdiam = 0.0; t.lgap = 0.0;
+        t.gblk = gb; t.wseg = off + br.wbeg;
         t.D = rD; t.L = rL; t.DL = DLr + off + br.bbeg; t.DLL = DLLr + off + br.bbeg;
         out.S[pos] = t;
         Wsh[cb] = W[cb];
@@ -1134,11 +1145,30 @@ __global__ __launch_bounds__(64) void k_classify(const ClTask* __restrict__ C, i
 
 // Cuts every block's singletons of this level into wave tasks of at most 64 and
 // clears the block counters for the next level.  One thread per block.
+// Wave tasks of a level: every block with singletons gets ceil(count/tsize) tasks, listed
+// by decreasing block size (size classes of floor(log2 n)) so that the longest tasks
+// start first.  k_plan_count sizes the classes, k_plan_s writes the list.
+__device__ __forceinline__ int size_class(int n) { return 31 - __clz(max(n, 1)); }
+
+__global__ __launch_bounds__(64) void k_plan_count(const int* __restrict__ blist, int nblist,
+                                                   const BlockRec* __restrict__ blk,
+                                                   const int* __restrict__ bcnt, int tsize,
+                                                   LevelCounters* cnt) {
+    const int ib = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ib >= nblist) return;
+    const int gb = blist[ib];
+    const int c = bcnt[gb];
+    if (c == 0) return;
+    const int n = blk[gb].bend - blk[gb].bbeg + 1;
+    atomicAdd(&cnt->cls[size_class(n)], (c + tsize - 1) / tsize);
+}
+
 __global__ __launch_bounds__(64) void k_plan_s(const int* __restrict__ blist, int nblist,
                                                const int* __restrict__ smat,
                                                const MatDesc* __restrict__ md,
                                                const BlockRec* __restrict__ blk, int* bcnt,
-                                               WaveTask* tasks, int capT, LevelCounters* cnt) {
+                                               int tsize, WaveTask* tasks, int capT,
+                                               LevelCounters* cnt) {
     const int ib = blockIdx.x * blockDim.x + threadIdx.x;
     if (ib >= nblist) return;
     const int gb = blist[ib];
@@ -1146,10 +1176,15 @@ __global__ __launch_bounds__(64) void k_plan_s(const int* __restrict__ blist, in
     if (c == 0) return;
     bcnt[gb] = 0;
     const int wseg = md[smat[gb]].off + blk[gb].wbeg;
-    const int k = (c + 63) / 64;
-    const int base = atomicAdd(&cnt->nT, k);
+    const int n = blk[gb].bend - blk[gb].bbeg + 1;
+    const int q = size_class(n);
+    int base = 0;
+    for (int k2 = kSizeClasses - 1; k2 > q; --k2) base += cnt->cls[k2];
+    const int k = (c + tsize - 1) / tsize;
+    base += atomicAdd(&cnt->clsPos[q], k);
+    atomicAdd(&cnt->nT, k);
     if (base + k > capT) { atomicOr(&cnt->err, 2); return; }
-    for (int t = 0; t < k; ++t) tasks[base + t] = WaveTask{wseg + 64 * t, min(64, c - 64 * t)};
+    for (int t = 0; t < k; ++t) tasks[base + t] = WaveTask{wseg + tsize * t, min(tsize, c - tsize * t)};
 }
 
 // Assigns each cluster of a level its arena slot (4 * bsize doubles).
@@ -1424,8 +1459,8 @@ __global__ __launch_bounds__(256) void k_gaps_cl(const ClTask* __restrict__ C,
 // then the singleton kernel (process_singleton.c:65-324; dlar1v odr1v.f:156-369).
 // ---------------------------------------------------------------------------------
 constexpr int SW = 64;          // lanes per wave
-constexpr int TILE_R = 16;      // rows per LDS tile
-constexpr int kUS = 4;          // chunk of the eigenvector sweeps (register budget)
+constexpr int KC = kChunk;       // steps per chunk: LDS buffer rows and checkpoint spacing
+constexpr int kUS = 4;          // prefetch group of the full stationary sweep
 
 __device__ __forceinline__ int wave_max_u(int v) {
     for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o));
@@ -1436,19 +1471,38 @@ __device__ __forceinline__ int wave_min_u(int v) {
     return __builtin_amdgcn_readfirstlane(v);
 }
 
-// Lane-interleaved scratch of one wave with the three arrays interleaved per element:
-// element i of LPLUS, UMINUS and S/gamma of lane l sit at ((3i + a) * 64 + l).  With
-// i wave-uniform every access is one 512-byte row at a scalar base plus the lane.
-struct WScr {
+// Per-lane checkpoints of one wave in device memory, lane-interleaved so that every
+// access is one 512-byte row: S(c) is the stationary carry s entering step c*KC,
+// P(c) the progressive carry p entering the top step of chunk c (steps c*KC+1 ..
+// c*KC+KC).  Only these carries live in device memory; the L+, U-, S and z values
+// of a chunk are recomputed from them into the LDS buffer with the same operations,
+// so every value equals the one a full-length work array would hold.
+struct Ck {
     double* base;
+    int nck;
     int lane;
-    __device__ __forceinline__ double& LP(int i) const { return base[(3 * i) * SW + lane]; }
-    __device__ __forceinline__ double& UM(int i) const { return base[(3 * i + 1) * SW + lane]; }
-    __device__ __forceinline__ double& SG(int i) const { return base[(3 * i + 2) * SW + lane]; }
+    __device__ __forceinline__ double& S(int c) const { return base[(size_t)c * SW + lane]; }
+    __device__ __forceinline__ double& P(int c) const {
+        return base[(size_t)(nck + c) * SW + lane];
+    }
+};
+
+// One column per lane, padded so that the transposed reads of the column writes are
+// free of bank conflicts.
+typedef double ChunkBuf[KC][SW + 1];
+
+// Destination of the eigenvector columns of a wave (LDS tables filled by the solve).
+struct ColOut {
+    double** zcol;   // column pointer of each lane
+    int* wr0;        // per column: first row of the chunk in the LDS buffer
+    int* wlo;        // per column: rows to write, [wlo, whi]
+    int* whi;
+    double* Zc;      // this lane's column
+    int bbeg, nfull, cntw;
 };
 
 struct R1v {
-    int negcnt, r, isupp0, isupp1, sawnan;
+    int negcnt, r, isupp0, isupp1, nan1, nan2;
     double ztz, mingma, nrminv, resid, rqcorr;
 };
 
@@ -1459,324 +1513,506 @@ struct Rrr {
     const double* LLD;
 };
 
-__device__ void r1v_wave(int n, double lambda, const Rrr& R, double pivmin, int wantnc, int rin,
-                         bool act, const WScr& sc, R1v* o) {
-    const int r1 = rin ? rin : 1, r2 = rin ? rin : n;
-    // stationary transform (odr1v.f:185-229); loops 50 and 51 back to back
-    int neg1 = 0;
-    double s = 0.0 - lambda;
-    if (act) sc.SG(0) = 0.0;
-    const int stop = act ? r2 : 1;            // this lane's steps: i = 1 .. stop-1
+// One step t (1-based) of the stationary transform, odr1v.f:187-191; with safe set
+// the form of odr1v.f:206-213 (pivot floor, S = LLD where L+ underflows).
+__device__ __forceinline__ void stat_step(double d, double ld, double l, const double* LLD, int t,
+                                          double lambda, bool safe, double pivmin, double& s,
+                                          double& lp, double& si, bool& neg) {
+    double dplus = d + s;
+    if (safe && fabs(dplus) < pivmin) dplus = -pivmin;
+    lp = ld / dplus;
+    neg = dplus < 0.0;
+    si = s * lp * l;
+    if (safe && lp == 0.0) si = LLD[t - 1];
+    s = si - lambda;
+}
+
+// One step t of the progressive transform, odr1v.f:236-241; with safe set the form
+// of odr1v.f:247-255.
+__device__ __forceinline__ void prog_step(double d, double lld, double l, double lambda, bool safe,
+                                          double pivmin, double& p, double& um, bool& neg) {
+    double dminus = lld + p;
+    if (safe && fabs(dminus) < pivmin) dminus = -pivmin;
+    const double tmp = d / dminus;
+    neg = dminus < 0.0;
+    um = l * tmp;
+    double pn = p * tmp - lambda;
+    if (safe && tmp == 0.0) pn = d - lambda;
+    p = pn;
+}
+
+// Stationary transform over steps t = 1 .. stop-1 (odr1v.f:184-229), keeping the
+// carry every KC steps.  Lanes with stop <= 1 do nothing.  Outputs the final carry,
+// the negative pivots among steps t <= r1-1 and the last S value (0 if no step).
+__device__ void pass_stat(const Rrr& R, double lambda, double pivmin, int r1, int stop, bool safe,
+                          const Ck& ck, double& s_out, int& neg_out, double& slast_out) {
+    double s = 0.0 - lambda, slast = 0.0;
+    int neg = 0;
     const int stopU = wave_max_u(stop);
     if (stopU > 1) {
-        double dd[kUS], ld[kUS], ll[kUS], dn[kUS], ldn[kUS], lln[kUS];
-#pragma unroll
-        for (int u = 0; u < kUS; ++u) {
-            int j = min(1 + u, stopU - 1) - 1;
-            dd[u] = R.D[j]; ld[u] = R.LD[j]; ll[u] = R.L[j];
-        }
-        for (int i0 = 1; i0 < stopU; i0 += kUS) {
+        // two register buffers in turn: the loads of one group are in flight while the
+        // other group is computed, and no register copy waits for them
+        double A[kUS][3], B[kUS][3];
+        auto load = [&](double(&X)[kUS][3], int t0) {
 #pragma unroll
             for (int u = 0; u < kUS; ++u) {
-                int j = min(i0 + kUS + u, stopU - 1) - 1;
-                dn[u] = R.D[j]; ldn[u] = R.LD[j]; lln[u] = R.L[j];
+                const int j = min(max(t0 + u, 1), stopU - 1) - 1;
+                X[u][0] = R.D[j]; X[u][1] = R.LD[j]; X[u][2] = R.L[j];
             }
+        };
+        auto run = [&](const double(&X)[kUS][3], int t0) {
+            if (t0 > 0 && (t0 % KC) == 0 && t0 < stop) ck.S(t0 / KC) = s;
 #pragma unroll
             for (int u = 0; u < kUS; ++u) {
-                const int i = i0 + u;
-                if (i < stop) {
-                    double dplus = dd[u] + s;
-                    double lp = ld[u] / dplus;
-                    sc.LP(i) = lp;
-                    neg1 += (i <= r1 - 1) && (dplus < 0.0);
-                    double si = s * lp * ll[u];
-                    sc.SG(i) = si;
-                    s = si - lambda;
+                const int t = t0 + u;
+                if (t >= 1 && t < stop) {
+                    double lp, si;
+                    bool ng;
+                    stat_step(X[u][0], X[u][1], X[u][2], R.LLD, t, lambda, safe, pivmin, s, lp, si, ng);
+                    neg += (t <= r1 - 1) && ng;
+                    slast = si;
                 }
             }
-#pragma unroll
-            for (int u = 0; u < kUS; ++u) { dd[u] = dn[u]; ld[u] = ldn[u]; ll[u] = lln[u]; }
+        };
+        load(A, 0);
+        for (int t0 = 0; t0 < stopU; t0 += 2 * kUS) {
+            load(B, t0 + kUS);
+            run(A, t0);
+            if (t0 + kUS >= stopU) break;
+            load(A, t0 + 2 * kUS);
+            run(B, t0 + kUS);
         }
     }
-    const int sawnan1 = act && isnan(s);
-    if (sawnan1) {
-        neg1 = 0;
-        s = 0.0 - lambda;
-        for (int i = 1; i <= r2 - 1; ++i) {
-            double dplus = R.D[i - 1] + s;
-            if (fabs(dplus) < pivmin) dplus = -pivmin;
-            double lp = R.LD[i - 1] / dplus;
-            sc.LP(i) = lp;
-            if (i <= r1 - 1) neg1 += (dplus < 0.0);
-            double si = s * lp * R.L[i - 1];
-            if (lp == 0.0) si = R.LLD[i - 1];
-            sc.SG(i) = si;
-            s = si - lambda;
-        }
-    }
-    // progressive transform (odr1v.f:234-260)
-    int neg2 = 0;
-    const double pn = R.D[n - 1] - lambda;   // P(BN-1)
-    double gam_r1m1 = 0.0;
-    double pi = pn;
-    if (act) {
-        if (r1 - 1 == n - 1) gam_r1m1 = sc.SG(n - 1) + pi;
-        else if (rin == 0) {
-            double gm = sc.SG(n - 1) + pi;
-            if (gm == 0.0) gm = kEps * sc.SG(n - 1);
-            sc.SG(n - 1) = gm;
-        }
-    }
-    const int lo = act ? r1 : n;               // this lane's steps: i = n-1 down to lo
-    const int loU = wave_min_u(lo);
-    if (n - 1 >= loU) {
-        double dd[kUS], ll[kUS], lld[kUS], sv[kUS], dn[kUS], lln[kUS], lldn[kUS], svn[kUS];
+    s_out = s;
+    neg_out = neg;
+    slast_out = slast;
+}
+
+// Visits steps t = lo .. hi (wave-uniform bounds, ascending) and calls f(t, v) with
+// v[a] = P[a][t - 1].  Two register buffers take turns: the loads of one group of kUS
+// steps are in flight while the other group is computed.
+template <int NA, typename F>
+__device__ __forceinline__ void sweep_up(int lo, int hi, const double* const (&P)[NA], F f) {
+    if (lo > hi) return;
+    double A[kUS][NA], B[kUS][NA];
+    auto load = [&](double(&X)[kUS][NA], int t0) {
 #pragma unroll
-        for (int u = 0; u < kUS; ++u) {
-            int i = max(n - 1 - u, loU);
-            dd[u] = R.D[i - 1]; ll[u] = R.L[i - 1]; lld[u] = R.LLD[i - 1]; sv[u] = sc.SG(i - 1);
-        }
-        for (int i0 = n - 1; i0 >= loU; i0 -= kUS) {
+        for (int u = 0; u < kUS; ++u)
 #pragma unroll
-            for (int u = 0; u < kUS; ++u) {
-                int i = max(i0 - kUS - u, loU);
-                dn[u] = R.D[i - 1]; lln[u] = R.L[i - 1]; lldn[u] = R.LLD[i - 1]; svn[u] = sc.SG(i - 1);
-            }
+            for (int a = 0; a < NA; ++a) X[u][a] = P[a][min(t0 + u, hi) - 1];
+    };
+    auto run = [&](const double(&X)[kUS][NA], int t0) {
 #pragma unroll
-            for (int u = 0; u < kUS; ++u) {
-                const int i = i0 - u;
-                if (i >= lo) {
-                    double dminus = lld[u] + pi;
-                    double tmp = dd[u] / dminus;
-                    neg2 += (dminus < 0.0);
-                    sc.UM(i) = ll[u] * tmp;
-                    double pim1 = pi * tmp - lambda;   // P(I-1)
-                    if (i - 1 == r1 - 1) gam_r1m1 = sv[u] + pim1;
-                    else if (rin == 0) {
-                        double gm = sv[u] + pim1;
-                        if (gm == 0.0) gm = kEps * sv[u];
-                        sc.SG(i - 1) = gm;
-                    }
-                    pi = pim1;
-                }
-            }
-#pragma unroll
-            for (int u = 0; u < kUS; ++u) {
-                dd[u] = dn[u]; ll[u] = lln[u]; lld[u] = lldn[u]; sv[u] = svn[u];
-            }
-        }
-    }
-    const int sawnan2 = act && isnan(pi);
-    if (sawnan2) {
-        // slow progressive form (odr1v.f:240-258); S is formed again because the fast
-        // sweep overwrote it with gamma values
-        neg2 = 0;
-        double ss = 0.0 - lambda;
-        sc.SG(0) = 0.0;
-        for (int i = 1; i <= r2 - 1; ++i) {
-            double dplus = R.D[i - 1] + ss;
-            if (sawnan1 && fabs(dplus) < pivmin) dplus = -pivmin;
-            double lp = R.LD[i - 1] / dplus;
-            double si = ss * lp * R.L[i - 1];
-            if (sawnan1 && lp == 0.0) si = R.LLD[i - 1];
-            sc.SG(i) = si;
-            ss = si - lambda;
-        }
-        pi = pn;
-        if (r1 - 1 == n - 1) gam_r1m1 = sc.SG(n - 1) + pi;
-        else if (rin == 0) {
-            double gm = sc.SG(n - 1) + pi;
-            if (gm == 0.0) gm = kEps * sc.SG(n - 1);
-            sc.SG(n - 1) = gm;
-        }
-        for (int i = n - 1; i >= r1; --i) {
-            double dminus = R.LLD[i - 1] + pi;
-            if (fabs(dminus) < pivmin) dminus = -pivmin;
-            double tmp = R.D[i - 1] / dminus;
-            neg2 += (dminus < 0.0);
-            sc.UM(i) = R.L[i - 1] * tmp;
-            double pim1 = pi * tmp - lambda;
-            if (tmp == 0.0) pim1 = R.D[i - 1] - lambda;
-            if (i - 1 == r1 - 1) gam_r1m1 = sc.SG(i - 1) + pim1;
-            else if (rin == 0) {
-                double gm = sc.SG(i - 1) + pim1;
-                if (gm == 0.0) gm = kEps * sc.SG(i - 1);
-                sc.SG(i - 1) = gm;
-            }
-            pi = pim1;
-        }
-    }
-    // twist index (odr1v.f:265-283): lanes with a free twist scan I = 1 .. n-1
-    double mingma = gam_r1m1;
-    if (act && mingma < 0.0) ++neg1;
-    if (act && fabs(mingma) == 0.0) mingma = kEps * sc.SG(r1 - 1);
-    int r = r1;
-    const bool scan = act && rin == 0;
-    if (__any(scan) && n > 1) {
-        double g[kUS], gn[kUS];
-#pragma unroll
-        for (int u = 0; u < kUS; ++u) g[u] = sc.SG(min(1 + u, n - 1));
-        for (int i0 = 1; i0 <= n - 1; i0 += kUS) {
-#pragma unroll
-            for (int u = 0; u < kUS; ++u) gn[u] = sc.SG(min(i0 + kUS + u, n - 1));
-#pragma unroll
-            for (int u = 0; u < kUS; ++u) {
-                if (scan && i0 + u <= n - 1 && fabs(g[u]) <= fabs(mingma)) {
-                    mingma = g[u];
-                    r = i0 + u + 1;
-                }
-            }
-#pragma unroll
-            for (int u = 0; u < kUS; ++u) g[u] = gn[u];
-        }
-    }
-    if (act) {
-        o->negcnt = wantnc ? neg1 + neg2 : -1;
-        o->r = r;
-        o->mingma = mingma;
-        o->sawnan = sawnan1 || sawnan2;
+        for (int u = 0; u < kUS; ++u)
+            if (t0 + u <= hi) f(t0 + u, X[u]);
+    };
+    load(A, lo);
+    for (int t0 = lo; t0 <= hi; t0 += 2 * kUS) {
+        load(B, t0 + kUS);
+        run(A, t0);
+        if (t0 + kUS > hi) break;
+        load(A, t0 + 2 * kUS);
+        run(B, t0 + kUS);
     }
 }
 
-// The vector solve of odr1v.f:287-355 for every lane with act set.  With stage ==
-// false it forms ZTZ and the support; with stage == true it writes z * scale for the
-// block rows 1..n into SG (zero outside the support).  Index ranges are wave-uniform
-// as in r1v_wave.
-__device__ void solve_wave(int n, const double* __restrict__ LD, double gaptol, const WScr& sc,
-                           R1v* o, bool act, bool stage, double scale) {
+// Descending twin of sweep_up: t = hi down to lo.
+template <int NA, typename F>
+__device__ __forceinline__ void sweep_down(int hi, int lo, const double* const (&P)[NA], F f) {
+    if (lo > hi) return;
+    double A[kUS][NA], B[kUS][NA];
+    auto load = [&](double(&X)[kUS][NA], int t0) {
+#pragma unroll
+        for (int u = 0; u < kUS; ++u)
+#pragma unroll
+            for (int a = 0; a < NA; ++a) X[u][a] = P[a][max(t0 - u, lo) - 1];
+    };
+    auto run = [&](const double(&X)[kUS][NA], int t0) {
+#pragma unroll
+        for (int u = 0; u < kUS; ++u)
+            if (t0 - u >= lo) f(t0 - u, X[u]);
+    };
+    load(A, hi);
+    for (int t0 = hi; t0 >= lo; t0 -= 2 * kUS) {
+        load(B, t0 - kUS);
+        run(A, t0);
+        if (t0 - kUS < lo) break;
+        load(A, t0 - 2 * kUS);
+        run(B, t0 - kUS);
+    }
+}
+
+// Progressive transform from step n-1 down to r1 with the gamma values and the
+// twist index (odr1v.f:231-283).  Free-twist lanes (r1 = 1, r2 = n) scan every gamma;
+// the scan runs downwards here, which selects the same index as the upward scan of
+// the reference (the last index of minimal |gamma| wins there; NaNs never win except
+// at R1).  The S values come from recomputing each chunk into the LDS buffer.
+__device__ void pass_prog(int n, const Rrr& R, double lambda, double pivmin, int r1, bool freetw,
+                          bool act, bool safe1, bool safe2, double slast, const Ck& ck,
+                          ChunkBuf& buf, int lane, double& p_out, int& neg2_out,
+                          double& g0raw_out, double& mingma_out, int& r_out) {
+    const double pn = R.D[n - 1] - lambda;   // P(BN-1)
+    double p = pn;
+    int neg2 = 0;
+    bool have = false;
+    double best = 0.0, g0raw = 0.0;
+    int rbest = 0;
+    if (act && freetw) {   // gamma at index n-1 (twist n), the first of the downward scan
+        double g = slast + pn;
+        if (g == 0.0) g = kEps * slast;
+        if (!isnan(g)) { have = true; best = g; rbest = n; }
+    }
+    if (act && !freetw && r1 == n) g0raw = slast + pn;
+    const int lo = act ? r1 : n;   // this lane's steps: n-1 down to lo
+    const int loU = wave_min_u(lo);
+    const bool needS = act && freetw;
+    const bool anyfree = __any(needS);
+    if (n - 1 >= loU) {
+        const double* const PS[3] = {R.D, R.LD, R.L};
+        const double* const PP[3] = {R.D, R.LLD, R.L};
+        const int cTop = (n - 2) / KC, cBot = (loU - 1) / KC;
+        double sck = (needS && cTop >= 1) ? ck.S(cTop) : 0.0;
+        for (int c = cTop; c >= cBot; --c) {
+            const int base = c * KC;
+            const int tlo = base + 1, thi = min(base + KC, n - 1);
+            if (anyfree) {
+                double s = (c == 0) ? 0.0 - lambda : sck;
+                sck = (needS && c - 1 >= 1) ? ck.S(c - 1) : 0.0;   // next chunk's carry
+                if (c == 0) buf[0][lane] = 0.0;                    // S(0)
+                sweep_up<3>(max(base, 1), thi - 1, PS, [&](int j, const double(&v)[3]) {
+                    if (needS) {
+                        double lp, si;
+                        bool ng;
+                        stat_step(v[0], v[1], v[2], R.LLD, j, lambda, safe1, pivmin, s, lp, si, ng);
+                        buf[j - base][lane] = si;
+                    }
+                });
+            }
+            if (act && lo <= thi) ck.P(c) = p;
+            sweep_down<3>(thi, max(tlo, loU), PP, [&](int t, const double(&v)[3]) {
+                if (act && t >= lo) {
+                    double um;
+                    bool ng;
+                    prog_step(v[0], v[1], v[2], lambda, safe2, pivmin, p, um, ng);
+                    neg2 += ng;
+                    if (freetw) {
+                        const double sg = buf[t - 1 - base][lane];   // S(t-1)
+                        double gm = sg + p;
+                        if (t == 1) {
+                            g0raw = gm;
+                        } else {
+                            if (gm == 0.0) gm = kEps * sg;
+                            if (!isnan(gm) && (!have || fabs(gm) < fabs(best))) {
+                                have = true;
+                                best = gm;
+                                rbest = t;
+                            }
+                        }
+                    } else if (t == r1) {
+                        g0raw = slast + p;
+                    }
+                }
+            });
+        }
+    }
+    double mingma;
+    int r;
+    if (freetw) {
+        double g0 = g0raw;
+        if (g0 == 0.0) g0 = kEps * 0.0;   // EPS * S(0)
+        if (isnan(g0) || !have || fabs(g0) < fabs(best)) { mingma = g0; r = 1; }
+        else { mingma = best; r = rbest; }
+    } else {
+        mingma = g0raw;
+        if (fabs(mingma) == 0.0) mingma = kEps * slast;
+        r = r1;
+    }
+    p_out = p;
+    neg2_out = neg2;
+    g0raw_out = g0raw;
+    mingma_out = mingma;
+    r_out = r;
+}
+
+// dlar1v up to the twist (odr1v.f:156-283) for the lanes with act set; rin is the
+// twist index or 0 for a free twist.
+__device__ void r1v_ck(int n, double lambda, const Rrr& R, double pivmin, int wantnc, int rin,
+                       bool act, const Ck& ck, ChunkBuf& buf, int lane, R1v* o) {
+    const int r1 = rin ? rin : 1, r2 = rin ? rin : n;
+    const bool freetw = rin == 0;
+    double s, slast;
+    int neg1;
+    pass_stat(R, lambda, pivmin, r1, act ? r2 : 1, false, ck, s, neg1, slast);
+    const bool nan1 = act && isnan(s);
+    if (__any(nan1)) {   // slow stationary form (odr1v.f:202-228)
+        double s2, sl2;
+        int ng2;
+        pass_stat(R, lambda, pivmin, r1, nan1 ? r2 : 1, true, ck, s2, ng2, sl2);
+        if (nan1) { neg1 = ng2; slast = sl2; }
+    }
+    double p, g0raw, mingma;
+    int neg2, r;
+    pass_prog(n, R, lambda, pivmin, r1, freetw, act, nan1, false, slast, ck, buf, lane, p, neg2,
+              g0raw, mingma, r);
+    const bool nan2 = act && isnan(p);
+    if (__any(nan2)) {   // slow progressive form (odr1v.f:244-257)
+        double p2, g2, m2;
+        int ng2, rr2;
+        pass_prog(n, R, lambda, pivmin, r1, freetw, nan2, nan1, true, slast, ck, buf, lane, p2,
+                  ng2, g2, m2, rr2);
+        if (nan2) { neg2 = ng2; g0raw = g2; mingma = m2; r = rr2; }
+    }
+    if (act) {
+        if (g0raw < 0.0) ++neg1;
+        o->negcnt = wantnc ? neg1 + neg2 : -1;
+        o->r = r;
+        o->mingma = mingma;
+        o->nan1 = nan1;
+        o->nan2 = nan2;
+    }
+}
+
+// Column write of one chunk with per-lane rows: column c takes rows wr0[c] + sub for
+// sub = 0 .. KC-1 within [wlo[c], whi[c]] (1-based block rows) from buf[sub][c].
+__device__ __forceinline__ void write_chunk(const ChunkBuf& buf, const ColOut& co, int lane,
+                                            int row0, int rlo, int rhi) {
+    co.wr0[lane] = row0;
+    co.wlo[lane] = rlo;
+    co.whi[lane] = rhi;
+    __syncthreads();
+    const int sub = lane & 15, cq = lane >> 4;
+#pragma unroll 4
+    for (int cg = 0; cg < SW / 4; ++cg) {
+        const int c = cg * 4 + cq;
+        if (c < co.cntw) {
+            const int row = co.wr0[c] + sub;
+            if (row >= co.wlo[c] && row <= co.whi[c]) co.zcol[c][co.bbeg + row - 1] = buf[sub][c];
+        }
+    }
+    __syncthreads();
+}
+
+// The vector solve of odr1v.f:285-355 for the lanes with act set.  Every lane walks its
+// own chunks (KC rows each): upwards from the twist with L+ recomputed from the
+// stationary checkpoints, downwards with U- recomputed from the progressive ones, and
+// stops at its support boundary, so a wave costs its longest support rather than the
+// union of its lanes' ranges.  The first loads and the checkpoint of a lane's next
+// chunk are issued while the current chunk is computed.  Lanes with stg set recompute
+// the vector of their last dlar1v and write z * scale into their columns (co), zeros
+// outside the support and outside the block; the other active lanes form ZTZ, the
+// residual and the support.
+__device__ __forceinline__ void pass_solve(int n, const Rrr& R, double lambda, double pivmin,
+                                           double gaptol, const Ck& ck, ChunkBuf& buf,
+                                           ChunkBuf& bld, int lane, R1v* o, bool act, bool stg,
+                                           double scale, const ColOut& co) {
     const int r = act ? o->r : 1;
-    const bool nan = act && o->sawnan;
+    const bool safe1 = act && o->nan1, safe2 = act && o->nan2;
+    const bool nan = safe1 || safe2;
     int is0 = 1, is1 = n;
     double ztz = 1.0;
-    if (act && stage) sc.SG(r - 1) = scale;   // Z(r) = 1
-    // upwards from r: this lane covers I = r-1 down to 1
+    int upper_lo = n + 1, lower_hi = 0;
+    const bool stage = __any(act && stg);   // some lane writes its column
+    // upwards: Z(j) for j = r-1 .. 1; chunk c holds steps c*KC .. c*KC+KC-1
     {
-        double zi1 = 1.0, zi2 = 0.0;   // Z(I+1), Z(I+2)
-        bool live = act && !nan && r - 1 >= 1;
-        const int topU = wave_max_u((act && !nan) ? r - 1 : 0);
-        if (topU >= 1) {
-            double lp[kUS], ld[kUS], lpn[kUS], ldn[kUS];
+        double zi1 = 1.0, zi2 = 0.0;   // Z(j+1), Z(j+2)
+        bool live = act && r - 1 >= 1;
+        int c = live ? (r - 1) / KC : 0;
+        double pre[kUS][3];
+        double pck = 0.0;
+#pragma unroll
+        for (int u = 0; u < kUS; ++u) {
+            const int j = min(max(c * KC + u, 1), n - 1);
+            pre[u][0] = R.D[j - 1]; pre[u][1] = R.LD[j - 1]; pre[u][2] = R.L[j - 1];
+        }
+        if (live && c >= 1) pck = ck.S(c);
+        while (__any(live)) {
+            const bool mine = live;
+            const int base = c * KC;
+            const int jlo = max(base, 1), jhi = min(base + KC - 1, r - 1);
+            double s = (c == 0) ? 0.0 - lambda : pck;
+            double cur[kUS][3], nxt[kUS][3];
+#pragma unroll
+            for (int u = 0; u < kUS; ++u)
+#pragma unroll
+                for (int a = 0; a < 3; ++a) cur[u][a] = pre[u][a];
+            // next chunk of this lane (c - 1): first loads and carry
+            const int cn = max(c - 1, 0);
 #pragma unroll
             for (int u = 0; u < kUS; ++u) {
-                int i = max(topU - u, 1);
-                lp[u] = sc.LP(i); ld[u] = LD[i - 1];
+                const int j = min(max(cn * KC + u, 1), n - 1);
+                pre[u][0] = R.D[j - 1]; pre[u][1] = R.LD[j - 1]; pre[u][2] = R.L[j - 1];
             }
-            for (int i0 = topU; i0 >= 1; i0 -= kUS) {
-                if (!stage && !__any(live)) break;
+            pck = (mine && c - 1 >= 1) ? ck.S(c - 1) : 0.0;
+            // recompute L+ (and keep LD) for the chunk
 #pragma unroll
-                for (int u = 0; u < kUS; ++u) {
-                    int i = max(i0 - kUS - u, 1);
-                    lpn[u] = sc.LP(i); ldn[u] = LD[i - 1];
-                }
+            for (int g = 0; g < KC / kUS; ++g) {
+                if (g + 1 < KC / kUS) {
 #pragma unroll
-                for (int u = 0; u < kUS; ++u) {
-                    const int i = i0 - u;
-                    if (act && !nan && i >= 1 && i <= r - 1) {
-                        if (live) {
-                            double zi = -(lp[u] * zi1);
-                            if ((fabs(zi) + fabs(zi1)) * fabs(ld[u]) < gaptol) {
-                                is0 = i + 1;
-                                live = false;
-                                if (stage) sc.SG(i - 1) = 0.0;
-                            } else {
-                                ztz = ztz + zi * zi;
-                                if (stage) sc.SG(i - 1) = zi * scale;
-                                zi1 = zi;
-                            }
-                        } else if (stage) {
-                            sc.SG(i - 1) = 0.0;
-                        }
+                    for (int u = 0; u < kUS; ++u) {
+                        const int j = min(max(base + (g + 1) * kUS + u, 1), n - 1);
+                        nxt[u][0] = R.D[j - 1]; nxt[u][1] = R.LD[j - 1]; nxt[u][2] = R.L[j - 1];
                     }
                 }
 #pragma unroll
-                for (int u = 0; u < kUS; ++u) { lp[u] = lpn[u]; ld[u] = ldn[u]; }
-            }
-        }
-        if (nan) {   // slow form after a NaN (odr1v.f:308-322)
-            bool cut = false;
-            for (int i = r - 1; i >= 1; --i) {
-                if (cut) { if (stage) sc.SG(i - 1) = 0.0; continue; }
-                double zi;
-                if (zi1 == 0.0) zi = -(LD[i] / LD[i - 1]) * zi2;
-                else zi = -(sc.LP(i) * zi1);
-                if ((fabs(zi) + fabs(zi1)) * fabs(LD[i - 1]) < gaptol) {
-                    is0 = i + 1;
-                    cut = true;
-                    if (stage) sc.SG(i - 1) = 0.0;
-                    continue;
+                for (int u = 0; u < kUS; ++u) {
+                    const int k = g * kUS + u, j = base + k;
+                    if (mine && j >= jlo && j <= jhi) {
+                        double lp, si;
+                        bool ng;
+                        stat_step(cur[u][0], cur[u][1], cur[u][2], R.LLD, j, lambda, safe1, pivmin,
+                                  s, lp, si, ng);
+                        buf[k][lane] = lp;
+                        bld[k][lane] = cur[u][1];
+                    }
                 }
-                ztz = ztz + zi * zi;
-                if (stage) sc.SG(i - 1) = zi * scale;
-                zi2 = zi1;
-                zi1 = zi;
+                if (g + 1 < KC / kUS) {
+#pragma unroll
+                    for (int u = 0; u < kUS; ++u)
+#pragma unroll
+                        for (int a = 0; a < 3; ++a) cur[u][a] = nxt[u][a];
+                }
             }
+            // consume downwards
+#pragma unroll
+            for (int k = KC - 1; k >= 0; --k) {
+                const int j = base + k;
+                if (mine && j >= jlo && j <= jhi) {
+                    const double ldj = bld[k][lane];
+                    double zv = 0.0;
+                    if (live) {
+                        double zi;
+                        if (nan && zi1 == 0.0) zi = -(R.LD[j] / ldj) * zi2;
+                        else zi = -(buf[k][lane] * zi1);
+                        if ((fabs(zi) + fabs(zi1)) * fabs(ldj) < gaptol) {
+                            is0 = j + 1;
+                            live = false;
+                        } else {
+                            ztz = ztz + zi * zi;
+                            zv = zi;
+                            zi2 = zi1;
+                            zi1 = zi;
+                        }
+                    }
+                    if (stg) buf[k][lane] = zv * scale;
+                }
+            }
+            if (mine) upper_lo = jlo;
+            if (stage) write_chunk(buf, co, lane, base, mine && stg ? jlo : 1, mine && stg ? jhi : 0);
+            if (c == 0) live = false;
+            c = cn;
         }
     }
-    // downwards from r: this lane covers I = r .. n-1
+    // downwards: Z(t+1) for t = r .. n-1; chunk c holds steps c*KC+1 .. c*KC+KC
     {
-        double zi = 1.0, zim1 = 0.0;   // Z(I), Z(I-1)
-        bool live = act && !nan && r <= n - 1;
-        const int loU = wave_min_u((act && !nan) ? r : n);
-        if (loU <= n - 1) {
-            double um[kUS], ld[kUS], umn[kUS], ldn[kUS];
+        double zi = 1.0, zim1 = 0.0;   // Z(t), Z(t-1)
+        bool live = act && r <= n - 1;
+        const int cEnd = (n - 2) / KC;
+        int c = live ? (r - 1) / KC : 0;
+        double pre[kUS][4];
+        double pcp = 0.0;
+#pragma unroll
+        for (int u = 0; u < kUS; ++u) {
+            const int t = min(c * KC + KC - u, n - 1);
+            pre[u][0] = R.D[t - 1]; pre[u][1] = R.LLD[t - 1]; pre[u][2] = R.L[t - 1];
+            pre[u][3] = R.LD[t - 1];
+        }
+        if (live) pcp = ck.P(c);
+        while (__any(live)) {
+            const bool mine = live;
+            const int tlo = c * KC + 1, thi = min(c * KC + KC, n - 1);
+            const int tl2 = max(tlo, r);
+            double p = pcp;
+            double cur[kUS][4], nxt[kUS][4];
+#pragma unroll
+            for (int u = 0; u < kUS; ++u)
+#pragma unroll
+                for (int a = 0; a < 4; ++a) cur[u][a] = pre[u][a];
+            const int cn = min(c + 1, cEnd);
 #pragma unroll
             for (int u = 0; u < kUS; ++u) {
-                int i = min(loU + u, n - 1);
-                um[u] = sc.UM(i); ld[u] = LD[i - 1];
+                const int t = min(cn * KC + KC - u, n - 1);
+                pre[u][0] = R.D[t - 1]; pre[u][1] = R.LLD[t - 1]; pre[u][2] = R.L[t - 1];
+                pre[u][3] = R.LD[t - 1];
             }
-            for (int i0 = loU; i0 <= n - 1; i0 += kUS) {
-                if (!stage && !__any(live)) break;
+            pcp = (mine && c + 1 <= cEnd) ? ck.P(c + 1) : 0.0;
+            // recompute U- (and keep LD) for the chunk, from its top down
 #pragma unroll
-                for (int u = 0; u < kUS; ++u) {
-                    int i = min(i0 + kUS + u, n - 1);
-                    umn[u] = sc.UM(i); ldn[u] = LD[i - 1];
-                }
+            for (int g = 0; g < KC / kUS; ++g) {
+                if (g + 1 < KC / kUS) {
 #pragma unroll
-                for (int u = 0; u < kUS; ++u) {
-                    const int i = i0 + u;
-                    if (act && !nan && i >= r && i <= n - 1) {
-                        if (live) {
-                            double zn = -(um[u] * zi);
-                            if ((fabs(zi) + fabs(zn)) * fabs(ld[u]) < gaptol) {
-                                is1 = i;
-                                live = false;
-                                if (stage) sc.SG(i) = 0.0;
-                            } else {
-                                ztz = ztz + zn * zn;
-                                if (stage) sc.SG(i) = zn * scale;
-                                zi = zn;
-                            }
-                        } else if (stage) {
-                            sc.SG(i) = 0.0;
-                        }
+                    for (int u = 0; u < kUS; ++u) {
+                        const int t = max(min(tlo + KC - 1 - (g + 1) * kUS - u, n - 1), 1);
+                        nxt[u][0] = R.D[t - 1]; nxt[u][1] = R.LLD[t - 1]; nxt[u][2] = R.L[t - 1];
+                        nxt[u][3] = R.LD[t - 1];
                     }
                 }
 #pragma unroll
-                for (int u = 0; u < kUS; ++u) { um[u] = umn[u]; ld[u] = ldn[u]; }
-            }
-        }
-        if (nan) {   // slow form after a NaN (odr1v.f:336-350)
-            bool cut = false;
-            for (int i = r; i <= n - 1; ++i) {
-                if (cut) { if (stage) sc.SG(i) = 0.0; continue; }
-                double zn;
-                if (zi == 0.0) zn = -(LD[i - 2] / LD[i - 1]) * zim1;
-                else zn = -(sc.UM(i) * zi);
-                if ((fabs(zi) + fabs(zn)) * fabs(LD[i - 1]) < gaptol) {
-                    is1 = i;
-                    cut = true;
-                    if (stage) sc.SG(i) = 0.0;
-                    continue;
+                for (int u = 0; u < kUS; ++u) {
+                    const int k = KC - 1 - (g * kUS + u), t = tlo + k;
+                    if (mine && t >= tl2 && t <= thi) {
+                        double um;
+                        bool ng;
+                        prog_step(cur[u][0], cur[u][1], cur[u][2], lambda, safe2, pivmin, p, um, ng);
+                        buf[k][lane] = um;
+                        bld[k][lane] = cur[u][3];
+                    }
                 }
-                ztz = ztz + zn * zn;
-                if (stage) sc.SG(i) = zn * scale;
-                zim1 = zi;
-                zi = zn;
+                if (g + 1 < KC / kUS) {
+#pragma unroll
+                    for (int u = 0; u < kUS; ++u)
+#pragma unroll
+                        for (int a = 0; a < 4; ++a) cur[u][a] = nxt[u][a];
+                }
             }
+            // consume upwards in t
+#pragma unroll
+            for (int k = 0; k < KC; ++k) {
+                const int t = tlo + k;
+                if (mine && t >= tl2 && t <= thi) {
+                    const double ldt = bld[k][lane];
+                    double zv = 0.0;
+                    if (live) {
+                        double zn;
+                        if (nan && zi == 0.0) zn = -(R.LD[max(t - 2, 0)] / ldt) * zim1;
+                        else zn = -(buf[k][lane] * zi);
+                        if ((fabs(zi) + fabs(zn)) * fabs(ldt) < gaptol) {
+                            is1 = t;
+                            live = false;
+                        } else {
+                            ztz = ztz + zn * zn;
+                            zv = zn;
+                            zim1 = zi;
+                            zi = zn;
+                        }
+                    }
+                    if (stg) buf[k][lane] = zv * scale;
+                }
+            }
+            if (mine) lower_hi = thi + 1;
+            if (stage)
+                write_chunk(buf, co, lane, tlo + 1, mine && stg ? tl2 + 1 : 1, mine && stg ? thi + 1 : 0);
+            if (c == cEnd) live = false;
+            c = cn;
         }
     }
-    if (act && !stage) {
-        double tmp = 1.0 / ztz;
+    if (stage) {
+        // Z(r) = 1, zeros above and below the rows written by the chunks
+        const bool w = act && stg;
+        if (w) co.Zc[co.bbeg + r - 1] = scale;
+        // rows [0, a) and [b, nfull) are zero; other columns are left alone
+        co.wlo[lane] = w ? co.bbeg + min(upper_lo, r) - 1 : 0;
+        co.whi[lane] = w ? co.bbeg + max(lower_hi, r) : co.nfull;
+        __syncthreads();
+        for (int c = 0; c < co.cntw; ++c) {
+            double* zc = co.zcol[c];
+            const int a = co.wlo[c], b = co.whi[c];
+            for (int rr = lane; rr < a; rr += SW) zc[rr] = 0.0;
+            for (int rr = b + lane; rr < co.nfull; rr += SW) zc[rr] = 0.0;
+        }
+        __syncthreads();
+    }
+    if (act && !stg) {
+        const double tmp = 1.0 / ztz;
         o->ztz = ztz;
         o->nrminv = sqrt(tmp);
         o->resid = fabs(o->mingma) * o->nrminv;
@@ -1788,174 +2024,227 @@ __device__ void solve_wave(int n, const double* __restrict__ LD, double gaptol, 
 
 // k_singleton: eigenvectors by twisted factorisations with Rayleigh-quotient
 // correction (PMR_process_s_task, process_singleton.c:65-324), one lane per
-// eigenvector.  Every wave takes a task of up to 64 singletons of one block, so the
-// block size, rows and sweep indices are wave-uniform; lanes that share a
-// representation (all of them at the root of a block) hit the same cache lines.  Each
-// wave owns a
-// scratch region of 3 * cap * 64 doubles.  Columns are written once, at the end,
-// through an LDS tile so that each store covers contiguous rows of four columns.
-__device__ void singleton_group(const STask& t, bool valid, int n, int bbeg,
-                                const MatDesc* __restrict__ md, const MatState* __restrict__ ms,
-                                const int* __restrict__ Windex, double* W, double* Wsh,
-                                double* Werr, const double* __restrict__ Wgap, const WScr& sc,
-                                int* supp0, int* supp1) {
-    Rrr R;
-    R.D = t.D; R.L = t.L; R.LD = t.DL; R.LLD = t.DLL;
-    const MatState& S = ms[t.mat];
-    const int off = md[t.mat].off;
-    const int i = t.slot - off;               // matrix-local eigenvalue index
-    const int bl_begin = bbeg, bl_end = bbeg + n - 1;
-    const double pivmin = S.pivmin;
-    const double tol = 4.0 * log((double)n) * DBL_EPSILON;
-    const double RQtol = 2.0 * DBL_EPSILON;
-    double lambda = valid ? Wsh[t.slot] : 0.0;
-    double werr = valid ? Werr[t.slot] : 0.0;
-    double left = lambda - werr, right = lambda + werr;
-    const int i_local = valid ? Windex[t.slot] : 1;
-    double lgap, rgap;
-    if (i == bl_begin) lgap = DBL_EPSILON * fmax(fabs(left), fabs(right));
-    else lgap = t.lgap;
-    if (i == bl_end) rgap = DBL_EPSILON * fmax(fabs(left), fabs(right));
-    else rgap = valid ? Wgap[t.slot] : 0.0;
-    const double gap = fmin(lgap, rgap);
-    const double gaptol = (i == bl_begin || i == bl_end) ? 0.0 : gap * DBL_EPSILON;
-    int usedBS = 0, usedRQ = 0, needBS = 0, wantnc = 1, r = 0;
-    bool done = !valid;
-    double bstres = 0.0, bstw = 0.0, wsh = lambda;
-    R1v o;
-    o.r = 1; o.sawnan = 0; o.resid = 0.0; o.rqcorr = 0.0; o.nrminv = 1.0; o.mingma = 0.0;
-    o.isupp0 = 1; o.isupp1 = n; o.negcnt = 0; o.ztz = 1.0;
-    unsigned long long r1v_el = 0, bis_el = 0, wave_calls = 0;
-    for (int kk = 1; kk <= kMaxIterRQI; ++kk) {
-        const bool act = !done;
-        if (!__any(act)) break;
-        ++wave_calls;
-        if (act && needBS) {
-            usedBS = 1;
-            double w = wsh, we = werr;
-            bis_el += (unsigned long long)n *
-                      dlarrb1(n, t.D, t.DLL, i_local, 0.0, 2.0 * DBL_EPSILON, 0.0, &w, &we, pivmin,
-                              t.bl_spdiam, r);
-            wsh = w;
-            werr = we;
-            lambda = wsh;
-            r = 0;
-        }
-        if (act) wantnc = usedBS ? 0 : 1;
-        r1v_wave(n, lambda, R, pivmin, wantnc, r, act, sc, &o);
-        solve_wave(n, R.LD, gaptol, sc, &o, act, false, 0.0);
-        if (act) {
-            r1v_el += n;
-            r = o.r;
-            if (kk == 1 || o.resid < bstres) { bstres = o.resid; bstw = lambda; }
-            if (!usedBS && o.resid > tol * gap && fabs(o.rqcorr) > RQtol * fabs(lambda)) {
-                int sgndef = (i_local <= o.negcnt) ? -1 : 1;
-                if (o.rqcorr * sgndef >= 0.0 && lambda + o.rqcorr <= right &&
-                    lambda + o.rqcorr >= left) {
-                    usedRQ = 1;
-                    if (sgndef == 1) left = lambda;
-                    else right = lambda;
-                    wsh = 0.5 * (left + right);
-                    lambda += o.rqcorr;
-                } else needBS = 1;
-                if (right - left < RQtol * fabs(lambda)) usedBS = 1;
-                else if (kk == kMaxIterRQI - 1) needBS = 1;
-            } else done = true;
-        }
-    }
-    const bool step2 = valid && usedRQ && usedBS && bstres <= o.resid;
-    if (__any(step2)) {
-        ++wave_calls;
-        if (step2) lambda = bstw;
-        r1v_wave(n, lambda, R, pivmin, wantnc, r, step2, sc, &o);
-        solve_wave(n, R.LD, gaptol, sc, &o, step2, false, 0.0);
-        if (step2) r1v_el += n;
-    }
-    solve_wave(n, R.LD, gaptol, sc, &o, valid, true, o.nrminv);
-    add_steps(kStepR1v, r1v_el);
-    add_steps(kStepR1vBis, bis_el);
-    {
-        unsigned long long bmax = bis_el;
-        for (int o2 = 32; o2 > 0; o2 >>= 1) {
-            unsigned long long ot = __shfl_xor(bmax, o2);
-            bmax = ot > bmax ? ot : bmax;
-        }
-        if (threadIdx.x == 0) {
-            add_steps(kStepR1vWave, wave_calls * (unsigned long long)n);
-            add_steps(kStepBisWave, bmax);
-        }
-    }
-    *supp0 = o.isupp0;
-    *supp1 = o.isupp1;
-    if (valid) {
-        Wsh[t.slot] = lambda;
-        Werr[t.slot] = werr;
-        W[t.slot] = lambda + t.L[n - 1];
-    }
-}
+// eigenvector.  A wave takes a task: a list of up to kSGen * 64 singletons of one
+// block, so the block size and rows are wave-uniform.  The lanes work through the
+// list: a lane whose iteration has converged recomputes and writes its vector in the
+// next solve pass and then takes the next singleton of the list, so a slow lane no
+// longer holds 63 idle ones.  Per lane the iteration is that of the reference
+// (odrrv.f / process_singleton.c): RQI with bisection fallback, at most kMaxIterRQI
+// dlar1v calls, the best-residual step after a bisection.
+enum : int { kLaneEmpty = 0, kLaneActive = 1, kLaneStep2 = 2, kLaneStage = 3 };
 
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_singleton(const WaveTask* __restrict__ tasks,
-                                                  const LevelCounters* __restrict__ cnt,
-                                                  const STask* __restrict__ T,
-                                                  const MatDesc* __restrict__ md,
-                                                  const MatState* __restrict__ ms,
-                                                  const int* __restrict__ Windex,
-                                                  const int* __restrict__ Zidx, double* W,
-                                                  double* Wsh, double* Werr,
-                                                  const double* __restrict__ Wgap, double* scratch,
-                                                  int cap) {
-    __shared__ double tile[TILE_R][SW + 1];
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_singleton(
+    const WaveTask* __restrict__ tasks, LevelCounters* cnt, const STask* __restrict__ T,
+    const MatDesc* __restrict__ md, const MatState* __restrict__ ms,
+    const int* __restrict__ Windex, const int* __restrict__ Zidx, double* W, double* Wsh,
+    double* Werr, const double* __restrict__ Wgap, double* scratch, int cap) {
+    __shared__ ChunkBuf buf, bld;
+    __shared__ double* zcol[SW];
+    __shared__ int wr0[SW], wlo[SW], whi[SW];
+    __shared__ int s_task;
     const int lane = threadIdx.x;
     const int nT = __builtin_amdgcn_readfirstlane(cnt->nT);
-    WScr sc;
-    sc.base = scratch + (size_t)blockIdx.x * 3 * (size_t)cap * SW;
-    sc.lane = lane;
-    for (int ti = blockIdx.x; ti < nT; ti += gridDim.x) {
+    Ck ck;
+    ck.nck = cap / KC + 2;
+    ck.lane = lane;
+    ck.base = scratch + (size_t)blockIdx.x * 2 * (size_t)ck.nck * SW;
+    const double RQtol = 2.0 * DBL_EPSILON;
+    for (;;) {
+        // tasks are handed out in list order (longest blocks first) as waves free up;
+        // the index goes through LDS between barriers so that every lane of the wave
+        // takes the same task (a value read from one lane of a lane-0-only branch is
+        // not safe against the compiler splitting the wave)
+        __syncthreads();
+        if (lane == 0) s_task = atomicAdd(&cnt->next, 1);
+        __syncthreads();
+        const int ti = s_task;
+        if (ti >= nT) break;
+        const unsigned long long tick0 = wall_clock64();
         const WaveTask wt = tasks[ti];
         const int cntw = __builtin_amdgcn_readfirstlane(wt.cnt);
         const int start = __builtin_amdgcn_readfirstlane(wt.start);
-        const bool valid = lane < cntw;
-        // lanes past the end replay the last task so that every pointer is valid
-        const STask t = T[start + min(lane, cntw - 1)];
-        const int n = __builtin_amdgcn_readfirstlane(t.bsize);
-        const int bbeg = __builtin_amdgcn_readfirstlane(t.bbeg);
-        const MatDesc& dsc = md[t.mat];
-        int col = Zidx[t.slot];
-        if (col >= dsc.mcols) col = dsc.mcols - 1;   // never write past the caller's Z
-        double* Zc = dsc.Z + (size_t)col * dsc.ldz;
-        const int nfull = __builtin_amdgcn_readfirstlane(dsc.n);
-        int s0 = 1, s1 = 1;
+        const STask t0 = T[start];
+        const int n = __builtin_amdgcn_readfirstlane(t0.bsize);
+        const int bbeg = __builtin_amdgcn_readfirstlane(t0.bbeg);
+        const int nfull = __builtin_amdgcn_readfirstlane(md[t0.mat].n);
+        ColOut co;
+        co.zcol = zcol;
+        co.wr0 = wr0;
+        co.wlo = wlo;
+        co.whi = whi;
+        co.bbeg = bbeg;
+        co.nfull = nfull;
+        co.cntw = SW;
+        unsigned diag = 0;
         if (n == 1) {
-            sc.SG(0) = 1.0;
+            // 1x1 blocks: e_bbeg
+            for (int b0 = 0; b0 < cntw; b0 += SW) {
+                const int j = b0 + lane;
+                const STask t = T[start + min(j, cntw - 1)];
+                const MatDesc& dsc = md[t.mat];
+                int col = Zidx[t.slot];
+                if (col >= dsc.mcols) col = dsc.mcols - 1;   // never write past the caller's Z
+                zcol[lane] = dsc.Z + (size_t)col * dsc.ldz;
+                if (j < cntw) {
+                    dsc.Zsupp[2 * col] = 1 + bbeg;
+                    dsc.Zsupp[2 * col + 1] = 1 + bbeg;
+                }
+                __syncthreads();
+                for (int c = 0; c < min(SW, cntw - b0); ++c) {
+                    double* zc = zcol[c];
+                    for (int rr = lane; rr < nfull; rr += SW) zc[rr] = rr == bbeg ? 1.0 : 0.0;
+                }
+                __syncthreads();
+            }
         } else {
-            singleton_group(t, valid, n, bbeg, md, ms, Windex, W, Wsh, Werr, Wgap, sc, &s0, &s1);
-        }
-        if (valid) {
-            dsc.Zsupp[2 * col] = s0 + bbeg;
-            dsc.Zsupp[2 * col + 1] = s1 + bbeg;
-        }
-        // ---- column write: zero rows outside the block, then the block rows through
-        // an LDS tile (16 rows x 4 columns per store)
-        for (int c = 0; c < cntw; ++c) {
-            double* zc = (double*)__shfl((long long)Zc, c);
-            for (int rr = lane; rr < bbeg; rr += SW) zc[rr] = 0.0;
-            for (int rr = bbeg + n + lane; rr < nfull; rr += SW) zc[rr] = 0.0;
-        }
-        const int sub = lane & 15, cq = lane >> 4;
-        for (int r0 = 0; r0 < n; r0 += TILE_R) {
-#pragma unroll
-            for (int rr = 0; rr < TILE_R; ++rr) {
-                const int row = min(r0 + rr, n - 1);
-                tile[rr][lane] = sc.SG(row);
+            const int bl_begin = bbeg, bl_end = bbeg + n - 1;
+            const double tol = 4.0 * log((double)n) * DBL_EPSILON;
+            // lane state
+            int state = kLaneEmpty, idx = 0;
+            STask t = t0;
+            Rrr R;
+            R.D = t.D; R.L = t.L; R.LD = t.DL; R.LLD = t.DLL;
+            double pivmin = 0.0, lambda = 0.0, werr = 0.0, left = 0.0, right = 0.0;
+            double gap = 0.0, gaptol = 0.0, bstres = 0.0, bstw = 0.0, wsh = 0.0;
+            double lamv = 0.0;   // shift of this lane's last dlar1v (its vector's shift)
+            unsigned long long t_bis = 0, t_r1v = 0, t_solve = 0;
+            int i_local = 1, usedBS = 0, usedRQ = 0, needBS = 0, wantnc = 1, r = 0, kk = 1;
+            int col = 0;
+            R1v o;
+            o.r = 1; o.nan1 = 0; o.nan2 = 0; o.resid = 0.0; o.rqcorr = 0.0; o.nrminv = 1.0;
+            o.mingma = 0.0; o.isupp0 = 1; o.isupp1 = n; o.negcnt = 0; o.ztz = 1.0;
+            unsigned long long r1v_el = 0, bis_el = 0, wave_calls = 0, bmax_w = 0;
+            int next = 0;   // next list entry to hand out
+            for (;;) {
+                // refill empty lanes from the list
+                {
+                    const bool empty = state == kLaneEmpty;
+                    const unsigned long long em = __ballot(empty);
+                    const int rank = __builtin_amdgcn_mbcnt_hi((unsigned)(em >> 32),
+                                                               __builtin_amdgcn_mbcnt_lo((unsigned)em, 0));
+                    if (empty && next + rank < cntw) {
+                        idx = start + next + rank;
+                        t = T[idx];
+                        R.D = t.D; R.L = t.L; R.LD = t.DL; R.LLD = t.DLL;
+                        const MatState& S = ms[t.mat];
+                        const MatDesc& dsc = md[t.mat];
+                        const int i = t.slot - dsc.off;   // matrix-local eigenvalue index
+                        pivmin = S.pivmin;
+                        lambda = Wsh[t.slot];
+                        werr = Werr[t.slot];
+                        left = lambda - werr;
+                        right = lambda + werr;
+                        i_local = Windex[t.slot];
+                        const double lg = (i == bl_begin) ? DBL_EPSILON * fmax(fabs(left), fabs(right)) : t.lgap;
+                        const double rg = (i == bl_end) ? DBL_EPSILON * fmax(fabs(left), fabs(right)) : Wgap[t.slot];
+                        gap = fmin(lg, rg);
+                        gaptol = (i == bl_begin || i == bl_end) ? 0.0 : gap * DBL_EPSILON;
+                        usedBS = 0; usedRQ = 0; needBS = 0; wantnc = 1; r = 0; kk = 1;
+                        bstres = 0.0; bstw = 0.0; wsh = lambda;
+                        col = Zidx[t.slot];
+                        if (col >= dsc.mcols) col = dsc.mcols - 1;   // never write past the caller's Z
+                        zcol[lane] = dsc.Z + (size_t)col * dsc.ldz;
+                        co.Zc = zcol[lane];
+                        state = kLaneActive;
+                    }
+                    next = min(next + __popcll(em), cntw);
+                }
+                const bool act = state == kLaneActive;
+                const bool s2 = state == kLaneStep2;
+                const bool stg = state == kLaneStage;
+                if (!__any(act || s2 || stg)) break;
+                ++wave_calls;
+                unsigned long long tk = wall_clock64();
+                // bisection fallback (odrrb.f via PMR_process_s_task)
+                if (act && needBS) {
+                    usedBS = 1;
+                    double w = wsh, we = werr;
+                    const unsigned long long sw = dlarrb1(n, t.D, t.DLL, i_local, 0.0, 2.0 * DBL_EPSILON,
+                                                          0.0, &w, &we, pivmin, t.bl_spdiam, r);
+                    bis_el += (unsigned long long)n * sw;
+                    wsh = w;
+                    werr = we;
+                    lambda = wsh;
+                    r = 0;
+                }
+                {
+                    unsigned long long bm = (act && needBS) ? 1ULL : 0ULL;
+                    bmax_w += __any(bm) ? 1 : 0;
+                }
+                if (act) wantnc = usedBS ? 0 : 1;
+                if (s2) lambda = bstw;
+                if (act || s2) lamv = lambda;
+                unsigned long long tk1 = wall_clock64();
+                t_bis += tk1 - tk;
+                r1v_ck(n, lamv, R, pivmin, wantnc, r, act || s2, ck, buf, lane, &o);
+                tk = wall_clock64();
+                t_r1v += tk - tk1;
+                pass_solve(n, R, lamv, pivmin, gaptol, ck, buf, bld, lane, &o, act || s2 || stg,
+                           stg, o.nrminv, co);
+                t_solve += wall_clock64() - tk;
+                if (stg) {
+                    // the vector is written: report and free the lane
+                    const MatDesc& dsc = md[t.mat];
+                    dsc.Zsupp[2 * col] = o.isupp0 + bbeg;
+                    dsc.Zsupp[2 * col + 1] = o.isupp1 + bbeg;
+                    Wsh[t.slot] = lambda;
+                    Werr[t.slot] = werr;
+                    W[t.slot] = lambda + t.L[n - 1];
+                    state = kLaneEmpty;
+                }
+                if (s2) {
+                    r1v_el += n;
+                    state = kLaneStage;
+                }
+                if (act) {
+                    r1v_el += n;
+                    r = o.r;
+                    bool done = false;
+                    if (kk == 1 || o.resid < bstres) { bstres = o.resid; bstw = lambda; }
+                    if (!usedBS && o.resid > tol * gap && fabs(o.rqcorr) > RQtol * fabs(lambda)) {
+                        int sgndef = (i_local <= o.negcnt) ? -1 : 1;
+                        if (o.rqcorr * sgndef >= 0.0 && lambda + o.rqcorr <= right &&
+                            lambda + o.rqcorr >= left) {
+                            usedRQ = 1;
+                            if (sgndef == 1) left = lambda;
+                            else right = lambda;
+                            wsh = 0.5 * (left + right);
+                            lambda += o.rqcorr;
+                        } else needBS = 1;
+                        if (right - left < RQtol * fabs(lambda)) usedBS = 1;
+                        else if (kk == kMaxIterRQI - 1) needBS = 1;
+                    } else done = true;
+                    ++kk;
+                    if (done || kk > kMaxIterRQI) {
+                        // the best-residual step after a bisection, else the vector
+                        state = (usedRQ && usedBS && bstres <= o.resid) ? kLaneStep2 : kLaneStage;
+                    }
+                }
             }
-            __syncthreads();
-            for (int cg = 0; cg < SW / 4; ++cg) {
-                const int c = cg * 4 + cq;
-                double* zc = (double*)__shfl((long long)Zc, c);
-                const int row = r0 + sub;
-                if (c < cntw && row < n) zc[bbeg + row] = tile[sub][c];
+            add_steps(kStepR1v, r1v_el);
+            add_steps(kStepR1vBis, bis_el);
+            if (lane == 0) {
+                add_steps(kStepR1vWave, wave_calls * (unsigned long long)n);
+                add_steps(kStepBisWave, bmax_w * (unsigned long long)n);
+                add_steps(kStepTBis, t_bis);
+                add_steps(kStepTR1v, t_r1v);
+                add_steps(kStepTSolve, t_solve);
             }
-            __syncthreads();
+            diag = (unsigned)min(wave_calls, 255ULL) | ((unsigned)min(bmax_w, 65535ULL) << 8);
+        }
+        if (lane == 0) {
+            const unsigned long long tick1 = wall_clock64();
+            add_steps(kStepTaskTicks, tick1 - tick0);
+            add_steps(kStepTasks, 1);
+            unsigned long long* lg = g_tasklog;
+            if (lg && ti < g_tasklog_cap) {
+                const unsigned hw = __builtin_amdgcn_s_getreg((4 << 0) | (0 << 6) | (31 << 11));
+                const unsigned xcc = __builtin_amdgcn_s_getreg((20 << 0) | (0 << 6) | (31 << 11));
+                lg[4 * ti + 0] = tick0;
+                lg[4 * ti + 1] = tick1;
+                lg[4 * ti + 2] = ((unsigned long long)n << 32) | ((unsigned)min(cntw, 255) << 24) | diag;
+                lg[4 * ti + 3] = ((unsigned long long)xcc << 32) | hw;
+            }
         }
     }
 }
