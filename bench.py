@@ -136,6 +136,9 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--copy-calib", type=int, default=0,
+                    help="only run a device-to-device copy of this many MiB (five times): the "
+                         "known-size kernel that scales the HBM byte counters")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1",
@@ -146,6 +149,15 @@ def main():
     import mr3smp_amd as M
     dev = torch.device("cuda:0")
     torch.cuda.set_device(0)
+    if args.copy_calib:
+        nel = args.copy_calib * (1 << 20) // 8
+        src = torch.rand(nel, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calib_bytes_each_way": nel * 8, "copies": 5}))
+        return
     log("building workload")
     mats, probs = build_workload(torch, dev, args.count)
     pairs = sum(len(D) for D, _, _ in mats)

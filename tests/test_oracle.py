@@ -140,6 +140,24 @@ def test_dlarnv_bit_exact(oracle_lib):
 # ---------------------------------------------------------------------------------
 # what the mathematics fixes
 # ---------------------------------------------------------------------------------
+@pytest.mark.parametrize("seed", [(1, 1, 1, 1), (4095, 1, 77, 3001), (12, 345, 678, 911)])
+def test_dlarnv_is_the_48bit_lcg(oracle_lib, seed):
+    """dlaruv's 128 multipliers (odruv.f) are the powers a^i mod 2^48 of
+    a = 33952834046453: entry i of the stream is seed * a^(i+1) mod 2^48 scaled to
+    (0,1), and dlarnv IDIST=2 maps u to 2u - 1.  700 entries cross five 128-blocks."""
+    a, mod = 33952834046453, 1 << 48
+    x = ((seed[0] * 4096 + seed[1]) * 4096 + seed[2]) * 4096 + seed[3]
+    want = []
+    for _ in range(700):
+        x = (x * a) % mod
+        want.append(2.0 * (x / float(mod)) - 1.0)
+    iseed = (ctypes.c_int * 4)(*seed)
+    got = np.zeros(700)
+    oracle_lib.oracle_dlarnv2(iseed, 700, ptr(got))
+    assert np.array_equal(got, np.array(want))
+    assert list(iseed) == [(x >> 36) & 4095, (x >> 24) & 4095, (x >> 12) & 4095, x & 4095]
+
+
 @pytest.mark.parametrize("n", [3, 17, 64, 250])
 def test_toeplitz_closed_form(n, oracle):
     _, fn = oracle
