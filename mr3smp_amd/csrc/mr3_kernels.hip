@@ -570,7 +570,7 @@ __global__ __launch_bounds__(256) void k_coarse(const MatDesc* __restrict__ md,
         double tmp1 = fabs(right - left), tmp2 = fmax(fabs(right), fabs(left));
         if (tmp1 < fmax(fmax(atoli, pivmin), rtoli * tmp2)) break;
         double mid = 0.5 * (left + right);
-        int c = count_T_ijob2(nbl, d, e2, mid, pivmin);
+        int c = count_T_ijob2_w(nbl, d, e2, mid, pivmin);
         ++sweeps;
         c = min(nr, max(nl, c));
         if (c >= i) { right = mid; nr = c; }
@@ -834,8 +834,8 @@ __global__ __launch_bounds__(256) void k_refine_root(const MatDesc* __restrict__
         double rgap = Wgap[gidx];
         double lgap = (s == br.wbeg) ? rgap : Wgap[gidx - 1];
         double gap = fmin(lgap, rgap);
-        int sw = dlarrb1(nbl, D + off + br.bbeg, DLLr + off + br.bbeg, Windex[gidx], S.rtol1,
-                         S.rtol2, gap, &w, &we, S.pivmin, br.spdiam, nbl);
+        int sw = dlarrb1_w(nbl, D + off + br.bbeg, DLLr + off + br.bbeg, Windex[gidx], S.rtol1,
+                           S.rtol2, gap, &w, &we, S.pivmin, br.spdiam, nbl);
         add_steps(kStepRefine, (unsigned long long)sw * nbl);
     }
     Wtmp[gidx] = w;
@@ -905,8 +905,8 @@ __global__ __launch_bounds__(256) void k_rrj(const MatDesc* __restrict__ md,
     const BlockRec br = blk[off + iblock[gidx] - 1];
     const int nbl = br.bend - br.bbeg + 1;
     double w = W[gidx], we = Werr[gidx];
-    int sw = dlarrj1(nbl, Dcopy + off + br.bbeg, E2copy + off + br.bbeg, Windex[gidx],
-                     4 * DBL_EPSILON, &w, &we, S.pivmin, S.spdiam);
+    int sw = dlarrj1_w(nbl, Dcopy + off + br.bbeg, E2copy + off + br.bbeg, Windex[gidx],
+                       4 * DBL_EPSILON, &w, &we, S.pivmin, S.spdiam);
     add_steps(kStepRrj, (unsigned long long)sw * nbl);
     W[gidx] = w;
     Werr[gidx] = we;
@@ -1236,37 +1236,36 @@ __device__ int dlarrf_lane(int n, const double* __restrict__ d, const double* __
     const double growthbound = kMaxGrowth1 * spdiam;
     double max1 = 0.0, max2 = 0.0;
     int sweeps = 0;
+    // L+ D+ L+^T = L D L^T - sig I (odrrf.f:206-225 and 231-249): growth and pivot
+    // floor hits; the factors are stored only when store is set (the refined growth
+    // test below reads them; the chosen factorisation is written by the caller)
+    auto trial = [&](double sig, double* Dst, double* Lst, bool store, double& mx, int& nan) {
+        double s = -sig;
+        double dp = d[0] + s;
+        nan = 0;
+        if (fabs(dp) < pivmin) { dp = -pivmin; nan = 1; }
+        mx = fabs(dp);
+        if (store) Dst[0] = dp;
+        for (int i = 0; i < n - 1; ++i) {
+            const double lp = ld[i] / dp;
+            if (store) Lst[i] = lp;
+            s = s * lp * l[i] - sig;
+            dp = d[i + 1] + s;
+            if (fabs(dp) < pivmin) { dp = -pivmin; nan = 1; }
+            mx = fmax(mx, fabs(dp));
+            if (store) Dst[i + 1] = dp;
+        }
+        nan = nan || isnan(mx);
+    };
     for (;;) {
-        ++sweeps;
         int sawnan1 = 0, sawnan2 = 0;
         ldelta = fmin(ldmax, ldelta);
         rdelta = fmin(rdmax, rdelta);
-        double s = -lsigma;
-        dplus[0] = d[0] + s;
-        if (fabs(dplus[0]) < pivmin) { dplus[0] = -pivmin; sawnan1 = 1; }
-        max1 = fabs(dplus[0]);
-        for (int i = 0; i < n - 1; ++i) {
-            lplus[i] = ld[i] / dplus[i];
-            s = s * lplus[i] * l[i] - lsigma;
-            dplus[i + 1] = d[i + 1] + s;
-            if (fabs(dplus[i + 1]) < pivmin) { dplus[i + 1] = -pivmin; sawnan1 = 1; }
-            max1 = fmax(max1, fabs(dplus[i + 1]));
-        }
-        sawnan1 = sawnan1 || isnan(max1);
+        ++sweeps;
+        trial(lsigma, dplus, lplus, false, max1, sawnan1);
         if (forcer || (max1 <= growthbound && !sawnan1)) { *sigma = lsigma; shift = 1; break; }
         ++sweeps;
-        s = -rsigma;
-        work[0] = d[0] + s;
-        if (fabs(work[0]) < pivmin) { work[0] = -pivmin; sawnan2 = 1; }
-        max2 = fabs(work[0]);
-        for (int i = 0; i < n - 1; ++i) {
-            work[n + i] = ld[i] / work[i];
-            s = s * work[n + i] * l[i] - rsigma;
-            work[i + 1] = d[i + 1] + s;
-            if (fabs(work[i + 1]) < pivmin) { work[i + 1] = -pivmin; sawnan2 = 1; }
-            max2 = fmax(max2, fabs(work[i + 1]));
-        }
-        sawnan2 = sawnan2 || isnan(max2);
+        trial(rsigma, work, work + n, false, max2, sawnan2);
         if (forcer || (max2 <= growthbound && !sawnan2)) { *sigma = rsigma; shift = 2; break; }
         if (!(sawnan1 && sawnan2)) {
             if (!sawnan1) {
@@ -1279,6 +1278,12 @@ __device__ int dlarrf_lane(int n, const double* __restrict__ d, const double* __
             }
             int dorrr1 = (clwdth < mingap / 128.0) && (fmin(max1, max2) < fail2) && !sawnan1 && !sawnan2;
             if (dorrr1) {
+                // the refined test reads both factorisations (odrrf.f:265-305)
+                double m1, m2;
+                int n1, n2;
+                trial(lsigma, dplus, lplus, true, m1, n1);
+                trial(rsigma, work, work + n, true, m2, n2);
+                sweeps += 2;
                 if (indx == 1) {
                     double tmp = fabs(dplus[n - 1]), znm2 = 1.0, prod = 1.0, oldp = 1.0;
                     for (int i = n - 2; i >= 0; --i) {
@@ -1338,8 +1343,8 @@ __global__ __launch_bounds__(64) void k_endref(const ClTask* __restrict__ C, int
     const int slot = (e & 1) ? t.ce : t.cb;
     const MatState& S = ms[t.mat];
     double w = Wsh[slot], we = Werr[slot];
-    int sw = dlarrb1(t.bsize, t.pD, t.pDLL, Windex[slot], S.rqtol, S.rqtol, 0.0, &w, &we,
-                     S.pivmin, t.bl_spdiam, t.bsize);
+    int sw = dlarrb1_w(t.bsize, t.pD, t.pDLL, Windex[slot], S.rqtol, S.rqtol, 0.0, &w, &we,
+                       S.pivmin, t.bl_spdiam, t.bsize);
     add_steps(kStepEndref, (unsigned long long)sw * t.bsize);
     Wtmp[slot] = w;
     Werrtmp[slot] = we;
@@ -1352,26 +1357,79 @@ __global__ __launch_bounds__(64) void k_dlarrf(ClTask* C, int nC, double* Wsh, d
                                                double* Wgap, const double* __restrict__ Wtmp,
                                                const double* __restrict__ Werrtmp,
                                                const MatState* __restrict__ ms) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= nC) return;
-    ClTask& t = C[c];
-    const int cb = t.cb, ce = t.ce, clsize = ce - cb + 1;
-    // Wgap[cb] from the refined left end and the neighbour as it was before the right
-    // end was refined (process_cluster.c:186-189), then commit both ends
-    Wgap[cb] = fmax(0.0, (Wsh[cb + 1] - Werr[cb + 1]) - (Wtmp[cb] + Werrtmp[cb]));
-    Wsh[cb] = Wtmp[cb];
-    Werr[cb] = Werrtmp[cb];
-    Wsh[ce] = Wtmp[ce];
-    Werr[ce] = Werrtmp[ce];
-    double tau;
-    int sh = dlarrf_lane(t.bsize, t.pD, t.pL, t.pDL, Wsh + cb, Wgap + cb, Werr + cb, clsize,
-                         t.bl_spdiam, t.lgap, Wgap[ce], ms[t.mat].pivmin, &tau, t.cD, t.cL,
-                         t.cDL);
-    t.tau = tau;
-    t.shift = sh;
+    __shared__ double bD[16][65], bL[16][65];
+    __shared__ double* dD[64];
+    __shared__ double* dL[64];
+    __shared__ int nn[64];
+    const int lane = threadIdx.x;
+    const int c = blockIdx.x * blockDim.x + lane;
+    const bool valid = c < nC;
+    double tau = 0.0, pivmin = 0.0;
+    int n = 0;
+    const double *pd = nullptr, *pl = nullptr, *pld = nullptr;
+    if (valid) {
+        ClTask& t = C[c];
+        const int cb = t.cb, ce = t.ce, clsize = ce - cb + 1;
+        // Wgap[cb] from the refined left end and the neighbour as it was before the
+        // right end was refined (process_cluster.c:186-189), then commit both ends
+        Wgap[cb] = fmax(0.0, (Wsh[cb + 1] - Werr[cb + 1]) - (Wtmp[cb] + Werrtmp[cb]));
+        Wsh[cb] = Wtmp[cb];
+        Werr[cb] = Werrtmp[cb];
+        Wsh[ce] = Wtmp[ce];
+        Werr[ce] = Werrtmp[ce];
+        pivmin = ms[t.mat].pivmin;
+        int sh = dlarrf_lane(t.bsize, t.pD, t.pL, t.pDL, Wsh + cb, Wgap + cb, Werr + cb, clsize,
+                             t.bl_spdiam, t.lgap, Wgap[ce], pivmin, &tau, t.cD, t.cL, t.cDL);
+        t.tau = tau;
+        t.shift = sh;
+        n = t.bsize;
+        pd = t.pD; pl = t.pL; pld = t.pDL;
+        dD[lane] = t.cD;
+        dL[lane] = t.cL;
+    }
+    nn[lane] = n;
+    // The chosen factorisation L+ D+ L+^T = L D L^T - tau I, recomputed with the
+    // operations of the accepted trial, into cD (D+) and cL (L+).  Each lane writes
+    // chunks of 16 values to LDS; the wave then stores 16 contiguous rows of four
+    // clusters per instruction, so no cache line is left partly written.
+    int nmax = n;
+    for (int o = 32; o > 0; o >>= 1) nmax = max(nmax, __shfl_xor(nmax, o));
+    double s = -tau, dp = 0.0;
+    if (valid) {
+        dp = pd[0] + s;
+        if (fabs(dp) < pivmin) dp = -pivmin;
+    }
+    const int sub = lane & 15, cq = lane >> 4;
+    for (int i0 = 0; i0 < nmax; i0 += 16) {
+#pragma unroll 4
+        for (int k = 0; k < 16; ++k) {
+            const int i = i0 + k;
+            if (valid && i < n) {
+                bD[k][lane] = dp;
+                if (i < n - 1) {
+                    const double lp = pld[i] / dp;
+                    bL[k][lane] = lp;
+                    s = s * lp * pl[i] - tau;
+                    dp = pd[i + 1] + s;
+                    if (fabs(dp) < pivmin) dp = -pivmin;
+                }
+            }
+        }
+        __syncthreads();
+        for (int cg = 0; cg < 16; ++cg) {
+            const int col = cg * 4 + cq;
+            const int row = i0 + sub;
+            const int nc = nn[col];
+            if (row < nc) {
+                dD[col][row] = bD[sub][col];
+                if (row < nc - 1) dL[col][row] = bL[sub][col];
+            }
+        }
+        __syncthreads();
+    }
 }
 
-// Keeps the chosen factorisation, accumulates the shift, forms DL and DLL and moves
+// Accumulates the shift, forms DL and DLL from the chosen factorisation and moves
 // the cluster's eigenvalues to the new representation (process_cluster.c:206-232).
 __global__ __launch_bounds__(64) void k_newrrr_fin(const ClTask* __restrict__ C, int nC,
                                                    double* Wsh, double* Werr) {
@@ -1380,18 +1438,9 @@ __global__ __launch_bounds__(64) void k_newrrr_fin(const ClTask* __restrict__ C,
     const ClTask t = C[c];
     const int n = t.bsize;
     const int lane = threadIdx.x;
-    const bool right = (t.shift == 2);
     for (int i = lane; i < n; i += 64) {
-        double dv, lv;
-        if (right) {
-            dv = t.cDL[i];                               // WORK(i+1)
-            lv = (i < n - 1) ? t.cDL[n + i] : 0.0;       // WORK(N+i+1)
-            t.cD[i] = dv;
-            if (i < n - 1) t.cL[i] = lv;
-        } else {
-            dv = t.cD[i];
-            lv = (i < n - 1) ? t.cL[i] : 0.0;
-        }
+        const double dv = t.cD[i];
+        const double lv = (i < n - 1) ? t.cL[i] : 0.0;
         if (i < n - 1) {
             double tmp = dv * lv;
             t.cDL[i] = tmp;
@@ -1430,8 +1479,8 @@ __global__ __launch_bounds__(256) void k_refine_cl(const ClTask* __restrict__ C,
     double w = Wsh[slot], we = Werr[slot];
     double rgap = Wgap[slot];
     double lgap = (slot == t.cb) ? rgap : Wgap[slot - 1];
-    int sw = dlarrb1(t.bsize, t.cD, t.cDLL, Windex[slot], S.rtol1, S.rtol2, fmin(lgap, rgap),
-                     &w, &we, S.pivmin, t.bl_spdiam, t.bsize);
+    int sw = dlarrb1_w(t.bsize, t.cD, t.cDLL, Windex[slot], S.rtol1, S.rtol2, fmin(lgap, rgap),
+                       &w, &we, S.pivmin, t.bl_spdiam, t.bsize);
     add_steps(kStepRefineCl, (unsigned long long)sw * t.bsize);
     Wtmp[slot] = w;
     Werrtmp[slot] = we;

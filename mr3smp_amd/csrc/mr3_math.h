@@ -22,11 +22,30 @@ constexpr int kMaxExpand = 128;        // cap on bracket doublings (2^128 spans 
 // by every dependent step.
 constexpr int kU = 8;
 
+// Read-only device arrays that every active lane of a wave reads at the same index
+// can go through the constant address space: the compiler then loads them with
+// scalar loads into SGPRs (no per-lane address arithmetic, one fetch per wave) and
+// the FP64 operations read them from there.  The sweeps below are templates over the
+// pointer type P: `const double*` (per-lane arrays) or CPtr (wave-uniform arrays,
+// chosen at run time by the caller with wave_uniform()).
+typedef const __attribute__((address_space(4))) double* CPtr;
+
+__device__ __forceinline__ int rfl(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ const double* rfl(const double* p) {
+    const unsigned long long u = (unsigned long long)p;
+    const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(u & 0xffffffffu));
+    const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(u >> 32));
+    return (const double*)(((unsigned long long)hi << 32) | lo);
+}
+// True when every active lane holds the same value (wave-uniform result).
+__device__ __forceinline__ bool wave_uniform(int v) { return __all(v == rfl(v)); }
+__device__ __forceinline__ bool wave_uniform(const double* p) { return __all(p == rfl(p)); }
+__device__ __forceinline__ CPtr as_const(const double* p) { return (CPtr)rfl(p); }
+
 // Sturm count of T - x I, dlaebz IJOB=1 convention (odebz.f:265-277): a pivot with
 // |t| < pivmin becomes -pivmin, a pivot <= 0 counts.
-__device__ __forceinline__ int count_T_ijob1(int n, const double* __restrict__ d,
-                                             const double* __restrict__ e2, double x,
-                                             double pivmin) {
+template <class P>
+__device__ __forceinline__ int count_T_ijob1(int n, P d, P e2, double x, double pivmin) {
     double t = d[0] - x;
     if (fabs(t) < pivmin) t = -pivmin;
     int c = (t <= 0.0);
@@ -59,9 +78,8 @@ __device__ __forceinline__ int count_T_ijob1(int n, const double* __restrict__ d
 
 // Sturm count of T - x I, dlaebz IJOB=2/3 convention (odebz.f:395-415): a pivot
 // <= pivmin counts and is replaced by min(t, -pivmin).
-__device__ __forceinline__ int count_T_ijob2(int n, const double* __restrict__ d,
-                                             const double* __restrict__ e2, double x,
-                                             double pivmin) {
+template <class P>
+__device__ __forceinline__ int count_T_ijob2(int n, P d, P e2, double x, double pivmin) {
     double t = d[0] - x;
     int c = 0;
     if (t <= pivmin) { c = 1; t = fmin(t, -pivmin); }
@@ -96,8 +114,8 @@ __device__ __forceinline__ int count_T_ijob2(int n, const double* __restrict__ d
 // when the block ends in NaN.  A NaN in T can only come from the quotient and then
 // persists to the end of the block, and the re-run differs from the fast loop only
 // at quotients that are NaN; substituting per step therefore gives the same count.
-__device__ __forceinline__ int dlaneg(int n, const double* __restrict__ d,
-                                      const double* __restrict__ lld, double sigma, int r) {
+template <class P>
+__device__ __forceinline__ int dlaneg(int n, P d, P lld, double sigma, int r) {
     int neg = 0;
     double t = -sigma;
     const int top = r - 1;   // steps j = 0 .. r-2
@@ -166,10 +184,10 @@ __device__ __forceinline__ int dlaneg(int n, const double* __restrict__ d,
 // exact.  `gap` is min(left gap, right gap) as dlarrb forms it.  On return *w and
 // *werr hold the refined midpoint and half width, or are left unchanged when the
 // interval already met the tolerance (odrrb.f:198-205).
-__device__ __forceinline__ int dlarrb1(int n, const double* __restrict__ d,
-                                        const double* __restrict__ lld, int i, double rtol1,
-                                        double rtol2, double gap, double* w, double* werr,
-                                        double pivmin, double spdiam, int twist) {
+template <class P>
+__device__ __forceinline__ int dlarrb1(int n, P d, P lld, int i, double rtol1, double rtol2,
+                                        double gap, double* w, double* werr, double pivmin,
+                                        double spdiam, int twist) {
     const int maxitr = (int)((log(spdiam + pivmin) - log(pivmin)) / log(2.0)) + 2;
     const double mnwdth = 2.0 * pivmin;
     int r = twist;
@@ -209,8 +227,8 @@ __device__ __forceinline__ int dlarrb1(int n, const double* __restrict__ d,
 }
 
 // Sturm count on T for dlarrj (odrrj.f:160-166): no pivmin clamp, count if < 0.
-__device__ __forceinline__ int count_T_rrj(int n, const double* __restrict__ d,
-                                           const double* __restrict__ e2, double s) {
+template <class P>
+__device__ __forceinline__ int count_T_rrj(int n, P d, P e2, double s) {
     double dplus = d[0] - s;
     int cnt = (dplus < 0.0);
     if (n < 2) return cnt;
@@ -240,10 +258,9 @@ __device__ __forceinline__ int count_T_rrj(int n, const double* __restrict__ d,
 }
 
 // dlarrj (odrrj.f:100-281) for one interval (intervals are independent).
-__device__ __forceinline__ int dlarrj1(int n, const double* __restrict__ d,
-                                        const double* __restrict__ e2, int i, double rtol,
-                                        double* w, double* werr, double pivmin,
-                                        double spdiam) {
+template <class P>
+__device__ __forceinline__ int dlarrj1(int n, P d, P e2, int i, double rtol, double* w,
+                                        double* werr, double pivmin, double spdiam) {
     const int maxitr = (int)((log(spdiam + pivmin) - log(pivmin)) / log(2.0)) + 2;
     double left = *w - *werr, mid = *w, right = *w + *werr;
     double width = right - mid;
@@ -298,6 +315,34 @@ __host__ __device__ __forceinline__ double lcg_uniform_pm1(unsigned long long x)
     double it3 = (double)((x >> 12) & 4095ULL), it4 = (double)(x & 4095ULL);
     double u = R * (it1 + R * (it2 + R * (it3 + R * it4)));
     return 2.0 * u - 1.0;
+}
+
+// Dispatchers: the scalar-load form when every active lane of the wave sweeps the
+// same arrays with the same length and twist (the usual case: consecutive
+// eigenvalues of one block or cluster), the per-lane form otherwise.  Both forms run
+// the same operations, so results do not depend on the choice.
+__device__ __forceinline__ int count_T_ijob2_w(int n, const double* d, const double* e2,
+                                               double x, double pivmin) {
+    if (wave_uniform(d) && wave_uniform(e2) && wave_uniform(n))
+        return count_T_ijob2(rfl(n), as_const(d), as_const(e2), x, pivmin);
+    return count_T_ijob2(n, d, e2, x, pivmin);
+}
+
+__device__ __forceinline__ int dlarrb1_w(int n, const double* d, const double* lld, int i,
+                                         double rtol1, double rtol2, double gap, double* w,
+                                         double* werr, double pivmin, double spdiam, int twist) {
+    if (wave_uniform(d) && wave_uniform(lld) && wave_uniform(n) && wave_uniform(twist))
+        return dlarrb1(rfl(n), as_const(d), as_const(lld), i, rtol1, rtol2, gap, w, werr, pivmin,
+                       spdiam, rfl(twist));
+    return dlarrb1(n, d, lld, i, rtol1, rtol2, gap, w, werr, pivmin, spdiam, twist);
+}
+
+__device__ __forceinline__ int dlarrj1_w(int n, const double* d, const double* e2, int i,
+                                         double rtol, double* w, double* werr, double pivmin,
+                                         double spdiam) {
+    if (wave_uniform(d) && wave_uniform(e2) && wave_uniform(n))
+        return dlarrj1(rfl(n), as_const(d), as_const(e2), i, rtol, w, werr, pivmin, spdiam);
+    return dlarrj1(n, d, e2, i, rtol, w, werr, pivmin, spdiam);
 }
 
 }  // namespace mr3
