@@ -109,6 +109,13 @@ struct Alloc {
         ptrs.push_back(p);
         return (T*)p;
     }
+    // An array that the sweeps read: kSweepPad spare doubles on both sides, so that a
+    // sweep may read a few entries past either end of a block without an index clamp
+    // (the values read there are never used).
+    double* get_swept(size_t count) {
+        double* p = get<double>(count + 2 * (size_t)kSweepPad);
+        return p ? p + kSweepPad : nullptr;
+    }
     void release() {
         for (void* p : ptrs) (void)hipFreeAsync(p, st);
         ptrs.clear();
@@ -139,8 +146,6 @@ static size_t singleton_scratch_budget() {
     return std::min(b, cap);
 }
 
-constexpr int kMaxGen = 4;   // longest singleton list of a wave, in multiples of 64
-
 static int env_int(const char* name, int dflt) {
     const char* v = getenv(name);
     return v ? atoi(v) : dflt;
@@ -168,14 +173,14 @@ int run_pipeline(std::vector<MatDesc>& md, std::vector<MatState>& ms, hipStream_
     ce.on = g_prof.on.load() != 0;
     MatDesc* dmd = A.get<MatDesc>(nmat);
     MatState* dms = A.get<MatState>(nmat);
-    double* D = A.get<double>(N);
-    double* E = A.get<double>(N);
-    double* E2 = A.get<double>(N);
+    double* D = A.get_swept(N);
+    double* E = A.get_swept(N);
+    double* E2 = A.get_swept(N);
     double* gers = A.get<double>(2 * (size_t)N);
-    double* Dcopy = A.get<double>(N);
-    double* E2copy = A.get<double>(N);
-    double* DLr = A.get<double>(N);
-    double* DLLr = A.get<double>(N);
+    double* Dcopy = A.get_swept(N);
+    double* E2copy = A.get_swept(N);
+    double* DLr = A.get_swept(N);
+    double* DLLr = A.get_swept(N);
     double* W = A.get<double>(N);
     double* Werr = A.get<double>(N);
     double* Wgap = A.get<double>(N);
@@ -241,8 +246,17 @@ int run_pipeline(std::vector<MatDesc>& md, std::vector<MatState>& ms, hipStream_
         int* bcnt = A.get<int>(N);
         const int capT = N / 64 + nbl + 1;
         WaveTask* wtasks = A.get<WaveTask>(capT);
+        // singleton iterations: per-singleton state, two generations of compacted
+        // lists (S entries per block), their tasks and counters, bisection list
+        SState* ST = A.get<SState>(capS);
+        WaveTask* tit[2] = {A.get<WaveTask>(capT), A.get<WaveTask>(capT)};
+        int* sit[2] = {A.get<int>(N), A.get<int>(N)};
+        LevelCounters* cit[2] = {A.get<LevelCounters>(1), A.get<LevelCounters>(1)};
+        int* sbc = A.get<int>(N);
+        int* BI = A.get<int>(N);
         if (!A.ok) return 2;
         MR3_CHECK(hipMemsetAsync(bcnt, 0, sizeof(int) * (size_t)N, st));
+        MR3_CHECK(hipMemsetAsync(sbc, 0, sizeof(int) * (size_t)N, st));
         // eigenvector scratch: two checkpoint arrays of maxbl/kChunk + 2 carries per
         // lane, 64 lanes per wave
         const size_t ck_per_lane = (size_t)2 * (maxbl / kChunk + 2);
@@ -270,48 +284,61 @@ int run_pipeline(std::vector<MatDesc>& md, std::vector<MatState>& ms, hipStream_
         int cur = 0;
         const bool trace = getenv("MR3_TRACE_LEVELS") != nullptr;   // per-level timing to stderr
         unsigned long long prev_ticks = 0, prev_tasks = 0;
-        unsigned long long* tlog = nullptr;
-        struct LogReset {   // the device pointer never outlives this solve's buffer
-            bool armed = false;
-            ~LogReset() {
-                if (!armed) return;
-                unsigned long long* z = nullptr;
-                (void)hipDeviceSynchronize();
-                (void)hipMemcpyToSymbol(HIP_SYMBOL(g_tasklog), &z, sizeof(z));
-            }
-        } log_reset;
-        if (trace && getenv("MR3_TASKLOG")) {
-            log_reset.armed = true;
-            tlog = A.get<unsigned long long>((size_t)4 * capT);
-            if (!A.ok) return 2;
-            MR3_CHECK(hipMemsetAsync(tlog, 0, (size_t)32 * capT, st));
-            MR3_CHECK(hipMemcpyToSymbolAsync(HIP_SYMBOL(g_tasklog), &tlog, sizeof(tlog), 0,
-                                             hipMemcpyHostToDevice, st));
-            MR3_CHECK(hipMemcpyToSymbolAsync(HIP_SYMBOL(g_tasklog_cap), &capT, sizeof(int), 0,
-                                             hipMemcpyHostToDevice, st));
-        }
         for (int level = 0;; ++level) {
             const int nS = hc.nS, nC = hc.nC, nE = hc.nEig;
             const long long arena_need = hc.arena;
             if (nS > 0) {
-                // wave tasks: lists of up to tsize singletons of a block, longest blocks
-                // first; a wave refills its 64 lanes from its list, so lists longer than 64
-                // keep lanes busy past a slow one, as long as there are waves enough to
-                // fill the device
-                const int gen = std::max(1, std::min(kMaxGen, nS / (64 * 2 * 2048)));
-                const int tsize = 64 * gen;
-                LAUNCH(k_plan_count, dim3(cdiv(nbl, 64)), dim3(64), dbl, nbl, blk, bcnt, tsize, dcnt);
-                LAUNCH(k_plan_s, dim3(cdiv(nbl, 64)), dim3(64), dbl, nbl, smat, dmd, blk, bcnt,
-                       tsize, wtasks, capT, dcnt);
-                int g = std::min(nwaves, nS / tsize + nbl + 1);
                 hipEvent_t t0 = nullptr, t1 = nullptr;
                 if (trace) {
                     (void)hipEventCreate(&t0);
                     (void)hipEventCreate(&t1);
                     (void)hipEventRecord(t0, st);
                 }
-                LAUNCH(k_singleton, dim3(g), dim3(64), wtasks, dcnt, Sl, dmd, dms, Windex, Zidx, W,
-                       Wsh, Werr, Wgap, scratch, maxbl);
+                // first iteration over the level's singleton lists (64 per task,
+                // longest blocks first)
+                LAUNCH(k_plan_count, dim3(cdiv(nbl, 64)), dim3(64), dbl, nbl, blk, bcnt, 64, dcnt);
+                LAUNCH(k_plan_s, dim3(cdiv(nbl, 64)), dim3(64), dbl, nbl, smat, dmd, blk, bcnt,
+                       64, wtasks, capT, dcnt);
+                const int g1 = nS / 64 + nbl + 1;   // bound on the first iteration's tasks
+                LAUNCH(k_s_iter, dim3(std::min(nwaves, g1)), dim3(64), wtasks, dcnt, 1,
+                       (const int*)nullptr, Sl, dmd, dms, Windex, Zidx, W, Wsh, Werr, Wgap, ST,
+                       scratch, maxbl);
+                // further iterations on the compacted singletons that go on
+                const WaveTask* ptasks = wtasks;
+                const LevelCounters* pcnt = dcnt;
+                const int* psi = nullptr;
+                int pfirst = 1, pgrid = g1, iters = 1;
+                for (int it = 1; it <= kMaxIterRQI + 1; ++it) {
+                    const int b = it & 1;
+                    MR3_CHECK(hipMemsetAsync(cit[b], 0, sizeof(LevelCounters), st));
+                    LAUNCH(k_s_collect, dim3(pgrid), dim3(64), ptasks, pcnt, pfirst, psi, Sl, ST,
+                           sit[b], sbc, BI, cit[b]);
+                    LAUNCH(k_plan_count, dim3(cdiv(nbl, 64)), dim3(64), dbl, nbl, blk, sbc, 64,
+                           cit[b]);
+                    LAUNCH(k_plan_s, dim3(cdiv(nbl, 64)), dim3(64), dbl, nbl, smat, dmd, blk, sbc,
+                           64, tit[b], capT, cit[b]);
+                    LevelCounters hi;
+                    MR3_CHECK(hipMemcpyAsync(&hi, cit[b], sizeof(hi), hipMemcpyDeviceToHost, st));
+                    MR3_CHECK(hipStreamSynchronize(st));
+                    if (hi.nS == 0) break;
+                    if (hi.nEig > 0) {
+                        // lanes per bisection: as many as keep about 128K lanes busy
+                        int G = 64;
+                        while (G > 2 && (long long)hi.nEig * G > 131072) G >>= 1;
+                        LAUNCH(k_s_bisect, dim3(cdiv(hi.nEig, 64 / G)), dim3(64), BI, cit[b], G,
+                               Sl, dms, Windex, ST);
+                    }
+                    LAUNCH(k_s_iter, dim3(std::min(nwaves, hi.nT)), dim3(64), tit[b], cit[b], 0,
+                           sit[b], Sl, dmd, dms, Windex, Zidx, W, Wsh, Werr, Wgap, ST, scratch,
+                           maxbl);
+                    ptasks = tit[b];
+                    pcnt = cit[b];
+                    psi = sit[b];
+                    pfirst = 0;
+                    pgrid = hi.nT;
+                    ++iters;
+                }
+                LAUNCH(k_s_normalize, dim3(g1), dim3(64), wtasks, dcnt, Sl, dmd, Zidx, ST);
                 MR3_CHECK(hipGetLastError());
                 if (trace) {
                     float ms = 0.f;
@@ -321,32 +348,19 @@ int run_pipeline(std::vector<MatDesc>& md, std::vector<MatState>& ms, hipStream_
                     unsigned long long stv[kStepKinds] = {};
                     (void)hipMemcpyFromSymbol(stv, HIP_SYMBOL(g_steps), sizeof(stv));
                     fprintf(stderr,
-                            "mr3 level %d: singletons %d clusters %d waves %d  %.3f ms  "
+                            "mr3 level %d: singletons %d clusters %d iterations %d  %.3f ms  "
                             "tasks %llu  task time %.3f wave-ms\n",
-                            level, nS, nC, g, ms, stv[kStepTasks] - prev_tasks,
+                            level, nS, nC, iters, ms, stv[kStepTasks] - prev_tasks,
                             (stv[kStepTaskTicks] - prev_ticks) * 1e-5);
                     prev_ticks = stv[kStepTaskTicks];
                     prev_tasks = stv[kStepTasks];
                     (void)hipEventDestroy(t0);
                     (void)hipEventDestroy(t1);
-                    if (tlog) {   // per-task log of this level to MR3_TASKLOG.<level>
-                        std::vector<unsigned long long> h((size_t)4 * capT);
-                        (void)hipMemcpy(h.data(), tlog, h.size() * 8, hipMemcpyDeviceToHost);
-                        char fn[512];
-                        snprintf(fn, sizeof(fn), "%s.%d", getenv("MR3_TASKLOG"), level);
-                        FILE* f = fopen(fn, "wb");
-                        if (f) {
-                            fwrite(&capT, sizeof(int), 1, f);
-                            fwrite(h.data(), 8, h.size(), f);
-                            fclose(f);
-                        }
-                        (void)hipMemset(tlog, 0, h.size() * 8);
-                    }
                 }
             }
             if (nC == 0) break;
             if (level > 64) return 2;
-            double* arena = A.get<double>((size_t)arena_need);
+            double* arena = A.get_swept((size_t)arena_need);
             if (!A.ok) return 2;
             MR3_CHECK(hipMemsetAsync(bump, 0, sizeof(unsigned long long), st));
             LAUNCH(k_arena_assign, dim3(cdiv(nC, 256)), dim3(256), Cl[cur], nC,

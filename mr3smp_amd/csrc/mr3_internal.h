@@ -22,6 +22,9 @@ constexpr double kRandFactor = 8.0;
 constexpr int kMaxIterRQI = 10;
 // steps per chunk of the eigenvector sweeps: checkpoint spacing and LDS buffer rows
 constexpr int kChunk = 16;
+// spare doubles on both sides of every array a sweep reads (>= the furthest a sweep
+// reads past a block end: two prefetch groups plus a chunk)
+constexpr int kSweepPad = 64;
 constexpr double kMaxGrowth1 = 8.0;
 constexpr double kMaxGrowth2 = 8.0;
 
@@ -98,6 +101,14 @@ struct WaveTask {
 };
 
 // One eigenvector to compute (process_singleton.c singleton_t with begin == end).
+// Rayleigh-quotient state of one singleton between the iteration launches (the loop
+// variables of process_singleton.c:150-300).  flags: usedBS | usedRQ << 1 |
+// needBS << 2 | wantnc << 3 | state << 4 (active, best-residual step, done).
+struct SState {
+    double lam, lamv, werr, left, right, wsh, bstres, bstw, nrminv, resid;
+    int kk, r, flags, is0, is1, pad;
+};
+
 struct STask {
     int slot, mat;
     int bbeg, bsize;

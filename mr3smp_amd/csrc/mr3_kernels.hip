@@ -12,7 +12,8 @@
 //                                                    -> k_classify_root / k_classify
 //   process_cluster.c:105-235 compute_new_rrr / dlarrf -> k_newrrr (one wave per cluster)
 //   process_cluster.c:366-432 cluster refinement       -> k_refine_cl + k_gaps_cl
-//   process_singleton.c:65-324 S-task / dlar1v         -> k_singleton (one lane per vector)
+//   process_singleton.c:65-324 S-task / dlar1v         -> k_s_iter, k_s_bisect,
+//                                                        k_s_collect, k_s_normalize
 //   mrrr.c:263-288 shift back, dlarrj, unscale, sort   -> k_shift_n, k_rrj, k_finalize
 #include <hip/hip_runtime.h>
 
@@ -43,9 +44,6 @@ enum StepKind : int {
 };
 __device__ unsigned long long g_steps[kStepKinds];
 __constant__ int g_count_steps;
-// Optional per-task log of the eigenvector kernel (diagnostics): 4 words per task.
-__device__ unsigned long long* g_tasklog;
-__device__ int g_tasklog_cap;
 
 __device__ __forceinline__ void add_steps(int kind, unsigned long long v) {
     if (g_count_steps && v) atomicAdd(&g_steps[kind], v);
@@ -1605,7 +1603,7 @@ __device__ void pass_stat(const Rrr& R, double lambda, double pivmin, int r1, in
         auto load = [&](double(&X)[kUS][3], int t0) {
 #pragma unroll
             for (int u = 0; u < kUS; ++u) {
-                const int j = min(max(t0 + u, 1), stopU - 1) - 1;
+                const int j = t0 + u - 1;
                 X[u][0] = R.D[j]; X[u][1] = R.LD[j]; X[u][2] = R.L[j];
             }
         };
@@ -1648,7 +1646,7 @@ __device__ __forceinline__ void sweep_up(int lo, int hi, const double* const (&P
 #pragma unroll
         for (int u = 0; u < kUS; ++u)
 #pragma unroll
-            for (int a = 0; a < NA; ++a) X[u][a] = P[a][min(t0 + u, hi) - 1];
+            for (int a = 0; a < NA; ++a) X[u][a] = P[a][t0 + u - 1];
     };
     auto run = [&](const double(&X)[kUS][NA], int t0) {
 #pragma unroll
@@ -1674,7 +1672,7 @@ __device__ __forceinline__ void sweep_down(int hi, int lo, const double* const (
 #pragma unroll
         for (int u = 0; u < kUS; ++u)
 #pragma unroll
-            for (int a = 0; a < NA; ++a) X[u][a] = P[a][max(t0 - u, lo) - 1];
+            for (int a = 0; a < NA; ++a) X[u][a] = P[a][t0 - u - 1];
     };
     auto run = [&](const double(&X)[kUS][NA], int t0) {
 #pragma unroll
@@ -1846,10 +1844,11 @@ __device__ __forceinline__ void write_chunk(const ChunkBuf& buf, const ColOut& c
 // stationary checkpoints, downwards with U- recomputed from the progressive ones, and
 // stops at its support boundary, so a wave costs its longest support rather than the
 // union of its lanes' ranges.  The first loads and the checkpoint of a lane's next
-// chunk are issued while the current chunk is computed.  Lanes with stg set recompute
-// the vector of their last dlar1v and write z * scale into their columns (co), zeros
-// outside the support and outside the block; the other active lanes form ZTZ, the
-// residual and the support.
+// chunk are issued while the current chunk is computed.  Every active lane forms ZTZ,
+// the residual and the support; lanes with stg set also write z * scale into their
+// columns (co) for the rows their chunks cover, as the reference's dlar1v writes Z
+// on every call (rows outside the final support are cleared when the vector is
+// normalised).
 __device__ __forceinline__ void pass_solve(int n, const Rrr& R, double lambda, double pivmin,
                                            double gaptol, const Ck& ck, ChunkBuf& buf,
                                            ChunkBuf& bld, int lane, R1v* o, bool act, bool stg,
@@ -1870,7 +1869,7 @@ __device__ __forceinline__ void pass_solve(int n, const Rrr& R, double lambda, d
         double pck = 0.0;
 #pragma unroll
         for (int u = 0; u < kUS; ++u) {
-            const int j = min(max(c * KC + u, 1), n - 1);
+            const int j = c * KC + u;
             pre[u][0] = R.D[j - 1]; pre[u][1] = R.LD[j - 1]; pre[u][2] = R.L[j - 1];
         }
         if (live && c >= 1) pck = ck.S(c);
@@ -1888,7 +1887,7 @@ __device__ __forceinline__ void pass_solve(int n, const Rrr& R, double lambda, d
             const int cn = max(c - 1, 0);
 #pragma unroll
             for (int u = 0; u < kUS; ++u) {
-                const int j = min(max(cn * KC + u, 1), n - 1);
+                const int j = cn * KC + u;
                 pre[u][0] = R.D[j - 1]; pre[u][1] = R.LD[j - 1]; pre[u][2] = R.L[j - 1];
             }
             pck = (mine && c - 1 >= 1) ? ck.S(c - 1) : 0.0;
@@ -1898,7 +1897,7 @@ __device__ __forceinline__ void pass_solve(int n, const Rrr& R, double lambda, d
                 if (g + 1 < KC / kUS) {
 #pragma unroll
                     for (int u = 0; u < kUS; ++u) {
-                        const int j = min(max(base + (g + 1) * kUS + u, 1), n - 1);
+                        const int j = base + (g + 1) * kUS + u;
                         nxt[u][0] = R.D[j - 1]; nxt[u][1] = R.LD[j - 1]; nxt[u][2] = R.L[j - 1];
                     }
                 }
@@ -1961,7 +1960,7 @@ __device__ __forceinline__ void pass_solve(int n, const Rrr& R, double lambda, d
         double pcp = 0.0;
 #pragma unroll
         for (int u = 0; u < kUS; ++u) {
-            const int t = min(c * KC + KC - u, n - 1);
+            const int t = c * KC + KC - u;
             pre[u][0] = R.D[t - 1]; pre[u][1] = R.LLD[t - 1]; pre[u][2] = R.L[t - 1];
             pre[u][3] = R.LD[t - 1];
         }
@@ -1979,7 +1978,7 @@ __device__ __forceinline__ void pass_solve(int n, const Rrr& R, double lambda, d
             const int cn = min(c + 1, cEnd);
 #pragma unroll
             for (int u = 0; u < kUS; ++u) {
-                const int t = min(cn * KC + KC - u, n - 1);
+                const int t = cn * KC + KC - u;
                 pre[u][0] = R.D[t - 1]; pre[u][1] = R.LLD[t - 1]; pre[u][2] = R.L[t - 1];
                 pre[u][3] = R.LD[t - 1];
             }
@@ -1990,7 +1989,7 @@ __device__ __forceinline__ void pass_solve(int n, const Rrr& R, double lambda, d
                 if (g + 1 < KC / kUS) {
 #pragma unroll
                     for (int u = 0; u < kUS; ++u) {
-                        const int t = max(min(tlo + KC - 1 - (g + 1) * kUS - u, n - 1), 1);
+                        const int t = tlo + KC - 1 - (g + 1) * kUS - u;
                         nxt[u][0] = R.D[t - 1]; nxt[u][1] = R.LLD[t - 1]; nxt[u][2] = R.L[t - 1];
                         nxt[u][3] = R.LD[t - 1];
                     }
@@ -2044,23 +2043,10 @@ __device__ __forceinline__ void pass_solve(int n, const Rrr& R, double lambda, d
             c = cn;
         }
     }
-    if (stage) {
-        // Z(r) = 1, zeros above and below the rows written by the chunks
-        const bool w = act && stg;
-        if (w) co.Zc[co.bbeg + r - 1] = scale;
-        // rows [0, a) and [b, nfull) are zero; other columns are left alone
-        co.wlo[lane] = w ? co.bbeg + min(upper_lo, r) - 1 : 0;
-        co.whi[lane] = w ? co.bbeg + max(lower_hi, r) : co.nfull;
-        __syncthreads();
-        for (int c = 0; c < co.cntw; ++c) {
-            double* zc = co.zcol[c];
-            const int a = co.wlo[c], b = co.whi[c];
-            for (int rr = lane; rr < a; rr += SW) zc[rr] = 0.0;
-            for (int rr = b + lane; rr < co.nfull; rr += SW) zc[rr] = 0.0;
-        }
-        __syncthreads();
-    }
-    if (act && !stg) {
+    if (act && stg) co.Zc[co.bbeg + r - 1] = scale;   // Z(r) = 1
+    (void)upper_lo;
+    (void)lower_hi;
+    if (act) {
         const double tmp = 1.0 / ztz;
         o->ztz = ztz;
         o->nrminv = sqrt(tmp);
@@ -2071,22 +2057,27 @@ __device__ __forceinline__ void pass_solve(int n, const Rrr& R, double lambda, d
     }
 }
 
-// k_singleton: eigenvectors by twisted factorisations with Rayleigh-quotient
-// correction (PMR_process_s_task, process_singleton.c:65-324), one lane per
-// eigenvector.  A wave takes a task: a list of up to kSGen * 64 singletons of one
-// block, so the block size and rows are wave-uniform.  The lanes work through the
-// list: a lane whose iteration has converged recomputes and writes its vector in the
-// next solve pass and then takes the next singleton of the list, so a slow lane no
-// longer holds 63 idle ones.  Per lane the iteration is that of the reference
-// (odrrv.f / process_singleton.c): RQI with bisection fallback, at most kMaxIterRQI
-// dlar1v calls, the best-residual step after a bisection.
-enum : int { kLaneEmpty = 0, kLaneActive = 1, kLaneStep2 = 2, kLaneStage = 3 };
+// ---------------------------------------------------------------------------------
+// Eigenvectors of singletons (PMR_process_s_task, process_singleton.c:65-324): one
+// lane per eigenvector, run as bulk iterations.  Each launch of k_s_iter performs
+// one Rayleigh-quotient iteration (dlar1v to the twist, the vector solve and the
+// update of odrrv.f / process_singleton.c) for every singleton still iterating,
+// packed 64 to a wave by block; k_s_collect compacts the ones that go on, and the
+// bisection fallbacks run in their own packed launch (k_s_bisect), so a lane that
+// needs many iterations or a bisection no longer holds a wave of finished ones.
+// Every solve writes its unscaled vector into the column, as the reference's dlar1v
+// writes Z on every call; k_s_normalize scales the final support by 1/||z|| and
+// clears the rest of the column.  The per-singleton state between launches is an
+// SState record.
+// ---------------------------------------------------------------------------------
+enum : int { kSActive = 0, kSStep2 = 1, kSDone = 2 };
+__device__ __forceinline__ int sflag_state(int f) { return (f >> 4) & 3; }
 
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_singleton(
-    const WaveTask* __restrict__ tasks, LevelCounters* cnt, const STask* __restrict__ T,
-    const MatDesc* __restrict__ md, const MatState* __restrict__ ms,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_s_iter(
+    const WaveTask* __restrict__ tasks, LevelCounters* cnt, int first, const int* __restrict__ SI,
+    const STask* __restrict__ T, const MatDesc* __restrict__ md, const MatState* __restrict__ ms,
     const int* __restrict__ Windex, const int* __restrict__ Zidx, double* W, double* Wsh,
-    double* Werr, const double* __restrict__ Wgap, double* scratch, int cap) {
+    double* Werr, const double* __restrict__ Wgap, SState* ST, double* scratch, int cap) {
     __shared__ ChunkBuf buf, bld;
     __shared__ double* zcol[SW];
     __shared__ int wr0[SW], wlo[SW], whi[SW];
@@ -2112,191 +2103,342 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
         const WaveTask wt = tasks[ti];
         const int cntw = __builtin_amdgcn_readfirstlane(wt.cnt);
         const int start = __builtin_amdgcn_readfirstlane(wt.start);
-        const STask t0 = T[start];
-        const int n = __builtin_amdgcn_readfirstlane(t0.bsize);
-        const int bbeg = __builtin_amdgcn_readfirstlane(t0.bbeg);
-        const int nfull = __builtin_amdgcn_readfirstlane(md[t0.mat].n);
+        const bool valid = lane < cntw;
+        // lanes past the end replay the last entry so that every pointer is valid
+        const int pos = start + min(lane, cntw - 1);
+        const int idx = first ? pos : SI[pos];
+        const STask t = T[idx];
+        const int n = __builtin_amdgcn_readfirstlane(t.bsize);
+        const int bbeg = __builtin_amdgcn_readfirstlane(t.bbeg);
+        const MatDesc& dsc = md[t.mat];
+        int col = Zidx[t.slot];
+        if (col >= dsc.mcols) col = dsc.mcols - 1;   // never write past the caller's Z
         ColOut co;
         co.zcol = zcol;
         co.wr0 = wr0;
         co.wlo = wlo;
         co.whi = whi;
+        co.Zc = dsc.Z + (size_t)col * dsc.ldz;
         co.bbeg = bbeg;
-        co.nfull = nfull;
-        co.cntw = SW;
-        unsigned diag = 0;
+        co.nfull = __builtin_amdgcn_readfirstlane(dsc.n);
+        co.cntw = cntw;
+        zcol[lane] = co.Zc;
         if (n == 1) {
-            // 1x1 blocks: e_bbeg
-            for (int b0 = 0; b0 < cntw; b0 += SW) {
-                const int j = b0 + lane;
-                const STask t = T[start + min(j, cntw - 1)];
-                const MatDesc& dsc = md[t.mat];
-                int col = Zidx[t.slot];
-                if (col >= dsc.mcols) col = dsc.mcols - 1;   // never write past the caller's Z
-                zcol[lane] = dsc.Z + (size_t)col * dsc.ldz;
-                if (j < cntw) {
-                    dsc.Zsupp[2 * col] = 1 + bbeg;
-                    dsc.Zsupp[2 * col + 1] = 1 + bbeg;
-                }
-                __syncthreads();
-                for (int c = 0; c < min(SW, cntw - b0); ++c) {
-                    double* zc = zcol[c];
-                    for (int rr = lane; rr < nfull; rr += SW) zc[rr] = rr == bbeg ? 1.0 : 0.0;
-                }
-                __syncthreads();
+            // a 1x1 block: e_bbeg, finished at once
+            if (valid) {
+                co.Zc[bbeg] = 1.0;
+                dsc.Zsupp[2 * col] = 1 + bbeg;
+                dsc.Zsupp[2 * col + 1] = 1 + bbeg;
+                SState q;
+                q.lam = 0.0; q.lamv = 0.0; q.werr = 0.0; q.left = 0.0; q.right = 0.0;
+                q.wsh = 0.0; q.bstres = 0.0; q.bstw = 0.0; q.nrminv = 1.0; q.resid = 0.0;
+                q.kk = 0; q.r = 1; q.flags = kSDone << 4; q.is0 = 1; q.is1 = 1; q.pad = 0;
+                ST[idx] = q;
             }
-        } else {
-            const int bl_begin = bbeg, bl_end = bbeg + n - 1;
-            const double tol = 4.0 * log((double)n) * DBL_EPSILON;
-            // lane state
-            int state = kLaneEmpty, idx = 0;
-            STask t = t0;
-            Rrr R;
-            R.D = t.D; R.L = t.L; R.LD = t.DL; R.LLD = t.DLL;
-            double pivmin = 0.0, lambda = 0.0, werr = 0.0, left = 0.0, right = 0.0;
-            double gap = 0.0, gaptol = 0.0, bstres = 0.0, bstw = 0.0, wsh = 0.0;
-            double lamv = 0.0;   // shift of this lane's last dlar1v (its vector's shift)
-            unsigned long long t_bis = 0, t_r1v = 0, t_solve = 0;
-            int i_local = 1, usedBS = 0, usedRQ = 0, needBS = 0, wantnc = 1, r = 0, kk = 1;
-            int col = 0;
-            R1v o;
-            o.r = 1; o.nan1 = 0; o.nan2 = 0; o.resid = 0.0; o.rqcorr = 0.0; o.nrminv = 1.0;
-            o.mingma = 0.0; o.isupp0 = 1; o.isupp1 = n; o.negcnt = 0; o.ztz = 1.0;
-            unsigned long long r1v_el = 0, bis_el = 0, wave_calls = 0, bmax_w = 0;
-            int next = 0;   // next list entry to hand out
-            for (;;) {
-                // refill empty lanes from the list
-                {
-                    const bool empty = state == kLaneEmpty;
-                    const unsigned long long em = __ballot(empty);
-                    const int rank = __builtin_amdgcn_mbcnt_hi((unsigned)(em >> 32),
-                                                               __builtin_amdgcn_mbcnt_lo((unsigned)em, 0));
-                    if (empty && next + rank < cntw) {
-                        idx = start + next + rank;
-                        t = T[idx];
-                        R.D = t.D; R.L = t.L; R.LD = t.DL; R.LLD = t.DLL;
-                        const MatState& S = ms[t.mat];
-                        const MatDesc& dsc = md[t.mat];
-                        const int i = t.slot - dsc.off;   // matrix-local eigenvalue index
-                        pivmin = S.pivmin;
-                        lambda = Wsh[t.slot];
-                        werr = Werr[t.slot];
-                        left = lambda - werr;
-                        right = lambda + werr;
-                        i_local = Windex[t.slot];
-                        const double lg = (i == bl_begin) ? DBL_EPSILON * fmax(fabs(left), fabs(right)) : t.lgap;
-                        const double rg = (i == bl_end) ? DBL_EPSILON * fmax(fabs(left), fabs(right)) : Wgap[t.slot];
-                        gap = fmin(lg, rg);
-                        gaptol = (i == bl_begin || i == bl_end) ? 0.0 : gap * DBL_EPSILON;
-                        usedBS = 0; usedRQ = 0; needBS = 0; wantnc = 1; r = 0; kk = 1;
-                        bstres = 0.0; bstw = 0.0; wsh = lambda;
-                        col = Zidx[t.slot];
-                        if (col >= dsc.mcols) col = dsc.mcols - 1;   // never write past the caller's Z
-                        zcol[lane] = dsc.Z + (size_t)col * dsc.ldz;
-                        co.Zc = zcol[lane];
-                        state = kLaneActive;
-                    }
-                    next = min(next + __popcll(em), cntw);
-                }
-                const bool act = state == kLaneActive;
-                const bool s2 = state == kLaneStep2;
-                const bool stg = state == kLaneStage;
-                if (!__any(act || s2 || stg)) break;
-                ++wave_calls;
-                unsigned long long tk = wall_clock64();
-                // bisection fallback (odrrb.f via PMR_process_s_task)
-                if (act && needBS) {
-                    usedBS = 1;
-                    double w = wsh, we = werr;
-                    const unsigned long long sw = dlarrb1(n, t.D, t.DLL, i_local, 0.0, 2.0 * DBL_EPSILON,
-                                                          0.0, &w, &we, pivmin, t.bl_spdiam, r);
-                    bis_el += (unsigned long long)n * sw;
-                    wsh = w;
-                    werr = we;
-                    lambda = wsh;
-                    r = 0;
-                }
-                {
-                    unsigned long long bm = (act && needBS) ? 1ULL : 0ULL;
-                    bmax_w += __any(bm) ? 1 : 0;
-                }
-                if (act) wantnc = usedBS ? 0 : 1;
-                if (s2) lambda = bstw;
-                if (act || s2) lamv = lambda;
-                unsigned long long tk1 = wall_clock64();
-                t_bis += tk1 - tk;
-                r1v_ck(n, lamv, R, pivmin, wantnc, r, act || s2, ck, buf, lane, &o);
-                tk = wall_clock64();
-                t_r1v += tk - tk1;
-                pass_solve(n, R, lamv, pivmin, gaptol, ck, buf, bld, lane, &o, act || s2 || stg,
-                           stg, o.nrminv, co);
-                t_solve += wall_clock64() - tk;
-                if (stg) {
-                    // the vector is written: report and free the lane
-                    const MatDesc& dsc = md[t.mat];
-                    dsc.Zsupp[2 * col] = o.isupp0 + bbeg;
-                    dsc.Zsupp[2 * col + 1] = o.isupp1 + bbeg;
-                    Wsh[t.slot] = lambda;
-                    Werr[t.slot] = werr;
-                    W[t.slot] = lambda + t.L[n - 1];
-                    state = kLaneEmpty;
-                }
-                if (s2) {
-                    r1v_el += n;
-                    state = kLaneStage;
-                }
-                if (act) {
-                    r1v_el += n;
-                    r = o.r;
-                    bool done = false;
-                    if (kk == 1 || o.resid < bstres) { bstres = o.resid; bstw = lambda; }
-                    if (!usedBS && o.resid > tol * gap && fabs(o.rqcorr) > RQtol * fabs(lambda)) {
-                        int sgndef = (i_local <= o.negcnt) ? -1 : 1;
-                        if (o.rqcorr * sgndef >= 0.0 && lambda + o.rqcorr <= right &&
-                            lambda + o.rqcorr >= left) {
-                            usedRQ = 1;
-                            if (sgndef == 1) left = lambda;
-                            else right = lambda;
-                            wsh = 0.5 * (left + right);
-                            lambda += o.rqcorr;
-                        } else needBS = 1;
-                        if (right - left < RQtol * fabs(lambda)) usedBS = 1;
-                        else if (kk == kMaxIterRQI - 1) needBS = 1;
-                    } else done = true;
-                    ++kk;
-                    if (done || kk > kMaxIterRQI) {
-                        // the best-residual step after a bisection, else the vector
-                        state = (usedRQ && usedBS && bstres <= o.resid) ? kLaneStep2 : kLaneStage;
-                    }
-                }
-            }
-            add_steps(kStepR1v, r1v_el);
-            add_steps(kStepR1vBis, bis_el);
-            if (lane == 0) {
-                add_steps(kStepR1vWave, wave_calls * (unsigned long long)n);
-                add_steps(kStepBisWave, bmax_w * (unsigned long long)n);
-                add_steps(kStepTBis, t_bis);
-                add_steps(kStepTR1v, t_r1v);
-                add_steps(kStepTSolve, t_solve);
-            }
-            diag = (unsigned)min(wave_calls, 255ULL) | ((unsigned)min(bmax_w, 65535ULL) << 8);
+            continue;
         }
+        Rrr R;
+        R.D = t.D; R.L = t.L; R.LD = t.DL; R.LLD = t.DLL;
+        const MatState& S = ms[t.mat];
+        const int i = t.slot - dsc.off;   // matrix-local eigenvalue index
+        const int bl_begin = bbeg, bl_end = bbeg + n - 1;
+        const double pivmin = S.pivmin;
+        const double tol = 4.0 * log((double)n) * DBL_EPSILON;
+        // gaps from the eigenvalue as the level handed it over (Wsh/Werr of the slot
+        // change only when the vector is done)
+        const double w0 = Wsh[t.slot], e0 = Werr[t.slot];
+        const double lg = (i == bl_begin) ? DBL_EPSILON * fmax(fabs(w0 - e0), fabs(w0 + e0)) : t.lgap;
+        const double rg = (i == bl_end) ? DBL_EPSILON * fmax(fabs(w0 - e0), fabs(w0 + e0)) : Wgap[t.slot];
+        const double gap = fmin(lg, rg);
+        const double gaptol = (i == bl_begin || i == bl_end) ? 0.0 : gap * DBL_EPSILON;
+        const int i_local = Windex[t.slot];
+        SState q;
+        if (first) {
+            q.lam = w0; q.lamv = w0; q.werr = e0; q.left = w0 - e0; q.right = w0 + e0;
+            q.wsh = w0; q.bstres = 0.0; q.bstw = 0.0; q.nrminv = 1.0; q.resid = 0.0;
+            q.kk = 1; q.r = 0; q.flags = (1 << 3) | (kSActive << 4); q.is0 = 1; q.is1 = n; q.pad = 0;
+        } else {
+            q = ST[idx];
+        }
+        int usedBS = q.flags & 1, usedRQ = (q.flags >> 1) & 1, needBS = (q.flags >> 2) & 1;
+        int wantnc = (q.flags >> 3) & 1, state = sflag_state(q.flags);
+        const bool act = valid && (state == kSActive || state == kSStep2);
+        const bool s2 = state == kSStep2;
+        double lambda = q.lam;
+        if (s2) lambda = q.bstw;
+        if (act && !s2) wantnc = usedBS ? 0 : 1;
+        R1v o;
+        o.r = 1; o.nan1 = 0; o.nan2 = 0; o.resid = q.resid; o.rqcorr = 0.0; o.nrminv = q.nrminv;
+        o.mingma = 0.0; o.isupp0 = q.is0; o.isupp1 = q.is1; o.negcnt = 0; o.ztz = 1.0;
+        const unsigned long long tk0 = wall_clock64();
+        r1v_ck(n, lambda, R, pivmin, wantnc, q.r, act, ck, buf, lane, &o);
+        const unsigned long long tk1 = wall_clock64();
+        pass_solve(n, R, lambda, pivmin, gaptol, ck, buf, bld, lane, &o, act, act, 1.0, co);
+        const unsigned long long tk2 = wall_clock64();
+        bool done = false;
+        if (act) {
+            q.lamv = lambda;
+            if (s2) {
+                done = true;
+            } else {
+                int r = o.r;
+                double left = q.left, right = q.right, bstres = q.bstres, bstw = q.bstw;
+                const int kk = q.kk;
+                bool conv = false;
+                if (kk == 1 || o.resid < bstres) { bstres = o.resid; bstw = lambda; }
+                if (!usedBS && o.resid > tol * gap && fabs(o.rqcorr) > RQtol * fabs(lambda)) {
+                    int sgndef = (i_local <= o.negcnt) ? -1 : 1;
+                    if (o.rqcorr * sgndef >= 0.0 && lambda + o.rqcorr <= right &&
+                        lambda + o.rqcorr >= left) {
+                        usedRQ = 1;
+                        if (sgndef == 1) left = lambda;
+                        else right = lambda;
+                        q.wsh = 0.5 * (left + right);
+                        lambda += o.rqcorr;
+                    } else needBS = 1;
+                    if (right - left < RQtol * fabs(lambda)) usedBS = 1;
+                    else if (kk == kMaxIterRQI - 1) needBS = 1;
+                } else conv = true;
+                q.kk = kk + 1;
+                q.r = r;
+                q.left = left; q.right = right; q.bstres = bstres; q.bstw = bstw;
+                q.lam = lambda;
+                if (conv || q.kk > kMaxIterRQI) {
+                    // the best-residual step after a bisection, else the vector is final
+                    if (usedRQ && usedBS && bstres <= o.resid) state = kSStep2;
+                    else done = true;
+                }
+            }
+            if (done) {
+                state = kSDone;
+                dsc.Zsupp[2 * col] = o.isupp0 + bbeg;
+                dsc.Zsupp[2 * col + 1] = o.isupp1 + bbeg;
+                Wsh[t.slot] = lambda;
+                Werr[t.slot] = q.werr;
+                W[t.slot] = lambda + t.L[n - 1];
+                q.lam = lambda;
+            }
+            q.resid = o.resid;
+            q.nrminv = o.nrminv;
+            q.is0 = o.isupp0;
+            q.is1 = o.isupp1;
+            q.flags = usedBS | (usedRQ << 1) | (needBS << 2) | (wantnc << 3) | (state << 4);
+            ST[idx] = q;
+        }
+        add_steps(kStepR1v, act ? (unsigned long long)n : 0ULL);
         if (lane == 0) {
+            add_steps(kStepR1vWave, (unsigned long long)n);
+            add_steps(kStepTR1v, tk1 - tk0);
+            add_steps(kStepTSolve, tk2 - tk1);
             const unsigned long long tick1 = wall_clock64();
             add_steps(kStepTaskTicks, tick1 - tick0);
             add_steps(kStepTasks, 1);
-            unsigned long long* lg = g_tasklog;
-            if (lg && ti < g_tasklog_cap) {
-                const unsigned hw = __builtin_amdgcn_s_getreg((4 << 0) | (0 << 6) | (31 << 11));
-                const unsigned xcc = __builtin_amdgcn_s_getreg((20 << 0) | (0 << 6) | (31 << 11));
-                lg[4 * ti + 0] = tick0;
-                lg[4 * ti + 1] = tick1;
-                lg[4 * ti + 2] = ((unsigned long long)n << 32) | ((unsigned)min(cntw, 255) << 24) | diag;
-                lg[4 * ti + 3] = ((unsigned long long)xcc << 32) | hw;
-            }
         }
     }
 }
+
+// The bisection fallback of the singletons that need it in this iteration
+// (process_singleton.c via dlarrb, odrrb.f:135-299) by multisection: a group of G
+// lanes (G a power of two) serves one singleton.  The bracket expansion evaluates G
+// consecutive candidates of the reference's sequence (left - back, back doubled) per
+// sweep and keeps the first that brackets; each bisection round evaluates the G - 1
+// midpoints of the next log2(G) halvings of the interval (a binary tree, computed
+// with the reference's operations) in one sweep, then walks the tree with the
+// reference's convergence test.  The walk takes the same path and the same midpoints
+// as one-point bisection, so w and werr are bit-identical to dlarrb.
+__global__ __launch_bounds__(64) void k_s_bisect(const int* __restrict__ BI,
+                                                 const LevelCounters* __restrict__ cnt, int G,
+                                                 const STask* __restrict__ T,
+                                                 const MatState* __restrict__ ms,
+                                                 const int* __restrict__ Windex, SState* ST) {
+    const int lane = threadIdx.x;
+    const int grp = lane / G, sub = lane % G;
+    const int b = blockIdx.x * (SW / G) + grp;
+    const int nb = cnt->nEig;
+    const bool valid = b < nb;
+    const int idx = BI[min(b, nb - 1)];
+    const STask t = T[idx];
+    SState q = ST[idx];
+    const int n = t.bsize;
+    const int i = Windex[t.slot];
+    const double pivmin = ms[t.mat].pivmin;
+    const double spdiam = t.bl_spdiam;
+    const double rtol1 = 0.0, rtol2 = 2.0 * DBL_EPSILON, gap = 0.0;
+    const int maxitr = (int)((log(spdiam + pivmin) - log(pivmin)) / log(2.0)) + 2;
+    const double mnwdth = 2.0 * pivmin;
+    int r = q.r;
+    if (r < 1 || r > n) r = n;
+    const double* d = t.D;
+    const double* lld = t.DLL;
+    const bool uni = wave_uniform(d) && wave_uniform(lld) && wave_uniform(n) && wave_uniform(r);
+    auto count = [&](double x) -> int {
+        return uni ? dlaneg(rfl(n), as_const(d), as_const(lld), x, rfl(r)) : dlaneg(n, d, lld, x, r);
+    };
+    const int gbase = grp * G;
+    long long sweeps = 0;   // one-point sweeps of the reference (algorithmic work)
+    int rounds = 0;
+    // bracket expansion to the left and to the right (odrrb.f:176-196)
+    double left = q.wsh - q.werr, right = q.wsh + q.werr;
+    for (int side = 0; side < 2; ++side) {
+        double x = side == 0 ? left : right, back = q.werr;
+        int done = 0, g0 = 0;
+        while (__any(valid && !done)) {
+            // candidate sub of this batch: sub more expansion steps from x
+            double xs = x, bs = back;
+            for (int k = 0; k < sub; ++k) {
+                if (side == 0) xs = xs - bs;
+                else xs = xs + bs;
+                bs = 2.0 * bs;
+            }
+            const int c = count(xs);
+            ++rounds;
+            const bool brk = side == 0 ? !(c > i - 1) : !(c < i);
+            const bool ok = g0 + sub < kMaxExpand;   // the reference stops after kMaxExpand checks
+            const unsigned long long m = __ballot(brk || !ok);
+            const unsigned gm = (unsigned)((m >> gbase) & ((G == 64) ? ~0ULL : ((1ULL << G) - 1)));
+            if (!done) {
+                if (gm) {
+                    const int k = __ffs(gm) - 1;   // first candidate that brackets (or the cap)
+                    sweeps += min(k + 1, kMaxExpand - g0);
+                    x = __shfl(xs, gbase + k);
+                    done = 1;
+                } else {
+                    sweeps += G;
+                    x = __shfl(xs, gbase + G - 1);
+                    back = __shfl(bs, gbase + G - 1);
+                    // advance one more step past the last candidate
+                    if (side == 0) x = x - back;
+                    else x = x + back;
+                    back = 2.0 * back;
+                    g0 += G;
+                }
+            }
+        }
+        if (side == 0) left = x;
+        else right = x;
+    }
+    double w = q.wsh, we = q.werr;
+    double width = 0.5 * fabs(left - right);
+    double tmp = fmax(fabs(left), fabs(right));
+    double cvrgd = fmax(rtol1 * gap, rtol2 * tmp);
+    int conv = (width <= cvrgd || width <= mnwdth);   // interval already tight: unchanged
+    int iter = 0;
+    int depth = 0;
+    for (int g2 = G; g2 > 1; g2 >>= 1) ++depth;
+    bool fin = conv;
+    while (__any(valid && !fin)) {
+        // midpoint of heap node sub (1 .. G-1) of the next depth halvings
+        double xm = 0.0;
+        if (sub >= 1) {
+            int h = 0;
+            while ((sub >> (h + 1)) != 0) ++h;   // depth of node sub
+            double a = left, bb = right;
+            for (int k = h - 1; k >= 0; --k) {
+                const double m2 = 0.5 * (a + bb);
+                if ((sub >> k) & 1) a = m2;
+                else bb = m2;
+            }
+            xm = 0.5 * (a + bb);
+        }
+        const int c = (sub >= 1) ? count(xm) : 0;
+        ++rounds;
+        // walk the tree with the reference's test at every node
+        int j = 1;
+        for (int lev = 0; lev < depth; ++lev) {
+            const int cj = __shfl(c, gbase + j);
+            if (!fin) {
+                const double mid = 0.5 * (left + right);
+                width = right - mid;
+                tmp = fmax(fabs(left), fabs(right));
+                cvrgd = fmax(rtol1 * gap, rtol2 * tmp);
+                if (width <= cvrgd || width <= mnwdth || iter == maxitr) {
+                    fin = true;
+                } else {
+                    ++sweeps;
+                    ++iter;
+                    if (cj <= i - 1) { left = mid; j = 2 * j + 1; }
+                    else { right = mid; j = 2 * j; }
+                }
+            }
+            if (j >= G) j = G - 1;   // keeps the shuffle source in the group once finished
+        }
+    }
+    if (!conv) {
+        w = 0.5 * (left + right);
+        we = right - w;
+    }
+    if (valid && sub == 0) {
+        add_steps(kStepR1vBis, (unsigned long long)n * (unsigned long long)sweeps);
+        q.wsh = w;
+        q.werr = we;
+        q.lam = w;
+        q.r = 0;
+        q.flags |= 1;   // usedBS
+        ST[idx] = q;
+    }
+    if (lane == 0) add_steps(kStepBisWave, (unsigned long long)n * rounds);
+}
+
+// Collects the singletons that go on iterating (per block, segmented like the
+// singleton list) and the ones that need the bisection fallback first.  One wave per
+// task of the previous iteration.  Counters: nS = singletons going on, nEig = of
+// them needing bisection.
+__global__ __launch_bounds__(64) void k_s_collect(const WaveTask* __restrict__ tasks,
+                                                  const LevelCounters* __restrict__ pcnt,
+                                                  int first, const int* __restrict__ SIp,
+                                                  const STask* __restrict__ T,
+                                                  const SState* __restrict__ ST, int* SIn,
+                                                  int* bcnt, int* BI, LevelCounters* cnt) {
+    if ((int)blockIdx.x >= pcnt->nT) return;
+    const WaveTask wt = tasks[blockIdx.x];
+    const int lane = threadIdx.x;
+    if (lane >= wt.cnt) return;
+    const int idx = first ? wt.start + lane : SIp[wt.start + lane];
+    const int f = ST[idx].flags;
+    const int st = sflag_state(f);
+    if (st == kSDone) return;
+    const STask& t = T[idx];
+    SIn[t.wseg + atomicAdd(&bcnt[t.gblk], 1)] = idx;
+    atomicAdd(&cnt->nS, 1);
+    if (st == kSActive && ((f >> 2) & 1)) BI[atomicAdd(&cnt->nEig, 1)] = idx;
+}
+
+// Final vectors: Z(isupp) scaled by 1/||z|| (the last dlar1v's NRMINV), zeros in the
+// rest of the column (process_singleton.c:300-324).  One wave per task of the first
+// iteration, which lists every singleton of the level.
+__global__ __launch_bounds__(64) void k_s_normalize(const WaveTask* __restrict__ tasks,
+                                                    const LevelCounters* __restrict__ cnt,
+                                                    const STask* __restrict__ T,
+                                                    const MatDesc* __restrict__ md,
+                                                    const int* __restrict__ Zidx,
+                                                    const SState* __restrict__ ST) {
+    __shared__ double* zc_s[SW];
+    __shared__ int lo_s[SW], hi_s[SW];
+    __shared__ double sc_s[SW];
+    if ((int)blockIdx.x >= cnt->nT) return;
+    const WaveTask wt = tasks[blockIdx.x];
+    const int lane = threadIdx.x;
+    const int cntw = wt.cnt;
+    const int idx = wt.start + min(lane, cntw - 1);
+    const STask t = T[idx];
+    const MatDesc& dsc = md[t.mat];
+    int col = Zidx[t.slot];
+    if (col >= dsc.mcols) col = dsc.mcols - 1;
+    const SState q = ST[idx];
+    zc_s[lane] = dsc.Z + (size_t)col * dsc.ldz;
+    lo_s[lane] = t.bbeg + q.is0 - 1;
+    hi_s[lane] = t.bbeg + q.is1 - 1;
+    sc_s[lane] = q.nrminv;
+    const int nfull = dsc.n;
+    __syncthreads();
+    for (int c = 0; c < cntw; ++c) {
+        double* zc = zc_s[c];
+        const int lo = lo_s[c], hi = hi_s[c];
+        const double sc = sc_s[c];
+        for (int rr = lane; rr < nfull; rr += SW) zc[rr] = (rr >= lo && rr <= hi) ? zc[rr] * sc : 0.0;
+    }
+}
+
 // ---------------------------------------------------------------------------------
 // k_finalize: unscale (mrrr.c:371-386), place eigenvalues in column order
 // (sort_eigenpairs, mrrr.c:525-588) and report the outputs.  One workgroup per

@@ -19,7 +19,9 @@ constexpr int kMaxExpand = 128;        // cap on bracket doublings (2^128 spans 
 // Loop chunks: every sweep runs in chunks of kU steps and issues the loads of the
 // next chunk before it computes the current one (software pipelining), so a chunk's
 // memory latency is covered by the previous chunk's arithmetic instead of being paid
-// by every dependent step.
+// by every dependent step.  The prefetch reads up to 2 kU entries past either end of
+// the arrays unclamped: every array a sweep reads is allocated with kSweepPad spare
+// entries on both sides, and entries read outside the block are never used.
 constexpr int kU = 8;
 
 // Read-only device arrays that every active lane of a wave reads at the same index
@@ -53,13 +55,13 @@ __device__ __forceinline__ int count_T_ijob1(int n, P d, P e2, double x, double 
     double dd[kU], ee[kU], dn[kU], en[kU];
 #pragma unroll
     for (int u = 0; u < kU; ++u) {
-        int j = max(min(1 + u, n - 1), 1);
+        int j = 1 + u;
         dd[u] = d[j]; ee[u] = e2[j - 1];
     }
     for (int j0 = 1; j0 < n; j0 += kU) {
 #pragma unroll
         for (int u = 0; u < kU; ++u) {
-            int j = max(min(j0 + kU + u, n - 1), 1);
+            int j = j0 + kU + u;
             dn[u] = d[j]; en[u] = e2[j - 1];
         }
 #pragma unroll
@@ -87,13 +89,13 @@ __device__ __forceinline__ int count_T_ijob2(int n, P d, P e2, double x, double 
     double dd[kU], ee[kU], dn[kU], en[kU];
 #pragma unroll
     for (int u = 0; u < kU; ++u) {
-        int j = max(min(1 + u, n - 1), 1);
+        int j = 1 + u;
         dd[u] = d[j]; ee[u] = e2[j - 1];
     }
     for (int j0 = 1; j0 < n; j0 += kU) {
 #pragma unroll
         for (int u = 0; u < kU; ++u) {
-            int j = max(min(j0 + kU + u, n - 1), 1);
+            int j = j0 + kU + u;
             dn[u] = d[j]; en[u] = e2[j - 1];
         }
 #pragma unroll
@@ -123,13 +125,13 @@ __device__ __forceinline__ int dlaneg(int n, P d, P lld, double sigma, int r) {
     if (top > 0) {
 #pragma unroll
         for (int u = 0; u < kU; ++u) {
-            int j = min(u, top - 1);
+            int j = u;
             dd[u] = d[j]; ll[u] = lld[j];
         }
         for (int j0 = 0; j0 < top; j0 += kU) {
 #pragma unroll
             for (int u = 0; u < kU; ++u) {
-                int j = min(j0 + kU + u, top - 1);
+                int j = j0 + kU + u;
                 dn[u] = d[j]; ln[u] = lld[j];
             }
 #pragma unroll
@@ -150,13 +152,13 @@ __device__ __forceinline__ int dlaneg(int n, P d, P lld, double sigma, int r) {
     if (n - 2 >= r - 1) {
 #pragma unroll
         for (int u = 0; u < kU; ++u) {
-            int j = max(n - 2 - u, r - 1);
+            int j = n - 2 - u;
             dd[u] = d[j]; ll[u] = lld[j];
         }
         for (int j0 = n - 2; j0 >= r - 1; j0 -= kU) {
 #pragma unroll
             for (int u = 0; u < kU; ++u) {
-                int j = max(j0 - kU - u, r - 1);
+                int j = j0 - kU - u;
                 dn[u] = d[j]; ln[u] = lld[j];
             }
 #pragma unroll
@@ -235,13 +237,13 @@ __device__ __forceinline__ int count_T_rrj(int n, P d, P e2, double s) {
     double dd[kU], ee[kU], dn[kU], en[kU];
 #pragma unroll
     for (int u = 0; u < kU; ++u) {
-        int j = max(min(1 + u, n - 1), 1);
+        int j = 1 + u;
         dd[u] = d[j]; ee[u] = e2[j - 1];
     }
     for (int j0 = 1; j0 < n; j0 += kU) {
 #pragma unroll
         for (int u = 0; u < kU; ++u) {
-            int j = max(min(j0 + kU + u, n - 1), 1);
+            int j = j0 + kU + u;
             dn[u] = d[j]; en[u] = e2[j - 1];
         }
 #pragma unroll
