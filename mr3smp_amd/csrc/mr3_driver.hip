@@ -78,20 +78,36 @@ struct CallEvents {
     }
 };
 
-#define LAUNCH(KERN, GRID, BLOCK, ...)                                              \
+#define LAUNCH_ON(STRM, KERN, GRID, BLOCK, ...)                                     \
     do {                                                                            \
         hipEvent_t e0_ = nullptr, e1_ = nullptr;                                    \
         if (ce.on) {                                                                \
             (void)hipEventCreate(&e0_);                                             \
             (void)hipEventCreate(&e1_);                                             \
-            (void)hipEventRecord(e0_, st);                                          \
+            (void)hipEventRecord(e0_, STRM);                                        \
         }                                                                           \
-        hipLaunchKernelGGL(KERN, GRID, BLOCK, 0, st, __VA_ARGS__);                   \
+        hipLaunchKernelGGL(KERN, GRID, BLOCK, 0, STRM, __VA_ARGS__);                 \
         if (ce.on) {                                                                \
-            (void)hipEventRecord(e1_, st);                                          \
+            (void)hipEventRecord(e1_, STRM);                                        \
             ce.ev.push_back({#KERN, {e0_, e1_}});                                   \
         }                                                                           \
     } while (0)
+#define LAUNCH(KERN, GRID, BLOCK, ...) LAUNCH_ON(st, KERN, GRID, BLOCK, __VA_ARGS__)
+
+// The singleton iterations of a tree level run on a second stream of the calling
+// thread, beside the cluster work of the same level (they touch disjoint eigenvalue
+// slots, Z columns and representations, as the reference's S- and C-tasks do).
+// Two of them take consecutive levels in turn, so that the last few singletons of
+// level L (the ones that need all ten iterations) run beside the bulk of level L + 1.
+static hipStream_t side_stream(int k) {
+    thread_local hipStream_t s[2] = {nullptr, nullptr};
+    thread_local bool tried[2] = {false, false};
+    if (!tried[k]) {
+        tried[k] = true;
+        if (hipStreamCreateWithFlags(&s[k], hipStreamNonBlocking) != hipSuccess) s[k] = nullptr;
+    }
+    return s[k];
+}
 
 // Device memory in one stream-ordered pool: buffers of a solve are taken with
 // hipMallocAsync and returned with hipFreeAsync on the same stream, so that
@@ -237,43 +253,64 @@ int run_pipeline(std::vector<MatDesc>& md, std::vector<MatState>& ms, hipStream_
                            Zidx);
         // task lists: singletons (<= N), clusters (<= N/2 + 1), eigen entries (<= N)
         const int capS = N, capC = N / 2 + 1, capE = N;
-        STask* Sl = A.get<STask>(capS);
+        // singleton lists of two consecutive levels (the side stream may still work on
+        // level L while the main stream classifies level L + 1)
+        STask* Sl2[2] = {A.get<STask>(capS), A.get<STask>(capS)};
         ClTask* Cl[2] = {A.get<ClTask>(capC), A.get<ClTask>(capC)};
         int* es[2] = {A.get<int>(capE), A.get<int>(capE)};
         int* ec[2] = {A.get<int>(capE), A.get<int>(capE)};
         LevelCounters* dcnt = A.get<LevelCounters>(1);
         unsigned long long* bump = A.get<unsigned long long>(1);
-        int* bcnt = A.get<int>(N);
+        int* bcnt2[2] = {A.get<int>(N), A.get<int>(N)};
         const int capT = N / 64 + nbl + 1;
-        WaveTask* wtasks = A.get<WaveTask>(capT);
-        // singleton iterations: per-singleton state, two generations of compacted
-        // lists (S entries per block), their tasks and counters, bisection list
-        SState* ST = A.get<SState>(capS);
-        WaveTask* tit[2] = {A.get<WaveTask>(capT), A.get<WaveTask>(capT)};
-        int* sit[2] = {A.get<int>(N), A.get<int>(N)};
-        LevelCounters* cit[2] = {A.get<LevelCounters>(1), A.get<LevelCounters>(1)};
-        int* sbc = A.get<int>(N);
-        int* BI = A.get<int>(N);
+        // singleton iterations, one set per side stream: the first iteration's tasks,
+        // per-singleton state, two generations of compacted lists (S entries per
+        // block) with their tasks and counters, the bisection list
+        struct SSet {
+            WaveTask* wtasks;
+            LevelCounters* scnt;
+            SState* ST;
+            WaveTask* tit[2];
+            int* sit[2];
+            LevelCounters* cit[2];
+            int* sbc;
+            int* BI;
+            double* scratch;
+        } ss[2];
+        for (int k = 0; k < 2; ++k) {
+            ss[k].wtasks = A.get<WaveTask>(capT);
+            ss[k].scnt = A.get<LevelCounters>(1);
+            ss[k].ST = A.get<SState>(capS);
+            for (int b = 0; b < 2; ++b) {
+                ss[k].tit[b] = A.get<WaveTask>(capT);
+                ss[k].sit[b] = A.get<int>(N);
+                ss[k].cit[b] = A.get<LevelCounters>(1);
+            }
+            ss[k].sbc = A.get<int>(N);
+            ss[k].BI = A.get<int>(N);
+        }
         if (!A.ok) return 2;
-        MR3_CHECK(hipMemsetAsync(bcnt, 0, sizeof(int) * (size_t)N, st));
-        MR3_CHECK(hipMemsetAsync(sbc, 0, sizeof(int) * (size_t)N, st));
+        MR3_CHECK(hipMemsetAsync(bcnt2[0], 0, sizeof(int) * (size_t)N, st));
+        MR3_CHECK(hipMemsetAsync(bcnt2[1], 0, sizeof(int) * (size_t)N, st));
+        MR3_CHECK(hipMemsetAsync(ss[0].sbc, 0, sizeof(int) * (size_t)N, st));
+        MR3_CHECK(hipMemsetAsync(ss[1].sbc, 0, sizeof(int) * (size_t)N, st));
         // eigenvector scratch: two checkpoint arrays of maxbl/kChunk + 2 carries per
         // lane, 64 lanes per wave
         const size_t ck_per_lane = (size_t)2 * (maxbl / kChunk + 2);
         const size_t per_wave = ck_per_lane * 64 * sizeof(double);
-        const size_t budget = singleton_scratch_budget();
+        const size_t budget = singleton_scratch_budget() / 2;   // two side streams
         int max_waves = (int)std::max<size_t>(1, std::min<size_t>(budget / per_wave,
                                                                   (size_t)env_int("MR3_SWAVES", 8192)));
         const int want_waves = capT;
         const int nwaves = std::min(max_waves, want_waves);
-        double* scratch = A.get<double>((size_t)nwaves * ck_per_lane * 64);
+        for (int k = 0; k < 2; ++k) ss[k].scratch = A.get<double>((size_t)nwaves * ck_per_lane * 64);
         if (!A.ok) return 2;
         LevelCounters hc;
         auto reset = [&]() { return hipMemsetAsync(dcnt, 0, sizeof(LevelCounters), st); };
         MR3_CHECK(reset());
         TreeOut out;
-        out.S = Sl; out.C = Cl[0]; out.eig_slot = es[0]; out.eig_cl = ec[0]; out.cnt = dcnt;
-        out.bcnt = bcnt;
+        out.S = Sl2[0]; out.C = Cl[0]; out.eig_slot = es[0]; out.eig_cl = ec[0]; out.cnt = dcnt;
+        out.bcnt = bcnt2[0];
         out.capS = capS; out.capC = capC; out.capEig = capE;
         LAUNCH(k_classify_root, dim3(cdiv(nbl, 64)), dim3(64), dmd, dms, blk,
                            smat, dbl, gers, D, E, DLr, DLLr, W, Werr, Wgap, Wsh, out, nbl);
@@ -284,73 +321,133 @@ int run_pipeline(std::vector<MatDesc>& md, std::vector<MatState>& ms, hipStream_
         int cur = 0;
         const bool trace = getenv("MR3_TRACE_LEVELS") != nullptr;   // per-level timing to stderr
         unsigned long long prev_ticks = 0, prev_tasks = 0;
+        hipStream_t s2p[2] = {side_stream(0), side_stream(1)};
+        if (!s2p[0] || !s2p[1]) return 2;
+        hipEvent_t evS[2] = {nullptr, nullptr};   // end of each side stream's last phase
+        MR3_CHECK(hipEventCreateWithFlags(&evS[0], hipEventDisableTiming));
+        MR3_CHECK(hipEventCreateWithFlags(&evS[1], hipEventDisableTiming));
+        struct EvFree {
+            hipEvent_t a, b;
+            ~EvFree() { (void)hipEventDestroy(a); (void)hipEventDestroy(b); }
+        } ev_free{evS[0], evS[1]};
+        struct SideSync {   // no buffer or event goes away while a side stream may use it
+            hipStream_t a, b;
+            ~SideSync() {
+                if (a) (void)hipStreamSynchronize(a);
+                if (b) (void)hipStreamSynchronize(b);
+            }
+        } side_sync{s2p[0], s2p[1]};
+        bool have_evS[2] = {false, false};
+        // the side streams start after everything so far on the main stream
+        hipEvent_t evStart = nullptr;
+        MR3_CHECK(hipEventCreateWithFlags(&evStart, hipEventDisableTiming));
+        MR3_CHECK(hipEventRecord(evStart, st));
+        MR3_CHECK(hipStreamWaitEvent(s2p[0], evStart, 0));
+        MR3_CHECK(hipStreamWaitEvent(s2p[1], evStart, 0));
+        (void)hipEventDestroy(evStart);
         for (int level = 0;; ++level) {
             const int nS = hc.nS, nC = hc.nC, nE = hc.nEig;
             const long long arena_need = hc.arena;
+            const int p = level & 1;
+            LevelCounters hcn = hc;
+            // 1. cluster work of this level and the next level's classification, on the
+            // main stream
+            if (nC > 0) {
+                if (level > 64) return 2;
+                double* arena = A.get_swept((size_t)arena_need);
+                if (!A.ok) return 2;
+                MR3_CHECK(hipMemsetAsync(bump, 0, sizeof(unsigned long long), st));
+                LAUNCH(k_arena_assign, dim3(cdiv(nC, 256)), dim3(256), Cl[cur], nC, arena, bump);
+                LAUNCH(k_endref, dim3(cdiv(2LL * nC, 64)), dim3(64), Cl[cur], nC, Windex, Wsh, Werr,
+                       dms, Wtmp, Werrtmp);
+                LAUNCH(k_dlarrf, dim3(cdiv(nC, 64)), dim3(64), Cl[cur], nC, Wsh, Werr, Wgap, Wtmp,
+                       Werrtmp, dms);
+                LAUNCH(k_newrrr_fin, dim3(nC), dim3(64), Cl[cur], nC, Wsh, Werr);
+                LAUNCH(k_refine_cl, dim3(cdiv(nE, 256)), dim3(256), Cl[cur], es[cur], ec[cur], nE,
+                       Windex, Wsh, Werr, Wgap, dms, Wtmp, Werrtmp);
+                LAUNCH(k_gaps_cl, dim3(cdiv(nE, 256)), dim3(256), Cl[cur], es[cur], ec[cur], nE,
+                       Wtmp, Werrtmp, Wsh, Werr, Wgap, W);
+                MR3_CHECK(reset());
+                // the next level's singleton list reuses the buffers of the phase one
+                // level back, which its side stream may still be reading
+                if (have_evS[p ^ 1]) MR3_CHECK(hipStreamWaitEvent(st, evS[p ^ 1], 0));
+                const int nxt = cur ^ 1;
+                out.C = Cl[nxt]; out.eig_slot = es[nxt]; out.eig_cl = ec[nxt];
+                out.S = Sl2[p ^ 1];
+                out.bcnt = bcnt2[p ^ 1];
+                LAUNCH(k_classify, dim3(cdiv(nC, 64)), dim3(64), Cl[cur], nC, Wgap, Wsh, out);
+                MR3_CHECK(hipGetLastError());
+                MR3_CHECK(hipMemcpyAsync(&hcn, dcnt, sizeof(hcn), hipMemcpyDeviceToHost, st));
+                cur = nxt;
+            }
+            // 2. the singletons of this level on side stream p, concurrently with the
+            // main stream and with the other side stream's previous level; every launch
+            // sizes itself on the device, so the whole phase is queued without waiting
             if (nS > 0) {
+                hipStream_t s2 = s2p[p];
+                SSet& q = ss[p];
+                STask* Sl = Sl2[p];
+                int* bcnt = bcnt2[p];
+                // (the level's lists, representations and eigenvalues were produced on
+                // the main stream before the host read this level's counts, which waited
+                // for it; the main stream's later work touches other slots)
                 hipEvent_t t0 = nullptr, t1 = nullptr;
                 if (trace) {
                     (void)hipEventCreate(&t0);
                     (void)hipEventCreate(&t1);
-                    (void)hipEventRecord(t0, st);
+                    (void)hipEventRecord(t0, s2);
                 }
                 // first iteration over the level's singleton lists (64 per task,
                 // longest blocks first)
-                LAUNCH(k_plan_count, dim3(cdiv(nbl, 64)), dim3(64), dbl, nbl, blk, bcnt, 64, dcnt);
-                LAUNCH(k_plan_s, dim3(cdiv(nbl, 64)), dim3(64), dbl, nbl, smat, dmd, blk, bcnt,
-                       64, wtasks, capT, dcnt);
-                const int g1 = nS / 64 + nbl + 1;   // bound on the first iteration's tasks
-                LAUNCH(k_s_iter, dim3(std::min(nwaves, g1)), dim3(64), wtasks, dcnt, 1,
-                       (const int*)nullptr, Sl, dmd, dms, Windex, Zidx, W, Wsh, Werr, Wgap, ST,
-                       scratch, maxbl);
-                // further iterations on the compacted singletons that go on
-                const WaveTask* ptasks = wtasks;
-                const LevelCounters* pcnt = dcnt;
+                MR3_CHECK(hipMemsetAsync(q.scnt, 0, sizeof(LevelCounters), s2));
+                LAUNCH_ON(s2, k_plan_count, dim3(cdiv(nbl, 64)), dim3(64), dbl, nbl, blk, bcnt, 64,
+                          q.scnt);
+                LAUNCH_ON(s2, k_plan_s, dim3(cdiv(nbl, 64)), dim3(64), dbl, nbl, smat, dmd, blk, bcnt,
+                          64, q.wtasks, capT, q.scnt);
+                const int g1 = nS / 64 + nbl + 1;   // bound on the tasks of any iteration
+                const int gw = std::min(nwaves, g1);
+                LAUNCH_ON(s2, k_s_iter, dim3(gw), dim3(64), q.wtasks, q.scnt, 1, (const int*)nullptr,
+                          Sl, dmd, dms, Windex, Zidx, W, Wsh, Werr, Wgap, q.ST, q.scratch, maxbl);
+                // further iterations on the compacted singletons that go on (the
+                // reference's loop runs at most kMaxIterRQI dlar1v calls plus the
+                // best-residual step)
+                const WaveTask* ptasks = q.wtasks;
+                const LevelCounters* pcnt = q.scnt;
                 const int* psi = nullptr;
-                int pfirst = 1, pgrid = g1, iters = 1;
+                int pfirst = 1;
                 for (int it = 1; it <= kMaxIterRQI + 1; ++it) {
                     const int b = it & 1;
-                    MR3_CHECK(hipMemsetAsync(cit[b], 0, sizeof(LevelCounters), st));
-                    LAUNCH(k_s_collect, dim3(pgrid), dim3(64), ptasks, pcnt, pfirst, psi, Sl, ST,
-                           sit[b], sbc, BI, cit[b]);
-                    LAUNCH(k_plan_count, dim3(cdiv(nbl, 64)), dim3(64), dbl, nbl, blk, sbc, 64,
-                           cit[b]);
-                    LAUNCH(k_plan_s, dim3(cdiv(nbl, 64)), dim3(64), dbl, nbl, smat, dmd, blk, sbc,
-                           64, tit[b], capT, cit[b]);
-                    LevelCounters hi;
-                    MR3_CHECK(hipMemcpyAsync(&hi, cit[b], sizeof(hi), hipMemcpyDeviceToHost, st));
-                    MR3_CHECK(hipStreamSynchronize(st));
-                    if (hi.nS == 0) break;
-                    if (hi.nEig > 0) {
-                        // lanes per bisection: as many as keep about 128K lanes busy
-                        int G = 64;
-                        while (G > 2 && (long long)hi.nEig * G > 131072) G >>= 1;
-                        LAUNCH(k_s_bisect, dim3(cdiv(hi.nEig, 64 / G)), dim3(64), BI, cit[b], G,
-                               Sl, dms, Windex, ST);
-                    }
-                    LAUNCH(k_s_iter, dim3(std::min(nwaves, hi.nT)), dim3(64), tit[b], cit[b], 0,
-                           sit[b], Sl, dmd, dms, Windex, Zidx, W, Wsh, Werr, Wgap, ST, scratch,
-                           maxbl);
-                    ptasks = tit[b];
-                    pcnt = cit[b];
-                    psi = sit[b];
+                    MR3_CHECK(hipMemsetAsync(q.cit[b], 0, sizeof(LevelCounters), s2));
+                    LAUNCH_ON(s2, k_s_collect, dim3(g1), dim3(64), ptasks, pcnt, pfirst, psi, Sl,
+                              q.ST, q.sit[b], q.sbc, q.BI, q.cit[b]);
+                    LAUNCH_ON(s2, k_plan_count, dim3(cdiv(nbl, 64)), dim3(64), dbl, nbl, blk, q.sbc,
+                              64, q.cit[b]);
+                    LAUNCH_ON(s2, k_plan_s, dim3(cdiv(nbl, 64)), dim3(64), dbl, nbl, smat, dmd, blk,
+                              q.sbc, 64, q.tit[b], capT, q.cit[b]);
+                    LAUNCH_ON(s2, k_s_bisect, dim3(std::min(gw, 2048)), dim3(64), q.BI, q.cit[b], Sl,
+                              dms, Windex, q.ST);
+                    LAUNCH_ON(s2, k_s_iter, dim3(gw), dim3(64), q.tit[b], q.cit[b], 0, q.sit[b], Sl,
+                              dmd, dms, Windex, Zidx, W, Wsh, Werr, Wgap, q.ST, q.scratch, maxbl);
+                    ptasks = q.tit[b];
+                    pcnt = q.cit[b];
+                    psi = q.sit[b];
                     pfirst = 0;
-                    pgrid = hi.nT;
-                    ++iters;
                 }
-                LAUNCH(k_s_normalize, dim3(g1), dim3(64), wtasks, dcnt, Sl, dmd, Zidx, ST);
+                LAUNCH_ON(s2, k_s_normalize, dim3(g1), dim3(64), q.wtasks, q.scnt, Sl, dmd, Zidx, q.ST);
                 MR3_CHECK(hipGetLastError());
+                MR3_CHECK(hipEventRecord(evS[p], s2));
+                have_evS[p] = true;
                 if (trace) {
                     float ms = 0.f;
-                    (void)hipEventRecord(t1, st);
+                    (void)hipEventRecord(t1, s2);
                     (void)hipEventSynchronize(t1);
                     (void)hipEventElapsedTime(&ms, t0, t1);
                     unsigned long long stv[kStepKinds] = {};
                     (void)hipMemcpyFromSymbol(stv, HIP_SYMBOL(g_steps), sizeof(stv));
                     fprintf(stderr,
-                            "mr3 level %d: singletons %d clusters %d iterations %d  %.3f ms  "
+                            "mr3 level %d: singletons %d clusters %d  %.3f ms  "
                             "tasks %llu  task time %.3f wave-ms\n",
-                            level, nS, nC, iters, ms, stv[kStepTasks] - prev_tasks,
+                            level, nS, nC, ms, stv[kStepTasks] - prev_tasks,
                             (stv[kStepTaskTicks] - prev_ticks) * 1e-5);
                     prev_ticks = stv[kStepTaskTicks];
                     prev_tasks = stv[kStepTasks];
@@ -359,32 +456,13 @@ int run_pipeline(std::vector<MatDesc>& md, std::vector<MatState>& ms, hipStream_
                 }
             }
             if (nC == 0) break;
-            if (level > 64) return 2;
-            double* arena = A.get_swept((size_t)arena_need);
-            if (!A.ok) return 2;
-            MR3_CHECK(hipMemsetAsync(bump, 0, sizeof(unsigned long long), st));
-            LAUNCH(k_arena_assign, dim3(cdiv(nC, 256)), dim3(256), Cl[cur], nC,
-                               arena, bump);
-            LAUNCH(k_endref, dim3(cdiv(2LL * nC, 64)), dim3(64), Cl[cur], nC,
-                               Windex, Wsh, Werr, dms, Wtmp, Werrtmp);
-            LAUNCH(k_dlarrf, dim3(cdiv(nC, 64)), dim3(64), Cl[cur], nC, Wsh, Werr,
-                               Wgap, Wtmp, Werrtmp, dms);
-            LAUNCH(k_newrrr_fin, dim3(nC), dim3(64), Cl[cur], nC, Wsh, Werr);
-            LAUNCH(k_refine_cl, dim3(cdiv(nE, 256)), dim3(256), Cl[cur], es[cur],
-                               ec[cur], nE, Windex, Wsh, Werr, Wgap, dms, Wtmp, Werrtmp);
-            LAUNCH(k_gaps_cl, dim3(cdiv(nE, 256)), dim3(256), Cl[cur], es[cur],
-                               ec[cur], nE, Wtmp, Werrtmp, Wsh, Werr, Wgap, W);
-            MR3_CHECK(reset());
-            const int nxt = cur ^ 1;
-            out.C = Cl[nxt]; out.eig_slot = es[nxt]; out.eig_cl = ec[nxt];
-            LAUNCH(k_classify, dim3(cdiv(nC, 64)), dim3(64), Cl[cur], nC, Wgap,
-                               Wsh, out);
-            MR3_CHECK(hipGetLastError());
-            MR3_CHECK(hipMemcpyAsync(&hc, dcnt, sizeof(hc), hipMemcpyDeviceToHost, st));
+            // 3. the next level's counts
             MR3_CHECK(hipStreamSynchronize(st));
+            hc = hcn;
             if (hc.err) return 2;
-            cur = nxt;
         }
+        for (int k = 0; k < 2; ++k)
+            if (have_evS[k]) MR3_CHECK(hipStreamWaitEvent(st, evS[k], 0));
     }
     if (nbl > 0) {
         LAUNCH(k_rrj, dim3(cdiv(N, 256)), dim3(256), dmd, dms, blk, smat, iblock,

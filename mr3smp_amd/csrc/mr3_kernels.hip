@@ -2246,15 +2246,33 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
 // with the reference's operations) in one sweep, then walks the tree with the
 // reference's convergence test.  The walk takes the same path and the same midpoints
 // as one-point bisection, so w and werr are bit-identical to dlarrb.
+__device__ __forceinline__ void s_bisect_one(int b, int nb, int G, const int* __restrict__ BI,
+                                             const STask* __restrict__ T,
+                                             const MatState* __restrict__ ms,
+                                             const int* __restrict__ Windex, SState* ST);
+
+// Grid-stride over the bisection list; the group size G is chosen from the list
+// length on the device: as many lanes per singleton as keep about 128K lanes busy.
 __global__ __launch_bounds__(64) void k_s_bisect(const int* __restrict__ BI,
-                                                 const LevelCounters* __restrict__ cnt, int G,
+                                                 const LevelCounters* __restrict__ cnt,
                                                  const STask* __restrict__ T,
                                                  const MatState* __restrict__ ms,
                                                  const int* __restrict__ Windex, SState* ST) {
+    const int nb = __builtin_amdgcn_readfirstlane(cnt->nEig);
+    if (nb == 0) return;
+    int G = 64;
+    while (G > 2 && (long long)nb * G > 131072) G >>= 1;
+    const int per = SW / G;
+    for (int b0 = blockIdx.x * per; b0 < nb; b0 += gridDim.x * per)
+        s_bisect_one(b0 + (int)threadIdx.x / G, nb, G, BI, T, ms, Windex, ST);
+}
+
+__device__ __forceinline__ void s_bisect_one(int b, int nb, int G, const int* __restrict__ BI,
+                                             const STask* __restrict__ T,
+                                             const MatState* __restrict__ ms,
+                                             const int* __restrict__ Windex, SState* ST) {
     const int lane = threadIdx.x;
     const int grp = lane / G, sub = lane % G;
-    const int b = blockIdx.x * (SW / G) + grp;
-    const int nb = cnt->nEig;
     const bool valid = b < nb;
     const int idx = BI[min(b, nb - 1)];
     const STask t = T[idx];
