@@ -162,6 +162,29 @@ static size_t singleton_scratch_budget() {
     return std::min(b, cap);
 }
 
+// Pinned host copies of the level counters, reused across solves (allocating pinned
+// memory per solve would cost more than the solve of a small matrix).
+static std::mutex g_pin_mu;
+static std::vector<LevelCounters*> g_pin_free;
+static LevelCounters* pinned_counters_get() {
+    {
+        std::lock_guard<std::mutex> lk(g_pin_mu);
+        if (!g_pin_free.empty()) {
+            LevelCounters* p = g_pin_free.back();
+            g_pin_free.pop_back();
+            return p;
+        }
+    }
+    LevelCounters* p = nullptr;
+    if (hipHostMalloc((void**)&p, sizeof(LevelCounters), hipHostMallocDefault) != hipSuccess)
+        return nullptr;
+    return p;
+}
+static void pinned_counters_put(LevelCounters* p) {
+    std::lock_guard<std::mutex> lk(g_pin_mu);
+    g_pin_free.push_back(p);
+}
+
 static int env_int(const char* name, int dflt) {
     const char* v = getenv(name);
     return v ? atoi(v) : dflt;
@@ -253,15 +276,20 @@ int run_pipeline(std::vector<MatDesc>& md, std::vector<MatState>& ms, hipStream_
                            Zidx);
         // task lists: singletons (<= N), clusters (<= N/2 + 1), eigen entries (<= N)
         const int capS = N, capC = N / 2 + 1, capE = N;
-        // singleton lists of two consecutive levels (the side stream may still work on
-        // level L while the main stream classifies level L + 1)
-        STask* Sl2[2] = {A.get<STask>(capS), A.get<STask>(capS)};
+        // singleton lists of four consecutive levels: the side streams may still work on
+        // levels L - 2 .. L while the main stream classifies level L + 1
+        constexpr int kLB = 4;
+        STask* Sl2[kLB];
+        int* bcnt2[kLB];
+        for (int k = 0; k < kLB; ++k) {
+            Sl2[k] = A.get<STask>(capS);
+            bcnt2[k] = A.get<int>(N);
+        }
         ClTask* Cl[2] = {A.get<ClTask>(capC), A.get<ClTask>(capC)};
         int* es[2] = {A.get<int>(capE), A.get<int>(capE)};
         int* ec[2] = {A.get<int>(capE), A.get<int>(capE)};
         LevelCounters* dcnt = A.get<LevelCounters>(1);
         unsigned long long* bump = A.get<unsigned long long>(1);
-        int* bcnt2[2] = {A.get<int>(N), A.get<int>(N)};
         const int capT = N / 64 + nbl + 1;
         // singleton iterations, one set per side stream: the first iteration's tasks,
         // per-singleton state, two generations of compacted lists (S entries per
@@ -290,8 +318,8 @@ int run_pipeline(std::vector<MatDesc>& md, std::vector<MatState>& ms, hipStream_
             ss[k].BI = A.get<int>(N);
         }
         if (!A.ok) return 2;
-        MR3_CHECK(hipMemsetAsync(bcnt2[0], 0, sizeof(int) * (size_t)N, st));
-        MR3_CHECK(hipMemsetAsync(bcnt2[1], 0, sizeof(int) * (size_t)N, st));
+        for (int k = 0; k < kLB; ++k)
+            MR3_CHECK(hipMemsetAsync(bcnt2[k], 0, sizeof(int) * (size_t)N, st));
         MR3_CHECK(hipMemsetAsync(ss[0].sbc, 0, sizeof(int) * (size_t)N, st));
         MR3_CHECK(hipMemsetAsync(ss[1].sbc, 0, sizeof(int) * (size_t)N, st));
         // eigenvector scratch: two checkpoint arrays of maxbl/kChunk + 2 carries per
@@ -323,13 +351,21 @@ int run_pipeline(std::vector<MatDesc>& md, std::vector<MatState>& ms, hipStream_
         unsigned long long prev_ticks = 0, prev_tasks = 0;
         hipStream_t s2p[2] = {side_stream(0), side_stream(1)};
         if (!s2p[0] || !s2p[1]) return 2;
-        hipEvent_t evS[2] = {nullptr, nullptr};   // end of each side stream's last phase
-        MR3_CHECK(hipEventCreateWithFlags(&evS[0], hipEventDisableTiming));
-        MR3_CHECK(hipEventCreateWithFlags(&evS[1], hipEventDisableTiming));
+        // end of the singleton phase that last read list buffer k
+        hipEvent_t evS[kLB] = {};
+        for (int k = 0; k < kLB; ++k) MR3_CHECK(hipEventCreateWithFlags(&evS[k], hipEventDisableTiming));
         struct EvFree {
-            hipEvent_t a, b;
-            ~EvFree() { (void)hipEventDestroy(a); (void)hipEventDestroy(b); }
-        } ev_free{evS[0], evS[1]};
+            hipEvent_t* e;
+            ~EvFree() { for (int k = 0; k < kLB; ++k) (void)hipEventDestroy(e[k]); }
+        } ev_free{evS};
+        // the next level's counts arrive in pinned memory, so that queueing their copy
+        // does not wait for the main stream
+        LevelCounters* hpin = pinned_counters_get();
+        if (!hpin) return 2;
+        struct PinFree {
+            LevelCounters* p;
+            ~PinFree() { pinned_counters_put(p); }
+        } pin_free{hpin};
         struct SideSync {   // no buffer or event goes away while a side stream may use it
             hipStream_t a, b;
             ~SideSync() {
@@ -337,7 +373,7 @@ int run_pipeline(std::vector<MatDesc>& md, std::vector<MatState>& ms, hipStream_
                 if (b) (void)hipStreamSynchronize(b);
             }
         } side_sync{s2p[0], s2p[1]};
-        bool have_evS[2] = {false, false};
+        bool have_evS[kLB] = {};
         // the side streams start after everything so far on the main stream
         hipEvent_t evStart = nullptr;
         MR3_CHECK(hipEventCreateWithFlags(&evStart, hipEventDisableTiming));
@@ -348,8 +384,9 @@ int run_pipeline(std::vector<MatDesc>& md, std::vector<MatState>& ms, hipStream_
         for (int level = 0;; ++level) {
             const int nS = hc.nS, nC = hc.nC, nE = hc.nEig;
             const long long arena_need = hc.arena;
-            const int p = level & 1;
-            LevelCounters hcn = hc;
+            const int p = level & 1;           // side stream and its buffer set
+            const int lb = level % kLB;        // this level's list buffer
+            const int lbn = (level + 1) % kLB; // the next level's
             // 1. cluster work of this level and the next level's classification, on the
             // main stream
             if (nC > 0) {
@@ -368,16 +405,16 @@ int run_pipeline(std::vector<MatDesc>& md, std::vector<MatState>& ms, hipStream_
                 LAUNCH(k_gaps_cl, dim3(cdiv(nE, 256)), dim3(256), Cl[cur], es[cur], ec[cur], nE,
                        Wtmp, Werrtmp, Wsh, Werr, Wgap, W);
                 MR3_CHECK(reset());
-                // the next level's singleton list reuses the buffers of the phase one
-                // level back, which its side stream may still be reading
-                if (have_evS[p ^ 1]) MR3_CHECK(hipStreamWaitEvent(st, evS[p ^ 1], 0));
+                // the next level's singleton list reuses the buffers of the phase four
+                // levels back, which a side stream may still be reading
+                if (have_evS[lbn]) MR3_CHECK(hipStreamWaitEvent(st, evS[lbn], 0));
                 const int nxt = cur ^ 1;
                 out.C = Cl[nxt]; out.eig_slot = es[nxt]; out.eig_cl = ec[nxt];
-                out.S = Sl2[p ^ 1];
-                out.bcnt = bcnt2[p ^ 1];
+                out.S = Sl2[lbn];
+                out.bcnt = bcnt2[lbn];
                 LAUNCH(k_classify, dim3(cdiv(nC, 64)), dim3(64), Cl[cur], nC, Wgap, Wsh, out);
                 MR3_CHECK(hipGetLastError());
-                MR3_CHECK(hipMemcpyAsync(&hcn, dcnt, sizeof(hcn), hipMemcpyDeviceToHost, st));
+                MR3_CHECK(hipMemcpyAsync(hpin, dcnt, sizeof(LevelCounters), hipMemcpyDeviceToHost, st));
                 cur = nxt;
             }
             // 2. the singletons of this level on side stream p, concurrently with the
@@ -386,8 +423,8 @@ int run_pipeline(std::vector<MatDesc>& md, std::vector<MatState>& ms, hipStream_
             if (nS > 0) {
                 hipStream_t s2 = s2p[p];
                 SSet& q = ss[p];
-                STask* Sl = Sl2[p];
-                int* bcnt = bcnt2[p];
+                STask* Sl = Sl2[lb];
+                int* bcnt = bcnt2[lb];
                 // (the level's lists, representations and eigenvalues were produced on
                 // the main stream before the host read this level's counts, which waited
                 // for it; the main stream's later work touches other slots)
@@ -435,8 +472,8 @@ int run_pipeline(std::vector<MatDesc>& md, std::vector<MatState>& ms, hipStream_
                 }
                 LAUNCH_ON(s2, k_s_normalize, dim3(g1), dim3(64), q.wtasks, q.scnt, Sl, dmd, Zidx, q.ST);
                 MR3_CHECK(hipGetLastError());
-                MR3_CHECK(hipEventRecord(evS[p], s2));
-                have_evS[p] = true;
+                MR3_CHECK(hipEventRecord(evS[lb], s2));
+                have_evS[lb] = true;
                 if (trace) {
                     float ms = 0.f;
                     (void)hipEventRecord(t1, s2);
@@ -458,10 +495,10 @@ int run_pipeline(std::vector<MatDesc>& md, std::vector<MatState>& ms, hipStream_
             if (nC == 0) break;
             // 3. the next level's counts
             MR3_CHECK(hipStreamSynchronize(st));
-            hc = hcn;
+            hc = *hpin;
             if (hc.err) return 2;
         }
-        for (int k = 0; k < 2; ++k)
+        for (int k = 0; k < kLB; ++k)
             if (have_evS[k]) MR3_CHECK(hipStreamWaitEvent(st, evS[k], 0));
     }
     if (nbl > 0) {
