@@ -9,8 +9,8 @@ outputs are resident in HBM before the timed region starts.
 Prints one JSON line.  Besides the contract fields it reports the plain path's
 rate and the ratio, a short served stream (p50/p99 latency), resid/orth of a
 sample, per-kernel times from HIP events recorded on the library's streams during
-one extra profiled step, the roofline of the dominant kernel (algorithmic FLOPs counted on
-the device), and the CPU baseline (the compiled reference on the host's cores).
+the timed steps, the roofline of the dominant kernel (algorithmic FLOPs counted on
+the device in one extra step), and the CPU baseline (the compiled reference on the host's cores).
 """
 import argparse
 import json
@@ -165,25 +165,32 @@ def main():
     for _ in range(args.warmup):
         rc, arr, keep = M.solve_batch(probs, stream=stream)
         assert rc == 0
+    # timed steps: every launch bracketed by HIP events on its own stream (timing only,
+    # the kernels run exactly as unprofiled)
+    M.profile_reset()
+    M.profile_enable(True, counters=False)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         rc, arr, keep = M.solve_batch(probs, stream=stream)
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
+    M.profile_enable(False)
     assert rc == 0 and sum(a.m for a in arr) == pairs
     value = pairs * args.steps / dt
     log(f"batched: {dt / args.steps * 1e3:.1f} ms/step, {value:.4g} eigenpairs/s")
-    # one more step with per-launch HIP events and device work counters for the
-    # per-kernel breakdown and the roofline (kept out of the timed steps)
+    prof = M.profile_report()
+    prof_steps = args.steps
+    # one more step with the device work counters on: the algorithmic work of a step
+    # (the counts do not depend on timing, so every step has the same)
     M.profile_reset()
     M.profile_enable(True)
     rc, arr, keep = M.solve_batch(probs, stream=stream)
     torch.cuda.synchronize()
     M.profile_enable(False)
     assert rc == 0
-    prof = M.profile_report()
-    prof_steps = 1
+    counted = M.profile_report()
+    M.profile_reset()
 
     # plain path: one matrix per solve, same device-resident buffers
     sample = list(range(0, len(probs), args.plain_stride))
@@ -200,13 +207,16 @@ def main():
 
     worst_r, worst_o = accuracy_sample(mats, keep, arr, 64)
 
-    # roofline of the dominant kernel (per launch averages over the profiled step)
+    # roofline of the dominant kernel: its algorithmic flops per step (counted step)
+    # over its launches per step, against its average launch time in the timed steps
     kern = prof["kernels"]
     dom = max(kern, key=lambda k: kern[k]["ms"])
-    steps = prof["steps"]
+    steps = counted["steps"]
+    launches_per_step = kern[dom]["launches"] / prof_steps
+    assert counted["kernels"][dom]["launches"] == launches_per_step, "launch count differs between steps"
     flops = sum(steps.get(f, 0) * FLOPS_PER_STEP[f] for f in KERNEL_STEPS.get(dom, []))
     per_launch_s = kern[dom]["ms"] / 1e3 / kern[dom]["launches"]
-    flops_per_launch = flops / kern[dom]["launches"]
+    flops_per_launch = flops / launches_per_step
     achieved = flops_per_launch / per_launch_s / 1e12 if per_launch_s > 0 else 0.0
     traffic = None
     tfile = os.path.join(REPO, "profiles", "traffic.json")
@@ -215,7 +225,7 @@ def main():
             traffic = json.load(open(tfile)).get(dom)
         except Exception:
             traffic = None
-    total_flops = sum(steps.get(f, 0) * FLOPS_PER_STEP[f] for f in FLOPS_PER_STEP)
+    total_flops = sum(steps.get(f, 0) * FLOPS_PER_STEP[f] for f in FLOPS_PER_STEP) * prof_steps
     gpu_ms = sum(v["ms"] for v in kern.values())
     z_bytes = 8.0 * sum(len(D) ** 2 for D, _, _ in mats) * prof_steps
     # Z is written once in full by the normalisation pass (support scaled, rest zero)
@@ -266,7 +276,8 @@ def main():
                         "peak_GBps": HBM_PEAK_GBS},
         "kernels_ms_per_step": {k: v["ms"] / prof_steps for k, v in sorted(kern.items(), key=lambda kv: -kv[1]["ms"])},
         "gpu_fp64_tflops_all_kernels": total_flops / (gpu_ms / 1e3) / 1e12 if gpu_ms else None,
-        "sweep_elements_per_step": {k: v / prof_steps for k, v in steps.items()},
+        "launches_per_step": {k: v["launches"] / prof_steps for k, v in kern.items()},
+        "sweep_elements_per_step": dict(steps),
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -107,9 +107,11 @@ void mr3_server_destroy(mr3_server *s);
 double mr3_now(void);
 
 /*
- * Profiling.  While enabled, every kernel launch of the library is bracketed by HIP
+ * Profiling.  With on = 1, every kernel launch of the library is bracketed by HIP
  * events on the stream it runs on and the kernels count their algorithmic work
- * (sweep elements per kernel family); mr3_profile_report writes a JSON object
+ * (sweep elements per kernel family); on = 2 records the events only (the kernels
+ * then run exactly as with profiling off); on = 0 turns both off.
+ * mr3_profile_report writes a JSON object
  * {"kernels": {name: {"ms", "launches"}}, "steps": {family: elements}} into buf and
  * returns its length (or minus the needed size if buf is too small).  No counterpart
  * in the reference, which has no instrumentation.

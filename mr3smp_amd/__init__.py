@@ -176,9 +176,10 @@ def solve_batch(problems, stream=None):
     return rc, arr, keep
 
 
-def profile_enable(on=True):
-    """Per-kernel HIP-event timing and on-device work counters for later solves."""
-    lib().mr3_profile_enable(1 if on else 0)
+def profile_enable(on=True, counters=True):
+    """Per-kernel HIP-event timing for later solves, with the on-device work counters
+    unless counters is False."""
+    lib().mr3_profile_enable((1 if counters else 2) if on else 0)
 
 
 def profile_reset():

@@ -52,6 +52,29 @@ struct Profiler {
 };
 static Profiler g_prof;
 
+// Timing events are reused: creating two per launch costs the host more than the
+// short launches of a level take on the device.
+struct EventPool {
+    std::vector<hipEvent_t> free;
+    hipEvent_t get() {
+        if (!free.empty()) {
+            hipEvent_t e = free.back();
+            free.pop_back();
+            return e;
+        }
+        hipEvent_t e = nullptr;
+        (void)hipEventCreate(&e);
+        return e;
+    }
+    void put(hipEvent_t e) {
+        if (e) free.push_back(e);
+    }
+};
+static EventPool& event_pool() {
+    thread_local EventPool p;
+    return p;
+}
+
 struct CallEvents {
     bool on = false;
     std::vector<std::pair<const char*, std::pair<hipEvent_t, hipEvent_t>>> ev;
@@ -65,10 +88,11 @@ struct CallEvents {
                 pe.ms += ms;
                 pe.launches += 1;
             }
-            (void)hipEventDestroy(e.second.first);
-            (void)hipEventDestroy(e.second.second);
+            event_pool().put(e.second.first);
+            event_pool().put(e.second.second);
         }
         ev.clear();
+        if (g_prof.on.load() != 1) return;   // timing only: the work counters are off
         unsigned long long st[kStepKinds];
         if (hipMemcpyFromSymbol(st, HIP_SYMBOL(g_steps), sizeof(st)) == hipSuccess) {
             for (int i = 0; i < kStepKinds; ++i) g_prof.steps[i] += st[i];
@@ -82,8 +106,8 @@ struct CallEvents {
     do {                                                                            \
         hipEvent_t e0_ = nullptr, e1_ = nullptr;                                    \
         if (ce.on) {                                                                \
-            (void)hipEventCreate(&e0_);                                             \
-            (void)hipEventCreate(&e1_);                                             \
+            e0_ = event_pool().get();                                               \
+            e1_ = event_pool().get();                                               \
             (void)hipEventRecord(e0_, STRM);                                        \
         }                                                                           \
         hipLaunchKernelGGL(KERN, GRID, BLOCK, 0, STRM, __VA_ARGS__);                 \
@@ -340,7 +364,7 @@ int run_pipeline(std::vector<MatDesc>& md, std::vector<MatState>& ms, hipStream_
         out.S = Sl2[0]; out.C = Cl[0]; out.eig_slot = es[0]; out.eig_cl = ec[0]; out.cnt = dcnt;
         out.bcnt = bcnt2[0];
         out.capS = capS; out.capC = capC; out.capEig = capE;
-        LAUNCH(k_classify_root, dim3(cdiv(nbl, 64)), dim3(64), dmd, dms, blk,
+        LAUNCH(k_classify_root, dim3(nbl), dim3(64), dmd, dms, blk,
                            smat, dbl, gers, D, E, DLr, DLLr, W, Werr, Wgap, Wsh, out, nbl);
         MR3_CHECK(hipGetLastError());
         MR3_CHECK(hipMemcpyAsync(&hc, dcnt, sizeof(hc), hipMemcpyDeviceToHost, st));
@@ -393,18 +417,19 @@ int run_pipeline(std::vector<MatDesc>& md, std::vector<MatState>& ms, hipStream_
                 if (level > 64) return 2;
                 double* arena = A.get_swept((size_t)arena_need);
                 if (!A.ok) return 2;
-                MR3_CHECK(hipMemsetAsync(bump, 0, sizeof(unsigned long long), st));
-                LAUNCH(k_arena_assign, dim3(cdiv(nC, 256)), dim3(256), Cl[cur], nC, arena, bump);
+                // the arena cursor and the level counters start at zero
+                LAUNCH(k_zero, dim3(1), dim3(64), (int*)bump, 2, (int*)dcnt,
+                       (int)(sizeof(LevelCounters) / sizeof(int)));
+                LAUNCH(k_arena_assign, dim3(cdiv(nC, 64)), dim3(64), Cl[cur], nC, arena, bump);
                 LAUNCH(k_endref, dim3(cdiv(2LL * nC, 64)), dim3(64), Cl[cur], nC, Windex, Wsh, Werr,
                        dms, Wtmp, Werrtmp);
                 LAUNCH(k_dlarrf, dim3(cdiv(nC, 64)), dim3(64), Cl[cur], nC, Wsh, Werr, Wgap, Wtmp,
                        Werrtmp, dms);
                 LAUNCH(k_newrrr_fin, dim3(nC), dim3(64), Cl[cur], nC, Wsh, Werr);
-                LAUNCH(k_refine_cl, dim3(cdiv(nE, 256)), dim3(256), Cl[cur], es[cur], ec[cur], nE,
+                LAUNCH(k_refine_cl, dim3(cdiv(nE, 64)), dim3(64), Cl[cur], es[cur], ec[cur], nE,
                        Windex, Wsh, Werr, Wgap, dms, Wtmp, Werrtmp);
-                LAUNCH(k_gaps_cl, dim3(cdiv(nE, 256)), dim3(256), Cl[cur], es[cur], ec[cur], nE,
+                LAUNCH(k_gaps_cl, dim3(cdiv(nE, 64)), dim3(64), Cl[cur], es[cur], ec[cur], nE,
                        Wtmp, Werrtmp, Wsh, Werr, Wgap, W);
-                MR3_CHECK(reset());
                 // the next level's singleton list reuses the buffers of the phase four
                 // levels back, which a side stream may still be reading
                 if (have_evS[lbn]) MR3_CHECK(hipStreamWaitEvent(st, evS[lbn], 0));
@@ -436,9 +461,10 @@ int run_pipeline(std::vector<MatDesc>& md, std::vector<MatState>& ms, hipStream_
                 }
                 // first iteration over the level's singleton lists (64 per task,
                 // longest blocks first)
-                MR3_CHECK(hipMemsetAsync(q.scnt, 0, sizeof(LevelCounters), s2));
+                LAUNCH_ON(s2, k_zero, dim3(1), dim3(64), (int*)q.scnt,
+                          (int)(sizeof(LevelCounters) / sizeof(int)), (int*)nullptr, 0);
                 LAUNCH_ON(s2, k_plan_count, dim3(cdiv(nbl, 64)), dim3(64), dbl, nbl, blk, bcnt, 64,
-                          q.scnt);
+                          q.scnt, q.cit[1]);
                 LAUNCH_ON(s2, k_plan_s, dim3(cdiv(nbl, 64)), dim3(64), dbl, nbl, smat, dmd, blk, bcnt,
                           64, q.wtasks, capT, q.scnt);
                 const int g1 = nS / 64 + nbl + 1;   // bound on the tasks of any iteration
@@ -454,11 +480,11 @@ int run_pipeline(std::vector<MatDesc>& md, std::vector<MatState>& ms, hipStream_
                 int pfirst = 1;
                 for (int it = 1; it <= kMaxIterRQI + 1; ++it) {
                     const int b = it & 1;
-                    MR3_CHECK(hipMemsetAsync(q.cit[b], 0, sizeof(LevelCounters), s2));
+                    // q.cit[b] was cleared by the previous iteration's k_plan_count
                     LAUNCH_ON(s2, k_s_collect, dim3(g1), dim3(64), ptasks, pcnt, pfirst, psi, Sl,
                               q.ST, q.sit[b], q.sbc, q.BI, q.cit[b]);
                     LAUNCH_ON(s2, k_plan_count, dim3(cdiv(nbl, 64)), dim3(64), dbl, nbl, blk, q.sbc,
-                              64, q.cit[b]);
+                              64, q.cit[b], it < kMaxIterRQI + 1 ? q.cit[b ^ 1] : (LevelCounters*)nullptr);
                     LAUNCH_ON(s2, k_plan_s, dim3(cdiv(nbl, 64)), dim3(64), dbl, nbl, smat, dmd, blk,
                               q.sbc, 64, q.tit[b], capT, q.cit[b]);
                     LAUNCH_ON(s2, k_s_bisect, dim3(std::min(gw, 2048)), dim3(64), q.BI, q.cit[b], Sl,
@@ -515,8 +541,9 @@ int run_pipeline(std::vector<MatDesc>& md, std::vector<MatState>& ms, hipStream_
 }
 
 void profile_enable(int on) {
-    g_prof.on.store(on ? 1 : 0);
-    int flag = on ? 1 : 0;
+    const int mode = (on == 1 || on == 2) ? on : 0;
+    g_prof.on.store(mode);
+    int flag = mode == 1 ? 1 : 0;
     (void)hipMemcpyToSymbol(HIP_SYMBOL(g_count_steps), &flag, sizeof(int));
 }
 

@@ -975,6 +975,27 @@ struct TreeOut {
     int capS, capC, capEig;
 };
 
+// Child cluster record of a level (PMR_create_subtasks, process_cluster.c:560-600).
+__device__ __forceinline__ ClTask make_cluster(int f, int l, int cb, double cl_lgap, int depth,
+                                               int mat, int gblk, int wseg, int bbeg, int bsize,
+                                               double bl_spdiam, const double* rD, const double* rL,
+                                               const double* rDL, const double* rDLL,
+                                               const double* __restrict__ Wgap, int ebase) {
+    ClTask c;
+    c.cb = f; c.ce = l; c.mat = mat; c.depth = depth + 1;
+    c.bbeg = bbeg; c.bsize = bsize; c.bl_spdiam = bl_spdiam;
+    c.lgap = (f == cb) ? cl_lgap : Wgap[f - 1];
+    c.pD = rD; c.pL = rL; c.pDL = rDL; c.pDLL = rDLL;
+    c.cD = c.cL = c.cDL = c.cDLL = nullptr;
+    c.tau = 0.0;
+    c.shift = 0;
+    c.arena_off = -1;
+    c.eig_base = ebase;
+    c.gblk = gblk;
+    c.wseg = wseg;
+    return c;
+}
+
 // Walk one cluster's eigenvalues and emit singleton and child-cluster tasks
 // (PMR_create_subtasks, process_cluster.c:438-629).  Every singleton is its own
 // task, the schedule the reference takes when max_size = 1 (:481-482).
@@ -1036,24 +1057,14 @@ __device__ void classify_cluster(int cb, int ce, int depth, int mat, int gblk, i
             out.S[sbase++] = t;
         } else {
             auto emit = [&](int f, int l) {
-                ClTask c;
-                c.cb = f; c.ce = l; c.mat = mat; c.depth = depth + 1;
-                c.bbeg = bbeg; c.bsize = bsize; c.bl_spdiam = bl_spdiam;
-                c.lgap = (f == cb) ? cl_lgap : Wgap[f - 1];
-                c.pD = rD; c.pL = rL; c.pDL = rDL; c.pDLL = rDLL;
-                c.cD = c.cL = c.cDL = c.cDLL = nullptr;
-                c.tau = 0.0;
-                c.shift = 0;
-                c.arena_off = -1;
-                c.eig_base = ebase;
-                c.gblk = gblk;
-                c.wseg = wseg;
+                out.C[cbase] = make_cluster(f, l, cb, cl_lgap, depth, mat, gblk, wseg, bbeg, bsize,
+                                            bl_spdiam, rD, rL, rDL, rDLL, Wgap, ebase);
                 for (int j = f; j <= l; ++j) {
                     out.eig_slot[ebase] = j;
                     out.eig_cl[ebase] = cbase;
                     ++ebase;
                 }
-                out.C[cbase++] = c;
+                ++cbase;
             };
             if (depth == 0 && new_size > 3) {
                 int cl_first = new_first;
@@ -1072,8 +1083,24 @@ __device__ void classify_cluster(int cb, int ce, int depth, int mat, int gblk, i
     }
 }
 
-// Level 0: one thread per block slot.  1x1 blocks become singletons of size 1
-// (mrrr_vec.c:330-346).
+// Level 0: one wave per block.  The eigenvalues of the block are walked in tiles of
+// 64; a lane whose index ends a group (the gap test of process_cluster.c:500-503)
+// takes that group: a singleton, a cluster, or at the root the sub-clusters cut at
+// gaps above twice the average gap (process_cluster.c:520-556).  Counts are reduced
+// over the wave and positions come from wave scans, so the tasks of a block are
+// written in index order, as the sequential walk writes them.  1x1 blocks become
+// singletons of size 1 (mrrr_vec.c:330-346).
+__device__ __forceinline__ int wave_exscan64(int v, int* total) {
+    const int lane = threadIdx.x & 63;
+    int x = v;
+    for (int o = 1; o < 64; o <<= 1) {
+        int t = __shfl_up(x, o, 64);
+        if (lane >= o) x += t;
+    }
+    *total = __shfl(x, 63, 64);
+    return x - v;
+}
+
 __global__ __launch_bounds__(64) void k_classify_root(const MatDesc* __restrict__ md,
                                                       const MatState* __restrict__ ms,
                                                       const BlockRec* __restrict__ blk,
@@ -1087,7 +1114,9 @@ __global__ __launch_bounds__(64) void k_classify_root(const MatDesc* __restrict_
                                                       const double* __restrict__ Werr,
                                                       const double* __restrict__ Wgap, double* Wsh,
                                                       TreeOut out, int nblist) {
-    const int ib = blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ int sh_base[3];
+    const int ib = blockIdx.x;
+    const int lane = threadIdx.x;
     if (ib >= nblist) return;
     const int gb = blist[ib];
     const int k = smat[gb];
@@ -1103,15 +1132,19 @@ __global__ __launch_bounds__(64) void k_classify_root(const MatDesc* __restrict_
     const int cb = off + br.wbeg, ce = off + br.wend;
     const double* rD = D + off + br.bbeg;
     const double* rL = E + off + br.bbeg;
+    const double* rDL = DLr + off + br.bbeg;
+    const double* rDLL = DLLr + off + br.bbeg;
+    const int wseg = off + br.wbeg;
     if (nbl == 1) {
+        if (lane != 0) return;
         atomicAdd(&out.cnt->nS, 1);
-        int pos = off + br.wbeg + atomicAdd(&out.bcnt[gb], 1);
+        int pos = wseg + atomicAdd(&out.bcnt[gb], 1);
         atomicMax(&out.cnt->maxbl, 1);
         if (pos >= out.capS) { atomicOr(&out.cnt->err, 1); return; }
         STask t;
         t.slot = cb; t.mat = k; t.bbeg = br.bbeg; t.bsize = 1; t.bl_spdiam = 0.0; t.lgap = 0.0;
-        t.gblk = gb; t.wseg = off + br.wbeg;
-        t.D = rD; t.L = rL; t.DL = DLr + off + br.bbeg; t.DLL = DLLr + off + br.bbeg;
+        t.gblk = gb; t.wseg = wseg;
+        t.D = rD; t.L = rL; t.DL = rDL; t.DLL = rDLL;
         out.S[pos] = t;
         Wsh[cb] = W[cb];
         return;
@@ -1120,14 +1153,140 @@ __global__ __launch_bounds__(64) void k_classify_root(const MatDesc* __restrict_
     // from gersch rows bl_begin .. bl_end-1, Wshifted = W, W += sigma.
     const double* g = gers + 2 * (size_t)off;
     double gl = g[2 * br.bbeg], gu = g[2 * br.bbeg + 1];
-    for (int i = br.bbeg + 1; i < br.bend; ++i) { gl = fmin(gl, g[2 * i]); gu = fmax(gu, g[2 * i + 1]); }
+    for (int i = br.bbeg + 1 + lane; i < br.bend; i += 64) {
+        gl = fmin(gl, g[2 * i]);
+        gu = fmax(gu, g[2 * i + 1]);
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        gl = fmin(gl, __shfl_xor(gl, o, 64));
+        gu = fmax(gu, __shfl_xor(gu, o, 64));
+    }
     const double bl_spdiam = gu - gl;
     const double sigma = br.sigma;
-    for (int s = cb; s <= ce; ++s) { Wsh[s] = W[s]; W[s] += sigma; }
-    // lgap of the root task (mrrr_vec.c:357)
-    const double lgap = fmax(0.0, (Wsh[cb] + sigma) - Werr[cb] - S.vl);
-    classify_cluster(cb, ce, 0, k, gb, off + br.wbeg, br.bbeg, nbl, bl_spdiam, lgap, rD, rL,
-                     DLr + off + br.bbeg, DLLr + off + br.bbeg, Wgap, Wsh, out);
+    // lgap of the root task (mrrr_vec.c:357), from the unshifted value of the first slot
+    const double lgap = fmax(0.0, (W[cb] + sigma) - Werr[cb] - S.vl);
+    const double avggap = bl_spdiam / (nbl - 1);
+    // group end test; the shifted value of slot i is the incoming W[i]
+    auto ends = [&](int i, double wi) { return i == ce || Wgap[i] >= kMinRelgap * fabs(wi); };
+    // the sub-cluster cuts of a root group [f, l] of more than 3 (process_cluster.c:520-556)
+    auto cut = [&](int kk, int cl_first, int l) {
+        return kk == l - 1 || (kk != cl_first && Wgap[kk] > 2.0 * avggap);
+    };
+    // pass 1: counts, and W / Wsh updated in place (each slot by the lane that reads it)
+    int nS = 0, nC = 0, nE = 0;
+    int first = cb;
+    for (int t0 = cb; t0 <= ce; t0 += 64) {
+        const int i = t0 + lane;
+        bool e = false;
+        if (i <= ce) {
+            const double wi = W[i];
+            e = ends(i, wi);
+            Wsh[i] = wi;
+            W[i] = wi + sigma;
+        }
+        const unsigned long long m = __ballot(e);
+        if (e) {
+            const unsigned long long below = m & ((1ull << lane) - 1ull);
+            const int f = below ? t0 + 64 - __clzll(below) : first;
+            const int size = i - f + 1;
+            if (size == 1) ++nS;
+            else if (size > 3) {
+                int cl_first = f;
+                for (int kk = f + 1; kk < i; ++kk) {
+                    if (!cut(kk, cl_first, i)) continue;
+                    const int cl_last = (kk == i - 1) ? i : kk;
+                    ++nC; nE += cl_last - cl_first + 1;
+                    cl_first = kk + 1;
+                }
+            } else { ++nC; nE += size; }
+        }
+        if (m) first = t0 + 64 - __clzll(m);
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        nS += __shfl_xor(nS, o, 64);
+        nC += __shfl_xor(nC, o, 64);
+        nE += __shfl_xor(nE, o, 64);
+    }
+    const int tS = nS, tC = nC, tE = nE;
+    if (lane == 0) {
+        int sb = 0, cbs = 0, eb = 0;
+        if (tS) { atomicAdd(&out.cnt->nS, tS); sb = atomicAdd(&out.bcnt[gb], tS); }
+        if (tC) cbs = atomicAdd(&out.cnt->nC, tC);
+        if (tE) eb = atomicAdd(&out.cnt->nEig, tE);
+        bool bad = wseg + sb + tS > out.capS || cbs + tC > out.capC || eb + tE > out.capEig;
+        if (bad) atomicOr(&out.cnt->err, 1);
+        if (tC) {
+            atomicAdd((unsigned long long*)&out.cnt->arena, (unsigned long long)(4LL * nbl * tC));
+            atomicMax(&out.cnt->maxbl, nbl);
+        }
+        if (tS) atomicMax(&out.cnt->maxbl, nbl);
+        sh_base[0] = bad ? -1 : wseg + sb;
+        sh_base[1] = cbs;
+        sh_base[2] = eb;
+    }
+    __syncthreads();
+    if (sh_base[0] < 0) return;
+    int sbase = sh_base[0], cbase = sh_base[1], ebase = sh_base[2];
+    // pass 2: emit in index order
+    first = cb;
+    for (int t0 = cb; t0 <= ce; t0 += 64) {
+        const int i = t0 + lane;
+        bool e = false;
+        if (i <= ce) e = ends(i, Wsh[i]);
+        const unsigned long long m = __ballot(e);
+        int f = first, cS = 0, cC = 0, cE = 0;
+        if (e) {
+            const unsigned long long below = m & ((1ull << lane) - 1ull);
+            if (below) f = t0 + 64 - __clzll(below);
+            const int size = i - f + 1;
+            if (size == 1) cS = 1;
+            else if (size > 3) {
+                int cl_first = f;
+                for (int kk = f + 1; kk < i; ++kk) {
+                    if (!cut(kk, cl_first, i)) continue;
+                    const int cl_last = (kk == i - 1) ? i : kk;
+                    ++cC; cE += cl_last - cl_first + 1;
+                    cl_first = kk + 1;
+                }
+            } else { cC = 1; cE = size; }
+        }
+        int aS, aC, aE;
+        int oS = sbase + wave_exscan64(cS, &aS);
+        int oC = cbase + wave_exscan64(cC, &aC);
+        int oE = ebase + wave_exscan64(cE, &aE);
+        if (cS) {
+            STask t;
+            t.slot = f; t.mat = k; t.bbeg = br.bbeg; t.bsize = nbl;
+            t.gblk = gb; t.wseg = wseg;
+            t.bl_spdiam = bl_spdiam;
+            t.lgap = (f == cb) ? lgap : Wgap[f - 1];
+            t.D = rD; t.L = rL; t.DL = rDL; t.DLL = rDLL;
+            out.S[oS] = t;
+        }
+        if (cC) {
+            auto emit = [&](int cf, int cl) {
+                for (int j = cf; j <= cl; ++j) {
+                    out.eig_slot[oE] = j;
+                    out.eig_cl[oE] = oC;
+                    ++oE;
+                }
+                out.C[oC] = make_cluster(cf, cl, cb, lgap, 0, k, gb, wseg, br.bbeg, nbl, bl_spdiam,
+                                         rD, rL, rDL, rDLL, Wgap, oE - (cl - cf + 1));
+                ++oC;
+            };
+            if (i - f + 1 > 3) {
+                int cl_first = f;
+                for (int kk = f + 1; kk < i; ++kk) {
+                    if (!cut(kk, cl_first, i)) continue;
+                    const int cl_last = (kk == i - 1) ? i : kk;
+                    emit(cl_first, cl_last);
+                    cl_first = kk + 1;
+                }
+            } else emit(f, i);
+        }
+        sbase += aS; cbase += aC; ebase += aE;
+        if (m) first = t0 + 64 - __clzll(m);
+    }
 }
 
 // Deeper levels: one thread per cluster, after its eigenvalues were refined.
@@ -1148,10 +1307,24 @@ __global__ __launch_bounds__(64) void k_classify(const ClTask* __restrict__ C, i
 // start first.  k_plan_count sizes the classes, k_plan_s writes the list.
 __device__ __forceinline__ int size_class(int n) { return 31 - __clz(max(n, 1)); }
 
+// Clears small counter records with one wave: a one-wave launch fits beside long
+// running kernels of other streams, where a fill of several waves may wait for a
+// whole compute unit.
+__global__ __launch_bounds__(64) void k_zero(int* a, int na, int* b, int nb) {
+    for (int i = threadIdx.x; i < na; i += 64) a[i] = 0;
+    for (int i = threadIdx.x; i < nb; i += 64) b[i] = 0;
+}
+
+// zero_next (optional): the counters of the iteration after this one, which nothing
+// reads from here until then; cleared by the first wave.
 __global__ __launch_bounds__(64) void k_plan_count(const int* __restrict__ blist, int nblist,
                                                    const BlockRec* __restrict__ blk,
                                                    const int* __restrict__ bcnt, int tsize,
-                                                   LevelCounters* cnt) {
+                                                   LevelCounters* cnt, LevelCounters* zero_next) {
+    if (zero_next && blockIdx.x == 0) {
+        int* z = reinterpret_cast<int*>(zero_next);
+        for (int i = threadIdx.x; i < (int)(sizeof(LevelCounters) / sizeof(int)); i += 64) z[i] = 0;
+    }
     const int ib = blockIdx.x * blockDim.x + threadIdx.x;
     if (ib >= nblist) return;
     const int gb = blist[ib];
@@ -1186,7 +1359,7 @@ __global__ __launch_bounds__(64) void k_plan_s(const int* __restrict__ blist, in
 }
 
 // Assigns each cluster of a level its arena slot (4 * bsize doubles).
-__global__ __launch_bounds__(256) void k_arena_assign(ClTask* C, int nC, double* arena,
+__global__ __launch_bounds__(64) void k_arena_assign(ClTask* C, int nC, double* arena,
                                                       unsigned long long* bump) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= nC) return;
@@ -1460,7 +1633,7 @@ __global__ __launch_bounds__(64) void k_newrrr_fin(const ClTask* __restrict__ C,
 
 // Refinement of a cluster's eigenvalues against its new representation, one lane
 // per eigenvalue (process_cluster.c:414-418; the cluster is one dlarrb call).
-__global__ __launch_bounds__(256) void k_refine_cl(const ClTask* __restrict__ C,
+__global__ __launch_bounds__(64) void k_refine_cl(const ClTask* __restrict__ C,
                                                    const int* __restrict__ eig_slot,
                                                    const int* __restrict__ eig_cl, int nE,
                                                    const int* __restrict__ Windex,
@@ -1484,7 +1657,7 @@ __global__ __launch_bounds__(256) void k_refine_cl(const ClTask* __restrict__ C,
     Werrtmp[slot] = we;
 }
 
-__global__ __launch_bounds__(256) void k_gaps_cl(const ClTask* __restrict__ C,
+__global__ __launch_bounds__(64) void k_gaps_cl(const ClTask* __restrict__ C,
                                                  const int* __restrict__ eig_slot,
                                                  const int* __restrict__ eig_cl, int nE,
                                                  const double* __restrict__ Wtmp,
