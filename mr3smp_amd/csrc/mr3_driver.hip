@@ -257,13 +257,34 @@ int run_pipeline(std::vector<MatDesc>& md, std::vector<MatState>& ms, hipStream_
     int* smat = A.get<int>(N);
     int* isplit = A.get<int>(N);
     BlockRec* blk = A.get<BlockRec>(N);
+    int* tcount = A.get<int>(N);
     if (!A.ok) return 2;
     MR3_CHECK(hipMemcpyAsync(dmd, md.data(), sizeof(MatDesc) * nmat, hipMemcpyHostToDevice, st));
 
     LAUNCH(k_prep, dim3(nmat), dim3(256), dmd, dms, D, E, E2, gers, Dcopy,
                        E2copy, isplit, blk, W, Werr, Wcoarse, iblock, Windex, smat);
+    MR3_CHECK(hipGetLastError());
+    ms.resize(nmat);
+    MR3_CHECK(hipMemcpyAsync(ms.data(), dms, sizeof(MatState) * nmat, hipMemcpyDeviceToHost, st));
+    MR3_CHECK(hipStreamSynchronize(st));
+    {
+        // blocks of the split, for the shared top of the coarse bisection
+        std::vector<int> plist;
+        for (int k = 0; k < nmat; ++k) {
+            if (ms[k].done || ms[k].maxbl < 128) continue;
+            for (int j = 0; j < ms[k].nsplit; ++j) plist.push_back(md[k].off + j);
+        }
+        if (!plist.empty()) {
+            int* dpl = A.get<int>(plist.size());
+            if (!A.ok) return 2;
+            MR3_CHECK(hipMemcpyAsync(dpl, plist.data(), sizeof(int) * plist.size(),
+                                     hipMemcpyHostToDevice, st));
+            LAUNCH(k_coarse_tree, dim3((unsigned)plist.size()), dim3(256), dmd, dms, D, E2, blk,
+                   smat, dpl, (int)plist.size(), tcount);
+        }
+    }
     LAUNCH(k_coarse, dim3(cdiv(N, 256)), dim3(256), dmd, dms, D, E2, blk, smat,
-                       iblock, Windex, W, Werr, Wcoarse, N);
+                       iblock, Windex, tcount, W, Werr, Wcoarse, N);
     LAUNCH(k_discard, dim3(cdiv(nmat, 64)), dim3(64), dmd, dms, blk, W, Werr,
                        Wcoarse, iblock, Windex, nmat);
     MR3_CHECK(hipGetLastError());

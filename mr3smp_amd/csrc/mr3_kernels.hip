@@ -531,7 +531,72 @@ __global__ __launch_bounds__(PREP_NT) void k_prep(const MatDesc* __restrict__ md
 // ---------------------------------------------------------------------------------
 // k_coarse: eigenvalue approximations by bisection on T, one lane per wanted
 // eigenvalue (dlarrd/dlaebz, odrrd.f:415-574, to RELTOL = bsrtol).
+//
+// Every eigenvalue of a block starts from the same interval, and a bisection step's
+// midpoint depends only on the interval, so the first steps of all eigenvalues of a
+// block visit the nodes of one binary tree: node t (root 1, children 2t for "the
+// eigenvalue lies left of the midpoint", 2t + 1 otherwise).  k_coarse_tree counts the
+// eigenvalues below the midpoint of every node of the top coarse_tree_depth levels
+// once per block; k_coarse then takes those counts from the table and sweeps only
+// below it.  Each lane's sequence of intervals, counts and tests is the one of its own
+// bisection, so the results are the same bit for bit.
 // ---------------------------------------------------------------------------------
+__device__ __forceinline__ int coarse_tree_depth(int nbl) {
+    return nbl >= 128 ? (31 - __clz(nbl)) - 1 : 0;
+}
+
+// Initial interval of a block's coarse bisection: dlarrd called on the block alone
+// (mrrr_val.c:1291-1295) with its own fudged bounds (odrrd.f:515-519, FUDGE = 2).
+__device__ __forceinline__ void coarse_bounds(const BlockRec& br, int nbl, double pivmin,
+                                              double* left, double* right) {
+    const double tnormb = fmax(fabs(br.glraw), fabs(br.guraw));
+    *left = br.glraw - 2.0 * tnormb * kEps * nbl - 2.0 * pivmin;
+    *right = br.guraw + 2.0 * tnormb * kEps * nbl + 2.0 * pivmin;
+}
+
+// Counts at the tree nodes 1 .. 2^depth - 1 of every block of at least 128 rows; one
+// workgroup per block, tcount[off + bbeg + t - 1] = count at node t.
+__global__ __launch_bounds__(256) void k_coarse_tree(const MatDesc* __restrict__ md,
+                                                     const MatState* __restrict__ ms,
+                                                     const double* __restrict__ D,
+                                                     const double* __restrict__ E2,
+                                                     const BlockRec* __restrict__ blk,
+                                                     const int* __restrict__ smat,
+                                                     const int* __restrict__ blist, int nblist,
+                                                     int* tcount) {
+    const int ib = blockIdx.x;
+    if (ib >= nblist) return;
+    const int gb = blist[ib];
+    const int k = smat[gb];
+    const MatState& S = ms[k];
+    const int off = md[k].off;
+    if (S.done || gb - off >= S.nsplit) return;
+    const BlockRec br = blk[gb];
+    const int nbl = br.bend - br.bbeg + 1;
+    if (br.wend < br.wbeg) return;
+    const int depth = coarse_tree_depth(nbl);
+    if (depth == 0) return;
+    const double pivmin = S.pivmin;
+    const double* d = D + off + br.bbeg;
+    const double* e2 = E2 + off + br.bbeg;
+    double left0, right0;
+    coarse_bounds(br, nbl, pivmin, &left0, &right0);
+    int sweeps = 0;
+    for (int t = threadIdx.x + 1; t < (1 << depth); t += blockDim.x) {
+        const int dt = 31 - __clz(t);
+        double l = left0, r = right0;
+        for (int b = dt - 1; b >= 0; --b) {
+            const double mid = 0.5 * (l + r);
+            if ((t >> b) & 1) l = mid;
+            else r = mid;
+        }
+        const double mid = 0.5 * (l + r);
+        tcount[off + br.bbeg + t - 1] = count_T_ijob2_w(nbl, d, e2, mid, pivmin);
+        ++sweeps;
+    }
+    add_steps(kStepCoarse, (unsigned long long)sweeps * nbl);
+}
+
 __global__ __launch_bounds__(256) void k_coarse(const MatDesc* __restrict__ md,
                                                 const MatState* __restrict__ ms,
                                                 const double* __restrict__ D,
@@ -539,7 +604,8 @@ __global__ __launch_bounds__(256) void k_coarse(const MatDesc* __restrict__ md,
                                                 const BlockRec* __restrict__ blk,
                                                 const int* __restrict__ smat,
                                                 const int* __restrict__ iblock,
-                                                const int* __restrict__ Windex, double* W,
+                                                const int* __restrict__ Windex,
+                                                const int* __restrict__ tcount, double* W,
                                                 double* Werr, double* Wcoarse, int ntot) {
     const int gidx = blockIdx.x * blockDim.x + threadIdx.x;
     if (gidx >= ntot) return;
@@ -555,24 +621,28 @@ __global__ __launch_bounds__(256) void k_coarse(const MatDesc* __restrict__ md,
     const double* d = D + off + br.bbeg;
     const double* e2 = E2 + off + br.bbeg;
     const int i = Windex[gidx];
-    // dlarrd called on the block alone (mrrr_val.c:1291-1295): its own fudged
-    // bounds (odrrd.f:515-519 with FUDGE = 2) and tolerances (odrrd.f:337-341).
-    const double tnormb = fmax(fabs(br.glraw), fabs(br.guraw));
-    double left = br.glraw - 2.0 * tnormb * kEps * nbl - 2.0 * pivmin;
-    double right = br.guraw + 2.0 * tnormb * kEps * nbl + 2.0 * pivmin;
+    // the block's own bounds and tolerances (odrrd.f:337-341)
+    double left, right;
+    coarse_bounds(br, nbl, pivmin, &left, &right);
     const double atoli = 2.0 * 2.0 * kSafmin + 2.0 * 2.0 * pivmin;
     const double rtoli = S.bsrtol;
     const int itmax = (int)((log(right - left + pivmin) - log(pivmin)) / log(2.0)) + 2;
-    int nl = 0, nr = nbl, sweeps = 0;
+    const int depth = coarse_tree_depth(nbl);
+    const int* tc = tcount + off + br.bbeg - 1;   // tc[t]: count at tree node t
+    int nl = 0, nr = nbl, sweeps = 0, t = 1;
     for (int it = 0; it < itmax; ++it) {
         double tmp1 = fabs(right - left), tmp2 = fmax(fabs(right), fabs(left));
         if (tmp1 < fmax(fmax(atoli, pivmin), rtoli * tmp2)) break;
         double mid = 0.5 * (left + right);
-        int c = count_T_ijob2_w(nbl, d, e2, mid, pivmin);
-        ++sweeps;
+        int c;
+        if (it < depth) c = tc[t];
+        else {
+            c = count_T_ijob2_w(nbl, d, e2, mid, pivmin);
+            ++sweeps;
+        }
         c = min(nr, max(nl, c));
-        if (c >= i) { right = mid; nr = c; }
-        else { left = mid; nl = c; }
+        if (c >= i) { right = mid; nr = c; t = 2 * t; }
+        else { left = mid; nl = c; t = 2 * t + 1; }
     }
     add_steps(kStepCoarse, (unsigned long long)sweeps * nbl);
     const double w = 0.5 * (left + right);
