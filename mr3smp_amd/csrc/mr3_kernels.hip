@@ -1449,6 +1449,51 @@ __global__ __launch_bounds__(64) void k_arena_assign(ClTask* C, int nC, double* 
 // dlarrb calls are independent: each runs with a zero gap); lane 0 then runs dlarrf
 // (odrrf.f:140-384); all lanes apply the shift and form DL, DLL.
 // ---------------------------------------------------------------------------------
+// One dlarrf trial (odrrf.f:206-225 and 231-249): L+ D+ L+^T = L D L^T - sig I, its
+// largest |D+| and whether a pivot hit the floor or the growth is NaN.  The loads of
+// the next kU steps are issued before the current ones are computed.
+template <class P>
+__device__ __forceinline__ void rrf_trial(int n, P d, P l, P ld, double sig, double pivmin,
+                                          double& mx, int& nan) {
+    double s = -sig;
+    double dp = d[0] + s;
+    nan = 0;
+    if (fabs(dp) < pivmin) { dp = -pivmin; nan = 1; }
+    mx = fabs(dp);
+    double a[kU], b[kU], c[kU], an[kU], bn[kU], cn[kU];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) { a[u] = ld[u]; b[u] = l[u]; c[u] = d[u + 1]; }
+    for (int i0 = 0; i0 < n - 1; i0 += kU) {
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            const int i = i0 + kU + u;
+            an[u] = ld[i]; bn[u] = l[i]; cn[u] = d[i + 1];
+        }
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            if (i0 + u < n - 1) {
+                const double lp = a[u] / dp;
+                s = s * lp * b[u] - sig;
+                dp = c[u] + s;
+                if (fabs(dp) < pivmin) { dp = -pivmin; nan = 1; }
+                mx = fmax(mx, fabs(dp));
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kU; ++u) { a[u] = an[u]; b[u] = bn[u]; c[u] = cn[u]; }
+    }
+    nan = nan || isnan(mx);
+}
+
+__device__ __forceinline__ void rrf_trial_w(int n, const double* d, const double* l,
+                                            const double* ld, double sig, double pivmin,
+                                            double& mx, int& nan) {
+    if (wave_uniform(d) && wave_uniform(l) && wave_uniform(ld) && wave_uniform(n))
+        rrf_trial(rfl(n), as_const(d), as_const(l), as_const(ld), sig, pivmin, mx, nan);
+    else
+        rrf_trial(n, d, l, ld, sig, pivmin, mx, nan);
+}
+
 __device__ int dlarrf_lane(int n, const double* __restrict__ d, const double* __restrict__ l,
                            const double* __restrict__ ld, const double* w, const double* wgap,
                            const double* werr, int clsize, double spdiam, double clgapl,
@@ -1477,9 +1522,8 @@ __device__ int dlarrf_lane(int n, const double* __restrict__ d, const double* __
     const double growthbound = kMaxGrowth1 * spdiam;
     double max1 = 0.0, max2 = 0.0;
     int sweeps = 0;
-    // L+ D+ L+^T = L D L^T - sig I (odrrf.f:206-225 and 231-249): growth and pivot
-    // floor hits; the factors are stored only when store is set (the refined growth
-    // test below reads them; the chosen factorisation is written by the caller)
+    // the trial with its factors stored, for the refined growth test below (the chosen
+    // factorisation itself is written by the caller)
     auto trial = [&](double sig, double* Dst, double* Lst, bool store, double& mx, int& nan) {
         double s = -sig;
         double dp = d[0] + s;
@@ -1503,10 +1547,10 @@ __device__ int dlarrf_lane(int n, const double* __restrict__ d, const double* __
         ldelta = fmin(ldmax, ldelta);
         rdelta = fmin(rdmax, rdelta);
         ++sweeps;
-        trial(lsigma, dplus, lplus, false, max1, sawnan1);
+        rrf_trial_w(n, d, l, ld, lsigma, pivmin, max1, sawnan1);
         if (forcer || (max1 <= growthbound && !sawnan1)) { *sigma = lsigma; shift = 1; break; }
         ++sweeps;
-        trial(rsigma, work, work + n, false, max2, sawnan2);
+        rrf_trial_w(n, d, l, ld, rsigma, pivmin, max2, sawnan2);
         if (forcer || (max2 <= growthbound && !sawnan2)) { *sigma = rsigma; shift = 2; break; }
         if (!(sawnan1 && sawnan2)) {
             if (!sawnan1) {
@@ -1591,6 +1635,67 @@ __global__ __launch_bounds__(64) void k_endref(const ClTask* __restrict__ C, int
     Werrtmp[slot] = we;
 }
 
+// The chosen factorisation L+ D+ L+^T = L D L^T - tau I of each lane's cluster,
+// recomputed with the operations of the accepted trial into cD (D+) and cL (L+).
+// Each lane writes chunks of 16 values to LDS; the wave then stores 16 contiguous
+// rows of four clusters per instruction, so no cache line is left partly written.
+// The loads run kU steps ahead of the arithmetic.
+template <class P>
+__device__ __forceinline__ void rrf_store(int n, int nmax, P d, P l, P ld, double tau,
+                                          double pivmin, bool valid, int lane,
+                                          double (*bD)[65], double (*bL)[65], double** dD,
+                                          double** dL, const int* nn) {
+    double s = -tau;
+    double dp = d[0] + s;
+    if (fabs(dp) < pivmin) dp = -pivmin;
+    const int sub = lane & 15, cq = lane >> 4;
+    double a[kU], b[kU], c[kU], an[kU], bn[kU], cn[kU];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) { a[u] = ld[u]; b[u] = l[u]; c[u] = d[u + 1]; }
+    for (int i0 = 0; i0 < nmax; i0 += 16) {
+#pragma unroll
+        for (int h = 0; h < 16; h += kU) {
+            // the wave runs to its longest block: a lane loads only while its own
+            // sweep goes on (then at most kU + 1 entries past its block)
+            if (i0 + h + kU < n) {
+#pragma unroll
+                for (int u = 0; u < kU; ++u) {
+                    const int i = i0 + h + kU + u;
+                    an[u] = ld[i]; bn[u] = l[i]; cn[u] = d[i + 1];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < kU; ++u) {
+                const int i = i0 + h + u;
+                if (i < n) {
+                    bD[h + u][lane] = dp;
+                    if (i < n - 1) {
+                        const double lp = a[u] / dp;
+                        bL[h + u][lane] = lp;
+                        s = s * lp * b[u] - tau;
+                        dp = c[u] + s;
+                        if (fabs(dp) < pivmin) dp = -pivmin;
+                    }
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < kU; ++u) { a[u] = an[u]; b[u] = bn[u]; c[u] = cn[u]; }
+        }
+        __syncthreads();
+        for (int cg = 0; cg < 16; ++cg) {
+            const int col = cg * 4 + cq;
+            const int row = i0 + sub;
+            const int nc = nn[col];
+            if (row < nc) {
+                dD[col][row] = bD[sub][col];
+                if (row < nc - 1) dL[col][row] = bL[sub][col];
+            }
+        }
+        __syncthreads();
+    }
+    (void)valid;
+}
+
 // New representation of every cluster by dlarrf (odrrf.f:140-384), one lane per
 // cluster.  The left-shift factorisation goes to cD/cL, the right-shift one to the
 // cDL/cDLL space; k_newrrr_fin keeps the chosen one.
@@ -1635,39 +1740,20 @@ __global__ __launch_bounds__(64) void k_dlarrf(ClTask* C, int nC, double* Wsh, d
     // clusters per instruction, so no cache line is left partly written.
     int nmax = n;
     for (int o = 32; o > 0; o >>= 1) nmax = max(nmax, __shfl_xor(nmax, o));
-    double s = -tau, dp = 0.0;
-    if (valid) {
-        dp = pd[0] + s;
-        if (fabs(dp) < pivmin) dp = -pivmin;
+    // lanes past the last cluster sweep the last cluster's parent (nothing is stored
+    // for them), so every lane reads valid memory
+    if (!valid) {
+        const ClTask& tl = C[nC - 1];
+        pd = tl.pD; pl = tl.pL; pld = tl.pDL;
+        n = tl.bsize;
+        tau = 0.0;
+        pivmin = ms[tl.mat].pivmin;
     }
-    const int sub = lane & 15, cq = lane >> 4;
-    for (int i0 = 0; i0 < nmax; i0 += 16) {
-#pragma unroll 4
-        for (int k = 0; k < 16; ++k) {
-            const int i = i0 + k;
-            if (valid && i < n) {
-                bD[k][lane] = dp;
-                if (i < n - 1) {
-                    const double lp = pld[i] / dp;
-                    bL[k][lane] = lp;
-                    s = s * lp * pl[i] - tau;
-                    dp = pd[i + 1] + s;
-                    if (fabs(dp) < pivmin) dp = -pivmin;
-                }
-            }
-        }
-        __syncthreads();
-        for (int cg = 0; cg < 16; ++cg) {
-            const int col = cg * 4 + cq;
-            const int row = i0 + sub;
-            const int nc = nn[col];
-            if (row < nc) {
-                dD[col][row] = bD[sub][col];
-                if (row < nc - 1) dL[col][row] = bL[sub][col];
-            }
-        }
-        __syncthreads();
-    }
+    if (wave_uniform(pd) && wave_uniform(pl) && wave_uniform(pld) && wave_uniform(n))
+        rrf_store(rfl(n), nmax, as_const(pd), as_const(pl), as_const(pld), tau, pivmin, valid, lane,
+                  bD, bL, dD, dL, nn);
+    else
+        rrf_store(n, nmax, pd, pl, pld, tau, pivmin, valid, lane, bD, bL, dD, dL, nn);
 }
 
 // Accumulates the shift, forms DL and DLL from the chosen factorisation and moves
