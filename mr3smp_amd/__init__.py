@@ -17,7 +17,7 @@ import os
 
 import numpy as np
 
-__all__ = ["lib", "mrrr", "solve_batch", "Problem", "Server", "Request", "build", "LIB_PATH"]
+__all__ = ["lib", "mrrr", "solve_batch", "prepare_batch", "run_batch", "Problem", "Server", "Request", "build", "LIB_PATH"]
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libmr3hip.so")
@@ -140,8 +140,22 @@ def solve_batch(problems, stream=None):
     ``Zsupp`` (int32, 2*mcols).  Missing outputs are allocated.  Returns the
     Problem array (scalar outputs) and the list of dicts with the outputs filled.
     """
+    arr, keep = prepare_batch(problems)
+    return run_batch(arr, stream), arr, keep
+
+
+def run_batch(arr, stream=None):
+    """One mr3_solve_batch call on a Problem array from prepare_batch (the inputs
+    it points at are read again, the outputs rewritten); returns the return code."""
+    st = None if stream is None else ctypes.c_void_p(stream.cuda_stream)
+    return lib().mr3_solve_batch(arr, len(arr), st)
+
+
+def prepare_batch(problems):
+    """The Problem array of solve_batch, and the dicts with their outputs allocated,
+    without solving."""
     import torch
-    L = lib()
+    lib()
     arr = (Problem * len(problems))()
     keep = []
     for i, p in enumerate(problems):
@@ -171,9 +185,7 @@ def solve_batch(problems, stream=None):
         q.Z = p["Z"].data_ptr() if wantz else None
         q.Zsupp = p["Zsupp"].data_ptr() if wantz else None
         keep.append(p)
-    st = None if stream is None else ctypes.c_void_p(stream.cuda_stream)
-    rc = L.mr3_solve_batch(arr, len(problems), st)
-    return rc, arr, keep
+    return arr, keep
 
 
 def profile_enable(on=True, counters=True):

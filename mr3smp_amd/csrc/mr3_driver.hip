@@ -38,6 +38,19 @@ namespace mr3 {
 
 static inline unsigned cdiv(long long a, long long b) { return (unsigned)((a + b - 1) / b); }
 
+// Lanes per eigenvalue for the cluster refinements of a level with `items`
+// eigenvalues: 1 (one-lane kernels) unless a group of 4 or more lanes per eigenvalue
+// keeps the launch within kGroupLanes lanes; then the largest such power of two up
+// to a wave.  Multisection with G lanes does (G - 1) / log2(G) times the sweeps of
+// one-point bisection in 1 / log2(G) of the sequential rounds.
+constexpr long long kGroupLanes = 262144;
+static int group_lanes(long long items) {
+    if (items * 4 > kGroupLanes) return 1;
+    int G = 4;
+    while (G < 64 && items * 2 * G <= kGroupLanes) G *= 2;
+    return G;
+}
+
 // ---- profiling: HIP events around every launch (on the launch stream) and the
 // on-device step counters, accumulated over the solves run while enabled.
 struct ProfEntry {
@@ -118,14 +131,17 @@ struct CallEvents {
     } while (0)
 #define LAUNCH(KERN, GRID, BLOCK, ...) LAUNCH_ON(st, KERN, GRID, BLOCK, __VA_ARGS__)
 
-// The singleton iterations of a tree level run on a second stream of the calling
+// The singleton iterations of a tree level run on a side stream of the calling
 // thread, beside the cluster work of the same level (they touch disjoint eigenvalue
 // slots, Z columns and representations, as the reference's S- and C-tasks do).
-// Two of them take consecutive levels in turn, so that the last few singletons of
-// level L (the ones that need all ten iterations) run beside the bulk of level L + 1.
+// kSide of them take consecutive levels in turn, so that the last few singletons of
+// level L (the ones that need all ten iterations) run beside the bulk of the next
+// levels.  With the caller's stream that is four streams: the hardware queues a
+// process gets by default.
+constexpr int kSide = 3;
 static hipStream_t side_stream(int k) {
-    thread_local hipStream_t s[2] = {nullptr, nullptr};
-    thread_local bool tried[2] = {false, false};
+    thread_local hipStream_t s[kSide] = {};
+    thread_local bool tried[kSide] = {};
     if (!tried[k]) {
         tried[k] = true;
         if (hipStreamCreateWithFlags(&s[k], hipStreamNonBlocking) != hipSuccess) s[k] = nullptr;
@@ -349,8 +365,8 @@ int run_pipeline(std::vector<MatDesc>& md, std::vector<MatState>& ms, hipStream_
             int* sbc;
             int* BI;
             double* scratch;
-        } ss[2];
-        for (int k = 0; k < 2; ++k) {
+        } ss[kSide];
+        for (int k = 0; k < kSide; ++k) {
             ss[k].wtasks = A.get<WaveTask>(capT);
             ss[k].scnt = A.get<LevelCounters>(1);
             ss[k].ST = A.get<SState>(capS);
@@ -365,18 +381,18 @@ int run_pipeline(std::vector<MatDesc>& md, std::vector<MatState>& ms, hipStream_
         if (!A.ok) return 2;
         for (int k = 0; k < kLB; ++k)
             MR3_CHECK(hipMemsetAsync(bcnt2[k], 0, sizeof(int) * (size_t)N, st));
-        MR3_CHECK(hipMemsetAsync(ss[0].sbc, 0, sizeof(int) * (size_t)N, st));
-        MR3_CHECK(hipMemsetAsync(ss[1].sbc, 0, sizeof(int) * (size_t)N, st));
+        for (int k = 0; k < kSide; ++k)
+            MR3_CHECK(hipMemsetAsync(ss[k].sbc, 0, sizeof(int) * (size_t)N, st));
         // eigenvector scratch: two checkpoint arrays of maxbl/kChunk + 2 carries per
         // lane, 64 lanes per wave
         const size_t ck_per_lane = (size_t)2 * (maxbl / kChunk + 2);
         const size_t per_wave = ck_per_lane * 64 * sizeof(double);
-        const size_t budget = singleton_scratch_budget() / 2;   // two side streams
+        const size_t budget = singleton_scratch_budget() / kSide;
         int max_waves = (int)std::max<size_t>(1, std::min<size_t>(budget / per_wave,
                                                                   (size_t)env_int("MR3_SWAVES", 8192)));
         const int want_waves = capT;
         const int nwaves = std::min(max_waves, want_waves);
-        for (int k = 0; k < 2; ++k) ss[k].scratch = A.get<double>((size_t)nwaves * ck_per_lane * 64);
+        for (int k = 0; k < kSide; ++k) ss[k].scratch = A.get<double>((size_t)nwaves * ck_per_lane * 64);
         if (!A.ok) return 2;
         LevelCounters hc;
         auto reset = [&]() { return hipMemsetAsync(dcnt, 0, sizeof(LevelCounters), st); };
@@ -394,8 +410,11 @@ int run_pipeline(std::vector<MatDesc>& md, std::vector<MatState>& ms, hipStream_
         int cur = 0;
         const bool trace = getenv("MR3_TRACE_LEVELS") != nullptr;   // per-level timing to stderr
         unsigned long long prev_ticks = 0, prev_tasks = 0;
-        hipStream_t s2p[2] = {side_stream(0), side_stream(1)};
-        if (!s2p[0] || !s2p[1]) return 2;
+        hipStream_t s2p[kSide];
+        for (int k = 0; k < kSide; ++k) {
+            s2p[k] = side_stream(k);
+            if (!s2p[k]) return 2;
+        }
         // end of the singleton phase that last read list buffer k
         hipEvent_t evS[kLB] = {};
         for (int k = 0; k < kLB; ++k) MR3_CHECK(hipEventCreateWithFlags(&evS[k], hipEventDisableTiming));
@@ -412,24 +431,22 @@ int run_pipeline(std::vector<MatDesc>& md, std::vector<MatState>& ms, hipStream_
             ~PinFree() { pinned_counters_put(p); }
         } pin_free{hpin};
         struct SideSync {   // no buffer or event goes away while a side stream may use it
-            hipStream_t a, b;
+            const hipStream_t* s;
             ~SideSync() {
-                if (a) (void)hipStreamSynchronize(a);
-                if (b) (void)hipStreamSynchronize(b);
+                for (int k = 0; k < kSide; ++k) (void)hipStreamSynchronize(s[k]);
             }
-        } side_sync{s2p[0], s2p[1]};
+        } side_sync{s2p};
         bool have_evS[kLB] = {};
         // the side streams start after everything so far on the main stream
         hipEvent_t evStart = nullptr;
         MR3_CHECK(hipEventCreateWithFlags(&evStart, hipEventDisableTiming));
         MR3_CHECK(hipEventRecord(evStart, st));
-        MR3_CHECK(hipStreamWaitEvent(s2p[0], evStart, 0));
-        MR3_CHECK(hipStreamWaitEvent(s2p[1], evStart, 0));
+        for (int k = 0; k < kSide; ++k) MR3_CHECK(hipStreamWaitEvent(s2p[k], evStart, 0));
         (void)hipEventDestroy(evStart);
         for (int level = 0;; ++level) {
             const int nS = hc.nS, nC = hc.nC, nE = hc.nEig;
             const long long arena_need = hc.arena;
-            const int p = level & 1;           // side stream and its buffer set
+            const int p = level % kSide;       // side stream and its buffer set
             const int lb = level % kLB;        // this level's list buffer
             const int lbn = (level + 1) % kLB; // the next level's
             // 1. cluster work of this level and the next level's classification, on the
@@ -442,13 +459,23 @@ int run_pipeline(std::vector<MatDesc>& md, std::vector<MatState>& ms, hipStream_
                 LAUNCH(k_zero, dim3(1), dim3(64), (int*)bump, 2, (int*)dcnt,
                        (int)(sizeof(LevelCounters) / sizeof(int)));
                 LAUNCH(k_arena_assign, dim3(cdiv(nC, 64)), dim3(64), Cl[cur], nC, arena, bump);
-                LAUNCH(k_endref, dim3(cdiv(2LL * nC, 64)), dim3(64), Cl[cur], nC, Windex, Wsh, Werr,
-                       dms, Wtmp, Werrtmp);
+                const int Ge = group_lanes(2LL * nC);
+                if (Ge == 1)
+                    LAUNCH(k_endref, dim3(cdiv(2LL * nC, 64)), dim3(64), Cl[cur], nC, Windex, Wsh,
+                           Werr, dms, Wtmp, Werrtmp);
+                else
+                    LAUNCH(k_endref_grp, dim3(cdiv(2LL * nC, 64 / Ge)), dim3(64), Cl[cur], nC, Ge,
+                           Windex, Wsh, Werr, dms, Wtmp, Werrtmp);
                 LAUNCH(k_dlarrf, dim3(cdiv(nC, 64)), dim3(64), Cl[cur], nC, Wsh, Werr, Wgap, Wtmp,
                        Werrtmp, dms);
                 LAUNCH(k_newrrr_fin, dim3(nC), dim3(64), Cl[cur], nC, Wsh, Werr);
-                LAUNCH(k_refine_cl, dim3(cdiv(nE, 64)), dim3(64), Cl[cur], es[cur], ec[cur], nE,
-                       Windex, Wsh, Werr, Wgap, dms, Wtmp, Werrtmp);
+                const int Gr = group_lanes(nE);
+                if (Gr == 1)
+                    LAUNCH(k_refine_cl, dim3(cdiv(nE, 64)), dim3(64), Cl[cur], es[cur], ec[cur], nE,
+                           Windex, Wsh, Werr, Wgap, dms, Wtmp, Werrtmp);
+                else
+                    LAUNCH(k_refine_cl_grp, dim3(cdiv(nE, 64 / Gr)), dim3(64), Cl[cur], es[cur],
+                           ec[cur], nE, Gr, Windex, Wsh, Werr, Wgap, dms, Wtmp, Werrtmp);
                 LAUNCH(k_gaps_cl, dim3(cdiv(nE, 64)), dim3(64), Cl[cur], es[cur], ec[cur], nE,
                        Wtmp, Werrtmp, Wsh, Werr, Wgap, W);
                 // the next level's singleton list reuses the buffers of the phase four

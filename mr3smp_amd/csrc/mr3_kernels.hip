@@ -2566,68 +2566,31 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
     }
 }
 
-// The bisection fallback of the singletons that need it in this iteration
-// (process_singleton.c via dlarrb, odrrb.f:135-299) by multisection: a group of G
-// lanes (G a power of two) serves one singleton.  The bracket expansion evaluates G
-// consecutive candidates of the reference's sequence (left - back, back doubled) per
-// sweep and keeps the first that brackets; each bisection round evaluates the G - 1
-// midpoints of the next log2(G) halvings of the interval (a binary tree, computed
-// with the reference's operations) in one sweep, then walks the tree with the
-// reference's convergence test.  The walk takes the same path and the same midpoints
-// as one-point bisection, so w and werr are bit-identical to dlarrb.
-__device__ __forceinline__ void s_bisect_one(int b, int nb, int G, const int* __restrict__ BI,
-                                             const STask* __restrict__ T,
-                                             const MatState* __restrict__ ms,
-                                             const int* __restrict__ Windex, SState* ST);
-
-// Grid-stride over the bisection list; the group size G is chosen from the list
-// length on the device: as many lanes per singleton as keep about 128K lanes busy.
-__global__ __launch_bounds__(64) void k_s_bisect(const int* __restrict__ BI,
-                                                 const LevelCounters* __restrict__ cnt,
-                                                 const STask* __restrict__ T,
-                                                 const MatState* __restrict__ ms,
-                                                 const int* __restrict__ Windex, SState* ST) {
-    const int nb = __builtin_amdgcn_readfirstlane(cnt->nEig);
-    if (nb == 0) return;
-    int G = 64;
-    while (G > 2 && (long long)nb * G > 131072) G >>= 1;
-    const int per = SW / G;
-    for (int b0 = blockIdx.x * per; b0 < nb; b0 += gridDim.x * per)
-        s_bisect_one(b0 + (int)threadIdx.x / G, nb, G, BI, T, ms, Windex, ST);
-}
-
-__device__ __forceinline__ void s_bisect_one(int b, int nb, int G, const int* __restrict__ BI,
-                                             const STask* __restrict__ T,
-                                             const MatState* __restrict__ ms,
-                                             const int* __restrict__ Windex, SState* ST) {
-    const int lane = threadIdx.x;
-    const int grp = lane / G, sub = lane % G;
-    const bool valid = b < nb;
-    const int idx = BI[min(b, nb - 1)];
-    const STask t = T[idx];
-    SState q = ST[idx];
-    const int n = t.bsize;
-    const int i = Windex[t.slot];
-    const double pivmin = ms[t.mat].pivmin;
-    const double spdiam = t.bl_spdiam;
-    const double rtol1 = 0.0, rtol2 = 2.0 * DBL_EPSILON, gap = 0.0;
+// dlarrb (odrrb.f:135-299) for one eigenvalue by a group of G lanes (G a power of
+// two, 4 or more; the group is lanes gbase .. gbase + G - 1 of the wave and every lane
+// of it passes the same arguments).  The bracket expansion evaluates G consecutive
+// candidates of the reference's sequence (left - back, back doubled) per sweep and
+// keeps the first that brackets; each bisection round evaluates the G - 1 midpoints
+// of the next log2(G) halvings of the interval (a binary tree, computed with the
+// reference's operations) in one sweep, then walks the tree with the reference's
+// convergence test.  The walk takes the same path and the same midpoints as one-point
+// bisection, so w and werr are bit-identical to dlarrb1.  count(x) is the negcount
+// at x; lanes with valid false replay a valid group and their results are not used.
+// Returns the reference's one-point sweeps; *rounds_out gets the group's sweeps.
+template <class CountF>
+__device__ __forceinline__ long long dlarrb_grp(int G, int sub, int gbase, bool valid,
+                                                CountF count, int i, double rtol1,
+                                                double rtol2, double gap, double pivmin,
+                                                double spdiam, double& w, double& werr,
+                                                int* rounds_out) {
     const int maxitr = (int)((log(spdiam + pivmin) - log(pivmin)) / log(2.0)) + 2;
     const double mnwdth = 2.0 * pivmin;
-    int r = q.r;
-    if (r < 1 || r > n) r = n;
-    const double* d = t.D;
-    const double* lld = t.DLL;
-    const bool uni = wave_uniform(d) && wave_uniform(lld) && wave_uniform(n) && wave_uniform(r);
-    auto count = [&](double x) -> int {
-        return uni ? dlaneg(rfl(n), as_const(d), as_const(lld), x, rfl(r)) : dlaneg(n, d, lld, x, r);
-    };
-    const int gbase = grp * G;
     long long sweeps = 0;   // one-point sweeps of the reference (algorithmic work)
     int rounds = 0;
     // bracket expansion to the left and to the right (odrrb.f:176-196)
-    double left = q.wsh - q.werr, right = q.wsh + q.werr;
+    double left = w - werr, right = w + werr;
     for (int side = 0; side < 2; ++side) {
-        double x = side == 0 ? left : right, back = q.werr;
+        double x = side == 0 ? left : right, back = werr;
         int done = 0, g0 = 0;
         while (__any(valid && !done)) {
             // candidate sub of this batch: sub more expansion steps from x
@@ -2664,7 +2627,6 @@ __device__ __forceinline__ void s_bisect_one(int b, int nb, int G, const int* __
         if (side == 0) left = x;
         else right = x;
     }
-    double w = q.wsh, we = q.werr;
     double width = 0.5 * fabs(left - right);
     double tmp = fmax(fabs(left), fabs(right));
     double cvrgd = fmax(rtol1 * gap, rtol2 * tmp);
@@ -2712,8 +2674,143 @@ __device__ __forceinline__ void s_bisect_one(int b, int nb, int G, const int* __
     }
     if (!conv) {
         w = 0.5 * (left + right);
-        we = right - w;
+        werr = right - w;
     }
+    *rounds_out = rounds;
+    return sweeps;
+}
+
+// The cluster refinements of levels with few clusters run with a group of G lanes
+// per eigenvalue (G = 4 .. 64, chosen on the host from the level's counts): a lane
+// per eigenvalue would leave them a long chain of dependent sweeps on few waves, and
+// they gate the next level.  Same results as k_endref and k_refine_cl.
+__global__ __launch_bounds__(64) void k_endref_grp(const ClTask* __restrict__ C, int nC, int G,
+                                                   const int* __restrict__ Windex,
+                                                   const double* __restrict__ Wsh,
+                                                   const double* __restrict__ Werr,
+                                                   const MatState* __restrict__ ms, double* Wtmp,
+                                                   double* Werrtmp) {
+    const int lane = threadIdx.x;
+    const int sub = lane % G, gbase = lane - sub;
+    const int ne = 2 * nC;
+    const int e = blockIdx.x * (SW / G) + lane / G;
+    const bool valid = e < ne;
+    const int ee = min(e, ne - 1);
+    const ClTask t = C[ee >> 1];
+    const int slot = (ee & 1) ? t.ce : t.cb;
+    const MatState& S = ms[t.mat];
+    double w = Wsh[slot], we = Werr[slot];
+    const int n = t.bsize;
+    const double* d = t.pD;
+    const double* lld = t.pDLL;
+    const bool uni = wave_uniform(d) && wave_uniform(lld) && wave_uniform(n);
+    auto count = [&](double x) -> int {
+        return uni ? dlaneg(rfl(n), as_const(d), as_const(lld), x, rfl(n)) : dlaneg(n, d, lld, x, n);
+    };
+    int rounds = 0;
+    const long long sw = dlarrb_grp(G, sub, gbase, valid, count, Windex[slot], S.rqtol, S.rqtol,
+                                    0.0, S.pivmin, t.bl_spdiam, w, we, &rounds);
+    if (valid && sub == 0) {
+        add_steps(kStepEndref, (unsigned long long)sw * n);
+        Wtmp[slot] = w;
+        Werrtmp[slot] = we;
+    }
+}
+
+__global__ __launch_bounds__(64) void k_refine_cl_grp(const ClTask* __restrict__ C,
+                                                      const int* __restrict__ eig_slot,
+                                                      const int* __restrict__ eig_cl, int nE,
+                                                      int G, const int* __restrict__ Windex,
+                                                      const double* __restrict__ Wsh,
+                                                      const double* __restrict__ Werr,
+                                                      const double* __restrict__ Wgap,
+                                                      const MatState* __restrict__ ms,
+                                                      double* Wtmp, double* Werrtmp) {
+    const int lane = threadIdx.x;
+    const int sub = lane % G, gbase = lane - sub;
+    const int e = blockIdx.x * (SW / G) + lane / G;
+    const bool valid = e < nE;
+    const int ee = min(e, nE - 1);
+    const int slot = eig_slot[ee];
+    const ClTask t = C[eig_cl[ee]];
+    const MatState& S = ms[t.mat];
+    double w = Wsh[slot], we = Werr[slot];
+    const double rgap = Wgap[slot];
+    const double lgap = (slot == t.cb) ? rgap : Wgap[slot - 1];
+    const int n = t.bsize;
+    const double* d = t.cD;
+    const double* lld = t.cDLL;
+    const bool uni = wave_uniform(d) && wave_uniform(lld) && wave_uniform(n);
+    auto count = [&](double x) -> int {
+        return uni ? dlaneg(rfl(n), as_const(d), as_const(lld), x, rfl(n)) : dlaneg(n, d, lld, x, n);
+    };
+    int rounds = 0;
+    const long long sw = dlarrb_grp(G, sub, gbase, valid, count, Windex[slot], S.rtol1, S.rtol2,
+                                    fmin(lgap, rgap), S.pivmin, t.bl_spdiam, w, we, &rounds);
+    if (valid && sub == 0) {
+        add_steps(kStepRefineCl, (unsigned long long)sw * n);
+        Wtmp[slot] = w;
+        Werrtmp[slot] = we;
+    }
+}
+
+// The bisection fallback of the singletons that need it in this iteration
+// (process_singleton.c via dlarrb, odrrb.f:135-299) by multisection: a group of G
+// lanes (G a power of two) serves one singleton.  The bracket expansion evaluates G
+// consecutive candidates of the reference's sequence (left - back, back doubled) per
+// sweep and keeps the first that brackets; each bisection round evaluates the G - 1
+// midpoints of the next log2(G) halvings of the interval (a binary tree, computed
+// with the reference's operations) in one sweep, then walks the tree with the
+// reference's convergence test.  The walk takes the same path and the same midpoints
+// as one-point bisection, so w and werr are bit-identical to dlarrb.
+__device__ __forceinline__ void s_bisect_one(int b, int nb, int G, const int* __restrict__ BI,
+                                             const STask* __restrict__ T,
+                                             const MatState* __restrict__ ms,
+                                             const int* __restrict__ Windex, SState* ST);
+
+// Grid-stride over the bisection list; the group size G is chosen from the list
+// length on the device: as many lanes per singleton as keep about 128K lanes busy.
+__global__ __launch_bounds__(64) void k_s_bisect(const int* __restrict__ BI,
+                                                 const LevelCounters* __restrict__ cnt,
+                                                 const STask* __restrict__ T,
+                                                 const MatState* __restrict__ ms,
+                                                 const int* __restrict__ Windex, SState* ST) {
+    const int nb = __builtin_amdgcn_readfirstlane(cnt->nEig);
+    if (nb == 0) return;
+    int G = 64;
+    while (G > 2 && (long long)nb * G > 131072) G >>= 1;
+    const int per = SW / G;
+    for (int b0 = blockIdx.x * per; b0 < nb; b0 += gridDim.x * per)
+        s_bisect_one(b0 + (int)threadIdx.x / G, nb, G, BI, T, ms, Windex, ST);
+}
+
+__device__ __forceinline__ void s_bisect_one(int b, int nb, int G, const int* __restrict__ BI,
+                                             const STask* __restrict__ T,
+                                             const MatState* __restrict__ ms,
+                                             const int* __restrict__ Windex, SState* ST) {
+    const int lane = threadIdx.x;
+    const int grp = lane / G, sub = lane % G;
+    const bool valid = b < nb;
+    const int idx = BI[min(b, nb - 1)];
+    const STask t = T[idx];
+    SState q = ST[idx];
+    const int n = t.bsize;
+    const int i = Windex[t.slot];
+    const double pivmin = ms[t.mat].pivmin;
+    const double spdiam = t.bl_spdiam;
+    int r = q.r;
+    if (r < 1 || r > n) r = n;
+    const double* d = t.D;
+    const double* lld = t.DLL;
+    const bool uni = wave_uniform(d) && wave_uniform(lld) && wave_uniform(n) && wave_uniform(r);
+    auto count = [&](double x) -> int {
+        return uni ? dlaneg(rfl(n), as_const(d), as_const(lld), x, rfl(r)) : dlaneg(n, d, lld, x, r);
+    };
+    const int gbase = grp * G;
+    double w = q.wsh, we = q.werr;
+    int rounds = 0;
+    const long long sweeps = dlarrb_grp(G, sub, gbase, valid, count, i, 0.0, 2.0 * DBL_EPSILON, 0.0,
+                                        pivmin, spdiam, w, we, &rounds);
     if (valid && sub == 0) {
         add_steps(kStepR1vBis, (unsigned long long)n * (unsigned long long)sweeps);
         q.wsh = w;
