@@ -364,6 +364,7 @@ int run_pipeline(std::vector<MatDesc>& md, std::vector<MatState>& ms, hipStream_
             LevelCounters* cit[2];
             int* sbc;
             int* BI;
+            int* DI;          // dense list of the tail kernel
             double* scratch;
         } ss[kSide];
         for (int k = 0; k < kSide; ++k) {
@@ -377,6 +378,7 @@ int run_pipeline(std::vector<MatDesc>& md, std::vector<MatState>& ms, hipStream_
             }
             ss[k].sbc = A.get<int>(N);
             ss[k].BI = A.get<int>(N);
+            ss[k].DI = A.get<int>(N);
         }
         if (!A.ok) return 2;
         for (int k = 0; k < kLB; ++k)
@@ -530,7 +532,7 @@ int run_pipeline(std::vector<MatDesc>& md, std::vector<MatState>& ms, hipStream_
                     const int b = it & 1;
                     // q.cit[b] was cleared by the previous iteration's k_plan_count
                     LAUNCH_ON(s2, k_s_collect, dim3(g1), dim3(64), ptasks, pcnt, pfirst, psi, Sl,
-                              q.ST, q.sit[b], q.sbc, q.BI, q.cit[b]);
+                              q.ST, q.sit[b], q.sbc, q.BI, q.DI, q.cit[b]);
                     LAUNCH_ON(s2, k_plan_count, dim3(cdiv(nbl, 64)), dim3(64), dbl, nbl, blk, q.sbc,
                               64, q.cit[b], it < kMaxIterRQI + 1 ? q.cit[b ^ 1] : (LevelCounters*)nullptr);
                     LAUNCH_ON(s2, k_plan_s, dim3(cdiv(nbl, 64)), dim3(64), dbl, nbl, smat, dmd, blk,
@@ -539,10 +541,23 @@ int run_pipeline(std::vector<MatDesc>& md, std::vector<MatState>& ms, hipStream_
                               dms, Windex, q.ST);
                     LAUNCH_ON(s2, k_s_iter, dim3(gw), dim3(64), q.tit[b], q.cit[b], 0, q.sit[b], Sl,
                               dmd, dms, Windex, Zidx, W, Wsh, Werr, Wgap, q.ST, q.scratch, maxbl);
+                    LAUNCH_ON(s2, k_s_tail, dim3(std::min(nS, 1024)), dim3(128), q.DI, q.cit[b], Sl,
+                              dmd, dms, Windex, Zidx, W, Wsh, Werr, Wgap, q.ST);
                     ptasks = q.tit[b];
                     pcnt = q.cit[b];
                     psi = q.sit[b];
                     pfirst = 0;
+                    if (trace) {   // the iteration's counts, waiting for it
+                        LevelCounters ic;
+                        (void)hipMemcpyAsync(&ic, q.cit[b], sizeof(ic), hipMemcpyDeviceToHost, s2);
+                        (void)hipStreamSynchronize(s2);
+                        float ms = 0.f;
+                        (void)hipEventRecord(t1, s2);
+                        (void)hipEventSynchronize(t1);
+                        (void)hipEventElapsedTime(&ms, t0, t1);
+                        fprintf(stderr, "mr3 level %d iteration %d: going on %d, bisected %d, "
+                                "lists %d, %.3f ms\n", level, it, ic.nS, ic.nEig, ic.nT, ms);
+                    }
                 }
                 LAUNCH_ON(s2, k_s_normalize, dim3(g1), dim3(64), q.wtasks, q.scnt, Sl, dmd, Zidx, q.ST);
                 MR3_CHECK(hipGetLastError());

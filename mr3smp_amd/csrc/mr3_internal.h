@@ -129,6 +129,7 @@ struct LevelCounters {
     long long arena;        // doubles needed by the next level's cluster slots
     int err, nT;            // nT: wave tasks planned for the singleton kernel
     int next;               // next wave task to hand out
+    int nD;                 // singletons in the dense list of the tail kernel
     int cls[kSizeClasses];  // wave tasks per size class
     int clsPos[kSizeClasses];
 };
