@@ -136,9 +136,10 @@ struct CallEvents {
 // slots, Z columns and representations, as the reference's S- and C-tasks do).
 // kSide of them take consecutive levels in turn, so that the last few singletons of
 // level L (the ones that need all ten iterations) run beside the bulk of the next
-// levels.  With the caller's stream that is four streams: the hardware queues a
-// process gets by default.
-constexpr int kSide = 3;
+// level.  A process gets four hardware queues by default and the caller's program
+// may hold some of them: with a third side stream two of them shared one queue and
+// a level waited for the tail of an unrelated one (measured slower than two).
+constexpr int kSide = 2;
 static hipStream_t side_stream(int k) {
     thread_local hipStream_t s[kSide] = {};
     thread_local bool tried[kSide] = {};
