@@ -150,20 +150,50 @@ static hipStream_t side_stream(int k) {
     return s[k];
 }
 
-// Device memory in one stream-ordered pool: buffers of a solve are taken with
-// hipMallocAsync and returned with hipFreeAsync on the same stream, so that
-// back-to-back solves reuse memory without device-wide synchronisation.
+// Device memory of a solve: one block carved by a bump pointer.  Blocks are kept in a
+// process-wide free list between solves, so that a solve does not allocate and
+// release its sixty-odd buffers one by one (returning them to the stream-ordered pool
+// took about 12 ms per batch of 4096 matrices).  A solve that needs more than its
+// block takes the rest from the stream-ordered pool and, at its end, replaces the
+// block by one large enough for next time.  Concurrent solves (host threads, server
+// batches) each hold their own block.
+struct Workspace {
+    char* base = nullptr;
+    size_t cap = 0;
+};
+static std::mutex g_ws_mu;
+static std::vector<Workspace> g_ws_free;
+
 struct Alloc {
     hipStream_t st;
-    std::vector<void*> ptrs;
+    Workspace ws;
+    size_t off = 0, need = 0;
+    std::vector<void*> extra;   // buffers beyond the block
     bool ok = true;
-    explicit Alloc(hipStream_t s) : st(s) {}
+    explicit Alloc(hipStream_t s) : st(s) {
+        std::lock_guard<std::mutex> lk(g_ws_mu);
+        if (!g_ws_free.empty()) {
+            // the largest block on the list
+            size_t k = 0;
+            for (size_t j = 1; j < g_ws_free.size(); ++j)
+                if (g_ws_free[j].cap > g_ws_free[k].cap) k = j;
+            ws = g_ws_free[k];
+            g_ws_free.erase(g_ws_free.begin() + (long)k);
+        }
+    }
     template <class T>
     T* get(size_t count) {
-        void* p = nullptr;
         if (count == 0) count = 1;
-        if (hipMallocAsync(&p, count * sizeof(T), st) != hipSuccess) { ok = false; return nullptr; }
-        ptrs.push_back(p);
+        const size_t bytes = (count * sizeof(T) + 255) & ~(size_t)255;
+        need += bytes;
+        if (ws.base && off + bytes <= ws.cap) {
+            void* p = ws.base + off;
+            off += bytes;
+            return (T*)p;
+        }
+        void* p = nullptr;
+        if (hipMallocAsync(&p, bytes, st) != hipSuccess) { ok = false; return nullptr; }
+        extra.push_back(p);
         return (T*)p;
     }
     // An array that the sweeps read: kSweepPad spare doubles on both sides, so that a
@@ -173,11 +203,28 @@ struct Alloc {
         double* p = get<double>(count + 2 * (size_t)kSweepPad);
         return p ? p + kSweepPad : nullptr;
     }
-    void release() {
-        for (void* p : ptrs) (void)hipFreeAsync(p, st);
-        ptrs.clear();
+    ~Alloc() {
+        // every stream of the solve is idle once the main stream is: the side streams
+        // were synchronised when the level loop's scope ended
+        (void)hipStreamSynchronize(st);
+        if (!extra.empty()) {
+            for (void* p : extra) (void)hipFreeAsync(p, st);
+            if (ws.base) (void)hipFreeAsync(ws.base, st);
+            ws = Workspace{};
+            const size_t cap = need + need / 4;
+            void* p = nullptr;
+            if (hipMallocAsync(&p, cap, st) == hipSuccess) {
+                ws.base = (char*)p;
+                ws.cap = cap;
+            }
+            // the block is used on other streams by later solves
+            (void)hipStreamSynchronize(st);
+        }
+        if (ws.base) {
+            std::lock_guard<std::mutex> lk(g_ws_mu);
+            g_ws_free.push_back(ws);
+        }
     }
-    ~Alloc() { release(); }
 };
 
 static void configure_pool_once() {
