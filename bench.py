@@ -145,6 +145,14 @@ def main():
                           "n_gpus": args.gpus}))
         sys.exit(2)
 
+    # the solver runs a batch on the caller's stream and up to three side streams, and
+    # the served stream adds a stream per worker: give the process enough hardware
+    # queues that they do not share (HIP's default is 4; set before HIP starts)
+    try:
+        hwq = int(os.environ.get("GPU_MAX_HW_QUEUES", "4"))
+    except ValueError:
+        hwq = 4
+    os.environ["GPU_MAX_HW_QUEUES"] = str(max(hwq, 16))
     import torch
     import mr3smp_amd as M
     dev = torch.device("cuda:0")

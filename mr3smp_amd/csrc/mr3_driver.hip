@@ -134,20 +134,52 @@ struct CallEvents {
 // The singleton iterations of a tree level run on a side stream of the calling
 // thread, beside the cluster work of the same level (they touch disjoint eigenvalue
 // slots, Z columns and representations, as the reference's S- and C-tasks do).
-// kSide of them take consecutive levels in turn, so that the last few singletons of
+// Side streams take consecutive levels in turn, so that the last few singletons of
 // level L (the ones that need all ten iterations) run beside the bulk of the next
-// level.  A process gets four hardware queues by default and the caller's program
-// may hold some of them: with a third side stream two of them shared one queue and
-// a level waited for the tail of an unrelated one (measured slower than two).
-constexpr int kSide = 2;
-static hipStream_t side_stream(int k) {
-    thread_local hipStream_t s[kSide] = {};
-    thread_local bool tried[kSide] = {};
-    if (!tried[k]) {
-        tried[k] = true;
-        if (hipStreamCreateWithFlags(&s[k], hipStreamNonBlocking) != hipSuccess) s[k] = nullptr;
+// levels.  How many pay depends on the hardware queues of the process: HIP gives it
+// GPU_MAX_HW_QUEUES of them (4 by default) and streams beyond that share one, so
+// that a level can wait for the tail of an unrelated one.  Measured on the batched
+// workload: two side streams with 4 queues (three were slower), three with 8 or more
+// queues.  MR3_SIDE_STREAMS (1 .. 3) overrides the choice.
+constexpr int kSide = 3;   // most side streams of a solve
+static int side_count() {
+    if (const char* v = getenv("MR3_SIDE_STREAMS")) {
+        const int k = atoi(v);
+        return k < 1 ? 1 : (k > kSide ? kSide : k);
     }
-    return s[k];
+    if (const char* q = getenv("GPU_MAX_HW_QUEUES"))
+        if (atoi(q) >= 8) return 3;
+    return 2;
+}
+struct SideStreams {
+    hipStream_t s[kSide] = {};
+    bool tried[kSide] = {};
+};
+static SideStreams& side_streams() {
+    thread_local SideStreams ss;
+    return ss;
+}
+static hipStream_t side_stream(int k) {
+    SideStreams& ss = side_streams();
+    if (!ss.tried[k]) {
+        ss.tried[k] = true;
+        if (hipStreamCreateWithFlags(&ss.s[k], hipStreamNonBlocking) != hipSuccess) ss.s[k] = nullptr;
+    }
+    return ss.s[k];
+}
+
+// Destroys the calling thread's side streams (a thread that solves and then ends
+// calls it, e.g. a server worker; the streams are created again if it solves later).
+void release_thread_streams() {
+    SideStreams& ss = side_streams();
+    for (int k = 0; k < kSide; ++k) {
+        if (ss.s[k]) {
+            (void)hipStreamSynchronize(ss.s[k]);
+            (void)hipStreamDestroy(ss.s[k]);
+        }
+        ss.s[k] = nullptr;
+        ss.tried[k] = false;
+    }
 }
 
 // Device memory of a solve: one block carved by a bump pointer.  Blocks are kept in a
@@ -414,8 +446,9 @@ int run_pipeline(std::vector<MatDesc>& md, std::vector<MatState>& ms, hipStream_
             int* BI;
             int* DI;          // dense list of the tail kernel
             double* scratch;
-        } ss[kSide];
-        for (int k = 0; k < kSide; ++k) {
+        } ss[kSide] = {};
+        const int nside = side_count();
+        for (int k = 0; k < nside; ++k) {
             ss[k].wtasks = A.get<WaveTask>(capT);
             ss[k].scnt = A.get<LevelCounters>(1);
             ss[k].ST = A.get<SState>(capS);
@@ -431,18 +464,18 @@ int run_pipeline(std::vector<MatDesc>& md, std::vector<MatState>& ms, hipStream_
         if (!A.ok) return 2;
         for (int k = 0; k < kLB; ++k)
             MR3_CHECK(hipMemsetAsync(bcnt2[k], 0, sizeof(int) * (size_t)N, st));
-        for (int k = 0; k < kSide; ++k)
+        for (int k = 0; k < nside; ++k)
             MR3_CHECK(hipMemsetAsync(ss[k].sbc, 0, sizeof(int) * (size_t)N, st));
         // eigenvector scratch: two checkpoint arrays of maxbl/kChunk + 2 carries per
         // lane, 64 lanes per wave
         const size_t ck_per_lane = (size_t)2 * (maxbl / kChunk + 2);
         const size_t per_wave = ck_per_lane * 64 * sizeof(double);
-        const size_t budget = singleton_scratch_budget() / kSide;
+        const size_t budget = singleton_scratch_budget() / nside;
         int max_waves = (int)std::max<size_t>(1, std::min<size_t>(budget / per_wave,
                                                                   (size_t)env_int("MR3_SWAVES", 8192)));
         const int want_waves = capT;
         const int nwaves = std::min(max_waves, want_waves);
-        for (int k = 0; k < kSide; ++k) ss[k].scratch = A.get<double>((size_t)nwaves * ck_per_lane * 64);
+        for (int k = 0; k < nside; ++k) ss[k].scratch = A.get<double>((size_t)nwaves * ck_per_lane * 64);
         if (!A.ok) return 2;
         LevelCounters hc;
         auto reset = [&]() { return hipMemsetAsync(dcnt, 0, sizeof(LevelCounters), st); };
@@ -460,8 +493,8 @@ int run_pipeline(std::vector<MatDesc>& md, std::vector<MatState>& ms, hipStream_
         int cur = 0;
         const bool trace = getenv("MR3_TRACE_LEVELS") != nullptr;   // per-level timing to stderr
         unsigned long long prev_ticks = 0, prev_tasks = 0;
-        hipStream_t s2p[kSide];
-        for (int k = 0; k < kSide; ++k) {
+        hipStream_t s2p[kSide] = {};
+        for (int k = 0; k < nside; ++k) {
             s2p[k] = side_stream(k);
             if (!s2p[k]) return 2;
         }
@@ -483,7 +516,8 @@ int run_pipeline(std::vector<MatDesc>& md, std::vector<MatState>& ms, hipStream_
         struct SideSync {   // no buffer or event goes away while a side stream may use it
             const hipStream_t* s;
             ~SideSync() {
-                for (int k = 0; k < kSide; ++k) (void)hipStreamSynchronize(s[k]);
+                for (int k = 0; k < kSide; ++k)
+                    if (s[k]) (void)hipStreamSynchronize(s[k]);
             }
         } side_sync{s2p};
         bool have_evS[kLB] = {};
@@ -491,12 +525,12 @@ int run_pipeline(std::vector<MatDesc>& md, std::vector<MatState>& ms, hipStream_
         hipEvent_t evStart = nullptr;
         MR3_CHECK(hipEventCreateWithFlags(&evStart, hipEventDisableTiming));
         MR3_CHECK(hipEventRecord(evStart, st));
-        for (int k = 0; k < kSide; ++k) MR3_CHECK(hipStreamWaitEvent(s2p[k], evStart, 0));
+        for (int k = 0; k < nside; ++k) MR3_CHECK(hipStreamWaitEvent(s2p[k], evStart, 0));
         (void)hipEventDestroy(evStart);
         for (int level = 0;; ++level) {
             const int nS = hc.nS, nC = hc.nC, nE = hc.nEig;
             const long long arena_need = hc.arena;
-            const int p = level % kSide;       // side stream and its buffer set
+            const int p = level % nside;       // side stream and its buffer set
             const int lb = level % kLB;        // this level's list buffer
             const int lbn = (level + 1) % kLB; // the next level's
             // 1. cluster work of this level and the next level's classification, on the

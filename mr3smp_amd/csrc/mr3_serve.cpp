@@ -2,13 +2,23 @@
 //
 // A scheduler thread admits queued requests in arrival order whenever a concurrency
 // slot is free and the device-memory budget (8 n m bytes for Z plus a per-row
-// workspace estimate) has room; each admitted group runs as one batched solve on
-// its own stream, so groups that were admitted at different times overlap on the
+// workspace estimate) has room; each admitted group runs as one batched solve on a
+// worker's stream, so groups that were admitted at different times overlap on the
 // device.  Requests arriving while groups run wait only for a slot, not for the
 // whole queue to drain.  Results of a problem do not depend on the batch it ran
 // in: every decision of the pipeline is taken per matrix.
+//
+// The workers are persistent threads, one per concurrency slot, each with its own
+// stream (and the solver's per-thread side streams); they run a small solve when the
+// server starts, so the first requests do not pay for stream creation, device
+// workspace or code loading.  Large results go back to the caller's host arrays
+// through a pinned double buffer: while one chunk is in flight from the device,
+// several host threads copy the previous one into place (a copy straight into
+// pageable memory runs at a few GB/s, most of it spent faulting the destination
+// pages in on one thread).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
@@ -20,9 +30,16 @@
 
 #include "../../include/mr3hip.h"
 
+namespace mr3 {
+void release_thread_streams();   // mr3_driver.hip: the solver's per-thread side streams
+}
+
 namespace {
 
-constexpr long long kRowBytes = 512;   // per-row workspace estimate of the pipeline
+constexpr long long kRowBytes = 512;       // per-row workspace estimate of the pipeline
+constexpr size_t kChunk = 64ull << 20;     // staging chunk of the result copies
+constexpr size_t kDirect = 8ull << 20;     // results up to this size are copied directly
+constexpr int kCopyThreads = 8;            // host threads filling one chunk
 
 long long footprint(const mr3_request& r) {
     long long cols = 0;
@@ -36,30 +53,84 @@ struct Entry {
     long long bytes;
 };
 
+// Pinned double buffer of one worker.
+struct Staging {
+    char* buf[2] = {nullptr, nullptr};
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    bool ok = false;
+    void init() {
+        ok = hipHostMalloc((void**)&buf[0], kChunk, hipHostMallocDefault) == hipSuccess &&
+             hipHostMalloc((void**)&buf[1], kChunk, hipHostMallocDefault) == hipSuccess &&
+             hipEventCreateWithFlags(&ev[0], hipEventDisableTiming) == hipSuccess &&
+             hipEventCreateWithFlags(&ev[1], hipEventDisableTiming) == hipSuccess;
+    }
+    void release() {
+        for (int k = 0; k < 2; ++k) {
+            if (buf[k]) (void)hipHostFree(buf[k]);
+            if (ev[k]) (void)hipEventDestroy(ev[k]);
+        }
+    }
+};
+
+void par_copy(char* dst, const char* src, size_t bytes) {
+    const size_t per = (bytes + kCopyThreads - 1) / kCopyThreads;
+    std::vector<std::thread> th;
+    for (int t = 1; t < kCopyThreads; ++t) {
+        const size_t o = per * t;
+        if (o >= bytes) break;
+        th.emplace_back([=] { memcpy(dst + o, src + o, std::min(per, bytes - o)); });
+    }
+    memcpy(dst, src, std::min(per, bytes));
+    for (auto& x : th) x.join();
+}
+
+// Device-to-host copy of a result after the work queued on st; returns 0 on success.
+int copy_out(void* dst, const void* src, size_t bytes, hipStream_t st, Staging& sg) {
+    if (bytes <= kDirect || !sg.ok)
+        return hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, st) == hipSuccess ? 0 : 2;
+    char* d = (char*)dst;
+    const char* s = (const char*)src;
+    size_t prev_o = 0, prev_c = 0;
+    int k = 0;
+    for (size_t o = 0; o < bytes; o += kChunk, ++k) {
+        const size_t c = std::min(kChunk, bytes - o);
+        const int b = k & 1;
+        if (hipMemcpyAsync(sg.buf[b], s + o, c, hipMemcpyDeviceToHost, st) != hipSuccess) return 2;
+        if (hipEventRecord(sg.ev[b], st) != hipSuccess) return 2;
+        if (k > 0) {
+            if (hipEventSynchronize(sg.ev[b ^ 1]) != hipSuccess) return 2;
+            par_copy(d + prev_o, sg.buf[b ^ 1], prev_c);
+        }
+        prev_o = o;
+        prev_c = c;
+    }
+    if (hipEventSynchronize(sg.ev[(k - 1) & 1]) != hipSuccess) return 2;
+    par_copy(d + prev_o, sg.buf[(k - 1) & 1], prev_c);
+    return 0;
+}
+
 }  // namespace
 
 struct mr3_server {
     long long budget = 0, in_use = 0;
-    int max_inflight = 2, max_batch = 256, inflight = 0;
+    int max_inflight = 2, max_batch = 256, inflight = 0, ready = 0;
     bool stop = false;
     long long next_id = 0;
     std::deque<Entry> pending;
+    std::deque<std::vector<Entry>> groups;               // admitted, waiting for a worker
     std::unordered_map<long long, mr3_request*> live;   // submitted, not yet complete
     std::mutex mu;
-    std::condition_variable cv_sched, cv_done;
+    std::condition_variable cv_sched, cv_done, cv_work;
     std::thread sched;
     std::vector<std::thread> workers;
-    std::vector<hipStream_t> streams;
-    std::vector<int> stream_free;
     int device = 0;
 
-    void run_group(std::vector<Entry> grp, int sidx);
+    void run_group(std::vector<Entry>& grp, hipStream_t st, Staging& sg);
     void scheduler();
+    void worker();
 };
 
-void mr3_server::run_group(std::vector<Entry> grp, int sidx) {
-    (void)hipSetDevice(device);
-    hipStream_t st = streams[sidx];
+void mr3_server::run_group(std::vector<Entry>& grp, hipStream_t st, Staging& sg) {
     const double t0 = mr3_now();
     for (auto& e : grp) e.req->t_start = t0;
     // one device allocation for the whole group
@@ -105,11 +176,11 @@ void mr3_server::run_group(std::vector<Entry> grp, int sidx) {
             r.m = p.m;
             r.info = p.info;
             if (p.info == 0 && p.m > 0) {
-                if (r.W) (void)hipMemcpyAsync(r.W, p.W, sizeof(double) * p.m, hipMemcpyDeviceToHost, st);
-                if (p.Z && r.Z)
-                    (void)hipMemcpyAsync(r.Z, p.Z, sizeof(double) * (size_t)r.n * p.m, hipMemcpyDeviceToHost, st);
-                if (p.Zsupp && r.Zsupp)
-                    (void)hipMemcpyAsync(r.Zsupp, p.Zsupp, sizeof(int) * 2 * p.m, hipMemcpyDeviceToHost, st);
+                int c = 0;
+                if (r.W) c |= copy_out(r.W, p.W, sizeof(double) * p.m, st, sg);
+                if (p.Z && r.Z) c |= copy_out(r.Z, p.Z, sizeof(double) * (size_t)r.n * p.m, st, sg);
+                if (p.Zsupp && r.Zsupp) c |= copy_out(r.Zsupp, p.Zsupp, sizeof(int) * 2 * p.m, st, sg);
+                if (c) rc = 2;
             }
         }
         (void)hipFreeAsync(base, st);
@@ -124,13 +195,50 @@ void mr3_server::run_group(std::vector<Entry> grp, int sidx) {
         live.erase(e.id);
     }
     inflight -= 1;
-    stream_free[sidx] = 1;
     cv_done.notify_all();
     cv_sched.notify_all();
 }
 
-void mr3_server::scheduler() {
+void mr3_server::worker() {
     (void)hipSetDevice(device);
+    hipStream_t st = nullptr;
+    (void)hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+    Staging sg;
+    sg.init();
+    {
+        // warm-up: a 3x3 solve creates this thread's streams and device workspace
+        double h[9] = {2.0, 2.0, 2.0, -1.0, -1.0, 0.0};
+        double* d = nullptr;
+        if (st && hipMallocAsync((void**)&d, sizeof(double) * 32, st) == hipSuccess) {
+            (void)hipMemcpyAsync(d, h, sizeof(double) * 6, hipMemcpyHostToDevice, st);
+            mr3_problem p;
+            memset(&p, 0, sizeof(p));
+            p.n = 3; p.jobz = 'V'; p.range = 'A'; p.tryrac = 1; p.ldz = 3; p.mcols = 3;
+            p.D = d; p.E = d + 3; p.W = d + 8; p.Z = d + 12; p.Zsupp = (int*)(d + 24);
+            (void)mr3_solve_batch(&p, 1, st);
+            (void)hipFreeAsync(d, st);
+            (void)hipStreamSynchronize(st);
+        }
+    }
+    std::unique_lock<std::mutex> lk(mu);
+    ++ready;
+    cv_done.notify_all();
+    for (;;) {
+        cv_work.wait(lk, [&] { return stop || !groups.empty(); });
+        if (groups.empty()) break;   // stop, nothing left
+        std::vector<Entry> grp = std::move(groups.front());
+        groups.pop_front();
+        lk.unlock();
+        run_group(grp, st, sg);
+        lk.lock();
+    }
+    lk.unlock();
+    sg.release();
+    mr3::release_thread_streams();
+    if (st) (void)hipStreamDestroy(st);
+}
+
+void mr3_server::scheduler() {
     std::unique_lock<std::mutex> lk(mu);
     for (;;) {
         cv_sched.wait(lk, [&] {
@@ -138,7 +246,6 @@ void mr3_server::scheduler() {
             if (pending.empty() || inflight >= max_inflight) return false;
             return in_use + pending.front().bytes <= budget;
         });
-        if (stop && pending.empty()) break;
         if (stop) break;
         // admit in arrival order while the budget and the group size allow
         std::vector<Entry> grp;
@@ -149,13 +256,9 @@ void mr3_server::scheduler() {
             pending.pop_front();
         }
         if (grp.empty()) continue;
-        int sidx = 0;
-        while (!stream_free[sidx]) ++sidx;
-        stream_free[sidx] = 0;
         inflight += 1;
-        lk.unlock();
-        std::thread(&mr3_server::run_group, this, std::move(grp), sidx).detach();
-        lk.lock();
+        groups.push_back(std::move(grp));
+        cv_work.notify_one();
     }
 }
 
@@ -165,10 +268,10 @@ extern "C" mr3_server* mr3_server_create(long long budget_bytes, int max_infligh
     s->max_inflight = max_inflight > 0 ? max_inflight : 2;
     s->max_batch = max_batch > 0 ? max_batch : 256;
     (void)hipGetDevice(&s->device);
-    s->streams.resize(s->max_inflight);
-    s->stream_free.assign(s->max_inflight, 1);
-    for (auto& st : s->streams) {
-        if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) st = nullptr;
+    for (int w = 0; w < s->max_inflight; ++w) s->workers.emplace_back(&mr3_server::worker, s);
+    {
+        std::unique_lock<std::mutex> lk(s->mu);
+        s->cv_done.wait(lk, [&] { return s->ready == s->max_inflight; });
     }
     s->sched = std::thread(&mr3_server::scheduler, s);
     return s;
@@ -212,9 +315,9 @@ extern "C" void mr3_server_destroy(mr3_server* s) {
         std::lock_guard<std::mutex> lk(s->mu);
         s->stop = true;
         s->cv_sched.notify_all();
+        s->cv_work.notify_all();
     }
     s->sched.join();
-    for (auto& st : s->streams)
-        if (st) (void)hipStreamDestroy(st);
+    for (auto& w : s->workers) w.join();
     delete s;
 }

@@ -248,6 +248,40 @@ def test_gpu_batch_equals_single(M):
             assert torch.equal(keep[i]["Zsupp"][:2 * m], k1[0]["Zsupp"][:2 * m])
 
 
+def test_gpu_side_stream_count_bitwise(M):
+    """The tree levels' singleton phases may run on one, two or three side streams;
+    the results are the same bit for bit (the streams only change what overlaps)."""
+    import os
+    import torch
+    from mr3smp_amd import workloads as WL
+    dev = torch.device("cuda:0")
+    mats = WL.cfg3(64, seed_extra=5)
+    outs = []
+    old = os.environ.get("MR3_SIDE_STREAMS")
+    try:
+        for k in ("1", "2", "3"):
+            os.environ["MR3_SIDE_STREAMS"] = k
+            probs = [dict(D=torch.tensor(D, device=dev), E=torch.tensor(E, device=dev))
+                     for D, E, _ in mats]
+            rc, arr, keep = M.solve_batch(probs)
+            torch.cuda.synchronize()
+            assert rc == 0
+            outs.append((arr, keep))
+    finally:
+        if old is None:
+            os.environ.pop("MR3_SIDE_STREAMS", None)
+        else:
+            os.environ["MR3_SIDE_STREAMS"] = old
+    a0, k0 = outs[0]
+    for arr, keep in outs[1:]:
+        for i in range(len(mats)):
+            m = a0[i].m
+            assert arr[i].m == m
+            assert torch.equal(keep[i]["W"][:m], k0[i]["W"][:m]), f"W differs for matrix {i}"
+            assert torch.equal(keep[i]["Z"][:m], k0[i]["Z"][:m]), f"Z differs for matrix {i}"
+            assert torch.equal(keep[i]["Zsupp"][:2 * m], k0[i]["Zsupp"][:2 * m])
+
+
 def test_gpu_server_alone_vs_mix(M):
     """Requests served in a concurrent mix return exactly what they return alone."""
     from mr3smp_amd import workloads as WL
