@@ -42,7 +42,7 @@ def log(*a):
 
 def build_workload(torch, dev, count):
     from mr3smp_amd import workloads as WL
-    mats = WL.cfg3(count)
+    mats = WL.batched_mixed(count)
     probs = []
     for D, E, kind in mats:
         n = len(D)
@@ -103,7 +103,7 @@ def cpu_baseline(mats, stride, threads):
 
 def serve_stream(M, count, rate, budget):
     from mr3smp_amd import workloads as WL
-    reqs = WL.cfg5(count, rate)
+    reqs = WL.served_stream(count, rate)
     srv = M.Server(budget_bytes=budget, max_inflight=3, max_batch=64)
     t0 = time.perf_counter()
     ids = []

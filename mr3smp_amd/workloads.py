@@ -16,8 +16,8 @@ import numpy as np
 SEED_BASE = 0x4D5233_0000
 
 
-def rng_for(cfg, extra=0):
-    return np.random.default_rng([SEED_BASE + cfg, extra])
+def rng_for(stream, extra=0):
+    return np.random.default_rng([SEED_BASE + stream, extra])
 
 
 def uniform(n, rng):
@@ -55,12 +55,12 @@ def glued_wilkinson(n, rng, glue=1e-8):
     return D, E
 
 
-def cfg1():
+def single_uniform_1000():
     """n = 1000, U(-1, 1), jobz V, range A."""
     return uniform(1000, rng_for(1))
 
 
-def cfg2():
+def single_glued_20000():
     """n = 20000 glued Wilkinson (952 W21+ blocks, glue 1e-8), jobz V, range A."""
     return glued_wilkinson(20000, rng_for(2), 1e-8)
 
@@ -80,7 +80,7 @@ def mixed_matrix(n, rng, glue=1e-10):
     return glued_wilkinson(n, rng, glue), "wilkinson"
 
 
-def cfg3(count=4096, seed_extra=0):
+def batched_mixed(count=4096, seed_extra=0):
     """Batched: `count` matrices, n log-uniform in [32, 2048], the 50/25/25 mix,
     jobz V, range A.  Returns a list of (D, E, kind)."""
     rng = rng_for(3, seed_extra)
@@ -92,7 +92,7 @@ def cfg3(count=4096, seed_extra=0):
     return out
 
 
-def cfg4(count=256, seed_extra=0):
+def batched_range_i(count=256, seed_extra=0):
     """Batched subset: n log-uniform in [4096, 50000], N(0,1), jobz V, range I over a
     contiguous window of ceil(0.1 n) indices.  Returns (D, E, il, iu)."""
     rng = rng_for(4, seed_extra)
@@ -107,7 +107,7 @@ def cfg4(count=256, seed_extra=0):
     return out
 
 
-def cfg5(count, rate, seed_extra=0):
+def served_stream(count, rate, seed_extra=0):
     """Served stream: Poisson arrivals at `rate` per second, n log-uniform in
     [64, 16384], jobz 70% V / 30% N, range 60% A / 25% I (5-20% window) / 15% V
     (10% of the Gershgorin span).  Returns a list of dicts with arrival times."""

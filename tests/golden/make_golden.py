@@ -41,8 +41,8 @@ def cases():
         ("w21_NA", D21, E21, dict(jobz="N", rng="A")),
     ]
     D, E = WL.uniform(200, rng); out.append(("uni200_VA", D, E, dict(jobz="V", rng="A")))
-    D, E = WL.uniform(1000, WL.rng_for(1)); out.append(("cfg1_VA", D, E, dict(jobz="V", rng="A")))
-    out.append(("cfg1_VA_threads", D, E, dict(jobz="V", rng="A", threads=16)))
+    D, E = WL.uniform(1000, WL.rng_for(1)); out.append(("u1000_VA", D, E, dict(jobz="V", rng="A")))
+    out.append(("u1000_VA_threads", D, E, dict(jobz="V", rng="A", threads=16)))
     D, E = WL.uniform(500, rng); out.append(("uni500_NA", D, E, dict(jobz="N", rng="A")))
     D, E = WL.toeplitz_pert(300, rng); out.append(("toep300_VA", D, E, dict(jobz="V", rng="A")))
     D, E = WL.glued_wilkinson(420, rng, 1e-10); out.append(("glued420_VA", D, E, dict(jobz="V", rng="A")))

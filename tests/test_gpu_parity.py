@@ -153,23 +153,23 @@ def test_gpu_edge_cases(oracle, M):
     check_pair(D, E, g, o, "ldz64")
 
 
-def test_gpu_cfg1_full(oracle, M):
+def test_gpu_uniform1000_full(oracle, M):
     from mr3smp_amd import workloads as WL
-    D, E = WL.cfg1()
+    D, E = WL.single_uniform_1000()
     g = M.mrrr("V", "A", D, E)
     o = call_mrrr(oracle[1], "V", "A", D, E)
-    check_pair(D, E, g, o, "cfg1")
-    print("cfg1 W bit-identical to oracle:", np.array_equal(g["W"], o["W"]),
+    check_pair(D, E, g, o, "uniform n=1000")
+    print("uniform n=1000: W bit-identical to oracle:", np.array_equal(g["W"], o["W"]),
           "Z bit-identical:", np.array_equal(g["Z"], o["Z"]), flush=True)
 
 
-def test_gpu_cfg2_full_properties(M):
+def test_gpu_glued20000_properties(M):
     """n = 20000 glued Wilkinson: eigenvalue/index consistency by Sturm counts, trace
     and Frobenius identities, resid/orth bounds (the reference reports resid 3.7e-4).
     The solve runs on device tensors and Z^T Z is formed on the device in FP64."""
     import torch
     from mr3smp_amd import workloads as WL
-    D, E = WL.cfg2()
+    D, E = WL.single_glued_20000()
     n = len(D)
     dev = torch.device("cuda:0")
     rc, arr, keep = M.solve_batch([dict(D=torch.tensor(D, device=dev), E=torch.tensor(E, device=dev))])
@@ -196,7 +196,7 @@ def test_gpu_cfg2_full_properties(M):
     G.diagonal().sub_(1.0)
     orth = float(G.abs().max()) / (n * EPS)
     del G
-    print(f"cfg2 resid {res:.3g} orth {orth:.3g}", flush=True)
+    print(f"glued n=20000: resid {res:.3g} orth {orth:.3g}", flush=True)
     assert res <= 2 * max(3.7e-4, 1.0) and orth <= 100.0
     Zs = keep[0]["Zsupp"][: 2 * n].cpu().numpy()
     cols = np.arange(0, n, 97)
@@ -225,7 +225,7 @@ def test_gpu_batch_equals_single(M):
     from mr3smp_amd import workloads as WL
     dev = torch.device("cuda:0")
     rng = np.random.default_rng(77)
-    mats = WL.cfg3(40, seed_extra=3)
+    mats = WL.batched_mixed(40, seed_extra=3)
     probs, singles = [], []
     for i, (D, E, kind) in enumerate(mats):
         n = len(D)
@@ -255,7 +255,7 @@ def test_gpu_side_stream_count_bitwise(M):
     import torch
     from mr3smp_amd import workloads as WL
     dev = torch.device("cuda:0")
-    mats = WL.cfg3(64, seed_extra=5)
+    mats = WL.batched_mixed(64, seed_extra=5)
     outs = []
     old = os.environ.get("MR3_SIDE_STREAMS")
     try:
@@ -285,7 +285,7 @@ def test_gpu_side_stream_count_bitwise(M):
 def test_gpu_server_alone_vs_mix(M):
     """Requests served in a concurrent mix return exactly what they return alone."""
     from mr3smp_amd import workloads as WL
-    reqs = WL.cfg5(48, rate=1e9, seed_extra=11)
+    reqs = WL.served_stream(48, rate=1e9, seed_extra=11)
     reqs = [r for r in reqs if r["n"] <= 3000][:24]
     alone = []
     for r in reqs:

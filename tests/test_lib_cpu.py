@@ -37,19 +37,19 @@ def test_library_is_gfx950_code_object():
 
 def test_workload_generators():
     from mr3smp_amd import workloads as WL
-    D, E = WL.cfg2()
+    D, E = WL.single_glued_20000()
     assert len(D) == 20000 and E[-1] == 0.0
     assert np.count_nonzero(D[:21] - np.abs(10 - np.arange(21.0))) == 0
     glue = E[20:952 * 21:21]
     assert np.all((glue >= 0.5e-8) & (glue <= 1e-8))
     assert np.all(D[952 * 21:] == 2.0)
-    mats = WL.cfg3(64)
+    mats = WL.batched_mixed(64)
     ns = [len(m[0]) for m in mats]
     assert min(ns) >= 32 and max(ns) <= 2048
-    c4 = WL.cfg4(4)
+    c4 = WL.batched_range_i(4)
     for D, E, il, iu in c4:
         n = len(D)
         assert 4096 <= n <= 50000 and iu - il + 1 == int(np.ceil(0.1 * n)) and 1 <= il <= iu <= n
-    reqs = WL.cfg5(50, 10.0)
+    reqs = WL.served_stream(50, 10.0)
     assert all(np.diff([r["t"] for r in reqs]) >= 0)
     assert {r["range"] for r in reqs} <= {"A", "I", "V"}
