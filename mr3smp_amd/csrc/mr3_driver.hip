@@ -492,7 +492,7 @@ int run_pipeline(std::vector<MatDesc>& md, std::vector<MatState>& ms, hipStream_
         if (hc.err) return 2;
         int cur = 0;
         const bool trace = getenv("MR3_TRACE_LEVELS") != nullptr;   // per-level timing to stderr
-        unsigned long long prev_ticks = 0, prev_tasks = 0;
+        unsigned long long prev_rrf = 0, prev_rrfn = 0;
         hipStream_t s2p[kSide] = {};
         for (int k = 0; k < nside; ++k) {
             s2p[k] = side_stream(k);
@@ -624,7 +624,8 @@ int run_pipeline(std::vector<MatDesc>& md, std::vector<MatState>& ms, hipStream_
                     LAUNCH_ON(s2, k_s_iter, dim3(gw), dim3(64), q.tit[b], q.cit[b], 0, q.sit[b], Sl,
                               dmd, dms, Windex, Zidx, W, Wsh, Werr, Wgap, q.ST, q.scratch, maxbl);
                     LAUNCH_ON(s2, k_s_tail, dim3(std::min(nS, 1024)), dim3(128), q.DI, q.cit[b], Sl,
-                              dmd, dms, Windex, Zidx, W, Wsh, Werr, Wgap, q.ST);
+                              dmd, dms, Windex, Zidx, W, Wsh, Werr, Wgap, q.ST,
+                              kMaxIterRQI + 2 - it);
                     ptasks = q.tit[b];
                     pcnt = q.cit[b];
                     psi = q.sit[b];
@@ -654,11 +655,13 @@ int run_pipeline(std::vector<MatDesc>& md, std::vector<MatState>& ms, hipStream_
                     (void)hipMemcpyFromSymbol(stv, HIP_SYMBOL(g_steps), sizeof(stv));
                     fprintf(stderr,
                             "mr3 level %d: singletons %d clusters %d  %.3f ms  "
-                            "tasks %llu  task time %.3f wave-ms\n",
-                            level, nS, nC, ms, stv[kStepTasks] - prev_tasks,
-                            (stv[kStepTaskTicks] - prev_ticks) * 1e-5);
-                    prev_ticks = stv[kStepTaskTicks];
-                    prev_tasks = stv[kStepTasks];
+                            "dlarrf sweeps per cluster %.2f (with the work counters on)\n",
+                            level, nS, nC, ms,
+                            stv[kStepRrfN] > prev_rrfn
+                                ? (double)(stv[kStepRrf] - prev_rrf) / (double)(stv[kStepRrfN] - prev_rrfn)
+                                : 0.0);
+                    prev_rrf = stv[kStepRrf];
+                    prev_rrfn = stv[kStepRrfN];
                     (void)hipEventDestroy(t0);
                     (void)hipEventDestroy(t1);
                 }
@@ -702,7 +705,7 @@ int profile_report(char* buf, int len) {
     static const char* kinds[kStepKinds] = {"coarse", "refine", "endref", "r1v",
                                             "r1v_bisect", "rrf", "rrj", "refine_cl",
                                             "r1v_wave", "bisect_wave", "task_ticks",
-                                            "tasks", "t_bisect", "t_r1v", "t_solve"};
+                                            "tasks", "t_bisect", "t_r1v", "t_solve", "rrf_rows"};
     std::string js = "{\"kernels\": {";
     std::lock_guard<std::mutex> lk(g_prof.mu);
     bool first = true;

@@ -130,6 +130,7 @@ struct LevelCounters {
     int err, nT;            // nT: wave tasks planned for the singleton kernel
     int next;               // next wave task to hand out
     int nD;                 // singletons in the dense list of the tail kernel
+    int nextD;              // next entry of that list to hand out
     int cls[kSizeClasses];  // wave tasks per size class
     int clsPos[kSizeClasses];
 };

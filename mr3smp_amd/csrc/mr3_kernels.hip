@@ -40,7 +40,8 @@ enum StepKind : int {
     kStepTBis = 12,      // wave time (ticks) in the bisection fallback
     kStepTR1v = 13,      // wave time in the twisted factorisations (dlar1v to the twist)
     kStepTSolve = 14,    // wave time in the vector solves and column writes
-    kStepKinds = 15
+    kStepRrfN = 15,      // block rows of the clusters given to dlarrf (one count per cluster)
+    kStepKinds = 16
 };
 __device__ unsigned long long g_steps[kStepKinds];
 __constant__ int g_count_steps;
@@ -1609,6 +1610,7 @@ __device__ int dlarrf_lane(int n, const double* __restrict__ d, const double* __
         (void)fail;
     }
     add_steps(kStepRrf, (unsigned long long)sweeps * n);
+    add_steps(kStepRrfN, (unsigned long long)n);
     return shift;
 }
 
@@ -2584,367 +2586,6 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
     }
 }
 
-// ---------------------------------------------------------------------------------
-// The tail of a level's singleton iterations: once few singletons go on (at most
-// kTailS), each one of a block up to kTailN rows gets a workgroup of two waves.  The
-// stationary transform (wave 0) and the progressive transform (wave 1) run at the
-// same time, each on one lane, keeping their S and P values in LDS; the twist search
-// is a reduction over all lanes; L+ and U- are recomputed from the stored carries by
-// all lanes (each value needs only its own carry); the two halves of the vector solve
-// run on one lane of each wave again.  Every operation and its order are those of
-// r1v_ck and pass_solve (odr1v.f:156-369), so the results are bit-identical; only
-// the lanes that carry them differ.
-// ---------------------------------------------------------------------------------
-
-// Stationary transform over steps 1 .. stop-1 (odr1v.f:184-229) on one lane, with S
-// after step t in SV[t] (SV[0] = 0).
-template <class P>
-__device__ __forceinline__ void tail_stat(P D, P LD, P L, const double* LLD, double lambda,
-                                          double pivmin, int r1, int stop, bool safe,
-                                          double* SV, double& s_out, int& neg_out,
-                                          double& slast_out) {
-    double s = 0.0 - lambda;
-    int neg = 0;
-    SV[0] = 0.0;
-    double a[kU], b[kU], c[kU], an[kU], bn[kU], cn[kU];
-#pragma unroll
-    for (int u = 0; u < kU; ++u) { a[u] = D[u]; b[u] = LD[u]; c[u] = L[u]; }
-    for (int t0 = 1; t0 < stop; t0 += kU) {
-#pragma unroll
-        for (int u = 0; u < kU; ++u) {
-            const int j = t0 - 1 + kU + u;
-            an[u] = D[j]; bn[u] = LD[j]; cn[u] = L[j];
-        }
-#pragma unroll
-        for (int u = 0; u < kU; ++u) {
-            const int t = t0 + u;
-            if (t < stop) {
-                double lp, si;
-                bool ng;
-                stat_step(a[u], b[u], c[u], LLD, t, lambda, safe, pivmin, s, lp, si, ng);
-                neg += (t <= r1 - 1) && ng;
-                SV[t] = si;
-            }
-        }
-#pragma unroll
-        for (int u = 0; u < kU; ++u) { a[u] = an[u]; b[u] = bn[u]; c[u] = cn[u]; }
-    }
-    s_out = s;
-    neg_out = neg;
-    slast_out = stop > 1 ? SV[stop - 1] : 0.0;
-}
-
-// Progressive transform from step n-1 down to lo (odr1v.f:231-257) on one lane, with P
-// after step t in PV[t] and PV[n] = D(n) - lambda.
-template <class P>
-__device__ __forceinline__ void tail_prog(int n, P D, P LLD, P L, double lambda, double pivmin,
-                                          int lo, bool safe, double* PV, double& p_out,
-                                          int& neg_out) {
-    double p = D[n - 1] - lambda;
-    PV[n] = p;
-    int neg = 0;
-    double a[kU], b[kU], c[kU], an[kU], bn[kU], cn[kU];
-#pragma unroll
-    for (int u = 0; u < kU; ++u) {
-        const int j = n - 2 - u;
-        a[u] = D[j]; b[u] = LLD[j]; c[u] = L[j];
-    }
-    for (int t0 = n - 1; t0 >= lo; t0 -= kU) {
-#pragma unroll
-        for (int u = 0; u < kU; ++u) {
-            const int j = t0 - kU - u - 1;
-            an[u] = D[j]; bn[u] = LLD[j]; cn[u] = L[j];
-        }
-#pragma unroll
-        for (int u = 0; u < kU; ++u) {
-            const int t = t0 - u;
-            if (t >= lo) {
-                double um;
-                bool ng;
-                prog_step(a[u], b[u], c[u], lambda, safe, pivmin, p, um, ng);
-                neg += ng;
-                PV[t] = p;
-            }
-        }
-#pragma unroll
-        for (int u = 0; u < kU; ++u) { a[u] = an[u]; b[u] = bn[u]; c[u] = cn[u]; }
-    }
-    p_out = p;
-    neg_out = neg;
-}
-
-// Upward half of the vector solve (odr1v.f:285-320) on one lane: Z(j) for j = r-1 down
-// to the support cut, read L+(j) from SV[j] and written there.  Returns ISUPPZ(1).
-template <class P>
-__device__ __forceinline__ int tail_solve_up(int r, P LD, bool nan, double gaptol, double* SV) {
-    double zi1 = 1.0, zi2 = 0.0;
-    int is0 = 1;
-    bool live = r - 1 >= 1;
-    double a[kU], an[kU];
-#pragma unroll
-    for (int u = 0; u < kU; ++u) a[u] = LD[r - 2 - u];
-    for (int j0 = r - 1; j0 >= 1 && live; j0 -= kU) {
-#pragma unroll
-        for (int u = 0; u < kU; ++u) an[u] = LD[j0 - kU - u - 1];
-#pragma unroll
-        for (int u = 0; u < kU; ++u) {
-            const int j = j0 - u;
-            if (live && j >= 1) {
-                const double ldj = a[u];
-                double zi;
-                if (nan && zi1 == 0.0) zi = -(LD[j] / ldj) * zi2;
-                else zi = -(SV[j] * zi1);
-                if ((fabs(zi) + fabs(zi1)) * fabs(ldj) < gaptol) {
-                    is0 = j + 1;
-                    live = false;
-                } else {
-                    SV[j] = zi;
-                    zi2 = zi1;
-                    zi1 = zi;
-                }
-            }
-        }
-#pragma unroll
-        for (int u = 0; u < kU; ++u) a[u] = an[u];
-    }
-    return is0;
-}
-
-// Downward half (odr1v.f:321-355) on one lane: Z(t+1) for t = r .. n-1 up to the cut,
-// read U-(t) from PV[t] and written there.  Returns ISUPPZ(2).
-template <class P>
-__device__ __forceinline__ int tail_solve_down(int n, int r, P LD, bool nan, double gaptol,
-                                               double* PV) {
-    double zi = 1.0, zim1 = 0.0;
-    int is1 = n;
-    bool live = r <= n - 1;
-    double a[kU], an[kU];
-#pragma unroll
-    for (int u = 0; u < kU; ++u) a[u] = LD[r - 1 + u];
-    for (int t0 = r; t0 <= n - 1 && live; t0 += kU) {
-#pragma unroll
-        for (int u = 0; u < kU; ++u) an[u] = LD[t0 + kU + u - 1];
-#pragma unroll
-        for (int u = 0; u < kU; ++u) {
-            const int t = t0 + u;
-            if (live && t <= n - 1) {
-                const double ldt = a[u];
-                double zn;
-                if (nan && zi == 0.0) zn = -(LD[max(t - 2, 0)] / ldt) * zim1;
-                else zn = -(PV[t] * zi);
-                if ((fabs(zi) + fabs(zn)) * fabs(ldt) < gaptol) {
-                    is1 = t;
-                    live = false;
-                } else {
-                    PV[t] = zn;
-                    zim1 = zi;
-                    zi = zn;
-                }
-            }
-        }
-#pragma unroll
-        for (int u = 0; u < kU; ++u) a[u] = an[u];
-    }
-    return is1;
-}
-
-// Twist candidate order: smaller |gamma| first, then the larger index (the index the
-// reference's scan keeps among equal minima).
-__device__ __forceinline__ bool tw_better(bool h1, double g1, int t1, bool h2, double g2, int t2) {
-    if (!h1) return false;
-    if (!h2) return true;
-    const double a1 = fabs(g1), a2 = fabs(g2);
-    return a1 < a2 || (a1 == a2 && t1 > t2);
-}
-
-__global__ __launch_bounds__(128) void k_s_tail(const int* __restrict__ DI,
-                                                const LevelCounters* __restrict__ cnt,
-                                                const STask* __restrict__ T,
-                                                const MatDesc* __restrict__ md,
-                                                const MatState* __restrict__ ms,
-                                                const int* __restrict__ Windex,
-                                                const int* __restrict__ Zidx, double* W,
-                                                double* Wsh, double* Werr,
-                                                const double* __restrict__ Wgap, SState* ST) {
-    __shared__ double SV[kTailN + 1];   // S values, then L+, then z above the twist
-    __shared__ double PV[kTailN + 1];   // P values, then U-, then z below the twist (shifted)
-    __shared__ double sh_slast;
-    __shared__ int sh_i[8];             // nan1, neg1, nan2, neg2, is0, is1
-    __shared__ double tw_g[2];
-    __shared__ int tw_t[2], tw_h[2];
-    if (cnt->nS > kTailS) return;
-    const int nD = cnt->nD;
-    const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
-    for (int k = blockIdx.x; k < nD; k += gridDim.x) {
-        const int idx = DI[k];
-        const STask t = T[idx];
-        const int n = t.bsize;
-        SState q = ST[idx];
-        int usedBS = q.flags & 1, usedRQ = (q.flags >> 1) & 1, needBS = (q.flags >> 2) & 1;
-        int wantnc = (q.flags >> 3) & 1, state = sflag_state(q.flags);
-        if (!(state == kSActive || state == kSStep2) || n < 2) continue;   // uniform
-        const MatDesc& dsc = md[t.mat];
-        const MatState& S = ms[t.mat];
-        int col = Zidx[t.slot];
-        if (col >= dsc.mcols) col = dsc.mcols - 1;
-        const int bbeg = t.bbeg;
-        const int i = t.slot - dsc.off;
-        const int bl_begin = bbeg, bl_end = bbeg + n - 1;
-        const double pivmin = S.pivmin;
-        const double tol = 4.0 * log((double)n) * DBL_EPSILON;
-        const double w0 = Wsh[t.slot], e0 = Werr[t.slot];
-        const double lg = (i == bl_begin) ? DBL_EPSILON * fmax(fabs(w0 - e0), fabs(w0 + e0)) : t.lgap;
-        const double rg = (i == bl_end) ? DBL_EPSILON * fmax(fabs(w0 - e0), fabs(w0 + e0)) : Wgap[t.slot];
-        const double gap = fmin(lg, rg);
-        const double gaptol = (i == bl_begin || i == bl_end) ? 0.0 : gap * DBL_EPSILON;
-        const int i_local = Windex[t.slot];
-        const bool s2 = state == kSStep2;
-        double lambda = s2 ? q.bstw : q.lam;
-        if (!s2) wantnc = usedBS ? 0 : 1;
-        const int rin = q.r;
-        const int r1 = rin ? rin : 1, r2 = rin ? rin : n;
-        const bool freetw = rin == 0;
-        // the two transforms, one lane of each wave
-        if (wv == 0 && lane == 0) {
-            double sfin, slast;
-            int neg1;
-            tail_stat(as_const(t.D), as_const(t.DL), as_const(t.L), t.DLL, lambda, pivmin, r1, r2,
-                      false, SV, sfin, neg1, slast);
-            const int nan1 = isnan(sfin);
-            if (nan1)   // slow stationary form (odr1v.f:202-228)
-                tail_stat(as_const(t.D), as_const(t.DL), as_const(t.L), t.DLL, lambda, pivmin, r1,
-                          r2, true, SV, sfin, neg1, slast);
-            sh_i[0] = nan1;
-            sh_i[1] = neg1;
-            sh_slast = slast;
-        }
-        if (wv == 1 && lane == 0) {
-            double pfin;
-            int neg2;
-            tail_prog(n, as_const(t.D), as_const(t.DLL), as_const(t.L), lambda, pivmin, r1, false,
-                      PV, pfin, neg2);
-            const int nan2 = isnan(pfin);
-            if (nan2)   // slow progressive form (odr1v.f:244-257)
-                tail_prog(n, as_const(t.D), as_const(t.DLL), as_const(t.L), lambda, pivmin, r1,
-                          true, PV, pfin, neg2);
-            sh_i[2] = nan2;
-            sh_i[3] = neg2;
-        }
-        __syncthreads();
-        const bool nan1 = sh_i[0], nan2 = sh_i[2];
-        const double slast = sh_slast;
-        // twist (odr1v.f:259-283): gamma(t) = S(t-1) + P(t) for t = 2 .. n
-        double mingma, g0raw;
-        int r;
-        if (freetw) {
-            bool have = false;
-            double best = 0.0;
-            int rb = 0;
-            for (int tt = 2 + tid; tt <= n; tt += 128) {
-                const double sg = SV[tt - 1];
-                double gm = sg + PV[tt];
-                if (gm == 0.0) gm = kEps * sg;
-                if (!isnan(gm) && tw_better(true, gm, tt, have, best, rb)) {
-                    have = true;
-                    best = gm;
-                    rb = tt;
-                }
-            }
-            for (int o = 32; o > 0; o >>= 1) {
-                const bool h2 = __shfl_xor((int)have, o);
-                const double g2 = __shfl_xor(best, o);
-                const int t2 = __shfl_xor(rb, o);
-                if (tw_better(h2, g2, t2, have, best, rb)) { have = true; best = g2; rb = t2; }
-            }
-            if (lane == 0) { tw_h[wv] = have; tw_g[wv] = best; tw_t[wv] = rb; }
-            __syncthreads();
-            have = tw_h[0];
-            best = tw_g[0];
-            rb = tw_t[0];
-            if (tw_better(tw_h[1], tw_g[1], tw_t[1], have, best, rb)) {
-                have = true;
-                best = tw_g[1];
-                rb = tw_t[1];
-            }
-            g0raw = 0.0 + PV[1];   // gamma(1) = S(0) + P(1)
-            double g0 = g0raw;
-            if (g0 == 0.0) g0 = kEps * 0.0;
-            if (isnan(g0) || !have || fabs(g0) < fabs(best)) { mingma = g0; r = 1; }
-            else { mingma = best; r = rb; }
-        } else {
-            g0raw = slast + PV[r1];
-            mingma = g0raw;
-            if (fabs(mingma) == 0.0) mingma = kEps * slast;
-            r = r1;
-        }
-        const int negcnt = wantnc ? sh_i[1] + (g0raw < 0.0 ? 1 : 0) + sh_i[3] : -1;
-        // L+(j) for j < r and U-(t) for t >= r, each from its own carry
-        {
-            double v[kTailN / 128];
-#pragma unroll
-            for (int u = 0; u < kTailN / 128; ++u) {
-                const int j = 1 + tid + 128 * u;
-                v[u] = 0.0;
-                if (j <= n - 1) {
-                    if (j < r) {
-                        double dplus = t.D[j - 1] + (SV[j - 1] - lambda);
-                        if (nan1 && fabs(dplus) < pivmin) dplus = -pivmin;
-                        v[u] = t.DL[j - 1] / dplus;
-                    } else {
-                        double dminus = t.DLL[j - 1] + PV[j + 1];
-                        if (nan2 && fabs(dminus) < pivmin) dminus = -pivmin;
-                        v[u] = t.L[j - 1] * (t.D[j - 1] / dminus);
-                    }
-                }
-            }
-            __syncthreads();
-#pragma unroll
-            for (int u = 0; u < kTailN / 128; ++u) {
-                const int j = 1 + tid + 128 * u;
-                if (j <= n - 1) {
-                    if (j < r) SV[j] = v[u];
-                    else PV[j] = v[u];
-                }
-            }
-        }
-        __syncthreads();
-        // the two halves of the solve, one lane of each wave
-        const bool nan = nan1 || nan2;
-        if (wv == 0 && lane == 0) sh_i[4] = tail_solve_up(r, as_const(t.DL), nan, gaptol, SV);
-        if (wv == 1 && lane == 0) sh_i[5] = tail_solve_down(n, r, as_const(t.DL), nan, gaptol, PV);
-        __syncthreads();
-        const int is0 = sh_i[4], is1 = sh_i[5];
-        // the column: z for the rows of the support, Z(r) = 1
-        double* Zc = dsc.Z + (size_t)col * dsc.ldz + bbeg - 1;
-        for (int row = is0 + tid; row <= is1; row += 128)
-            Zc[row] = row < r ? SV[row] : (row == r ? 1.0 : PV[row - 1]);
-        if (tid == 0) {
-            // ZTZ in the reference's order: the upper half from the twist up, then the
-            // lower half from the twist down
-            double ztz = 1.0;
-            for (int j = r - 1; j >= is0; --j) ztz = ztz + SV[j] * SV[j];
-            for (int tt = r; tt <= is1 - 1; ++tt) ztz = ztz + PV[tt] * PV[tt];
-            R1v o;
-            o.negcnt = negcnt;
-            o.r = r;
-            o.nan1 = nan1;
-            o.nan2 = nan2;
-            o.mingma = mingma;
-            const double tmp = 1.0 / ztz;
-            o.ztz = ztz;
-            o.nrminv = sqrt(tmp);
-            o.resid = fabs(mingma) * o.nrminv;
-            o.rqcorr = mingma * tmp;
-            o.isupp0 = is0;
-            o.isupp1 = is1;
-            s_update(q, o, s2, lambda, usedBS, usedRQ, needBS, wantnc, state, i_local, tol, gap, dsc,
-                     col, bbeg, t, n, W, Wsh, Werr, ST, idx);
-            add_steps(kStepR1v, (unsigned long long)n);
-        }
-        __syncthreads();
-    }
-}
-
 // dlarrb (odrrb.f:135-299) for one eigenvalue by a group of G lanes (G a power of
 // two, 4 or more; the group is lanes gbase .. gbase + G - 1 of the wave and every lane
 // of it passes the same arguments).  The bracket expansion evaluates G consecutive
@@ -3163,6 +2804,47 @@ __global__ __launch_bounds__(64) void k_s_bisect(const int* __restrict__ BI,
         s_bisect_one(b0 + (int)threadIdx.x / G, nb, G, BI, T, ms, Windex, ST);
 }
 
+// The bisection fallback of one singleton on a group of G lanes of this wave (lanes
+// gbase .. gbase + G - 1, sub = lane - gbase; every lane of the group passes the same
+// singleton): dlarrb to full accuracy from the current bracket (process_singleton.c:
+// 186-204), then the state the next dlar1v starts from.  Returns the reference's
+// one-point sweeps; *rounds gets the group's sweeps.
+__device__ __forceinline__ long long s_bisect_state(SState& q, const STask& t,
+                                                    const MatState& S, int i, int G, int sub,
+                                                    int gbase, bool valid, int* rounds,
+                                                    bool one_singleton = false) {
+    const int n = t.bsize;
+    const double pivmin = S.pivmin;
+    const double spdiam = t.bl_spdiam;
+    int r = q.r;
+    if (r < 1 || r > n) r = n;
+    const double* d = t.D;
+    const double* lld = t.DLL;
+    double w = q.wsh, we = q.werr;
+    long long sweeps;
+    if (one_singleton) {   // the whole wave serves this singleton: scalar loads only
+        auto count = [&](double x) -> int {
+            return dlaneg(rfl(n), as_const(d), as_const(lld), x, rfl(r));
+        };
+        sweeps = dlarrb_grp(G, sub, gbase, valid, count, i, 0.0, 2.0 * DBL_EPSILON, 0.0, pivmin,
+                            spdiam, w, we, rounds);
+    } else {
+        const bool uni = wave_uniform(d) && wave_uniform(lld) && wave_uniform(n) && wave_uniform(r);
+        auto count = [&](double x) -> int {
+            return uni ? dlaneg(rfl(n), as_const(d), as_const(lld), x, rfl(r))
+                       : dlaneg(n, d, lld, x, r);
+        };
+        sweeps = dlarrb_grp(G, sub, gbase, valid, count, i, 0.0, 2.0 * DBL_EPSILON, 0.0, pivmin,
+                            spdiam, w, we, rounds);
+    }
+    q.wsh = w;
+    q.werr = we;
+    q.lam = w;
+    q.r = 0;
+    q.flags |= 1;   // usedBS
+    return sweeps;
+}
+
 __device__ __forceinline__ void s_bisect_one(int b, int nb, int G, const int* __restrict__ BI,
                                              const STask* __restrict__ T,
                                              const MatState* __restrict__ ms,
@@ -3173,33 +2855,412 @@ __device__ __forceinline__ void s_bisect_one(int b, int nb, int G, const int* __
     const int idx = BI[min(b, nb - 1)];
     const STask t = T[idx];
     SState q = ST[idx];
-    const int n = t.bsize;
-    const int i = Windex[t.slot];
-    const double pivmin = ms[t.mat].pivmin;
-    const double spdiam = t.bl_spdiam;
-    int r = q.r;
-    if (r < 1 || r > n) r = n;
-    const double* d = t.D;
-    const double* lld = t.DLL;
-    const bool uni = wave_uniform(d) && wave_uniform(lld) && wave_uniform(n) && wave_uniform(r);
-    auto count = [&](double x) -> int {
-        return uni ? dlaneg(rfl(n), as_const(d), as_const(lld), x, rfl(r)) : dlaneg(n, d, lld, x, r);
-    };
-    const int gbase = grp * G;
-    double w = q.wsh, we = q.werr;
     int rounds = 0;
-    const long long sweeps = dlarrb_grp(G, sub, gbase, valid, count, i, 0.0, 2.0 * DBL_EPSILON, 0.0,
-                                        pivmin, spdiam, w, we, &rounds);
+    const long long sweeps = s_bisect_state(q, t, ms[t.mat], Windex[t.slot], G, sub, grp * G, valid,
+                                            &rounds);
     if (valid && sub == 0) {
-        add_steps(kStepR1vBis, (unsigned long long)n * (unsigned long long)sweeps);
-        q.wsh = w;
-        q.werr = we;
-        q.lam = w;
-        q.r = 0;
-        q.flags |= 1;   // usedBS
+        add_steps(kStepR1vBis, (unsigned long long)t.bsize * (unsigned long long)sweeps);
         ST[idx] = q;
     }
-    if (lane == 0) add_steps(kStepBisWave, (unsigned long long)n * rounds);
+    if (lane == 0) add_steps(kStepBisWave, (unsigned long long)t.bsize * rounds);
+}
+
+// ---------------------------------------------------------------------------------
+// The tail of a level's singleton iterations: once few singletons go on (at most
+// kTailS), each one of a block up to kTailN rows gets a workgroup of two waves, which
+// runs all the iterations left to it in the bulk schedule one after the other (with
+// the bisection fallback on its first wave when an iteration asks for it), so the
+// level's tail costs the chain of its slowest singleton instead of one round of
+// launches per iteration.  The
+// stationary transform (wave 0) and the progressive transform (wave 1) run at the
+// same time, each on one lane, keeping their S and P values in LDS; the twist search
+// is a reduction over all lanes; L+ and U- are recomputed from the stored carries by
+// all lanes (each value needs only its own carry); the two halves of the vector solve
+// run on one lane of each wave again.  Every operation and its order are those of
+// r1v_ck and pass_solve (odr1v.f:156-369), so the results are bit-identical; only
+// the lanes that carry them differ.
+// ---------------------------------------------------------------------------------
+
+// Prefetch depth of the one-lane sweeps of the tail kernel (scalar loads; a deeper
+// buffer spills scalar registers).
+constexpr int kUT = 4;
+
+// Stationary transform over steps 1 .. stop-1 (odr1v.f:184-229) on one lane, with S
+// after step t in SV[t] (SV[0] = 0).
+template <class P>
+__device__ __forceinline__ void tail_stat(P D, P LD, P L, const double* LLD, double lambda,
+                                          double pivmin, int r1, int stop, bool safe,
+                                          double* SV, double& s_out, int& neg_out,
+                                          double& slast_out) {
+    double s = 0.0 - lambda;
+    int neg = 0;
+    SV[0] = 0.0;
+    double a[kUT], b[kUT], c[kUT], an[kUT], bn[kUT], cn[kUT];
+#pragma unroll
+    for (int u = 0; u < kUT; ++u) { a[u] = D[u]; b[u] = LD[u]; c[u] = L[u]; }
+    for (int t0 = 1; t0 < stop; t0 += kUT) {
+#pragma unroll
+        for (int u = 0; u < kUT; ++u) {
+            const int j = t0 - 1 + kUT + u;
+            an[u] = D[j]; bn[u] = LD[j]; cn[u] = L[j];
+        }
+#pragma unroll
+        for (int u = 0; u < kUT; ++u) {
+            const int t = t0 + u;
+            if (t < stop) {
+                double lp, si;
+                bool ng;
+                stat_step(a[u], b[u], c[u], LLD, t, lambda, safe, pivmin, s, lp, si, ng);
+                neg += (t <= r1 - 1) && ng;
+                SV[t] = si;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kUT; ++u) { a[u] = an[u]; b[u] = bn[u]; c[u] = cn[u]; }
+    }
+    s_out = s;
+    neg_out = neg;
+    slast_out = stop > 1 ? SV[stop - 1] : 0.0;
+}
+
+// Progressive transform from step n-1 down to lo (odr1v.f:231-257) on one lane, with P
+// after step t in PV[t] and PV[n] = D(n) - lambda.
+template <class P>
+__device__ __forceinline__ void tail_prog(int n, P D, P LLD, P L, double lambda, double pivmin,
+                                          int lo, bool safe, double* PV, double& p_out,
+                                          int& neg_out) {
+    double p = D[n - 1] - lambda;
+    PV[n] = p;
+    int neg = 0;
+    double a[kUT], b[kUT], c[kUT], an[kUT], bn[kUT], cn[kUT];
+#pragma unroll
+    for (int u = 0; u < kUT; ++u) {
+        const int j = n - 2 - u;
+        a[u] = D[j]; b[u] = LLD[j]; c[u] = L[j];
+    }
+    for (int t0 = n - 1; t0 >= lo; t0 -= kUT) {
+#pragma unroll
+        for (int u = 0; u < kUT; ++u) {
+            const int j = t0 - kUT - u - 1;
+            an[u] = D[j]; bn[u] = LLD[j]; cn[u] = L[j];
+        }
+#pragma unroll
+        for (int u = 0; u < kUT; ++u) {
+            const int t = t0 - u;
+            if (t >= lo) {
+                double um;
+                bool ng;
+                prog_step(a[u], b[u], c[u], lambda, safe, pivmin, p, um, ng);
+                neg += ng;
+                PV[t] = p;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kUT; ++u) { a[u] = an[u]; b[u] = bn[u]; c[u] = cn[u]; }
+    }
+    p_out = p;
+    neg_out = neg;
+}
+
+// Upward half of the vector solve (odr1v.f:285-320) on one lane: Z(j) for j = r-1 down
+// to the support cut, read L+(j) from SV[j] and written there.  Returns ISUPPZ(1).
+template <class P>
+__device__ __forceinline__ int tail_solve_up(int r, P LD, bool nan, double gaptol, double* SV) {
+    double zi1 = 1.0, zi2 = 0.0;
+    int is0 = 1;
+    bool live = r - 1 >= 1;
+    double a[kUT], an[kUT];
+#pragma unroll
+    for (int u = 0; u < kUT; ++u) a[u] = LD[r - 2 - u];
+    for (int j0 = r - 1; j0 >= 1 && live; j0 -= kUT) {
+#pragma unroll
+        for (int u = 0; u < kUT; ++u) an[u] = LD[j0 - kUT - u - 1];
+#pragma unroll
+        for (int u = 0; u < kUT; ++u) {
+            const int j = j0 - u;
+            if (live && j >= 1) {
+                const double ldj = a[u];
+                double zi;
+                if (nan && zi1 == 0.0) zi = -(LD[j] / ldj) * zi2;
+                else zi = -(SV[j] * zi1);
+                if ((fabs(zi) + fabs(zi1)) * fabs(ldj) < gaptol) {
+                    is0 = j + 1;
+                    live = false;
+                } else {
+                    SV[j] = zi;
+                    zi2 = zi1;
+                    zi1 = zi;
+                }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kUT; ++u) a[u] = an[u];
+    }
+    return is0;
+}
+
+// Downward half (odr1v.f:321-355) on one lane: Z(t+1) for t = r .. n-1 up to the cut,
+// read U-(t) from PV[t] and written there.  Returns ISUPPZ(2).
+template <class P>
+__device__ __forceinline__ int tail_solve_down(int n, int r, P LD, bool nan, double gaptol,
+                                               double* PV) {
+    double zi = 1.0, zim1 = 0.0;
+    int is1 = n;
+    bool live = r <= n - 1;
+    double a[kUT], an[kUT];
+#pragma unroll
+    for (int u = 0; u < kUT; ++u) a[u] = LD[r - 1 + u];
+    for (int t0 = r; t0 <= n - 1 && live; t0 += kUT) {
+#pragma unroll
+        for (int u = 0; u < kUT; ++u) an[u] = LD[t0 + kUT + u - 1];
+#pragma unroll
+        for (int u = 0; u < kUT; ++u) {
+            const int t = t0 + u;
+            if (live && t <= n - 1) {
+                const double ldt = a[u];
+                double zn;
+                if (nan && zi == 0.0) zn = -(LD[max(t - 2, 0)] / ldt) * zim1;
+                else zn = -(PV[t] * zi);
+                if ((fabs(zi) + fabs(zn)) * fabs(ldt) < gaptol) {
+                    is1 = t;
+                    live = false;
+                } else {
+                    PV[t] = zn;
+                    zim1 = zi;
+                    zi = zn;
+                }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kUT; ++u) a[u] = an[u];
+    }
+    return is1;
+}
+
+// Twist candidate order: smaller |gamma| first, then the larger index (the index the
+// reference's scan keeps among equal minima).
+__device__ __forceinline__ bool tw_better(bool h1, double g1, int t1, bool h2, double g2, int t2) {
+    if (!h1) return false;
+    if (!h2) return true;
+    const double a1 = fabs(g1), a2 = fabs(g2);
+    return a1 < a2 || (a1 == a2 && t1 > t2);
+}
+
+__global__ __launch_bounds__(128) void k_s_tail(const int* __restrict__ DI,
+                                                LevelCounters* cnt,
+                                                const STask* __restrict__ T,
+                                                const MatDesc* __restrict__ md,
+                                                const MatState* __restrict__ ms,
+                                                const int* __restrict__ Windex,
+                                                const int* __restrict__ Zidx, double* W,
+                                                double* Wsh, double* Werr,
+                                                const double* __restrict__ Wgap, SState* ST,
+                                                int iters) {
+    __shared__ double SV[kTailN + 1];   // S values, then L+, then z above the twist
+    __shared__ double PV[kTailN + 1];   // P values, then U-, then z below the twist (shifted)
+    __shared__ double sh_slast;
+    __shared__ int sh_i[8];             // nan1, neg1, nan2, neg2, is0, is1
+    __shared__ double tw_g[2];
+    __shared__ int tw_t[2], tw_h[2];
+    __shared__ SState sq;               // the singleton's state between its iterations
+    __shared__ int s_k;
+    if (cnt->nS > kTailS) return;
+    const int nD = cnt->nD;
+    const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
+    for (;;) {
+        // singletons are handed out one at a time as workgroups free up (their chains
+        // differ by up to ten iterations and a bisection)
+        __syncthreads();
+        if (tid == 0) s_k = atomicAdd(&cnt->nextD, 1);
+        __syncthreads();
+        const int k = s_k;
+        if (k >= nD) break;
+        const int idx = DI[k];
+        const STask t = T[idx];
+        const int n = t.bsize;
+        SState q = ST[idx];
+        if (n < 2) continue;   // uniform
+        const MatDesc& dsc = md[t.mat];
+        const MatState& S = ms[t.mat];
+        int col = Zidx[t.slot];
+        if (col >= dsc.mcols) col = dsc.mcols - 1;
+        const int bbeg = t.bbeg;
+        const int i = t.slot - dsc.off;
+        const int bl_begin = bbeg, bl_end = bbeg + n - 1;
+        const double pivmin = S.pivmin;
+        const double tol = 4.0 * log((double)n) * DBL_EPSILON;
+        const double w0 = Wsh[t.slot], e0 = Werr[t.slot];
+        const double lg = (i == bl_begin) ? DBL_EPSILON * fmax(fabs(w0 - e0), fabs(w0 + e0)) : t.lgap;
+        const double rg = (i == bl_end) ? DBL_EPSILON * fmax(fabs(w0 - e0), fabs(w0 + e0)) : Wgap[t.slot];
+        const double gap = fmin(lg, rg);
+        const double gaptol = (i == bl_begin || i == bl_end) ? 0.0 : gap * DBL_EPSILON;
+        const int i_local = Windex[t.slot];
+        // the iterations left to this singleton in the level's bulk schedule, one after
+        // the other (the bisection of the first one has run in k_s_bisect)
+        for (int itk = 0; itk < iters; ++itk) {
+            int usedBS = q.flags & 1, usedRQ = (q.flags >> 1) & 1, needBS = (q.flags >> 2) & 1;
+            int wantnc = (q.flags >> 3) & 1, state = sflag_state(q.flags);
+            if (!(state == kSActive || state == kSStep2)) break;   // uniform
+            if (itk > 0 && state == kSActive && needBS) {
+                // the bisection fallback (process_singleton.c:186-204) on wave 0
+                if (wv == 0) {
+                    int rounds = 0;
+                    const long long sw = s_bisect_state(q, t, S, i_local, SW, lane, 0, true, &rounds, true);
+                    if (lane == 0) {
+                        add_steps(kStepR1vBis, (unsigned long long)n * (unsigned long long)sw);
+                        sq = q;
+                    }
+                }
+                __syncthreads();
+                q = sq;
+                __syncthreads();
+                usedBS = q.flags & 1;
+            }
+            const bool s2 = state == kSStep2;
+            double lambda = s2 ? q.bstw : q.lam;
+            if (!s2) wantnc = usedBS ? 0 : 1;
+            const int rin = q.r;
+            const int r1 = rin ? rin : 1, r2 = rin ? rin : n;
+            const bool freetw = rin == 0;
+            // the two transforms, one lane of each wave
+            if (wv == 0 && lane == 0) {
+                double sfin, slast;
+                int neg1;
+                tail_stat(as_const(t.D), as_const(t.DL), as_const(t.L), t.DLL, lambda, pivmin, r1, r2,
+                          false, SV, sfin, neg1, slast);
+                const int nan1 = isnan(sfin);
+                if (nan1)   // slow stationary form (odr1v.f:202-228)
+                    tail_stat(as_const(t.D), as_const(t.DL), as_const(t.L), t.DLL, lambda, pivmin, r1,
+                              r2, true, SV, sfin, neg1, slast);
+                sh_i[0] = nan1;
+                sh_i[1] = neg1;
+                sh_slast = slast;
+            }
+            if (wv == 1 && lane == 0) {
+                double pfin;
+                int neg2;
+                tail_prog(n, as_const(t.D), as_const(t.DLL), as_const(t.L), lambda, pivmin, r1, false,
+                          PV, pfin, neg2);
+                const int nan2 = isnan(pfin);
+                if (nan2)   // slow progressive form (odr1v.f:244-257)
+                    tail_prog(n, as_const(t.D), as_const(t.DLL), as_const(t.L), lambda, pivmin, r1,
+                              true, PV, pfin, neg2);
+                sh_i[2] = nan2;
+                sh_i[3] = neg2;
+            }
+            __syncthreads();
+            const bool nan1 = sh_i[0], nan2 = sh_i[2];
+            const double slast = sh_slast;
+            // twist (odr1v.f:259-283): gamma(t) = S(t-1) + P(t) for t = 2 .. n
+            double mingma, g0raw;
+            int r;
+            if (freetw) {
+                bool have = false;
+                double best = 0.0;
+                int rb = 0;
+                for (int tt = 2 + tid; tt <= n; tt += 128) {
+                    const double sg = SV[tt - 1];
+                    double gm = sg + PV[tt];
+                    if (gm == 0.0) gm = kEps * sg;
+                    if (!isnan(gm) && tw_better(true, gm, tt, have, best, rb)) {
+                        have = true;
+                        best = gm;
+                        rb = tt;
+                    }
+                }
+                for (int o = 32; o > 0; o >>= 1) {
+                    const bool h2 = __shfl_xor((int)have, o);
+                    const double g2 = __shfl_xor(best, o);
+                    const int t2 = __shfl_xor(rb, o);
+                    if (tw_better(h2, g2, t2, have, best, rb)) { have = true; best = g2; rb = t2; }
+                }
+                if (lane == 0) { tw_h[wv] = have; tw_g[wv] = best; tw_t[wv] = rb; }
+                __syncthreads();
+                have = tw_h[0];
+                best = tw_g[0];
+                rb = tw_t[0];
+                if (tw_better(tw_h[1], tw_g[1], tw_t[1], have, best, rb)) {
+                    have = true;
+                    best = tw_g[1];
+                    rb = tw_t[1];
+                }
+                g0raw = 0.0 + PV[1];   // gamma(1) = S(0) + P(1)
+                double g0 = g0raw;
+                if (g0 == 0.0) g0 = kEps * 0.0;
+                if (isnan(g0) || !have || fabs(g0) < fabs(best)) { mingma = g0; r = 1; }
+                else { mingma = best; r = rb; }
+            } else {
+                g0raw = slast + PV[r1];
+                mingma = g0raw;
+                if (fabs(mingma) == 0.0) mingma = kEps * slast;
+                r = r1;
+            }
+            const int negcnt = wantnc ? sh_i[1] + (g0raw < 0.0 ? 1 : 0) + sh_i[3] : -1;
+            // L+(j) for j < r and U-(t) for t >= r, each from its own carry, in place: L+
+            // in blocks of 128 rows from the top of its range down and U- from the bottom
+            // of its range up, so that no block overwrites a carry a later block reads
+            {
+                const int nl = r - 1, nu = n - r;   // rows 1 .. r-1 and r .. n-1
+                const int nb = (max(nl, nu) + 127) / 128;
+                for (int bk = 0; bk < nb; ++bk) {
+                    const int j = r - 1 - 128 * bk - tid;   // L+ row
+                    const int u = r + 128 * bk + tid;       // U- row
+                    double lv = 0.0, uv = 0.0;
+                    if (j >= 1 && j <= r - 1) {
+                        double dplus = t.D[j - 1] + (SV[j - 1] - lambda);
+                        if (nan1 && fabs(dplus) < pivmin) dplus = -pivmin;
+                        lv = t.DL[j - 1] / dplus;
+                    }
+                    if (u >= r && u <= n - 1) {
+                        double dminus = t.DLL[u - 1] + PV[u + 1];
+                        if (nan2 && fabs(dminus) < pivmin) dminus = -pivmin;
+                        uv = t.L[u - 1] * (t.D[u - 1] / dminus);
+                    }
+                    __syncthreads();
+                    if (j >= 1 && j <= r - 1) SV[j] = lv;
+                    if (u >= r && u <= n - 1) PV[u] = uv;
+                    __syncthreads();
+                }
+            }
+            // the two halves of the solve, one lane of each wave
+            const bool nan = nan1 || nan2;
+            if (wv == 0 && lane == 0) sh_i[4] = tail_solve_up(r, as_const(t.DL), nan, gaptol, SV);
+            if (wv == 1 && lane == 0) sh_i[5] = tail_solve_down(n, r, as_const(t.DL), nan, gaptol, PV);
+            __syncthreads();
+            const int is0 = sh_i[4], is1 = sh_i[5];
+            // the column: z for the rows of the support, Z(r) = 1
+            double* Zc = dsc.Z + (size_t)col * dsc.ldz + bbeg - 1;
+            for (int row = is0 + tid; row <= is1; row += 128)
+                Zc[row] = row < r ? SV[row] : (row == r ? 1.0 : PV[row - 1]);
+            if (tid == 0) {
+                // ZTZ in the reference's order: the upper half from the twist up, then the
+                // lower half from the twist down
+                double ztz = 1.0;
+                for (int j = r - 1; j >= is0; --j) ztz = ztz + SV[j] * SV[j];
+                for (int tt = r; tt <= is1 - 1; ++tt) ztz = ztz + PV[tt] * PV[tt];
+                R1v o;
+                o.negcnt = negcnt;
+                o.r = r;
+                o.nan1 = nan1;
+                o.nan2 = nan2;
+                o.mingma = mingma;
+                const double tmp = 1.0 / ztz;
+                o.ztz = ztz;
+                o.nrminv = sqrt(tmp);
+                o.resid = fabs(mingma) * o.nrminv;
+                o.rqcorr = mingma * tmp;
+                o.isupp0 = is0;
+                o.isupp1 = is1;
+                s_update(q, o, s2, lambda, usedBS, usedRQ, needBS, wantnc, state, i_local, tol, gap, dsc,
+                         col, bbeg, t, n, W, Wsh, Werr, ST, idx);
+                add_steps(kStepR1v, (unsigned long long)n);
+                sq = q;
+            }
+            __syncthreads();
+            q = sq;
+            __syncthreads();
+        }
+    }
 }
 
 // Collects the singletons that go on iterating (per block, segmented like the
