@@ -44,6 +44,8 @@ static inline unsigned cdiv(long long a, long long b) { return (unsigned)((a + b
 // to a wave.  Multisection with G lanes does (G - 1) / log2(G) times the sweeps of
 // one-point bisection in 1 / log2(G) of the sequential rounds.
 constexpr long long kGroupLanes = 262144;
+// Largest level (clusters) whose dlarrf trials run four lanes per cluster.
+constexpr int kRrfGroupMax = 1 << 16;
 static int group_lanes(long long items) {
     if (items * 4 > kGroupLanes) return 1;
     int G = 4;
@@ -550,8 +552,15 @@ int run_pipeline(std::vector<MatDesc>& md, std::vector<MatState>& ms, hipStream_
                 else
                     LAUNCH(k_endref_grp, dim3(cdiv(2LL * nC, 64 / Ge)), dim3(64), Cl[cur], nC, Ge,
                            Windex, Wsh, Werr, dms, Wtmp, Werrtmp);
-                LAUNCH(k_dlarrf, dim3(cdiv(nC, 64)), dim3(64), Cl[cur], nC, Wsh, Werr, Wgap, Wtmp,
-                       Werrtmp, dms);
+                // four lanes per cluster unless the clusters alone fill the device
+                if (nC > kRrfGroupMax) {
+                    LAUNCH(k_dlarrf<1>, dim3(cdiv(nC, 64)), dim3(64), Cl[cur], nC, Wsh, Werr, Wgap,
+                           Wtmp, Werrtmp, dms);
+                } else {
+                    LAUNCH(k_dlarrf<4>, dim3(cdiv(nC, 16)), dim3(64), Cl[cur], nC, Wsh, Werr, Wgap,
+                           Wtmp, Werrtmp, dms);
+                    LAUNCH(k_rrf_store, dim3(cdiv(nC, 64)), dim3(64), Cl[cur], nC, dms);
+                }
                 LAUNCH(k_newrrr_fin, dim3(nC), dim3(64), Cl[cur], nC, Wsh, Werr);
                 const int Gr = group_lanes(nE);
                 if (Gr == 1)

@@ -1495,64 +1495,132 @@ __device__ __forceinline__ void rrf_trial_w(int n, const double* d, const double
         rrf_trial(n, d, l, ld, sig, pivmin, mx, nan);
 }
 
-__device__ int dlarrf_lane(int n, const double* __restrict__ d, const double* __restrict__ l,
-                           const double* __restrict__ ld, const double* w, const double* wgap,
-                           const double* werr, int clsize, double spdiam, double clgapl,
-                           double clgapr, double pivmin, double* sigma, double* dplus,
-                           double* lplus, double* work) {
-    // w/wgap/werr point at the cluster's first eigenvalue; CLSTRT = 1, CLEND = clsize.
+// The trial factorisation with its factors stored (for the refined growth test); the
+// loads run kU steps ahead, as in rrf_trial.
+__device__ __forceinline__ void rrf_trial_store(int n, const double* d, const double* l,
+                                                const double* ld, double sig, double pivmin,
+                                                double* Dst, double* Lst) {
+    double s = -sig;
+    double dp = d[0] + s;
+    if (fabs(dp) < pivmin) dp = -pivmin;
+    Dst[0] = dp;
+    double a[kU], b[kU], c[kU], an[kU], bn[kU], cn[kU];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) { a[u] = ld[u]; b[u] = l[u]; c[u] = d[u + 1]; }
+    for (int i0 = 0; i0 < n - 1; i0 += kU) {
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            const int i = i0 + kU + u;
+            an[u] = ld[i]; bn[u] = l[i]; cn[u] = d[i + 1];
+        }
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            const int i = i0 + u;
+            if (i < n - 1) {
+                const double lp = a[u] / dp;
+                Lst[i] = lp;
+                s = s * lp * b[u] - sig;
+                dp = c[u] + s;
+                if (fabs(dp) < pivmin) dp = -pivmin;
+                Dst[i + 1] = dp;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kU; ++u) { a[u] = an[u]; b[u] = bn[u]; c[u] = cn[u]; }
+    }
+}
+
+// The refined growth measure of odrrf.f:265-305 over a D+ array A and an L+ array B
+// (the reference pairs the D+ of one trial with the L+ of the other); loads kU ahead.
+__device__ __forceinline__ double rrf_rrr(int n, const double* A, const double* B, double spdiam) {
+    double tmp = fabs(A[n - 1]), znm2 = 1.0, prod = 1.0, oldp = 1.0;
+    double a[kU], b[kU], a1[kU], b1[kU], an[kU], bn[kU];
+    // step i reads A[i], B[i] and A[i+1], B[i+1]
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+        const int i = n - 2 - u;
+        a[u] = A[i]; b[u] = B[i]; a1[u] = A[i + 1]; b1[u] = B[i + 1];
+    }
+    for (int i0 = n - 2; i0 >= 0; i0 -= kU) {
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            const int i = i0 - kU - u;
+            an[u] = A[i]; bn[u] = B[i];
+        }
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            const int i = i0 - u;
+            if (i >= 0) {
+                if (prod <= kEps) prod = ((a1[u] * b1[u]) / (a[u] * b[u])) * oldp;
+                else prod = prod * fabs(b[u]);
+                oldp = prod;
+                znm2 = znm2 + prod * prod;
+                tmp = fmax(tmp, fabs(a[u] * prod));
+            }
+        }
+        // the next group's A[i+1], B[i+1] are this group's A[i], B[i] shifted by one
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            a1[u] = (u == 0) ? a[kU - 1] : an[u - 1];
+            b1[u] = (u == 0) ? b[kU - 1] : bn[u - 1];
+            a[u] = an[u];
+            b[u] = bn[u];
+        }
+    }
+    return tmp / (spdiam * sqrt(znm2));
+}
+
+// The cluster's end values that dlarrf reads (after the ends were refined): W, WERR
+// of the first and last eigenvalue, WGAP of the first and of the one before the last,
+// and the gap to the right of the cluster.
+struct RrfEnds {
+    double w0, wl, e0, el, g0, gl, clgapr;
+};
+
+// dlarrf (odrrf.f:140-384) for one cluster.  trial(k, side, mx, nan) gives the growth
+// and NaN flag of the trial factorisation of try k (0 or 1) at the left (side 0) or
+// right (side 1) shift; stored() writes both trial factorisations of the current try
+// (left to dplus/lplus, right to work/work + n) for the refined growth test.  The
+// trial at the best shift after the last try is accepted without a test, so it is not
+// computed (its result is never read).  Returns the shift side; *sigma the shift.
+template <class TrialF, class StoreF>
+__device__ __forceinline__ int dlarrf_core(int n, const RrfEnds& e, int clsize, double spdiam,
+                                           double clgapl, double pivmin, double* sigma,
+                                           const double* dplus, const double* lplus,
+                                           const double* work, TrialF trial, StoreF stored,
+                                           int* sweeps_out) {
     const int KTRYMAX = 1;
     const double fact = (double)(1 << KTRYMAX);
     int shift = 0, forcer = 0, indx = 0;
-    const double clwdth = fabs(w[clsize - 1] - w[0]) + werr[clsize - 1] + werr[0];
+    const double clwdth = fabs(e.wl - e.w0) + e.el + e.e0;
     const double avgap = clwdth / (double)(clsize - 1);
-    const double mingap = fmin(clgapl, clgapr);
-    double lsigma = fmin(w[0], w[clsize - 1]) - werr[0];
-    double rsigma = fmax(w[0], w[clsize - 1]) + werr[clsize - 1];
+    const double mingap = fmin(clgapl, e.clgapr);
+    double lsigma = fmin(e.w0, e.wl) - e.e0;
+    double rsigma = fmax(e.w0, e.wl) + e.el;
     lsigma = lsigma - fabs(lsigma) * 4.0 * kEps;
     rsigma = rsigma + fabs(rsigma) * 4.0 * kEps;
     const double ldmax = 0.25 * mingap + 2.0 * pivmin;
     const double rdmax = 0.25 * mingap + 2.0 * pivmin;
-    double ldelta = fmax(avgap, wgap[0]) / fact;
-    double rdelta = fmax(avgap, wgap[clsize - 2]) / fact;
+    double ldelta = fmax(avgap, e.g0) / fact;
+    double rdelta = fmax(avgap, e.gl) / fact;
     double smlgrowth = 1.0 / kSafmin;
-    const double fail = (double)(n - 1) * mingap / (spdiam * kEps);
     const double fail2 = (double)(n - 1) * mingap / (spdiam * sqrt(kEps));
     double bestshift = lsigma;
     int ktry = 0;
     const double growthbound = kMaxGrowth1 * spdiam;
     double max1 = 0.0, max2 = 0.0;
     int sweeps = 0;
-    // the trial with its factors stored, for the refined growth test below (the chosen
-    // factorisation itself is written by the caller)
-    auto trial = [&](double sig, double* Dst, double* Lst, bool store, double& mx, int& nan) {
-        double s = -sig;
-        double dp = d[0] + s;
-        nan = 0;
-        if (fabs(dp) < pivmin) { dp = -pivmin; nan = 1; }
-        mx = fabs(dp);
-        if (store) Dst[0] = dp;
-        for (int i = 0; i < n - 1; ++i) {
-            const double lp = ld[i] / dp;
-            if (store) Lst[i] = lp;
-            s = s * lp * l[i] - sig;
-            dp = d[i + 1] + s;
-            if (fabs(dp) < pivmin) { dp = -pivmin; nan = 1; }
-            mx = fmax(mx, fabs(dp));
-            if (store) Dst[i + 1] = dp;
-        }
-        nan = nan || isnan(mx);
-    };
     for (;;) {
         int sawnan1 = 0, sawnan2 = 0;
         ldelta = fmin(ldmax, ldelta);
         rdelta = fmin(rdmax, rdelta);
+        if (forcer) { *sigma = lsigma; shift = 1; break; }
         ++sweeps;
-        rrf_trial_w(n, d, l, ld, lsigma, pivmin, max1, sawnan1);
-        if (forcer || (max1 <= growthbound && !sawnan1)) { *sigma = lsigma; shift = 1; break; }
+        trial(ktry, 0, lsigma, max1, sawnan1);
+        if (max1 <= growthbound && !sawnan1) { *sigma = lsigma; shift = 1; break; }
         ++sweeps;
-        rrf_trial_w(n, d, l, ld, rsigma, pivmin, max2, sawnan2);
-        if (forcer || (max2 <= growthbound && !sawnan2)) { *sigma = rsigma; shift = 2; break; }
+        trial(ktry, 1, rsigma, max2, sawnan2);
+        if (max2 <= growthbound && !sawnan2) { *sigma = rsigma; shift = 2; break; }
         if (!(sawnan1 && sawnan2)) {
             if (!sawnan1) {
                 indx = 1;
@@ -1565,32 +1633,13 @@ __device__ int dlarrf_lane(int n, const double* __restrict__ d, const double* __
             int dorrr1 = (clwdth < mingap / 128.0) && (fmin(max1, max2) < fail2) && !sawnan1 && !sawnan2;
             if (dorrr1) {
                 // the refined test reads both factorisations (odrrf.f:265-305)
-                double m1, m2;
-                int n1, n2;
-                trial(lsigma, dplus, lplus, true, m1, n1);
-                trial(rsigma, work, work + n, true, m2, n2);
+                stored(lsigma, rsigma);
                 sweeps += 2;
                 if (indx == 1) {
-                    double tmp = fabs(dplus[n - 1]), znm2 = 1.0, prod = 1.0, oldp = 1.0;
-                    for (int i = n - 2; i >= 0; --i) {
-                        if (prod <= kEps) prod = ((dplus[i + 1] * work[n + i + 1]) / (dplus[i] * work[n + i])) * oldp;
-                        else prod = prod * fabs(work[n + i]);
-                        oldp = prod;
-                        znm2 = znm2 + prod * prod;
-                        tmp = fmax(tmp, fabs(dplus[i] * prod));
-                    }
-                    double rrr1 = tmp / (spdiam * sqrt(znm2));
+                    const double rrr1 = rrf_rrr(n, dplus, work + n, spdiam);
                     if (rrr1 <= kMaxGrowth2) { *sigma = lsigma; shift = 1; break; }
                 } else if (indx == 2) {
-                    double tmp = fabs(work[n - 1]), znm2 = 1.0, prod = 1.0, oldp = 1.0;
-                    for (int i = n - 2; i >= 0; --i) {
-                        if (prod <= kEps) prod = ((work[i + 1] * lplus[i + 1]) / (work[i] * lplus[i])) * oldp;
-                        else prod = prod * fabs(lplus[i]);
-                        oldp = prod;
-                        znm2 = znm2 + prod * prod;
-                        tmp = fmax(tmp, fabs(work[i] * prod));
-                    }
-                    double rrr2 = tmp / (spdiam * sqrt(znm2));
+                    const double rrr2 = rrf_rrr(n, work, lplus, spdiam);
                     if (rrr2 <= kMaxGrowth2) { *sigma = rsigma; shift = 2; break; }
                 }
             }
@@ -1607,11 +1656,48 @@ __device__ int dlarrf_lane(int n, const double* __restrict__ d, const double* __
         lsigma = bestshift;
         rsigma = bestshift;
         forcer = 1;
-        (void)fail;
     }
-    add_steps(kStepRrf, (unsigned long long)sweeps * n);
-    add_steps(kStepRrfN, (unsigned long long)n);
+    *sweeps_out = sweeps;
     return shift;
+}
+
+// The shifts of tries 0 and 1 as dlarrf_core forms them (left and right of each).
+__device__ __forceinline__ void rrf_shifts(const RrfEnds& e, int clsize, double clgapl,
+                                           double pivmin, double sh[4]) {
+    const double fact = 2.0;
+    const double clwdth = fabs(e.wl - e.w0) + e.el + e.e0;
+    const double avgap = clwdth / (double)(clsize - 1);
+    const double mingap = fmin(clgapl, e.clgapr);
+    double lsigma = fmin(e.w0, e.wl) - e.e0;
+    double rsigma = fmax(e.w0, e.wl) + e.el;
+    lsigma = lsigma - fabs(lsigma) * 4.0 * kEps;
+    rsigma = rsigma + fabs(rsigma) * 4.0 * kEps;
+    const double ldmax = 0.25 * mingap + 2.0 * pivmin;
+    const double rdmax = 0.25 * mingap + 2.0 * pivmin;
+    const double ldelta = fmin(ldmax, fmax(avgap, e.g0) / fact);
+    const double rdelta = fmin(rdmax, fmax(avgap, e.gl) / fact);
+    sh[0] = lsigma;
+    sh[1] = rsigma;
+    sh[2] = fmax(lsigma - ldelta, lsigma - ldmax);
+    sh[3] = fmin(rsigma + rdelta, rsigma + rdmax);
+}
+
+// The refined ends of cluster t as dlarrf reads them, and their commit
+// (process_cluster.c:186-189): WGAP(cb) from the refined left end and the neighbour
+// as it was before the right end was refined.
+__device__ __forceinline__ RrfEnds rrf_ends(const ClTask& t, const double* Wsh, const double* Werr,
+                                            const double* Wgap, const double* Wtmp,
+                                            const double* Werrtmp) {
+    const int cb = t.cb, ce = t.ce;
+    RrfEnds e;
+    e.w0 = Wtmp[cb];
+    e.e0 = Werrtmp[cb];
+    e.wl = Wtmp[ce];
+    e.el = Werrtmp[ce];
+    e.g0 = fmax(0.0, (Wsh[cb + 1] - Werr[cb + 1]) - (Wtmp[cb] + Werrtmp[cb]));
+    e.gl = (ce - 1 == cb) ? e.g0 : Wgap[ce - 1];
+    e.clgapr = Wgap[ce];
+    return e;
 }
 
 // Extremal eigenvalues of every cluster to full accuracy against the parent
@@ -1698,13 +1784,104 @@ __device__ __forceinline__ void rrf_store(int n, int nmax, P d, P l, P ld, doubl
     (void)valid;
 }
 
-// New representation of every cluster by dlarrf (odrrf.f:140-384), one lane per
-// cluster.  The left-shift factorisation goes to cD/cL, the right-shift one to the
-// cDL/cDLL space; k_newrrr_fin keeps the chosen one.
+// The shift of every cluster's new representation by dlarrf (odrrf.f:140-384), with
+// G = 1 or 4 lanes per cluster.  With 4, the lanes evaluate the trials at the left and
+// right shifts of both tries at once (the shifts do not depend on the trial results),
+// and every lane of the group then runs the reference's sequence of tests on them, so
+// the shift is the same as with one lane; the refined growth test's two stored trials
+// run on two lanes side by side.  The ends refined by k_endref are committed here.
+// With one lane per cluster the kernel also stores the chosen factorisation; with four,
+// k_rrf_store does.
+template <int G>
 __global__ __launch_bounds__(64) void k_dlarrf(ClTask* C, int nC, double* Wsh, double* Werr,
                                                double* Wgap, const double* __restrict__ Wtmp,
                                                const double* __restrict__ Werrtmp,
                                                const MatState* __restrict__ ms) {
+    const int lane = threadIdx.x;
+    const int sub = lane % G, gbase = lane - sub;
+    const int c = blockIdx.x * (64 / G) + lane / G;
+    const bool valid = c < nC;
+    ClTask& t = C[min(c, nC - 1)];
+    const int cb = t.cb, ce = t.ce, clsize = ce - cb + 1, n = t.bsize;
+    const RrfEnds e = rrf_ends(t, Wsh, Werr, Wgap, Wtmp, Werrtmp);
+    const double pivmin = ms[t.mat].pivmin;
+    double tau = 0.0;
+    int sw = 0, sh;
+    if constexpr (G == 1) {
+        auto trial = [&](int, int, double sig, double& mx, int& nan) {
+            rrf_trial_w(n, t.pD, t.pL, t.pDL, sig, pivmin, mx, nan);
+        };
+        auto stored = [&](double ls, double rs) {
+            if (!valid) return;   // lanes past the last cluster only replay it
+            rrf_trial_store(n, t.pD, t.pL, t.pDL, ls, pivmin, t.cD, t.cL);
+            rrf_trial_store(n, t.pD, t.pL, t.pDL, rs, pivmin, t.cDL, t.cDL + n);
+        };
+        sh = dlarrf_core(n, e, clsize, t.bl_spdiam, t.lgap, pivmin, &tau, t.cD, t.cL, t.cDL, trial,
+                         stored, &sw);
+    } else {
+        double shv[4];
+        rrf_shifts(e, clsize, t.lgap, pivmin, shv);
+        double mx;
+        int nn;
+        rrf_trial_w(n, t.pD, t.pL, t.pDL, shv[sub & 3], pivmin, mx, nn);
+        double pm[4];
+        int pn[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            pm[k] = __shfl(mx, gbase + k);
+            pn[k] = __shfl(nn, gbase + k);
+        }
+        auto trial = [&](int k, int side, double, double& m2, int& nan2) {
+            m2 = pm[2 * k + side];
+            nan2 = pn[2 * k + side];
+        };
+        auto stored = [&](double ls, double rs) {
+            if (valid && sub < 2)   // lanes past the last cluster only replay it
+                rrf_trial_store(n, t.pD, t.pL, t.pDL, sub == 0 ? ls : rs, pivmin,
+                                sub == 0 ? t.cD : t.cDL, sub == 0 ? t.cL : t.cDL + n);
+            __threadfence_block();   // the group reads both stored trials next
+        };
+        sh = dlarrf_core(n, e, clsize, t.bl_spdiam, t.lgap, pivmin, &tau, t.cD, t.cL, t.cDL, trial,
+                         stored, &sw);
+    }
+    if (valid && sub == 0) {
+        add_steps(kStepRrf, (unsigned long long)sw * n);
+        add_steps(kStepRrfN, (unsigned long long)n);
+        Wgap[cb] = e.g0;
+        Wsh[cb] = e.w0;
+        Werr[cb] = e.e0;
+        Wsh[ce] = e.wl;
+        Werr[ce] = e.el;
+        t.tau = tau;
+        t.shift = sh;
+    }
+    if constexpr (G == 1) {
+        // one lane per cluster: the chosen factorisation is stored right away, while
+        // the parent representation is still in the caches
+        __shared__ double bD[16][65], bL[16][65];
+        __shared__ double* dD[64];
+        __shared__ double* dL[64];
+        __shared__ int nn[64];
+        const double *pd = t.pD, *pl = t.pL, *pld = t.pDL;
+        dD[lane] = t.cD;
+        dL[lane] = t.cL;
+        nn[lane] = valid ? n : 0;
+        int nmax = valid ? n : 0;
+        for (int o = 32; o > 0; o >>= 1) nmax = max(nmax, __shfl_xor(nmax, o));
+        const double tv = valid ? tau : 0.0;
+        if (wave_uniform(pd) && wave_uniform(pl) && wave_uniform(pld) && wave_uniform(n))
+            rrf_store(rfl(n), nmax, as_const(pd), as_const(pl), as_const(pld), tv, pivmin, valid,
+                      lane, bD, bL, dD, dL, nn);
+        else
+            rrf_store(n, nmax, pd, pl, pld, tv, pivmin, valid, lane, bD, bL, dD, dL, nn);
+    }
+}
+
+// The chosen factorisation L+ D+ L+^T = L D L^T - tau I of every cluster, recomputed
+// with the operations of the accepted trial, into cD (D+) and cL (L+); one lane per
+// cluster, stored through LDS (rrf_store).
+__global__ __launch_bounds__(64) void k_rrf_store(const ClTask* __restrict__ C, int nC,
+                                                  const MatState* __restrict__ ms) {
     __shared__ double bD[16][65], bL[16][65];
     __shared__ double* dD[64];
     __shared__ double* dL[64];
@@ -1712,45 +1889,18 @@ __global__ __launch_bounds__(64) void k_dlarrf(ClTask* C, int nC, double* Wsh, d
     const int lane = threadIdx.x;
     const int c = blockIdx.x * blockDim.x + lane;
     const bool valid = c < nC;
-    double tau = 0.0, pivmin = 0.0;
-    int n = 0;
-    const double *pd = nullptr, *pl = nullptr, *pld = nullptr;
-    if (valid) {
-        ClTask& t = C[c];
-        const int cb = t.cb, ce = t.ce, clsize = ce - cb + 1;
-        // Wgap[cb] from the refined left end and the neighbour as it was before the
-        // right end was refined (process_cluster.c:186-189), then commit both ends
-        Wgap[cb] = fmax(0.0, (Wsh[cb + 1] - Werr[cb + 1]) - (Wtmp[cb] + Werrtmp[cb]));
-        Wsh[cb] = Wtmp[cb];
-        Werr[cb] = Werrtmp[cb];
-        Wsh[ce] = Wtmp[ce];
-        Werr[ce] = Werrtmp[ce];
-        pivmin = ms[t.mat].pivmin;
-        int sh = dlarrf_lane(t.bsize, t.pD, t.pL, t.pDL, Wsh + cb, Wgap + cb, Werr + cb, clsize,
-                             t.bl_spdiam, t.lgap, Wgap[ce], pivmin, &tau, t.cD, t.cL, t.cDL);
-        t.tau = tau;
-        t.shift = sh;
-        n = t.bsize;
-        pd = t.pD; pl = t.pL; pld = t.pDL;
-        dD[lane] = t.cD;
-        dL[lane] = t.cL;
-    }
-    nn[lane] = n;
-    // The chosen factorisation L+ D+ L+^T = L D L^T - tau I, recomputed with the
-    // operations of the accepted trial, into cD (D+) and cL (L+).  Each lane writes
-    // chunks of 16 values to LDS; the wave then stores 16 contiguous rows of four
-    // clusters per instruction, so no cache line is left partly written.
-    int nmax = n;
-    for (int o = 32; o > 0; o >>= 1) nmax = max(nmax, __shfl_xor(nmax, o));
     // lanes past the last cluster sweep the last cluster's parent (nothing is stored
     // for them), so every lane reads valid memory
-    if (!valid) {
-        const ClTask& tl = C[nC - 1];
-        pd = tl.pD; pl = tl.pL; pld = tl.pDL;
-        n = tl.bsize;
-        tau = 0.0;
-        pivmin = ms[tl.mat].pivmin;
-    }
+    const ClTask& t = C[min(c, nC - 1)];
+    const double tau = valid ? t.tau : 0.0;
+    const double pivmin = ms[t.mat].pivmin;
+    int n = t.bsize;
+    const double *pd = t.pD, *pl = t.pL, *pld = t.pDL;
+    dD[lane] = t.cD;
+    dL[lane] = t.cL;
+    nn[lane] = valid ? n : 0;
+    int nmax = valid ? n : 0;
+    for (int o = 32; o > 0; o >>= 1) nmax = max(nmax, __shfl_xor(nmax, o));
     if (wave_uniform(pd) && wave_uniform(pl) && wave_uniform(pld) && wave_uniform(n))
         rrf_store(rfl(n), nmax, as_const(pd), as_const(pl), as_const(pld), tau, pivmin, valid, lane,
                   bD, bL, dD, dL, nn);
