@@ -20,7 +20,8 @@ import numpy as np
 __all__ = ["lib", "mrrr", "solve_batch", "prepare_batch", "run_batch", "Problem", "Server", "Request", "build", "LIB_PATH"]
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libmr3hip.so")
+# MR3_LIB selects another build of the library (a debug or profiling build)
+LIB_PATH = os.environ.get("MR3_LIB") or os.path.join(HERE, "libmr3hip.so")
 _lib = None
 _P = ctypes.c_void_p
 

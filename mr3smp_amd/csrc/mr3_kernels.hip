@@ -2311,10 +2311,11 @@ __device__ __forceinline__ void write_chunk(const ChunkBuf& buf, const ColOut& c
     co.wlo[lane] = rlo;
     co.whi[lane] = rhi;
     __syncthreads();
-    const int sub = lane & 15, cq = lane >> 4;
+    // KC rows of SW / KC columns per store instruction
+    const int sub = lane % KC, cq = lane / KC;
 #pragma unroll 4
-    for (int cg = 0; cg < SW / 4; ++cg) {
-        const int c = cg * 4 + cq;
+    for (int cg = 0; cg < KC; ++cg) {
+        const int c = cg * (SW / KC) + cq;
         if (c < co.cntw) {
             const int row = co.wr0[c] + sub;
             if (row >= co.wlo[c] && row <= co.whi[c]) co.zcol[c][co.bbeg + row - 1] = buf[sub][c];
@@ -2719,11 +2720,22 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
         const unsigned long long tk0 = wall_clock64();
         r1v_ck(n, lambda, R, pivmin, wantnc, q.r, act, ck, buf, lane, &o);
         const unsigned long long tk1 = wall_clock64();
-        pass_solve(n, R, lambda, pivmin, gaptol, ck, buf, bld, lane, &o, act, act, 1.0, co);
+        // The first solve of a level almost never gives the final vector (nearly every
+        // singleton takes a second dlar1v), so it does not write its column; a singleton
+        // that finishes here solves once more and writes (the same operations on the
+        // same checkpoints give the same z).
+        bool sa = act, sw = act && !first;
+        for (int pass = 0; pass < 2; ++pass) {
+            pass_solve(n, R, lambda, pivmin, gaptol, ck, buf, bld, lane, &o, sa, sw, 1.0, co);
+            if (pass == 1) break;
+            if (act)
+                s_update(q, o, s2, lambda, usedBS, usedRQ, needBS, wantnc, state, i_local, tol, gap,
+                         dsc, col, bbeg, t, n, W, Wsh, Werr, ST, idx);
+            if (!first) break;
+            sa = sw = act && sflag_state(q.flags) == kSDone;
+            if (!__any(sa)) break;
+        }
         const unsigned long long tk2 = wall_clock64();
-        if (act)
-            s_update(q, o, s2, lambda, usedBS, usedRQ, needBS, wantnc, state, i_local, tol, gap, dsc,
-                     col, bbeg, t, n, W, Wsh, Werr, ST, idx);
         add_steps(kStepR1v, act ? (unsigned long long)n : 0ULL);
         if (lane == 0) {
             add_steps(kStepR1vWave, (unsigned long long)n);
