@@ -34,6 +34,14 @@ KERNEL_STEPS = {"k_coarse": ["coarse"], "k_refine_root": ["refine"], "k_endref":
                 "k_refine_cl": ["refine_cl"]}
 FP64_PEAK_TFLOPS = 78.6
 HBM_PEAK_GBS = 8000.0
+# The divide-bound ceiling: an IEEE FP64 division step issues in about 110 cycles per
+# wave64 on one SIMD (measured with an in-register probe, the same at 1-8 waves per
+# SIMD), so 256 CUs x 4 SIMDs x 2.4 GHz x 64 lanes / 110 cycles divide steps per second.
+DIVIDE_STEPS_PER_S = 256 * 4 * 2.4e9 * 64 / 110.0
+# divisions the reference's algorithm needs per counted step of each family (dlar1v: the
+# stationary and the progressive transform; the vector solve multiplies)
+DIVIDES_PER_STEP = {"coarse": 1, "refine": 1, "endref": 1, "r1v": 2, "r1v_bisect": 1,
+                    "rrf": 1, "rrj": 1, "refine_cl": 1}
 
 
 def log(*a):
@@ -230,6 +238,11 @@ def main():
     per_launch_s = kern[dom]["ms"] / 1e3 / kern[dom]["launches"]
     flops_per_launch = flops / launches_per_step
     achieved = flops_per_launch / per_launch_s / 1e12 if per_launch_s > 0 else 0.0
+    # the same kernel's ceiling when every SIMD issues division steps back to back
+    fams = KERNEL_STEPS.get(dom, [])
+    fl = sum(FLOPS_PER_STEP[f] * steps.get(f, 0) for f in fams)
+    dv = sum(DIVIDES_PER_STEP[f] * steps.get(f, 0) for f in fams)
+    divide_ceiling = DIVIDE_STEPS_PER_S * fl / dv / 1e12 if dv else None
     traffic = None
     tfile = os.path.join(REPO, "profiles", "traffic.json")
     if os.path.exists(tfile):
@@ -281,6 +294,8 @@ def main():
         "roofline": {"bound": "mfma", "engine": "fp64 vector ALU (no MFMA on this path)",
                      "kernel": dom, "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
+                     "divide_ceiling": divide_ceiling,
+                     "frac_of_divide_ceiling": (achieved / divide_ceiling) if divide_ceiling else None,
                      "traffic": traffic, "flops_per_launch": flops_per_launch,
                      "ms_per_launch": per_launch_s * 1e3},
         "z_writeback": {"bytes_per_step": z_bytes / prof_steps,
