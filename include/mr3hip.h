@@ -33,6 +33,54 @@ void mrrr_(char *jobz, char *range, int *n, double *D, double *E, double *vl,
            int *ldz, int *Zsupp, int *info);
 
 /*
+ * Dense symmetric eigenproblem, replacing the reference routine
+ *   int dsyeig(char *jobz, char *range, char *uplo, int *n, double *A, int *lda,
+ *              double *vl, double *vu, int *il, int *iu, int *m, double *W,
+ *              double *Z, int *ldz);
+ * (INCLUDE/mrrr.h:270-272, SRC/dense.c:151-230).  A is column-major, only its uplo
+ * triangle is read, and it is overwritten by the Householder reduction; the matrix is
+ * scaled into the safe range as dense.c:476-526 does, reduced on the GPU, solved by
+ * the tridiagonal MRRR pipeline and back-transformed on the GPU.  Returns 0, 1 for an
+ * invalid argument (dense.c:174-183, and lda or ldz < n), 2 on a device failure.
+ */
+int dsyeig(char *jobz, char *range, char *uplo, int *n, double *A, int *lda,
+           double *vl, double *vu, int *il, int *iu, int *m, double *W, double *Z,
+           int *ldz);
+
+/*
+ * Generalized symmetric-definite eigenproblem, replacing the reference routine
+ *   int dsygeig(int *itype, char *jobz, char *range, char *uplo, int *n, double *A,
+ *               int *lda, double *B, int *ldb, double *vl, double *vu, int *il,
+ *               int *iu, int *m, double *W);
+ * (INCLUDE/mrrr.h:352-356, SRC/dense.c:236-316).  itype 1: A x = lambda B x,
+ * 2: A B x = lambda x, 3: B A x = lambda x.  On return the first m columns of A hold
+ * the eigenvectors and B its Cholesky factor.  Returns 0, 1 for an invalid argument
+ * (dense.c:254-258, and the range arguments as dsyeig checks them), 2 if B is not
+ * positive definite or the device fails (the reference asserts there).
+ */
+int dsygeig(int *itype, char *jobz, char *range, char *uplo, int *n, double *A,
+            int *lda, double *B, int *ldb, double *vl, double *vu, int *il, int *iu,
+            int *m, double *W);
+
+/* Fortran bindings of the two dense routines (SRC/dense.c:1037-1054). */
+void dsyeig_(char *jobz, char *range, char *uplo, int *n, double *A, int *lda,
+             double *vl, double *vu, int *il, int *iu, int *m, double *W, double *Z,
+             int *ldz, int *info);
+void dsygeig_(int *itype, char *jobz, char *range, char *uplo, int *n, double *A,
+              int *lda, double *B, int *ldb, double *vl, double *vu, int *il, int *iu,
+              int *m, double *W, int *info);
+
+/*
+ * dsyeig on DEVICE arrays: dA (lda x n) is overwritten by the reduction, dW gets m
+ * eigenvalues and dZ (ldz x zcols) the eigenvectors, on the given stream (NULL = the
+ * library's per-thread stream).  vl, vu, il, iu and m are host scalars as in dsyeig.
+ * No counterpart in the reference, whose dense routines take host arrays only.
+ */
+int mr3_dsyeig_device(char jobz, char range, char uplo, int n, double *dA, int lda,
+                      double *vl, double *vu, int *il, int *iu, int *m, double *dW,
+                      double *dZ, int ldz, int zcols, void *stream);
+
+/*
  * One problem of a batched solve.  All array pointers are DEVICE pointers on the
  * current HIP device; D and E are read only.  Inputs: n, jobz ('N', 'V' or 'C'),
  * range ('A', 'V' or 'I'), vl/vu or il/iu as for mrrr(), tryrac, ldz and mcols

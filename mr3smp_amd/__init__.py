@@ -70,6 +70,13 @@ def lib():
         L = ctypes.CDLL(LIB_PATH)
         L.mrrr.restype = ctypes.c_int
         L.mrrr.argtypes = [ctypes.c_char_p, ctypes.c_char_p] + [_P] * 13
+        L.dsyeig.restype = ctypes.c_int
+        L.dsyeig.argtypes = [ctypes.c_char_p] * 3 + [_P] * 11
+        L.dsygeig.restype = ctypes.c_int
+        L.dsygeig.argtypes = [_P] + [ctypes.c_char_p] * 3 + [_P] * 11
+        L.mr3_dsyeig_device.restype = ctypes.c_int
+        L.mr3_dsyeig_device.argtypes = ([ctypes.c_char] * 3 + [ctypes.c_int, _P, ctypes.c_int]
+                                        + [_P] * 7 + [ctypes.c_int, ctypes.c_int, _P])
         L.mr3_solve_batch.restype = ctypes.c_int
         L.mr3_solve_batch.argtypes = [ctypes.POINTER(Problem), ctypes.c_int, _P]
         L.mr3_solve_one.restype = ctypes.c_int
@@ -130,6 +137,55 @@ def mrrr(jobz, rng, D, E, vl=0.0, vu=0.0, il=1, iu=1, tryrac=1, ldz=None, mcols=
                vu=cvu.value, il=cil.value, iu=ciu.value, tryrac=ctr.value)
     out["Z"] = Z[:mm, :n].T.copy() if wantz else None
     return out
+
+
+def dsyeig(jobz, rng, uplo, A, vl=0.0, vu=0.0, il=1, iu=1):
+    """Dense symmetric eigensolver on a host matrix (SRC/dense.c:151-230).
+
+    A (n x n) is read through its uplo triangle and not modified.  Returns a dict
+    with info, m, W[m], Z (n x m or None) and the in/out scalars vl, vu, il, iu.
+    """
+    L = lib()
+    A = np.asarray(A, dtype=np.float64)
+    n = int(A.shape[0])
+    Af = np.asfortranarray(A).copy(order="F")
+    wantz = jobz[0].upper() == "V"
+    mcols = (iu - il + 1) if rng[0].upper() == "I" else n
+    W = np.zeros(max(n, 1))
+    Z = np.zeros((max(n, 1), max(mcols, 1)), order="F") if wantz else None
+    ni, lda, ldz, m = ctypes.c_int(n), ctypes.c_int(max(n, 1)), ctypes.c_int(max(n, 1)), ctypes.c_int(0)
+    cvl, cvu, cil, ciu = ctypes.c_double(vl), ctypes.c_double(vu), ctypes.c_int(il), ctypes.c_int(iu)
+    info = L.dsyeig(jobz.encode(), rng.encode(), uplo.encode(), ctypes.byref(ni), _ptr(Af),
+                    ctypes.byref(lda), ctypes.byref(cvl), ctypes.byref(cvu), ctypes.byref(cil),
+                    ctypes.byref(ciu), ctypes.byref(m), _ptr(W), _ptr(Z), ctypes.byref(ldz))
+    mm = m.value
+    return dict(info=info, m=mm, W=W[:mm].copy(), Z=Z[:, :mm].copy() if wantz else None,
+                vl=cvl.value, vu=cvu.value, il=cil.value, iu=ciu.value)
+
+
+def dsygeig(itype, jobz, rng, uplo, A, B, vl=0.0, vu=0.0, il=1, iu=1):
+    """Generalized symmetric-definite eigensolver on host matrices (SRC/dense.c:236-316).
+
+    itype 1: A x = lambda B x, 2: A B x = lambda x, 3: B A x = lambda x.  A and B are
+    not modified (the routine works on copies).  Returns info, m, W[m], X (n x m or
+    None, the eigenvectors the reference leaves in A) and L (the Cholesky factor it
+    leaves in B).
+    """
+    L = lib()
+    A = np.asarray(A, dtype=np.float64)
+    n = int(A.shape[0])
+    Af = np.asfortranarray(A).copy(order="F")
+    Bf = np.asfortranarray(np.asarray(B, dtype=np.float64)).copy(order="F")
+    W = np.zeros(max(n, 1))
+    it, ni, lda, ldb, m = (ctypes.c_int(itype), ctypes.c_int(n), ctypes.c_int(max(n, 1)),
+                           ctypes.c_int(max(n, 1)), ctypes.c_int(0))
+    cvl, cvu, cil, ciu = ctypes.c_double(vl), ctypes.c_double(vu), ctypes.c_int(il), ctypes.c_int(iu)
+    info = L.dsygeig(ctypes.byref(it), jobz.encode(), rng.encode(), uplo.encode(), ctypes.byref(ni),
+                     _ptr(Af), ctypes.byref(lda), _ptr(Bf), ctypes.byref(ldb), ctypes.byref(cvl),
+                     ctypes.byref(cvu), ctypes.byref(cil), ctypes.byref(ciu), ctypes.byref(m), _ptr(W))
+    mm = m.value
+    wantz = jobz[0].upper() == "V"
+    return dict(info=info, m=mm, W=W[:mm].copy(), X=Af[:, :mm].copy() if wantz else None, L=Bf)
 
 
 def solve_batch(problems, stream=None):

@@ -8,6 +8,7 @@ HIPCC=${HIPCC:-/opt/rocm/bin/hipcc}
 # no packed-FP32 forms.
 FLAGS="--offload-arch=gfx950 -O3 -fPIC -std=c++17 -ffp-contract=off -fno-fast-math \
   -Xclang -target-feature -Xclang -packed-fp32-ops -Wno-unused-result"
-"$HIPCC" $FLAGS -shared -o "$OUT.tmp" "$HERE/mr3_driver.hip" "$HERE/mr3_api.cpp" "$HERE/mr3_serve.cpp" -lpthread
+"$HIPCC" $FLAGS -shared -o "$OUT.tmp" "$HERE/mr3_driver.hip" "$HERE/mr3_dense.hip" \
+  "$HERE/mr3_api.cpp" "$HERE/mr3_serve.cpp" -lrocsolver -lrocblas -lpthread
 mv -f "$OUT.tmp" "$OUT"
 echo "built $OUT"

@@ -53,3 +53,49 @@ def test_workload_generators():
     reqs = WL.served_stream(50, 10.0)
     assert all(np.diff([r["t"] for r in reqs]) >= 0)
     assert {r["range"] for r in reqs} <= {"A", "I", "V"}
+
+
+def test_dense_drivers_reject_invalid_arguments():
+    """dsyeig / dsygeig return 1 for the argument errors of dense.c:174-183 and
+    :254-258 before touching the device (so this runs without a GPU)."""
+    import mr3smp_amd
+    L = mr3smp_amd.lib()
+    P = ctypes.byref
+    A = np.eye(4)
+    W = np.zeros(4)
+    Z = np.zeros((4, 4))
+    a = A.ctypes.data_as(ctypes.c_void_p)
+
+    def syeig(jobz, rng, uplo, n=4, lda=4, vl=0.0, vu=1.0, il=1, iu=4, ldz=4):
+        args = [ctypes.c_int(n), ctypes.c_int(lda), ctypes.c_double(vl), ctypes.c_double(vu),
+                ctypes.c_int(il), ctypes.c_int(iu), ctypes.c_int(0), ctypes.c_int(ldz)]
+        return L.dsyeig(jobz, rng, uplo, P(args[0]), a, P(args[1]), P(args[2]), P(args[3]),
+                        P(args[4]), P(args[5]), P(args[6]), W.ctypes.data_as(ctypes.c_void_p),
+                        Z.ctypes.data_as(ctypes.c_void_p), P(args[7]))
+
+    assert syeig(b"X", b"A", b"L") == 1
+    assert syeig(b"V", b"X", b"L") == 1
+    assert syeig(b"V", b"A", b"X") == 1
+    assert syeig(b"V", b"A", b"L", n=0) == 1
+    assert syeig(b"V", b"A", b"L", lda=3) == 1
+    assert syeig(b"V", b"A", b"L", ldz=3) == 1
+    assert syeig(b"V", b"V", b"L", vl=1.0, vu=1.0) == 1
+    assert syeig(b"V", b"I", b"U", il=0) == 1
+    assert syeig(b"V", b"I", b"U", il=3, iu=2) == 1
+    assert syeig(b"V", b"I", b"U", il=1, iu=5) == 1
+
+    def sygeig(itype, jobz, rng, uplo, n=4):
+        args = [ctypes.c_int(itype), ctypes.c_int(n), ctypes.c_int(4), ctypes.c_int(4),
+                ctypes.c_double(0.0), ctypes.c_double(1.0), ctypes.c_int(1), ctypes.c_int(4),
+                ctypes.c_int(0)]
+        B = np.eye(4)
+        return L.dsygeig(P(args[0]), jobz, rng, uplo, P(args[1]), a, P(args[2]),
+                         B.ctypes.data_as(ctypes.c_void_p), P(args[3]), P(args[4]), P(args[5]),
+                         P(args[6]), P(args[7]), P(args[8]), W.ctypes.data_as(ctypes.c_void_p))
+
+    assert sygeig(1, b"V", b"A", b"L", n=1) == 1
+    assert sygeig(0, b"V", b"A", b"L") == 1
+    assert sygeig(4, b"V", b"A", b"L") == 1
+    assert sygeig(1, b"Q", b"A", b"L") == 1
+    assert sygeig(2, b"V", b"Q", b"L") == 1
+    assert sygeig(3, b"V", b"A", b"Q") == 1
