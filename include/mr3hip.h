@@ -33,42 +33,81 @@ void mrrr_(char *jobz, char *range, int *n, double *D, double *E, double *vl,
            int *ldz, int *Zsupp, int *info);
 
 /*
- * Dense symmetric eigenproblem, replacing the reference routine
- *   int dsyeig(char *jobz, char *range, char *uplo, int *n, double *A, int *lda,
- *              double *vl, double *vu, int *il, int *iu, int *m, double *W,
- *              double *Z, int *ldz);
- * (INCLUDE/mrrr.h:270-272, SRC/dense.c:151-230).  A is column-major, only its uplo
- * triangle is read, and it is overwritten by the Householder reduction; the matrix is
- * scaled into the safe range as dense.c:476-526 does, reduced on the GPU, solved by
- * the tridiagonal MRRR pipeline and back-transformed on the GPU.  Returns 0, 1 for an
- * invalid argument (dense.c:174-183, and lda or ldz < n), 2 on a device failure.
+ * Dense drivers.  Each replaces the reference routine of the same name and prototype
+ * (INCLUDE/mrrr.h:270-806, SRC/dense.c:151-1114), with the same argument meaning:
+ * jobz "N"/"V"/"C", range "A"/"V"/"I", uplo "L"/"U" (first character, any case),
+ * column-major host arrays.  The matrix is scaled into the safe range as
+ * dense.c:476-566 and :940-1034 do, reduced to tridiagonal form on the GPU
+ * (rocSOLVER), solved by the tridiagonal MRRR pipeline and back-transformed on the GPU.
+ * Packed arrays (AP, BP) hold the uplo triangle by columns.  Complex arrays are
+ * interleaved (re, im) pairs of doubles -- the layout of C99 double complex,
+ * std::complex<double> and Fortran COMPLEX*16 -- passed as void *.
+ *
+ * Standard problem (dsyeig, dspeig, zheeig, zhpeig): W gets m eigenvalues, Z
+ * (ldz x m) the eigenvectors.  A / AP is not modified: the reference leaves the
+ * Householder vectors of its reduction there, which no caller can use since their
+ * scalar factors are freed.
+ * Generalized problem (dsygeig, zhegeig: itype 1 A x = lambda B x, 2 A B x =
+ * lambda x, 3 B A x = lambda x): the first m columns of A receive the eigenvectors
+ * and B its Cholesky factor, as in the reference; the packed forms (dspgeig,
+ * zhpgeig) return the eigenvectors in Z and the packed factor in BP.
+ *
+ * Returns 0; 1 for an invalid argument (the checks of dense.c:174-180, :256-260,
+ * :341-347, :422-426 and their complex twins, plus a leading dimension below n and, in the
+ * generalized drivers, the range checks of the standard ones); 2 if B is not positive
+ * definite or the device fails (the reference asserts there).
  */
 int dsyeig(char *jobz, char *range, char *uplo, int *n, double *A, int *lda,
            double *vl, double *vu, int *il, int *iu, int *m, double *W, double *Z,
-           int *ldz);
-
-/*
- * Generalized symmetric-definite eigenproblem, replacing the reference routine
- *   int dsygeig(int *itype, char *jobz, char *range, char *uplo, int *n, double *A,
- *               int *lda, double *B, int *ldb, double *vl, double *vu, int *il,
- *               int *iu, int *m, double *W);
- * (INCLUDE/mrrr.h:352-356, SRC/dense.c:236-316).  itype 1: A x = lambda B x,
- * 2: A B x = lambda x, 3: B A x = lambda x.  On return the first m columns of A hold
- * the eigenvectors and B its Cholesky factor.  Returns 0, 1 for an invalid argument
- * (dense.c:254-258, and the range arguments as dsyeig checks them), 2 if B is not
- * positive definite or the device fails (the reference asserts there).
- */
+           int *ldz);                                        /* mrrr.h:270-272 */
+int dspeig(char *jobz, char *range, char *uplo, int *n, double *AP, double *vl,
+           double *vu, int *il, int *iu, int *m, double *W, double *Z,
+           int *ldz);                                        /* mrrr.h:444-446 */
 int dsygeig(int *itype, char *jobz, char *range, char *uplo, int *n, double *A,
             int *lda, double *B, int *ldb, double *vl, double *vu, int *il, int *iu,
-            int *m, double *W);
+            int *m, double *W);                              /* mrrr.h:352-356 */
+int dspgeig(int *itype, char *jobz, char *range, char *uplo, int *n, double *AP,
+            double *BP, double *vl, double *vu, int *il, int *iu, int *m,
+            double *W, double *Z, int *ldz);                 /* mrrr.h:526-530 */
+int zheeig(char *jobz, char *range, char *uplo, int *n, void *A, int *lda,
+           double *vl, double *vu, int *il, int *iu, int *m, double *W, void *Z,
+           int *ldz);                                        /* mrrr.h:538-541 */
+int zhpeig(char *jobz, char *range, char *uplo, int *n, void *AP, double *vl,
+           double *vu, int *il, int *iu, int *m, double *W, void *Z,
+           int *ldz);                                        /* mrrr.h:718-721 */
+int zhegeig(int *itype, char *jobz, char *range, char *uplo, int *n, void *A,
+            int *lda, void *B, int *ldb, double *vl, double *vu, int *il, int *iu,
+            int *m, double *W);                              /* mrrr.h:624-628 */
+int zhpgeig(int *itype, char *jobz, char *range, char *uplo, int *n, void *AP,
+            void *BP, double *vl, double *vu, int *il, int *iu, int *m, double *W,
+            void *Z, int *ldz);                              /* mrrr.h:802-806 */
 
-/* Fortran bindings of the two dense routines (SRC/dense.c:1037-1054). */
+/* Fortran bindings of the dense drivers (SRC/dense.c:1037-1114): info receives the
+ * return value. */
 void dsyeig_(char *jobz, char *range, char *uplo, int *n, double *A, int *lda,
              double *vl, double *vu, int *il, int *iu, int *m, double *W, double *Z,
              int *ldz, int *info);
+void dspeig_(char *jobz, char *range, char *uplo, int *n, double *AP, double *vl,
+             double *vu, int *il, int *iu, int *m, double *W, double *Z, int *ldz,
+             int *info);
 void dsygeig_(int *itype, char *jobz, char *range, char *uplo, int *n, double *A,
               int *lda, double *B, int *ldb, double *vl, double *vu, int *il, int *iu,
               int *m, double *W, int *info);
+void dspgeig_(int *itype, char *jobz, char *range, char *uplo, int *n, double *AP,
+              double *BP, double *vl, double *vu, int *il, int *iu, int *m,
+              double *W, double *Z, int *ldz, int *info);
+void zheeig_(char *jobz, char *range, char *uplo, int *n, void *A, int *lda,
+             double *vl, double *vu, int *il, int *iu, int *m, double *W, void *Z,
+             int *ldz, int *info);
+void zhpeig_(char *jobz, char *range, char *uplo, int *n, void *AP, double *vl,
+             double *vu, int *il, int *iu, int *m, double *W, void *Z, int *ldz,
+             int *info);
+void zhegeig_(int *itype, char *jobz, char *range, char *uplo, int *n, void *A,
+              int *lda, void *B, int *ldb, double *vl, double *vu, int *il, int *iu,
+              int *m, double *W, int *info);
+void zhpgeig_(int *itype, char *jobz, char *range, char *uplo, int *n, void *AP,
+              void *BP, double *vl, double *vu, int *il, int *iu, int *m, double *W,
+              void *Z, int *ldz, int *info);
 
 /*
  * dsyeig on DEVICE arrays: dA (lda x n) is overwritten by the reduction, dW gets m

@@ -70,10 +70,15 @@ def lib():
         L = ctypes.CDLL(LIB_PATH)
         L.mrrr.restype = ctypes.c_int
         L.mrrr.argtypes = [ctypes.c_char_p, ctypes.c_char_p] + [_P] * 13
-        L.dsyeig.restype = ctypes.c_int
-        L.dsyeig.argtypes = [ctypes.c_char_p] * 3 + [_P] * 11
-        L.dsygeig.restype = ctypes.c_int
-        L.dsygeig.argtypes = [_P] + [ctypes.c_char_p] * 3 + [_P] * 11
+        for name in ("dsyeig", "zheeig"):
+            getattr(L, name).restype = ctypes.c_int
+            getattr(L, name).argtypes = [ctypes.c_char_p] * 3 + [_P] * 11
+        for name in ("dspeig", "zhpeig"):
+            getattr(L, name).restype = ctypes.c_int
+            getattr(L, name).argtypes = [ctypes.c_char_p] * 3 + [_P] * 10
+        for name in ("dsygeig", "zhegeig", "dspgeig", "zhpgeig"):
+            getattr(L, name).restype = ctypes.c_int
+            getattr(L, name).argtypes = [_P] + [ctypes.c_char_p] * 3 + [_P] * 11
         L.mr3_dsyeig_device.restype = ctypes.c_int
         L.mr3_dsyeig_device.argtypes = ([ctypes.c_char] * 3 + [ctypes.c_int, _P, ctypes.c_int]
                                         + [_P] * 7 + [ctypes.c_int, ctypes.c_int, _P])
@@ -139,28 +144,109 @@ def mrrr(jobz, rng, D, E, vl=0.0, vu=0.0, il=1, iu=1, tryrac=1, ldz=None, mcols=
     return out
 
 
+def pack(A, uplo):
+    """The uplo triangle of A packed by columns (LAPACK packed storage)."""
+    A = np.asarray(A)
+    n = A.shape[0]
+    if uplo[0].upper() == "U":
+        return np.concatenate([A[:j + 1, j] for j in range(n)]) if n else A.ravel()
+    return np.concatenate([A[j:, j] for j in range(n)]) if n else A.ravel()
+
+
+def unpack(AP, n, uplo):
+    """Full matrix with the packed triangle of AP in place (the other triangle zero)."""
+    AP = np.asarray(AP)
+    A = np.zeros((n, n), dtype=AP.dtype)
+    k = 0
+    for j in range(n):
+        if uplo[0].upper() == "U":
+            A[:j + 1, j] = AP[k:k + j + 1]
+            k += j + 1
+        else:
+            A[j:, j] = AP[k:k + n - j]
+            k += n - j
+    return A
+
+
+def _dense_standard(name, cplx, packed, jobz, rng, uplo, A, n, vl, vu, il, iu):
+    L = lib()
+    dt = np.complex128 if cplx else np.float64
+    if packed:
+        Ah = np.ascontiguousarray(A, dtype=dt).copy()
+    else:
+        Ah = np.asfortranarray(np.asarray(A, dtype=dt)).copy(order="F")
+        n = int(Ah.shape[0])
+    wantz = jobz[0].upper() == "V"
+    mcols = (iu - il + 1) if rng[0].upper() == "I" else n
+    W = np.zeros(max(n, 1))
+    Z = np.zeros((max(n, 1), max(mcols, 1)), dtype=dt, order="F") if wantz else None
+    ni, lda, ldz, m = ctypes.c_int(n), ctypes.c_int(max(n, 1)), ctypes.c_int(max(n, 1)), ctypes.c_int(0)
+    cvl, cvu, cil, ciu = ctypes.c_double(vl), ctypes.c_double(vu), ctypes.c_int(il), ctypes.c_int(iu)
+    head = [jobz.encode(), rng.encode(), uplo.encode(), ctypes.byref(ni), _ptr(Ah)]
+    if not packed:
+        head.append(ctypes.byref(lda))
+    info = getattr(L, name)(*head, ctypes.byref(cvl), ctypes.byref(cvu), ctypes.byref(cil),
+                            ctypes.byref(ciu), ctypes.byref(m), _ptr(W), _ptr(Z), ctypes.byref(ldz))
+    mm = m.value
+    return dict(info=info, m=mm, W=W[:mm].copy(), Z=Z[:, :mm].copy() if wantz else None,
+                vl=cvl.value, vu=cvu.value, il=cil.value, iu=ciu.value)
+
+
+def _dense_general(name, cplx, packed, itype, jobz, rng, uplo, A, B, n, vl, vu, il, iu):
+    L = lib()
+    dt = np.complex128 if cplx else np.float64
+    if packed:
+        Ah = np.ascontiguousarray(A, dtype=dt).copy()
+        Bh = np.ascontiguousarray(B, dtype=dt).copy()
+    else:
+        Ah = np.asfortranarray(np.asarray(A, dtype=dt)).copy(order="F")
+        Bh = np.asfortranarray(np.asarray(B, dtype=dt)).copy(order="F")
+        n = int(Ah.shape[0])
+    wantz = jobz[0].upper() == "V"
+    mcols = (iu - il + 1) if rng[0].upper() == "I" else n
+    W = np.zeros(max(n, 1))
+    Z = np.zeros((max(n, 1), max(mcols, 1)), dtype=dt, order="F") if (packed and wantz) else None
+    it, ni, ld, m = ctypes.c_int(itype), ctypes.c_int(n), ctypes.c_int(max(n, 1)), ctypes.c_int(0)
+    cvl, cvu, cil, ciu = ctypes.c_double(vl), ctypes.c_double(vu), ctypes.c_int(il), ctypes.c_int(iu)
+    args = [ctypes.byref(it), jobz.encode(), rng.encode(), uplo.encode(), ctypes.byref(ni), _ptr(Ah)]
+    if packed:
+        args += [_ptr(Bh)]
+    else:
+        args += [ctypes.byref(ld), _ptr(Bh), ctypes.byref(ld)]
+    args += [ctypes.byref(cvl), ctypes.byref(cvu), ctypes.byref(cil), ctypes.byref(ciu),
+             ctypes.byref(m), _ptr(W)]
+    if packed:
+        args += [_ptr(Z), ctypes.byref(ld)]
+    info = getattr(L, name)(*args)
+    mm = m.value
+    X = None
+    if wantz:
+        X = Z[:, :mm].copy() if packed else Ah[:, :mm].copy()
+    return dict(info=info, m=mm, W=W[:mm].copy(), X=X, L=Bh)
+
+
 def dsyeig(jobz, rng, uplo, A, vl=0.0, vu=0.0, il=1, iu=1):
     """Dense symmetric eigensolver on a host matrix (SRC/dense.c:151-230).
 
     A (n x n) is read through its uplo triangle and not modified.  Returns a dict
     with info, m, W[m], Z (n x m or None) and the in/out scalars vl, vu, il, iu.
     """
-    L = lib()
-    A = np.asarray(A, dtype=np.float64)
-    n = int(A.shape[0])
-    Af = np.asfortranarray(A).copy(order="F")
-    wantz = jobz[0].upper() == "V"
-    mcols = (iu - il + 1) if rng[0].upper() == "I" else n
-    W = np.zeros(max(n, 1))
-    Z = np.zeros((max(n, 1), max(mcols, 1)), order="F") if wantz else None
-    ni, lda, ldz, m = ctypes.c_int(n), ctypes.c_int(max(n, 1)), ctypes.c_int(max(n, 1)), ctypes.c_int(0)
-    cvl, cvu, cil, ciu = ctypes.c_double(vl), ctypes.c_double(vu), ctypes.c_int(il), ctypes.c_int(iu)
-    info = L.dsyeig(jobz.encode(), rng.encode(), uplo.encode(), ctypes.byref(ni), _ptr(Af),
-                    ctypes.byref(lda), ctypes.byref(cvl), ctypes.byref(cvu), ctypes.byref(cil),
-                    ctypes.byref(ciu), ctypes.byref(m), _ptr(W), _ptr(Z), ctypes.byref(ldz))
-    mm = m.value
-    return dict(info=info, m=mm, W=W[:mm].copy(), Z=Z[:, :mm].copy() if wantz else None,
-                vl=cvl.value, vu=cvu.value, il=cil.value, iu=ciu.value)
+    return _dense_standard("dsyeig", False, False, jobz, rng, uplo, A, None, vl, vu, il, iu)
+
+
+def dspeig(jobz, rng, uplo, AP, n, vl=0.0, vu=0.0, il=1, iu=1):
+    """dsyeig on the packed triangle AP of order n (SRC/dense.c:319-398)."""
+    return _dense_standard("dspeig", False, True, jobz, rng, uplo, AP, n, vl, vu, il, iu)
+
+
+def zheeig(jobz, rng, uplo, A, vl=0.0, vu=0.0, il=1, iu=1):
+    """Dense Hermitian eigensolver (SRC/dense.c:568-672); Z is complex."""
+    return _dense_standard("zheeig", True, False, jobz, rng, uplo, A, None, vl, vu, il, iu)
+
+
+def zhpeig(jobz, rng, uplo, AP, n, vl=0.0, vu=0.0, il=1, iu=1):
+    """zheeig on the packed triangle AP of order n (SRC/dense.c:760-865)."""
+    return _dense_standard("zhpeig", True, True, jobz, rng, uplo, AP, n, vl, vu, il, iu)
 
 
 def dsygeig(itype, jobz, rng, uplo, A, B, vl=0.0, vu=0.0, il=1, iu=1):
@@ -168,24 +254,25 @@ def dsygeig(itype, jobz, rng, uplo, A, B, vl=0.0, vu=0.0, il=1, iu=1):
 
     itype 1: A x = lambda B x, 2: A B x = lambda x, 3: B A x = lambda x.  A and B are
     not modified (the routine works on copies).  Returns info, m, W[m], X (n x m or
-    None, the eigenvectors the reference leaves in A) and L (the Cholesky factor it
-    leaves in B).
+    None, the eigenvectors the reference leaves in A) and L (what the reference leaves
+    in B: its Cholesky factor in the uplo triangle).
     """
-    L = lib()
-    A = np.asarray(A, dtype=np.float64)
-    n = int(A.shape[0])
-    Af = np.asfortranarray(A).copy(order="F")
-    Bf = np.asfortranarray(np.asarray(B, dtype=np.float64)).copy(order="F")
-    W = np.zeros(max(n, 1))
-    it, ni, lda, ldb, m = (ctypes.c_int(itype), ctypes.c_int(n), ctypes.c_int(max(n, 1)),
-                           ctypes.c_int(max(n, 1)), ctypes.c_int(0))
-    cvl, cvu, cil, ciu = ctypes.c_double(vl), ctypes.c_double(vu), ctypes.c_int(il), ctypes.c_int(iu)
-    info = L.dsygeig(ctypes.byref(it), jobz.encode(), rng.encode(), uplo.encode(), ctypes.byref(ni),
-                     _ptr(Af), ctypes.byref(lda), _ptr(Bf), ctypes.byref(ldb), ctypes.byref(cvl),
-                     ctypes.byref(cvu), ctypes.byref(cil), ctypes.byref(ciu), ctypes.byref(m), _ptr(W))
-    mm = m.value
-    wantz = jobz[0].upper() == "V"
-    return dict(info=info, m=mm, W=W[:mm].copy(), X=Af[:, :mm].copy() if wantz else None, L=Bf)
+    return _dense_general("dsygeig", False, False, itype, jobz, rng, uplo, A, B, None, vl, vu, il, iu)
+
+
+def dspgeig(itype, jobz, rng, uplo, AP, BP, n, vl=0.0, vu=0.0, il=1, iu=1):
+    """dsygeig on packed triangles (SRC/dense.c:403-471); L is the packed factor."""
+    return _dense_general("dspgeig", False, True, itype, jobz, rng, uplo, AP, BP, n, vl, vu, il, iu)
+
+
+def zhegeig(itype, jobz, rng, uplo, A, B, vl=0.0, vu=0.0, il=1, iu=1):
+    """Generalized Hermitian-definite eigensolver (SRC/dense.c:676-757)."""
+    return _dense_general("zhegeig", True, False, itype, jobz, rng, uplo, A, B, None, vl, vu, il, iu)
+
+
+def zhpgeig(itype, jobz, rng, uplo, AP, BP, n, vl=0.0, vu=0.0, il=1, iu=1):
+    """zhegeig on packed triangles (SRC/dense.c:869-935)."""
+    return _dense_general("zhpgeig", True, True, itype, jobz, rng, uplo, AP, BP, n, vl, vu, il, iu)
 
 
 def solve_batch(problems, stream=None):

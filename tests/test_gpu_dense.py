@@ -1,12 +1,15 @@
-"""GPU tests of the dense drivers dsyeig / dsygeig (SRC/dense.c:151-316) and of the
+"""GPU tests of the dense drivers (SRC/dense.c:151-1114): dsyeig, dspeig, dsygeig,
+dspgeig, their complex Hermitian twins zheeig, zhpeig, zhegeig, zhpgeig, and the
 device-array variant of dsyeig.
 
 The reference reduces with LAPACK's dsytrd/dormtr and dpotrf/dsygst/dtrsm/dtrmm,
 which it links from the system and does not ship; here rocSOLVER and rocBLAS do
 those steps, so the dense results agree with the reference to rounding, not bit for
 bit.  The bars: eigenvalues within 4 n eps ||A|| of LAPACK's (numpy/scipy eigh),
-residual ||A Z - Z W|| and orthogonality ||Z^T Z - I|| within 20 n eps (scaled by
-||A||), counts exact.  The composition itself is pinned bit for bit on tridiagonal
+residual ||A Z - Z W|| and orthogonality ||Z^H Z - I|| within 20 n eps (scaled by
+||A||), counts exact.  Packed storage is unpacked on the device into the same
+full-storage reduction, so every packed driver must agree bit for bit with its
+full-storage twin.  The composition itself is pinned bit for bit on tridiagonal
 input: there every Householder vector is zero, so dsyeig must return exactly what
 the tridiagonal solver returns for the same D and E.
 """
@@ -26,23 +29,27 @@ def M():
     return mr3smp_amd
 
 
-def sym(n, seed, spread=1.0):
+def sym(n, seed, spread=1.0, cplx=False):
     rng = np.random.default_rng(seed)
     X = rng.standard_normal((n, n)) * spread
-    return (X + X.T) / 2
+    if cplx:
+        X = X + 1j * rng.standard_normal((n, n)) * spread
+    return (X + X.conj().T) / 2
 
 
-def spd(n, seed):
+def spd(n, seed, cplx=False):
     rng = np.random.default_rng(seed)
     X = rng.standard_normal((n, n))
-    return X @ X.T / n + np.eye(n)
+    if cplx:
+        X = X + 1j * rng.standard_normal((n, n))
+    return X @ X.conj().T / n + np.eye(n)
 
 
 def only_triangle(A, uplo):
     """A with the triangle the routine must not read overwritten by garbage."""
     B = A.copy()
     n = len(A)
-    junk = np.full((n, n), np.nan)
+    junk = np.full((n, n), np.nan, dtype=A.dtype)
     if uplo == "L":
         B[np.triu_indices(n, 1)] = junk[np.triu_indices(n, 1)]
     else:
@@ -66,7 +73,7 @@ def check_standard(A, out, Wref, label):
     if Z is not None and out["m"] > 0:
         R = A @ Z - Z * W
         res = np.max(np.abs(R)) / (n * EPS * nrm)
-        orth = np.max(np.abs(Z.T @ Z - np.eye(out["m"]))) / (n * EPS)
+        orth = np.max(np.abs(Z.conj().T @ Z - np.eye(out["m"]))) / (n * EPS)
         assert res <= 20 and orth <= 20, (label, res, orth)
 
 
@@ -217,3 +224,102 @@ def test_dsyeig_device_arrays(M):
     check_standard(A0, out, Wr, "device")
     host = M.dsyeig("V", "A", "L", A0)
     assert np.array_equal(host["W"], W)
+
+
+@pytest.mark.parametrize("uplo", ["L", "U"])
+def test_packed_drivers_equal_full_storage(M, uplo):
+    """dspeig / zhpeig unpack AP into the full-storage reduction: bit-identical W and Z."""
+    for cplx in (False, True):
+        n = 257
+        A = sym(n, 31, cplx=cplx)
+        full, packed = (M.zheeig, M.zhpeig) if cplx else (M.dsyeig, M.dspeig)
+        print(f"packed standard cplx={cplx} uplo={uplo}", flush=True)
+        for kw in (dict(jobz="V", rng="A"), dict(jobz="V", rng="I", il=30, iu=90),
+                   dict(jobz="N", rng="A")):
+            a = full(kw["jobz"], kw["rng"], uplo, only_triangle(A, uplo),
+                     il=kw.get("il", 1), iu=kw.get("iu", 1))
+            b = packed(kw["jobz"], kw["rng"], uplo, M.pack(A, uplo), n,
+                       il=kw.get("il", 1), iu=kw.get("iu", 1))
+            assert a["info"] == 0 and b["info"] == 0 and a["m"] == b["m"], kw
+            assert np.array_equal(a["W"], b["W"]), kw
+            if kw["jobz"] == "V":
+                assert np.array_equal(a["Z"], b["Z"]), kw
+
+
+@pytest.mark.parametrize("uplo", ["L", "U"])
+@pytest.mark.parametrize("n", [1, 2, 33, 400])
+def test_zheeig(M, n, uplo):
+    print(f"zheeig n={n} uplo={uplo}", flush=True)
+    A = sym(n, 200 + n, cplx=True)
+    Wr = np.linalg.eigvalsh(A)
+    out = M.zheeig("V", "A", uplo, only_triangle(A, uplo))
+    assert out["info"] == 0 and out["Z"].dtype == np.complex128
+    check_standard(A, out, Wr, f"zheeig n={n} {uplo}")
+    if n >= 33:
+        il, iu = n // 4, n // 2
+        out = M.zheeig("V", "I", uplo, only_triangle(A, uplo), il=il, iu=iu)
+        assert out["info"] == 0
+        check_standard(A, out, Wr[il - 1:iu], f"zheeig I n={n} {uplo}")
+        vl, vu = (Wr[4] + Wr[5]) / 2, (Wr[n - 6] + Wr[n - 5]) / 2
+        out = M.zheeig("V", "V", uplo, only_triangle(A, uplo), vl=vl, vu=vu)
+        assert out["info"] == 0
+        check_standard(A, out, Wr[5:n - 5], f"zheeig V n={n} {uplo}")
+
+
+def test_zheeig_scaled(M):
+    n = 90
+    A0 = sym(n, 5, cplx=True)
+    Wr = np.linalg.eigvalsh(A0)
+    for s in (1e-160, 1e160):
+        out = M.zheeig("V", "A", "U", A0 * s)
+        assert out["info"] == 0 and out["m"] == n
+        nrm = np.linalg.norm(A0, 2)
+        assert np.max(np.abs(out["W"] / s - Wr)) <= 4 * n * EPS * nrm
+        Z = out["Z"]
+        assert np.max(np.abs(A0 @ Z - Z * (out["W"] / s))) <= 20 * n * EPS * nrm
+
+
+def check_general(itype, A, B, out, Wr, label):
+    n = len(A)
+    W, X = out["W"], out["X"]
+    scale = np.linalg.norm(A, 2) * max(np.linalg.norm(B, 2), np.linalg.norm(np.linalg.inv(B), 2))
+    cb = np.linalg.cond(B)
+    assert out["m"] == len(Wr), label
+    assert np.max(np.abs(W - Wr), initial=0.0) <= 4 * n * EPS * scale * cb, label
+    if X is None:
+        return
+    if itype == 1:
+        R, G = A @ X - (B @ X) * W, X.conj().T @ B @ X
+    elif itype == 2:
+        R, G = A @ (B @ X) - X * W, X.conj().T @ B @ X
+    else:
+        R, G = B @ (A @ X) - X * W, X.conj().T @ np.linalg.solve(B, X)
+    xn = np.max(np.abs(X))
+    assert np.max(np.abs(R)) <= 50 * n * EPS * scale * max(xn, 1.0), label
+    assert np.max(np.abs(G - np.eye(out["m"]))) <= 50 * n * EPS * cb, label
+
+
+@pytest.mark.parametrize("itype", [1, 2, 3])
+@pytest.mark.parametrize("uplo", ["L", "U"])
+def test_zhegeig_and_packed_generalized(M, itype, uplo):
+    import scipy.linalg as sl
+    n = 300
+    for cplx in (True, False):
+        A, B = sym(n, 41, cplx=cplx), spd(n, 42, cplx=cplx)
+        Wr = sl.eigh(A, B, type=itype, eigvals_only=True)
+        full, packed = (M.zhegeig, M.zhpgeig) if cplx else (M.dsygeig, M.dspgeig)
+        print(f"generalized cplx={cplx} itype={itype} uplo={uplo}", flush=True)
+        a = full(itype, "V", "A", uplo, only_triangle(A, uplo), only_triangle(B, uplo))
+        assert a["info"] == 0
+        check_general(itype, A, B, a, Wr, f"full cplx={cplx}")
+        Lc = np.linalg.cholesky(B)
+        tri = np.tril(a["L"]) if uplo == "L" else np.triu(a["L"])
+        ref = Lc if uplo == "L" else Lc.conj().T
+        assert np.max(np.abs(tri - ref)) <= 1e-12 * np.abs(Lc).max()
+        b = packed(itype, "V", "A", uplo, M.pack(A, uplo), M.pack(B, uplo), n)
+        assert b["info"] == 0 and b["m"] == n
+        assert np.array_equal(a["W"], b["W"]) and np.array_equal(a["X"], b["X"])
+        assert np.array_equal(b["L"], M.pack(a["L"], uplo))
+        c = packed(itype, "N", "I", uplo, M.pack(A, uplo), M.pack(B, uplo), n, il=10, iu=50)
+        assert c["info"] == 0 and c["m"] == 41
+        check_general(itype, A, B, c, Wr[9:50], f"packed I cplx={cplx}")

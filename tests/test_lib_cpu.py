@@ -56,8 +56,8 @@ def test_workload_generators():
 
 
 def test_dense_drivers_reject_invalid_arguments():
-    """dsyeig / dsygeig return 1 for the argument errors of dense.c:174-183 and
-    :254-258 before touching the device (so this runs without a GPU)."""
+    """dsyeig / dsygeig return 1 for the argument errors of dense.c:174-180 and
+    :256-260 before touching the device (so this runs without a GPU)."""
     import mr3smp_amd
     L = mr3smp_amd.lib()
     P = ctypes.byref
@@ -99,3 +99,46 @@ def test_dense_drivers_reject_invalid_arguments():
     assert sygeig(1, b"Q", b"A", b"L") == 1
     assert sygeig(2, b"V", b"Q", b"L") == 1
     assert sygeig(3, b"V", b"A", b"Q") == 1
+
+
+def test_packed_and_complex_drivers_reject_invalid_arguments():
+    """The packed and complex dense drivers share the argument checks (dense.c:341-347,
+    :422-426, :596-602, :697-701, :788-794, :888-892) and return 1 without a device."""
+    import mr3smp_amd
+    L = mr3smp_amd.lib()
+    P = ctypes.byref
+    buf = np.zeros(64, dtype=np.complex128)
+    W = np.zeros(4)
+    b = buf.ctypes.data_as(ctypes.c_void_p)
+    w = W.ctypes.data_as(ctypes.c_void_p)
+
+    def scal():
+        return [ctypes.c_double(0.0), ctypes.c_double(1.0), ctypes.c_int(1), ctypes.c_int(4),
+                ctypes.c_int(0), ctypes.c_int(4)]
+
+    for name in ("dspeig", "zhpeig"):
+        for jobz, rng, uplo, n in [(b"X", b"A", b"L", 4), (b"V", b"X", b"L", 4), (b"V", b"A", b"X", 4),
+                                   (b"V", b"A", b"L", 0)]:
+            vl, vu, il, iu, m, ldz = scal()
+            nn = ctypes.c_int(n)
+            assert getattr(L, name)(jobz, rng, uplo, P(nn), b, P(vl), P(vu), P(il), P(iu), P(m), w, b,
+                                    P(ldz)) == 1, (name, jobz, rng, uplo, n)
+    for name in ("zheeig",):
+        vl, vu, il, iu, m, ldz = scal()
+        nn, lda = ctypes.c_int(4), ctypes.c_int(4)
+        il.value = 3
+        iu.value = 2
+        assert L.zheeig(b"V", b"I", b"U", P(nn), b, P(lda), P(vl), P(vu), P(il), P(iu), P(m), w, b,
+                        P(ldz)) == 1
+    for name in ("zhegeig",):
+        for itype, n in [(0, 4), (4, 4), (1, 1)]:
+            vl, vu, il, iu, m, ldz = scal()
+            it, nn, ld = ctypes.c_int(itype), ctypes.c_int(n), ctypes.c_int(4)
+            assert L.zhegeig(P(it), b"V", b"A", b"L", P(nn), b, P(ld), b, P(ld), P(vl), P(vu), P(il),
+                             P(iu), P(m), w) == 1
+    for name, small in (("dspgeig", 0), ("zhpgeig", 1)):
+        for itype, n in [(0, 4), (4, 4), (1, small)]:
+            vl, vu, il, iu, m, ldz = scal()
+            it, nn = ctypes.c_int(itype), ctypes.c_int(n)
+            assert getattr(L, name)(P(it), b"V", b"A", b"L", P(nn), b, b, P(vl), P(vu), P(il), P(iu),
+                                    P(m), w, b, P(ldz)) == 1, (name, itype, n)
