@@ -1492,14 +1492,16 @@ __device__ __forceinline__ void rrf_trial_w(int n, const double* d, const double
     if (wave_uniform(d) && wave_uniform(l) && wave_uniform(ld) && wave_uniform(n))
         rrf_trial(rfl(n), as_const(d), as_const(l), as_const(ld), sig, pivmin, mx, nan);
     else
-        rrf_trial(n, d, l, ld, sig, pivmin, mx, nan);
+        rrf_trial(n, as_global(d), as_global(l), as_global(ld), sig, pivmin, mx, nan);
 }
 
 // The trial factorisation with its factors stored (for the refined growth test); the
 // loads run kU steps ahead, as in rrf_trial.
-__device__ __forceinline__ void rrf_trial_store(int n, const double* d, const double* l,
-                                                const double* ld, double sig, double pivmin,
-                                                double* Dst, double* Lst) {
+__device__ __forceinline__ void rrf_trial_store(int n, const double* d0, const double* l0,
+                                                const double* ld0, double sig, double pivmin,
+                                                double* Dst0, double* Lst0) {
+    const GPtr d = as_global(d0), l = as_global(l0), ld = as_global(ld0);
+    const GPtrW Dst = as_global_w(Dst0), Lst = as_global_w(Lst0);
     double s = -sig;
     double dp = d[0] + s;
     if (fabs(dp) < pivmin) dp = -pivmin;
@@ -1532,7 +1534,8 @@ __device__ __forceinline__ void rrf_trial_store(int n, const double* d, const do
 
 // The refined growth measure of odrrf.f:265-305 over a D+ array A and an L+ array B
 // (the reference pairs the D+ of one trial with the L+ of the other); loads kU ahead.
-__device__ __forceinline__ double rrf_rrr(int n, const double* A, const double* B, double spdiam) {
+__device__ __forceinline__ double rrf_rrr(int n, const double* A0, const double* B0, double spdiam) {
+    const GPtr A = as_global(A0), B = as_global(B0);
     double tmp = fabs(A[n - 1]), znm2 = 1.0, prod = 1.0, oldp = 1.0;
     double a[kU], b[kU], a1[kU], b1[kU], an[kU], bn[kU];
     // step i reads A[i], B[i] and A[i+1], B[i+1]
@@ -1775,8 +1778,8 @@ __device__ __forceinline__ void rrf_store(int n, int nmax, P d, P l, P ld, doubl
             const int row = i0 + sub;
             const int nc = nn[col];
             if (row < nc) {
-                dD[col][row] = bD[sub][col];
-                if (row < nc - 1) dL[col][row] = bL[sub][col];
+                as_global_w(dD[col])[row] = bD[sub][col];
+                if (row < nc - 1) as_global_w(dL[col])[row] = bL[sub][col];
             }
         }
         __syncthreads();
@@ -1873,7 +1876,8 @@ __global__ __launch_bounds__(64) void k_dlarrf(ClTask* C, int nC, double* Wsh, d
             rrf_store(rfl(n), nmax, as_const(pd), as_const(pl), as_const(pld), tv, pivmin, valid,
                       lane, bD, bL, dD, dL, nn);
         else
-            rrf_store(n, nmax, pd, pl, pld, tv, pivmin, valid, lane, bD, bL, dD, dL, nn);
+            rrf_store(n, nmax, as_global(pd), as_global(pl), as_global(pld), tv, pivmin, valid, lane,
+                      bD, bL, dD, dL, nn);
     }
 }
 
@@ -1905,7 +1909,8 @@ __global__ __launch_bounds__(64) void k_rrf_store(const ClTask* __restrict__ C, 
         rrf_store(rfl(n), nmax, as_const(pd), as_const(pl), as_const(pld), tau, pivmin, valid, lane,
                   bD, bL, dD, dL, nn);
     else
-        rrf_store(n, nmax, pd, pl, pld, tau, pivmin, valid, lane, bD, bL, dD, dL, nn);
+        rrf_store(n, nmax, as_global(pd), as_global(pl), as_global(pld), tau, pivmin, valid, lane, bD,
+                  bL, dD, dL, nn);
 }
 
 // Accumulates the shift, forms DL and DLL from the chosen factorisation and moves
@@ -2009,11 +2014,11 @@ __device__ __forceinline__ int wave_min_u(int v) {
 // of a chunk are recomputed from them into the LDS buffer with the same operations,
 // so every value equals the one a full-length work array would hold.
 struct Ck {
-    double* base;
+    GPtrW base;
     int nck;
     int lane;
-    __device__ __forceinline__ double& S(int c) const { return base[(size_t)c * SW + lane]; }
-    __device__ __forceinline__ double& P(int c) const {
+    __device__ __forceinline__ GDouble& S(int c) const { return base[(size_t)c * SW + lane]; }
+    __device__ __forceinline__ GDouble& P(int c) const {
         return base[(size_t)(nck + c) * SW + lane];
     }
 };
@@ -2038,15 +2043,16 @@ struct R1v {
 };
 
 struct Rrr {
-    const double* D;
-    const double* L;
-    const double* LD;
-    const double* LLD;
+    GPtr D;
+    GPtr L;
+    GPtr LD;
+    GPtr LLD;
 };
 
 // One step t (1-based) of the stationary transform, odr1v.f:187-191; with safe set
 // the form of odr1v.f:206-213 (pivot floor, S = LLD where L+ underflows).
-__device__ __forceinline__ void stat_step(double d, double ld, double l, const double* LLD, int t,
+template <class P>
+__device__ __forceinline__ void stat_step(double d, double ld, double l, P LLD, int t,
                                           double lambda, bool safe, double pivmin, double& s,
                                           double& lp, double& si, bool& neg) {
     double dplus = d + s;
@@ -2123,7 +2129,7 @@ __device__ void pass_stat(const Rrr& R, double lambda, double pivmin, int r1, in
 // v[a] = P[a][t - 1].  Two register buffers take turns: the loads of one group of kUS
 // steps are in flight while the other group is computed.
 template <int NA, typename F>
-__device__ __forceinline__ void sweep_up(int lo, int hi, const double* const (&P)[NA], F f) {
+__device__ __forceinline__ void sweep_up(int lo, int hi, const GPtr (&P)[NA], F f) {
     if (lo > hi) return;
     double A[kUS][NA], B[kUS][NA];
     auto load = [&](double(&X)[kUS][NA], int t0) {
@@ -2149,7 +2155,7 @@ __device__ __forceinline__ void sweep_up(int lo, int hi, const double* const (&P
 
 // Descending twin of sweep_up: t = hi down to lo.
 template <int NA, typename F>
-__device__ __forceinline__ void sweep_down(int hi, int lo, const double* const (&P)[NA], F f) {
+__device__ __forceinline__ void sweep_down(int hi, int lo, const GPtr (&P)[NA], F f) {
     if (lo > hi) return;
     double A[kUS][NA], B[kUS][NA];
     auto load = [&](double(&X)[kUS][NA], int t0) {
@@ -2199,8 +2205,8 @@ __device__ void pass_prog(int n, const Rrr& R, double lambda, double pivmin, int
     const bool needS = act && freetw;
     const bool anyfree = __any(needS);
     if (n - 1 >= loU) {
-        const double* const PS[3] = {R.D, R.LD, R.L};
-        const double* const PP[3] = {R.D, R.LLD, R.L};
+        const GPtr PS[3] = {R.D, R.LD, R.L};
+        const GPtr PP[3] = {R.D, R.LLD, R.L};
         const int cTop = (n - 2) / KC, cBot = (loU - 1) / KC;
         double sck = (needS && cTop >= 1) ? ck.S(cTop) : 0.0;
         for (int c = cTop; c >= cBot; --c) {
@@ -2318,7 +2324,8 @@ __device__ __forceinline__ void write_chunk(const ChunkBuf& buf, const ColOut& c
         const int c = cg * (SW / KC) + cq;
         if (c < co.cntw) {
             const int row = co.wr0[c] + sub;
-            if (row >= co.wlo[c] && row <= co.whi[c]) co.zcol[c][co.bbeg + row - 1] = buf[sub][c];
+            if (row >= co.wlo[c] && row <= co.whi[c])
+                as_global_w(co.zcol[c])[co.bbeg + row - 1] = buf[sub][c];
         }
     }
     __syncthreads();
@@ -2528,7 +2535,7 @@ __device__ __forceinline__ void pass_solve(int n, const Rrr& R, double lambda, d
             c = cn;
         }
     }
-    if (act && stg) co.Zc[co.bbeg + r - 1] = scale;   // Z(r) = 1
+    if (act && stg) as_global_w(co.Zc)[co.bbeg + r - 1] = scale;   // Z(r) = 1
     (void)upper_lo;
     (void)lower_hi;
     if (act) {
@@ -2603,8 +2610,8 @@ __device__ __forceinline__ void s_update(SState& q, const R1v& o, bool s2, doubl
     }
     if (done) {
         state = kSDone;
-        dsc.Zsupp[2 * col] = o.isupp0 + bbeg;
-        dsc.Zsupp[2 * col + 1] = o.isupp1 + bbeg;
+        as_global_w(dsc.Zsupp)[2 * col] = o.isupp0 + bbeg;
+        as_global_w(dsc.Zsupp)[2 * col + 1] = o.isupp1 + bbeg;
         Wsh[t.slot] = lambda;
         Werr[t.slot] = q.werr;
         W[t.slot] = lambda + t.L[n - 1];
@@ -2634,7 +2641,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
     Ck ck;
     ck.nck = cap / KC + 2;
     ck.lane = lane;
-    ck.base = scratch + (size_t)blockIdx.x * 2 * (size_t)ck.nck * SW;
+    ck.base = as_global_w(scratch + (size_t)blockIdx.x * 2 * (size_t)ck.nck * SW);
     for (;;) {
         // tasks are handed out in list order (longest blocks first) as waves free up;
         // the index goes through LDS between barriers so that every lane of the wave
@@ -2685,7 +2692,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
             continue;
         }
         Rrr R;
-        R.D = t.D; R.L = t.L; R.LD = t.DL; R.LLD = t.DLL;
+        R.D = as_global(t.D); R.L = as_global(t.L); R.LD = as_global(t.DL); R.LLD = as_global(t.DLL);
         const MatState& S = ms[t.mat];
         const int i = t.slot - dsc.off;   // matrix-local eigenvalue index
         const int bl_begin = bbeg, bl_end = bbeg + n - 1;
@@ -2887,7 +2894,8 @@ __global__ __launch_bounds__(64) void k_endref_grp(const ClTask* __restrict__ C,
     const double* lld = t.pDLL;
     const bool uni = wave_uniform(d) && wave_uniform(lld) && wave_uniform(n);
     auto count = [&](double x) -> int {
-        return uni ? dlaneg(rfl(n), as_const(d), as_const(lld), x, rfl(n)) : dlaneg(n, d, lld, x, n);
+        return uni ? dlaneg(rfl(n), as_const(d), as_const(lld), x, rfl(n))
+                   : dlaneg(n, as_global(d), as_global(lld), x, n);
     };
     int rounds = 0;
     const long long sw = dlarrb_grp(G, sub, gbase, valid, count, Windex[slot], S.rqtol, S.rqtol,
@@ -2924,7 +2932,8 @@ __global__ __launch_bounds__(64) void k_refine_cl_grp(const ClTask* __restrict__
     const double* lld = t.cDLL;
     const bool uni = wave_uniform(d) && wave_uniform(lld) && wave_uniform(n);
     auto count = [&](double x) -> int {
-        return uni ? dlaneg(rfl(n), as_const(d), as_const(lld), x, rfl(n)) : dlaneg(n, d, lld, x, n);
+        return uni ? dlaneg(rfl(n), as_const(d), as_const(lld), x, rfl(n))
+                   : dlaneg(n, as_global(d), as_global(lld), x, n);
     };
     int rounds = 0;
     const long long sw = dlarrb_grp(G, sub, gbase, valid, count, Windex[slot], S.rtol1, S.rtol2,
@@ -2994,7 +3003,7 @@ __device__ __forceinline__ long long s_bisect_state(SState& q, const STask& t,
         const bool uni = wave_uniform(d) && wave_uniform(lld) && wave_uniform(n) && wave_uniform(r);
         auto count = [&](double x) -> int {
             return uni ? dlaneg(rfl(n), as_const(d), as_const(lld), x, rfl(r))
-                       : dlaneg(n, d, lld, x, r);
+                       : dlaneg(n, as_global(d), as_global(lld), x, r);
         };
         sweeps = dlarrb_grp(G, sub, gbase, valid, count, i, 0.0, 2.0 * DBL_EPSILON, 0.0, pivmin,
                             spdiam, w, we, rounds);

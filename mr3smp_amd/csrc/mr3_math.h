@@ -39,6 +39,18 @@ __device__ __forceinline__ const double* rfl(const double* p) {
     const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(u >> 32));
     return (const double*)(((unsigned long long)hi << 32) | lo);
 }
+// Arrays that the lanes of a wave read at different addresses go through the global
+// address space.  A load through a generic pointer (flat_load) also counts against the
+// LDS / scalar-memory counter, so the compiler's wait for any LDS or scalar result
+// would also wait for the loads a sweep has prefetched for its next chunk.
+typedef const __attribute__((address_space(1))) double* GPtr;
+typedef __attribute__((address_space(1))) double* GPtrW;
+typedef __attribute__((address_space(1))) double GDouble;
+typedef __attribute__((address_space(1))) int* GIntW;
+__device__ __forceinline__ GPtr as_global(const double* p) { return (GPtr)p; }
+__device__ __forceinline__ GPtrW as_global_w(double* p) { return (GPtrW)p; }
+__device__ __forceinline__ GIntW as_global_w(int* p) { return (GIntW)p; }
+
 // True when every active lane holds the same value (wave-uniform result).
 __device__ __forceinline__ bool wave_uniform(int v) { return __all(v == rfl(v)); }
 __device__ __forceinline__ bool wave_uniform(const double* p) { return __all(p == rfl(p)); }
@@ -336,7 +348,8 @@ __device__ __forceinline__ int dlarrb1_w(int n, const double* d, const double* l
     if (wave_uniform(d) && wave_uniform(lld) && wave_uniform(n) && wave_uniform(twist))
         return dlarrb1(rfl(n), as_const(d), as_const(lld), i, rtol1, rtol2, gap, w, werr, pivmin,
                        spdiam, rfl(twist));
-    return dlarrb1(n, d, lld, i, rtol1, rtol2, gap, w, werr, pivmin, spdiam, twist);
+    return dlarrb1(n, as_global(d), as_global(lld), i, rtol1, rtol2, gap, w, werr, pivmin, spdiam,
+                   twist);
 }
 
 __device__ __forceinline__ int dlarrj1_w(int n, const double* d, const double* e2, int i,
@@ -344,7 +357,7 @@ __device__ __forceinline__ int dlarrj1_w(int n, const double* d, const double* e
                                          double spdiam) {
     if (wave_uniform(d) && wave_uniform(e2) && wave_uniform(n))
         return dlarrj1(rfl(n), as_const(d), as_const(e2), i, rtol, w, werr, pivmin, spdiam);
-    return dlarrj1(n, d, e2, i, rtol, w, werr, pivmin, spdiam);
+    return dlarrj1(n, as_global(d), as_global(e2), i, rtol, w, werr, pivmin, spdiam);
 }
 
 }  // namespace mr3
