@@ -320,7 +320,15 @@ int run_pipeline(std::vector<MatDesc>& md, std::vector<MatState>& ms, hipStream_
     long long ntot = 0;
     int nmax = 0;
     bool any_v = false, any_n = false;
-    for (auto& d : md) {
+    // Matrices take their slots of the eigenvalue arrays in order of decreasing size, so
+    // that the lanes of a wave, which take consecutive slots, mostly belong to matrices of
+    // about the same size and sweep about the same number of rows.  Results do not
+    // depend on the order (each matrix is solved on its own slots).
+    std::vector<int> order(nmat);
+    for (int k = 0; k < nmat; ++k) order[k] = k;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return md[a].n > md[b].n; });
+    for (int k : order) {
+        MatDesc& d = md[k];
         d.off = (int)ntot;
         ntot += d.n;
         nmax = std::max(nmax, d.n);
@@ -368,7 +376,7 @@ int run_pipeline(std::vector<MatDesc>& md, std::vector<MatState>& ms, hipStream_
     {
         // blocks of the split, for the shared top of the coarse bisection
         std::vector<int> plist;
-        for (int k = 0; k < nmat; ++k) {
+        for (int k : order) {
             if (ms[k].done || ms[k].maxbl < 128) continue;
             for (int j = 0; j < ms[k].nsplit; ++j) plist.push_back(md[k].off + j);
         }
@@ -392,7 +400,7 @@ int run_pipeline(std::vector<MatDesc>& md, std::vector<MatState>& ms, hipStream_
     // block list for the root kernel: (matrix offset + block index) of every block
     std::vector<int> blist;
     int maxbl = 1;
-    for (int k = 0; k < nmat; ++k) {
+    for (int k : order) {
         if (ms[k].done || ms[k].status != kOk) continue;
         for (int j = 0; j < ms[k].nsplit; ++j) blist.push_back(md[k].off + j);
         maxbl = std::max(maxbl, ms[k].maxbl);

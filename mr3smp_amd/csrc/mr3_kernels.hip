@@ -2355,6 +2355,9 @@ __device__ __forceinline__ void pass_solve(int n, const Rrr& R, double lambda, d
     // upwards: Z(j) for j = r-1 .. 1; chunk c holds steps c*KC .. c*KC+KC-1
     {
         double zi1 = 1.0, zi2 = 0.0;   // Z(j+1), Z(j+2)
+        // LD(j+1), the LD of the row solved before row j; the first row (j = r-1) has
+        // Z(r) = 1 and never reads it
+        double ld_next = 0.0;
         bool live = act && r - 1 >= 1;
         int c = live ? (r - 1) / KC : 0;
         double pre[kUS][3];
@@ -2421,8 +2424,9 @@ __device__ __forceinline__ void pass_solve(int n, const Rrr& R, double lambda, d
                     double zv = 0.0;
                     if (live) {
                         double zi;
-                        if (nan && zi1 == 0.0) zi = -(R.LD[j] / ldj) * zi2;
+                        if (nan && zi1 == 0.0) zi = -(ld_next / ldj) * zi2;
                         else zi = -(buf[k][lane] * zi1);
+                        ld_next = ldj;
                         if ((fabs(zi) + fabs(zi1)) * fabs(ldj) < gaptol) {
                             is0 = j + 1;
                             live = false;
@@ -2445,6 +2449,9 @@ __device__ __forceinline__ void pass_solve(int n, const Rrr& R, double lambda, d
     // downwards: Z(t+1) for t = r .. n-1; chunk c holds steps c*KC+1 .. c*KC+KC
     {
         double zi = 1.0, zim1 = 0.0;   // Z(t), Z(t-1)
+        // LD(t-1), the LD of the row solved before row t; the first row (t = r) has
+        // Z(r) = 1 and never reads it
+        double ld_prev = 0.0;
         bool live = act && r <= n - 1;
         const int cEnd = (n - 2) / KC;
         int c = live ? (r - 1) / KC : 0;
@@ -2513,8 +2520,9 @@ __device__ __forceinline__ void pass_solve(int n, const Rrr& R, double lambda, d
                     double zv = 0.0;
                     if (live) {
                         double zn;
-                        if (nan && zi == 0.0) zn = -(R.LD[max(t - 2, 0)] / ldt) * zim1;
+                        if (nan && zi == 0.0) zn = -(ld_prev / ldt) * zim1;
                         else zn = -(buf[k][lane] * zi);
+                        ld_prev = ldt;
                         if ((fabs(zi) + fabs(zn)) * fabs(ldt) < gaptol) {
                             is1 = t;
                             live = false;
