@@ -17,6 +17,8 @@
 //   mrrr.c:263-288 shift back, dlarrj, unscale, sort   -> k_shift_n, k_rrj, k_finalize
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "mr3_internal.h"
 #include "mr3_math.h"
 
@@ -2352,197 +2354,379 @@ __device__ __forceinline__ void pass_solve(int n, const Rrr& R, double lambda, d
     double ztz = 1.0;
     int upper_lo = n + 1, lower_hi = 0;
     const bool stage = __any(act && stg);   // some lane writes its column
-    // upwards: Z(j) for j = r-1 .. 1; chunk c holds steps c*KC .. c*KC+KC-1
-    {
-        double zi1 = 1.0, zi2 = 0.0;   // Z(j+1), Z(j+2)
-        // LD(j+1), the LD of the row solved before row j; the first row (j = r-1) has
-        // Z(r) = 1 and never reads it
-        double ld_next = 0.0;
-        bool live = act && r - 1 >= 1;
-        int c = live ? (r - 1) / KC : 0;
-        double pre[kUS][3];
-        double pck = 0.0;
-#pragma unroll
-        for (int u = 0; u < kUS; ++u) {
-            const int j = c * KC + u;
-            pre[u][0] = R.D[j - 1]; pre[u][1] = R.LD[j - 1]; pre[u][2] = R.L[j - 1];
-        }
-        if (live && c >= 1) pck = ck.S(c);
-        while (__any(live)) {
-            const bool mine = live;
-            const int base = c * KC;
-            const int jlo = max(base, 1), jhi = min(base + KC - 1, r - 1);
-            double s = (c == 0) ? 0.0 - lambda : pck;
-            double cur[kUS][3], nxt[kUS][3];
-#pragma unroll
-            for (int u = 0; u < kUS; ++u)
-#pragma unroll
-                for (int a = 0; a < 3; ++a) cur[u][a] = pre[u][a];
-            // next chunk of this lane (c - 1): first loads and carry
-            const int cn = max(c - 1, 0);
-#pragma unroll
-            for (int u = 0; u < kUS; ++u) {
-                const int j = cn * KC + u;
-                pre[u][0] = R.D[j - 1]; pre[u][1] = R.LD[j - 1]; pre[u][2] = R.L[j - 1];
-            }
-            pck = (mine && c - 1 >= 1) ? ck.S(c - 1) : 0.0;
-            // recompute L+ (and keep LD) for the chunk
-#pragma unroll
-            for (int g = 0; g < KC / kUS; ++g) {
-                if (g + 1 < KC / kUS) {
-#pragma unroll
-                    for (int u = 0; u < kUS; ++u) {
-                        const int j = base + (g + 1) * kUS + u;
-                        nxt[u][0] = R.D[j - 1]; nxt[u][1] = R.LD[j - 1]; nxt[u][2] = R.L[j - 1];
-                    }
-                }
-#pragma unroll
+    // Waves in which no lane saw a NaN in its transforms (almost all) run the solve
+    // without the slow-form branches, each row's updates as selects instead of
+    // divergent branches; the others run the reference's slow forms.
+    auto solve = [&](auto slow_t) {
+        constexpr bool kSlow = decltype(slow_t)::value;
+        if constexpr (kSlow) {
+            // upwards: Z(j) for j = r-1 .. 1; chunk c holds steps c*KC .. c*KC+KC-1
+            {
+                double zi1 = 1.0, zi2 = 0.0;   // Z(j+1), Z(j+2)
+                // LD(j+1), the LD of the row solved before row j; the first row (j = r-1) has
+                // Z(r) = 1 and never reads it
+                double ld_next = 0.0;
+                bool live = act && r - 1 >= 1;
+                int c = live ? (r - 1) / KC : 0;
+                double pre[kUS][3];
+                double pck = 0.0;
+        #pragma unroll
                 for (int u = 0; u < kUS; ++u) {
-                    const int k = g * kUS + u, j = base + k;
-                    if (mine && j >= jlo && j <= jhi) {
-                        double lp, si;
-                        bool ng;
-                        stat_step(cur[u][0], cur[u][1], cur[u][2], R.LLD, j, lambda, safe1, pivmin,
-                                  s, lp, si, ng);
-                        buf[k][lane] = lp;
-                        bld[k][lane] = cur[u][1];
-                    }
+                    const int j = c * KC + u;
+                    pre[u][0] = R.D[j - 1]; pre[u][1] = R.LD[j - 1]; pre[u][2] = R.L[j - 1];
                 }
-                if (g + 1 < KC / kUS) {
-#pragma unroll
+                if (live && c >= 1) pck = ck.S(c);
+                while (__any(live)) {
+                    const bool mine = live;
+                    const int base = c * KC;
+                    const int jlo = max(base, 1), jhi = min(base + KC - 1, r - 1);
+                    double s = (c == 0) ? 0.0 - lambda : pck;
+                    double cur[kUS][3], nxt[kUS][3];
+        #pragma unroll
                     for (int u = 0; u < kUS; ++u)
-#pragma unroll
-                        for (int a = 0; a < 3; ++a) cur[u][a] = nxt[u][a];
-                }
-            }
-            // consume downwards
-#pragma unroll
-            for (int k = KC - 1; k >= 0; --k) {
-                const int j = base + k;
-                if (mine && j >= jlo && j <= jhi) {
-                    const double ldj = bld[k][lane];
-                    double zv = 0.0;
-                    if (live) {
-                        double zi;
-                        if (nan && zi1 == 0.0) zi = -(ld_next / ldj) * zi2;
-                        else zi = -(buf[k][lane] * zi1);
-                        ld_next = ldj;
-                        if ((fabs(zi) + fabs(zi1)) * fabs(ldj) < gaptol) {
-                            is0 = j + 1;
-                            live = false;
-                        } else {
-                            ztz = ztz + zi * zi;
-                            zv = zi;
-                            zi2 = zi1;
-                            zi1 = zi;
+        #pragma unroll
+                        for (int a = 0; a < 3; ++a) cur[u][a] = pre[u][a];
+                    // next chunk of this lane (c - 1): first loads and carry
+                    const int cn = max(c - 1, 0);
+        #pragma unroll
+                    for (int u = 0; u < kUS; ++u) {
+                        const int j = cn * KC + u;
+                        pre[u][0] = R.D[j - 1]; pre[u][1] = R.LD[j - 1]; pre[u][2] = R.L[j - 1];
+                    }
+                    pck = (mine && c - 1 >= 1) ? ck.S(c - 1) : 0.0;
+                    // recompute L+ (and keep LD) for the chunk
+        #pragma unroll
+                    for (int g = 0; g < KC / kUS; ++g) {
+                        if (g + 1 < KC / kUS) {
+        #pragma unroll
+                            for (int u = 0; u < kUS; ++u) {
+                                const int j = base + (g + 1) * kUS + u;
+                                nxt[u][0] = R.D[j - 1]; nxt[u][1] = R.LD[j - 1]; nxt[u][2] = R.L[j - 1];
+                            }
+                        }
+        #pragma unroll
+                        for (int u = 0; u < kUS; ++u) {
+                            const int k = g * kUS + u, j = base + k;
+                            if (mine && j >= jlo && j <= jhi) {
+                                double lp, si;
+                                bool ng;
+                                stat_step(cur[u][0], cur[u][1], cur[u][2], R.LLD, j, lambda, safe1, pivmin,
+                                          s, lp, si, ng);
+                                buf[k][lane] = lp;
+                                bld[k][lane] = cur[u][1];
+                            }
+                        }
+                        if (g + 1 < KC / kUS) {
+        #pragma unroll
+                            for (int u = 0; u < kUS; ++u)
+        #pragma unroll
+                                for (int a = 0; a < 3; ++a) cur[u][a] = nxt[u][a];
                         }
                     }
-                    if (stg) buf[k][lane] = zv * scale;
-                }
-            }
-            if (mine) upper_lo = jlo;
-            if (stage) write_chunk(buf, co, lane, base, mine && stg ? jlo : 1, mine && stg ? jhi : 0);
-            if (c == 0) live = false;
-            c = cn;
-        }
-    }
-    // downwards: Z(t+1) for t = r .. n-1; chunk c holds steps c*KC+1 .. c*KC+KC
-    {
-        double zi = 1.0, zim1 = 0.0;   // Z(t), Z(t-1)
-        // LD(t-1), the LD of the row solved before row t; the first row (t = r) has
-        // Z(r) = 1 and never reads it
-        double ld_prev = 0.0;
-        bool live = act && r <= n - 1;
-        const int cEnd = (n - 2) / KC;
-        int c = live ? (r - 1) / KC : 0;
-        double pre[kUS][4];
-        double pcp = 0.0;
-#pragma unroll
-        for (int u = 0; u < kUS; ++u) {
-            const int t = c * KC + KC - u;
-            pre[u][0] = R.D[t - 1]; pre[u][1] = R.LLD[t - 1]; pre[u][2] = R.L[t - 1];
-            pre[u][3] = R.LD[t - 1];
-        }
-        if (live) pcp = ck.P(c);
-        while (__any(live)) {
-            const bool mine = live;
-            const int tlo = c * KC + 1, thi = min(c * KC + KC, n - 1);
-            const int tl2 = max(tlo, r);
-            double p = pcp;
-            double cur[kUS][4], nxt[kUS][4];
-#pragma unroll
-            for (int u = 0; u < kUS; ++u)
-#pragma unroll
-                for (int a = 0; a < 4; ++a) cur[u][a] = pre[u][a];
-            const int cn = min(c + 1, cEnd);
-#pragma unroll
-            for (int u = 0; u < kUS; ++u) {
-                const int t = cn * KC + KC - u;
-                pre[u][0] = R.D[t - 1]; pre[u][1] = R.LLD[t - 1]; pre[u][2] = R.L[t - 1];
-                pre[u][3] = R.LD[t - 1];
-            }
-            pcp = (mine && c + 1 <= cEnd) ? ck.P(c + 1) : 0.0;
-            // recompute U- (and keep LD) for the chunk, from its top down
-#pragma unroll
-            for (int g = 0; g < KC / kUS; ++g) {
-                if (g + 1 < KC / kUS) {
-#pragma unroll
-                    for (int u = 0; u < kUS; ++u) {
-                        const int t = tlo + KC - 1 - (g + 1) * kUS - u;
-                        nxt[u][0] = R.D[t - 1]; nxt[u][1] = R.LLD[t - 1]; nxt[u][2] = R.L[t - 1];
-                        nxt[u][3] = R.LD[t - 1];
-                    }
-                }
-#pragma unroll
-                for (int u = 0; u < kUS; ++u) {
-                    const int k = KC - 1 - (g * kUS + u), t = tlo + k;
-                    if (mine && t >= tl2 && t <= thi) {
-                        double um;
-                        bool ng;
-                        prog_step(cur[u][0], cur[u][1], cur[u][2], lambda, safe2, pivmin, p, um, ng);
-                        buf[k][lane] = um;
-                        bld[k][lane] = cur[u][3];
-                    }
-                }
-                if (g + 1 < KC / kUS) {
-#pragma unroll
-                    for (int u = 0; u < kUS; ++u)
-#pragma unroll
-                        for (int a = 0; a < 4; ++a) cur[u][a] = nxt[u][a];
-                }
-            }
-            // consume upwards in t
-#pragma unroll
-            for (int k = 0; k < KC; ++k) {
-                const int t = tlo + k;
-                if (mine && t >= tl2 && t <= thi) {
-                    const double ldt = bld[k][lane];
-                    double zv = 0.0;
-                    if (live) {
-                        double zn;
-                        if (nan && zi == 0.0) zn = -(ld_prev / ldt) * zim1;
-                        else zn = -(buf[k][lane] * zi);
-                        ld_prev = ldt;
-                        if ((fabs(zi) + fabs(zn)) * fabs(ldt) < gaptol) {
-                            is1 = t;
-                            live = false;
-                        } else {
-                            ztz = ztz + zn * zn;
-                            zv = zn;
-                            zim1 = zi;
-                            zi = zn;
+                    // consume downwards
+        #pragma unroll
+                    for (int k = KC - 1; k >= 0; --k) {
+                        const int j = base + k;
+                        if (mine && j >= jlo && j <= jhi) {
+                            const double ldj = bld[k][lane];
+                            double zv = 0.0;
+                            if (live) {
+                                double zi;
+                                if (nan && zi1 == 0.0) zi = -(ld_next / ldj) * zi2;
+                                else zi = -(buf[k][lane] * zi1);
+                                ld_next = ldj;
+                                if ((fabs(zi) + fabs(zi1)) * fabs(ldj) < gaptol) {
+                                    is0 = j + 1;
+                                    live = false;
+                                } else {
+                                    ztz = ztz + zi * zi;
+                                    zv = zi;
+                                    zi2 = zi1;
+                                    zi1 = zi;
+                                }
+                            }
+                            if (stg) buf[k][lane] = zv * scale;
                         }
                     }
-                    if (stg) buf[k][lane] = zv * scale;
+                    if (mine) upper_lo = jlo;
+                    if (stage) write_chunk(buf, co, lane, base, mine && stg ? jlo : 1, mine && stg ? jhi : 0);
+                    if (c == 0) live = false;
+                    c = cn;
                 }
             }
-            if (mine) lower_hi = thi + 1;
-            if (stage)
-                write_chunk(buf, co, lane, tlo + 1, mine && stg ? tl2 + 1 : 1, mine && stg ? thi + 1 : 0);
-            if (c == cEnd) live = false;
-            c = cn;
+
+            // downwards: Z(t+1) for t = r .. n-1; chunk c holds steps c*KC+1 .. c*KC+KC
+            {
+                double zi = 1.0, zim1 = 0.0;   // Z(t), Z(t-1)
+                // LD(t-1), the LD of the row solved before row t; the first row (t = r) has
+                // Z(r) = 1 and never reads it
+                double ld_prev = 0.0;
+                bool live = act && r <= n - 1;
+                const int cEnd = (n - 2) / KC;
+                int c = live ? (r - 1) / KC : 0;
+                double pre[kUS][4];
+                double pcp = 0.0;
+        #pragma unroll
+                for (int u = 0; u < kUS; ++u) {
+                    const int t = c * KC + KC - u;
+                    pre[u][0] = R.D[t - 1]; pre[u][1] = R.LLD[t - 1]; pre[u][2] = R.L[t - 1];
+                    pre[u][3] = R.LD[t - 1];
+                }
+                if (live) pcp = ck.P(c);
+                while (__any(live)) {
+                    const bool mine = live;
+                    const int tlo = c * KC + 1, thi = min(c * KC + KC, n - 1);
+                    const int tl2 = max(tlo, r);
+                    double p = pcp;
+                    double cur[kUS][4], nxt[kUS][4];
+        #pragma unroll
+                    for (int u = 0; u < kUS; ++u)
+        #pragma unroll
+                        for (int a = 0; a < 4; ++a) cur[u][a] = pre[u][a];
+                    const int cn = min(c + 1, cEnd);
+        #pragma unroll
+                    for (int u = 0; u < kUS; ++u) {
+                        const int t = cn * KC + KC - u;
+                        pre[u][0] = R.D[t - 1]; pre[u][1] = R.LLD[t - 1]; pre[u][2] = R.L[t - 1];
+                        pre[u][3] = R.LD[t - 1];
+                    }
+                    pcp = (mine && c + 1 <= cEnd) ? ck.P(c + 1) : 0.0;
+                    // recompute U- (and keep LD) for the chunk, from its top down
+        #pragma unroll
+                    for (int g = 0; g < KC / kUS; ++g) {
+                        if (g + 1 < KC / kUS) {
+        #pragma unroll
+                            for (int u = 0; u < kUS; ++u) {
+                                const int t = tlo + KC - 1 - (g + 1) * kUS - u;
+                                nxt[u][0] = R.D[t - 1]; nxt[u][1] = R.LLD[t - 1]; nxt[u][2] = R.L[t - 1];
+                                nxt[u][3] = R.LD[t - 1];
+                            }
+                        }
+        #pragma unroll
+                        for (int u = 0; u < kUS; ++u) {
+                            const int k = KC - 1 - (g * kUS + u), t = tlo + k;
+                            if (mine && t >= tl2 && t <= thi) {
+                                double um;
+                                bool ng;
+                                prog_step(cur[u][0], cur[u][1], cur[u][2], lambda, safe2, pivmin, p, um, ng);
+                                buf[k][lane] = um;
+                                bld[k][lane] = cur[u][3];
+                            }
+                        }
+                        if (g + 1 < KC / kUS) {
+        #pragma unroll
+                            for (int u = 0; u < kUS; ++u)
+        #pragma unroll
+                                for (int a = 0; a < 4; ++a) cur[u][a] = nxt[u][a];
+                        }
+                    }
+                    // consume upwards in t
+        #pragma unroll
+                    for (int k = 0; k < KC; ++k) {
+                        const int t = tlo + k;
+                        if (mine && t >= tl2 && t <= thi) {
+                            const double ldt = bld[k][lane];
+                            double zv = 0.0;
+                            if (live) {
+                                double zn;
+                                if (nan && zi == 0.0) zn = -(ld_prev / ldt) * zim1;
+                                else zn = -(buf[k][lane] * zi);
+                                ld_prev = ldt;
+                                if ((fabs(zi) + fabs(zn)) * fabs(ldt) < gaptol) {
+                                    is1 = t;
+                                    live = false;
+                                } else {
+                                    ztz = ztz + zn * zn;
+                                    zv = zn;
+                                    zim1 = zi;
+                                    zi = zn;
+                                }
+                            }
+                            if (stg) buf[k][lane] = zv * scale;
+                        }
+                    }
+                    if (mine) lower_hi = thi + 1;
+                    if (stage)
+                        write_chunk(buf, co, lane, tlo + 1, mine && stg ? tl2 + 1 : 1, mine && stg ? thi + 1 : 0);
+                    if (c == cEnd) live = false;
+                    c = cn;
+                }
+            }
+
+        } else {
+            // upwards: Z(j) for j = r-1 .. 1; chunk c holds steps c*KC .. c*KC+KC-1
+            {
+                double zi1 = 1.0;   // Z(j+1)
+                bool live = act && r - 1 >= 1;
+                int c = live ? (r - 1) / KC : 0;
+                double pre[kUS][3];
+                double pck = 0.0;
+        #pragma unroll
+                for (int u = 0; u < kUS; ++u) {
+                    const int j = c * KC + u;
+                    pre[u][0] = R.D[j - 1]; pre[u][1] = R.LD[j - 1]; pre[u][2] = R.L[j - 1];
+                }
+                if (live && c >= 1) pck = ck.S(c);
+                while (__any(live)) {
+                    const bool mine = live;
+                    const int base = c * KC;
+                    const int jlo = max(base, 1), jhi = min(base + KC - 1, r - 1);
+                    double s = (c == 0) ? 0.0 - lambda : pck;
+                    double cur[kUS][3], nxt[kUS][3];
+        #pragma unroll
+                    for (int u = 0; u < kUS; ++u)
+        #pragma unroll
+                        for (int a = 0; a < 3; ++a) cur[u][a] = pre[u][a];
+                    // next chunk of this lane (c - 1): first loads and carry
+                    const int cn = max(c - 1, 0);
+        #pragma unroll
+                    for (int u = 0; u < kUS; ++u) {
+                        const int j = cn * KC + u;
+                        pre[u][0] = R.D[j - 1]; pre[u][1] = R.LD[j - 1]; pre[u][2] = R.L[j - 1];
+                    }
+                    pck = (mine && c - 1 >= 1) ? ck.S(c - 1) : 0.0;
+                    // recompute L+ (and keep LD) for the chunk
+        #pragma unroll
+                    for (int g = 0; g < KC / kUS; ++g) {
+                        if (g + 1 < KC / kUS) {
+        #pragma unroll
+                            for (int u = 0; u < kUS; ++u) {
+                                const int j = base + (g + 1) * kUS + u;
+                                nxt[u][0] = R.D[j - 1]; nxt[u][1] = R.LD[j - 1]; nxt[u][2] = R.L[j - 1];
+                            }
+                        }
+        #pragma unroll
+                        for (int u = 0; u < kUS; ++u) {
+                            const int k = g * kUS + u, j = base + k;
+                            const bool in = mine && j >= jlo && j <= jhi;
+                            double sn = s, lp, si;
+                            bool ng;
+                            stat_step(cur[u][0], cur[u][1], cur[u][2], R.LLD, j, lambda, false, pivmin, sn,
+                                      lp, si, ng);
+                            s = in ? sn : s;
+                            buf[k][lane] = lp;
+                            bld[k][lane] = cur[u][1];
+                        }
+                        if (g + 1 < KC / kUS) {
+        #pragma unroll
+                            for (int u = 0; u < kUS; ++u)
+        #pragma unroll
+                                for (int a = 0; a < 3; ++a) cur[u][a] = nxt[u][a];
+                        }
+                    }
+                    // consume downwards
+        #pragma unroll
+                    for (int k = KC - 1; k >= 0; --k) {
+                        const int j = base + k;
+                        const bool in = mine && j >= jlo && j <= jhi;
+                        const double ldj = bld[k][lane];
+                        const double zi = -(buf[k][lane] * zi1);
+                        const bool go = in && live;
+                        const bool cut = go && (fabs(zi) + fabs(zi1)) * fabs(ldj) < gaptol;
+                        const bool upd = go && !cut;
+                        is0 = cut ? j + 1 : is0;
+                        live = cut ? false : live;
+                        ztz = upd ? ztz + zi * zi : ztz;
+                        zi1 = upd ? zi : zi1;
+                        if (stage) buf[k][lane] = (upd ? zi : 0.0) * scale;
+                    }
+                    if (mine) upper_lo = jlo;
+                    if (stage) write_chunk(buf, co, lane, base, mine && stg ? jlo : 1, mine && stg ? jhi : 0);
+                    if (c == 0) live = false;
+                    c = cn;
+                }
+            }
+
+            // downwards: Z(t+1) for t = r .. n-1; chunk c holds steps c*KC+1 .. c*KC+KC
+            {
+                double zi = 1.0;   // Z(t)
+                bool live = act && r <= n - 1;
+                const int cEnd = (n - 2) / KC;
+                int c = live ? (r - 1) / KC : 0;
+                double pre[kUS][4];
+                double pcp = 0.0;
+        #pragma unroll
+                for (int u = 0; u < kUS; ++u) {
+                    const int t = c * KC + KC - u;
+                    pre[u][0] = R.D[t - 1]; pre[u][1] = R.LLD[t - 1]; pre[u][2] = R.L[t - 1];
+                    pre[u][3] = R.LD[t - 1];
+                }
+                if (live) pcp = ck.P(c);
+                while (__any(live)) {
+                    const bool mine = live;
+                    const int tlo = c * KC + 1, thi = min(c * KC + KC, n - 1);
+                    const int tl2 = max(tlo, r);
+                    double p = pcp;
+                    double cur[kUS][4], nxt[kUS][4];
+        #pragma unroll
+                    for (int u = 0; u < kUS; ++u)
+        #pragma unroll
+                        for (int a = 0; a < 4; ++a) cur[u][a] = pre[u][a];
+                    const int cn = min(c + 1, cEnd);
+        #pragma unroll
+                    for (int u = 0; u < kUS; ++u) {
+                        const int t = cn * KC + KC - u;
+                        pre[u][0] = R.D[t - 1]; pre[u][1] = R.LLD[t - 1]; pre[u][2] = R.L[t - 1];
+                        pre[u][3] = R.LD[t - 1];
+                    }
+                    pcp = (mine && c + 1 <= cEnd) ? ck.P(c + 1) : 0.0;
+                    // recompute U- (and keep LD) for the chunk, from its top down
+        #pragma unroll
+                    for (int g = 0; g < KC / kUS; ++g) {
+                        if (g + 1 < KC / kUS) {
+        #pragma unroll
+                            for (int u = 0; u < kUS; ++u) {
+                                const int t = tlo + KC - 1 - (g + 1) * kUS - u;
+                                nxt[u][0] = R.D[t - 1]; nxt[u][1] = R.LLD[t - 1]; nxt[u][2] = R.L[t - 1];
+                                nxt[u][3] = R.LD[t - 1];
+                            }
+                        }
+        #pragma unroll
+                        for (int u = 0; u < kUS; ++u) {
+                            const int k = KC - 1 - (g * kUS + u), t = tlo + k;
+                            const bool in = mine && t >= tl2 && t <= thi;
+                            double pn = p, um;
+                            bool ng;
+                            prog_step(cur[u][0], cur[u][1], cur[u][2], lambda, false, pivmin, pn, um, ng);
+                            p = in ? pn : p;
+                            buf[k][lane] = um;
+                            bld[k][lane] = cur[u][3];
+                        }
+                        if (g + 1 < KC / kUS) {
+        #pragma unroll
+                            for (int u = 0; u < kUS; ++u)
+        #pragma unroll
+                                for (int a = 0; a < 4; ++a) cur[u][a] = nxt[u][a];
+                        }
+                    }
+                    // consume upwards in t
+        #pragma unroll
+                    for (int k = 0; k < KC; ++k) {
+                        const int t = tlo + k;
+                        const bool in = mine && t >= tl2 && t <= thi;
+                        const double ldt = bld[k][lane];
+                        const double zn = -(buf[k][lane] * zi);
+                        const bool go = in && live;
+                        const bool cut = go && (fabs(zi) + fabs(zn)) * fabs(ldt) < gaptol;
+                        const bool upd = go && !cut;
+                        is1 = cut ? t : is1;
+                        live = cut ? false : live;
+                        ztz = upd ? ztz + zn * zn : ztz;
+                        zi = upd ? zn : zi;
+                        if (stage) buf[k][lane] = (upd ? zn : 0.0) * scale;
+                    }
+                    if (mine) lower_hi = thi + 1;
+                    if (stage)
+                        write_chunk(buf, co, lane, tlo + 1, mine && stg ? tl2 + 1 : 1, mine && stg ? thi + 1 : 0);
+                    if (c == cEnd) live = false;
+                    c = cn;
+                }
+            }
+
         }
-    }
+    };
+    if (__any(nan)) solve(std::true_type{});
+    else solve(std::false_type{});
     if (act && stg) as_global_w(co.Zc)[co.bbeg + r - 1] = scale;   // Z(r) = 1
     (void)upper_lo;
     (void)lower_hi;
