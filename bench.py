@@ -34,10 +34,14 @@ KERNEL_STEPS = {"k_coarse": ["coarse"], "k_refine_root": ["refine"], "k_endref":
                 "k_refine_cl": ["refine_cl"]}
 FP64_PEAK_TFLOPS = 78.6
 HBM_PEAK_GBS = 8000.0
-# The divide-bound ceiling: an IEEE FP64 division step issues in about 110 cycles per
-# wave64 on one SIMD (measured with an in-register probe, the same at 1-8 waves per
-# SIMD), so 256 CUs x 4 SIMDs x 2.4 GHz x 64 lanes / 110 cycles divide steps per second.
-DIVIDE_STEPS_PER_S = 256 * 4 * 2.4e9 * 64 / 110.0
+# The divide-bound ceiling: the IEEE FP64 division sequence the compiler emits is ten
+# VALU instructions (v_div_scale x2, v_rcp_f64, four FMAs, v_mul, v_div_fmas,
+# v_div_fixup); at 32 FP64 flops per cycle per SIMD (78.6 TF / (256 CUs x 4 SIMDs x
+# 2.4 GHz)) a wave64 FP64 instruction takes 4 cycles and the quarter-rate v_rcp_f64 16,
+# so one division issues in 52 cycles per wave64 on one SIMD.  An upper bound for any
+# loop with one division per step (it leaves out the step's other instructions):
+# 256 CUs x 4 SIMDs x 2.4 GHz x 64 lanes / 52 cycles division steps per second.
+DIVIDE_STEPS_PER_S = 256 * 4 * 2.4e9 * 64 / 52.0
 # divisions the reference's algorithm needs per counted step of each family (dlar1v: the
 # stationary and the progressive transform; the vector solve multiplies)
 DIVIDES_PER_STEP = {"coarse": 1, "refine": 1, "endref": 1, "r1v": 2, "r1v_bisect": 1,
