@@ -148,6 +148,11 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--autotune", action="store_true",
+                    help="search the launch-shape parameters on this workload before timing "
+                         "(mr3smp_amd/tune.py) and write them to --tuning-file")
+    ap.add_argument("--tuning-file", default=os.path.join(REPO, "mr3smp_amd", "tuning_mi355x.json"),
+                    help="launch-shape parameters applied before timing, if the file exists")
     ap.add_argument("--copy-calib", type=int, default=0,
                     help="only run a device-to-device copy of this many MiB (five times): the "
                          "known-size kernel that scales the HBM byte counters")
@@ -185,6 +190,19 @@ def main():
     # the problem descriptors (pointers into HBM and scalars) are built once; a step
     # is one mr3_solve_batch call over them
     arr, keep = M.prepare_batch(probs)
+    tuning_src = "library defaults"
+    if args.autotune:
+        from mr3smp_amd import tune
+        best, bt, tried = tune.autotune(arr, stream=stream, log=log)
+        tune.save(best, args.tuning_file,
+                  note=f"coordinate descent on the batched mixed workload ({len(tried)} settings, "
+                       f"best {bt * 1e3:.1f} ms per solve)")
+        tuning_src = f"autotuned in this run ({len(tried)} settings)"
+    elif args.tuning_file and os.path.exists(args.tuning_file):
+        from mr3smp_amd import tune
+        tune.apply(args.tuning_file)
+        tuning_src = os.path.relpath(args.tuning_file, REPO)
+    tuning = dict(M.get_tuning(), source=tuning_src)
     for _ in range(args.warmup):
         rc = M.run_batch(arr, stream=stream)
         assert rc == 0
@@ -309,6 +327,7 @@ def main():
         "gpu_fp64_tflops_all_kernels": total_flops / (gpu_ms / 1e3) / 1e12 if gpu_ms else None,
         "launches_per_step": {k: v["launches"] / prof_steps for k, v in kern.items()},
         "sweep_elements_per_step": dict(steps),
+        "tuning": tuning,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

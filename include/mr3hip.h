@@ -207,6 +207,30 @@ void mr3_profile_enable(int on);
 void mr3_profile_reset(void);
 int mr3_profile_report(char *buf, int len);
 
+/*
+ * Launch-shape parameters, process-wide, read at the start of every solve.  They
+ * change how the work is spread over the device, never the results.  No counterpart
+ * in the reference (whose only knob is PMR_NUM_THREADS); the defaults were measured on
+ * the batched workload, and mr3smp_amd/tune.py searches them for another workload.
+ *   side_streams     streams for the singleton phases, 1..3 (0: 3 when
+ *                    GPU_MAX_HW_QUEUES >= 8, else 2)
+ *   tail_singletons  iterations with at most this many singletons going on run in the
+ *                    latency kernel (one workgroup per singleton), 0..2^24
+ *   group_lanes      lane budget of the multisection cluster refinements, >= 4096
+ *   rrf_group_max    levels with more clusters run dlarrf on one lane per cluster
+ *   singleton_waves  most waves of one singleton launch, >= 64
+ * mr3_set_tuning returns 0, or 1 (and changes nothing) when a value is out of range.
+ */
+typedef struct mr3_tuning {
+    int side_streams;
+    int tail_singletons;
+    int group_lanes;
+    int rrf_group_max;
+    int singleton_waves;
+} mr3_tuning;
+void mr3_get_tuning(mr3_tuning *t);
+int mr3_set_tuning(const mr3_tuning *t);
+
 /* Library version string. */
 const char *mr3_version(void);
 

@@ -41,6 +41,13 @@ class Problem(ctypes.Structure):
     ]
 
 
+class Tuning(ctypes.Structure):
+    """Mirror of ``mr3_tuning`` (include/mr3hip.h)."""
+    _fields_ = [("side_streams", ctypes.c_int), ("tail_singletons", ctypes.c_int),
+                ("group_lanes", ctypes.c_int), ("rrf_group_max", ctypes.c_int),
+                ("singleton_waves", ctypes.c_int)]
+
+
 class Request(ctypes.Structure):
     """Mirror of ``mr3_request`` (include/mr3hip.h)."""
     _fields_ = [
@@ -82,6 +89,10 @@ def lib():
         L.mr3_dsyeig_device.restype = ctypes.c_int
         L.mr3_dsyeig_device.argtypes = ([ctypes.c_char] * 3 + [ctypes.c_int, _P, ctypes.c_int]
                                         + [_P] * 7 + [ctypes.c_int, ctypes.c_int, _P])
+        L.mr3_get_tuning.restype = None
+        L.mr3_get_tuning.argtypes = [ctypes.POINTER(Tuning)]
+        L.mr3_set_tuning.restype = ctypes.c_int
+        L.mr3_set_tuning.argtypes = [ctypes.POINTER(Tuning)]
         L.mr3_solve_batch.restype = ctypes.c_int
         L.mr3_solve_batch.argtypes = [ctypes.POINTER(Problem), ctypes.c_int, _P]
         L.mr3_solve_one.restype = ctypes.c_int
@@ -330,6 +341,27 @@ def prepare_batch(problems):
         q.Zsupp = p["Zsupp"].data_ptr() if wantz else None
         keep.append(p)
     return arr, keep
+
+
+def get_tuning():
+    """The process-wide launch-shape parameters (mr3_get_tuning) as a dict."""
+    t = Tuning()
+    lib().mr3_get_tuning(ctypes.byref(t))
+    return {f: getattr(t, f) for f, _ in Tuning._fields_}
+
+
+def set_tuning(**kw):
+    """Changes some of the launch-shape parameters; raises ValueError when the library
+    rejects a value (nothing is changed then).  Results never depend on them."""
+    cur = get_tuning()
+    unknown = set(kw) - set(cur)
+    if unknown:
+        raise ValueError(f"unknown tuning parameters {sorted(unknown)}")
+    cur.update({k: int(v) for k, v in kw.items()})
+    t = Tuning(**cur)
+    if lib().mr3_set_tuning(ctypes.byref(t)) != 0:
+        raise ValueError(f"tuning rejected: {cur}")
+    return cur
 
 
 def profile_enable(on=True, counters=True):

@@ -18,6 +18,8 @@
 namespace mr3 {
 int run_pipeline(std::vector<MatDesc>& md, std::vector<MatState>& ms, hipStream_t st);
 void profile_enable(int on);
+int set_tuning(const mr3_tuning* t);
+void get_tuning(mr3_tuning* t);
 void profile_reset();
 int profile_report(char* buf, int len);
 
@@ -181,6 +183,8 @@ extern "C" void mrrr_(char* jobz, char* range, int* n, double* D, double* E, dou
 }
 
 extern "C" void mr3_profile_enable(int on) { mr3::profile_enable(on); }
+extern "C" void mr3_get_tuning(mr3_tuning* t) { mr3::get_tuning(t); }
+extern "C" int mr3_set_tuning(const mr3_tuning* t) { return mr3::set_tuning(t); }
 extern "C" void mr3_profile_reset(void) { mr3::profile_reset(); }
 extern "C" int mr3_profile_report(char* buf, int len) { return mr3::profile_report(buf, len); }
 

@@ -38,18 +38,37 @@ namespace mr3 {
 
 static inline unsigned cdiv(long long a, long long b) { return (unsigned)((a + b - 1) / b); }
 
+// Launch-shape parameters of a solve (mr3_set_tuning).  The results do not depend on
+// them; the defaults were measured on the batched workload and an autotuner
+// (mr3smp_amd/tune.py) can search them for another workload or device.
+//   side_streams    singleton side streams (1 .. 3; 0: three with 8 or more hardware
+//                   queues, else two)
+//   tail_singletons the tail kernel takes over an iteration with at most this many
+//                   singletons going on
+//   group_lanes     lane budget of the group (multisection) cluster refinements
+//   rrf_group_max   levels with more clusters run dlarrf with one lane per cluster
+//   singleton_waves most waves of one singleton launch (also bounds its scratch)
+static std::mutex g_tune_mu;
+static mr3_tuning g_tune = {-1, 8192, 262144, 1 << 16, 8192};
+static mr3_tuning tuning_snapshot() {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    if (g_tune.side_streams < 0) {   // first use: the environment's overrides
+        g_tune.side_streams = 0;
+        if (const char* v = getenv("MR3_SIDE_STREAMS")) g_tune.side_streams = std::max(1, std::min(3, atoi(v)));
+        if (const char* v = getenv("MR3_SWAVES")) g_tune.singleton_waves = std::max(64, atoi(v));
+    }
+    return g_tune;
+}
+
 // Lanes per eigenvalue for the cluster refinements of a level with `items`
 // eigenvalues: 1 (one-lane kernels) unless a group of 4 or more lanes per eigenvalue
-// keeps the launch within kGroupLanes lanes; then the largest such power of two up
-// to a wave.  Multisection with G lanes does (G - 1) / log2(G) times the sweeps of
+// keeps the launch within the lane budget; then the largest such power of two up to a
+// wave.  Multisection with G lanes does (G - 1) / log2(G) times the sweeps of
 // one-point bisection in 1 / log2(G) of the sequential rounds.
-constexpr long long kGroupLanes = 262144;
-// Largest level (clusters) whose dlarrf trials run four lanes per cluster.
-constexpr int kRrfGroupMax = 1 << 16;
-static int group_lanes(long long items) {
-    if (items * 4 > kGroupLanes) return 1;
+static int group_lanes(long long items, long long budget) {
+    if (items * 4 > budget) return 1;
     int G = 4;
-    while (G < 64 && items * 2 * G <= kGroupLanes) G *= 2;
+    while (G < 64 && items * 2 * G <= budget) G *= 2;
     return G;
 }
 
@@ -142,13 +161,10 @@ struct CallEvents {
 // GPU_MAX_HW_QUEUES of them (4 by default) and streams beyond that share one, so
 // that a level can wait for the tail of an unrelated one.  Measured on the batched
 // workload: two side streams with 4 queues (three were slower), three with 8 or more
-// queues.  MR3_SIDE_STREAMS (1 .. 3) overrides the choice.
+// queues.  The side_streams tuning parameter (or MR3_SIDE_STREAMS) overrides it.
 constexpr int kSide = 3;   // most side streams of a solve
-static int side_count() {
-    if (const char* v = getenv("MR3_SIDE_STREAMS")) {
-        const int k = atoi(v);
-        return k < 1 ? 1 : (k > kSide ? kSide : k);
-    }
+static int side_count(int tuned) {
+    if (tuned >= 1) return std::min(tuned, kSide);
     if (const char* q = getenv("GPU_MAX_HW_QUEUES"))
         if (atoi(q) >= 8) return 3;
     return 2;
@@ -316,6 +332,7 @@ static int env_int(const char* name, int dflt) {
 // outputs land in the caller's arrays named in md; states are copied back to ms.
 int run_pipeline(std::vector<MatDesc>& md, std::vector<MatState>& ms, hipStream_t st) {
     configure_pool_once();
+    const mr3_tuning tune = tuning_snapshot();
     const int nmat = (int)md.size();
     long long ntot = 0;
     int nmax = 0;
@@ -457,7 +474,7 @@ int run_pipeline(std::vector<MatDesc>& md, std::vector<MatState>& ms, hipStream_
             int* DI;          // dense list of the tail kernel
             double* scratch;
         } ss[kSide] = {};
-        const int nside = side_count();
+        const int nside = side_count(tune.side_streams);
         for (int k = 0; k < nside; ++k) {
             ss[k].wtasks = A.get<WaveTask>(capT);
             ss[k].scnt = A.get<LevelCounters>(1);
@@ -482,7 +499,7 @@ int run_pipeline(std::vector<MatDesc>& md, std::vector<MatState>& ms, hipStream_
         const size_t per_wave = ck_per_lane * 64 * sizeof(double);
         const size_t budget = singleton_scratch_budget() / nside;
         int max_waves = (int)std::max<size_t>(1, std::min<size_t>(budget / per_wave,
-                                                                  (size_t)env_int("MR3_SWAVES", 8192)));
+                                                                  (size_t)tune.singleton_waves));
         const int want_waves = capT;
         const int nwaves = std::min(max_waves, want_waves);
         for (int k = 0; k < nside; ++k) ss[k].scratch = A.get<double>((size_t)nwaves * ck_per_lane * 64);
@@ -553,7 +570,7 @@ int run_pipeline(std::vector<MatDesc>& md, std::vector<MatState>& ms, hipStream_
                 LAUNCH(k_zero, dim3(1), dim3(64), (int*)bump, 2, (int*)dcnt,
                        (int)(sizeof(LevelCounters) / sizeof(int)));
                 LAUNCH(k_arena_assign, dim3(cdiv(nC, 64)), dim3(64), Cl[cur], nC, arena, bump);
-                const int Ge = group_lanes(2LL * nC);
+                const int Ge = group_lanes(2LL * nC, tune.group_lanes);
                 if (Ge == 1)
                     LAUNCH(k_endref, dim3(cdiv(2LL * nC, 64)), dim3(64), Cl[cur], nC, Windex, Wsh,
                            Werr, dms, Wtmp, Werrtmp);
@@ -561,7 +578,7 @@ int run_pipeline(std::vector<MatDesc>& md, std::vector<MatState>& ms, hipStream_
                     LAUNCH(k_endref_grp, dim3(cdiv(2LL * nC, 64 / Ge)), dim3(64), Cl[cur], nC, Ge,
                            Windex, Wsh, Werr, dms, Wtmp, Werrtmp);
                 // four lanes per cluster unless the clusters alone fill the device
-                if (nC > kRrfGroupMax) {
+                if (nC > tune.rrf_group_max) {
                     LAUNCH(k_dlarrf<1>, dim3(cdiv(nC, 64)), dim3(64), Cl[cur], nC, Wsh, Werr, Wgap,
                            Wtmp, Werrtmp, dms);
                 } else {
@@ -570,7 +587,7 @@ int run_pipeline(std::vector<MatDesc>& md, std::vector<MatState>& ms, hipStream_
                     LAUNCH(k_rrf_store, dim3(cdiv(nC, 64)), dim3(64), Cl[cur], nC, dms);
                 }
                 LAUNCH(k_newrrr_fin, dim3(nC), dim3(64), Cl[cur], nC, Wsh, Werr);
-                const int Gr = group_lanes(nE);
+                const int Gr = group_lanes(nE, tune.group_lanes);
                 if (Gr == 1)
                     LAUNCH(k_refine_cl, dim3(cdiv(nE, 64)), dim3(64), Cl[cur], es[cur], ec[cur], nE,
                            Windex, Wsh, Werr, Wgap, dms, Wtmp, Werrtmp);
@@ -619,7 +636,8 @@ int run_pipeline(std::vector<MatDesc>& md, std::vector<MatState>& ms, hipStream_
                 const int g1 = nS / 64 + nbl + 1;   // bound on the tasks of any iteration
                 const int gw = std::min(nwaves, g1);
                 LAUNCH_ON(s2, k_s_iter, dim3(gw), dim3(64), q.wtasks, q.scnt, 1, (const int*)nullptr,
-                          Sl, dmd, dms, Windex, Zidx, W, Wsh, Werr, Wgap, q.ST, q.scratch, maxbl);
+                          Sl, dmd, dms, Windex, Zidx, W, Wsh, Werr, Wgap, q.ST, q.scratch, maxbl,
+                          tune.tail_singletons);
                 // further iterations on the compacted singletons that go on (the
                 // reference's loop runs at most kMaxIterRQI dlar1v calls plus the
                 // best-residual step)
@@ -639,10 +657,11 @@ int run_pipeline(std::vector<MatDesc>& md, std::vector<MatState>& ms, hipStream_
                     LAUNCH_ON(s2, k_s_bisect, dim3(std::min(gw, 2048)), dim3(64), q.BI, q.cit[b], Sl,
                               dms, Windex, q.ST);
                     LAUNCH_ON(s2, k_s_iter, dim3(gw), dim3(64), q.tit[b], q.cit[b], 0, q.sit[b], Sl,
-                              dmd, dms, Windex, Zidx, W, Wsh, Werr, Wgap, q.ST, q.scratch, maxbl);
+                              dmd, dms, Windex, Zidx, W, Wsh, Werr, Wgap, q.ST, q.scratch, maxbl,
+                              tune.tail_singletons);
                     LAUNCH_ON(s2, k_s_tail, dim3(std::min(nS, 1024)), dim3(128), q.DI, q.cit[b], Sl,
                               dmd, dms, Windex, Zidx, W, Wsh, Werr, Wgap, q.ST,
-                              kMaxIterRQI + 2 - it);
+                              kMaxIterRQI + 2 - it, tune.tail_singletons);
                     ptasks = q.tit[b];
                     pcnt = q.cit[b];
                     psi = q.sit[b];
@@ -743,6 +762,21 @@ int profile_report(char* buf, int len) {
     if ((int)js.size() + 1 > len) return -(int)(js.size() + 1);
     memcpy(buf, js.c_str(), js.size() + 1);
     return (int)js.size();
+}
+
+int set_tuning(const mr3_tuning* t) {
+    if (!t) return 1;
+    if (t->side_streams < 0 || t->side_streams > kSide) return 1;
+    if (t->tail_singletons < 0 || t->tail_singletons > (1 << 24)) return 1;
+    if (t->group_lanes < 4096 || t->rrf_group_max < 0 || t->singleton_waves < 64) return 1;
+    (void)tuning_snapshot();   // environment defaults first, so that they do not override
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune = *t;
+    return 0;
+}
+
+void get_tuning(mr3_tuning* t) {
+    if (t) *t = tuning_snapshot();
 }
 
 }  // namespace mr3

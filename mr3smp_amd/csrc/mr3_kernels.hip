@@ -1995,8 +1995,6 @@ __global__ __launch_bounds__(64) void k_gaps_cl(const ClTask* __restrict__ C,
 // ---------------------------------------------------------------------------------
 constexpr int SW = 64;          // lanes per wave
 constexpr int kTailN = 2048;    // largest block of the tail kernel (2 (n + 1) doubles of LDS)
-constexpr int kTailS = 8192;    // iterations with at most this many singletons going on give
-                                // the singletons of blocks up to kTailN to the tail kernel
 constexpr int KC = kChunk;       // steps per chunk: LDS buffer rows and checkpoint spacing
 constexpr int kUS = 4;          // prefetch group of the full stationary sweep
 
@@ -2821,7 +2819,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
     const WaveTask* __restrict__ tasks, LevelCounters* cnt, int first, const int* __restrict__ SI,
     const STask* __restrict__ T, const MatDesc* __restrict__ md, const MatState* __restrict__ ms,
     const int* __restrict__ Windex, const int* __restrict__ Zidx, double* W, double* Wsh,
-    double* Werr, const double* __restrict__ Wgap, SState* ST, double* scratch, int cap) {
+    double* Werr, const double* __restrict__ Wgap, SState* ST, double* scratch, int cap,
+    int tailS) {
     __shared__ ChunkBuf buf, bld;
     __shared__ double* zcol[SW];
     __shared__ int wr0[SW], wlo[SW], whi[SW];
@@ -2829,7 +2828,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
     const int lane = threadIdx.x;
     const int nT = __builtin_amdgcn_readfirstlane(cnt->nT);
     // late iterations with few singletons leave the blocks that fit the tail kernel to it
-    const bool leave_small = !first && __builtin_amdgcn_readfirstlane(cnt->nS) <= kTailS;
+    const bool leave_small = !first && __builtin_amdgcn_readfirstlane(cnt->nS) <= tailS;
     Ck ck;
     ck.nck = cap / KC + 2;
     ck.lane = lane;
@@ -3230,11 +3229,11 @@ __device__ __forceinline__ void s_bisect_one(int b, int nb, int G, const int* __
 
 // ---------------------------------------------------------------------------------
 // The tail of a level's singleton iterations: once few singletons go on (at most
-// kTailS), each one of a block up to kTailN rows gets a workgroup of two waves, which
-// runs all the iterations left to it in the bulk schedule one after the other (with
-// the bisection fallback on its first wave when an iteration asks for it), so the
-// level's tail costs the chain of its slowest singleton instead of one round of
-// launches per iteration.  The
+// tailS, a launch argument), each one of a block up to kTailN rows gets a workgroup of
+// two waves, which runs all the iterations left to it in the bulk schedule one after
+// the other (with the bisection fallback on its first wave when an iteration asks for
+// it), so the level's tail costs the chain of its slowest singleton instead of one
+// round of launches per iteration.  The
 // stationary transform (wave 0) and the progressive transform (wave 1) run at the
 // same time, each on one lane, keeping their S and P values in LDS; the twist search
 // is a reduction over all lanes; L+ and U- are recomputed from the stored carries by
@@ -3418,7 +3417,7 @@ __global__ __launch_bounds__(128) void k_s_tail(const int* __restrict__ DI,
                                                 const int* __restrict__ Zidx, double* W,
                                                 double* Wsh, double* Werr,
                                                 const double* __restrict__ Wgap, SState* ST,
-                                                int iters) {
+                                                int iters, int tailS) {
     __shared__ double SV[kTailN + 1];   // S values, then L+, then z above the twist
     __shared__ double PV[kTailN + 1];   // P values, then U-, then z below the twist (shifted)
     __shared__ double sh_slast;
@@ -3427,7 +3426,7 @@ __global__ __launch_bounds__(128) void k_s_tail(const int* __restrict__ DI,
     __shared__ int tw_t[2], tw_h[2];
     __shared__ SState sq;               // the singleton's state between its iterations
     __shared__ int s_k;
-    if (cnt->nS > kTailS) return;
+    if (cnt->nS > tailS) return;
     const int nD = cnt->nD;
     const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
     for (;;) {

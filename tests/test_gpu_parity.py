@@ -249,18 +249,24 @@ def test_gpu_batch_equals_single(M):
 
 
 def test_gpu_side_stream_count_bitwise(M):
-    """The tree levels' singleton phases may run on one, two or three side streams;
-    the results are the same bit for bit (the streams only change what overlaps)."""
-    import os
+    """The tree levels' singleton phases may run on one, two or three side streams, and
+    the other launch-shape parameters (mr3_set_tuning) move work between kernels; the
+    results are the same bit for bit (they only change what overlaps and where)."""
     import torch
     from mr3smp_amd import workloads as WL
     dev = torch.device("cuda:0")
     mats = WL.batched_mixed(64, seed_extra=5)
+    settings = [dict(side_streams=1), dict(side_streams=2), dict(side_streams=3),
+                dict(side_streams=2, tail_singletons=0, group_lanes=4096, rrf_group_max=0,
+                     singleton_waves=64),
+                dict(side_streams=3, tail_singletons=1 << 20, group_lanes=1 << 22,
+                     rrf_group_max=1 << 20)]
     outs = []
-    old = os.environ.get("MR3_SIDE_STREAMS")
+    before = M.get_tuning()
     try:
-        for k in ("1", "2", "3"):
-            os.environ["MR3_SIDE_STREAMS"] = k
+        for kw in settings:
+            M.set_tuning(**dict(before, **kw))
+            print("tuning", M.get_tuning(), flush=True)
             probs = [dict(D=torch.tensor(D, device=dev), E=torch.tensor(E, device=dev))
                      for D, E, _ in mats]
             rc, arr, keep = M.solve_batch(probs)
@@ -268,10 +274,7 @@ def test_gpu_side_stream_count_bitwise(M):
             assert rc == 0
             outs.append((arr, keep))
     finally:
-        if old is None:
-            os.environ.pop("MR3_SIDE_STREAMS", None)
-        else:
-            os.environ["MR3_SIDE_STREAMS"] = old
+        M.set_tuning(**before)
     a0, k0 = outs[0]
     for arr, keep in outs[1:]:
         for i in range(len(mats)):

@@ -142,3 +142,24 @@ def test_packed_and_complex_drivers_reject_invalid_arguments():
             it, nn = ctypes.c_int(itype), ctypes.c_int(n)
             assert getattr(L, name)(P(it), b"V", b"A", b"L", P(nn), b, b, P(vl), P(vu), P(il), P(iu),
                                     P(m), w, b, P(ldz)) == 1, (name, itype, n)
+
+
+def test_tuning_parameters_round_trip_and_reject_bad_values():
+    """mr3_set_tuning / mr3_get_tuning are host-only: values round-trip, out-of-range
+    values are rejected without changing anything."""
+    import pytest
+    import mr3smp_amd as M
+    before = M.get_tuning()
+    try:
+        got = M.set_tuning(tail_singletons=4096, group_lanes=131072)
+        assert got["tail_singletons"] == 4096 and M.get_tuning()["group_lanes"] == 131072
+        for bad in (dict(side_streams=4), dict(group_lanes=100), dict(singleton_waves=1),
+                    dict(tail_singletons=-1)):
+            with pytest.raises(ValueError):
+                M.set_tuning(**bad)
+        assert M.get_tuning()["tail_singletons"] == 4096
+        with pytest.raises(ValueError):
+            M.set_tuning(no_such_knob=1)
+    finally:
+        M.set_tuning(**before)
+    assert M.get_tuning() == before
