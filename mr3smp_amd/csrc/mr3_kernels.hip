@@ -3680,11 +3680,24 @@ __global__ __launch_bounds__(64) void k_s_normalize(const WaveTask* __restrict__
     sc_s[lane] = q.nrminv;
     const int nfull = dsc.n;
     __syncthreads();
+    // four rows per lane in flight; only the support is read back
     for (int c = 0; c < cntw; ++c) {
-        double* zc = zc_s[c];
+        const GPtrW zc = as_global_w(zc_s[c]);
         const int lo = lo_s[c], hi = hi_s[c];
         const double sc = sc_s[c];
-        for (int rr = lane; rr < nfull; rr += SW) zc[rr] = (rr >= lo && rr <= hi) ? zc[rr] * sc : 0.0;
+        for (int base = 0; base < nfull; base += 4 * SW) {
+            double v[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int rr = base + u * SW + lane;
+                v[u] = (rr < nfull && rr >= lo && rr <= hi) ? zc[rr] * sc : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int rr = base + u * SW + lane;
+                if (rr < nfull) zc[rr] = v[u];
+            }
+        }
     }
 }
 
