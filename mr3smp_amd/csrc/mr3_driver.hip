@@ -714,8 +714,9 @@ int run_pipeline(std::vector<MatDesc>& md, std::vector<MatState>& ms, hipStream_
     if (nbl > 0) {
         LAUNCH(k_rrj, dim3(cdiv(N, 256)), dim3(256), dmd, dms, blk, smat, iblock,
                            Windex, Dcopy, E2copy, W, Werr, N);
+        // (tcount and isplit are free by now: the reordering's permutation and marks)
         LAUNCH(k_finalize, dim3(nmat), dim3(256), dmd, dms, W, Zidx, blk, iblock,
-                           Windex);
+                           Windex, tcount, isplit, Wcoarse);
         MR3_CHECK(hipGetLastError());
     }
     MR3_CHECK(hipMemcpyAsync(ms.data(), dms, sizeof(MatState) * nmat, hipMemcpyDeviceToHost, st));

@@ -3711,15 +3711,21 @@ __global__ __launch_bounds__(256) void k_finalize(const MatDesc* __restrict__ md
                                                   const int* __restrict__ Zidx,
                                                   const BlockRec* __restrict__ blk,
                                                   const int* __restrict__ iblock,
-                                                  const int* __restrict__ Windex) {
+                                                  const int* __restrict__ Windex, int* perm_all,
+                                                  int* mark_all, double* colbuf_all) {
+    __shared__ int sh_changed, sh_unsorted;
+    __shared__ int sh_cand[256];
+    __shared__ int sh_ncand;
     const int k = blockIdx.x;
+    const int tid = threadIdx.x;
     MatState& S = ms[k];
     const MatDesc dsc = md[k];
     if (S.done || S.status != kOk) return;
     const int off = dsc.off, m = S.m;
     const double inv = 1.0 / S.scale;
     const bool wantz = dsc.jobz == kV;
-    for (int s = threadIdx.x; s < m; s += blockDim.x) {
+    const GPtrW Wout = as_global_w(dsc.Wout);
+    for (int s = tid; s < m; s += blockDim.x) {
         double w = W[off + s];
         if (S.scale != 1.0) w *= inv;
         int pos = s;
@@ -3733,31 +3739,98 @@ __global__ __launch_bounds__(256) void k_finalize(const MatDesc* __restrict__ md
             }
             pos = rank;
         }
-        dsc.Wout[pos] = w;
+        Wout[pos] = w;
     }
+    if (tid == 0) sh_unsorted = 0;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        if (S.scale != 1.0) { S.vl *= inv; S.vu *= inv; }
-        if (wantz) {
-            // ordering check of sort_eigenpairs (mrrr.c:557-586)
-            bool sorted = false;
-            while (!sorted) {
-                sorted = true;
-                for (int i = 0; i < m - 1; ++i) {
-                    if (dsc.Wout[i] > dsc.Wout[i + 1]) {
-                        sorted = false;
-                        double t = dsc.Wout[i]; dsc.Wout[i] = dsc.Wout[i + 1]; dsc.Wout[i + 1] = t;
-                        int a = dsc.Zsupp[2 * i]; dsc.Zsupp[2 * i] = dsc.Zsupp[2 * i + 2]; dsc.Zsupp[2 * i + 2] = a;
-                        a = dsc.Zsupp[2 * i + 1]; dsc.Zsupp[2 * i + 1] = dsc.Zsupp[2 * i + 3]; dsc.Zsupp[2 * i + 3] = a;
-                        double* z0 = dsc.Z + (size_t)i * dsc.ldz;
-                        double* z1 = dsc.Z + (size_t)(i + 1) * dsc.ldz;
-                        for (int r = 0; r < dsc.n; ++r) { double u = z0[r]; z0[r] = z1[r]; z1[r] = u; }
+    if (tid == 0 && S.scale != 1.0) { S.vl *= inv; S.vu *= inv; }
+    if (wantz) {
+        // The ordering check of sort_eigenpairs (mrrr.c:557-586): passes that swap
+        // neighbours with W(i) > W(i+1) until none does, moving Zsupp and the Z columns
+        // along.  That is the stable sort by W of the order Zindex gives, which the
+        // odd-even transposition sort below reaches too (it also swaps only strictly
+        // larger neighbours); the columns then move once, along the cycles of the
+        // resulting permutation.  Almost always nothing is out of order.
+        for (int i = tid; i < m - 1; i += blockDim.x)
+            if (Wout[i] > Wout[i + 1]) sh_unsorted = 1;
+        __syncthreads();
+        if (sh_unsorted) {
+            int* perm = perm_all + off;    // perm[q]: the column that ends at position q
+            int* mark = mark_all + off;
+            const GPtrW colbuf = as_global_w(colbuf_all + off);
+            const GIntW Zs = as_global_w(dsc.Zsupp);
+            for (int q = tid; q < m; q += blockDim.x) { perm[q] = q; mark[q] = 0; }
+            __syncthreads();
+            int prev = 1;
+            for (int phase = 0;; ++phase) {
+                if (tid == 0) sh_changed = 0;
+                __syncthreads();
+                for (int i = 2 * tid + (phase & 1); i < m - 1; i += 2 * blockDim.x) {
+                    const double a = Wout[i], b = Wout[i + 1];
+                    if (a > b) {
+                        Wout[i] = b;
+                        Wout[i + 1] = a;
+                        const int s0 = Zs[2 * i], s1 = Zs[2 * i + 1];
+                        Zs[2 * i] = Zs[2 * i + 2];
+                        Zs[2 * i + 1] = Zs[2 * i + 3];
+                        Zs[2 * i + 2] = s0;
+                        Zs[2 * i + 3] = s1;
+                        const int p = perm[i];
+                        perm[i] = perm[i + 1];
+                        perm[i + 1] = p;
+                        sh_changed = 1;
                     }
                 }
+                __syncthreads();
+                const int ch = sh_changed;
+                __syncthreads();
+                if (!ch && !prev) break;   // an even and an odd phase without a swap
+                prev = ch;
+            }
+            // move the columns along the permutation's cycles, one column of temporary
+            const GPtrW Z = as_global_w(dsc.Z);
+            const size_t ldz = (size_t)dsc.ldz;
+            const int n = dsc.n;
+            for (int base = 0; base < m; base += 256) {
+                if (tid == 0) sh_ncand = 0;
+                __syncthreads();
+                const int q = base + tid;
+                // candidates in index order: every thread's flag, then one thread lists them
+                sh_cand[tid] = (q < m && perm[q] != q) ? 1 : 0;
+                __syncthreads();
+                if (tid == 0) {
+                    int c = 0;
+                    for (int t = 0; t < 256; ++t)
+                        if (sh_cand[t]) sh_cand[c++] = base + t;
+                    sh_ncand = c;
+                }
+                __syncthreads();
+                const int nc = sh_ncand;
+                for (int ci = 0; ci < nc; ++ci) {
+                    const int start = sh_cand[ci];
+                    if (mark[start]) continue;   // uniform: every thread reads the same flag
+                    for (int r = tid; r < n; r += blockDim.x) colbuf[r] = Z[(size_t)start * ldz + r];
+                    __syncthreads();
+                    int cur = start;
+                    for (;;) {
+                        const int src = perm[cur];
+                        if (tid == 0) mark[cur] = 1;
+                        if (src == start) {
+                            for (int r = tid; r < n; r += blockDim.x) Z[(size_t)cur * ldz + r] = colbuf[r];
+                            __syncthreads();
+                            break;
+                        }
+                        for (int r = tid; r < n; r += blockDim.x)
+                            Z[(size_t)cur * ldz + r] = Z[(size_t)src * ldz + r];
+                        __syncthreads();
+                        cur = src;
+                    }
+                }
+                __syncthreads();
             }
         }
-        S.done = 1;
     }
+    if (tid == 0) S.done = 1;
 }
 
 }  // namespace mr3
