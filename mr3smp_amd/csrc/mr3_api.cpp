@@ -14,6 +14,7 @@
 
 #include "../../include/mr3hip.h"
 #include "mr3_internal.h"
+#include "mr3_staging.h"
 
 namespace mr3 {
 int run_pipeline(std::vector<MatDesc>& md, std::vector<MatState>& ms, hipStream_t st);
@@ -34,6 +35,7 @@ static hipStream_t thread_stream() {
 }
 
 static int upper(char c) { return (c >= 'a' && c <= 'z') ? c - 32 : c; }
+
 
 // Argument checks of mrrr.c:118-126.  Returns 0 if valid.
 static int check_args(int jz, int rg, int n, double vl, double vu, int il, int iu) {
@@ -166,9 +168,12 @@ extern "C" int mrrr(char* jobz, char* range, int* np, double* D, double* E, doub
         if (hipMemcpyAsync(W, dW, sizeof(double) * p.m, hipMemcpyDeviceToHost, st) != hipSuccess) return fail(2);
         if (wantz) {
             const int lz = *ldz;
-            if (hipMemcpy2DAsync(Z, sizeof(double) * (size_t)lz, dZ, sizeof(double) * (size_t)n,
-                                 sizeof(double) * (size_t)n, p.m, hipMemcpyDeviceToHost, st) != hipSuccess)
-                return fail(2);
+            // large blocks through pinned memory, several host threads placing them
+            Staging* sg = (sizeof(double) * (size_t)n * p.m > kStageDirect) ? staging_get() : nullptr;
+            Staging direct;
+            const int rcz = copy_cols_out(Z, (size_t)lz, dZ, (size_t)n, (size_t)p.m, st, sg ? *sg : direct);
+            if (sg) staging_put(sg);
+            if (rcz != 0) return fail(2);
             if (hipMemcpyAsync(Zsupp, dZs, sizeof(int) * 2 * p.m, hipMemcpyDeviceToHost, st) != hipSuccess)
                 return fail(2);
         }

@@ -29,6 +29,7 @@
 #include <type_traits>
 
 #include "../../include/mr3hip.h"
+#include "mr3_staging.h"
 
 namespace {
 
@@ -220,6 +221,20 @@ struct Buf {
     }
 };
 
+// Columns 0 .. m-1 of a device block (leading dimension n) into host memory with
+// leading dimension ld, large blocks through the pinned staging buffers.
+template <class T>
+int copy_block_out(T* dst, int ld, const T* src, int n, int m, hipStream_t st) {
+    constexpr size_t w = sizeof(T) / sizeof(double);
+    const bool big = sizeof(T) * (size_t)n * m > mr3::kStageDirect;
+    mr3::Staging* sg = big ? mr3::staging_get() : nullptr;
+    mr3::Staging direct;
+    const int rc = mr3::copy_cols_out((double*)dst, w * (size_t)ld, (const double*)src, w * (size_t)n,
+                                      (size_t)m, st, sg ? *sg : direct);
+    if (sg) mr3::staging_put(sg);
+    return rc;
+}
+
 // Argument checks of the standard drivers (dense.c:174-180, 341-347, 596-602, 788-794).
 int check_std(int jz, int rg, int ul, int n, const double* vl, const double* vu, const int* il,
               const int* iu) {
@@ -352,10 +367,7 @@ int standard_host(int jz, int rg, int ul, int n, const T* A, int lda, double* vl
     if (rc) return rc;
     if (*m > 0 && jz != 'C') {
         if (hipMemcpyAsync(W, dW, sizeof(double) * *m, hipMemcpyDeviceToHost, st) != hipSuccess) return 2;
-        if (jz == 'V' &&
-            hipMemcpy2DAsync(Z, sizeof(T) * (size_t)ldz, dZ, sizeof(T) * (size_t)n, sizeof(T) * (size_t)n, *m,
-                             hipMemcpyDeviceToHost, st) != hipSuccess)
-            return 2;
+        if (jz == 'V' && copy_block_out(Z, ldz, dZ, n, *m, st) != 0) return 2;
     }
     return hipStreamSynchronize(st) == hipSuccess ? 0 : 2;
 }
@@ -430,10 +442,8 @@ int general_host(int it, int jz, int rg, int ul, int n, T* A, int lda, T* Bm, in
         if (jz == 'V') {
             // eigenvectors into A (dense.c:307-308) or into Z (the packed drivers)
             T* dst = packed ? Z : A;
-            const size_t ld = packed ? (size_t)ldz : (size_t)lda;
-            if (hipMemcpy2DAsync(dst, sizeof(T) * ld, dX, sizeof(T) * (size_t)n, sizeof(T) * (size_t)n, *m,
-                                 hipMemcpyDeviceToHost, st) != hipSuccess)
-                return 2;
+            const int ld = packed ? ldz : lda;
+            if (copy_block_out(dst, ld, dX, n, *m, st) != 0) return 2;
         }
     }
     // B holds its Cholesky factor on return, as in the reference

@@ -29,6 +29,7 @@
 #include <vector>
 
 #include "../../include/mr3hip.h"
+#include "mr3_staging.h"
 
 namespace mr3 {
 void release_thread_streams();   // mr3_driver.hip: the solver's per-thread side streams
@@ -36,11 +37,10 @@ void release_thread_streams();   // mr3_driver.hip: the solver's per-thread side
 
 namespace {
 
-constexpr long long kRowBytes = 512;       // per-row workspace estimate of the pipeline
-constexpr size_t kChunk = 64ull << 20;     // staging chunk of the result copies
-constexpr size_t kDirect = 8ull << 20;     // results up to this size are copied directly
-constexpr int kCopyThreads = 8;            // host threads filling one chunk
+using mr3::copy_out;
+using mr3::Staging;
 
+constexpr long long kRowBytes = 512;       // per-row workspace estimate of the pipeline
 long long footprint(const mr3_request& r) {
     long long cols = 0;
     if (r.jobz == 'V' || r.jobz == 'v') cols = r.mcols > 0 ? r.mcols : r.n;
@@ -52,62 +52,6 @@ struct Entry {
     mr3_request* req;
     long long bytes;
 };
-
-// Pinned double buffer of one worker.
-struct Staging {
-    char* buf[2] = {nullptr, nullptr};
-    hipEvent_t ev[2] = {nullptr, nullptr};
-    bool ok = false;
-    void init() {
-        ok = hipHostMalloc((void**)&buf[0], kChunk, hipHostMallocDefault) == hipSuccess &&
-             hipHostMalloc((void**)&buf[1], kChunk, hipHostMallocDefault) == hipSuccess &&
-             hipEventCreateWithFlags(&ev[0], hipEventDisableTiming) == hipSuccess &&
-             hipEventCreateWithFlags(&ev[1], hipEventDisableTiming) == hipSuccess;
-    }
-    void release() {
-        for (int k = 0; k < 2; ++k) {
-            if (buf[k]) (void)hipHostFree(buf[k]);
-            if (ev[k]) (void)hipEventDestroy(ev[k]);
-        }
-    }
-};
-
-void par_copy(char* dst, const char* src, size_t bytes) {
-    const size_t per = (bytes + kCopyThreads - 1) / kCopyThreads;
-    std::vector<std::thread> th;
-    for (int t = 1; t < kCopyThreads; ++t) {
-        const size_t o = per * t;
-        if (o >= bytes) break;
-        th.emplace_back([=] { memcpy(dst + o, src + o, std::min(per, bytes - o)); });
-    }
-    memcpy(dst, src, std::min(per, bytes));
-    for (auto& x : th) x.join();
-}
-
-// Device-to-host copy of a result after the work queued on st; returns 0 on success.
-int copy_out(void* dst, const void* src, size_t bytes, hipStream_t st, Staging& sg) {
-    if (bytes <= kDirect || !sg.ok)
-        return hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, st) == hipSuccess ? 0 : 2;
-    char* d = (char*)dst;
-    const char* s = (const char*)src;
-    size_t prev_o = 0, prev_c = 0;
-    int k = 0;
-    for (size_t o = 0; o < bytes; o += kChunk, ++k) {
-        const size_t c = std::min(kChunk, bytes - o);
-        const int b = k & 1;
-        if (hipMemcpyAsync(sg.buf[b], s + o, c, hipMemcpyDeviceToHost, st) != hipSuccess) return 2;
-        if (hipEventRecord(sg.ev[b], st) != hipSuccess) return 2;
-        if (k > 0) {
-            if (hipEventSynchronize(sg.ev[b ^ 1]) != hipSuccess) return 2;
-            par_copy(d + prev_o, sg.buf[b ^ 1], prev_c);
-        }
-        prev_o = o;
-        prev_c = c;
-    }
-    if (hipEventSynchronize(sg.ev[(k - 1) & 1]) != hipSuccess) return 2;
-    par_copy(d + prev_o, sg.buf[(k - 1) & 1], prev_c);
-    return 0;
-}
 
 }  // namespace
 
