@@ -163,6 +163,7 @@ struct CallEvents {
 // workload: two side streams with 4 queues (three were slower), three with 8 or more
 // queues.  The side_streams tuning parameter (or MR3_SIDE_STREAMS) overrides it.
 constexpr int kSide = 3;   // most side streams of a solve
+constexpr int kTailBigWG = 512;   // workgroups of the large-block tail kernel
 static int side_count(int tuned) {
     if (tuned >= 1) return std::min(tuned, kSide);
     if (const char* q = getenv("GPU_MAX_HW_QUEUES"))
@@ -472,7 +473,9 @@ int run_pipeline(std::vector<MatDesc>& md, std::vector<MatState>& ms, hipStream_
             int* sbc;
             int* BI;
             int* DI;          // dense list of the tail kernel
+            int* DIb;         // the same for blocks above kTailN rows
             double* scratch;
+            double* tscratch; // S and P of the large-block tail kernel, per workgroup
         } ss[kSide] = {};
         const int nside = side_count(tune.side_streams);
         for (int k = 0; k < nside; ++k) {
@@ -487,6 +490,7 @@ int run_pipeline(std::vector<MatDesc>& md, std::vector<MatState>& ms, hipStream_
             ss[k].sbc = A.get<int>(N);
             ss[k].BI = A.get<int>(N);
             ss[k].DI = A.get<int>(N);
+            ss[k].DIb = A.get<int>(N);
         }
         if (!A.ok) return 2;
         for (int k = 0; k < kLB; ++k)
@@ -503,6 +507,11 @@ int run_pipeline(std::vector<MatDesc>& md, std::vector<MatState>& ms, hipStream_
         const int want_waves = capT;
         const int nwaves = std::min(max_waves, want_waves);
         for (int k = 0; k < nside; ++k) ss[k].scratch = A.get<double>((size_t)nwaves * ck_per_lane * 64);
+        // the large-block tail kernel: S and P of up to kTailBigWG singletons at a time
+        const int tstride = 2 * (maxbl + 2);
+        const int tail_rows = (maxbl > kTailN) ? maxbl : kTailN;   // blocks the tail kernels take
+        for (int k = 0; k < nside; ++k)
+            ss[k].tscratch = (maxbl > kTailN) ? A.get<double>((size_t)kTailBigWG * tstride) : nullptr;
         if (!A.ok) return 2;
         LevelCounters hc;
         auto reset = [&]() { return hipMemsetAsync(dcnt, 0, sizeof(LevelCounters), st); };
@@ -637,7 +646,7 @@ int run_pipeline(std::vector<MatDesc>& md, std::vector<MatState>& ms, hipStream_
                 const int gw = std::min(nwaves, g1);
                 LAUNCH_ON(s2, k_s_iter, dim3(gw), dim3(64), q.wtasks, q.scnt, 1, (const int*)nullptr,
                           Sl, dmd, dms, Windex, Zidx, W, Wsh, Werr, Wgap, q.ST, q.scratch, maxbl,
-                          tune.tail_singletons);
+                          tune.tail_singletons, tail_rows);
                 // further iterations on the compacted singletons that go on (the
                 // reference's loop runs at most kMaxIterRQI dlar1v calls plus the
                 // best-residual step)
@@ -649,7 +658,7 @@ int run_pipeline(std::vector<MatDesc>& md, std::vector<MatState>& ms, hipStream_
                     const int b = it & 1;
                     // q.cit[b] was cleared by the previous iteration's k_plan_count
                     LAUNCH_ON(s2, k_s_collect, dim3(g1), dim3(64), ptasks, pcnt, pfirst, psi, Sl,
-                              q.ST, q.sit[b], q.sbc, q.BI, q.DI, q.cit[b]);
+                              q.ST, q.sit[b], q.sbc, q.BI, q.DI, q.DIb, q.cit[b]);
                     LAUNCH_ON(s2, k_plan_count, dim3(cdiv(nbl, 64)), dim3(64), dbl, nbl, blk, q.sbc,
                               64, q.cit[b], it < kMaxIterRQI + 1 ? q.cit[b ^ 1] : (LevelCounters*)nullptr);
                     LAUNCH_ON(s2, k_plan_s, dim3(cdiv(nbl, 64)), dim3(64), dbl, nbl, smat, dmd, blk,
@@ -658,10 +667,14 @@ int run_pipeline(std::vector<MatDesc>& md, std::vector<MatState>& ms, hipStream_
                               dms, Windex, q.ST);
                     LAUNCH_ON(s2, k_s_iter, dim3(gw), dim3(64), q.tit[b], q.cit[b], 0, q.sit[b], Sl,
                               dmd, dms, Windex, Zidx, W, Wsh, Werr, Wgap, q.ST, q.scratch, maxbl,
-                              tune.tail_singletons);
-                    LAUNCH_ON(s2, k_s_tail, dim3(std::min(nS, 1024)), dim3(128), q.DI, q.cit[b], Sl,
-                              dmd, dms, Windex, Zidx, W, Wsh, Werr, Wgap, q.ST,
-                              kMaxIterRQI + 2 - it, tune.tail_singletons);
+                              tune.tail_singletons, tail_rows);
+                    LAUNCH_ON(s2, k_s_tail<false>, dim3(std::min(nS, 1024)), dim3(128), q.DI, q.cit[b],
+                              Sl, dmd, dms, Windex, Zidx, W, Wsh, Werr, Wgap, q.ST,
+                              kMaxIterRQI + 2 - it, tune.tail_singletons, (double*)nullptr, 0);
+                    if (maxbl > kTailN)
+                        LAUNCH_ON(s2, k_s_tail<true>, dim3(std::min(nS, kTailBigWG)), dim3(128), q.DIb,
+                                  q.cit[b], Sl, dmd, dms, Windex, Zidx, W, Wsh, Werr, Wgap, q.ST,
+                                  kMaxIterRQI + 2 - it, tune.tail_singletons, q.tscratch, tstride);
                     ptasks = q.tit[b];
                     pcnt = q.cit[b];
                     psi = q.sit[b];

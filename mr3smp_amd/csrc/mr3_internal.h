@@ -129,8 +129,11 @@ struct LevelCounters {
     long long arena;        // doubles needed by the next level's cluster slots
     int err, nT;            // nT: wave tasks planned for the singleton kernel
     int next;               // next wave task to hand out
-    int nD;                 // singletons in the dense list of the tail kernel
+    int nD;                 // singletons in the dense list of the tail kernel (blocks
+                            // up to kTailN rows, S and P in LDS)
     int nextD;              // next entry of that list to hand out
+    int nDb;                // the same for larger blocks (S and P in device memory)
+    int nextDb;
     int cls[kSizeClasses];  // wave tasks per size class
     int clsPos[kSizeClasses];
 };

@@ -2820,14 +2820,15 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
     const STask* __restrict__ T, const MatDesc* __restrict__ md, const MatState* __restrict__ ms,
     const int* __restrict__ Windex, const int* __restrict__ Zidx, double* W, double* Wsh,
     double* Werr, const double* __restrict__ Wgap, SState* ST, double* scratch, int cap,
-    int tailS) {
+    int tailS, int tailN) {
     __shared__ ChunkBuf buf, bld;
     __shared__ double* zcol[SW];
     __shared__ int wr0[SW], wlo[SW], whi[SW];
     __shared__ int s_task;
     const int lane = threadIdx.x;
     const int nT = __builtin_amdgcn_readfirstlane(cnt->nT);
-    // late iterations with few singletons leave the blocks that fit the tail kernel to it
+    // late iterations with few singletons leave those of blocks up to tailN rows to the
+    // tail kernels (all blocks when the large-block tail kernel runs)
     const bool leave_small = !first && __builtin_amdgcn_readfirstlane(cnt->nS) <= tailS;
     Ck ck;
     ck.nck = cap / KC + 2;
@@ -2853,7 +2854,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
         const int idx = first ? pos : SI[pos];
         const STask t = T[idx];
         const int n = __builtin_amdgcn_readfirstlane(t.bsize);
-        if (leave_small && n <= kTailN) continue;
+        if (leave_small && n <= tailN) continue;
         const int bbeg = __builtin_amdgcn_readfirstlane(t.bbeg);
         const MatDesc& dsc = md[t.mat];
         int col = Zidx[t.slot];
@@ -3249,10 +3250,10 @@ constexpr int kUT = 4;
 
 // Stationary transform over steps 1 .. stop-1 (odr1v.f:184-229) on one lane, with S
 // after step t in SV[t] (SV[0] = 0).
-template <class P>
+template <class P, class SP>
 __device__ __forceinline__ void tail_stat(P D, P LD, P L, const double* LLD, double lambda,
                                           double pivmin, int r1, int stop, bool safe,
-                                          double* SV, double& s_out, int& neg_out,
+                                          SP SV, double& s_out, int& neg_out,
                                           double& slast_out) {
     double s = 0.0 - lambda;
     int neg = 0;
@@ -3287,9 +3288,9 @@ __device__ __forceinline__ void tail_stat(P D, P LD, P L, const double* LLD, dou
 
 // Progressive transform from step n-1 down to lo (odr1v.f:231-257) on one lane, with P
 // after step t in PV[t] and PV[n] = D(n) - lambda.
-template <class P>
+template <class P, class SP>
 __device__ __forceinline__ void tail_prog(int n, P D, P LLD, P L, double lambda, double pivmin,
-                                          int lo, bool safe, double* PV, double& p_out,
+                                          int lo, bool safe, SP PV, double& p_out,
                                           int& neg_out) {
     double p = D[n - 1] - lambda;
     PV[n] = p;
@@ -3326,8 +3327,8 @@ __device__ __forceinline__ void tail_prog(int n, P D, P LLD, P L, double lambda,
 
 // Upward half of the vector solve (odr1v.f:285-320) on one lane: Z(j) for j = r-1 down
 // to the support cut, read L+(j) from SV[j] and written there.  Returns ISUPPZ(1).
-template <class P>
-__device__ __forceinline__ int tail_solve_up(int r, P LD, bool nan, double gaptol, double* SV) {
+template <class P, class SP>
+__device__ __forceinline__ int tail_solve_up(int r, P LD, bool nan, double gaptol, SP SV) {
     double zi1 = 1.0, zi2 = 0.0;
     int is0 = 1;
     bool live = r - 1 >= 1;
@@ -3363,9 +3364,9 @@ __device__ __forceinline__ int tail_solve_up(int r, P LD, bool nan, double gapto
 
 // Downward half (odr1v.f:321-355) on one lane: Z(t+1) for t = r .. n-1 up to the cut,
 // read U-(t) from PV[t] and written there.  Returns ISUPPZ(2).
-template <class P>
+template <class P, class SP>
 __device__ __forceinline__ int tail_solve_down(int n, int r, P LD, bool nan, double gaptol,
-                                               double* PV) {
+                                               SP PV) {
     double zi = 1.0, zim1 = 0.0;
     int is1 = n;
     bool live = r <= n - 1;
@@ -3408,6 +3409,9 @@ __device__ __forceinline__ bool tw_better(bool h1, double g1, int t1, bool h2, d
     return a1 < a2 || (a1 == a2 && t1 > t2);
 }
 
+// BIG: blocks above kTailN rows, with S and P in a slice of device memory per
+// workgroup (gsv + blockIdx.x * gstride, two arrays of gstride / 2) instead of LDS.
+template <bool BIG>
 __global__ __launch_bounds__(128) void k_s_tail(const int* __restrict__ DI,
                                                 LevelCounters* cnt,
                                                 const STask* __restrict__ T,
@@ -3417,9 +3421,17 @@ __global__ __launch_bounds__(128) void k_s_tail(const int* __restrict__ DI,
                                                 const int* __restrict__ Zidx, double* W,
                                                 double* Wsh, double* Werr,
                                                 const double* __restrict__ Wgap, SState* ST,
-                                                int iters, int tailS) {
-    __shared__ double SV[kTailN + 1];   // S values, then L+, then z above the twist
-    __shared__ double PV[kTailN + 1];   // P values, then U-, then z below the twist (shifted)
+                                                int iters, int tailS, double* gsv, int gstride) {
+    __shared__ double SVs[BIG ? 1 : kTailN + 1];   // S values, then L+, then z above the twist
+    __shared__ double PVs[BIG ? 1 : kTailN + 1];   // P values, then U-, then z below the twist (shifted)
+    const auto SV = [&] {
+        if constexpr (BIG) return as_global_w(gsv + (size_t)blockIdx.x * gstride);
+        else return &SVs[0];
+    }();
+    const auto PV = [&] {
+        if constexpr (BIG) return as_global_w(gsv + (size_t)blockIdx.x * gstride + gstride / 2);
+        else return &PVs[0];
+    }();
     __shared__ double sh_slast;
     __shared__ int sh_i[8];             // nan1, neg1, nan2, neg2, is0, is1
     __shared__ double tw_g[2];
@@ -3427,13 +3439,13 @@ __global__ __launch_bounds__(128) void k_s_tail(const int* __restrict__ DI,
     __shared__ SState sq;               // the singleton's state between its iterations
     __shared__ int s_k;
     if (cnt->nS > tailS) return;
-    const int nD = cnt->nD;
+    const int nD = BIG ? cnt->nDb : cnt->nD;
     const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
     for (;;) {
         // singletons are handed out one at a time as workgroups free up (their chains
         // differ by up to ten iterations and a bisection)
         __syncthreads();
-        if (tid == 0) s_k = atomicAdd(&cnt->nextD, 1);
+        if (tid == 0) s_k = atomicAdd(BIG ? &cnt->nextDb : &cnt->nextD, 1);
         __syncthreads();
         const int k = s_k;
         if (k >= nD) break;
@@ -3626,8 +3638,8 @@ __global__ __launch_bounds__(128) void k_s_tail(const int* __restrict__ DI,
 }
 
 // Collects the singletons that go on iterating (per block, segmented like the
-// singleton list; those of blocks up to kTailN also in the dense list DI of the tail
-// kernel) and the ones that need the bisection fallback first.  One wave per
+// singleton list; also in the dense list of the tail kernel: DI for blocks up to kTailN
+// rows, DIb for larger ones) and the ones that need the bisection fallback first.  One wave per
 // task of the previous iteration.  Counters: nS = singletons going on, nEig = of
 // them needing bisection.
 __global__ __launch_bounds__(64) void k_s_collect(const WaveTask* __restrict__ tasks,
@@ -3635,21 +3647,45 @@ __global__ __launch_bounds__(64) void k_s_collect(const WaveTask* __restrict__ t
                                                   int first, const int* __restrict__ SIp,
                                                   const STask* __restrict__ T,
                                                   const SState* __restrict__ ST, int* SIn,
-                                                  int* bcnt, int* BI, int* DI,
+                                                  int* bcnt, int* BI, int* DI, int* DIb,
                                                   LevelCounters* cnt) {
     if ((int)blockIdx.x >= pcnt->nT) return;
     const WaveTask wt = tasks[blockIdx.x];
     const int lane = threadIdx.x;
-    if (lane >= wt.cnt) return;
-    const int idx = first ? wt.start + lane : SIp[wt.start + lane];
-    const int f = ST[idx].flags;
+    const bool valid = lane < wt.cnt;
+    const int idx = valid ? (first ? wt.start + lane : SIp[wt.start + lane]) : 0;
+    const int f = valid ? ST[idx].flags : (kSDone << 4);
     const int st = sflag_state(f);
-    if (st == kSDone) return;
-    const STask& t = T[idx];
-    SIn[t.wseg + atomicAdd(&bcnt[t.gblk], 1)] = idx;
-    atomicAdd(&cnt->nS, 1);
-    if (t.bsize <= kTailN) DI[atomicAdd(&cnt->nD, 1)] = idx;
-    if (st == kSActive && ((f >> 2) & 1)) BI[atomicAdd(&cnt->nEig, 1)] = idx;
+    const bool on = valid && st != kSDone;
+    // one atomic per wave and list (a wave's task is one block, so the block counter is
+    // wave-uniform too); each lane takes its position from the ballot below it
+    const unsigned long long below = (1ULL << lane) - 1ULL;
+    const unsigned long long m_on = __ballot(on);
+    if (m_on == 0) return;
+    const int lead = __ffsll((long long)m_on) - 1;
+    const STask& t = T[on ? idx : 0];
+    const bool small = on && t.bsize <= kTailN;
+    const bool big = on && t.bsize > kTailN;
+    const bool bis = on && st == kSActive && ((f >> 2) & 1);
+    const unsigned long long m_s = __ballot(small), m_b = __ballot(big), m_i = __ballot(bis);
+    int b_on = 0, b_s = 0, b_b = 0, b_i = 0;
+    const int gblk = __shfl(on ? t.gblk : 0, lead);
+    const int wseg = __shfl(on ? t.wseg : 0, lead);
+    if (lane == lead) {
+        b_on = atomicAdd(&bcnt[gblk], __popcll(m_on));
+        atomicAdd(&cnt->nS, __popcll(m_on));
+        if (m_s) b_s = atomicAdd(&cnt->nD, __popcll(m_s));
+        if (m_b) b_b = atomicAdd(&cnt->nDb, __popcll(m_b));
+        if (m_i) b_i = atomicAdd(&cnt->nEig, __popcll(m_i));
+    }
+    b_on = __shfl(b_on, lead);
+    b_s = __shfl(b_s, lead);
+    b_b = __shfl(b_b, lead);
+    b_i = __shfl(b_i, lead);
+    if (on) SIn[wseg + b_on + __popcll(m_on & below)] = idx;
+    if (small) DI[b_s + __popcll(m_s & below)] = idx;
+    if (big) DIb[b_b + __popcll(m_b & below)] = idx;
+    if (bis) BI[b_i + __popcll(m_i & below)] = idx;
 }
 
 // Final vectors: Z(isupp) scaled by 1/||z|| (the last dlar1v's NRMINV), zeros in the
