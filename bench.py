@@ -113,6 +113,93 @@ def cpu_baseline(mats, stride, threads):
                        f"PMR_NUM_THREADS=1, wall {dt:.2f} s")
 
 
+def timed_steps(M, torch, arr, stream, steps, warmup, pairs):
+    """Warmup solves, then `steps` timed mr3_solve_batch calls over the same problem
+    array, then one solve with the device work counters on.  Returns (seconds of the
+    timed steps, their per-kernel report, the counted step's report)."""
+    for _ in range(warmup):
+        rc = M.run_batch(arr, stream=stream)
+        assert rc == 0
+    # timed steps: every launch bracketed by HIP events on its own stream (timing only,
+    # the kernels run exactly as unprofiled)
+    M.profile_reset()
+    M.profile_enable(True, counters=False)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        rc = M.run_batch(arr, stream=stream)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    M.profile_enable(False)
+    assert rc == 0 and sum(a.m for a in arr) == pairs
+    log(f"batched: {dt / steps * 1e3:.1f} ms/step, {pairs * steps / dt:.4g} eigenpairs/s")
+    prof = M.profile_report()
+    # one more step with the device work counters on: the algorithmic work of a step
+    # (the counts do not depend on timing, so every step has the same)
+    M.profile_reset()
+    M.profile_enable(True)
+    rc = M.run_batch(arr, stream=stream)
+    torch.cuda.synchronize()
+    M.profile_enable(False)
+    assert rc == 0
+    counted = M.profile_report()
+    M.profile_reset()
+    return dt, prof, counted
+
+
+def plain_path(M, torch, probs, stream, stride):
+    """The plain path: one solve per matrix over every stride-th problem, same
+    device-resident buffers.  Returns (eigenpairs/s, matrices solved)."""
+    sample = list(range(0, len(probs), stride))
+    singles = [M.prepare_batch([probs[i]])[0] for i in sample]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ppairs = 0
+    for one in singles:
+        rc = M.run_batch(one, stream=stream)
+        assert rc == 0
+        ppairs += one[0].m
+    torch.cuda.synchronize()
+    plain = ppairs / (time.perf_counter() - t0)
+    log(f"plain: {plain:.4g} eigenpairs/s on {len(sample)} matrices")
+    return plain, len(sample)
+
+
+def roofline(prof, counted, prof_steps, traffic_key):
+    """Roofline of the dominant kernel: its algorithmic flops per step (from the step
+    run with the device work counters on) over its launches per step, against its
+    average launch time in the timed steps.  Returns (kernel name, roofline dict)."""
+    kern = prof["kernels"]
+    dom = max(kern, key=lambda k: kern[k]["ms"])
+    steps = counted["steps"]
+    launches_per_step = kern[dom]["launches"] / prof_steps
+    assert counted["kernels"][dom]["launches"] == launches_per_step, "launch count differs between steps"
+    fams = KERNEL_STEPS.get(dom, [])
+    flops = sum(steps.get(f, 0) * FLOPS_PER_STEP[f] for f in fams)
+    per_launch_s = kern[dom]["ms"] / 1e3 / kern[dom]["launches"]
+    flops_per_launch = flops / launches_per_step
+    achieved = flops_per_launch / per_launch_s / 1e12 if per_launch_s > 0 else 0.0
+    # the same kernel's ceiling when every SIMD issues division steps back to back
+    dv = sum(DIVIDES_PER_STEP[f] * steps.get(f, 0) for f in fams)
+    divide_ceiling = DIVIDE_STEPS_PER_S * flops / dv / 1e12 if dv else None
+    # HBM bytes per launch measured by rocprofv3 on the same workload, when recorded
+    traffic = None
+    tfile = os.path.join(REPO, "profiles", "traffic.json")
+    if os.path.exists(tfile):
+        try:
+            rec = json.load(open(tfile)).get(traffic_key, {})
+            traffic = rec.get(dom) if isinstance(rec, dict) else None
+        except Exception:
+            traffic = None
+    return dom, {"bound": "mfma", "engine": "fp64 vector ALU (no MFMA on this path)",
+                 "kernel": dom, "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
+                 "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
+                 "divide_ceiling": divide_ceiling,
+                 "frac_of_divide_ceiling": (achieved / divide_ceiling) if divide_ceiling else None,
+                 "traffic": traffic, "flops_per_launch": flops_per_launch,
+                 "ms_per_launch": per_launch_s * 1e3}
+
+
 def serve_stream(M, count, rate, budget):
     from mr3smp_amd import workloads as WL
     reqs = WL.served_stream(count, rate)
@@ -136,6 +223,115 @@ def serve_stream(M, count, rate, budget):
                 wall_s=dt)
 
 
+def cpu_baseline_subset(mats, stride, threads):
+    """The compiled reference (oracle/_ref) on every stride-th matrix of the subset
+    workload, one matrix at a time with PMR_NUM_THREADS=threads (its task-parallel
+    route); the oracle instead, when the reference is not built, as `threads`
+    concurrent callers (it is sequential)."""
+    from concurrent.futures import ThreadPoolExecutor
+    from mrrr_ctypes import ORACLE_SO, REF_SO, call_mrrr, load
+    if os.path.exists(REF_SO):
+        _, fn = load(REF_SO, "mrrr")
+        kind = "reference"
+    elif os.path.exists(ORACLE_SO):
+        _, fn = load(ORACLE_SO, "oracle_mrrr")
+        kind = "port"
+    else:
+        return None
+    os.environ["PMR_NUM_THREADS"] = str(threads)
+    sample = mats[::stride]
+
+    def one(t):
+        D, E, il, iu = t
+        r = call_mrrr(fn, "V", "I", D, E, il=il, iu=iu, mcols=iu - il + 1)
+        assert r["info"] == 0 and r["m"] == iu - il + 1
+        return r["m"]
+
+    t0 = time.perf_counter()
+    if kind == "reference":
+        pairs = sum(one(t) for t in sample)
+        how = f"one matrix at a time with PMR_NUM_THREADS={threads}"
+    else:
+        with ThreadPoolExecutor(max_workers=threads) as ex:
+            pairs = sum(ex.map(one, sample))
+        how = f"{threads} concurrent callers"
+    dt = time.perf_counter() - t0
+    return dict(value=pairs / dt, unit="eigenpairs/s", cores=threads, kind=kind,
+                sample=f"every {stride}th matrix of the subset workload ({len(sample)} matrices, "
+                       f"{pairs} eigenpairs, jobz=V range=I), {how}, wall {dt:.2f} s")
+
+
+def subset_main(args, torch, M, dev):
+    """The subset workload: N(0,1) tridiagonals with n log-uniform in [4096, 50000],
+    jobz V, range I over a seeded window of ceil(0.1 n) indices, all in one batch."""
+    from mr3smp_amd import workloads as WL
+    from mrrr_ctypes import resid_orth_device
+    if args.autotune:
+        sys.exit("--autotune searches on the mixed workload only")
+    tuning_src = "library defaults"
+    if args.tuning_file and os.path.exists(args.tuning_file):
+        from mr3smp_amd import tune
+        tune.apply(args.tuning_file)
+        tuning_src = os.path.relpath(args.tuning_file, REPO)
+    tuning = dict(M.get_tuning(), source=tuning_src)
+    log("building workload")
+    mats = WL.batched_range_i(args.subset_count)
+    probs = [dict(D=torch.tensor(D, device=dev), E=torch.tensor(E, device=dev), jobz="V",
+                  range="I", il=il, iu=iu, mcols=iu - il + 1) for D, E, il, iu in mats]
+    pairs = sum(iu - il + 1 for _, _, il, iu in mats)
+    stream = torch.cuda.Stream()
+    arr, keep = M.prepare_batch(probs)
+    dt, prof, counted = timed_steps(M, torch, arr, stream, args.steps, args.warmup, pairs)
+    value = pairs * args.steps / dt
+    plain, nplain = plain_path(M, torch, probs, stream, args.plain_stride)
+    worst_r, worst_o = 0.0, 0.0
+    for i in range(0, len(mats), 16):
+        m = arr[i].m
+        r, o = resid_orth_device(probs[i]["D"], probs[i]["E"], keep[i]["W"][:m], keep[i]["Z"][:m])
+        worst_r, worst_o = max(worst_r, r), max(worst_o, o)
+    dom, roof = roofline(prof, counted, args.steps, traffic_key="batched_subset")
+    kern = prof["kernels"]
+    z_bytes = 8.0 * sum(len(D) * (iu - il + 1) for D, _, il, iu in mats)
+    norm_ms = kern.get("k_s_normalize", {}).get("ms", 0.0) / args.steps
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline_subset(mats, args.cpu_stride or 32, min(16, os.cpu_count() or 1))
+        log(f"cpu baseline: {cpu}")
+    out = {
+        "metric": "eigenpairs/sec batched vs plain on 1 GPU; p50/p99 latency; orth/resid <= ref",
+        "value": value,
+        "unit": "eigenpairs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": dt / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded), inputs and outputs resident in HBM",
+        "config": {"workload": f"batched subset: {len(mats)} matrices, n log-uniform [4096,50000], "
+                               f"D, E ~ N(0,1), jobz=V range=I over a window of ceil(0.1 n) indices",
+                   "matrices": len(mats), "eigenpairs_per_step": pairs,
+                   "z_bytes": z_bytes},
+        "plain_value": plain,
+        "batched_over_plain": value / plain,
+        "plain_sample": f"every {args.plain_stride}th matrix ({nplain}), one solve per matrix",
+        "accuracy_sample": {"max_resid": worst_r, "max_orth": worst_o, "matrices": len(range(0, len(mats), 16)),
+                            "units": "n*eps*||T|| and n*eps"},
+        "roofline": roof,
+        "z_writeback": {"bytes_per_step": z_bytes,
+                        "GBps_over_normalize_kernel": (z_bytes / (norm_ms / 1e3) / 1e9) if norm_ms else None,
+                        "peak_GBps": HBM_PEAK_GBS},
+        "kernels_ms_per_step": {k: v["ms"] / args.steps for k, v in sorted(kern.items(), key=lambda kv: -kv[1]["ms"])},
+        "launches_per_step": {k: v["launches"] / args.steps for k, v in kern.items()},
+        "sweep_elements_per_step": dict(counted["steps"]),
+        "tuning": tuning,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -143,7 +339,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--count", type=int, default=4096)
     ap.add_argument("--plain-stride", type=int, default=8)
-    ap.add_argument("--cpu-stride", type=int, default=16)
+    ap.add_argument("--cpu-stride", type=int, default=None,
+                    help="the CPU baseline runs on every k-th matrix (default 16 on the mixed "
+                         "workload, 32 on the subset workload)")
     ap.add_argument("--serve-requests", type=int, default=48)
     ap.add_argument("--serve-rate", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -153,6 +351,11 @@ def main():
                          "(mr3smp_amd/tune.py) and write them to --tuning-file")
     ap.add_argument("--tuning-file", default=os.path.join(REPO, "mr3smp_amd", "tuning_mi355x.json"),
                     help="launch-shape parameters applied before timing, if the file exists")
+    ap.add_argument("--workload", choices=["mixed", "subset"], default="mixed",
+                    help="mixed: the batched mixed workload (the headline line); subset: 256 "
+                         "N(0,1) tridiagonals, n log-uniform in [4096, 50000], jobz V, range I "
+                         "over a 10%% window, in one batch")
+    ap.add_argument("--subset-count", type=int, default=256)
     ap.add_argument("--copy-calib", type=int, default=0,
                     help="only run a device-to-device copy of this many MiB (five times): the "
                          "known-size kernel that scales the HBM byte counters")
@@ -183,6 +386,9 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"copy_calib_bytes_each_way": nel * 8, "copies": 5}))
         return
+    if args.workload == "subset":
+        subset_main(args, torch, M, dev)
+        return
     log("building workload")
     mats, probs = build_workload(torch, dev, args.count)
     pairs = sum(len(D) for D, _, _ in mats)
@@ -203,75 +409,16 @@ def main():
         tune.apply(args.tuning_file)
         tuning_src = os.path.relpath(args.tuning_file, REPO)
     tuning = dict(M.get_tuning(), source=tuning_src)
-    for _ in range(args.warmup):
-        rc = M.run_batch(arr, stream=stream)
-        assert rc == 0
-    # timed steps: every launch bracketed by HIP events on its own stream (timing only,
-    # the kernels run exactly as unprofiled)
-    M.profile_reset()
-    M.profile_enable(True, counters=False)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        rc = M.run_batch(arr, stream=stream)
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
-    M.profile_enable(False)
-    assert rc == 0 and sum(a.m for a in arr) == pairs
+    dt, prof, counted = timed_steps(M, torch, arr, stream, args.steps, args.warmup, pairs)
     value = pairs * args.steps / dt
-    log(f"batched: {dt / args.steps * 1e3:.1f} ms/step, {value:.4g} eigenpairs/s")
-    prof = M.profile_report()
     prof_steps = args.steps
-    # one more step with the device work counters on: the algorithmic work of a step
-    # (the counts do not depend on timing, so every step has the same)
-    M.profile_reset()
-    M.profile_enable(True)
-    rc = M.run_batch(arr, stream=stream)
-    torch.cuda.synchronize()
-    M.profile_enable(False)
-    assert rc == 0
-    counted = M.profile_report()
-    M.profile_reset()
-
-    # plain path: one matrix per solve, same device-resident buffers
-    sample = list(range(0, len(probs), args.plain_stride))
-    singles = [M.prepare_batch([probs[i]])[0] for i in sample]
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    ppairs = 0
-    for one in singles:
-        rcp = M.run_batch(one, stream=stream)
-        ppairs += one[0].m
-    torch.cuda.synchronize()
-    dtp = time.perf_counter() - t0
-    plain = ppairs / dtp
-    log(f"plain: {plain:.4g} eigenpairs/s on {len(sample)} matrices")
+    plain, nplain = plain_path(M, torch, probs, stream, args.plain_stride)
 
     worst_r, worst_o = accuracy_sample(mats, keep, arr, 64)
 
-    # roofline of the dominant kernel: its algorithmic flops per step (counted step)
-    # over its launches per step, against its average launch time in the timed steps
+    dom, roof = roofline(prof, counted, prof_steps, traffic_key="batched_mixed")
     kern = prof["kernels"]
-    dom = max(kern, key=lambda k: kern[k]["ms"])
     steps = counted["steps"]
-    launches_per_step = kern[dom]["launches"] / prof_steps
-    assert counted["kernels"][dom]["launches"] == launches_per_step, "launch count differs between steps"
-    flops = sum(steps.get(f, 0) * FLOPS_PER_STEP[f] for f in KERNEL_STEPS.get(dom, []))
-    per_launch_s = kern[dom]["ms"] / 1e3 / kern[dom]["launches"]
-    flops_per_launch = flops / launches_per_step
-    achieved = flops_per_launch / per_launch_s / 1e12 if per_launch_s > 0 else 0.0
-    # the same kernel's ceiling when every SIMD issues division steps back to back
-    fams = KERNEL_STEPS.get(dom, [])
-    fl = sum(FLOPS_PER_STEP[f] * steps.get(f, 0) for f in fams)
-    dv = sum(DIVIDES_PER_STEP[f] * steps.get(f, 0) for f in fams)
-    divide_ceiling = DIVIDE_STEPS_PER_S * fl / dv / 1e12 if dv else None
-    traffic = None
-    tfile = os.path.join(REPO, "profiles", "traffic.json")
-    if os.path.exists(tfile):
-        try:
-            traffic = json.load(open(tfile)).get(dom)
-        except Exception:
-            traffic = None
     total_flops = sum(steps.get(f, 0) * FLOPS_PER_STEP[f] for f in FLOPS_PER_STEP) * prof_steps
     gpu_ms = sum(v["ms"] for v in kern.values())
     z_bytes = 8.0 * sum(len(D) ** 2 for D, _, _ in mats) * prof_steps
@@ -287,7 +434,7 @@ def main():
             serve = {"error": repr(e)}
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(mats, args.cpu_stride, min(16, os.cpu_count() or 1))
+        cpu = cpu_baseline(mats, args.cpu_stride or 16, min(16, os.cpu_count() or 1))
         log(f"cpu baseline: {cpu}")
 
     out = {
@@ -313,13 +460,7 @@ def main():
         "serve": serve,
         "accuracy_sample": {"max_resid": worst_r, "max_orth": worst_o,
                             "units": "n*eps*||T|| and n*eps"},
-        "roofline": {"bound": "mfma", "engine": "fp64 vector ALU (no MFMA on this path)",
-                     "kernel": dom, "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
-                     "divide_ceiling": divide_ceiling,
-                     "frac_of_divide_ceiling": (achieved / divide_ceiling) if divide_ceiling else None,
-                     "traffic": traffic, "flops_per_launch": flops_per_launch,
-                     "ms_per_launch": per_launch_s * 1e3},
+        "roofline": roof,
         "z_writeback": {"bytes_per_step": z_bytes / prof_steps,
                         "GBps_over_normalize_kernel": (z_bytes / (norm_ms / 1e3) / 1e9) if norm_ms else None,
                         "peak_GBps": HBM_PEAK_GBS},

@@ -190,18 +190,64 @@ __device__ void small_case(const MatDesc& d, MatState& S, const double* D, const
     S.status = kOk;
 }
 
-// dlaebz IJOB=3 for one of the two target counts (mrrr_val.c:295-319); the two
-// intervals of the reference never interact, so each runs on its own lane.
-__device__ void ijob3_one(int n, const double* d, const double* e2, double pivmin,
-                          double abstol, double reltol, int nitmax, int nval, double c,
-                          double* a1, double* a2, int* n1, int* n2) {
-    for (int it = 0; it < nitmax; ++it) {
-        int cnt = count_T_ijob2(n, d, e2, c, pivmin);
-        if (cnt <= nval) { *a1 = c; *n1 = cnt; }
-        if (cnt >= nval) { *a2 = c; *n2 = cnt; }
-        double tmp1 = fabs(*a2 - *a1), tmp2 = fmax(fabs(*a2), fabs(*a1));
-        if (tmp1 < fmax(fmax(abstol, pivmin), reltol * tmp2) || *n1 >= *n2) break;
-        c = 0.5 * (*a1 + *a2);
+// dlaebz IJOB=3 for the two target counts il - 1 and iu (mrrr_val.c:295-319, the
+// loop of odebz.f:380-536 with one interval).  Each target has half of the workgroup
+// (the two intervals never interact).  A bisection point depends only on the interval
+// and the interval only on the decisions before it, so the points of the next kL3
+// steps of every possible path form a binary tree below the current point (node t:
+// root 1, child 2t when the count exceeds the target, 2t + 1 when it is below).  The
+// half evaluates all of them at once, one count per lane, then its first lane walks
+// the path the sequential loop takes, with its updates and its stopping test step by
+// step.  Points, counts and results are the sequential loop's bit for bit, with about
+// a seventh of its sweeps one after the other.
+constexpr int PREP_NT = 256;   // threads of k_prep
+constexpr int kL3 = 7;
+struct Ijob3 {
+    double a1, a2, c;
+    int n1, n2, it, done;
+};
+
+__device__ void ijob3_pair(int n, const double* d, const double* e2, double pivmin,
+                           double abstol, double reltol, int nitmax, const int nval[2],
+                           Ijob3* st, int (*cnt)[1 << kL3]) {
+    const int half = threadIdx.x / (PREP_NT / 2), g = threadIdx.x % (PREP_NT / 2);
+    static_assert(PREP_NT / 2 >= (1 << kL3) - 1, "one lane per tree node");
+    for (;;) {
+        __syncthreads();
+        if (st[0].done && st[1].done) break;
+        Ijob3& q = st[half];
+        const int t = g + 1;
+        if (!q.done && t < (1 << kL3)) {
+            const int dt = 31 - __clz(t);
+            if (q.it + dt < nitmax) {
+                double a1 = q.a1, a2 = q.a2, c = q.c;
+                for (int b = dt - 1; b >= 0; --b) {
+                    if ((t >> b) & 1) a1 = c;
+                    else a2 = c;
+                    c = 0.5 * (a1 + a2);
+                }
+                cnt[half][t] = count_T_ijob2_w(n, d, e2, c, pivmin);
+            }
+        }
+        __syncthreads();
+        if (g == 0 && !q.done) {
+            const int nv = nval[half];
+            double a1 = q.a1, a2 = q.a2, c = q.c;
+            int n1 = q.n1, n2 = q.n2, it = q.it, done = 0, t = 1;
+            for (int lv = 0; lv < kL3; ++lv) {
+                if (it >= nitmax) { done = 1; break; }
+                const int k = cnt[half][t];
+                if (k <= nv) { a1 = c; n1 = k; }
+                if (k >= nv) { a2 = c; n2 = k; }
+                ++it;
+                const double tmp1 = fabs(a2 - a1), tmp2 = fmax(fabs(a2), fabs(a1));
+                if (tmp1 < fmax(fmax(abstol, pivmin), reltol * tmp2) || n1 >= n2) { done = 1; break; }
+                c = 0.5 * (a1 + a2);
+                t = 2 * t + (k < nv ? 1 : 0);
+            }
+            if (it >= nitmax) done = 1;
+            q.a1 = a1; q.a2 = a2; q.c = c; q.n1 = n1; q.n2 = n2; q.it = it; q.done = done;
+        }
     }
 }
 
@@ -209,8 +255,6 @@ __device__ void ijob3_one(int n, const double* d, const double* e2, double pivmi
 // k_prep: scaling, rel-acc test, Gershgorin, pivmin, splitting, counts and the
 // per-block index ranges of one matrix (mrrr.c:155-243, mrrr_val.c:110-438).
 // ---------------------------------------------------------------------------------
-constexpr int PREP_NT = 256;
-
 __global__ __launch_bounds__(PREP_NT) void k_prep(const MatDesc* __restrict__ md,
                                                   MatState* __restrict__ ms, double* D, double* E,
                                                   double* E2, double* gers, double* Dcopy,
@@ -219,8 +263,7 @@ __global__ __launch_bounds__(PREP_NT) void k_prep(const MatDesc* __restrict__ md
                                                   int* iblock, int* Windex, int* smat) {
     __shared__ double shd[PREP_NT];
     __shared__ int shi[PREP_NT];
-    __shared__ double sh_wl, sh_wu, sh_wlu, sh_wul;
-    __shared__ int sh_nwl, sh_nwu, sh_c0, sh_c1;
+    __shared__ int sh_c0, sh_c1;
     const int k = blockIdx.x, tid = threadIdx.x;
     const MatDesc dsc = md[k];
     MatState& S = ms[k];
@@ -404,17 +447,18 @@ __global__ __launch_bounds__(PREP_NT) void k_prep(const MatDesc* __restrict__ md
     if (irange == kInd) {
         const int itmax = (int)((log(tnorm + pivmin) - log(pivmin)) / log(2.0) + 2);
         const double atoli = 2 * (2 * DBL_MIN + 2 * pivmin);
+        __shared__ Ijob3 st3[2];
+        __shared__ int cnt3[2][1 << kL3];
+        const int nval[2] = {il - 1, iu};
         if (tid < 2) {
-            double a1 = gl, a2 = gu;
-            int n1 = -1, n2 = n + 1;
-            ijob3_one(n, d, e2, pivmin, atoli, rqtol, itmax, tid == 0 ? il - 1 : iu,
-                      tid == 0 ? gl : gu, &a1, &a2, &n1, &n2);
-            if (tid == 0) { sh_wl = a1; sh_wlu = a2; sh_nwl = n1; }
-            else { sh_wu = a2; sh_wul = a1; sh_nwu = n2; }
+            // both start from [gl, gu] with counts -1, n + 1; the first point is the
+            // interval's own end (gl for il - 1, gu for iu)
+            st3[tid].a1 = gl; st3[tid].a2 = gu; st3[tid].c = tid == 0 ? gl : gu;
+            st3[tid].n1 = -1; st3[tid].n2 = n + 1; st3[tid].it = 0; st3[tid].done = itmax <= 0;
         }
-        __syncthreads();
-        wl = sh_wl; wlu = sh_wlu; nwl = sh_nwl;
-        wu = sh_wu; wul = sh_wul; nwu = sh_nwu;
+        ijob3_pair(n, d, e2, pivmin, atoli, rqtol, itmax, nval, st3, cnt3);
+        wl = st3[0].a1; wlu = st3[0].a2; nwl = st3[0].n1;
+        wu = st3[1].a2; wul = st3[1].a1; nwu = st3[1].n2;
         vl = wl;
         vu = wu;
     } else if (irange == kVal) {
@@ -544,8 +588,17 @@ __global__ __launch_bounds__(PREP_NT) void k_prep(const MatDesc* __restrict__ md
 // below it.  Each lane's sequence of intervals, counts and tests is the one of its own
 // bisection, so the results are the same bit for bit.
 // ---------------------------------------------------------------------------------
-__device__ __forceinline__ int coarse_tree_depth(int nbl) {
-    return nbl >= 128 ? (31 - __clz(nbl)) - 1 : 0;
+//
+// The table's depth is log2(nbl) - 1 when the whole block is wanted: each lane saves
+// that many sweeps for about nbl / 2 counts taken once.  When only mbl eigenvalues of
+// the block are wanted (range I or V), the table stops at log2(mbl) - 1 levels, or at 8
+// (one count per lane of the workgroup) if that is more: its counts run one after the
+// other on the workgroup, so levels that few lanes use would lengthen the critical path.
+__device__ __forceinline__ int coarse_tree_depth(int nbl, int mbl) {
+    if (nbl < 128) return 0;
+    const int full = (31 - __clz(nbl)) - 1;
+    const int want = mbl > 1 ? (31 - __clz(mbl)) - 1 : 0;
+    return min(full, max(8, want));
 }
 
 // Initial interval of a block's coarse bisection: dlarrd called on the block alone
@@ -577,7 +630,7 @@ __global__ __launch_bounds__(256) void k_coarse_tree(const MatDesc* __restrict__
     const BlockRec br = blk[gb];
     const int nbl = br.bend - br.bbeg + 1;
     if (br.wend < br.wbeg) return;
-    const int depth = coarse_tree_depth(nbl);
+    const int depth = coarse_tree_depth(nbl, br.wend - br.wbeg + 1);
     if (depth == 0) return;
     const double pivmin = S.pivmin;
     const double* d = D + off + br.bbeg;
@@ -630,7 +683,7 @@ __global__ __launch_bounds__(256) void k_coarse(const MatDesc* __restrict__ md,
     const double atoli = 2.0 * 2.0 * kSafmin + 2.0 * 2.0 * pivmin;
     const double rtoli = S.bsrtol;
     const int itmax = (int)((log(right - left + pivmin) - log(pivmin)) / log(2.0)) + 2;
-    const int depth = coarse_tree_depth(nbl);
+    const int depth = coarse_tree_depth(nbl, br.wend - br.wbeg + 1);
     const int* tc = tcount + off + br.bbeg - 1;   // tc[t]: count at tree node t
     int nl = 0, nr = nbl, sweeps = 0, t = 1;
     for (int it = 0; it < itmax; ++it) {
@@ -756,34 +809,50 @@ __global__ __launch_bounds__(64) void k_root(const MatDesc* __restrict__ md, Mat
     double* dl = DLr + off + br.bbeg;
     double* dll = DLLr + off + br.bbeg;
     const double pivmin = S.pivmin;
-    __shared__ double sh_sigma, sh_spdiam;
-    __shared__ int sh_fail;
+    __shared__ double sh_sigma, sh_spdiam, sh_tau, sh_dmax;
+    __shared__ int sh_fail, sh_sgndef;
+    // rows of the trial factorisation staged through LDS (d, e in; D, L out), so that the
+    // serial loop neither waits on global loads nor reads back its own global stores
+    constexpr int RC = 512;
+    __shared__ double sd[RC], se[RC], sdp[RC], sll[RC];
     // gaps of the coarse values (mrrr_val.c:431-435)
     for (int s = wbgn + lane; s <= wend; s += 64) {
         if (s < wend) Wgap[s] = fmax(0.0, (W[s + 1] - Werr[s + 1]) - (W[s] + Werr[s]));
         else Wgap[s] = fmax(0.0, S.wu - (W[s] + Werr[s]));
     }
     __syncthreads();
+    const double isleft = fmax(br.gl, (W[wbgn] - Werr[wbgn]) - 100.0 * kEps * fabs(W[wbgn] - Werr[wbgn]));
+    const double isright = fmin(br.gu, (W[wend] + Werr[wend]) + 100.0 * kEps * fabs(W[wend] - Werr[wend]));
+    int lcnt = 0, rcnt = 0;
+    if (mbl > 1) {
+        // dlarrc 'T' (odrrc.f:95-113) at the two quarter points, rows staged through LDS
+        const double tmp0 = fmin(isright, S.vu) - fmax(isleft, S.vl);
+        const double s1 = fmax(isleft, S.vl) + 0.25 * tmp0;
+        const double s2 = fmin(isright, S.vu) - 0.25 * tmp0;
+        double lp = d[0] - s1, rp = d[0] - s2;
+        lcnt = (lp <= 0.0); rcnt = (rp <= 0.0);
+        for (int base = 0; base < nbl - 1; base += RC) {
+            const int cnt = min(RC, nbl - 1 - base);
+            __syncthreads();
+            for (int j = lane; j < cnt; j += 64) {
+                sd[j] = d[base + j + 1];
+                se[j] = e[base + j];
+            }
+            __syncthreads();
+            if (lane == 0) {
+                for (int j = 0; j < cnt; ++j) {
+                    const double t = se[j] * se[j];
+                    lp = (sd[j] - s1) - t / lp;
+                    rp = (sd[j] - s2) - t / rp;
+                    lcnt += (lp <= 0.0); rcnt += (rp <= 0.0);
+                }
+            }
+        }
+    }
     if (lane == 0) {
         const double gl = br.gl, gu = br.gu, vl = S.vl, vu = S.vu;
         const int ilow = br.ilow, iupp = br.iupp;
-        double isleft = fmax(gl, (W[wbgn] - Werr[wbgn]) - 100.0 * kEps * fabs(W[wbgn] - Werr[wbgn]));
-        double isright = fmin(gu, (W[wend] + Werr[wend]) + 100.0 * kEps * fabs(W[wend] - Werr[wend]));
         double spdiam = gu - gl;
-        double tmp0 = fmin(isright, vu) - fmax(isleft, vl);
-        double s1 = fmax(isleft, vl) + 0.25 * tmp0;
-        double s2 = fmin(isright, vu) - 0.25 * tmp0;
-        int lcnt = 0, rcnt = 0;
-        if (mbl > 1) { // dlarrc 'T' (odrrc.f:95-113)
-            double lp = d[0] - s1, rp = d[0] - s2;
-            lcnt = (lp <= 0.0); rcnt = (rp <= 0.0);
-            for (int i = 0; i < nbl - 1; ++i) {
-                double t = e[i] * e[i];
-                lp = (d[i + 1] - s1) - t / lp;
-                rp = (d[i + 1] - s2) - t / rp;
-                lcnt += (lp <= 0.0); rcnt += (rp <= 0.0);
-            }
-        }
         double sigma, tau;
         int sgndef;
         if (mbl == 1) { sigma = gl; sgndef = 1; }
@@ -795,23 +864,53 @@ __global__ __launch_bounds__(64) void k_root(const MatDesc* __restrict__ md, Mat
             if (sgndef > 0) { tau = fmax(Wgap[wbgn], avgap); tau *= 0.5; tau = fmax(tau, Werr[wbgn]); }
             else { tau = fmax(Wgap[wend - 1], avgap); tau *= 0.5; tau = fmax(tau, Werr[wend]); }
         } else tau = Werr[wbgn];
+        sh_sigma = sigma;
+        sh_tau = tau;
+        sh_sgndef = sgndef;
+        sh_spdiam = spdiam;
+    }
+    __syncthreads();
+    {
+        // the tries at sigma (every lane holds the same sigma, tau and decisions)
+        double sigma = sh_sigma, tau = sh_tau;
+        const double spdiam = sh_spdiam;
+        const int sgndef = sh_sgndef;
         int ok = 0;
         for (int jtry = 0; jtry < kMaxTryRoot; ++jtry) {
+            // D of the trial factorisation in dl, L in dll
             double dpivot = d[0] - sigma;
-            dl[0] = dpivot;                 // D of the trial factorisation
             double dmax = fabs(dpivot);
-            for (int i = 0; i < nbl - 1; ++i) {
-                double invd = 1.0 / dl[i];
-                double t = e[i] * invd;
-                dll[i] = t;                 // L of the trial factorisation
-                dpivot = (d[i + 1] - sigma) - t * e[i];
-                dl[i + 1] = dpivot;
-                dmax = fmax(dmax, fabs(dpivot));
+            if (lane == 0) dl[0] = dpivot;
+            for (int base = 0; base < nbl - 1; base += RC) {
+                const int cnt = min(RC, nbl - 1 - base);
+                for (int j = lane; j < cnt; j += 64) {
+                    sd[j] = d[base + j + 1];
+                    se[j] = e[base + j];
+                }
+                __syncthreads();
+                if (lane == 0) {
+                    for (int j = 0; j < cnt; ++j) {
+                        const double invd = 1.0 / dpivot;
+                        const double t = se[j] * invd;
+                        sll[j] = t;
+                        dpivot = (sd[j] - sigma) - t * se[j];
+                        sdp[j] = dpivot;
+                        dmax = fmax(dmax, fabs(dpivot));
+                    }
+                }
+                __syncthreads();
+                for (int j = lane; j < cnt; j += 64) {
+                    dll[base + j] = sll[j];
+                    dl[base + j + 1] = sdp[j];
+                }
             }
+            if (lane == 0) sh_dmax = dmax;
+            __syncthreads();
+            dmax = sh_dmax;
             if (dmax > kMaxGrowth * spdiam) {
                 if (jtry == kMaxTryRoot - 2) {
-                    if (sgndef == 1) sigma = gl - kFudgeRoot * spdiam * kEps * nbl - kFudgeRoot * 2.0 * pivmin;
-                    else sigma = gu + kFudgeRoot * spdiam * kEps * nbl + kFudgeRoot * 2.0 * pivmin;
+                    if (sgndef == 1) sigma = br.gl - kFudgeRoot * spdiam * kEps * nbl - kFudgeRoot * 2.0 * pivmin;
+                    else sigma = br.gu + kFudgeRoot * spdiam * kEps * nbl + kFudgeRoot * 2.0 * pivmin;
                 } else if (jtry == kMaxTryRoot - 1) {
                     break;
                 } else {
@@ -820,9 +919,10 @@ __global__ __launch_bounds__(64) void k_root(const MatDesc* __restrict__ md, Mat
                 }
             } else { ok = 1; break; }
         }
-        sh_fail = !ok;
-        sh_sigma = sigma;
-        sh_spdiam = spdiam;
+        if (lane == 0) {
+            sh_fail = !ok;
+            sh_sigma = sigma;
+        }
     }
     __syncthreads();
     if (sh_fail) {

@@ -87,3 +87,47 @@ def resid_orth(D, E, W, Z):
     G[np.diag_indices_from(G)] -= 1.0
     orth = float(np.abs(G).max()) / (n * eps) if Z.size else 0.0
     return resid, orth
+
+
+def resid_orth_device(D, E, W, Z, chunk=512):
+    """resid and orth as in resid_orth, computed on the device: D, E, W float64 tensors,
+    Z of shape (m, n) with eigenvector j in row j (the batch API's layout)."""
+    import torch
+    n, m = D.numel(), W.numel()
+    if m == 0 or n == 0:
+        return 0.0, 0.0
+    eps = np.finfo(np.float64).eps
+    e = E[: n - 1]
+    t1 = D.abs().clone()
+    if n > 1:
+        t1[:-1] += e.abs()
+        t1[1:] += e.abs()
+    tn = max(float(t1.max()), np.finfo(np.float64).tiny)
+    rmax = 0.0
+    for a in range(0, m, chunk):
+        Zc = Z[a:a + chunk]
+        R = Zc * D[None, :] - Zc * W[a:a + chunk, None]
+        if n > 1:
+            R[:, :-1] += e[None, :] * Zc[:, 1:]
+            R[:, 1:] += e[None, :] * Zc[:, :-1]
+        rmax = max(rmax, float(R.abs().max()))
+    G = torch.mm(Z, Z.t().contiguous())
+    G.diagonal().sub_(1.0)
+    return rmax / (n * eps * tn), float(G.abs().max()) / (n * eps)
+
+
+def support_ok_device(Z, Zsupp, chunk=512):
+    """Zsupp (1-based, inclusive, two per row of Z) brackets the nonzeros of every
+    eigenvector; Z of shape (m, n) on the device."""
+    import torch
+    m, n = Z.shape
+    lo = Zsupp[0:2 * m:2].long()
+    hi = Zsupp[1:2 * m:2].long()
+    if not bool(((lo >= 1) & (lo <= hi) & (hi <= n)).all()):
+        return False
+    idx = torch.arange(n, device=Z.device)
+    for a in range(0, m, chunk):
+        out = (idx[None, :] < lo[a:a + chunk, None] - 1) | (idx[None, :] >= hi[a:a + chunk, None])
+        if bool(((Z[a:a + chunk] != 0) & out).any()):
+            return False
+    return True

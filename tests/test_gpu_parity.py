@@ -219,6 +219,83 @@ def test_gpu_subset_large(oracle, M):
     assert np.all(lo <= idx) and np.all(hi >= idx + 1)
 
 
+def test_gpu_subset_batch(oracle, oracle_lib, M):
+    """The batched subset workload at full size in one batch: 256 N(0,1) tridiagonals,
+    n log-uniform in [4096, 50000], jobz V, range I over a seeded window of ceil(0.1 n)
+    indices (about 4.7e5 eigenpairs, 100 GB of Z in HBM).  Every matrix: m exact,
+    W ascending, Sturm counts at W -+ 4 n eps ||T|| bracket the index of sampled
+    eigenvalues and of both ends of the window, sampled eigenvalues within
+    4 n eps ||T|| of the compiled reference's, resid/orth over all columns no worse
+    than 2 max(reference, 1) (the reference's figures are stored in
+    tests/golden/subset_ref.npz), supports.  The three smallest matrices also against
+    the oracle, every output."""
+    import ctypes
+    from concurrent.futures import ThreadPoolExecutor
+
+    import torch
+    from mr3smp_amd import workloads as WL
+    from mrrr_ctypes import resid_orth_device, support_ok_device
+
+    dev = torch.device("cuda:0")
+    mats = WL.batched_range_i(256)
+    probs = [dict(D=torch.tensor(D, device=dev), E=torch.tensor(E, device=dev), jobz="V",
+                  range="I", il=il, iu=iu, mcols=iu - il + 1) for D, E, il, iu in mats]
+    rc, arr, keep = M.solve_batch(probs)
+    torch.cuda.synchronize()
+    assert rc == 0
+    count = oracle_lib.oracle_sturm_count_T
+    count.restype = ctypes.c_int
+    count.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double,
+                      ctypes.c_double]
+    ref = np.load(os.path.join(os.path.dirname(__file__), "golden", "subset_ref.npz"))
+    Ws = []
+    for i, (D, E, il, iu) in enumerate(mats):
+        m = arr[i].m
+        assert (ref["n"][i], ref["il"][i], ref["iu"][i]) == (len(D), il, iu), "workload differs"
+        assert arr[i].info == 0 and m == iu - il + 1 == ref["m"][i], f"matrix {i}: m {m}"
+        W = keep[i]["W"][:m].cpu().numpy()
+        assert np.all(np.diff(W) >= 0), f"matrix {i}: W not ascending"
+        js = sorted(set([0, m - 1] + list(range(0, m, max(1, m // 32)))))
+        dw = np.abs(W[js] - ref["W"][ref["Woff"][i]:ref["Woff"][i + 1]]).max()
+        assert dw <= w_tol(D, E), f"matrix {i}: |W - W_ref| {dw:.3g}"
+        Ws.append(W)
+
+    def counts_ok(i):
+        D, E, il, iu = mats[i]
+        n, W = len(D), Ws[i]
+        e2 = np.ascontiguousarray(E[: n - 1] ** 2)
+        d = np.ascontiguousarray(D)
+        pivmin = np.finfo(np.float64).tiny * max(1.0, float(e2.max()))
+        tol = w_tol(D, E)
+        js = sorted(set([0, len(W) - 1] + list(range(0, len(W), max(1, len(W) // 32)))))
+        for j in js:
+            lo = count(n, d.ctypes.data, e2.ctypes.data, float(W[j] - tol), pivmin)
+            hi = count(n, d.ctypes.data, e2.ctypes.data, float(W[j] + tol), pivmin)
+            if not (lo <= il - 1 + j < hi):
+                return f"matrix {i}: eigenvalue {il + j} at {W[j]!r} has counts {lo}, {hi}"
+        return None
+
+    with ThreadPoolExecutor(max_workers=16) as ex:
+        bad = [r for r in ex.map(counts_ok, range(len(mats))) if r]
+    assert not bad, bad[:4]
+    for i in range(len(mats)):
+        m = arr[i].m
+        Z = keep[i]["Z"][:m]
+        r, o = resid_orth_device(probs[i]["D"], probs[i]["E"], keep[i]["W"][:m], Z)
+        assert r <= 2 * max(ref["resid"][i], 1.0), f"matrix {i}: resid {r:.3g} vs {ref['resid'][i]:.3g}"
+        assert o <= 2 * max(ref["orth"][i], 1.0), f"matrix {i}: orth {o:.3g} vs {ref['orth'][i]:.3g}"
+        assert support_ok_device(Z, keep[i]["Zsupp"]), f"matrix {i}: nonzero outside support"
+    for i in sorted(range(len(mats)), key=lambda k: len(mats[k][0]))[:3]:
+        D, E, il, iu = mats[i]
+        m = arr[i].m
+        g = dict(info=arr[i].info, m=m, W=Ws[i], Z=keep[i]["Z"][:m].cpu().numpy().T,
+                 Zsupp=keep[i]["Zsupp"][:2 * m].cpu().numpy())
+        o = call_mrrr(oracle[1], "V", "I", D, E, il=il, iu=iu, mcols=iu - il + 1)
+        check_pair(D, E, g, o, f"subset matrix {i} (n={len(D)})")
+    del probs, keep
+    torch.cuda.empty_cache()
+
+
 def test_gpu_batch_equals_single(M):
     """A problem gives the same bits alone and inside a mixed batch."""
     import torch
