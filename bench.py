@@ -200,10 +200,10 @@ def roofline(prof, counted, prof_steps, traffic_key):
                  "ms_per_launch": per_launch_s * 1e3}
 
 
-def serve_stream(M, count, rate, budget):
+def serve_stream(M, count, rate, budget, inflight=3, batch=64):
     from mr3smp_amd import workloads as WL
     reqs = WL.served_stream(count, rate)
-    srv = M.Server(budget_bytes=budget, max_inflight=3, max_batch=64)
+    srv = M.Server(budget_bytes=budget, max_inflight=inflight, max_batch=batch)
     t0 = time.perf_counter()
     ids = []
     for r in reqs:
@@ -221,6 +221,23 @@ def serve_stream(M, count, rate, budget):
     return dict(requests=count, rate_per_s=rate, p50_ms=float(np.percentile(lat, 50) * 1e3),
                 p99_ms=float(np.percentile(lat, 99) * 1e3), eigenpairs_per_s=pairs / dt,
                 wall_s=dt)
+
+
+def serve_plain(M, count):
+    """The served stream's requests solved one after the other through the drop-in
+    mrrr() (host arrays in and out), for a like-for-like plain rate."""
+    from mr3smp_amd import workloads as WL
+    reqs = WL.served_stream(count, 1e9)
+    t0 = time.perf_counter()
+    pairs = 0
+    for r in reqs:
+        kw = {k: r[k] for k in ("vl", "vu", "il", "iu") if k in r}
+        mc = (r["iu"] - r["il"] + 1) if r["range"] == "I" else None
+        o = M.mrrr(r["jobz"], r["range"], r["D"], r["E"], mcols=mc, **kw)
+        assert o["info"] == 0
+        pairs += o["m"]
+    dt = time.perf_counter() - t0
+    return dict(eigenpairs_per_s=pairs / dt, wall_s=dt)
 
 
 def cpu_baseline_subset(mats, stride, threads):
@@ -342,8 +359,13 @@ def main():
     ap.add_argument("--cpu-stride", type=int, default=None,
                     help="the CPU baseline runs on every k-th matrix (default 16 on the mixed "
                          "workload, 32 on the subset workload)")
-    ap.add_argument("--serve-requests", type=int, default=48)
-    ap.add_argument("--serve-rate", type=float, default=20.0)
+    ap.add_argument("--serve-requests", type=int, default=192)
+    ap.add_argument("--serve-rate", type=float, default=None,
+                    help="arrival rate of the served stream in requests/s (default: 0.8 x the "
+                         "capacity measured with every request arriving at once)")
+    ap.add_argument("--serve-inflight", type=int, default=8,
+                    help="server workers (concurrent batches)")
+    ap.add_argument("--serve-batch", type=int, default=16, help="most requests per batch")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--autotune", action="store_true",
@@ -428,7 +450,25 @@ def main():
     serve = None
     if not args.no_serve:
         try:
-            serve = serve_stream(M, args.serve_requests, args.serve_rate, 200 << 30)
+            cap = None
+            rate = args.serve_rate
+            if rate is None:
+                # capacity: the same requests, all arriving at once
+                cap = serve_stream(M, args.serve_requests, 1e9, 200 << 30,
+                                   args.serve_inflight, args.serve_batch)
+                rate = 0.8 * args.serve_requests / cap["wall_s"]
+                log(f"serve capacity: {cap}")
+            serve = serve_stream(M, args.serve_requests, rate, 200 << 30, args.serve_inflight,
+                                 args.serve_batch)
+            serve["server"] = {"max_inflight": args.serve_inflight, "max_batch": args.serve_batch}
+            if cap is not None:
+                serve["capacity"] = {"requests_per_s": args.serve_requests / cap["wall_s"],
+                                     "eigenpairs_per_s": cap["eigenpairs_per_s"],
+                                     "p50_ms": cap["p50_ms"], "p99_ms": cap["p99_ms"]}
+                serve["rate_rule"] = "0.8 x capacity"
+                plain_s = serve_plain(M, args.serve_requests)
+                serve["plain_same_requests"] = plain_s
+                serve["capacity_over_plain"] = cap["eigenpairs_per_s"] / plain_s["eigenpairs_per_s"]
             log(f"serve: {serve}")
         except Exception as e:  # reported, not fatal for the headline number
             serve = {"error": repr(e)}

@@ -151,7 +151,8 @@ def mrrr(jobz, rng, D, E, vl=0.0, vu=0.0, il=1, iu=1, tryrac=1, ldz=None, mcols=
     mm = m.value
     out = dict(info=info, m=mm, W=W[:mm].copy(), Zsupp=Zs[:2 * mm].copy(), vl=cvl.value,
                vu=cvu.value, il=cil.value, iu=ciu.value, tryrac=ctr.value)
-    out["Z"] = Z[:mm, :n].T.copy() if wantz else None
+    # n x m view of the column-major output (column j is eigenvector j), no copy
+    out["Z"] = Z[:mm, :n].T if wantz else None
     return out
 
 
@@ -420,7 +421,7 @@ class Server:
         info = self.L.mr3_server_wait(self.h, rid)
         req, D, E, W, Z, Zs = self._keep.pop(rid)
         m = req.m
-        return dict(info=info, m=m, W=W[:m].copy(), Z=None if Z is None else Z[:m].T.copy(),
+        return dict(info=info, m=m, W=W[:m].copy(), Z=None if Z is None else Z[:m].T,
                     Zsupp=None if Zs is None else Zs[:2 * m].copy(),
                     t_submit=req.t_submit, t_start=req.t_start, t_done=req.t_done)
 
