@@ -53,7 +53,7 @@ def call_mrrr(fn, jobz, rng, D, E, vl=0.0, vu=0.0, il=1, iu=1, tryrac=1,
     mm = m.value
     out = dict(info=info, m=mm, W=W[:mm].copy(), Zsupp=Zs[:2 * mm].copy(),
                vl=cvl.value, vu=cvu.value, il=cil.value, iu=ciu.value, tryrac=tr.value)
-    out["Z"] = Z[:mm].T.copy() if Z is not None else None
+    out["Z"] = Z[:mm].T if Z is not None else None   # n x m view, no copy
     return out
 
 
