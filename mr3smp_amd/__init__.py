@@ -183,10 +183,12 @@ def unpack(AP, n, uplo):
 def _dense_standard(name, cplx, packed, jobz, rng, uplo, A, n, vl, vu, il, iu):
     L = lib()
     dt = np.complex128 if cplx else np.float64
+    # the standard drivers read A (or AP) and do not write it: no copy unless the layout
+    # or type needs one
     if packed:
-        Ah = np.ascontiguousarray(A, dtype=dt).copy()
+        Ah = np.ascontiguousarray(A, dtype=dt)
     else:
-        Ah = np.asfortranarray(np.asarray(A, dtype=dt)).copy(order="F")
+        Ah = np.asfortranarray(np.asarray(A, dtype=dt))
         n = int(Ah.shape[0])
     wantz = jobz[0].upper() == "V"
     mcols = (iu - il + 1) if rng[0].upper() == "I" else n
@@ -200,7 +202,7 @@ def _dense_standard(name, cplx, packed, jobz, rng, uplo, A, n, vl, vu, il, iu):
     info = getattr(L, name)(*head, ctypes.byref(cvl), ctypes.byref(cvu), ctypes.byref(cil),
                             ctypes.byref(ciu), ctypes.byref(m), _ptr(W), _ptr(Z), ctypes.byref(ldz))
     mm = m.value
-    return dict(info=info, m=mm, W=W[:mm].copy(), Z=Z[:, :mm].copy() if wantz else None,
+    return dict(info=info, m=mm, W=W[:mm].copy(), Z=Z[:, :mm] if wantz else None,
                 vl=cvl.value, vu=cvu.value, il=cil.value, iu=ciu.value)
 
 

@@ -236,10 +236,14 @@ def test_packed_drivers_equal_full_storage(M, uplo):
         print(f"packed standard cplx={cplx} uplo={uplo}", flush=True)
         for kw in (dict(jobz="V", rng="A"), dict(jobz="V", rng="I", il=30, iu=90),
                    dict(jobz="N", rng="A")):
-            a = full(kw["jobz"], kw["rng"], uplo, only_triangle(A, uplo),
-                     il=kw.get("il", 1), iu=kw.get("iu", 1))
-            b = packed(kw["jobz"], kw["rng"], uplo, M.pack(A, uplo), n,
-                       il=kw.get("il", 1), iu=kw.get("iu", 1))
+            Af = np.asfortranarray(only_triangle(A, uplo))
+            AP = M.pack(A, uplo)
+            Af0, AP0 = Af.copy(), AP.copy()
+            a = full(kw["jobz"], kw["rng"], uplo, Af, il=kw.get("il", 1), iu=kw.get("iu", 1))
+            b = packed(kw["jobz"], kw["rng"], uplo, AP, n, il=kw.get("il", 1), iu=kw.get("iu", 1))
+            # the standard drivers leave A and AP as they were (the wrappers pass them
+            # without a copy)
+            assert np.array_equal(Af, Af0, equal_nan=True) and np.array_equal(AP, AP0, equal_nan=True), kw
             assert a["info"] == 0 and b["info"] == 0 and a["m"] == b["m"], kw
             assert np.array_equal(a["W"], b["W"]), kw
             if kw["jobz"] == "V":
