@@ -3049,9 +3049,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
 
 // dlarrb (odrrb.f:135-299) for one eigenvalue by a group of G lanes (G a power of
 // two, 4 or more; the group is lanes gbase .. gbase + G - 1 of the wave and every lane
-// of it passes the same arguments).  The bracket expansion evaluates G consecutive
-// candidates of the reference's sequence (left - back, back doubled) per sweep and
-// keeps the first that brackets; each bisection round evaluates the G - 1 midpoints
+// of it passes the same arguments).  The bracket expansion runs both ends at once: each
+// half of the group evaluates G/2 consecutive candidates of the reference's sequence
+// for its end (left - back or right + back, back doubled) per sweep and keeps the first
+// that brackets; each bisection round evaluates the G - 1 midpoints
 // of the next log2(G) halvings of the interval (a binary tree, computed with the
 // reference's operations) in one sweep, then walks the tree with the reference's
 // convergence test.  The walk takes the same path and the same midpoints as one-point
@@ -3066,48 +3067,49 @@ __device__ __forceinline__ long long dlarrb_grp(int G, int sub, int gbase, bool 
                                                 int* rounds_out) {
     const int maxitr = (int)((log(spdiam + pivmin) - log(pivmin)) / log(2.0)) + 2;
     const double mnwdth = 2.0 * pivmin;
-    long long sweeps = 0;   // one-point sweeps of the reference (algorithmic work)
     int rounds = 0;
-    // bracket expansion to the left and to the right (odrrb.f:176-196)
-    double left = w - werr, right = w + werr;
-    for (int side = 0; side < 2; ++side) {
-        double x = side == 0 ? left : right, back = werr;
-        int done = 0, g0 = 0;
-        while (__any(valid && !done)) {
-            // candidate sub of this batch: sub more expansion steps from x
-            double xs = x, bs = back;
-            for (int k = 0; k < sub; ++k) {
-                if (side == 0) xs = xs - bs;
-                else xs = xs + bs;
-                bs = 2.0 * bs;
-            }
-            const int c = count(xs);
-            ++rounds;
-            const bool brk = side == 0 ? !(c > i - 1) : !(c < i);
-            const bool ok = g0 + sub < kMaxExpand;   // the reference stops after kMaxExpand checks
-            const unsigned long long m = __ballot(brk || !ok);
-            const unsigned gm = (unsigned)((m >> gbase) & ((G == 64) ? ~0ULL : ((1ULL << G) - 1)));
-            if (!done) {
-                if (gm) {
-                    const int k = __ffs(gm) - 1;   // first candidate that brackets (or the cap)
-                    sweeps += min(k + 1, kMaxExpand - g0);
-                    x = __shfl(xs, gbase + k);
-                    done = 1;
-                } else {
-                    sweeps += G;
-                    x = __shfl(xs, gbase + G - 1);
-                    back = __shfl(bs, gbase + G - 1);
-                    // advance one more step past the last candidate
-                    if (side == 0) x = x - back;
-                    else x = x + back;
-                    back = 2.0 * back;
-                    g0 += G;
-                }
+    // bracket expansion to the left and to the right (odrrb.f:176-196), both in the same
+    // sweeps: the lower half of the group walks the left sequence, the upper half the
+    // right one, H candidates per sweep each
+    const int H = G / 2;
+    const int side = sub / H, hs = sub % H, hbase = gbase + side * H;
+    double x = side == 0 ? w - werr : w + werr, back = werr;
+    long long ssweeps = 0;   // this side's one-point sweeps of the reference
+    int done = 0, g0 = 0;
+    while (__any(valid && !done)) {
+        // candidate hs of this batch: hs more expansion steps from x
+        double xs = x, bs = back;
+        for (int k = 0; k < hs; ++k) {
+            if (side == 0) xs = xs - bs;
+            else xs = xs + bs;
+            bs = 2.0 * bs;
+        }
+        const int c = count(xs);
+        ++rounds;
+        const bool brk = side == 0 ? !(c > i - 1) : !(c < i);
+        const bool ok = g0 + hs < kMaxExpand;   // the reference stops after kMaxExpand checks
+        const unsigned long long m = __ballot(brk || !ok);
+        const unsigned long long hm = (m >> hbase) & ((1ULL << H) - 1ULL);
+        if (!done) {   // uniform over the half
+            if (hm) {
+                const int k = __ffsll((long long)hm) - 1;   // first candidate that brackets (or the cap)
+                ssweeps += min(k + 1, kMaxExpand - g0);
+                x = __shfl(xs, hbase + k);
+                done = 1;
+            } else {
+                ssweeps += H;
+                x = __shfl(xs, hbase + H - 1);
+                back = __shfl(bs, hbase + H - 1);
+                // advance one more step past the last candidate
+                if (side == 0) x = x - back;
+                else x = x + back;
+                back = 2.0 * back;
+                g0 += H;
             }
         }
-        if (side == 0) left = x;
-        else right = x;
     }
+    double left = __shfl(x, gbase), right = __shfl(x, gbase + H);
+    long long sweeps = __shfl(ssweeps, gbase) + __shfl(ssweeps, gbase + H);
     double width = 0.5 * fabs(left - right);
     double tmp = fmax(fabs(left), fabs(right));
     double cvrgd = fmax(rtol1 * gap, rtol2 * tmp);
