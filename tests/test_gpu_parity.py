@@ -296,6 +296,55 @@ def test_gpu_subset_batch(oracle, oracle_lib, M):
     torch.cuda.empty_cache()
 
 
+def test_gpu_range_v_large_batch(oracle_lib, M):
+    """Range V at n = 30000 (N(0,1)): three windows of the same matrix in one batch,
+    one of them holding the lower end of the spectrum.  m equals the Sturm-count
+    difference at vu and vl and the compiled reference's m; W lies in (vl, vu],
+    ascending, and within 4 n eps ||T|| of the reference's at sampled positions; resid
+    and orth over all columns no worse than 2 max(reference, 1)
+    (tests/golden/range_v_ref.npz); supports."""
+    import ctypes
+
+    import torch
+    from mrrr_ctypes import resid_orth_device, support_ok_device
+
+    from mr3smp_amd import workloads as WL
+    ref = np.load(os.path.join(os.path.dirname(__file__), "golden", "range_v_ref.npz"))
+    D, E = WL.normal(30000, WL.rng_for(6))
+    n = len(D)
+    dev = torch.device("cuda:0")
+    Dt, Et = torch.tensor(D, device=dev), torch.tensor(E, device=dev)
+    probs = [dict(D=Dt, E=Et, jobz="V", range="V", vl=float(vl), vu=float(vu))
+             for vl, vu in zip(ref["vl"], ref["vu"])]
+    rc, arr, keep = M.solve_batch(probs)
+    torch.cuda.synchronize()
+    assert rc == 0
+    count = oracle_lib.oracle_sturm_count_T
+    count.restype = ctypes.c_int
+    count.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double,
+                      ctypes.c_double]
+    e2 = np.ascontiguousarray(E[: n - 1] ** 2)
+    d = np.ascontiguousarray(D)
+    pivmin = np.finfo(np.float64).tiny * max(1.0, float(e2.max()))
+    tol = w_tol(D, E)
+    for k, (vl, vu) in enumerate(zip(ref["vl"], ref["vu"])):
+        m = arr[k].m
+        cm = count(n, d.ctypes.data, e2.ctypes.data, float(vu), pivmin) - \
+            count(n, d.ctypes.data, e2.ctypes.data, float(vl), pivmin)
+        assert arr[k].info == 0 and m == ref["m"][k] == cm, (k, m, ref["m"][k], cm)
+        W = keep[k]["W"][:m].cpu().numpy()
+        assert np.all(np.diff(W) >= 0) and W[0] > vl and W[-1] <= vu, k
+        js = sorted(set([0, m - 1] + list(range(0, m, max(1, m // 32)))))
+        dw = np.abs(W[js] - ref["W"][ref["Woff"][k]:ref["Woff"][k + 1]]).max()
+        assert dw <= tol, (k, dw)
+        Z = keep[k]["Z"][:m]
+        r, o = resid_orth_device(Dt, Et, keep[k]["W"][:m], Z)
+        assert r <= 2 * max(ref["resid"][k], 1.0) and o <= 2 * max(ref["orth"][k], 1.0), (k, r, o)
+        assert support_ok_device(Z, keep[k]["Zsupp"]), k
+    del probs, keep
+    torch.cuda.empty_cache()
+
+
 def test_gpu_batch_equals_single(M):
     """A problem gives the same bits alone and inside a mixed batch."""
     import torch
