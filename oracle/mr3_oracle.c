@@ -11,6 +11,7 @@
  * reference's x86-64 build.
  */
 #include "mr3_oracle.h"
+#include "mr3_oracle_int.h"
 
 #include <float.h>
 #include <math.h>
@@ -700,7 +701,7 @@ static int o_dlaebz(int ijob, int nitmax, int n, int mmax, int minp,
  * odrrd.f:241-722 (range I bracket :343-393, block loop :415-574, discard
  * :576-690).  gers holds 2n Gershgorin bounds.  Returns INFO.
  * ---------------------------------------------------------------------------- */
-static int o_dlarrd(char range, int n, double vl, double vu, int il, int iu,
+int o_dlarrd(char range, int n, double vl, double vu, int il, int iu,
                     const double *gers, double reltol, const double *d,
                     const double *e2, double pivmin, int nsplit,
                     const int *isplit, int *m, double *w, double *werr,
@@ -904,7 +905,7 @@ static int o_dlarrd(char range, int n, double vl, double vu, int il, int iu,
 }
 
 /* dlarrc with JOBT = 'T': counts on T at vl and vu, odrrc.f:95-113 */
-static void o_dlarrc_T(int n, double vl, double vu, const double *d,
+void o_dlarrc_T(int n, double vl, double vu, const double *d,
                        const double *e, int *lcnt, int *rcnt)
 {
     double lpivot = d[0] - vl, rpivot = d[0] - vu;
@@ -921,7 +922,7 @@ static void o_dlarrc_T(int n, double vl, double vu, const double *d,
 }
 
 /* dlarra: splitting points, odrra.f:90-123 (isplit 1-based ends) */
-static void o_dlarra(int n, const double *d, double *e, double *e2, double spltol,
+void o_dlarra(int n, const double *d, double *e, double *e2, double spltol,
                      double tnrm, int *nsplit, int *isplit)
 {
     *nsplit = 1;
@@ -943,7 +944,7 @@ static void o_dlarra(int n, const double *d, double *e, double *e2, double splto
 }
 
 /* dlarrr: test for relative accuracy, odrrr.f:60-121 (0 = yes) */
-static int o_dlarrr(int n, const double *d, const double *e)
+int o_dlarrr(int n, const double *d, const double *e)
 {
     const double RELCOND = 0.999;
     double smlnum = SAFMIN / EPS, rmin = sqrt(smlnum);
@@ -962,7 +963,7 @@ static int o_dlarrr(int n, const double *d, const double *e)
 }
 
 /* dlarrj: bisection on T to relative accuracy RTOL, odrrj.f:100-281 */
-static void o_dlarrj(int n, const double *d, const double *e2, int ifirst,
+void o_dlarrj(int n, const double *d, const double *e2, int ifirst,
                      int ilast, double rtol, int offset, double *w,
                      double *werr, double pivmin, double spdiam)
 {
@@ -1060,7 +1061,7 @@ static void o_dlarrj(int n, const double *d, const double *e2, int ifirst,
 }
 
 /* dlaev2 (odev2.f:70-170): eigen-decomposition of [[a,b],[b,c]] */
-static void o_dlaev2(double a, double b, double c, double *rt1, double *rt2,
+void o_dlaev2(double a, double b, double c, double *rt1, double *rt2,
                      double *cs1, double *sn1)
 {
     double sm = a + c, df = a - c, adf = fabs(df), tb = b + b, ab = fabs(tb);
@@ -1831,47 +1832,6 @@ static int cmpb(const void *a1, const void *a2)
 }
 
 /* dstemr's n = 1 and n = 2 cases, odstmr.f:357-430 */
-static int o_small_n(int wantz, int alleig, int valeig, int indeig, int n,
-                     const double *D, const double *E, double wl, double wu,
-                     int iil, int iiu, int *m, double *W, double *Z, int ldz,
-                     int *Zsupp)
-{
-    *m = 0;
-    if (n == 1) {
-        if (alleig || indeig) { *m = 1; W[0] = D[0]; }
-        else if (wl < D[0] && wu >= D[0]) { *m = 1; W[0] = D[0]; }
-        if (wantz) { Z[0] = 1.0; Zsupp[0] = 1; Zsupp[1] = 1; }
-        return 0;
-    }
-    double r1, r2, cs = 0, sn = 0;
-    o_dlaev2(D[0], E[0], D[1], &r1, &r2, &cs, &sn);
-    if (alleig || (valeig && r2 > wl && r2 <= wu) || (indeig && iil == 1)) {
-        (*m)++;
-        W[*m - 1] = r2;
-        if (wantz) {
-            Z[(*m - 1) * (size_t)ldz] = -sn;
-            Z[(*m - 1) * (size_t)ldz + 1] = cs;
-            if (sn != 0.0) {
-                if (cs != 0.0) { Zsupp[2 * *m - 2] = 1; Zsupp[2 * *m - 1] = 2; }
-                else { Zsupp[2 * *m - 2] = 1; Zsupp[2 * *m - 1] = 1; }
-            } else { Zsupp[2 * *m - 2] = 2; Zsupp[2 * *m - 1] = 2; }
-        }
-    }
-    if (alleig || (valeig && r1 > wl && r1 <= wu) || (indeig && iiu == 2)) {
-        (*m)++;
-        W[*m - 1] = r1;
-        if (wantz) {
-            Z[(*m - 1) * (size_t)ldz] = cs;
-            Z[(*m - 1) * (size_t)ldz + 1] = sn;
-            if (sn != 0.0) {
-                if (cs != 0.0) { Zsupp[2 * *m - 2] = 1; Zsupp[2 * *m - 1] = 2; }
-                else { Zsupp[2 * *m - 2] = 1; Zsupp[2 * *m - 1] = 1; }
-            } else { Zsupp[2 * *m - 2] = 2; Zsupp[2 * *m - 1] = 2; }
-        }
-    }
-    return 0;
-}
-
 /* ==============================================================================
  * mrrr(): mrrr.c:87-296 (with the scaling of :325-386, rel-acc refinement of
  * :452-520 and the eigenpair sort of :525-588).
@@ -1896,9 +1856,19 @@ int oracle_mrrr(char *jobz, char *range, int *np, double *D, double *E,
         if (alleig) { *mp = n; return 0; }
         if (indeig) { *mp = *iup - *ilp + 1; return 0; }
     }
-    if (n <= 2 && !cntval) /* dstemr's closed forms, reached via mrrr.c:149-153 */
-        return o_small_n(wantZ, alleig, valeig, indeig, n, D, E, *vlp, *vup, *ilp,
-                         *iup, mp, W, Z, *ldz, Zsupp);
+    if (n < MR3_DSTEMR_BELOW) { /* handle_small_cases, mrrr.c:149-153, :394-441 */
+        if (cntval) {
+            double cnt = 0.0;
+            int ld1 = *ldz > 0 ? *ldz : 1;
+            int rc = oracle_dstemr('V', 'V', n, D, E, *vlp, *vup, *ilp, *iup, mp, W, &cnt,
+                                   ld1 < n ? n : ld1, -1, Zsupp, tryracp);
+            *mp = (int)cnt;
+            return rc == 0 ? 0 : 2;
+        }
+        int rc = oracle_dstemr(wantZ ? 'V' : 'N', alleig ? 'A' : (valeig ? 'V' : 'I'), n, D, E,
+                               *vlp, *vup, *ilp, *iup, mp, W, Z, *ldz, n, Zsupp, tryracp);
+        return rc == 0 ? 0 : 2;
+    }
 
     char jz = wantZ ? 'V' : (onlyW ? 'N' : 'C');
     char rg = alleig ? 'A' : (valeig ? 'V' : 'I');

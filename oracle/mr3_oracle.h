@@ -8,15 +8,16 @@
  * bench.py's cpu_baseline leg as the checker, never by the product path
  * (mr3smp_amd/ never links, loads or calls it).
  *
- * Scope of the restatement:
- *   - the task-parallel mrrr() path (mrrr.c:155-296) run on one thread, with root
- *     eigenvalues by bisection (FORCE_BISECTION, mrrr.h:160-162; the dqds branch
- *     mrrr_val.c:464-507 is not restated);
- *   - n = 1 and n = 2 follow dstemr's closed forms (odstmr.f:357-430), which is
- *     what the reference's mrrr() does for those sizes (mrrr.c:149-153);
- *   - for 3 <= n < 500 the reference routes to LAPACK dstemr (mrrr.c:149-153);
- *     this oracle keeps the task-parallel algorithm for every n >= 3, so for those
- *     sizes it agrees with the reference to tolerance, not bit for bit.
+ * Scope of the restatement (the routing of mrrr.c:141-153 with PMR_NUM_THREADS >= 13,
+ * so that the task-parallel route takes its bisection branch for range A):
+ *   - n < 500: LAPACK dstemr as vendored by the reference (mr3_oracle_dstemr.c:
+ *     odstmr.f, odrre.f with dqds odsq2-6.f, odrrk.f, odrrv.f), bit for bit,
+ *     n = 1, 2 closed forms included;
+ *   - n >= 500: the task-parallel mrrr() path (mrrr.c:155-296) run on one thread,
+ *     with root eigenvalues by bisection (FORCE_BISECTION, mrrr.h:160-162; the dqds
+ *     branch mrrr_val.c:464-507 is taken by the reference only with <= 12 threads).
+ *     Its task sizes follow one thread; the reference's depend on its thread count
+ *     and timing, so results there agree with the reference to tolerance.
  */
 #ifndef MR3_ORACLE_H
 #define MR3_ORACLE_H
@@ -29,6 +30,17 @@ extern "C" {
 int oracle_mrrr(char *jobz, char *range, int *n, double *D, double *E,
                 double *vl, double *vu, int *il, int *iu, int *tryrac,
                 int *m, double *W, double *Z, int *ldz, int *Zsupp);
+
+/* mrrr() sends every matrix with n below this to dstemr (mrrr.c:149-153;
+ * DSTEMR_IF_SMALLER, mrrr.h:158). */
+#define MR3_DSTEMR_BELOW 500
+
+/* LAPACK dstemr as vendored by the reference (odstmr.f:233-647), for the calls that
+ * mrrr() makes: NZC = n, or NZC = -1 (count query, the count lands in Z[0]).
+ * Arrays 0-based; isuppz 1-based pairs.  Returns dstemr's INFO. */
+int oracle_dstemr(char jobz, char range, int n, double *D, double *E, double vl,
+                  double vu, int il, int iu, int *m, double *W, double *Z, int ldz,
+                  int nzc, int *isuppz, int *tryrac);
 
 /* Kernel-level entry points (1-based index arguments as in the Fortran). */
 int    oracle_dlaneg(int n, const double *d, const double *lld, double sigma,

@@ -57,6 +57,40 @@ def call_mrrr(fn, jobz, rng, D, E, vl=0.0, vu=0.0, il=1, iu=1, tryrac=1,
     return out
 
 
+def call_dstemr(lib, jobz, rng, D, E, vl=0.0, vu=0.0, il=1, iu=1, tryrac=1):
+    """Calls the oracle's restatement of the reference's dstemr (oracle_dstemr) as
+    mrrr() does for its sequential route (NZC = n); same output dict as call_mrrr."""
+    fn = lib.oracle_dstemr
+    fn.restype = ctypes.c_int
+    fn.argtypes = ([ctypes.c_char, ctypes.c_char, ctypes.c_int, _P, _P, ctypes.c_double,
+                    ctypes.c_double, ctypes.c_int, ctypes.c_int, _P, _P, _P, ctypes.c_int,
+                    ctypes.c_int, _P, _P])
+    n = len(D)
+    D = np.ascontiguousarray(D, dtype=np.float64).copy()
+    E = np.ascontiguousarray(E, dtype=np.float64).copy()
+    W = np.zeros(max(n, 1))
+    Z = np.zeros((max(n, 1), n)) if jobz.upper() == "V" else np.zeros(1)
+    Zs = np.zeros(2 * max(n, 1), dtype=np.int32)
+    m = ctypes.c_int(0)
+    tr = ctypes.c_int(tryrac)
+    info = fn(jobz.encode(), rng.encode(), n, _ptr(D), _ptr(E), vl, vu, il, iu,
+              ctypes.byref(m), _ptr(W), _ptr(Z), max(n, 1), n, _ptr(Zs), ctypes.byref(tr))
+    mm = m.value
+    return dict(info=info, m=mm, W=W[:mm].copy(), Zsupp=Zs[:2 * mm].copy(), tryrac=tr.value,
+                Z=Z[:mm].T if jobz.upper() == "V" else None)
+
+
+def same_bits(got, ref, vectors=True):
+    """True when two solver outputs agree bit for bit (m, W and, for jobz V, Z and
+    Zsupp)."""
+    if got["m"] != ref["m"] or not np.array_equal(got["W"], ref["W"]):
+        return False
+    if vectors and got.get("Z") is not None and ref.get("Z") is not None:
+        return (np.array_equal(got["Zsupp"], ref["Zsupp"]) and
+                np.array_equal(got["Z"], ref["Z"]))
+    return True
+
+
 def tnorm(D, E):
     n = len(D)
     T1 = np.abs(D).copy()

@@ -13,8 +13,8 @@ import os
 import numpy as np
 import pytest
 
-from accuracy import DSTEMR_FLOOR, EPS, check_pair, check_support, w_tol
-from mrrr_ctypes import REF_SO, call_mrrr, load, resid_orth, tnorm
+from accuracy import EPS, check_pair, check_support, w_tol
+from mrrr_ctypes import REF_SO, call_dstemr, call_mrrr, load, resid_orth, same_bits, tnorm
 
 P = ctypes.c_void_p
 
@@ -51,9 +51,30 @@ def test_oracle_matches_reference_outputs(name, golden, oracle):
         ref["resid"], ref["orth"] = (float(v) for v in c["resid_orth"])
     if "Z" in c:
         ref["Z"] = c["Z"]
-    dstemr_route = c["threads"] == 1 or len(D) < 500
-    check_pair(D, E, got, ref, name, vectors=(c["jobz"] == "V"),
-               floor=DSTEMR_FLOOR if dstemr_route else 1.0)
+        ref["Zsupp"] = c["Zsupp"]
+    if len(D) < 500:
+        # the reference took dstemr (mrrr.c:149-153); the oracle restates it bit for bit
+        assert got["info"] == info and got["m"] == m, name
+        assert same_bits(got, ref, vectors=("Z" in c)), name
+        return
+    check_pair(D, E, got, ref, name, vectors=(c["jobz"] == "V"))
+
+
+@pytest.mark.parametrize("name", [n for n in golden_names()])
+def test_dstemr_restatement_on_reference_outputs(name, golden, oracle_lib):
+    """Every stored case that the reference solved through dstemr (n < 500, or one
+    thread at any n: mrrr.c:149-153) is reproduced bit for bit by oracle_dstemr."""
+    c = golden[name]
+    D, E = c["D"], c["E"]
+    if not (len(D) < 500 or c["threads"] == 1) or c["jobz"] == "C":
+        pytest.skip("the reference took its task-parallel route")
+    kw = c["kw"]
+    got = call_dstemr(oracle_lib, c["jobz"], c["rng"], D, E, kw["vl"], kw["vu"], kw["il"],
+                      kw["iu"], kw["tryrac"])
+    info, m = (int(v) for v in c["info_m"])
+    assert got["info"] == 0 and got["m"] == m
+    ref = dict(m=m, W=c["W"], Z=c.get("Z"), Zsupp=c.get("Zsupp"))
+    assert same_bits(got, ref, vectors=("Z" in c)), name
 
 
 # ---------------------------------------------------------------------------------
@@ -284,8 +305,11 @@ needs_ref = pytest.mark.skipif(not os.path.exists(REF_SO), reason="oracle/_ref n
 @needs_ref
 @pytest.mark.parametrize("kind,n,threads", [("uniform", 150, 1), ("uniform", 700, 16),
                                             ("toeplitz", 400, 1), ("glued", 630, 16),
-                                            ("normal", 520, 4)])
-def test_oracle_vs_compiled_reference(kind, n, threads, oracle):
+                                            ("normal", 520, 16), ("uniform", 640, 1)])
+def test_oracle_vs_compiled_reference(kind, n, threads, oracle, oracle_lib):
+    """The dstemr route (n < 500, or one thread) bit for bit; the task-parallel route
+    (>= 13 threads: bisection for the root eigenvalues, mrrr_val.c:281-282) to the
+    parity bars."""
     from mr3smp_amd import workloads as WL
     rng = np.random.default_rng(n)
     if kind == "uniform":
@@ -302,11 +326,55 @@ def test_oracle_vs_compiled_reference(kind, n, threads, oracle):
     try:
         for jobz, rg, kw in (("V", "A", {}), ("N", "A", {}), ("V", "I", dict(il=n // 3, iu=n // 3 + n // 10))):
             r = call_mrrr(ref, jobz, rg, D, E, **kw)
-            g = call_mrrr(oracle[1], jobz, rg, D, E, **kw)
-            check_pair(D, E, g, r, f"{kind}{n}-{jobz}{rg}", vectors=(jobz == "V"),
-                       floor=DSTEMR_FLOOR if (threads == 1 or n < 500) else 1.0)
+            label = f"{kind}{n}-{jobz}{rg}"
+            if n < 500:
+                g = call_mrrr(oracle[1], jobz, rg, D, E, **kw)
+                assert g["info"] == r["info"] == 0 and same_bits(g, r, jobz == "V"), label
+            elif threads == 1:
+                g = call_dstemr(oracle_lib, jobz, rg, D, E, il=kw.get("il", 1), iu=kw.get("iu", 1))
+                assert g["info"] == r["info"] == 0 and same_bits(g, r, jobz == "V"), label
+            else:
+                g = call_mrrr(oracle[1], jobz, rg, D, E, **kw)
+                check_pair(D, E, g, r, label, vectors=(jobz == "V"))
     finally:
         if old is None:
             os.environ.pop("PMR_NUM_THREADS", None)
         else:
+            os.environ["PMR_NUM_THREADS"] = old
+
+
+# matrices of the batched mixed workload (n < 500) that exercise the dstemr route's
+# clusters: glued W21+ copies with tight pairs (37.6-42.9 n eps orthogonality in the
+# reference's task-parallel algorithm, 1.9-3.9 in dstemr), uniform and 1-2-1 ones
+MIXED_SMALL = [937, 3716, 1668, 3883, 0, 1, 2, 3, 5, 8, 13, 21, 34, 55]
+
+
+@needs_ref
+@pytest.mark.parametrize("idx", MIXED_SMALL)
+def test_dstemr_route_bit_exact_on_mixed_workload(idx, oracle):
+    from mr3smp_amd import workloads as WL
+    D, E, kind = WL.batched_mixed(4096)[idx]
+    n = len(D)
+    if n >= 500:
+        pytest.skip("n >= 500")
+    _, ref = load(REF_SO, "mrrr")
+    old = os.environ.pop("PMR_NUM_THREADS", None)
+    try:
+        Wd = np.linalg.eigvalsh(np.diag(D) + np.diag(E[: n - 1], 1) + np.diag(E[: n - 1], -1))
+        vl, vu = float(Wd[n // 4]) - 1e-3, float(Wd[n // 2]) + 1e-3
+        calls = (("V", "A", {}), ("N", "A", {}), ("V", "I", dict(il=n // 5 + 1, iu=n // 2)),
+                 ("V", "V", dict(vl=vl, vu=vu)), ("N", "V", dict(vl=vl, vu=vu)),
+                 ("V", "A", dict(tryrac=0)))
+        for jobz, rg, kw in calls:
+            r = call_mrrr(ref, jobz, rg, D, E, **kw)
+            g = call_mrrr(oracle[1], jobz, rg, D, E, **kw)
+            assert g["info"] == r["info"] == 0, (idx, jobz, rg)
+            assert same_bits(g, r, jobz == "V"), (idx, kind, jobz, rg)
+            for k in ("vl", "vu", "il", "iu", "tryrac"):
+                assert g[k] == r[k], (idx, jobz, rg, k)
+        c = call_mrrr(oracle[1], "C", "V", D, E, vl=vl, vu=vu)
+        cr = call_mrrr(ref, "C", "V", D, E, vl=vl, vu=vu)
+        assert c["m"] == cr["m"]
+    finally:
+        if old is not None:
             os.environ["PMR_NUM_THREADS"] = old
