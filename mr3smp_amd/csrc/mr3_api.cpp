@@ -18,6 +18,9 @@
 
 namespace mr3 {
 int run_pipeline(std::vector<MatDesc>& md, std::vector<MatState>& ms, hipStream_t st);
+int small_begin(SmallJob& job, const std::vector<MatDesc>& md, hipStream_t st);
+int small_end(SmallJob& job, std::vector<MatState>& ms, hipStream_t st);
+constexpr int kSmallRouteBelow = 500;   // mrrr.c:149-153 (DSTEMR_IF_SMALLER)
 void profile_enable(int on);
 int set_tuning(const mr3_tuning* t);
 void get_tuning(mr3_tuning* t);
@@ -68,9 +71,10 @@ using namespace mr3;
 extern "C" int mr3_solve_batch(mr3_problem* probs, int nprob, void* stream) {
     if (nprob <= 0) return 0;
     hipStream_t st = stream ? (hipStream_t)stream : thread_stream();
-    std::vector<MatDesc> md;
-    std::vector<int> idx;
-    md.reserve(nprob);
+    // n < 500 takes the reference's dstemr route (mrrr.c:149-153), larger n the
+    // task-parallel pipeline; both run on the device at the same time
+    std::vector<MatDesc> mdS, mdL;
+    std::vector<int> idxS, idxL;
     int worst = 0;
     for (int i = 0; i < nprob; ++i) {
         mr3_problem& p = probs[i];
@@ -84,31 +88,34 @@ extern "C" int mr3_solve_batch(mr3_problem* probs, int nprob, void* stream) {
             p.m = (rg == 'A') ? p.n : p.iu - p.il + 1;
             continue;
         }
-        if (jz == 'V' && (p.Z == nullptr || p.Zsupp == nullptr || (p.ldz > 0 && p.ldz < p.n))) {
+        if (jz == 'V' && (p.Z == nullptr || p.Zsupp == nullptr || (p.ldz > 0 && p.ldz < p.n) ||
+                          p.mcols < 1)) {
             p.info = 1; worst = std::max(worst, 1); continue;
         }
-        md.push_back(to_desc(p));
-        idx.push_back(i);
+        if (p.n < kSmallRouteBelow) { mdS.push_back(to_desc(p)); idxS.push_back(i); }
+        else { mdL.push_back(to_desc(p)); idxL.push_back(i); }
     }
-    if (md.empty()) return worst;
-    std::vector<MatState> ms;
-    int rc = run_pipeline(md, ms, st);
-    if (rc != 0) {
-        for (int j : idx) probs[j].info = 2;
-        return 2;
-    }
-    for (size_t j = 0; j < idx.size(); ++j) {
-        mr3_problem& p = probs[idx[j]];
-        const MatState& S = ms[j];
-        p.m = S.m;
-        p.info = (S.status == kOk) ? 0 : 2;
-        p.vl_out = S.vl; p.vu_out = S.vu; p.il_out = S.il; p.iu_out = S.iu;
-        p.tryrac_out = S.tryrac;
-        if (p.jobz == 'V' || p.jobz == 'v') {
-            if (p.m > p.mcols && p.mcols > 0) p.info = 2;
+    if (mdS.empty() && mdL.empty()) return worst;
+    SmallJob job;
+    const int rcS = small_begin(job, mdS, st);
+    std::vector<MatState> msL, msS;
+    const int rcL = mdL.empty() ? 0 : run_pipeline(mdL, msL, st);
+    const int rcE = (rcS == 0) ? small_end(job, msS, st) : rcS;
+    auto fill = [&](const std::vector<int>& idx, const std::vector<MatState>& ms, int rc) {
+        for (size_t j = 0; j < idx.size(); ++j) {
+            mr3_problem& p = probs[idx[j]];
+            if (rc != 0) { p.info = 2; worst = 2; continue; }
+            const MatState& S = ms[j];
+            p.m = S.m;
+            p.info = (S.status == kOk) ? 0 : 2;
+            p.vl_out = S.vl; p.vu_out = S.vu; p.il_out = S.il; p.iu_out = S.iu;
+            p.tryrac_out = S.tryrac;
+            if ((p.jobz == 'V' || p.jobz == 'v') && p.m > p.mcols) p.info = 2;
+            if (p.info) worst = std::max(worst, p.info);
         }
-        if (p.info) worst = std::max(worst, p.info);
-    }
+    };
+    fill(idxS, msS, rcE);
+    fill(idxL, msL, rcL);
     return worst;
 }
 

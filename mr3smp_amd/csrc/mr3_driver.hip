@@ -22,6 +22,7 @@
 #include "mr3_internal.h"
 
 #include "mr3_kernels.hip"
+#include "mr3_small.hip"
 
 namespace mr3 {
 
@@ -735,6 +736,114 @@ int run_pipeline(std::vector<MatDesc>& md, std::vector<MatState>& ms, hipStream_
     MR3_CHECK(hipMemcpyAsync(ms.data(), dms, sizeof(MatState) * nmat, hipMemcpyDeviceToHost, st));
     MR3_CHECK(hipStreamSynchronize(st));
     ce.collect();
+    return 0;
+}
+
+// ---------------------------------------------------------------------------------
+// The small-matrix route (n < 500, the reference's dstemr): one k_small launch on a
+// stream of its own, started before the large problems' pipeline and collected after
+// it, so that both run on the device at the same time.
+// ---------------------------------------------------------------------------------
+static hipStream_t small_stream() {
+    thread_local hipStream_t s = nullptr;
+    thread_local bool tried = false;
+    if (!tried) {
+        tried = true;
+        if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) s = nullptr;
+    }
+    return s;
+}
+
+struct SmallJob::Impl {
+    hipStream_t s = nullptr;
+    char* buf = nullptr;
+    MatState* dms = nullptr;
+    hipEvent_t ready = nullptr;
+    int nmat = 0;
+    std::vector<int> list;
+    std::vector<long long> off;
+    CallEvents ce;
+    ~Impl() {   // a job that failed before small_end
+        if (buf) {
+            (void)hipStreamSynchronize(s);
+            (void)hipFree(buf);
+        }
+        if (ready) event_pool().put(ready);
+    }
+};
+
+SmallJob::SmallJob() : p(new Impl) {}
+SmallJob::~SmallJob() { delete p; }
+
+int small_begin(SmallJob& job, const std::vector<MatDesc>& md, hipStream_t st) {
+    SmallJob::Impl& J = *job.p;
+    const int nm = (int)md.size();
+    J.nmat = nm;
+    if (nm == 0) return 0;
+    J.s = small_stream();
+    if (!J.s) return 2;
+    configure_pool_once();
+    // largest first: the longest solves start first
+    J.list.resize(nm);
+    for (int i = 0; i < nm; ++i) J.list[i] = i;
+    std::stable_sort(J.list.begin(), J.list.end(), [&](int a, int b) { return md[a].n > md[b].n; });
+    J.off.resize(nm);
+    long long tot = 0;
+    int nmax = 1;
+    for (int j = 0; j < nm; ++j) {
+        J.off[j] = tot;
+        tot += small_slice_bytes(md[J.list[j]].n);
+        nmax = std::max(nmax, md[J.list[j]].n);
+    }
+    auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    const size_t bmd = al(sizeof(MatDesc) * nm), bms = al(sizeof(MatState) * nm);
+    const size_t bl = al(sizeof(int) * nm), bo = al(sizeof(long long) * nm);
+    hipEvent_t e0 = event_pool().get();
+    MR3_CHECK(hipEventRecord(e0, st));
+    MR3_CHECK(hipStreamWaitEvent(J.s, e0, 0));
+    MR3_CHECK(hipMallocAsync((void**)&J.buf, bmd + bms + bl + bo + (size_t)tot, J.s));
+    MatDesc* dmd = (MatDesc*)J.buf;
+    J.dms = (MatState*)(J.buf + bmd);
+    int* dlist = (int*)(J.buf + bmd + bms);
+    long long* doff = (long long*)(J.buf + bmd + bms + bl);
+    char* slices = J.buf + bmd + bms + bl + bo;
+    MR3_CHECK(hipMemcpyAsync(dmd, md.data(), sizeof(MatDesc) * nm, hipMemcpyHostToDevice, J.s));
+    MR3_CHECK(hipMemcpyAsync(dlist, J.list.data(), sizeof(int) * nm, hipMemcpyHostToDevice, J.s));
+    MR3_CHECK(hipMemcpyAsync(doff, J.off.data(), sizeof(long long) * nm, hipMemcpyHostToDevice, J.s));
+    J.ce.on = g_prof.on.load() != 0;
+    CallEvents& ce = J.ce;
+    const size_t lds = sizeof(double) * (size_t)mr3sm::lds_doubles(nmax);
+    hipEvent_t e0_ = nullptr, e1_ = nullptr;
+    if (ce.on) {
+        e0_ = event_pool().get();
+        e1_ = event_pool().get();
+        (void)hipEventRecord(e0_, J.s);
+    }
+    hipLaunchKernelGGL(k_small, dim3(nm), dim3(kSmallNT), lds, J.s, dmd, J.dms, dlist, doff, slices);
+    if (ce.on) {
+        (void)hipEventRecord(e1_, J.s);
+        ce.ev.push_back({"k_small", {e0_, e1_}});
+    }
+    MR3_CHECK(hipGetLastError());
+    J.ready = event_pool().get();
+    MR3_CHECK(hipEventRecord(J.ready, J.s));
+    event_pool().put(e0);
+    return 0;
+}
+
+int small_end(SmallJob& job, std::vector<MatState>& ms, hipStream_t st) {
+    SmallJob::Impl& J = *job.p;
+    if (J.nmat == 0) return 0;
+    ms.resize(J.nmat);
+    MR3_CHECK(hipMemcpyAsync(ms.data(), J.dms, sizeof(MatState) * J.nmat, hipMemcpyDeviceToHost, J.s));
+    MR3_CHECK(hipStreamSynchronize(J.s));
+    J.ce.collect();
+    (void)hipFreeAsync(J.buf, J.s);
+    J.buf = nullptr;
+    MR3_CHECK(hipEventRecord(J.ready, J.s));
+    MR3_CHECK(hipStreamWaitEvent(st, J.ready, 0));
+    event_pool().put(J.ready);
+    J.ready = nullptr;
     return 0;
 }
 

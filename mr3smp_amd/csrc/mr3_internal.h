@@ -138,4 +138,16 @@ struct LevelCounters {
     int clsPos[kSizeClasses];
 };
 
+// A started small-matrix solve (the dstemr route, n < 500): small_begin launches it on
+// a stream of its own after the work queued on st, small_end waits for it, copies the
+// per-problem states back and orders st after it.
+struct SmallJob {
+    struct Impl;
+    Impl* p;
+    SmallJob();
+    ~SmallJob();
+    SmallJob(const SmallJob&) = delete;
+    SmallJob& operator=(const SmallJob&) = delete;
+};
+
 }  // namespace mr3

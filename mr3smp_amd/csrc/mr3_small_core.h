@@ -1,0 +1,2465 @@
+// mr3_small_core.h -- the reference's sequential route for small matrices.
+//
+// mrrr() hands every matrix with n < 500 to LAPACK's dstemr (HPAC/mr3smp
+// SRC/mrrr.c:149-153, :394-441; vendored as SRC/LAPACK/odstmr.f): root
+// representations and eigenvalues by dlarre (odrre.f, dqds for range A), eigenvectors
+// by dlarrv (odrrv.f, the representation tree breadth first), relative-accuracy
+// refinement by dlarrj.  This header restates that route for one matrix solved by one
+// team of threads (a wave on the GPU).  The parts that are sequential in the
+// reference (dqds, the tree walk with its gap bookkeeping) run on the team's first
+// thread; the parts whose items never interact (bisection of each eigenvalue in
+// dlarrd, dlarrb and dlarrj, the Sturm counts of the range-I bracket, zeroing and
+// swapping columns) are spread over the team.  Each such item repeats the reference's
+// operations in the reference's order, so the results are the reference's bit for bit.
+//
+// The includer defines SM_HD (function qualifiers) and SM_SYNC() (team barrier); the
+// team is described by (tid, nt).  All arithmetic is binary64 without contraction.
+#pragma once
+#include <float.h>
+#include <math.h>
+
+namespace mr3sm {
+
+constexpr double kEps = DBL_EPSILON;   // odmch.f 'P'
+constexpr double kSafmin = DBL_MIN;    // odmch.f 'S'
+constexpr int kAll = 1, kVal = 2, kInd = 3;
+constexpr int kMaxExpand = 128;        // cap on bracket doublings
+
+SM_HD inline double fmn(double a, double b) { return b < a ? b : a; }
+SM_HD inline double fmx(double a, double b) { return b > a ? b : a; }
+SM_HD inline int imn(int a, int b) { return a < b ? a : b; }
+SM_HD inline int imx(int a, int b) { return a > b ? a : b; }
+
+// ---------------------------------------------------------------------------------
+// Problem and work arrays of one matrix.  Pointers are 0-based; the comments give the
+// reference's 1-based names.
+// ---------------------------------------------------------------------------------
+struct Shared {           // scalars every thread of the team reads
+    int m, info, nsplit, stop;
+    int i0, i1, offset, ibeg, wbeg;
+    double pivmin, rtol1, rtol2, spdiam, wl, wu;
+    double w[4];          // small exchange area
+    double ends[4];       // dlarrk's end eigenvalues of a block (value, error) x 2
+    int iw[8];
+    int lv[8];            // loop control of dlarrv
+    int passes;           // children runs of dlarrv's nodes (first runs included)
+};
+
+struct Prob {
+    int n, wantz, cnt, range, il, iu, tryrac, ldz, mcols;
+    double vl, vu;
+    const double* Din;    // caller's T
+    const double* Ein;
+    double* W;            // caller's outputs
+    double* Z;
+    int* Zsupp;
+};
+
+struct Work {
+    double *D, *E;        // D, E / root L (overwritten as in the reference)
+    double *LD, *LLD;     // D*L, D*L*L of the current representation
+    double *WK;           // 4n+8: dqds array / dlar1v and dlarrf work
+    double *E2, *WERR, *WGAP, *WSH, *GERS, *DCOPY;
+    int *ISPLIT, *IBLOCK, *INDEXW, *TWIST, *CL1, *CL2;
+    // the parallel children of a tree node: each thread's dlar1v / dlarrf work (4n + 8
+    // doubles, interleaved: entry i of thread t at WKL[i * WKS + t]), the node's values
+    // before its children ran (W, WERR, WSH, WGAP), each child's input gap, its bounds
+    // and whether it must run again
+    double* WKL;
+    int WKS;
+    double *SNW, *SNE, *SNS, *SNG, *INP;
+    double* CERT;         // 5 per child: see sng_compute
+    int *CH, *DIRTY;
+};
+
+struct Result {
+    int m, info, tryrac;
+};
+
+// Doubles of LDS and bytes of device memory that carve() hands out for a matrix of
+// order n solved by a team whose dlar1v / dlarrf work is kept for wks threads.
+SM_HD inline long long lds_doubles(int n) { return 8LL * n + 8; }
+SM_HD inline long long slice_bytes(int n, int wks) {
+    const long long b = (17LL * n + (long long)wks * (4LL * n + 8)) * (long long)sizeof(double) +
+                        (11LL * n + 10) * (long long)sizeof(int);
+    return (b + 255) & ~255LL;
+}
+
+// Lays the work arrays of one matrix out in lds (lds_doubles(n) doubles) and base
+// (slice_bytes(n, wks) bytes): every pointer of Work is set here and nowhere else.
+SM_HD inline void carve(Work& w, double* lds, char* base, int n, int wks) {
+    w.D = lds;
+    w.E = w.D + n;
+    w.LD = w.E + n;
+    w.LLD = w.LD + n;
+    w.WK = w.LLD + n;                 // 4n + 8
+    w.E2 = (double*)base;
+    w.WERR = w.E2 + n;
+    w.WGAP = w.WERR + n;
+    w.WSH = w.WGAP + n;
+    w.GERS = w.WSH + n;               // 2n
+    w.DCOPY = w.GERS + 2 * n;
+    w.SNW = w.DCOPY + n;
+    w.SNE = w.SNW + n;
+    w.SNS = w.SNE + n;
+    w.SNG = w.SNS + n;
+    w.INP = w.SNG + n;
+    w.CERT = w.INP + n;               // 5n
+    w.WKS = wks;
+    w.WKL = w.CERT + 5 * n;           // wks (4n + 8)
+    w.ISPLIT = (int*)(w.WKL + (long long)wks * (4 * n + 8));
+    w.IBLOCK = w.ISPLIT + n;
+    w.INDEXW = w.IBLOCK + n;
+    w.TWIST = w.INDEXW + n;
+    w.CL1 = w.TWIST + n;              // 2n + 4
+    w.CL2 = w.CL1 + 2 * n + 4;        // 2n + 4
+    w.CH = w.CL2 + 2 * n + 4;         // 2n + 2
+    w.DIRTY = w.CH + 2 * n + 2;       // n
+}
+
+// ---------------------------------------------------------------------------------
+// Sturm counts
+// ---------------------------------------------------------------------------------
+// dlaneg (odneg.f:97-164): negcount of L D L^T - sigma I with twist r (1-based).  The
+// reference re-runs a block of 128 steps with the NaN-safe form when the block ends in
+// NaN; a NaN quotient persists to the block end, and the re-run differs from the fast
+// loop only at NaN quotients, which it replaces by 1, so replacing per step is the same.
+SM_HD inline int dlaneg(int n, const double* d, const double* lld, double sigma, int r) {
+    int neg = 0;
+    double t = -sigma;
+    for (int j = 0; j < r - 1; ++j) {
+        double dplus = d[j] + t;
+        if (dplus < 0.0) ++neg;
+        double tmp = t / dplus;
+        if (isnan(tmp)) tmp = 1.0;
+        t = tmp * lld[j] - sigma;
+    }
+    double p = d[n - 1] - sigma;
+    for (int j = n - 2; j >= r - 1; --j) {
+        double dminus = lld[j] + p;
+        if (dminus < 0.0) ++neg;
+        double tmp = p / dminus;
+        if (isnan(tmp)) tmp = 1.0;
+        p = tmp * d[j] - sigma;
+    }
+    double gamma = (t + sigma) + p;
+    if (gamma < 0.0) ++neg;
+    return neg;
+}
+
+// Count of eigenvalues of T below x, dlaebz's convention (odebz.f:265-277, :396-415).
+SM_HD inline int count_T(int n, const double* d, const double* e2, double x, double pivmin) {
+    double t = d[0] - x;
+    if (fabs(t) < pivmin) t = -pivmin;
+    int c = (t <= 0.0) ? 1 : 0;
+    for (int j = 1; j < n; ++j) {
+        t = d[j] - e2[j - 1] / t - x;
+        if (fabs(t) < pivmin) t = -pivmin;
+        if (t <= 0.0) ++c;
+    }
+    return c;
+}
+
+// dlaebz IJOB=2/3 step count (odebz.f:396-415): pivot <= pivmin counts, clamp to
+// -pivmin.
+SM_HD inline int count_T2(int n, const double* d, const double* e2, double x, double pivmin) {
+    double t = d[0] - x;
+    int c = 0;
+    if (t <= pivmin) { c = 1; t = fmn(t, -pivmin); }
+    for (int j = 1; j < n; ++j) {
+        t = d[j] - e2[j - 1] / t - x;
+        if (t <= pivmin) { ++c; t = fmn(t, -pivmin); }
+    }
+    return c;
+}
+
+// ---------------------------------------------------------------------------------
+// dlarrb for one eigenvalue (odrrb.f:135-299).  The reference bisects a list of
+// intervals that never interact; gap0 is the gap of its first pass (for the first
+// index of the list only the right gap), gap that of the bisection passes.  w, werr
+// are left as they were when the interval already meets the tolerance.
+// ---------------------------------------------------------------------------------
+SM_HD inline void dlarrb1(int n, const double* d, const double* lld, int i, double rtol1,
+                          double rtol2, double gap0, double gap, double* w, double* werr,
+                          double pivmin, double spdiam, int twist) {
+    const int maxitr = (int)((log(spdiam + pivmin) - log(pivmin)) / log(2.0)) + 2;
+    const double mnwdth = 2.0 * pivmin;
+    int r = twist;
+    if (r < 1 || r > n) r = n;
+    // The reference expands the bracket without a bound (odrrb.f:176-196); a zero error
+    // bound would never move it, so the doublings are capped (2^kMaxExpand spans any
+    // representable range, so the cap never binds otherwise).
+    double left = *w - *werr, right = *w + *werr, back = *werr;
+    for (int g = 0; g < kMaxExpand && dlaneg(n, d, lld, left, r) > i - 1; ++g) {
+        left = left - back;
+        back = 2.0 * back;
+    }
+    back = *werr;
+    for (int g = 0; g < kMaxExpand && dlaneg(n, d, lld, right, r) < i; ++g) {
+        right = right + back;
+        back = 2.0 * back;
+    }
+    double width = 0.5 * fabs(left - right);
+    double tmp = fmx(fabs(left), fabs(right));
+    double cvrgd = fmx(rtol1 * gap0, rtol2 * tmp);
+    if (width <= cvrgd || width <= mnwdth) return;
+    for (int iter = 0;; ++iter) {
+        double mid = 0.5 * (left + right);
+        width = right - mid;
+        tmp = fmx(fabs(left), fabs(right));
+        cvrgd = fmx(rtol1 * gap, rtol2 * tmp);
+        if (width <= cvrgd || width <= mnwdth || iter == maxitr) break;
+        if (dlaneg(n, d, lld, mid, r) <= i - 1) left = mid;
+        else right = mid;
+    }
+    *w = 0.5 * (left + right);
+    *werr = right - *w;
+}
+
+// dlarrb over indices ifirst..ilast (offset as the reference's OFFSET), shared by the
+// team: W, WGAP, WERR point at the block's first eigenvalue (1-based II = I - OFFSET).
+SM_HD inline void dlarrb_team(int tid, int nt, int n, const double* d, const double* lld,
+                              int ifirst, int ilast, double rtol1, double rtol2, int offset,
+                              double* w, double* wgap, double* werr, double pivmin,
+                              double spdiam, int twist) {
+    for (int i = ifirst + tid; i <= ilast; i += nt) {
+        const int ii = i - offset;
+        const double rg = wgap[ii - 1];
+        const double lg = (ii > 1) ? wgap[ii - 2] : rg;
+        const double gap0 = (i == ifirst) ? rg : fmn(lg, rg);
+        dlarrb1(n, d, lld, i, rtol1, rtol2, gap0, fmn(lg, rg), &w[ii - 1], &werr[ii - 1],
+                pivmin, spdiam, twist);
+    }
+    SM_SYNC();
+    for (int i = ifirst + 1 + tid; i <= ilast; i += nt) {
+        const int ii = i - offset;
+        wgap[ii - 2] = fmx(0.0, w[ii - 1] - werr[ii - 1] - w[ii - 2] - werr[ii - 2]);
+    }
+    SM_SYNC();
+}
+
+// dlarrj (odrrj.f:100-281) for one eigenvalue: bisection on T to relative width rtol.
+SM_HD inline void dlarrj1(int n, const double* d, const double* e2, int i, double rtol,
+                          double* w, double* werr, double pivmin, double spdiam) {
+    const int maxitr = (int)((log(spdiam + pivmin) - log(pivmin)) / log(2.0)) + 2;
+    double left = *w - *werr, mid = *w, right = *w + *werr;
+    double width = right - mid, tmp = fmx(fabs(left), fabs(right));
+    if (width < rtol * tmp) return;
+    auto cnt = [&](double s) {
+        int c = 0;
+        double dplus = d[0] - s;
+        if (dplus < 0.0) ++c;
+        for (int j = 1; j < n; ++j) {
+            dplus = d[j] - s - e2[j - 1] / dplus;
+            if (dplus < 0.0) ++c;
+        }
+        return c;
+    };
+    double fac = 1.0;
+    for (int g = 0; g < kMaxExpand && cnt(left) > i - 1; ++g) { left = left - *werr * fac; fac = 2.0 * fac; }
+    fac = 1.0;
+    for (int g = 0; g < kMaxExpand && cnt(right) < i; ++g) { right = right + *werr * fac; fac = 2.0 * fac; }
+    for (int iter = 0;; ++iter) {
+        mid = 0.5 * (left + right);
+        width = right - mid;
+        tmp = fmx(fabs(left), fabs(right));
+        if (width < rtol * tmp || iter == maxitr) break;
+        if (cnt(mid) <= i - 1) left = mid;
+        else right = mid;
+    }
+    *w = 0.5 * (left + right);
+    *werr = right - *w;
+}
+
+// dlarrk (odrrk.f:95-167): eigenvalue iw of T by bisection to relative width reltol.
+SM_HD inline void dlarrk(int n, int iw, double gl, double gu, const double* d, const double* e2,
+                         double pivmin, double reltol, double* w, double* werr) {
+    const double FUDGE = 2.0;
+    double tnorm = fmx(fabs(gl), fabs(gu));
+    double rtoli = reltol, atoli = FUDGE * 2.0 * pivmin;
+    int itmax = (int)((log(tnorm + pivmin) - log(pivmin)) / log(2.0)) + 2;
+    double left = gl - FUDGE * tnorm * kEps * n - FUDGE * 2.0 * pivmin;
+    double right = gu + FUDGE * tnorm * kEps * n + FUDGE * 2.0 * pivmin;
+    for (int it = 0;;) {
+        double tmp1 = fabs(right - left), tmp2 = fmx(fabs(right), fabs(left));
+        if (tmp1 < fmx(fmx(atoli, pivmin), rtoli * tmp2)) break;
+        if (it > itmax) break;
+        ++it;
+        double mid = 0.5 * (left + right);
+        if (count_T(n, d, e2, mid, pivmin) >= iw) right = mid;
+        else left = mid;
+    }
+    *w = 0.5 * (left + right);
+    *werr = 0.5 * fabs(right - left);
+}
+
+// dlarrc with JOBT = 'T' (odrrc.f:95-113): counts of eigenvalues <= vl and <= vu.
+SM_HD inline void dlarrc_T(int n, double vl, double vu, const double* d, const double* e,
+                           int* lcnt, int* rcnt) {
+    double lp = d[0] - vl, rp = d[0] - vu;
+    int lc = (lp <= 0.0) ? 1 : 0, rc = (rp <= 0.0) ? 1 : 0;
+    for (int i = 1; i < n; ++i) {
+        double tmp = e[i - 1] * e[i - 1];
+        lp = (d[i] - vl) - tmp / lp;
+        rp = (d[i] - vu) - tmp / rp;
+        if (lp <= 0.0) ++lc;
+        if (rp <= 0.0) ++rc;
+    }
+    *lcnt = lc;
+    *rcnt = rc;
+}
+
+// dlaev2 (odev2.f:70-170)
+SM_HD inline void dlaev2(double a, double b, double c, double* rt1, double* rt2, double* cs1,
+                         double* sn1) {
+    double sm = a + c, df = a - c, adf = fabs(df), tb = b + b, ab = fabs(tb);
+    double acmx, acmn, rt, cs, ct, tn, acs;
+    int sgn1, sgn2;
+    if (fabs(a) > fabs(c)) { acmx = a; acmn = c; } else { acmx = c; acmn = a; }
+    if (adf > ab) rt = adf * sqrt(1.0 + (ab / adf) * (ab / adf));
+    else if (adf < ab) rt = ab * sqrt(1.0 + (adf / ab) * (adf / ab));
+    else rt = ab * sqrt(2.0);
+    if (sm < 0.0) { *rt1 = 0.5 * (sm - rt); sgn1 = -1; *rt2 = (acmx / *rt1) * acmn - (b / *rt1) * b; }
+    else if (sm > 0.0) { *rt1 = 0.5 * (sm + rt); sgn1 = 1; *rt2 = (acmx / *rt1) * acmn - (b / *rt1) * b; }
+    else { *rt1 = 0.5 * rt; *rt2 = -0.5 * rt; sgn1 = 1; }
+    if (df >= 0.0) { cs = df + rt; sgn2 = 1; } else { cs = df - rt; sgn2 = -1; }
+    acs = fabs(cs);
+    if (acs > ab) { ct = -tb / cs; *sn1 = 1.0 / sqrt(1.0 + ct * ct); *cs1 = ct * *sn1; }
+    else if (ab == 0.0) { *cs1 = 1.0; *sn1 = 0.0; }
+    else { tn = -cs / tb; *cs1 = 1.0 / sqrt(1.0 + tn * tn); *sn1 = tn * *cs1; }
+    if (sgn1 == sgn2) { tn = *cs1; *cs1 = -*sn1; *sn1 = tn; }
+}
+
+// The dlarnv/dlaruv stream of ISEED = (1,1,1,1), IDIST = 2 (odrnv.f, odruv.f): entry i
+// (0-based) is 2 * (s * a^(i+1) mod 2^48) / 2^48 - 1 with a = 33952834046453 and s the
+// seed; the reference advances the generator in steps of the 128-entry multiplier
+// table, which is the same power series.
+SM_HD inline double dlarnv_entry(long long i) {
+    const unsigned long long A = 33952834046453ULL, M48 = (1ULL << 48) - 1ULL;
+    const unsigned long long seed = (1ULL << 36) | (1ULL << 24) | (1ULL << 12) | 1ULL;
+    unsigned long long r = 1ULL, b = A, e = (unsigned long long)(i + 1);
+    while (e) {
+        if (e & 1ULL) r = (r * b) & M48;
+        b = (b * b) & M48;
+        e >>= 1;
+    }
+    const unsigned long long x = (seed * r) & M48;
+    const double R = 1.0 / 4096.0;
+    double it1 = (double)((x >> 36) & 4095ULL), it2 = (double)((x >> 24) & 4095ULL);
+    double it3 = (double)((x >> 12) & 4095ULL), it4 = (double)(x & 4095ULL);
+    double u = R * (it1 + R * (it2 + R * (it3 + R * it4)));
+    return 2.0 * u - 1.0;
+}
+
+// ---------------------------------------------------------------------------------
+// dqds: dlasq2 with dlasq3-6 (odsq2.f .. odsq6.f).  Z is 1-based (Z(i) = z[i-1]).
+// ---------------------------------------------------------------------------------
+#define SMZ(i) z[(i) - 1]
+
+SM_HD inline void dlasq4(int i0, int n0, const double* z, int pp, int n0in, double dmin,
+                         double dmin1, double dmin2, double dn, double dn1, double dn2,
+                         double* tau, int* ttype, double* g) {
+    const double CNST1 = 0.5630, CNST2 = 1.010, CNST3 = 1.050;
+    const double QURTR = 0.250, THIRD = 0.3330, HALF = 0.50;
+    double a2, b1, b2, gam, gap1, gap2, s = 0.0;
+    int i4, nn, np;
+    if (dmin <= 0.0) { *tau = -dmin; *ttype = -1; return; }
+    nn = 4 * n0 + pp;
+    if (n0in == n0) {
+        if (dmin == dn || dmin == dn1) {
+            b1 = sqrt(SMZ(nn - 3)) * sqrt(SMZ(nn - 5));
+            b2 = sqrt(SMZ(nn - 7)) * sqrt(SMZ(nn - 9));
+            a2 = SMZ(nn - 7) + SMZ(nn - 5);
+            if (dmin == dn && dmin1 == dn1) {
+                gap2 = dmin2 - a2 - dmin2 * QURTR;
+                if (gap2 > 0.0 && gap2 > b2) gap1 = a2 - dn - (b2 / gap2) * b2;
+                else gap1 = a2 - dn - (b1 + b2);
+                if (gap1 > 0.0 && gap1 > b1) {
+                    s = fmx(dn - (b1 / gap1) * b1, HALF * dmin);
+                    *ttype = -2;
+                } else {
+                    s = 0.0;
+                    if (dn > b1) s = dn - b1;
+                    if (a2 > (b1 + b2)) s = fmn(s, a2 - (b1 + b2));
+                    s = fmx(s, THIRD * dmin);
+                    *ttype = -3;
+                }
+            } else {
+                *ttype = -4;
+                s = QURTR * dmin;
+                if (dmin == dn) {
+                    gam = dn;
+                    a2 = 0.0;
+                    if (SMZ(nn - 5) > SMZ(nn - 7)) return;
+                    b2 = SMZ(nn - 5) / SMZ(nn - 7);
+                    np = nn - 9;
+                } else {
+                    np = nn - 2 * pp;
+                    gam = dn1;
+                    if (SMZ(np - 4) > SMZ(np - 2)) return;
+                    a2 = SMZ(np - 4) / SMZ(np - 2);
+                    if (SMZ(nn - 9) > SMZ(nn - 11)) return;
+                    b2 = SMZ(nn - 9) / SMZ(nn - 11);
+                    np = nn - 13;
+                }
+                a2 = a2 + b2;
+                for (i4 = np; i4 >= 4 * i0 - 1 + pp; i4 -= 4) {
+                    if (b2 == 0.0) break;
+                    b1 = b2;
+                    if (SMZ(i4) > SMZ(i4 - 2)) return;
+                    b2 = b2 * (SMZ(i4) / SMZ(i4 - 2));
+                    a2 = a2 + b2;
+                    if (100.0 * fmx(b2, b1) < a2 || CNST1 < a2) break;
+                }
+                a2 = CNST3 * a2;
+                if (a2 < CNST1) s = gam * (1.0 - sqrt(a2)) / (1.0 + a2);
+            }
+        } else if (dmin == dn2) {
+            *ttype = -5;
+            s = QURTR * dmin;
+            np = nn - 2 * pp;
+            b1 = SMZ(np - 2);
+            b2 = SMZ(np - 6);
+            gam = dn2;
+            if (SMZ(np - 8) > b2 || SMZ(np - 4) > b1) return;
+            a2 = (SMZ(np - 8) / b2) * (1.0 + SMZ(np - 4) / b1);
+            if (n0 - i0 > 2) {
+                b2 = SMZ(nn - 13) / SMZ(nn - 15);
+                a2 = a2 + b2;
+                for (i4 = nn - 17; i4 >= 4 * i0 - 1 + pp; i4 -= 4) {
+                    if (b2 == 0.0) break;
+                    b1 = b2;
+                    if (SMZ(i4) > SMZ(i4 - 2)) return;
+                    b2 = b2 * (SMZ(i4) / SMZ(i4 - 2));
+                    a2 = a2 + b2;
+                    if (100.0 * fmx(b2, b1) < a2 || CNST1 < a2) break;
+                }
+                a2 = CNST3 * a2;
+            }
+            if (a2 < CNST1) s = gam * (1.0 - sqrt(a2)) / (1.0 + a2);
+        } else {
+            if (*ttype == -6) *g = *g + THIRD * (1.0 - *g);
+            else if (*ttype == -18) *g = QURTR * THIRD;
+            else *g = QURTR;
+            s = *g * dmin;
+            *ttype = -6;
+        }
+    } else if (n0in == n0 + 1) {
+        if (dmin1 == dn1 && dmin2 == dn2) {
+            *ttype = -7;
+            s = THIRD * dmin1;
+            if (SMZ(nn - 5) > SMZ(nn - 7)) return;
+            b1 = SMZ(nn - 5) / SMZ(nn - 7);
+            b2 = b1;
+            if (b2 != 0.0) {
+                for (i4 = 4 * n0 - 9 + pp; i4 >= 4 * i0 - 1 + pp; i4 -= 4) {
+                    a2 = b1;
+                    if (SMZ(i4) > SMZ(i4 - 2)) return;
+                    b1 = b1 * (SMZ(i4) / SMZ(i4 - 2));
+                    b2 = b2 + b1;
+                    if (100.0 * fmx(b1, a2) < b2) break;
+                }
+            }
+            b2 = sqrt(CNST3 * b2);
+            a2 = dmin1 / (1.0 + b2 * b2);
+            gap2 = HALF * dmin2 - a2;
+            if (gap2 > 0.0 && gap2 > b2 * a2) s = fmx(s, a2 * (1.0 - CNST2 * a2 * (b2 / gap2) * b2));
+            else { s = fmx(s, a2 * (1.0 - CNST2 * b2)); *ttype = -8; }
+        } else {
+            s = QURTR * dmin1;
+            if (dmin1 == dn1) s = HALF * dmin1;
+            *ttype = -9;
+        }
+    } else if (n0in == n0 + 2) {
+        if (dmin2 == dn2 && 2.0 * SMZ(nn - 5) < SMZ(nn - 7)) {
+            *ttype = -10;
+            s = THIRD * dmin2;
+            if (SMZ(nn - 5) > SMZ(nn - 7)) return;
+            b1 = SMZ(nn - 5) / SMZ(nn - 7);
+            b2 = b1;
+            if (b2 != 0.0) {
+                for (i4 = 4 * n0 - 9 + pp; i4 >= 4 * i0 - 1 + pp; i4 -= 4) {
+                    if (SMZ(i4) > SMZ(i4 - 2)) return;
+                    b1 = b1 * (SMZ(i4) / SMZ(i4 - 2));
+                    b2 = b2 + b1;
+                    if (100.0 * b1 < b2) break;
+                }
+            }
+            b2 = sqrt(CNST3 * b2);
+            a2 = dmin2 / (1.0 + b2 * b2);
+            gap2 = SMZ(nn - 7) + SMZ(nn - 9) - sqrt(SMZ(nn - 11)) * sqrt(SMZ(nn - 9)) - a2;
+            if (gap2 > 0.0 && gap2 > b2 * a2) s = fmx(s, a2 * (1.0 - CNST2 * a2 * (b2 / gap2) * b2));
+            else s = fmx(s, a2 * (1.0 - CNST2 * b2));
+        } else {
+            s = QURTR * dmin2;
+            *ttype = -11;
+        }
+    } else if (n0in > n0 + 2) {
+        s = 0.0;
+        *ttype = -12;
+    }
+    *tau = s;
+}
+
+// One dqds transform with shift tau, IEEE form (odsq5.f).
+SM_HD inline void dlasq5(int i0, int n0, double* z, int pp, double tau, double* dmin,
+                         double* dmin1, double* dmin2, double* dn, double* dnm1, double* dnm2) {
+    if ((n0 - i0 - 1) <= 0) return;
+    int j4 = 4 * i0 + pp - 3, j4p2;
+    double emin = SMZ(j4 + 4), d = SMZ(j4) - tau, temp;
+    *dmin = d;
+    *dmin1 = -SMZ(j4);
+    // The loads of the next step are issued before this step's stores: they read
+    // entries 4 beyond this step's, which its stores never touch, so the values are
+    // the same and the divide chain does not wait for memory.
+    if (pp == 0) {
+        j4 = 4 * i0;
+        if (j4 <= 4 * (n0 - 3)) {
+            double za = SMZ(j4 - 1), zb = SMZ(j4 + 1);
+            for (; j4 <= 4 * (n0 - 3); j4 += 4) {
+                const double na = SMZ(j4 + 3), nb = SMZ(j4 + 5);
+                const double sj = d + za;
+                SMZ(j4 - 2) = sj;
+                temp = zb / sj;
+                d = d * temp - tau;
+                *dmin = fmn(*dmin, d);
+                const double zj = za * temp;
+                SMZ(j4) = zj;
+                emin = fmn(zj, emin);
+                za = na;
+                zb = nb;
+            }
+        }
+    } else {
+        j4 = 4 * i0;
+        if (j4 <= 4 * (n0 - 3)) {
+            double za = SMZ(j4), zb = SMZ(j4 + 2);
+            for (; j4 <= 4 * (n0 - 3); j4 += 4) {
+                const double na = SMZ(j4 + 4), nb = SMZ(j4 + 6);
+                const double sj = d + za;
+                SMZ(j4 - 3) = sj;
+                temp = zb / sj;
+                d = d * temp - tau;
+                *dmin = fmn(*dmin, d);
+                const double zj = za * temp;
+                SMZ(j4 - 1) = zj;
+                emin = fmn(zj, emin);
+                za = na;
+                zb = nb;
+            }
+        }
+    }
+    *dnm2 = d;
+    *dmin2 = *dmin;
+    j4 = 4 * (n0 - 2) - pp;
+    j4p2 = j4 + 2 * pp - 1;
+    SMZ(j4 - 2) = *dnm2 + SMZ(j4p2);
+    SMZ(j4) = SMZ(j4p2 + 2) * (SMZ(j4p2) / SMZ(j4 - 2));
+    *dnm1 = SMZ(j4p2 + 2) * (*dnm2 / SMZ(j4 - 2)) - tau;
+    *dmin = fmn(*dmin, *dnm1);
+    *dmin1 = *dmin;
+    j4 = j4 + 4;
+    j4p2 = j4 + 2 * pp - 1;
+    SMZ(j4 - 2) = *dnm1 + SMZ(j4p2);
+    SMZ(j4) = SMZ(j4p2 + 2) * (SMZ(j4p2) / SMZ(j4 - 2));
+    *dn = SMZ(j4p2 + 2) * (*dnm1 / SMZ(j4 - 2)) - tau;
+    *dmin = fmn(*dmin, *dn);
+    SMZ(j4 + 2) = *dn;
+    SMZ(4 * n0 - pp) = emin;
+}
+
+// One dqd transform without shift, with the underflow guards of odsq6.f.
+SM_HD inline void dlasq6(int i0, int n0, double* z, int pp, double* dmin, double* dmin1,
+                         double* dmin2, double* dn, double* dnm1, double* dnm2) {
+    if ((n0 - i0 - 1) <= 0) return;
+    int j4 = 4 * i0 + pp - 3, j4p2;
+    double emin = SMZ(j4 + 4), d = SMZ(j4), temp;
+    *dmin = d;
+    if (pp == 0) {
+        for (j4 = 4 * i0; j4 <= 4 * (n0 - 3); j4 += 4) {
+            SMZ(j4 - 2) = d + SMZ(j4 - 1);
+            if (SMZ(j4 - 2) == 0.0) {
+                SMZ(j4) = 0.0; d = SMZ(j4 + 1); *dmin = d; emin = 0.0;
+            } else if (kSafmin * SMZ(j4 + 1) < SMZ(j4 - 2) && kSafmin * SMZ(j4 - 2) < SMZ(j4 + 1)) {
+                temp = SMZ(j4 + 1) / SMZ(j4 - 2);
+                SMZ(j4) = SMZ(j4 - 1) * temp;
+                d = d * temp;
+            } else {
+                SMZ(j4) = SMZ(j4 + 1) * (SMZ(j4 - 1) / SMZ(j4 - 2));
+                d = SMZ(j4 + 1) * (d / SMZ(j4 - 2));
+            }
+            *dmin = fmn(*dmin, d);
+            emin = fmn(emin, SMZ(j4));
+        }
+    } else {
+        for (j4 = 4 * i0; j4 <= 4 * (n0 - 3); j4 += 4) {
+            SMZ(j4 - 3) = d + SMZ(j4);
+            if (SMZ(j4 - 3) == 0.0) {
+                SMZ(j4 - 1) = 0.0; d = SMZ(j4 + 2); *dmin = d; emin = 0.0;
+            } else if (kSafmin * SMZ(j4 + 2) < SMZ(j4 - 3) && kSafmin * SMZ(j4 - 3) < SMZ(j4 + 2)) {
+                temp = SMZ(j4 + 2) / SMZ(j4 - 3);
+                SMZ(j4 - 1) = SMZ(j4) * temp;
+                d = d * temp;
+            } else {
+                SMZ(j4 - 1) = SMZ(j4 + 2) * (SMZ(j4) / SMZ(j4 - 3));
+                d = SMZ(j4 + 2) * (d / SMZ(j4 - 3));
+            }
+            *dmin = fmn(*dmin, d);
+            emin = fmn(emin, SMZ(j4 - 1));
+        }
+    }
+    *dnm2 = d;
+    *dmin2 = *dmin;
+    j4 = 4 * (n0 - 2) - pp;
+    j4p2 = j4 + 2 * pp - 1;
+    SMZ(j4 - 2) = *dnm2 + SMZ(j4p2);
+    if (SMZ(j4 - 2) == 0.0) {
+        SMZ(j4) = 0.0; *dnm1 = SMZ(j4p2 + 2); *dmin = *dnm1; emin = 0.0;
+    } else if (kSafmin * SMZ(j4p2 + 2) < SMZ(j4 - 2) && kSafmin * SMZ(j4 - 2) < SMZ(j4p2 + 2)) {
+        temp = SMZ(j4p2 + 2) / SMZ(j4 - 2);
+        SMZ(j4) = SMZ(j4p2) * temp;
+        *dnm1 = *dnm2 * temp;
+    } else {
+        SMZ(j4) = SMZ(j4p2 + 2) * (SMZ(j4p2) / SMZ(j4 - 2));
+        *dnm1 = SMZ(j4p2 + 2) * (*dnm2 / SMZ(j4 - 2));
+    }
+    *dmin = fmn(*dmin, *dnm1);
+    *dmin1 = *dmin;
+    j4 = j4 + 4;
+    j4p2 = j4 + 2 * pp - 1;
+    SMZ(j4 - 2) = *dnm1 + SMZ(j4p2);
+    if (SMZ(j4 - 2) == 0.0) {
+        SMZ(j4) = 0.0; *dn = SMZ(j4p2 + 2); *dmin = *dn; emin = 0.0;
+    } else if (kSafmin * SMZ(j4p2 + 2) < SMZ(j4 - 2) && kSafmin * SMZ(j4 - 2) < SMZ(j4p2 + 2)) {
+        temp = SMZ(j4p2 + 2) / SMZ(j4 - 2);
+        SMZ(j4) = SMZ(j4p2) * temp;
+        *dn = *dnm1 * temp;
+    } else {
+        SMZ(j4) = SMZ(j4p2 + 2) * (SMZ(j4p2) / SMZ(j4 - 2));
+        *dn = SMZ(j4p2 + 2) * (*dnm1 / SMZ(j4 - 2));
+    }
+    *dmin = fmn(*dmin, *dn);
+    SMZ(j4 + 2) = *dn;
+    SMZ(4 * n0 - pp) = emin;
+}
+
+struct Qd {
+    int n0, pp, nfail, iter, ndiv, ttype;
+    double dmin, sigma, desig, qmax, dmin1, dmin2, dn, dn1, dn2, g, tau;
+};
+
+// Deflation, shift and one transform (odsq3.f).
+SM_HD inline void dlasq3(int i0, double* z, Qd& q) {
+    const double CBIAS = 1.50, tol = kEps * 100.0, tol2 = tol * tol;
+    int n0 = q.n0, pp = q.pp, n0in = q.n0, nn = 0;
+    double s, t, temp;
+    for (;;) {
+        if (n0 < i0) { q.n0 = n0; q.pp = pp; return; }
+        bool defl1 = (n0 == i0), defl2 = false;
+        if (!defl1) {
+            nn = 4 * n0 + pp;
+            if (n0 == i0 + 1) defl2 = true;
+            else if (SMZ(nn - 5) > tol2 * (q.sigma + SMZ(nn - 3)) &&
+                     SMZ(nn - 2 * pp - 4) > tol2 * SMZ(nn - 7)) {
+                if (SMZ(nn - 9) > tol2 * q.sigma && SMZ(nn - 2 * pp - 8) > tol2 * SMZ(nn - 11)) break;
+                defl2 = true;
+            } else defl1 = true;
+        }
+        if (defl1) {
+            SMZ(4 * n0 - 3) = SMZ(4 * n0 + pp - 3) + q.sigma;
+            n0 = n0 - 1;
+            continue;
+        }
+        if (SMZ(nn - 3) > SMZ(nn - 7)) { s = SMZ(nn - 3); SMZ(nn - 3) = SMZ(nn - 7); SMZ(nn - 7) = s; }
+        if (SMZ(nn - 5) > SMZ(nn - 3) * tol2) {
+            t = 0.5 * ((SMZ(nn - 7) - SMZ(nn - 3)) + SMZ(nn - 5));
+            s = SMZ(nn - 3) * (SMZ(nn - 5) / t);
+            if (s <= t) s = SMZ(nn - 3) * (SMZ(nn - 5) / (t * (1.0 + sqrt(1.0 + s / t))));
+            else s = SMZ(nn - 3) * (SMZ(nn - 5) / (t + sqrt(t) * sqrt(t + s)));
+            t = SMZ(nn - 7) + (s + SMZ(nn - 5));
+            SMZ(nn - 3) = SMZ(nn - 3) * (SMZ(nn - 7) / t);
+            SMZ(nn - 7) = t;
+        }
+        SMZ(4 * n0 - 7) = SMZ(nn - 7) + q.sigma;
+        SMZ(4 * n0 - 3) = SMZ(nn - 3) + q.sigma;
+        n0 = n0 - 2;
+    }
+    if (pp == 2) pp = 0;
+    if (q.dmin <= 0.0 || n0 < n0in) {
+        if (CBIAS * SMZ(4 * i0 + pp - 3) < SMZ(4 * n0 + pp - 3)) {
+            const int ipn4 = 4 * (i0 + n0);
+            for (int j4 = 4 * i0; j4 <= 2 * (i0 + n0 - 1); j4 += 4) {
+                temp = SMZ(j4 - 3); SMZ(j4 - 3) = SMZ(ipn4 - j4 - 3); SMZ(ipn4 - j4 - 3) = temp;
+                temp = SMZ(j4 - 2); SMZ(j4 - 2) = SMZ(ipn4 - j4 - 2); SMZ(ipn4 - j4 - 2) = temp;
+                temp = SMZ(j4 - 1); SMZ(j4 - 1) = SMZ(ipn4 - j4 - 5); SMZ(ipn4 - j4 - 5) = temp;
+                temp = SMZ(j4);     SMZ(j4) = SMZ(ipn4 - j4 - 4);     SMZ(ipn4 - j4 - 4) = temp;
+            }
+            if (n0 - i0 <= 4) {
+                SMZ(4 * n0 + pp - 1) = SMZ(4 * i0 + pp - 1);
+                SMZ(4 * n0 - pp) = SMZ(4 * i0 - pp);
+            }
+            q.dmin2 = fmn(q.dmin2, SMZ(4 * n0 + pp - 1));
+            SMZ(4 * n0 + pp - 1) = fmn(fmn(SMZ(4 * n0 + pp - 1), SMZ(4 * i0 + pp - 1)), SMZ(4 * i0 + pp + 3));
+            SMZ(4 * n0 - pp) = fmn(fmn(SMZ(4 * n0 - pp), SMZ(4 * i0 - pp)), SMZ(4 * i0 - pp + 4));
+            q.qmax = fmx(fmx(q.qmax, SMZ(4 * i0 + pp - 3)), SMZ(4 * i0 + pp + 1));
+            q.dmin = -0.0;
+        }
+    }
+    dlasq4(i0, n0, z, pp, n0in, q.dmin, q.dmin1, q.dmin2, q.dn, q.dn1, q.dn2, &q.tau, &q.ttype, &q.g);
+    bool use6 = false;
+    for (;;) {
+        dlasq5(i0, n0, z, pp, q.tau, &q.dmin, &q.dmin1, &q.dmin2, &q.dn, &q.dn1, &q.dn2);
+        q.ndiv += n0 - i0 + 2;
+        q.iter++;
+        if (q.dmin >= 0.0 && q.dmin1 > 0.0) break;
+        if (q.dmin < 0.0 && q.dmin1 > 0.0 && SMZ(4 * (n0 - 1) - pp) < tol * (q.sigma + q.dn1) &&
+            fabs(q.dn) < tol * q.sigma) {
+            SMZ(4 * (n0 - 1) - pp + 2) = 0.0;
+            q.dmin = 0.0;
+            break;
+        }
+        if (q.dmin < 0.0) {
+            q.nfail++;
+            if (q.ttype < -22) q.tau = 0.0;
+            else if (q.dmin1 > 0.0) { q.tau = (q.tau + q.dmin) * (1.0 - 2.0 * kEps); q.ttype -= 11; }
+            else { q.tau = 0.25 * q.tau; q.ttype -= 12; }
+            continue;
+        }
+        if (isnan(q.dmin)) {
+            if (q.tau == 0.0) { use6 = true; break; }
+            q.tau = 0.0;
+            continue;
+        }
+        use6 = true;
+        break;
+    }
+    if (use6) {
+        dlasq6(i0, n0, z, pp, &q.dmin, &q.dmin1, &q.dmin2, &q.dn, &q.dn1, &q.dn2);
+        q.ndiv += n0 - i0 + 2;
+        q.iter++;
+        q.tau = 0.0;
+    }
+    if (q.tau < q.sigma) {
+        q.desig = q.desig + q.tau;
+        t = q.sigma + q.desig;
+        q.desig = q.desig - (t - q.sigma);
+    } else {
+        t = q.sigma + q.tau;
+        q.desig = q.sigma - (t - q.tau) + q.desig;
+    }
+    q.sigma = t;
+    q.n0 = n0;
+    q.pp = pp;
+}
+
+// Sorts z[0..n) in decreasing order (dlasrt 'D' leaves the same values in the same
+// order: equal values are interchangeable).
+SM_HD inline void sort_desc(double* z, int n) {
+    for (int i = 1; i < n; ++i) {
+        double v = z[i];
+        int j = i - 1;
+        while (j >= 0 && z[j] < v) { z[j + 1] = z[j]; --j; }
+        z[j + 1] = v;
+    }
+}
+
+// dlasq2 (odsq2.f): all eigenvalues of the qd array, decreasing, in Z(1:n).
+SM_HD inline int dlasq2(int n, double* z) {
+    const double CBIAS = 1.50, tol = kEps * 100.0, tol2 = tol * tol;
+    double d, dee, deemin, e, emax, emin, oldemn, qmax, qmin, s, t, temp, trace;
+    int i0, i1, i4, ipn4, k, kmin, n1, nbig, splt;
+    if (n < 0) return -1;
+    if (n == 0) return 0;
+    if (n == 1) return (SMZ(1) < 0.0) ? -201 : 0;
+    if (n == 2) {
+        if (SMZ(2) < 0.0 || SMZ(3) < 0.0) return -2;
+        if (SMZ(3) > SMZ(1)) { d = SMZ(3); SMZ(3) = SMZ(1); SMZ(1) = d; }
+        SMZ(5) = SMZ(1) + SMZ(2) + SMZ(3);
+        if (SMZ(2) > SMZ(3) * tol2) {
+            t = 0.5 * ((SMZ(1) - SMZ(3)) + SMZ(2));
+            s = SMZ(3) * (SMZ(2) / t);
+            if (s <= t) s = SMZ(3) * (SMZ(2) / (t * (1.0 + sqrt(1.0 + s / t))));
+            else s = SMZ(3) * (SMZ(2) / (t + sqrt(t) * sqrt(t + s)));
+            t = SMZ(1) + (s + SMZ(2));
+            SMZ(3) = SMZ(3) * (SMZ(1) / t);
+            SMZ(1) = t;
+        }
+        SMZ(2) = SMZ(3);
+        SMZ(6) = SMZ(2) + SMZ(1);
+        return 0;
+    }
+    SMZ(2 * n) = 0.0;
+    emin = SMZ(2);
+    qmax = 0.0;
+    d = 0.0;
+    e = 0.0;
+    for (k = 1; k <= 2 * (n - 1); k += 2) {
+        if (SMZ(k) < 0.0) return -(200 + k);
+        if (SMZ(k + 1) < 0.0) return -(200 + k + 1);
+        d = d + SMZ(k);
+        e = e + SMZ(k + 1);
+        qmax = fmx(qmax, SMZ(k));
+        emin = fmn(emin, SMZ(k + 1));
+    }
+    if (SMZ(2 * n - 1) < 0.0) return -(200 + 2 * n - 1);
+    d = d + SMZ(2 * n - 1);
+    qmax = fmx(qmax, SMZ(2 * n - 1));
+    if (e == 0.0) {
+        for (k = 2; k <= n; ++k) SMZ(k) = SMZ(2 * k - 1);
+        sort_desc(z, n);
+        SMZ(2 * n - 1) = d;
+        return 0;
+    }
+    trace = d + e;
+    if (trace == 0.0) { SMZ(2 * n - 1) = 0.0; return 0; }
+    for (k = 2 * n; k >= 2; k -= 2) {
+        SMZ(2 * k) = 0.0;
+        SMZ(2 * k - 1) = SMZ(k);
+        SMZ(2 * k - 2) = 0.0;
+        SMZ(2 * k - 3) = SMZ(k - 1);
+    }
+    i0 = 1;
+    int n0 = n;
+    if (CBIAS * SMZ(4 * i0 - 3) < SMZ(4 * n0 - 3)) {
+        ipn4 = 4 * (i0 + n0);
+        for (i4 = 4 * i0; i4 <= 2 * (i0 + n0 - 1); i4 += 4) {
+            temp = SMZ(i4 - 3); SMZ(i4 - 3) = SMZ(ipn4 - i4 - 3); SMZ(ipn4 - i4 - 3) = temp;
+            temp = SMZ(i4 - 1); SMZ(i4 - 1) = SMZ(ipn4 - i4 - 5); SMZ(ipn4 - i4 - 5) = temp;
+        }
+    }
+    int pp = 0;
+    for (k = 1; k <= 2; ++k) {
+        d = SMZ(4 * n0 + pp - 3);
+        for (i4 = 4 * (n0 - 1) + pp; i4 >= 4 * i0 + pp; i4 -= 4) {
+            if (SMZ(i4 - 1) <= tol2 * d) { SMZ(i4 - 1) = -0.0; d = SMZ(i4 - 3); }
+            else d = SMZ(i4 - 3) * (d / (d + SMZ(i4 - 1)));
+        }
+        emin = SMZ(4 * i0 + pp + 1);
+        d = SMZ(4 * i0 + pp - 3);
+        for (i4 = 4 * i0 + pp; i4 <= 4 * (n0 - 1) + pp; i4 += 4) {
+            SMZ(i4 - 2 * pp - 2) = d + SMZ(i4 - 1);
+            if (SMZ(i4 - 1) <= tol2 * d) {
+                SMZ(i4 - 1) = -0.0;
+                SMZ(i4 - 2 * pp - 2) = d;
+                SMZ(i4 - 2 * pp) = 0.0;
+                d = SMZ(i4 + 1);
+            } else if (kSafmin * SMZ(i4 + 1) < SMZ(i4 - 2 * pp - 2) &&
+                       kSafmin * SMZ(i4 - 2 * pp - 2) < SMZ(i4 + 1)) {
+                temp = SMZ(i4 + 1) / SMZ(i4 - 2 * pp - 2);
+                SMZ(i4 - 2 * pp) = SMZ(i4 - 1) * temp;
+                d = d * temp;
+            } else {
+                SMZ(i4 - 2 * pp) = SMZ(i4 + 1) * (SMZ(i4 - 1) / SMZ(i4 - 2 * pp - 2));
+                d = SMZ(i4 + 1) * (d / SMZ(i4 - 2 * pp - 2));
+            }
+            emin = fmn(emin, SMZ(i4 - 2 * pp));
+        }
+        SMZ(4 * n0 - pp - 2) = d;
+        qmax = SMZ(4 * i0 - pp - 2);
+        for (i4 = 4 * i0 - pp + 2; i4 <= 4 * n0 - pp - 2; i4 += 4) qmax = fmx(qmax, SMZ(i4));
+        pp = 1 - pp;
+    }
+    Qd q;
+    q.n0 = n0; q.pp = 0; q.nfail = 0; q.iter = 2; q.ndiv = 2 * (n0 - i0); q.ttype = 0;
+    q.dmin = 0.0; q.sigma = 0.0; q.desig = 0.0; q.qmax = qmax; q.dmin1 = 0.0; q.dmin2 = 0.0;
+    q.dn = 0.0; q.dn1 = 0.0; q.dn2 = 0.0; q.g = 0.0; q.tau = 0.0;
+    for (int iwhila = 1; iwhila <= n + 1; ++iwhila) {
+        if (n0 < 1) {
+            for (k = 2; k <= n; ++k) SMZ(k) = SMZ(4 * k - 3);
+            sort_desc(z, n);
+            return 0;
+        }
+        q.desig = 0.0;
+        q.sigma = (n0 == n) ? 0.0 : -SMZ(4 * n0 - 1);
+        if (q.sigma < 0.0) return 1;
+        emax = 0.0;
+        emin = (n0 > i0) ? fabs(SMZ(4 * n0 - 5)) : 0.0;
+        qmin = SMZ(4 * n0 - 3);
+        q.qmax = qmin;
+        for (i4 = 4 * n0; i4 >= 8; i4 -= 4) {
+            if (SMZ(i4 - 5) <= 0.0) break;
+            if (qmin >= 4.0 * emax) {
+                qmin = fmn(qmin, SMZ(i4 - 3));
+                emax = fmx(emax, SMZ(i4 - 5));
+            }
+            q.qmax = fmx(q.qmax, SMZ(i4 - 7) + SMZ(i4 - 5));
+            emin = fmn(emin, SMZ(i4 - 5));
+        }
+        if (i4 < 8) i4 = 4;
+        i0 = i4 / 4;
+        pp = 0;
+        if (n0 - i0 > 1) {
+            dee = SMZ(4 * i0 - 3);
+            deemin = dee;
+            kmin = i0;
+            for (i4 = 4 * i0 + 1; i4 <= 4 * n0 - 3; i4 += 4) {
+                dee = SMZ(i4) * (dee / (dee + SMZ(i4 - 2)));
+                if (dee <= deemin) { deemin = dee; kmin = (i4 + 3) / 4; }
+            }
+            if ((kmin - i0) * 2 < n0 - kmin && deemin <= 0.5 * SMZ(4 * n0 - 3)) {
+                ipn4 = 4 * (i0 + n0);
+                pp = 2;
+                for (i4 = 4 * i0; i4 <= 2 * (i0 + n0 - 1); i4 += 4) {
+                    temp = SMZ(i4 - 3); SMZ(i4 - 3) = SMZ(ipn4 - i4 - 3); SMZ(ipn4 - i4 - 3) = temp;
+                    temp = SMZ(i4 - 2); SMZ(i4 - 2) = SMZ(ipn4 - i4 - 2); SMZ(ipn4 - i4 - 2) = temp;
+                    temp = SMZ(i4 - 1); SMZ(i4 - 1) = SMZ(ipn4 - i4 - 5); SMZ(ipn4 - i4 - 5) = temp;
+                    temp = SMZ(i4);     SMZ(i4) = SMZ(ipn4 - i4 - 4);     SMZ(ipn4 - i4 - 4) = temp;
+                }
+            }
+        }
+        q.dmin = -fmx(0.0, qmin - 2.0 * sqrt(qmin) * sqrt(emax));
+        nbig = 100 * (n0 - i0 + 1);
+        bool split_done = false;
+        for (int iwhilb = 1; iwhilb <= nbig; ++iwhilb) {
+            if (i0 > n0) { split_done = true; break; }
+            q.n0 = n0;
+            q.pp = pp;
+            dlasq3(i0, z, q);
+            n0 = q.n0;
+            pp = 1 - q.pp;
+            if (pp == 0 && n0 - i0 >= 3) {
+                if (SMZ(4 * n0) <= tol2 * q.qmax || SMZ(4 * n0 - 1) <= tol2 * q.sigma) {
+                    splt = i0 - 1;
+                    q.qmax = SMZ(4 * i0 - 3);
+                    emin = SMZ(4 * i0 - 1);
+                    oldemn = SMZ(4 * i0);
+                    for (i4 = 4 * i0; i4 <= 4 * (n0 - 3); i4 += 4) {
+                        if (SMZ(i4) <= tol2 * SMZ(i4 - 3) || SMZ(i4 - 1) <= tol2 * q.sigma) {
+                            SMZ(i4 - 1) = -q.sigma;
+                            splt = i4 / 4;
+                            q.qmax = 0.0;
+                            emin = SMZ(i4 + 3);
+                            oldemn = SMZ(i4 + 4);
+                        } else {
+                            q.qmax = fmx(q.qmax, SMZ(i4 + 1));
+                            emin = fmn(emin, SMZ(i4 - 1));
+                            oldemn = fmn(oldemn, SMZ(i4));
+                        }
+                    }
+                    SMZ(4 * n0 - 1) = emin;
+                    SMZ(4 * n0) = oldemn;
+                    i0 = splt + 1;
+                }
+            }
+        }
+        if (split_done) continue;
+        // too many iterations: the reference restores the qd array and returns INFO = 2
+        i1 = i0;
+        n1 = n0;
+        for (;;) {
+            double tempq = SMZ(4 * i0 - 3);
+            SMZ(4 * i0 - 3) = SMZ(4 * i0 - 3) + q.sigma;
+            for (k = i0 + 1; k <= n0; ++k) {
+                double tempe = SMZ(4 * k - 5);
+                SMZ(4 * k - 5) = SMZ(4 * k - 5) * (tempq / SMZ(4 * k - 7));
+                tempq = SMZ(4 * k - 3);
+                SMZ(4 * k - 3) = SMZ(4 * k - 3) + q.sigma + tempe - SMZ(4 * k - 5);
+            }
+            if (i1 > 1) {
+                n1 = i1 - 1;
+                while (i1 >= 2 && SMZ(4 * i1 - 5) >= 0.0) i1 = i1 - 1;
+                q.sigma = -SMZ(4 * n1 - 1);
+                continue;
+            }
+            break;
+        }
+        for (k = 1; k <= n; ++k) {
+            SMZ(2 * k - 1) = SMZ(4 * k - 3);
+            SMZ(2 * k) = (k < n0) ? SMZ(4 * k - 1) : 0.0;
+        }
+        return 2;
+    }
+    return 3;
+}
+#undef SMZ
+
+// ---------------------------------------------------------------------------------
+// dlar1v (odr1v.f:156-369) on one representation; wk holds 4n + 2 doubles at stride ws
+// (a thread's slice of an array interleaved by thread).  z points at the block's first
+// row of the eigenvector column.  The support is cut where a test value falls below
+// gaptol; *cutmax receives the largest value that cut and *noncutmin the smallest
+// tested value that did not, so that the caller knows for which gaptol the vector
+// would come out the same.
+// ---------------------------------------------------------------------------------
+SM_HD inline void dlar1v(int n, int b1, int bn, double lambda, const double* d, const double* l,
+                         const double* ld, const double* lld, double pivmin, double gaptol,
+                         double* z, int wantnc, int* negcnt, double* ztz, double* mingma,
+                         int* r, int* isuppz, double* nrminv, double* resid, double* rqcorr,
+                         double* wk, int ws, double* cutmax, double* noncutmin) {
+#define D_(i) d[(i) - 1]
+#define L_(i) l[(i) - 1]
+#define LD_(i) ld[(i) - 1]
+#define LLD_(i) lld[(i) - 1]
+#define Z_(i) z[(i) - 1]
+#define WK(i) wk[(long long)((i) - 1) * ws]
+    int r1, r2, neg1, neg2, i;
+    double s, dplus, dminus, tmp;
+    bool sawnan1, sawnan2;
+    if (*r == 0) { r1 = b1; r2 = bn; } else { r1 = *r; r2 = *r; }
+    const int indlpl = 0, indumn = n, inds = 2 * n + 1, indp = 3 * n + 1;
+    if (b1 == 1) WK(inds) = 0.0;
+    else WK(inds + b1 - 1) = LLD_(b1 - 1);
+    neg1 = 0;
+    s = WK(inds + b1 - 1) - lambda;
+    for (i = b1; i <= r1 - 1; ++i) {
+        dplus = D_(i) + s;
+        WK(indlpl + i) = LD_(i) / dplus;
+        if (dplus < 0.0) ++neg1;
+        WK(inds + i) = s * WK(indlpl + i) * L_(i);
+        s = WK(inds + i) - lambda;
+    }
+    sawnan1 = isnan(s);
+    if (!sawnan1) {
+        for (i = r1; i <= r2 - 1; ++i) {
+            dplus = D_(i) + s;
+            WK(indlpl + i) = LD_(i) / dplus;
+            WK(inds + i) = s * WK(indlpl + i) * L_(i);
+            s = WK(inds + i) - lambda;
+        }
+        sawnan1 = isnan(s);
+    }
+    if (sawnan1) {
+        neg1 = 0;
+        s = WK(inds + b1 - 1) - lambda;
+        for (i = b1; i <= r1 - 1; ++i) {
+            dplus = D_(i) + s;
+            if (fabs(dplus) < pivmin) dplus = -pivmin;
+            WK(indlpl + i) = LD_(i) / dplus;
+            if (dplus < 0.0) ++neg1;
+            WK(inds + i) = s * WK(indlpl + i) * L_(i);
+            if (WK(indlpl + i) == 0.0) WK(inds + i) = LLD_(i);
+            s = WK(inds + i) - lambda;
+        }
+        for (i = r1; i <= r2 - 1; ++i) {
+            dplus = D_(i) + s;
+            if (fabs(dplus) < pivmin) dplus = -pivmin;
+            WK(indlpl + i) = LD_(i) / dplus;
+            WK(inds + i) = s * WK(indlpl + i) * L_(i);
+            if (WK(indlpl + i) == 0.0) WK(inds + i) = LLD_(i);
+            s = WK(inds + i) - lambda;
+        }
+    }
+    neg2 = 0;
+    WK(indp + bn - 1) = D_(bn) - lambda;
+    for (i = bn - 1; i >= r1; --i) {
+        dminus = LLD_(i) + WK(indp + i);
+        tmp = D_(i) / dminus;
+        if (dminus < 0.0) ++neg2;
+        WK(indumn + i) = L_(i) * tmp;
+        WK(indp + i - 1) = WK(indp + i) * tmp - lambda;
+    }
+    tmp = WK(indp + r1 - 1);
+    sawnan2 = isnan(tmp);
+    if (sawnan2) {
+        neg2 = 0;
+        for (i = bn - 1; i >= r1; --i) {
+            dminus = LLD_(i) + WK(indp + i);
+            if (fabs(dminus) < pivmin) dminus = -pivmin;
+            tmp = D_(i) / dminus;
+            if (dminus < 0.0) ++neg2;
+            WK(indumn + i) = L_(i) * tmp;
+            WK(indp + i - 1) = WK(indp + i) * tmp - lambda;
+            if (tmp == 0.0) WK(indp + i - 1) = D_(i) - lambda;
+        }
+    }
+    *mingma = WK(inds + r1 - 1) + WK(indp + r1 - 1);
+    if (*mingma < 0.0) ++neg1;
+    *negcnt = wantnc ? neg1 + neg2 : -1;
+    if (fabs(*mingma) == 0.0) *mingma = kEps * WK(inds + r1 - 1);
+    *r = r1;
+    for (i = r1; i <= r2 - 1; ++i) {
+        tmp = WK(inds + i) + WK(indp + i);
+        if (tmp == 0.0) tmp = kEps * WK(inds + i);
+        if (fabs(tmp) <= fabs(*mingma)) { *mingma = tmp; *r = i + 1; }
+    }
+    isuppz[0] = b1;
+    isuppz[1] = bn;
+    Z_(*r) = 1.0;
+    *ztz = 1.0;
+    const int rr = *r;
+    if (!sawnan1 && !sawnan2) {
+        for (i = rr - 1; i >= b1; --i) {
+            Z_(i) = -(WK(indlpl + i) * Z_(i + 1));
+            const double tv = (fabs(Z_(i)) + fabs(Z_(i + 1))) * fabs(LD_(i));
+            if (tv < gaptol) {
+                *cutmax = fmx(*cutmax, tv);
+                Z_(i) = 0.0;
+                isuppz[0] = i + 1;
+                break;
+            }
+            *noncutmin = fmn(*noncutmin, tv);
+            *ztz = *ztz + Z_(i) * Z_(i);
+        }
+        for (i = rr; i <= bn - 1; ++i) {
+            Z_(i + 1) = -(WK(indumn + i) * Z_(i));
+            const double tv = (fabs(Z_(i)) + fabs(Z_(i + 1))) * fabs(LD_(i));
+            if (tv < gaptol) {
+                *cutmax = fmx(*cutmax, tv);
+                Z_(i + 1) = 0.0;
+                isuppz[1] = i;
+                break;
+            }
+            *noncutmin = fmn(*noncutmin, tv);
+            *ztz = *ztz + Z_(i + 1) * Z_(i + 1);
+        }
+    } else {
+        for (i = rr - 1; i >= b1; --i) {
+            if (Z_(i + 1) == 0.0) Z_(i) = -(LD_(i + 1) / LD_(i)) * Z_(i + 2);
+            else Z_(i) = -(WK(indlpl + i) * Z_(i + 1));
+            const double tv = (fabs(Z_(i)) + fabs(Z_(i + 1))) * fabs(LD_(i));
+            if (tv < gaptol) {
+                *cutmax = fmx(*cutmax, tv);
+                Z_(i) = 0.0;
+                isuppz[0] = i + 1;
+                break;
+            }
+            *noncutmin = fmn(*noncutmin, tv);
+            *ztz = *ztz + Z_(i) * Z_(i);
+        }
+        for (i = rr; i <= bn - 1; ++i) {
+            if (Z_(i) == 0.0) Z_(i + 1) = -(LD_(i - 1) / LD_(i)) * Z_(i - 1);
+            else Z_(i + 1) = -(WK(indumn + i) * Z_(i));
+            const double tv = (fabs(Z_(i)) + fabs(Z_(i + 1))) * fabs(LD_(i));
+            if (tv < gaptol) {
+                *cutmax = fmx(*cutmax, tv);
+                Z_(i + 1) = 0.0;
+                isuppz[1] = i;
+                break;
+            }
+            *noncutmin = fmn(*noncutmin, tv);
+            *ztz = *ztz + Z_(i + 1) * Z_(i + 1);
+        }
+    }
+    tmp = 1.0 / *ztz;
+    *nrminv = sqrt(tmp);
+    *resid = fabs(*mingma) * *nrminv;
+    *rqcorr = *mingma * tmp;
+#undef D_
+#undef L_
+#undef LD_
+#undef LLD_
+#undef Z_
+#undef WK
+}
+
+// ---------------------------------------------------------------------------------
+// dlarrf (odrrf.f:140-384): a new representation for the cluster clstrt..clend.
+// w, wgap, werr are 1-based from the block's first eigenvalue; wk holds 2n doubles at
+// stride ws.
+// ---------------------------------------------------------------------------------
+SM_HD inline int dlarrf(int n, const double* d, const double* l, const double* ld, int clstrt,
+                        int clend, const double* w, const double* wgap, const double* werr,
+                        double spdiam, double clgapl, double clgapr, double pivmin,
+                        double* sigma, double* dplus, double* lplus, double* wk, int ws) {
+#define W_(i) w[(i) - 1]
+#define WGAP_(i) wgap[(i) - 1]
+#define WERR_(i) werr[(i) - 1]
+#define DP(i) dplus[(i) - 1]
+#define LP(i) lplus[(i) - 1]
+#define WK(i) wk[(long long)((i) - 1) * ws]
+    const double MAXGROWTH1 = 8.0, MAXGROWTH2 = 8.0;
+    const int KTRYMAX = 1, SLEFT = 1, SRIGHT = 2;
+    const double fact = (double)(1 << KTRYMAX);
+    int shift = 0, indx = 0, ktry = 0, i;
+    bool forcer = false, sawnan1, sawnan2, dorrr1;
+    const bool nofail = true;
+    double clwdth = fabs(W_(clend) - W_(clstrt)) + WERR_(clend) + WERR_(clstrt);
+    double avgap = clwdth / (double)(clend - clstrt);
+    double mingap = fmin(clgapl, clgapr);
+    double lsigma = fmin(W_(clstrt), W_(clend)) - WERR_(clstrt);
+    double rsigma = fmax(W_(clstrt), W_(clend)) + WERR_(clend);
+    lsigma = lsigma - fabs(lsigma) * 4.0 * kEps;
+    rsigma = rsigma + fabs(rsigma) * 4.0 * kEps;
+    double ldmax = 0.25 * mingap + 2.0 * pivmin;
+    double rdmax = 0.25 * mingap + 2.0 * pivmin;
+    double ldelta = fmax(avgap, WGAP_(clstrt)) / fact;
+    double rdelta = fmax(avgap, WGAP_(clend - 1)) / fact;
+    double s = kSafmin;
+    double smlgrowth = 1.0 / s;
+    double fail = (double)(n - 1) * mingap / (spdiam * kEps);
+    double fail2 = (double)(n - 1) * mingap / (spdiam * sqrt(kEps));
+    double bestshift = lsigma;
+    double max1 = 0, max2 = 0, tmp, znm2, prod, oldp;
+    const double growthbound = MAXGROWTH1 * spdiam;
+    for (;;) {
+        sawnan1 = false;
+        sawnan2 = false;
+        ldelta = fmin(ldmax, ldelta);
+        rdelta = fmin(rdmax, rdelta);
+        s = -lsigma;
+        DP(1) = d[0] + s;
+        if (fabs(DP(1)) < pivmin) { DP(1) = -pivmin; sawnan1 = true; }
+        max1 = fabs(DP(1));
+        for (i = 1; i <= n - 1; ++i) {
+            LP(i) = ld[i - 1] / DP(i);
+            s = s * LP(i) * l[i - 1] - lsigma;
+            DP(i + 1) = d[i] + s;
+            if (fabs(DP(i + 1)) < pivmin) { DP(i + 1) = -pivmin; sawnan1 = true; }
+            max1 = fmax(max1, fabs(DP(i + 1)));
+        }
+        sawnan1 = sawnan1 || isnan(max1);
+        if (forcer || (max1 <= growthbound && !sawnan1)) { *sigma = lsigma; shift = SLEFT; break; }
+        s = -rsigma;
+        WK(1) = d[0] + s;
+        if (fabs(WK(1)) < pivmin) { WK(1) = -pivmin; sawnan2 = true; }
+        max2 = fabs(WK(1));
+        for (i = 1; i <= n - 1; ++i) {
+            WK(n + i) = ld[i - 1] / WK(i);
+            s = s * WK(n + i) * l[i - 1] - rsigma;
+            WK(i + 1) = d[i] + s;
+            if (fabs(WK(i + 1)) < pivmin) { WK(i + 1) = -pivmin; sawnan2 = true; }
+            max2 = fmax(max2, fabs(WK(i + 1)));
+        }
+        sawnan2 = sawnan2 || isnan(max2);
+        if (forcer || (max2 <= growthbound && !sawnan2)) { *sigma = rsigma; shift = SRIGHT; break; }
+        if (!(sawnan1 && sawnan2)) {
+            if (!sawnan1) {
+                indx = 1;
+                if (max1 <= smlgrowth) { smlgrowth = max1; bestshift = lsigma; }
+            }
+            if (!sawnan2) {
+                if (sawnan1 || max2 <= max1) indx = 2;
+                if (max2 <= smlgrowth) { smlgrowth = max2; bestshift = rsigma; }
+            }
+            dorrr1 = (clwdth < mingap / 128.0) && (fmin(max1, max2) < fail2) && !sawnan1 && !sawnan2;
+            if (dorrr1) {
+                if (indx == 1) {
+                    tmp = fabs(DP(n));
+                    znm2 = 1.0; prod = 1.0; oldp = 1.0;
+                    for (i = n - 1; i >= 1; --i) {
+                        if (prod <= kEps) prod = ((DP(i + 1) * WK(n + i + 1)) / (DP(i) * WK(n + i))) * oldp;
+                        else prod = prod * fabs(WK(n + i));
+                        oldp = prod;
+                        znm2 = znm2 + prod * prod;
+                        tmp = fmax(tmp, fabs(DP(i) * prod));
+                    }
+                    if (tmp / (spdiam * sqrt(znm2)) <= MAXGROWTH2) { *sigma = lsigma; shift = SLEFT; break; }
+                } else if (indx == 2) {
+                    tmp = fabs(WK(n));
+                    znm2 = 1.0; prod = 1.0; oldp = 1.0;
+                    for (i = n - 1; i >= 1; --i) {
+                        if (prod <= kEps) prod = ((WK(i + 1) * LP(i + 1)) / (WK(i) * LP(i))) * oldp;
+                        else prod = prod * fabs(LP(i));
+                        oldp = prod;
+                        znm2 = znm2 + prod * prod;
+                        tmp = fmax(tmp, fabs(WK(i) * prod));
+                    }
+                    if (tmp / (spdiam * sqrt(znm2)) <= MAXGROWTH2) { *sigma = rsigma; shift = SRIGHT; break; }
+                }
+            }
+        }
+        if (ktry < KTRYMAX) {
+            lsigma = fmax(lsigma - ldelta, lsigma - ldmax);
+            rsigma = fmin(rsigma + rdelta, rsigma + rdmax);
+            ldelta = 2.0 * ldelta;
+            rdelta = 2.0 * rdelta;
+            ++ktry;
+            continue;
+        }
+        if (smlgrowth < fail || nofail) {
+            lsigma = bestshift;
+            rsigma = bestshift;
+            forcer = true;
+            continue;
+        }
+        return 1;
+    }
+    if (shift == SRIGHT) {
+        for (i = 1; i <= n; ++i) DP(i) = WK(i);
+        for (i = 1; i <= n - 1; ++i) LP(i) = WK(n + i);
+    }
+    return 0;
+#undef W_
+#undef WGAP_
+#undef WERR_
+#undef DP
+#undef LP
+#undef WK
+}
+
+
+// ---------------------------------------------------------------------------------
+// dlarrd with ORDER = 'B' (odrrd.f:241-722).  The bisection of dlaebz IJOB = 2 splits
+// an interval when its count lies strictly inside; the interval that holds eigenvalue j
+// is then always the half on j's side, and intervals never interact, so each
+// eigenvalue is bisected on its own by one thread: the same points, counts and
+// convergence tests as the reference's list.  Returns INFO; m, wl, wu in sh.
+// ---------------------------------------------------------------------------------
+SM_HD inline int dlarrd_team(int tid, int nt, Shared& sh, int irange, int n, double vl,
+                             double vu, int il, int iu, const double* gers, double reltol,
+                             const double* d, const double* e2, double pivmin, int nsplit,
+                             const int* isplit, double* w, double* werr, int* iblock,
+                             int* indexw) {
+    const double FUDGE = 2.0;
+    if (irange == kInd && il == 1 && iu == n) irange = kAll;
+    double gl = d[0], gu = d[0];
+    for (int i = 1; i <= n; ++i) {
+        gl = fmn(gl, gers[2 * i - 2]);
+        gu = fmx(gu, gers[2 * i - 1]);
+    }
+    const double tnorm = fmx(fabs(gl), fabs(gu));
+    gl = gl - FUDGE * tnorm * kEps * n - FUDGE * 2.0 * pivmin;
+    gu = gu + FUDGE * tnorm * kEps * n + FUDGE * 2.0 * pivmin;
+    const double rtoli = reltol;
+    const double atoli = FUDGE * 2.0 * kSafmin + FUDGE * 2.0 * pivmin;
+    // range I: dlaebz IJOB = 3 for the counts il - 1 and iu (odrrd.f:343-393); the two
+    // intervals never interact, one thread each
+    if (tid == 0) { sh.iw[6] = 0; sh.iw[7] = 0; }
+    SM_SYNC();
+    if (irange == kInd) {
+        const int itmax = (int)((log(tnorm + pivmin) - log(pivmin)) / log(2.0)) + 2;
+        for (int t = tid; t < 2; t += nt) {
+            const int nval = (t == 0) ? il - 1 : iu;
+            double a1 = gl, a2 = gu, c = (t == 0) ? gl : gu;
+            int n1 = -1, n2 = n + 1;
+            bool conv = false;
+            for (int jit = 1; jit <= itmax; ++jit) {
+                const int cnt = count_T2(n, d, e2, c, pivmin);
+                if (cnt <= nval) { a1 = c; n1 = cnt; }
+                if (cnt >= nval) { a2 = c; n2 = cnt; }
+                const double w1 = fabs(a2 - a1), w2 = fmx(fabs(a2), fabs(a1));
+                if (w1 < fmx(fmx(atoli, pivmin), rtoli * w2) || n1 >= n2) { conv = true; break; }
+                c = 0.5 * (a1 + a2);
+            }
+            if (!conv) sh.iw[6] = 1;
+            if (t == 0) { sh.w[0] = a1; sh.w[1] = a2; sh.iw[0] = n1; }   // wl, wlu, nwl
+            else { sh.w[2] = a2; sh.w[3] = a1; sh.iw[1] = n2; }          // wu, wul, nwu
+        }
+    }
+    SM_SYNC();
+    double wl, wu, wlu = 0.0, wul = 0.0;
+    int nwl = 0, nwu = 0;
+    if (irange == kInd) {
+        if (sh.iw[6]) return 1;
+        wl = sh.w[0]; wlu = sh.w[1]; nwl = sh.iw[0];
+        wu = sh.w[2]; wul = sh.w[3]; nwu = sh.iw[1];
+        if (nwl < 0 || nwl >= n || nwu < 1 || nwu > n) return 4;
+    } else if (irange == kVal) { wl = vl; wu = vu; }
+    else { wl = gl; wu = gu; }
+    SM_SYNC();
+    // block loop: the first thread keeps the counts, the team bisects each block
+    int mm = 0, iend = 0, nwlb = 0, nwub = 0;
+    for (int jblk = 1; jblk <= nsplit; ++jblk) {
+        if (tid == 0) {
+            const int ioff = iend, ibegin = ioff + 1;
+            iend = isplit[jblk - 1];
+            const int in = iend - ioff;
+            sh.iw[2] = 0;
+            if (in == 1) {
+                const double dd = d[ibegin - 1];
+                if (wl >= dd - pivmin) ++nwlb;
+                if (wu >= dd - pivmin) ++nwub;
+                if (irange == kAll || (wl < dd - pivmin && wu >= dd - pivmin)) {
+                    ++mm;
+                    w[mm - 1] = dd; werr[mm - 1] = 0.0; iblock[mm - 1] = jblk; indexw[mm - 1] = 1;
+                }
+            } else {
+                double bgu = d[ibegin - 1], bgl = d[ibegin - 1];
+                for (int j = ibegin; j <= iend; ++j) {
+                    bgl = fmn(bgl, gers[2 * j - 2]);
+                    bgu = fmx(bgu, gers[2 * j - 1]);
+                }
+                bgl = bgl - FUDGE * tnorm * kEps * in - FUDGE * pivmin;
+                bgu = bgu + FUDGE * tnorm * kEps * in + FUDGE * pivmin;
+                bool skip = false;
+                if (irange > 1) {
+                    if (bgu < wl) { nwlb += in; nwub += in; skip = true; }
+                    else {
+                        bgl = fmx(bgl, wl);
+                        bgu = fmn(bgu, wu);
+                        if (bgl >= bgu) skip = true;
+                    }
+                }
+                if (!skip) {
+                    const int c0 = count_T(in, d + ibegin - 1, e2 + ibegin - 1, bgl, pivmin);
+                    const int c1 = count_T(in, d + ibegin - 1, e2 + ibegin - 1, bgu, pivmin);
+                    nwlb += c0;
+                    nwub += c1;
+                    sh.iw[2] = 1;
+                    sh.iw[3] = ibegin;
+                    sh.iw[4] = in;
+                    sh.iw[5] = mm - c0;   // iwoff
+                    sh.i0 = c0;
+                    sh.i1 = c1;
+                    sh.w[0] = bgl;
+                    sh.w[1] = bgu;
+                    mm += c1 - c0;
+                }
+            }
+        }
+        SM_SYNC();
+        if (sh.iw[2]) {
+            const int ibegin = sh.iw[3], in = sh.iw[4], iwoff = sh.iw[5], c0 = sh.i0, c1 = sh.i1;
+            const double bgl = sh.w[0], bgu = sh.w[1];
+            const int itmax = (int)((log(bgu - bgl + pivmin) - log(pivmin)) / log(2.0)) + 2;
+            const double* db = d + ibegin - 1;
+            const double* eb = e2 + ibegin - 1;
+            for (int j = c0 + 1 + tid; j <= c1; j += nt) {
+                double a = bgl, b = bgu, c = 0.5 * (bgl + bgu);
+                int na = c0, nb = c1;
+                bool conv = false;
+                for (int jit = 1; jit <= itmax; ++jit) {
+                    const int cnt = count_T2(in, db, eb, c, pivmin);
+                    const int t1 = imn(nb, imx(na, cnt));
+                    if (t1 == nb) b = c;
+                    else if (t1 == na) a = c;
+                    else if (j <= t1) { b = c; nb = t1; }
+                    else { a = c; na = t1; }
+                    const double w1 = fabs(b - a), w2 = fmx(fabs(b), fabs(a));
+                    if (w1 < fmx(fmx(atoli, pivmin), rtoli * w2) || na >= nb) { conv = true; break; }
+                    c = 0.5 * (a + b);
+                }
+                const int je = j + iwoff;
+                w[je - 1] = 0.5 * (a + b);
+                werr[je - 1] = 0.5 * fabs(a - b);
+                indexw[je - 1] = j;
+                iblock[je - 1] = conv ? jblk : -jblk;
+                if (!conv) sh.iw[7] = 1;
+            }
+        }
+        SM_SYNC();
+    }
+    int info = 0;
+    if (tid == 0) {
+        bool toofew = false;
+        if (irange == kInd) {
+            int idiscl = il - 1 - nwlb, idiscu = nwub - iu;
+            if (idiscl > 0) {
+                int im = 0;
+                for (int je = 1; je <= mm; ++je) {
+                    if (w[je - 1] <= wlu && idiscl > 0) --idiscl;
+                    else {
+                        ++im;
+                        w[im - 1] = w[je - 1]; werr[im - 1] = werr[je - 1];
+                        indexw[im - 1] = indexw[je - 1]; iblock[im - 1] = iblock[je - 1];
+                    }
+                }
+                mm = im;
+            }
+            if (idiscu > 0) {
+                int im = mm + 1;
+                for (int je = mm; je >= 1; --je) {
+                    if (w[je - 1] >= wul && idiscu > 0) --idiscu;
+                    else {
+                        --im;
+                        w[im - 1] = w[je - 1]; werr[im - 1] = werr[je - 1];
+                        indexw[im - 1] = indexw[je - 1]; iblock[im - 1] = iblock[je - 1];
+                    }
+                }
+                int jee = 0;
+                for (int je = im; je <= mm; ++je) {
+                    ++jee;
+                    w[jee - 1] = w[je - 1]; werr[jee - 1] = werr[je - 1];
+                    indexw[jee - 1] = indexw[je - 1]; iblock[jee - 1] = iblock[je - 1];
+                }
+                mm = mm - im + 1;
+            }
+            if (idiscl > 0 || idiscu > 0) {
+                if (idiscl > 0) {
+                    double wkill = wu;
+                    for (int jd = 1; jd <= idiscl; ++jd) {
+                        int iw = 0;
+                        for (int je = 1; je <= mm; ++je)
+                            if (iblock[je - 1] != 0 && (w[je - 1] < wkill || iw == 0)) { iw = je; wkill = w[je - 1]; }
+                        iblock[iw - 1] = 0;
+                    }
+                }
+                if (idiscu > 0) {
+                    double wkill = wl;
+                    for (int jd = 1; jd <= idiscu; ++jd) {
+                        int iw = 0;
+                        for (int je = 1; je <= mm; ++je)
+                            if (iblock[je - 1] != 0 && (w[je - 1] >= wkill || iw == 0)) { iw = je; wkill = w[je - 1]; }
+                        iblock[iw - 1] = 0;
+                    }
+                }
+                int im = 0;
+                for (int je = 1; je <= mm; ++je)
+                    if (iblock[je - 1] != 0) {
+                        ++im;
+                        w[im - 1] = w[je - 1]; werr[im - 1] = werr[je - 1];
+                        indexw[im - 1] = indexw[je - 1]; iblock[im - 1] = iblock[je - 1];
+                    }
+                mm = im;
+            }
+            if (idiscl < 0 || idiscu < 0) toofew = true;
+        }
+        if ((irange == kAll && mm != n) || (irange == kInd && mm != iu - il + 1)) toofew = true;
+        if (sh.iw[7]) info += 1;
+        if (toofew) info += 2;
+        sh.m = mm;
+        sh.wl = wl;
+        sh.wu = wu;
+        sh.info = info;
+    }
+    SM_SYNC();
+    return sh.info;
+}
+
+// ---------------------------------------------------------------------------------
+// dlarre (odrre.f:296-891): splitting, root representation of every block (shift to
+// one end of the spectrum, growth <= 64 spdiam, relative perturbation by the dlarnv
+// stream) and its eigenvalues: dqds for range A (dlasq2), otherwise dlarrd's
+// approximations refined by dlarrb.  Returns INFO; m in sh.
+// ---------------------------------------------------------------------------------
+SM_HD inline int dlarre_team(int tid, int nt, Shared& sh, int irange, int n, double* vl,
+                             double* vu, int il, int iu, double* d, double* e, double* e2,
+                             double rtol1, double rtol2, double spltol, double* w,
+                             double* werr, double* wgap, int* iblock, int* indexw,
+                             int* isplit, double* gers, double* wk) {
+    const int MAXTRY = 6;
+    const double FAC = 0.5, FUDGE = 2.0, PERT = 8.0, MAXGROWTH = 64.0, HNDRD = 100.0;
+    const double rtl = sqrt(kEps), bsrtol = sqrt(kEps);
+    if (tid == 0) {
+        double gl = d[0], gu = d[0], eold = 0.0, emax = 0.0;
+        e[n - 1] = 0.0;
+        for (int i = 0; i < n; ++i) {
+            werr[i] = 0.0;
+            wgap[i] = 0.0;
+            const double eabs = fabs(e[i]);
+            if (eabs >= emax) emax = eabs;
+            const double tmp1 = eabs + eold;
+            gers[2 * i] = d[i] - tmp1;
+            gl = fmn(gl, gers[2 * i]);
+            gers[2 * i + 1] = d[i] + tmp1;
+            gu = fmx(gu, gers[2 * i + 1]);
+            eold = eabs;
+        }
+        sh.pivmin = kSafmin * fmx(1.0, emax * emax);
+        const double spdiam = gu - gl;
+        // dlarra (odrra.f:90-123)
+        int nsplit = 1;
+        if (spltol < 0.0) {
+            const double tmp1 = fabs(spltol) * spdiam;
+            for (int i = 1; i <= n - 1; ++i)
+                if (fabs(e[i - 1]) <= tmp1) {
+                    e[i - 1] = 0.0; e2[i - 1] = 0.0;
+                    isplit[nsplit - 1] = i; ++nsplit;
+                }
+        } else {
+            for (int i = 1; i <= n - 1; ++i)
+                if (fabs(e[i - 1]) <= spltol * sqrt(fabs(d[i - 1])) * sqrt(fabs(d[i]))) {
+                    e[i - 1] = 0.0; e2[i - 1] = 0.0;
+                    isplit[nsplit - 1] = i; ++nsplit;
+                }
+        }
+        isplit[nsplit - 1] = n;
+        sh.nsplit = nsplit;
+        sh.w[0] = gl;
+        sh.w[1] = gu;
+    }
+    SM_SYNC();
+    const double pivmin = sh.pivmin;
+    const int nsplit = sh.nsplit;
+    int mm = 0;
+    if (irange == kAll) {
+        *vl = sh.w[0];
+        *vu = sh.w[1];
+    } else {
+        const int iinfo = dlarrd_team(tid, nt, sh, irange, n, *vl, *vu, il, iu, gers, bsrtol, d,
+                                      e2, pivmin, nsplit, isplit, w, werr, iblock, indexw);
+        if (iinfo != 0) return -1;
+        mm = sh.m;
+        *vl = sh.wl;
+        *vu = sh.wu;
+        for (int i = mm + tid; i < n; i += nt) { w[i] = 0.0; werr[i] = 0.0; iblock[i] = 0; indexw[i] = 0; }
+        SM_SYNC();
+    }
+    // thread 0 keeps the running counts; sh carries each block's decisions
+    int m = 0, ibegin = 1, wbegin = 1;
+    for (int jblk = 1; jblk <= nsplit; ++jblk) {
+        // ---- phase 0: dlarrk for both ends of the block's spectrum (odrre.f:558-575),
+        // one thread each, when the block will need them (dqds)
+        if (tid == 0) {
+            const int iend = isplit[jblk - 1], in = iend - ibegin + 1;
+            int need = 0;
+            if (in > 1) {
+                if (irange == kAll) need = 1;
+                else {
+                    int mb = 0;
+                    for (int i = wbegin; i <= mm; ++i) {
+                        if (iblock[i - 1] == jblk) ++mb;
+                        else break;
+                    }
+                    need = (mb > 0 && mb > FAC * in) ? 1 : 0;
+                }
+            }
+            double gl = d[ibegin - 1], gu = d[ibegin - 1];
+            for (int i = ibegin; i <= iend; ++i) {
+                gl = fmn(gers[2 * i - 2], gl);
+                gu = fmx(gers[2 * i - 1], gu);
+            }
+            sh.iw[6] = need;
+            sh.iw[7] = ibegin;
+            sh.iw[5] = in;
+            sh.w[0] = gl;
+            sh.w[1] = gu;
+        }
+        SM_SYNC();
+        if (sh.iw[6]) {
+            const int ib = sh.iw[7], in = sh.iw[5];
+            const double gl = sh.w[0], gu = sh.w[1];
+            for (int e = tid; e < 2; e += nt)
+                dlarrk(in, e == 0 ? 1 : in, gl, gu, d + ib - 1, e2 + ib - 1, pivmin, rtl,
+                       &sh.ends[2 * e], &sh.ends[2 * e + 1]);
+        }
+        SM_SYNC();
+        // ---- phase 1 (thread 0): shift, root factorisation; what comes next
+        if (tid == 0) {
+            sh.stop = 0;
+            sh.iw[2] = 0;   // 0 nothing, 1 dqds, 2 refine
+            const int iend = isplit[jblk - 1], in = iend - ibegin + 1;
+            if (in == 1) {
+                if (irange == kAll || (irange == kVal && d[ibegin - 1] > *vl && d[ibegin - 1] <= *vu) ||
+                    (irange == kInd && iblock[wbegin - 1] == jblk)) {
+                    ++m;
+                    w[m - 1] = d[ibegin - 1]; werr[m - 1] = 0.0; wgap[m - 1] = 0.0;
+                    iblock[m - 1] = jblk; indexw[m - 1] = 1;
+                    ++wbegin;
+                }
+                e[iend - 1] = 0.0;
+                ibegin = iend + 1;
+            } else {
+                e[iend - 1] = 0.0;
+                double gl = d[ibegin - 1], gu = d[ibegin - 1];
+                for (int i = ibegin; i <= iend; ++i) {
+                    gl = fmn(gers[2 * i - 2], gl);
+                    gu = fmx(gers[2 * i - 1], gu);
+                }
+                double spdiam = gu - gl;
+                int mb = 0, wend = 0, indl = 0, indu = 0;
+                bool usedqd = (irange == kAll);
+                bool skip = false;
+                if (irange != kAll) {
+                    for (int i = wbegin; i <= mm; ++i) {
+                        if (iblock[i - 1] == jblk) ++mb;
+                        else break;
+                    }
+                    if (mb == 0) {
+                        e[iend - 1] = 0.0;
+                        ibegin = iend + 1;
+                        skip = true;
+                    } else {
+                        usedqd = (mb > FAC * in);
+                        wend = wbegin + mb - 1;
+                        for (int i = wbegin; i <= wend - 1; ++i)
+                            wgap[i - 1] = fmx(0.0, w[i] - werr[i] - (w[i - 1] + werr[i - 1]));
+                        wgap[wend - 1] = fmx(0.0, *vu - 0.0 - (w[wend - 1] + werr[wend - 1]));
+                        indl = indexw[wbegin - 1];
+                        indu = indexw[wend - 1];
+                    }
+                }
+                if (!skip) {
+                    double isleft, isrght, tmp, tmp1;
+                    if (irange == kAll || usedqd) {
+                        // the two end eigenvalues (computed by the team just before)
+                        tmp = sh.ends[0]; tmp1 = sh.ends[1];
+                        isleft = fmx(gl, tmp - tmp1 - HNDRD * kEps * fabs(tmp - tmp1));
+                        tmp = sh.ends[2]; tmp1 = sh.ends[3];
+                        isrght = fmn(gu, tmp + tmp1 + HNDRD * kEps * fabs(tmp + tmp1));
+                        spdiam = isrght - isleft;
+                    } else {
+                        isleft = fmx(gl, w[wbegin - 1] - werr[wbegin - 1] -
+                                             HNDRD * kEps * fabs(w[wbegin - 1] - werr[wbegin - 1]));
+                        isrght = fmn(gu, w[wend - 1] + werr[wend - 1] +
+                                             HNDRD * kEps * fabs(w[wend - 1] + werr[wend - 1]));
+                    }
+                    double s1, s2;
+                    if (irange == kAll) {
+                        usedqd = true;
+                        indl = 1; indu = in; mb = in; wend = wbegin + mb - 1;
+                        s1 = isleft + 0.25 * spdiam;
+                        s2 = isrght - 0.25 * spdiam;
+                    } else if (usedqd) {
+                        s1 = isleft + 0.25 * spdiam;
+                        s2 = isrght - 0.25 * spdiam;
+                    } else {
+                        tmp = fmn(isrght, *vu) - fmx(isleft, *vl);
+                        s1 = fmx(isleft, *vl) + 0.25 * tmp;
+                        s2 = fmn(isrght, *vu) - 0.25 * tmp;
+                    }
+                    int cnt1 = 0, cnt2 = 0;
+                    if (mb > 1) dlarrc_T(in, s1, s2, d + ibegin - 1, e + ibegin - 1, &cnt1, &cnt2);
+                    double sigma, sgndef;
+                    if (mb == 1) { sigma = gl; sgndef = 1.0; }
+                    else if (cnt1 - indl >= indu - cnt2) {
+                        if (irange == kAll) sigma = fmx(isleft, gl);
+                        else if (usedqd) sigma = isleft;
+                        else sigma = fmx(isleft, *vl);
+                        sgndef = 1.0;
+                    } else {
+                        if (irange == kAll) sigma = fmn(isrght, gu);
+                        else if (usedqd) sigma = isrght;
+                        else sigma = fmn(isrght, *vu);
+                        sgndef = -1.0;
+                    }
+                    double tau;
+                    if (usedqd) {
+                        tau = spdiam * kEps * n + 2.0 * pivmin;
+                        tau = fmx(tau, 2.0 * kEps * fabs(sigma));
+                    } else if (mb > 1) {
+                        const double clwdth = w[wend - 1] + werr[wend - 1] - w[wbegin - 1] - werr[wbegin - 1];
+                        const double avgap = fabs(clwdth / (double)(wend - wbegin));
+                        if (sgndef == 1.0) { tau = 0.5 * fmx(wgap[wbegin - 1], avgap); tau = fmx(tau, werr[wbegin - 1]); }
+                        else { tau = 0.5 * fmx(wgap[wend - 2], avgap); tau = fmx(tau, werr[wend - 1]); }
+                    } else tau = werr[wbegin - 1];
+                    bool ok = false;
+                    for (int idum = 1; idum <= MAXTRY; ++idum) {
+                        double dpivot = d[ibegin - 1] - sigma;
+                        wk[0] = dpivot;
+                        double dmax = fabs(wk[0]);
+                        int j = ibegin;
+                        for (int i = 1; i <= in - 1; ++i) {
+                            wk[2 * in + i - 1] = 1.0 / wk[i - 1];
+                            tmp = e[j - 1] * wk[2 * in + i - 1];
+                            wk[in + i - 1] = tmp;
+                            dpivot = (d[j] - sigma) - tmp * e[j - 1];
+                            wk[i] = dpivot;
+                            dmax = fmx(dmax, fabs(dpivot));
+                            ++j;
+                        }
+                        bool norep = (dmax > MAXGROWTH * spdiam);
+                        if (usedqd && !norep)
+                            for (int i = 1; i <= in; ++i)
+                                if (sgndef * wk[i - 1] < 0.0) norep = true;
+                        if (norep) {
+                            if (idum == MAXTRY - 1) {
+                                if (sgndef == 1.0) sigma = gl - FUDGE * spdiam * kEps * n - FUDGE * 2.0 * pivmin;
+                                else sigma = gu + FUDGE * spdiam * kEps * n + FUDGE * 2.0 * pivmin;
+                            } else {
+                                sigma = sigma - sgndef * tau;
+                                tau = 2.0 * tau;
+                            }
+                        } else { ok = true; break; }
+                    }
+                    if (!ok) sh.stop = 2;
+                    else {
+                        e[iend - 1] = sigma;
+                        for (int i = 0; i < in; ++i) d[ibegin - 1 + i] = wk[i];
+                        for (int i = 0; i < in - 1; ++i) e[ibegin - 1 + i] = wk[in + i];
+                        sh.iw[2] = usedqd ? 1 : 2;
+                        sh.iw[3] = ibegin; sh.iw[4] = in; sh.iw[5] = mb;
+                        sh.ibeg = wbegin; sh.wbeg = wend;
+                        sh.i0 = indl; sh.i1 = indu;
+                        sh.w[0] = sigma; sh.w[1] = sgndef; sh.w[2] = spdiam;
+                    }
+                }
+            }
+        }
+        SM_SYNC();
+        if (sh.stop) return sh.stop;
+        if (sh.iw[2] == 0) continue;
+        // ---- phase 2 (team): random relative perturbation (odrre.f:767-776)
+        {
+            const int ib = sh.iw[3], in = sh.iw[4], mb = sh.iw[5];
+            if (mb > 1) {
+                for (int i = 1 + tid; i <= in - 1; i += nt) {
+                    d[ib + i - 2] = d[ib + i - 2] * (1.0 + kEps * PERT * dlarnv_entry(i - 1));
+                    e[ib + i - 2] = e[ib + i - 2] * (1.0 + kEps * PERT * dlarnv_entry(in + i - 1));
+                }
+                if (tid == 0) d[ib + in - 2] = d[ib + in - 2] * (1.0 + kEps * 4.0 * dlarnv_entry(in - 1));
+            }
+        }
+        SM_SYNC();
+        // ---- phase 3: eigenvalues
+        if (sh.iw[2] == 2) {
+            const int ib = sh.iw[3], in = sh.iw[4], wb = sh.ibeg, we = sh.wbeg;
+            const int indl = sh.i0, indu = sh.i1;
+            const double sigma = sh.w[0], spdiam = sh.w[2];
+            const int iend = ib + in - 1;
+            for (int j = wb + tid; j <= we; j += nt) {
+                w[j - 1] = w[j - 1] - sigma;
+                werr[j - 1] = werr[j - 1] + fabs(w[j - 1]) * kEps;
+            }
+            for (int i = ib + tid; i <= iend - 1; i += nt) wk[i - 1] = d[i - 1] * (e[i - 1] * e[i - 1]);
+            SM_SYNC();
+            dlarrb_team(tid, nt, in, d + ib - 1, wk + ib - 1, indl, indu, rtol1, rtol2, indl - 1,
+                        w + wb - 1, wgap + wb - 1, werr + wb - 1, pivmin, spdiam, in);
+            if (tid == 0) {
+                wgap[we - 1] = fmx(0.0, (*vu - sigma) - (w[we - 1] + werr[we - 1]));
+                for (int i = indl; i <= indu; ++i) {
+                    ++m;
+                    iblock[m - 1] = jblk;
+                    indexw[m - 1] = i;
+                }
+                ibegin = iend + 1;
+                wbegin = we + 1;
+            }
+        } else {
+            if (tid == 0) {
+                const int ib = sh.iw[3], in = sh.iw[4], mb = sh.iw[5], we = sh.wbeg;
+                const int indl = sh.i0, indu = sh.i1;
+                const double sigma = sh.w[0], sgndef = sh.w[1];
+                const int iend = ib + in - 1;
+                const double rtol = log((double)in) * 4.0 * kEps;
+                int j = ib;
+                for (int i = 1; i <= in - 1; ++i) {
+                    wk[2 * i - 2] = fabs(d[j - 1]);
+                    wk[2 * i - 1] = e[j - 1] * e[j - 1] * wk[2 * i - 2];
+                    ++j;
+                }
+                wk[2 * in - 2] = fabs(d[iend - 1]);
+                wk[2 * in - 1] = 0.0;
+                int iinfo = dlasq2(in, wk);
+                if (iinfo != 0) sh.stop = -5;
+                else {
+                    for (int i = 1; i <= in; ++i)
+                        if (wk[i - 1] < 0.0) sh.stop = -6;
+                }
+                if (!sh.stop) {
+                    if (sgndef > 0.0) {
+                        for (int i = indl; i <= indu; ++i) {
+                            ++m;
+                            w[m - 1] = wk[in - i];
+                            iblock[m - 1] = jblk;
+                            indexw[m - 1] = i;
+                        }
+                    } else {
+                        for (int i = indl; i <= indu; ++i) {
+                            ++m;
+                            w[m - 1] = -wk[i - 1];
+                            iblock[m - 1] = jblk;
+                            indexw[m - 1] = i;
+                        }
+                    }
+                    for (int i = m - mb + 1; i <= m; ++i) werr[i - 1] = rtol * fabs(w[i - 1]);
+                    for (int i = m - mb + 1; i <= m - 1; ++i)
+                        wgap[i - 1] = fmx(0.0, w[i] - werr[i] - (w[i - 1] + werr[i - 1]));
+                    wgap[m - 1] = fmx(0.0, (*vu - sigma) - (w[m - 1] + werr[m - 1]));
+                }
+                ibegin = iend + 1;
+                wbegin = we + 1;
+            }
+        }
+        SM_SYNC();
+        if (sh.stop) return sh.stop;
+    }
+    if (tid == 0) sh.m = m;
+    SM_SYNC();
+    return 0;
+}
+
+
+// ---------------------------------------------------------------------------------
+// dlarrv (odrrv.f:150-896) for the whole spectrum of every block (DOL = 1, DOU = M as
+// dstemr calls it).  The reference walks the representation tree breadth first and
+// treats the children of a node one after the other: an eigenvector (singleton) or a
+// new representation (cluster).  A child reads the gap on its left, which the
+// singleton before it may have widened when it finished (odrrv.f:856-866), and nothing
+// else that its siblings change.  Here the children of a node run side by side, one
+// per thread, each on the gap it would see if no sibling had changed anything; the
+// first thread then replays the node's sequence of gap updates with the results,
+// finds the children whose input gap differs from the one they ran with, and those
+// run again with the right one.  A child's result depends on nothing else, so when
+// no input differs every child holds the result of the sequential order, bit for
+// bit.  Usually one rerun suffices; the first child is always right, so the k-th is
+// right after at most k reruns.
+// ---------------------------------------------------------------------------------
+struct VNode {
+    int ibegin, in, im, wbegin, wend;
+    double sigma, spdiam, pivmin, vl;
+};
+
+struct VArr {
+    double *w, *werr, *wgap, *wsh;
+    const int* indexw;
+    double* z;
+    int ldz;
+    int *isuppz, *twist;
+    const double *snw, *sne, *sns, *sng;
+};
+
+// The singleton child k (block-relative) with left/right gap minimum `gap`
+// (odrrv.f:620-855): RQI with twisted factorisations, bisection fallback, support,
+// normalisation.  Writes the eigenvector column, its support and its own W, WERR,
+// WSH; the gap updates are left to the replay.  The gap enters only the support cuts
+// (through gaptol) and the RQI's stopping test (resid > tol * gap); cert[0..3]
+// receive what decides them (largest cut value, smallest uncut value, smallest
+// residual that went on, largest residual that stopped), so that any other gap that
+// decides them all alike would give the same bits.  cert[4] = 1 if a NaN kept them
+// from being recorded.
+SM_HD inline int sng_compute(const VNode& v, const VArr& a, const double* D0, const double* L0,
+                             const double* LD0, const double* LLD0, int k, double gap,
+                             double* wk, int ws, double* cert) {
+    const int MAXITR = 10;
+    const double rqtol = 2.0 * kEps;
+    const int x = v.wbegin + k - 1, in = v.in, oldien = v.ibegin - 1;
+    a.wsh[x - 1] = a.sns[x - 1];
+    a.werr[x - 1] = a.sne[x - 1];
+    a.twist[x - 1] = 0;
+    double* zc = a.z + (long long)(x - 1) * a.ldz + oldien;
+    for (int i = 0; i < in; ++i) zc[i] = 0.0;
+    const double tol = 4.0 * log((double)in) * kEps;
+    double lambda = a.wsh[x - 1];
+    double left = a.wsh[x - 1] - a.werr[x - 1], right = a.wsh[x - 1] + a.werr[x - 1];
+    const int indeig = a.indexw[x - 1];
+    const double gaptol = (k == 1 || k == v.im) ? 0.0 : gap * kEps;
+    int isupmn = in, isupmx = 1, iter = 0, negcnt = 0;
+    bool usedbs = false, usedrq = false, needbs = false;
+    double ztz = 0, mingma = 0, nrminv = 0, resid = 0, rqcorr = 0, bstres = 0, bstw = 0;
+    int* sup = a.isuppz + 2 * (x - 1);
+    double cutmax = -INFINITY, noncutmin = INFINITY, rcont = INFINITY, rstop = -INFINITY;
+    bool nanseen = false;
+    for (;;) {
+        if (needbs) {
+            // dlarrb with RTOL1 = 0: the gaps it is given play no part
+            usedbs = true;
+            const int itmp1 = a.twist[x - 1];
+            dlarrb1(in, D0, LLD0, indeig, 0.0, 2.0 * kEps, gap, gap, &a.wsh[x - 1], &a.werr[x - 1],
+                    v.pivmin, v.spdiam, itmp1);
+            lambda = a.wsh[x - 1];
+            a.twist[x - 1] = 0;
+        }
+        dlar1v(in, 1, in, lambda, D0, L0, LD0, LLD0, v.pivmin, gaptol, zc, !usedbs, &negcnt, &ztz,
+               &mingma, &a.twist[x - 1], sup, &nrminv, &resid, &rqcorr, wk, ws, &cutmax, &noncutmin);
+        if (iter == 0) { bstres = resid; bstw = lambda; }
+        else if (resid < bstres) { bstres = resid; bstw = lambda; }
+        isupmn = imn(isupmn, sup[0]);
+        isupmx = imx(isupmx, sup[1]);
+        ++iter;
+        if (isnan(resid)) nanseen = true;
+        if (fabs(rqcorr) > rqtol * fabs(lambda) && !usedbs) {
+            if (resid > tol * gap) rcont = fmn(rcont, resid);
+            else rstop = fmx(rstop, resid);
+        }
+        if (resid > tol * gap && fabs(rqcorr) > rqtol * fabs(lambda) && !usedbs) {
+            const double sgndef = (indeig <= negcnt) ? -1.0 : 1.0;
+            if (rqcorr * sgndef >= 0.0 && lambda + rqcorr <= right && lambda + rqcorr >= left) {
+                usedrq = true;
+                if (sgndef == 1.0) left = lambda;
+                else right = lambda;
+                a.wsh[x - 1] = 0.5 * (right + left);
+                lambda = lambda + rqcorr;
+                a.werr[x - 1] = 0.5 * (right - left);
+            } else needbs = true;
+            if (right - left < rqtol * fabs(lambda)) { usedbs = true; continue; }
+            else if (iter < MAXITR) continue;
+            else if (iter == MAXITR) { needbs = true; continue; }
+            return 5;
+        }
+        if (usedrq && usedbs && bstres <= resid) {
+            lambda = bstw;
+            dlar1v(in, 1, in, lambda, D0, L0, LD0, LLD0, v.pivmin, gaptol, zc, !usedbs, &negcnt, &ztz,
+                   &mingma, &a.twist[x - 1], sup, &nrminv, &resid, &rqcorr, wk, ws, &cutmax, &noncutmin);
+        }
+        a.wsh[x - 1] = lambda;
+        break;
+    }
+    sup[0] += oldien;
+    sup[1] += oldien;
+    const int zfrom = sup[0], zto = sup[1];
+    isupmn += oldien;
+    isupmx += oldien;
+    double* zcol = a.z + (long long)(x - 1) * a.ldz;   // 1-based rows: zcol[ii - 1]
+    for (int ii = isupmn; ii <= zfrom - 1; ++ii) zcol[ii - 1] = 0.0;
+    for (int ii = zto + 1; ii <= isupmx; ++ii) zcol[ii - 1] = 0.0;
+    for (int ii = zfrom; ii <= zto; ++ii) zcol[ii - 1] = nrminv * zcol[ii - 1];
+    a.w[x - 1] = lambda + v.sigma;
+    cert[0] = cutmax;
+    cert[1] = noncutmin;
+    cert[2] = rcont;
+    cert[3] = rstop;
+    cert[4] = (nanseen || isnan(cutmax) || isnan(noncutmin)) ? 1.0 : 0.0;
+    return 0;
+}
+
+// True when a singleton child that ran with one gap would take exactly the same path,
+// and so give the same bits, with gap g (see sng_compute).
+SM_HD inline bool sng_same_path(const VNode& v, int k, double g, const double* cert) {
+    if (cert[4] != 0.0) return false;
+    const double gaptol = (k == 1 || k == v.im) ? 0.0 : g * kEps;
+    const double tol = 4.0 * log((double)v.in) * kEps;
+    return cert[0] < gaptol && gaptol <= cert[1] && tol * g < cert[2] && cert[3] <= tol * g;
+}
+
+// The cluster child f..l (block-relative) with left gap `lgap` (odrrv.f:498-611): its
+// end eigenvalues refined to full accuracy, a new representation by dlarrf stored in
+// Z's columns f and f+1 with the accumulated shift, shifted values and fudged errors.
+SM_HD inline int cl_compute(const VNode& v, const VArr& a, const double* D0, const double* L0,
+                            const double* LD0, const double* LLD0, int f, int l, double lgap,
+                            double* wk, int ws) {
+    const double rqtol = 2.0 * kEps;
+    const int wb = v.wbegin, in = v.in;
+    for (int k = f; k <= l; ++k) {
+        a.wsh[wb + k - 2] = a.sns[wb + k - 2];
+        a.werr[wb + k - 2] = a.sne[wb + k - 2];
+    }
+    double* WS = a.wsh + wb - 1;
+    double* WE = a.werr + wb - 1;
+    const double* SG = a.sng + wb - 1;   // the node's gaps: only its internal ones are read
+    const double rgap = SG[l - 1];
+    for (int kk = 1; kk <= 2; ++kk) {
+        const int pk = (kk == 1) ? f : l;
+        const double rg = SG[pk - 1];
+        const double lg = (pk > 1) ? ((pk == f) ? lgap : SG[pk - 2]) : rg;
+        dlarrb1(in, D0, LLD0, a.indexw[wb + pk - 2], rqtol, rqtol, rg, fmn(lg, rg), &WS[pk - 1],
+                &WE[pk - 1], v.pivmin, v.spdiam, in);
+    }
+    const int newftt = wb + f - 1;
+    double tau;
+    double* zs = a.z + (long long)(newftt - 1) * a.ldz + (v.ibegin - 1);
+    const int iinfo = dlarrf(in, D0, L0, LD0, f, l, WS, SG, WE, v.spdiam, lgap, rgap, v.pivmin, &tau,
+                             zs, zs + a.ldz, wk, ws);
+    if (iinfo != 0) return -2;
+    zs[a.ldz + in - 1] = v.sigma + tau;
+    for (int k = f; k <= l; ++k) {
+        double fudge = 3.0 * kEps * fabs(WS[k - 1]);
+        WS[k - 1] = WS[k - 1] - tau;
+        fudge = fudge + 4.0 * kEps * fabs(WS[k - 1]);
+        WE[k - 1] = WE[k - 1] + fudge;
+    }
+    return 0;
+}
+
+// The node's sequence of gap updates replayed with the children's current results
+// (one thread): the input each child should have run with, compared with the one it
+// ran with.  Returns the number of children to run again.
+SM_HD inline int node_replay(const VNode& v, const VArr& a, int oldfst, int oldlst, int nch,
+                             const int* ch, double* inp, int* dirty, const double* cert) {
+    const int wb = v.wbegin;
+    for (int j = imx(oldfst - 1, 1); j <= oldlst; ++j) a.wgap[wb + j - 2] = a.sng[wb + j - 2];
+    int ndirty = 0;
+    for (int c = 0; c < nch; ++c) {
+        const int f = ch[2 * c], l = ch[2 * c + 1];
+        double in_true;
+        if (f == l) {
+            const int k = f, x = wb + k - 1;
+            const double left0 = a.sns[x - 1] - a.sne[x - 1], right0 = a.sns[x - 1] + a.sne[x - 1];
+            const double lg = (k == 1) ? kEps * fmx(fabs(left0), fabs(right0)) : a.wgap[x - 2];
+            const double rg = (k == v.im) ? kEps * fmx(fabs(left0), fabs(right0)) : a.wgap[x - 1];
+            in_true = fmn(lg, rg);
+            // odrrv.f:856-866: widen the gaps on both sides, left neighbour as finished,
+            // right neighbour as it was before this node's children ran
+            const double wx = a.w[x - 1], ex = a.werr[x - 1];
+            if (k > 1) a.wgap[x - 2] = fmx(a.wgap[x - 2], wx - ex - a.w[x - 2] - a.werr[x - 2]);
+            if (x < v.wend) {
+                const bool inside = (k + 1 <= oldlst);
+                const double wn = inside ? a.snw[x] : a.w[x];
+                const double en = inside ? a.sne[x] : a.werr[x];
+                a.wgap[x - 1] = fmx(a.sng[x - 1], wn - en - wx - ex);
+            }
+        } else {
+            in_true = (f == 1) ? fmx(0.0, a.snw[wb - 1] - a.sne[wb - 1] - v.vl) : a.wgap[wb + f - 3];
+        }
+        if (!(in_true == inp[c])) {
+            const bool same = (f == l) && !isnan(inp[c]) && sng_same_path(v, f, in_true, cert + 5 * c);
+            inp[c] = in_true;
+            if (!same) {
+                dirty[c] = 1;
+                ++ndirty;
+            }
+        }
+    }
+    return ndirty;
+}
+
+// d, l, ld, lld: the block's current representation (the root from dlarre, later ones
+// restored from Z where the reference stores them).  wk: 4n + 2 doubles for thread 0.
+SM_HD inline int dlarrv_team(int tid, int nt, Shared& sh, int n, double vl, double* d,
+                             double* l, double* ld, double* lld, double pivmin,
+                             const int* isplit, int m, double rtol1, double rtol2, double* w,
+                             double* werr, double* wgap, double* wsh, const int* iblock,
+                             const int* indexw, const double* gers, double* z, int ldz,
+                             int* isuppz, int* twist, int* cl1, int* cl2, const Work& wv) {
+#define ZZ(i, j) z[(long long)((i) - 1) + (long long)((j) - 1) * (long long)ldz]
+    const double minrgp = 1.0e-3;
+    for (long long idx = tid; idx < (long long)n * m; idx += nt) {
+        const long long j = idx / n, i = idx - j * n;
+        z[i + j * (long long)ldz] = 0.0;
+    }
+    for (int i = tid; i < m; i += nt) twist[i] = 0;
+    if (tid == 0) sh.info = 0;
+    SM_SYNC();
+    VArr va;
+    va.w = w; va.werr = werr; va.wgap = wgap; va.wsh = wsh; va.indexw = indexw; va.z = z;
+    va.ldz = ldz; va.isuppz = isuppz; va.twist = twist;
+    va.snw = wv.SNW; va.sne = wv.SNE; va.sns = wv.SNS; va.sng = wv.SNG;
+    const int nblk = (m > 0) ? iblock[m - 1] : 0;
+    int ibegin = 1, wbegin = 1;          // thread 0's running state
+    double sigma = 0.0;
+    int ndepth = 0, parity = 1, nclus = 0, idone = 0;
+    for (int jblk = 1; jblk <= nblk; ++jblk) {
+        if (tid == 0) {
+            sh.lv[0] = 0;   // block active
+            const int iend = isplit[jblk - 1];
+            sigma = l[iend - 1];
+            int wend = wbegin - 1;
+            while (wend < m && iblock[wend] == jblk) ++wend;
+            if (wend < wbegin) {
+                ibegin = iend + 1;
+            } else {
+                double gl = gers[2 * ibegin - 2], gu = gers[2 * ibegin - 1];
+                for (int i = ibegin + 1; i <= iend; ++i) {
+                    gl = fmn(gers[2 * i - 2], gl);
+                    gu = fmx(gers[2 * i - 1], gu);
+                }
+                const int in = iend - ibegin + 1, im = wend - wbegin + 1;
+                if (ibegin == iend) {
+                    ZZ(ibegin, wbegin) = 1.0;
+                    isuppz[2 * wbegin - 2] = ibegin;
+                    isuppz[2 * wbegin - 1] = ibegin;
+                    w[wbegin - 1] = w[wbegin - 1] + sigma;
+                    wsh[wbegin - 1] = w[wbegin - 1];
+                    ibegin = iend + 1;
+                    wbegin = wbegin + 1;
+                } else {
+                    for (int i = 1; i <= im; ++i) wsh[wbegin + i - 2] = w[wbegin + i - 2];
+                    for (int i = 1; i <= im; ++i) w[wbegin + i - 2] = w[wbegin + i - 2] + sigma;
+                    ndepth = 0;
+                    parity = 1;
+                    nclus = 1;
+                    cl1[1] = 1;
+                    cl1[2] = im;
+                    idone = 0;
+                    sh.lv[0] = 1;
+                    sh.lv[1] = in;
+                    sh.lv[2] = im;
+                    sh.lv[3] = wend;
+                    sh.spdiam = gu - gl;
+                }
+            }
+        }
+        SM_SYNC();
+        if (!sh.lv[0]) continue;
+        const int in = sh.lv[1], im = sh.lv[2], wend = sh.lv[3];
+        const double spdiam = sh.spdiam;
+        for (;;) {
+            if (tid == 0) {
+                sh.lv[4] = (idone < im) ? 1 : 0;
+                if (idone < im) {
+                    if (ndepth > m) { sh.info = -2; sh.lv[4] = 0; }
+                    sh.lv[5] = nclus;            // nodes on this level
+                    sh.lv[6] = ndepth;
+                    nclus = 0;
+                    parity = 1 - parity;
+                }
+            }
+            SM_SYNC();
+            if (sh.info) return sh.info;
+            if (!sh.lv[4]) break;
+            const int oldncl = sh.lv[5], depth = sh.lv[6];
+            for (int ic = 1; ic <= oldncl; ++ic) {
+                // node setup (thread 0): restore its representation, D*L and D*L*L
+                if (tid == 0) {
+                    int* oldcl = (parity == 0) ? cl1 : cl2;
+                    const int oldfst = oldcl[2 * ic - 1], oldlst = oldcl[2 * ic];
+                    const int iend = ibegin + in - 1;
+                    if (ndepth > 0) {
+                        const int j = wbegin + oldfst - 1;
+                        for (int k = 0; k < in; ++k) d[ibegin - 1 + k] = ZZ(ibegin + k, j);
+                        for (int k = 0; k < in - 1; ++k) l[ibegin - 1 + k] = ZZ(ibegin + k, j + 1);
+                        sigma = ZZ(iend, j + 1);
+                        for (int k = 0; k < in; ++k) { ZZ(ibegin + k, j) = 0.0; ZZ(ibegin + k, j + 1) = 0.0; }
+                    }
+                    for (int j = ibegin; j <= iend - 1; ++j) {
+                        const double tmp = d[j - 1] * l[j - 1];
+                        ld[j - 1] = tmp;
+                        lld[j - 1] = tmp * l[j - 1];
+                    }
+                    sh.i0 = indexw[wbegin - 1 + oldfst - 1];
+                    sh.i1 = indexw[wbegin - 1 + oldlst - 1];
+                    sh.offset = indexw[wbegin - 1] - 1;
+                    sh.ibeg = ibegin;
+                    sh.wbeg = wbegin;
+                    sh.lv[7] = oldfst * 65536 + oldlst;
+                }
+                SM_SYNC();
+                if (depth > 0) {
+                    const int b0 = sh.ibeg, wb0 = sh.wbeg;
+                    dlarrb_team(tid, nt, in, d + b0 - 1, lld + b0 - 1, sh.i0, sh.i1, rtol1, rtol2,
+                                sh.offset, wsh + wb0 - 1, wgap + wb0 - 1, werr + wb0 - 1, pivmin,
+                                spdiam, in);
+                }
+                // the node's children (thread 0): odrrv.f:453-485 and the split into
+                // singletons and clusters; the node's values as they are before them
+                if (tid == 0) {
+                    const int oldfst = sh.lv[7] / 65536, oldlst = sh.lv[7] % 65536;
+                    double* W_ = w + wbegin - 1;      // 0-based within the block
+                    double* WE = werr + wbegin - 1;
+                    double* WG = wgap + wbegin - 1;
+                    double* WS = wsh + wbegin - 1;
+                    if (ndepth > 0) {
+                        if (oldfst > 1)
+                            WG[oldfst - 2] = fmx(WG[oldfst - 2], W_[oldfst - 1] - WE[oldfst - 1] -
+                                                                     W_[oldfst - 2] - WE[oldfst - 2]);
+                        if (wbegin + oldlst - 1 < wend)
+                            WG[oldlst - 1] = fmx(WG[oldlst - 1], W_[oldlst] - WE[oldlst] -
+                                                                     W_[oldlst - 1] - WE[oldlst - 1]);
+                        for (int j = oldfst; j <= oldlst; ++j) W_[j - 1] = WS[j - 1] + sigma;
+                    }
+                    int nch = 0, newfst = oldfst;
+                    for (int j = oldfst; j <= oldlst; ++j) {
+                        if (!(j == oldlst || WG[j - 1] >= minrgp * fabs(WS[j - 1]))) continue;
+                        wv.CH[2 * nch] = newfst;
+                        wv.CH[2 * nch + 1] = j;
+                        wv.INP[nch] = NAN;
+                        wv.DIRTY[nch] = 0;
+                        ++nch;
+                        newfst = j + 1;
+                    }
+                    for (int j = imx(oldfst - 1, 1); j <= oldlst; ++j) {
+                        const int x = wbegin + j - 1;
+                        wv.SNW[x - 1] = w[x - 1];
+                        wv.SNE[x - 1] = werr[x - 1];
+                        wv.SNS[x - 1] = wsh[x - 1];
+                        wv.SNG[x - 1] = wgap[x - 1];
+                    }
+                    sh.iw[0] = nch;
+                    sh.iw[1] = oldfst;
+                    sh.iw[2] = oldlst;
+                    sh.w[0] = sigma;
+                }
+                SM_SYNC();
+                VNode vn;
+                vn.ibegin = sh.ibeg; vn.in = in; vn.im = im; vn.wbegin = sh.wbeg; vn.wend = wend;
+                vn.sigma = sh.w[0]; vn.spdiam = spdiam; vn.pivmin = pivmin; vn.vl = vl;
+                const int nch = sh.iw[0], ofst = sh.iw[1], olst = sh.iw[2];
+                const double* D0 = d + vn.ibegin - 1;
+                const double* L0 = l + vn.ibegin - 1;
+                const double* LD0 = ld + vn.ibegin - 1;
+                const double* LLD0 = lld + vn.ibegin - 1;
+                double* wkt = wv.WKL + tid;
+                for (;;) {
+                    if (tid == 0) {
+                        sh.lv[4] = node_replay(vn, va, ofst, olst, nch, wv.CH, wv.INP, wv.DIRTY, wv.CERT);
+                        if (sh.lv[4]) ++sh.passes;
+                    }
+                    SM_SYNC();
+                    if (sh.lv[4] == 0) break;
+                    for (int c = tid; c < nch; c += nt) {
+                        if (!wv.DIRTY[c]) continue;
+                        wv.DIRTY[c] = 0;
+                        const int f = wv.CH[2 * c], lc = wv.CH[2 * c + 1];
+                        const int rc = (f == lc) ? sng_compute(vn, va, D0, L0, LD0, LLD0, f, wv.INP[c], wkt, wv.WKS,
+                                                               wv.CERT + 5 * c)
+                                                 : cl_compute(vn, va, D0, L0, LD0, LLD0, f, lc, wv.INP[c], wkt, wv.WKS);
+                        if (rc) sh.info = rc;
+                    }
+                    SM_SYNC();
+                    if (sh.info) return sh.info;
+                }
+                if (tid == 0) {
+                    int* newcl = (parity == 0) ? cl2 : cl1;
+                    for (int c = 0; c < nch; ++c) {
+                        const int f = wv.CH[2 * c], lc = wv.CH[2 * c + 1];
+                        if (f == lc) ++idone;
+                        else {
+                            ++nclus;
+                            newcl[2 * nclus - 1] = f;
+                            newcl[2 * nclus] = lc;
+                        }
+                    }
+                }
+                SM_SYNC();
+            }
+            if (tid == 0) ++ndepth;
+            SM_SYNC();
+        }
+        if (tid == 0) {
+            ibegin = ibegin + in;
+            wbegin = wend + 1;
+        }
+        SM_SYNC();
+    }
+    return 0;
+#undef ZZ
+}
+
+// dlarrr (odrrr.f:60-121): 0 when the matrix warrants relative accuracy.
+SM_HD inline int dlarrr(int n, const double* d, const double* e) {
+    const double RELCOND = 0.999;
+    const double smlnum = kSafmin / kEps, rmin = sqrt(smlnum);
+    double offdig = 0.0, tmp = sqrt(fabs(d[0]));
+    if (tmp < rmin) return 1;
+    for (int i = 2; i <= n; ++i) {
+        const double tmp2 = sqrt(fabs(d[i - 1]));
+        if (tmp2 < rmin) return 1;
+        const double offdig2 = fabs(e[i - 2]) / (tmp * tmp2);
+        if (offdig + offdig2 >= RELCOND) return 1;
+        tmp = tmp2;
+        offdig = offdig2;
+    }
+    return 0;
+}
+
+// ---------------------------------------------------------------------------------
+// dstemr (odstmr.f:233-647) as mrrr() calls it (NZC = n, or the count query of range V
+// for jobz C).  VL, VU, IL, IU are inputs only: dstemr leaves the caller's copies as
+// they were.  D and E are read from the caller and worked on in wv.D, wv.E.
+// ---------------------------------------------------------------------------------
+SM_HD inline Result dstemr_team(int tid, int nt, Shared& sh, const Prob& p, const Work& wv) {
+    const int n = p.n;
+    Result res;
+    res.m = 0;
+    res.info = 0;
+    res.tryrac = p.tryrac;
+    double* D = wv.D;
+    double* E = wv.E;
+    if (p.cnt) {   // odstmr.f:321-336 with NZC = -1 (mrrr.c:414-419)
+        if (tid == 0) {
+            int lc, rc;
+            dlarrc_T(n, p.vl, p.vu, p.Din, p.Ein, &lc, &rc);
+            sh.m = rc - lc;
+        }
+        SM_SYNC();
+        res.m = sh.m;
+        return res;
+    }
+    for (int i = tid; i < n; i += nt) {
+        D[i] = p.Din[i];
+        E[i] = (i < n - 1) ? p.Ein[i] : 0.0;
+    }
+    SM_SYNC();
+    double wl = p.range == kVal ? p.vl : 0.0, wu = p.range == kVal ? p.vu : 0.0;
+    const int iil = p.range == kInd ? p.il : 0, iiu = p.range == kInd ? p.iu : 0;
+    if (n <= 2) {   // odstmr.f:357-430
+        if (tid == 0) {
+            int m = 0;
+            if (n == 1) {
+                if (p.range == kAll || p.range == kInd) { m = 1; p.W[0] = D[0]; }
+                else if (wl < D[0] && wu >= D[0]) { m = 1; p.W[0] = D[0]; }
+                if (p.wantz) { p.Z[0] = 1.0; p.Zsupp[0] = 1; p.Zsupp[1] = 1; }
+            } else {
+                double r1, r2, cs, sn;
+                dlaev2(D[0], E[0], D[1], &r1, &r2, &cs, &sn);
+                for (int pass = 0; pass < 2; ++pass) {
+                    const double r = pass == 0 ? r2 : r1;
+                    const bool take = p.range == kAll || (p.range == kVal && r > wl && r <= wu) ||
+                                      (p.range == kInd && (pass == 0 ? iil == 1 : iiu == 2));
+                    if (!take) continue;
+                    ++m;
+                    p.W[m - 1] = r;
+                    if (p.wantz) {
+                        double* zc = p.Z + (long long)(m - 1) * p.ldz;
+                        zc[0] = pass == 0 ? -sn : cs;
+                        zc[1] = pass == 0 ? cs : sn;
+                        int a, b;
+                        if (sn != 0.0) { if (cs != 0.0) { a = 1; b = 2; } else { a = 1; b = 1; } }
+                        else { a = 2; b = 2; }
+                        p.Zsupp[2 * m - 2] = a;
+                        p.Zsupp[2 * m - 1] = b;
+                    }
+                }
+            }
+            sh.m = m;
+        }
+        SM_SYNC();
+        res.m = sh.m;
+        return res;
+    }
+    // scaling (odstmr.f:452-469), relative-accuracy test (:479-493)
+    const double smlnum = kSafmin / kEps, bignum = 1.0 / smlnum;
+    const double rmin = sqrt(smlnum), rmax = fmn(sqrt(bignum), 1.0 / sqrt(sqrt(kSafmin)));
+    if (tid == 0) {
+        double tnrm = 0.0;
+        for (int i = 0; i < n; ++i) { const double a = fabs(D[i]); if (tnrm < a || isnan(a)) tnrm = a; }
+        for (int i = 0; i < n - 1; ++i) { const double a = fabs(E[i]); if (tnrm < a || isnan(a)) tnrm = a; }
+        double scale = 1.0;
+        if (tnrm > 0.0 && tnrm < rmin) scale = rmin / tnrm;
+        else if (tnrm > rmax) scale = rmax / tnrm;
+        if (scale != 1.0) {
+            for (int i = 0; i < n; ++i) D[i] = scale * D[i];
+            for (int i = 0; i < n - 1; ++i) E[i] = scale * E[i];
+            tnrm = tnrm * scale;
+        }
+        int tryrac = p.tryrac;
+        const int iinfo = tryrac ? dlarrr(n, D, E) : -1;
+        if (iinfo != 0) tryrac = 0;
+        sh.w[0] = scale;
+        sh.w[1] = tnrm;
+        sh.iw[0] = tryrac;
+        sh.iw[1] = (iinfo == 0) ? 1 : 0;
+    }
+    SM_SYNC();
+    const double scale = sh.w[0], tnrm = sh.w[1];
+    const int tryrac = sh.iw[0];
+    const double thresh = sh.iw[1] ? kEps : -kEps;
+    if (scale != 1.0 && p.range == kVal) { wl = wl * scale; wu = wu * scale; }
+    for (int i = tid; i < n; i += nt) {
+        if (tryrac) wv.DCOPY[i] = D[i];
+        wv.E2[i] = (i < n - 1) ? E[i] * E[i] : 0.0;
+    }
+    SM_SYNC();
+    double rtol1, rtol2;
+    if (!p.wantz) { rtol1 = 4.0 * kEps; rtol2 = 4.0 * kEps; }
+    else { rtol1 = sqrt(kEps); rtol2 = fmx(sqrt(kEps) * 5.0e-3, 4.0 * kEps); }
+    int iinfo = dlarre_team(tid, nt, sh, p.range, n, &wl, &wu, iil, iiu, D, E, wv.E2, rtol1, rtol2,
+                            thresh, p.W, wv.WERR, wv.WGAP, wv.IBLOCK, wv.INDEXW, wv.ISPLIT, wv.GERS,
+                            wv.WK);
+    res.tryrac = tryrac;
+    if (iinfo != 0) { res.info = 10 + (iinfo < 0 ? -iinfo : iinfo); return res; }
+    const int m = sh.m;
+    const int nsplit = sh.nsplit;
+    const double pivmin = sh.pivmin;
+    res.m = m;
+    if (p.wantz) {
+        if (m > p.mcols) { res.info = 1; return res; }
+        iinfo = dlarrv_team(tid, nt, sh, n, wl, D, E, wv.LD, wv.LLD, pivmin, wv.ISPLIT, m, rtol1, rtol2,
+                            p.W, wv.WERR, wv.WGAP, wv.WSH, wv.IBLOCK, wv.INDEXW, wv.GERS, p.Z, p.ldz,
+                            p.Zsupp, wv.TWIST, wv.CL1, wv.CL2, wv);
+        if (iinfo != 0) { res.info = 20 + (iinfo < 0 ? -iinfo : iinfo); return res; }
+    } else {
+        for (int j = tid; j < m; j += nt) p.W[j] = p.W[j] + E[wv.ISPLIT[wv.IBLOCK[j] - 1] - 1];
+        SM_SYNC();
+    }
+    if (tryrac) {   // odstmr.f:560-590: dlarrj block by block
+        int ibegin = 1, wbegin = 1;
+        const int nblk = (m > 0) ? wv.IBLOCK[m - 1] : 0;
+        for (int jblk = 1; jblk <= nblk; ++jblk) {
+            const int iend = wv.ISPLIT[jblk - 1], in = iend - ibegin + 1;
+            int wend = wbegin - 1;
+            while (wend < m && wv.IBLOCK[wend] == jblk) ++wend;
+            if (wend < wbegin) { ibegin = iend + 1; continue; }
+            for (int q = wbegin + tid; q <= wend; q += nt)
+                dlarrj1(in, wv.DCOPY + ibegin - 1, wv.E2 + ibegin - 1, wv.INDEXW[q - 1], 4.0 * kEps,
+                        &p.W[q - 1], &wv.WERR[q - 1], pivmin, tnrm);
+            ibegin = iend + 1;
+            wbegin = wend + 1;
+        }
+        SM_SYNC();
+    }
+    if (scale != 1.0) {
+        for (int j = tid; j < m; j += nt) p.W[j] = (1.0 / scale) * p.W[j];
+        SM_SYNC();
+    }
+    if (nsplit > 1) {   // odstmr.f:596-633
+        if (!p.wantz) {
+            if (tid == 0) {
+                for (int i = 1; i < m; ++i) {
+                    const double v = p.W[i];
+                    int j = i - 1;
+                    while (j >= 0 && p.W[j] > v) { p.W[j + 1] = p.W[j]; --j; }
+                    p.W[j + 1] = v;
+                }
+            }
+            SM_SYNC();
+        } else {
+            for (int j = 1; j <= m - 1; ++j) {
+                if (tid == 0) {
+                    int i = 0;
+                    double tmp = p.W[j - 1];
+                    for (int jj = j + 1; jj <= m; ++jj)
+                        if (p.W[jj - 1] < tmp) { i = jj; tmp = p.W[jj - 1]; }
+                    sh.iw[2] = i;
+                    if (i != 0) {
+                        p.W[i - 1] = p.W[j - 1];
+                        p.W[j - 1] = tmp;
+                        int t1 = p.Zsupp[2 * i - 2]; p.Zsupp[2 * i - 2] = p.Zsupp[2 * j - 2]; p.Zsupp[2 * j - 2] = t1;
+                        int t2 = p.Zsupp[2 * i - 1]; p.Zsupp[2 * i - 1] = p.Zsupp[2 * j - 1]; p.Zsupp[2 * j - 1] = t2;
+                    }
+                }
+                SM_SYNC();
+                const int i = sh.iw[2];
+                if (i != 0) {
+                    double* zi = p.Z + (long long)(i - 1) * p.ldz;
+                    double* zj = p.Z + (long long)(j - 1) * p.ldz;
+                    for (int r = tid; r < n; r += nt) { const double t = zi[r]; zi[r] = zj[r]; zj[r] = t; }
+                }
+                SM_SYNC();
+            }
+        }
+    }
+    return res;
+}
+
+}  // namespace mr3sm
