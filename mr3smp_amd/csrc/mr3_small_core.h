@@ -1504,7 +1504,7 @@ SM_HD inline int dlarrd_team(int tid, int nt, Shared& sh, int irange, int n, dou
 // ---------------------------------------------------------------------------------
 SM_HD inline int dlarre_team(int tid, int nt, Shared& sh, int irange, int n, double* vl,
                              double* vu, int il, int iu, double* d, double* e, double* e2,
-                             double rtol1, double rtol2, double spltol, double* w,
+                             double rtol1, double rtol2, int relsplit, double* w,
                              double* werr, double* wgap, int* iblock, int* indexw,
                              int* isplit, double* gers, double* wk) {
     const int MAXTRY = 6;
@@ -1527,10 +1527,15 @@ SM_HD inline int dlarre_team(int tid, int nt, Shared& sh, int irange, int n, dou
         }
         sh.pivmin = kSafmin * fmx(1.0, emax * emax);
         const double spdiam = gu - gl;
-        // dlarra (odrra.f:90-123)
+        // dlarra (odrra.f:90-123) with SPLTOL = -eps (absolute criterion) or +eps
+        // (relative).  The sign is passed as an int and the tolerance is the constant:
+        // hipcc for gfx950 (ROCm 7.2) compiled the select of the signed tolerance from a
+        // flag in LDS into an s_cselect on a stale SCC, so that it came out -eps in the
+        // relative branch whenever n % 4 == 0 and no off-diagonal split.
+        const double spltol = kEps;
         int nsplit = 1;
-        if (spltol < 0.0) {
-            const double tmp1 = fabs(spltol) * spdiam;
+        if (!relsplit) {
+            const double tmp1 = spltol * spdiam;
             for (int i = 1; i <= n - 1; ++i)
                 if (fabs(e[i - 1]) <= tmp1) {
                     e[i - 1] = 0.0; e2[i - 1] = 0.0;
@@ -2079,6 +2084,7 @@ SM_HD inline int dlarrv_team(int tid, int nt, Shared& sh, int n, double vl, doub
     double sigma = 0.0;
     int ndepth = 0, parity = 1, nclus = 0, idone = 0;
     for (int jblk = 1; jblk <= nblk; ++jblk) {
+        SM_SYNC();   // every thread has read the previous block's flags
         if (tid == 0) {
             sh.lv[0] = 0;   // block active
             const int iend = isplit[jblk - 1];
@@ -2373,7 +2379,7 @@ SM_HD inline Result dstemr_team(int tid, int nt, Shared& sh, const Prob& p, cons
     SM_SYNC();
     const double scale = sh.w[0], tnrm = sh.w[1];
     const int tryrac = sh.iw[0];
-    const double thresh = sh.iw[1] ? kEps : -kEps;
+    const int relsplit = sh.iw[1];
     if (scale != 1.0 && p.range == kVal) { wl = wl * scale; wu = wu * scale; }
     for (int i = tid; i < n; i += nt) {
         if (tryrac) wv.DCOPY[i] = D[i];
@@ -2384,7 +2390,7 @@ SM_HD inline Result dstemr_team(int tid, int nt, Shared& sh, const Prob& p, cons
     if (!p.wantz) { rtol1 = 4.0 * kEps; rtol2 = 4.0 * kEps; }
     else { rtol1 = sqrt(kEps); rtol2 = fmx(sqrt(kEps) * 5.0e-3, 4.0 * kEps); }
     int iinfo = dlarre_team(tid, nt, sh, p.range, n, &wl, &wu, iil, iiu, D, E, wv.E2, rtol1, rtol2,
-                            thresh, p.W, wv.WERR, wv.WGAP, wv.IBLOCK, wv.INDEXW, wv.ISPLIT, wv.GERS,
+                            relsplit, p.W, wv.WERR, wv.WGAP, wv.IBLOCK, wv.INDEXW, wv.ISPLIT, wv.GERS,
                             wv.WK);
     res.tryrac = tryrac;
     if (iinfo != 0) { res.info = 10 + (iinfo < 0 ? -iinfo : iinfo); return res; }

@@ -1,15 +1,77 @@
 // CPU build of the small-matrix route of the GPU library (mr3smp_amd/csrc/
-// mr3_small_core.h), run by one thread, for the tests: it lets the device algorithm be
-// compared bit for bit with the oracle without a GPU.  Test infrastructure only.
+// mr3_small_core.h) for the tests: it lets the device algorithm be compared bit for bit
+// with the oracle without a GPU.  By default one thread runs every phase in order;
+// small_route_set_team(nt) makes nt POSIX threads run it as the device's team does, with
+// a barrier for each of the team's synchronisations, so that the work split and the
+// shared state of a 64-lane wave are exercised too.  Test infrastructure only.
+#include <pthread.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
+
+// The team's barrier: every thread must reach it from the same source line (the core's
+// synchronisations are meant to be met by the whole team at once); a thread waiting
+// longer than 20 s, or threads meeting at different lines, end the process with a
+// report of where each thread stands.
+static int g_team = 1;
+static pthread_mutex_t g_mu = PTHREAD_MUTEX_INITIALIZER;
+static pthread_cond_t g_cv = PTHREAD_COND_INITIALIZER;
+static int g_arrived = 0;
+static long g_gen = 0;
+static int g_site[1024];
+static __thread int t_tid = 0;
+
+static void team_report(const char* what) {
+    fprintf(stderr, "team barrier: %s; sites:", what);
+    for (int t = 0; t < g_team; ++t) fprintf(stderr, " %d", g_site[t]);
+    fprintf(stderr, "\n");
+    abort();
+}
+
+static inline void team_sync(int line) {
+    if (g_team <= 1) return;
+    pthread_mutex_lock(&g_mu);
+    g_site[t_tid] = line;
+    const long gen = g_gen;
+    if (++g_arrived == g_team) {
+        for (int t = 1; t < g_team; ++t)
+            if (g_site[t] != g_site[0]) team_report("threads meet at different lines");
+        g_arrived = 0;
+        ++g_gen;
+        pthread_cond_broadcast(&g_cv);
+    } else {
+        struct timespec ts;
+        clock_gettime(CLOCK_REALTIME, &ts);
+        ts.tv_sec += 20;
+        while (g_gen == gen)
+            if (pthread_cond_timedwait(&g_cv, &g_mu, &ts) != 0 && g_gen == gen) team_report("timeout");
+    }
+    pthread_mutex_unlock(&g_mu);
+}
 
 #define SM_HD
-#define SM_SYNC() ((void)0)
+#define SM_SYNC() team_sync(__LINE__)
 #include "../mr3smp_amd/csrc/mr3_small_core.h"
 
 using namespace mr3sm;
+
+struct TeamArg {
+    int tid, nt;
+    Shared* sh;
+    const Prob* p;
+    const Work* w;
+    Result r;
+};
+
+static void* team_main(void* a) {
+    TeamArg* t = (TeamArg*)a;
+    t_tid = t->tid;
+    t->r = dstemr_team(t->tid, t->nt, *t->sh, *t->p, *t->w);
+    return nullptr;
+}
+
+extern "C" void small_route_set_team(int nt) { g_team = nt < 1 ? 1 : (nt > 1024 ? 1024 : nt); }
 
 // Same arguments as mrrr() for the matrices it sends to dstemr (n < 500).
 extern "C" int small_route_mrrr(char* jobz, char* range, int* np, double* D, double* E,
@@ -28,13 +90,37 @@ extern "C" int small_route_mrrr(char* jobz, char* range, int* np, double* D, dou
     p.ldz = *ldz;
     p.mcols = n;
     p.Din = D; p.Ein = E; p.W = W; p.Z = Z; p.Zsupp = Zsupp;
+    const int nt = g_team;
+    const int wks = n < nt ? n : nt;   // threads whose dlar1v / dlarrf work is kept
     Work w;
-    double* lds = (double*)calloc((size_t)lds_doubles(n), sizeof(double));
-    char* slice = (char*)calloc((size_t)slice_bytes(n, 1), 1);
-    carve(w, lds, slice, n, 1);
+    // the device's LDS and work slice start with whatever they held: poison them here
+    // (a byte pattern that reads as NaN doubles and negative ints), so that a read of an
+    // entry before its first write changes the results
+    const char* pz = getenv("SMALL_ROUTE_POISON");
+    const int pat = pz ? (int)strtol(pz, nullptr, 0) : 0xA7;
+    double* lds = (double*)malloc((size_t)lds_doubles(n) * sizeof(double));
+    char* slice = (char*)malloc((size_t)slice_bytes(n, wks));
+    memset(lds, pat, (size_t)lds_doubles(n) * sizeof(double));
+    memset(slice, pat, (size_t)slice_bytes(n, wks));
+    carve(w, lds, slice, n, wks);
     Shared sh;
     memset(&sh, 0, sizeof(sh));
-    Result r = dstemr_team(0, 1, sh, p, w);
+    Result r;
+    if (nt == 1) {
+        r = dstemr_team(0, 1, sh, p, w);
+    } else {
+        g_arrived = 0;
+        TeamArg* args = new TeamArg[nt];
+        pthread_t* th = new pthread_t[nt];
+        for (int t = 0; t < nt; ++t) {
+            args[t].tid = t; args[t].nt = nt; args[t].sh = &sh; args[t].p = &p; args[t].w = &w;
+            pthread_create(&th[t], nullptr, team_main, &args[t]);
+        }
+        for (int t = 0; t < nt; ++t) pthread_join(th[t], nullptr);
+        r = args[0].r;
+        delete[] th;
+        delete[] args;
+    }
     if (getenv("SMALL_ROUTE_PASSES")) fprintf(stderr, "n=%d passes=%d\n", n, sh.passes);
     free(lds);
     free(slice);

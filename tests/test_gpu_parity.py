@@ -2,8 +2,9 @@
 and against the reference's stored outputs.
 
 Bars (tests/accuracy.py): m exact; |W - W_ref| <= 4 n eps ||T||; resid and orth no
-worse than 2 * max(ref, floor) in units of n eps, floor 1 against the oracle (same
-algorithm) and 10 against the reference's dstemr route.  At full benchmark sizes,
+worse than 2 * max(ref, 1) in units of n eps.  Below n = 500 (the reference's dstemr
+route, SRC/mrrr.c:149-153) W, Z and Zsupp must equal the oracle's and the reference's
+byte for byte.  At full benchmark sizes,
 where the oracle would take minutes, size-independent properties are checked:
 Sturm-count consistency of every eigenvalue with its index, trace and Frobenius
 identities, resid/orth bounds, and batch/serve identity.
@@ -13,8 +14,8 @@ import os
 import numpy as np
 import pytest
 
-from accuracy import DSTEMR_FLOOR, EPS, check_pair, check_support, w_tol
-from mrrr_ctypes import call_mrrr, resid_orth, tnorm
+from accuracy import EPS, check_pair, check_support, w_tol
+from mrrr_ctypes import call_mrrr, resid_orth, same_bits, tnorm
 
 pytestmark = pytest.mark.gpu
 
@@ -65,12 +66,16 @@ def test_gpu_matches_reference_outputs(name, golden, oracle, M):
         ref["resid"], ref["orth"] = (float(v) for v in c["resid_orth"])
     if "Z" in c:
         ref["Z"] = c["Z"]
-    dstemr_route = c["threads"] == 1 or len(D) < 500
-    check_pair(D, E, got, ref, name + " vs ref", vectors=(c["jobz"] == "V"),
-               floor=DSTEMR_FLOOR if dstemr_route else 1.0)
+    check_pair(D, E, got, ref, name + " vs ref", vectors=(c["jobz"] == "V"))
     # and against the oracle, same algorithm
     orc = call_mrrr(oracle[1], c["jobz"], c["rng"], D, E, **c["kw"])
     check_pair(D, E, got, orc, name + " vs oracle", vectors=(c["jobz"] == "V"))
+    if len(D) < 500:   # the reference's dstemr route: the same bytes
+        assert same_bits(got, orc, c["jobz"] == "V"), name + ": bits differ from the oracle"
+        assert np.array_equal(got["W"], c["W"]), name + ": W bits differ from the reference"
+        if "Z" in c:
+            assert np.array_equal(got["Z"], c["Z"]), name + ": Z bits differ from the reference"
+            assert np.array_equal(got["Zsupp"], c["Zsupp"]), name + ": Zsupp differs"
 
 
 def _matrix(kind, n, rng):
@@ -114,6 +119,8 @@ def test_gpu_vs_oracle(kind, n, oracle, M):
             assert g["info"] == 0 and g["m"] == o["m"]
             continue
         check_pair(D, E, g, o, f"{kind}{n}-{jobz}{rg}", vectors=(jobz == "V"))
+        if n < 500:
+            assert same_bits(g, o, jobz == "V"), f"{kind}{n}-{jobz}{rg}: bits differ"
         assert g["vl"] == pytest.approx(o["vl"], rel=1e-12, abs=1e-300)
         assert g["vu"] == pytest.approx(o["vu"], rel=1e-12, abs=1e-300)
         assert (g["il"], g["iu"], g["tryrac"]) == (o["il"], o["iu"], o["tryrac"])
@@ -139,6 +146,7 @@ def test_gpu_edge_cases(oracle, M):
             g = M.mrrr(jobz, rg, D, E, **kw)
             o = call_mrrr(oracle[1], jobz, rg, D, E, **kw)
             check_pair(D, E, g, o, f"{name}-{jobz}{rg}", vectors=(jobz == "V"))
+            assert same_bits(g, o, jobz == "V"), f"{name}-{jobz}{rg}: bits differ"
     # empty interval and invalid arguments
     D, E = rng.uniform(-1, 1, 50), rng.uniform(-1, 1, 50)
     g = M.mrrr("V", "V", D, E, vl=100.0, vu=101.0)
@@ -341,6 +349,64 @@ def test_gpu_range_v_large_batch(oracle_lib, M):
         r, o = resid_orth_device(Dt, Et, keep[k]["W"][:m], Z)
         assert r <= 2 * max(ref["resid"][k], 1.0) and o <= 2 * max(ref["orth"][k], 1.0), (k, r, o)
         assert support_ok_device(Z, keep[k]["Zsupp"]), k
+    del probs, keep
+    torch.cuda.empty_cache()
+
+
+# Matrices of the mixed workload with n >= 500 (all glued Wilkinson) whose
+# orthogonality on the library's task-parallel route exceeds 2 max(reference, 1), with
+# the library's figure (the oracle, restating the same route on one thread, gives the
+# same).  The reference's own figure on them depends on its thread count (matrix 2428:
+# 1.58 with 2 to 8 threads, 0.21 with 13 or 16; 89: 0.57, then 0.50), and the fixture
+# holds its 16-thread run.
+MIXED_KNOWN_GAP = {89: 3.81, 1615: 2.22, 2428: 2.30, 2792: 2.33}
+
+
+def test_gpu_mixed_workload_vs_reference(M):
+    """The headline workload at full size in one mr3_solve_batch call: 4096 matrices,
+    n log-uniform in [32, 2048], uniform / perturbed 1-2-1 / glued Wilkinson, jobz V,
+    range A.  Against the compiled reference's per-matrix figures
+    (tests/golden/mixed_ref.npz): below n = 500 the SHA-1 digests of W, Z and Zsupp
+    must equal the reference's (the dstemr route, bit for bit); at n >= 500 m exact,
+    sampled eigenvalues within 4 n eps ||T||, resid and orth (on the device) no worse
+    than 2 max(reference, 1) except the matrices listed in MIXED_KNOWN_GAP, supports."""
+    import hashlib
+
+    import torch
+    from mr3smp_amd import workloads as WL
+    from mrrr_ctypes import resid_orth_device, support_ok_device
+
+    ref = np.load(os.path.join(os.path.dirname(__file__), "golden", "mixed_ref.npz"))
+    mats = WL.batched_mixed(4096)
+    dev = torch.device("cuda:0")
+    probs = [dict(D=torch.tensor(D, device=dev), E=torch.tensor(E, device=dev)) for D, E, _ in mats]
+    print("mixed workload: solving 4096 matrices in one batch", flush=True)
+    rc, arr, keep = M.solve_batch(probs)
+    torch.cuda.synchronize()
+    assert rc == 0
+    print("mixed workload: solved, checking", flush=True)
+    sha = lambda t: hashlib.sha1(t.contiguous().cpu().numpy().tobytes()).hexdigest()
+    gaps = {}
+    for i, (D, E, kind) in enumerate(mats):
+        n, m = len(D), arr[i].m
+        assert n == ref["n"][i], "workload differs from the fixture"
+        assert arr[i].info == 0 and m == ref["m"][i], f"matrix {i}: info {arr[i].info} m {m}"
+        W, Z, Zs = keep[i]["W"][:m], keep[i]["Z"][:m, :n], keep[i]["Zsupp"][:2 * m]
+        if n < 500:
+            assert (sha(W), sha(Z), sha(Zs)) == (ref["hW"][i], ref["hZ"][i], ref["hS"][i]), \
+                f"matrix {i} ({kind}, n={n}): bits differ from the reference"
+            continue
+        Wh = W.cpu().numpy()
+        js = sorted(set([0, m - 1] + list(range(0, m, max(1, m // 32)))))
+        dw = np.abs(Wh[js] - ref["W"][ref["Woff"][i]:ref["Woff"][i + 1]]).max()
+        assert dw <= w_tol(D, E), f"matrix {i}: |W - W_ref| {dw:.3g}"
+        r, o = resid_orth_device(probs[i]["D"], probs[i]["E"], W, Z)
+        assert r <= 2 * max(ref["resid"][i], 1.0), f"matrix {i}: resid {r:.3g} vs {ref['resid'][i]:.3g}"
+        if o > 2 * max(ref["orth"][i], 1.0):
+            gaps[i] = round(o, 2)
+        assert support_ok_device(Z, Zs), f"matrix {i}: nonzero outside support"
+    print(f"mixed workload: orth gaps at n >= 500: {gaps}", flush=True)
+    assert set(gaps) <= set(MIXED_KNOWN_GAP), f"new orth gaps: {set(gaps) - set(MIXED_KNOWN_GAP)}"
     del probs, keep
     torch.cuda.empty_cache()
 

@@ -2,10 +2,12 @@
 
 The route restates the reference's dstemr (odstmr.f, odrre.f with dqds, odrrv.f), which
 mrrr() takes for every n < 500 (SRC/mrrr.c:149-153).  Its source is written for one
-team of threads (a wave on the GPU) with barriers between phases; compiled with g++ for
-a team of one thread (tests/small_route_cpu.cpp) it runs every phase in order, so the
-device algorithm itself can be checked bit for bit against the oracle's restatement
-(which is pinned against the compiled reference in test_oracle.py) without a GPU.
+team of threads (a wave on the GPU) with barriers between phases; compiled with g++
+(tests/small_route_cpu.cpp) it runs as a team of one thread, every phase in order, or
+as 64 POSIX threads with a barrier per synchronisation, as the device's wave does.  So
+the device algorithm and its work split can be checked bit for bit against the oracle's
+restatement (which is pinned against the compiled reference in test_oracle.py) without
+a GPU.  The harness poisons its work memory, as the device's starts uninitialised.
 """
 import os
 import subprocess
@@ -21,9 +23,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 @pytest.fixture(scope="session")
 def small_cpu(tmp_path_factory):
     out = str(tmp_path_factory.mktemp("smallcpu") / "libsmall_cpu.so")
-    subprocess.run(["g++", "-O2", "-fPIC", "-ffp-contract=off", "-shared", "-o", out,
+    subprocess.run(["g++", "-O2", "-fPIC", "-ffp-contract=off", "-shared", "-pthread", "-o", out,
                     os.path.join(HERE, "small_route_cpu.cpp")], check=True)
-    return load(out, "small_route_mrrr")[1]
+    return load(out, "small_route_mrrr")
 
 
 def _calls(D, E, rng):
@@ -38,9 +40,11 @@ def _calls(D, E, rng):
             ("V", "A", dict(tryrac=0)))
 
 
-def _check(small_cpu, oracle, D, E, label, rng):
+def _check(small_cpu, oracle, D, E, label, rng, team=1):
+    lib, fn = small_cpu
+    lib.small_route_set_team(team)
     for jobz, rg, kw in _calls(D, E, rng):
-        g = call_mrrr(small_cpu, jobz, rg, D, E, **kw)
+        g = call_mrrr(fn, jobz, rg, D, E, **kw)
         o = call_mrrr(oracle[1], jobz, rg, D, E, **kw)
         assert g["info"] == o["info"] == 0, (label, jobz, rg)
         assert g["m"] == o["m"], (label, jobz, rg, g["m"], o["m"])
@@ -61,9 +65,20 @@ def test_small_route_mixed_workload(small_cpu, oracle):
         _check(small_cpu, oracle, D, E, f"{i} {kind} n={len(D)}", np.random.default_rng(i))
 
 
-@pytest.mark.parametrize("name", ["n1", "n2", "n3", "diag", "zerosD", "split", "tiny", "huge",
-                                  "repeated", "relacc", "toeplitz499", "glued499"])
-def test_small_route_edge_cases(name, small_cpu, oracle):
+def test_small_route_mixed_workload_team64(small_cpu, oracle):
+    """Every 96th small matrix of the mixed workload run by a team of 64 threads."""
+    from mr3smp_amd import workloads as WL
+    mats = WL.batched_mixed(4096)
+    idx = [i for i, (D, _, _) in enumerate(mats) if len(D) < 500][5::96]
+    for i in idx:
+        D, E, kind = mats[i]
+        _check(small_cpu, oracle, D, E, f"{i} {kind} n={len(D)}", np.random.default_rng(i), team=64)
+
+
+@pytest.mark.parametrize("team", [1, 64])
+@pytest.mark.parametrize("name", ["n1", "n2", "n3", "diag", "diag8", "diag16", "zerosD", "split",
+                                  "tiny", "huge", "repeated", "relacc", "toeplitz499", "glued499"])
+def test_small_route_edge_cases(name, team, small_cpu, oracle):
     from mr3smp_amd import workloads as WL
     rng = np.random.default_rng(5)
     cases = {
@@ -71,6 +86,8 @@ def test_small_route_edge_cases(name, small_cpu, oracle):
         "n2": (np.array([1.0, 2.0]), np.array([0.5, 0.0])),
         "n3": (rng.uniform(-1, 1, 3), rng.uniform(-1, 1, 3)),
         "diag": (rng.uniform(-1, 1, 40), np.zeros(40)),
+        "diag8": (np.linspace(1, -1, 8), np.zeros(8)),
+        "diag16": (np.linspace(1, -1, 16), np.zeros(16)),
         "zerosD": (np.zeros(30), np.ones(30)),
         "split": (rng.uniform(-1, 1, 80), np.where(np.arange(80) % 9 == 0, 0.0, rng.uniform(-1, 1, 80))),
         "tiny": (1e-300 * rng.uniform(-1, 1, 25), 1e-300 * rng.uniform(-1, 1, 25)),
@@ -83,4 +100,4 @@ def test_small_route_edge_cases(name, small_cpu, oracle):
     D, E = cases[name]
     E = E.copy()
     E[-1] = 0.0
-    _check(small_cpu, oracle, D, E, name, np.random.default_rng(len(D)))
+    _check(small_cpu, oracle, D, E, name, np.random.default_rng(len(D)), team=team)
