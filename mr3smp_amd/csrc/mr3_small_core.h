@@ -18,6 +18,13 @@
 #include <float.h>
 #include <math.h>
 
+// SM_MARK(k): an optional hook at the route's phase boundaries (k = 1 after dlarre,
+// 2 after dlarrv, 3 after dlarrj, 16 + depth after each level of the tree); none by
+// default.  A profiling build defines it to record a timestamp.
+#ifndef SM_MARK
+#define SM_MARK(k) ((void)0)
+#endif
+
 namespace mr3sm {
 
 constexpr double kEps = DBL_EPSILON;   // odmch.f 'P'
@@ -61,15 +68,23 @@ struct Work {
     double *WK;           // 4n+8: dqds array / dlar1v and dlarrf work
     double *E2, *WERR, *WGAP, *WSH, *GERS, *DCOPY;
     int *ISPLIT, *IBLOCK, *INDEXW, *TWIST, *CL1, *CL2;
-    // the parallel children of a tree node: each thread's dlar1v / dlarrf work (4n + 8
-    // doubles, interleaved: entry i of thread t at WKL[i * WKS + t]), the node's values
-    // before its children ran (W, WERR, WSH, WGAP), each child's input gap, its bounds
-    // and whether it must run again
+    // the parallel children of tree nodes: each thread's dlar1v / dlarrf work (4n + 8
+    // doubles, contiguous: thread t's at WKL + t (4n + 8), so that a thread walking its
+    // own child stays within its own cache lines; WKS threads have one), the node's
+    // values before its children ran (W, WERR, WSH, WGAP), each child's input gap, its
+    // bounds and whether it must run again
     double* WKL;
     int WKS;
     double *SNW, *SNE, *SNS, *SNG, *INP;
     double* CERT;         // 5 per child: see sng_compute
     int *CH, *DIRTY;
+    // a group of up to kVGroup nodes of one tree level: each node's representation
+    // (D, L, D*L, D*L*L: 4n doubles a slot), values, errors and gaps as the level found
+    // them around the group, per node its shift and refinement record (10 doubles) and
+    // its range and number of children (4 ints), each eigenvalue's node, the list of
+    // eigenvalues to refine
+    double *RS, *LW, *LE, *LG, *GD;
+    int *GI, *NODEOF, *ITEMS;
 };
 
 struct Result {
@@ -78,10 +93,12 @@ struct Result {
 
 // Doubles of LDS and bytes of device memory that carve() hands out for a matrix of
 // order n solved by a team whose dlar1v / dlarrf work is kept for wks threads.
+constexpr int kVGroup = 32;   // nodes of a tree level handled side by side (dlarrv)
 SM_HD inline long long lds_doubles(int n) { return 8LL * n + 8; }
 SM_HD inline long long slice_bytes(int n, int wks) {
-    const long long b = (17LL * n + (long long)wks * (4LL * n + 8)) * (long long)sizeof(double) +
-                        (11LL * n + 10) * (long long)sizeof(int);
+    const long long dbl = 20LL * n + 10LL * kVGroup + 4LL * kVGroup * n + (long long)wks * (4LL * n + 8);
+    const long long b = dbl * (long long)sizeof(double) +
+                        (13LL * n + 10 + 4LL * kVGroup) * (long long)sizeof(int);
     return (b + 255) & ~255LL;
 }
 
@@ -105,8 +122,13 @@ SM_HD inline void carve(Work& w, double* lds, char* base, int n, int wks) {
     w.SNG = w.SNS + n;
     w.INP = w.SNG + n;
     w.CERT = w.INP + n;               // 5n
+    w.LW = w.CERT + 5 * n;
+    w.LE = w.LW + n;
+    w.LG = w.LE + n;
+    w.GD = w.LG + n;                  // 10 kVGroup
+    w.RS = w.GD + 10 * kVGroup;       // kVGroup 4n
     w.WKS = wks;
-    w.WKL = w.CERT + 5 * n;           // wks (4n + 8)
+    w.WKL = w.RS + 4LL * kVGroup * n; // wks (4n + 8)
     w.ISPLIT = (int*)(w.WKL + (long long)wks * (4 * n + 8));
     w.IBLOCK = w.ISPLIT + n;
     w.INDEXW = w.IBLOCK + n;
@@ -115,6 +137,9 @@ SM_HD inline void carve(Work& w, double* lds, char* base, int n, int wks) {
     w.CL2 = w.CL1 + 2 * n + 4;        // 2n + 4
     w.CH = w.CL2 + 2 * n + 4;         // 2n + 2
     w.DIRTY = w.CH + 2 * n + 2;       // n
+    w.NODEOF = w.DIRTY + n;
+    w.ITEMS = w.NODEOF + n;
+    w.GI = w.ITEMS + n;               // 4 kVGroup
 }
 
 // ---------------------------------------------------------------------------------
@@ -181,7 +206,7 @@ SM_HD inline int count_T2(int n, const double* d, const double* e2, double x, do
 // ---------------------------------------------------------------------------------
 SM_HD inline void dlarrb1(int n, const double* d, const double* lld, int i, double rtol1,
                           double rtol2, double gap0, double gap, double* w, double* werr,
-                          double pivmin, double spdiam, int twist) {
+                          double pivmin, double spdiam, int twist, double* cert = nullptr) {
     const int maxitr = (int)((log(spdiam + pivmin) - log(pivmin)) / log(2.0)) + 2;
     const double mnwdth = 2.0 * pivmin;
     int r = twist;
@@ -202,13 +227,31 @@ SM_HD inline void dlarrb1(int n, const double* d, const double* lld, int i, doub
     double width = 0.5 * fabs(left - right);
     double tmp = fmx(fabs(left), fabs(right));
     double cvrgd = fmx(rtol1 * gap0, rtol2 * tmp);
-    if (width <= cvrgd || width <= mnwdth) return;
+    if (width <= cvrgd || width <= mnwdth) {
+        if (cert) { cert[0] = INFINITY; cert[1] = 0.0; cert[2] = 1.0; cert[3] = 1.0; }
+        return;
+    }
+    // cert (optional): what decides the path's dependence on gap -- the smallest width
+    // that went on, the width it stopped at, whether that stop held regardless of gap,
+    // and 2 if a NaN took part (see rb_same_path)
+    double wmin = INFINITY;
+    bool nanseen = isnan(gap);
     for (int iter = 0;; ++iter) {
         double mid = 0.5 * (left + right);
         width = right - mid;
         tmp = fmx(fabs(left), fabs(right));
         cvrgd = fmx(rtol1 * gap, rtol2 * tmp);
-        if (width <= cvrgd || width <= mnwdth || iter == maxitr) break;
+        nanseen = nanseen || isnan(width) || isnan(tmp);
+        if (width <= cvrgd || width <= mnwdth || iter == maxitr) {
+            if (cert) {
+                cert[0] = wmin;
+                cert[1] = width;
+                cert[2] = (width <= rtol2 * tmp || width <= mnwdth || iter == maxitr) ? 1.0 : 0.0;
+                cert[3] = nanseen ? 2.0 : 0.0;
+            }
+            break;
+        }
+        wmin = fmn(wmin, width);
         if (dlaneg(n, d, lld, mid, r) <= i - 1) left = mid;
         else right = mid;
     }
@@ -975,132 +1018,156 @@ SM_HD inline int dlasq2(int n, double* z) {
 
 // ---------------------------------------------------------------------------------
 // dlar1v (odr1v.f:156-369) on one representation; wk holds 4n + 2 doubles at stride ws
-// (a thread's slice of an array interleaved by thread).  z points at the block's first
+// (1 for the tree's children: each thread's own contiguous array).  z points at the block's first
 // row of the eigenvector column.  The support is cut where a test value falls below
 // gaptol; *cutmax receives the largest value that cut and *noncutmin the smallest
 // tested value that did not, so that the caller knows for which gaptol the vector
 // would come out the same.
 // ---------------------------------------------------------------------------------
-SM_HD inline void dlar1v(int n, int b1, int bn, double lambda, const double* d, const double* l,
-                         const double* ld, const double* lld, double pivmin, double gaptol,
-                         double* z, int wantnc, int* negcnt, double* ztz, double* mingma,
+SM_HD inline void dlar1v(int n, int b1, int bn, double lambda, const double* __restrict__ d,
+                         const double* __restrict__ l, const double* __restrict__ ld,
+                         const double* __restrict__ lld, double pivmin, double gaptol,
+                         double* __restrict__ z, int wantnc, int* negcnt, double* ztz, double* mingma,
                          int* r, int* isuppz, double* nrminv, double* resid, double* rqcorr,
-                         double* wk, int ws, double* cutmax, double* noncutmin) {
+                         double* __restrict__ wk, int ws, double* cutmax, double* noncutmin) {
 #define D_(i) d[(i) - 1]
 #define L_(i) l[(i) - 1]
 #define LD_(i) ld[(i) - 1]
 #define LLD_(i) lld[(i) - 1]
 #define Z_(i) z[(i) - 1]
 #define WK(i) wk[(long long)((i) - 1) * ws]
+    // The recurrences carry their running value in a register and store it as they go
+    // (a value read back from the work array after another store would put a memory
+    // round trip on every step's dependency chain); the arithmetic is the reference's,
+    // step for step.
     int r1, r2, neg1, neg2, i;
     double s, dplus, dminus, tmp;
     bool sawnan1, sawnan2;
     if (*r == 0) { r1 = b1; r2 = bn; } else { r1 = *r; r2 = *r; }
     const int indlpl = 0, indumn = n, inds = 2 * n + 1, indp = 3 * n + 1;
-    if (b1 == 1) WK(inds) = 0.0;
-    else WK(inds + b1 - 1) = LLD_(b1 - 1);
+    const double s0 = (b1 == 1) ? 0.0 : LLD_(b1 - 1);
+    WK(inds + b1 - 1) = s0;
     neg1 = 0;
-    s = WK(inds + b1 - 1) - lambda;
+    s = s0 - lambda;
     for (i = b1; i <= r1 - 1; ++i) {
         dplus = D_(i) + s;
-        WK(indlpl + i) = LD_(i) / dplus;
+        const double lpl = LD_(i) / dplus;
+        WK(indlpl + i) = lpl;
         if (dplus < 0.0) ++neg1;
-        WK(inds + i) = s * WK(indlpl + i) * L_(i);
-        s = WK(inds + i) - lambda;
+        const double sv = s * lpl * L_(i);
+        WK(inds + i) = sv;
+        s = sv - lambda;
     }
     sawnan1 = isnan(s);
     if (!sawnan1) {
         for (i = r1; i <= r2 - 1; ++i) {
             dplus = D_(i) + s;
-            WK(indlpl + i) = LD_(i) / dplus;
-            WK(inds + i) = s * WK(indlpl + i) * L_(i);
-            s = WK(inds + i) - lambda;
+            const double lpl = LD_(i) / dplus;
+            WK(indlpl + i) = lpl;
+            const double sv = s * lpl * L_(i);
+            WK(inds + i) = sv;
+            s = sv - lambda;
         }
         sawnan1 = isnan(s);
     }
     if (sawnan1) {
         neg1 = 0;
-        s = WK(inds + b1 - 1) - lambda;
+        s = s0 - lambda;
         for (i = b1; i <= r1 - 1; ++i) {
             dplus = D_(i) + s;
             if (fabs(dplus) < pivmin) dplus = -pivmin;
-            WK(indlpl + i) = LD_(i) / dplus;
+            const double lpl = LD_(i) / dplus;
+            WK(indlpl + i) = lpl;
             if (dplus < 0.0) ++neg1;
-            WK(inds + i) = s * WK(indlpl + i) * L_(i);
-            if (WK(indlpl + i) == 0.0) WK(inds + i) = LLD_(i);
-            s = WK(inds + i) - lambda;
+            double sv = s * lpl * L_(i);
+            if (lpl == 0.0) sv = LLD_(i);
+            WK(inds + i) = sv;
+            s = sv - lambda;
         }
         for (i = r1; i <= r2 - 1; ++i) {
             dplus = D_(i) + s;
             if (fabs(dplus) < pivmin) dplus = -pivmin;
-            WK(indlpl + i) = LD_(i) / dplus;
-            WK(inds + i) = s * WK(indlpl + i) * L_(i);
-            if (WK(indlpl + i) == 0.0) WK(inds + i) = LLD_(i);
-            s = WK(inds + i) - lambda;
+            const double lpl = LD_(i) / dplus;
+            WK(indlpl + i) = lpl;
+            double sv = s * lpl * L_(i);
+            if (lpl == 0.0) sv = LLD_(i);
+            WK(inds + i) = sv;
+            s = sv - lambda;
         }
     }
     neg2 = 0;
-    WK(indp + bn - 1) = D_(bn) - lambda;
+    const double p0 = D_(bn) - lambda;
+    WK(indp + bn - 1) = p0;
+    double pv = p0;
     for (i = bn - 1; i >= r1; --i) {
-        dminus = LLD_(i) + WK(indp + i);
+        dminus = LLD_(i) + pv;
         tmp = D_(i) / dminus;
         if (dminus < 0.0) ++neg2;
         WK(indumn + i) = L_(i) * tmp;
-        WK(indp + i - 1) = WK(indp + i) * tmp - lambda;
+        pv = pv * tmp - lambda;
+        WK(indp + i - 1) = pv;
     }
-    tmp = WK(indp + r1 - 1);
+    tmp = pv;
     sawnan2 = isnan(tmp);
     if (sawnan2) {
         neg2 = 0;
+        pv = p0;
         for (i = bn - 1; i >= r1; --i) {
-            dminus = LLD_(i) + WK(indp + i);
+            dminus = LLD_(i) + pv;
             if (fabs(dminus) < pivmin) dminus = -pivmin;
             tmp = D_(i) / dminus;
             if (dminus < 0.0) ++neg2;
             WK(indumn + i) = L_(i) * tmp;
-            WK(indp + i - 1) = WK(indp + i) * tmp - lambda;
-            if (tmp == 0.0) WK(indp + i - 1) = D_(i) - lambda;
+            pv = pv * tmp - lambda;
+            if (tmp == 0.0) pv = D_(i) - lambda;
+            WK(indp + i - 1) = pv;
         }
     }
-    *mingma = WK(inds + r1 - 1) + WK(indp + r1 - 1);
-    if (*mingma < 0.0) ++neg1;
+    double mg = WK(inds + r1 - 1) + WK(indp + r1 - 1);
+    if (mg < 0.0) ++neg1;
     *negcnt = wantnc ? neg1 + neg2 : -1;
-    if (fabs(*mingma) == 0.0) *mingma = kEps * WK(inds + r1 - 1);
-    *r = r1;
+    if (fabs(mg) == 0.0) mg = kEps * WK(inds + r1 - 1);
+    int rr = r1;
     for (i = r1; i <= r2 - 1; ++i) {
         tmp = WK(inds + i) + WK(indp + i);
         if (tmp == 0.0) tmp = kEps * WK(inds + i);
-        if (fabs(tmp) <= fabs(*mingma)) { *mingma = tmp; *r = i + 1; }
+        if (fabs(tmp) <= fabs(mg)) { mg = tmp; rr = i + 1; }
     }
-    isuppz[0] = b1;
-    isuppz[1] = bn;
-    Z_(*r) = 1.0;
-    *ztz = 1.0;
-    const int rr = *r;
+    *mingma = mg;
+    *r = rr;
+    int sup0 = b1, sup1 = bn;
+    Z_(rr) = 1.0;
+    double zz = 1.0, cmax = *cutmax, ncmin = *noncutmin;
     if (!sawnan1 && !sawnan2) {
+        double zn = 1.0;   // Z(i + 1)
         for (i = rr - 1; i >= b1; --i) {
-            Z_(i) = -(WK(indlpl + i) * Z_(i + 1));
-            const double tv = (fabs(Z_(i)) + fabs(Z_(i + 1))) * fabs(LD_(i));
+            const double zi = -(WK(indlpl + i) * zn);
+            const double tv = (fabs(zi) + fabs(zn)) * fabs(LD_(i));
             if (tv < gaptol) {
-                *cutmax = fmx(*cutmax, tv);
+                cmax = fmx(cmax, tv);
                 Z_(i) = 0.0;
-                isuppz[0] = i + 1;
+                sup0 = i + 1;
                 break;
             }
-            *noncutmin = fmn(*noncutmin, tv);
-            *ztz = *ztz + Z_(i) * Z_(i);
+            Z_(i) = zi;
+            ncmin = fmn(ncmin, tv);
+            zz = zz + zi * zi;
+            zn = zi;
         }
+        double zc = 1.0;   // Z(i)
         for (i = rr; i <= bn - 1; ++i) {
-            Z_(i + 1) = -(WK(indumn + i) * Z_(i));
-            const double tv = (fabs(Z_(i)) + fabs(Z_(i + 1))) * fabs(LD_(i));
+            const double zi = -(WK(indumn + i) * zc);
+            const double tv = (fabs(zc) + fabs(zi)) * fabs(LD_(i));
             if (tv < gaptol) {
-                *cutmax = fmx(*cutmax, tv);
+                cmax = fmx(cmax, tv);
                 Z_(i + 1) = 0.0;
-                isuppz[1] = i;
+                sup1 = i;
                 break;
             }
-            *noncutmin = fmn(*noncutmin, tv);
-            *ztz = *ztz + Z_(i + 1) * Z_(i + 1);
+            Z_(i + 1) = zi;
+            ncmin = fmn(ncmin, tv);
+            zz = zz + zi * zi;
+            zc = zi;
         }
     } else {
         for (i = rr - 1; i >= b1; --i) {
@@ -1108,32 +1175,37 @@ SM_HD inline void dlar1v(int n, int b1, int bn, double lambda, const double* d, 
             else Z_(i) = -(WK(indlpl + i) * Z_(i + 1));
             const double tv = (fabs(Z_(i)) + fabs(Z_(i + 1))) * fabs(LD_(i));
             if (tv < gaptol) {
-                *cutmax = fmx(*cutmax, tv);
+                cmax = fmx(cmax, tv);
                 Z_(i) = 0.0;
-                isuppz[0] = i + 1;
+                sup0 = i + 1;
                 break;
             }
-            *noncutmin = fmn(*noncutmin, tv);
-            *ztz = *ztz + Z_(i) * Z_(i);
+            ncmin = fmn(ncmin, tv);
+            zz = zz + Z_(i) * Z_(i);
         }
         for (i = rr; i <= bn - 1; ++i) {
             if (Z_(i) == 0.0) Z_(i + 1) = -(LD_(i - 1) / LD_(i)) * Z_(i - 1);
             else Z_(i + 1) = -(WK(indumn + i) * Z_(i));
             const double tv = (fabs(Z_(i)) + fabs(Z_(i + 1))) * fabs(LD_(i));
             if (tv < gaptol) {
-                *cutmax = fmx(*cutmax, tv);
+                cmax = fmx(cmax, tv);
                 Z_(i + 1) = 0.0;
-                isuppz[1] = i;
+                sup1 = i;
                 break;
             }
-            *noncutmin = fmn(*noncutmin, tv);
-            *ztz = *ztz + Z_(i + 1) * Z_(i + 1);
+            ncmin = fmn(ncmin, tv);
+            zz = zz + Z_(i + 1) * Z_(i + 1);
         }
     }
-    tmp = 1.0 / *ztz;
+    isuppz[0] = sup0;
+    isuppz[1] = sup1;
+    *cutmax = cmax;
+    *noncutmin = ncmin;
+    *ztz = zz;
+    tmp = 1.0 / zz;
     *nrminv = sqrt(tmp);
-    *resid = fabs(*mingma) * *nrminv;
-    *rqcorr = *mingma * tmp;
+    *resid = fabs(mg) * *nrminv;
+    *rqcorr = mg * tmp;
 #undef D_
 #undef L_
 #undef LD_
@@ -1147,10 +1219,11 @@ SM_HD inline void dlar1v(int n, int b1, int bn, double lambda, const double* d, 
 // w, wgap, werr are 1-based from the block's first eigenvalue; wk holds 2n doubles at
 // stride ws.
 // ---------------------------------------------------------------------------------
-SM_HD inline int dlarrf(int n, const double* d, const double* l, const double* ld, int clstrt,
-                        int clend, const double* w, const double* wgap, const double* werr,
-                        double spdiam, double clgapl, double clgapr, double pivmin,
-                        double* sigma, double* dplus, double* lplus, double* wk, int ws) {
+SM_HD inline int dlarrf(int n, const double* __restrict__ d, const double* __restrict__ l,
+                        const double* __restrict__ ld, int clstrt, int clend, const double* w,
+                        const double* wgap, const double* werr, double spdiam, double clgapl,
+                        double clgapr, double pivmin, double* sigma, double* __restrict__ dplus,
+                        double* __restrict__ lplus, double* __restrict__ wk, int ws) {
 #define W_(i) w[(i) - 1]
 #define WGAP_(i) wgap[(i) - 1]
 #define WERR_(i) werr[(i) - 1]
@@ -1187,28 +1260,34 @@ SM_HD inline int dlarrf(int n, const double* d, const double* l, const double* l
         ldelta = fmin(ldmax, ldelta);
         rdelta = fmin(rdmax, rdelta);
         s = -lsigma;
-        DP(1) = d[0] + s;
-        if (fabs(DP(1)) < pivmin) { DP(1) = -pivmin; sawnan1 = true; }
-        max1 = fabs(DP(1));
+        double dp = d[0] + s;   // the running pivot lives in a register (see dlar1v)
+        if (fabs(dp) < pivmin) { dp = -pivmin; sawnan1 = true; }
+        DP(1) = dp;
+        max1 = fabs(dp);
         for (i = 1; i <= n - 1; ++i) {
-            LP(i) = ld[i - 1] / DP(i);
-            s = s * LP(i) * l[i - 1] - lsigma;
-            DP(i + 1) = d[i] + s;
-            if (fabs(DP(i + 1)) < pivmin) { DP(i + 1) = -pivmin; sawnan1 = true; }
-            max1 = fmax(max1, fabs(DP(i + 1)));
+            const double lp = ld[i - 1] / dp;
+            LP(i) = lp;
+            s = s * lp * l[i - 1] - lsigma;
+            dp = d[i] + s;
+            if (fabs(dp) < pivmin) { dp = -pivmin; sawnan1 = true; }
+            DP(i + 1) = dp;
+            max1 = fmax(max1, fabs(dp));
         }
         sawnan1 = sawnan1 || isnan(max1);
         if (forcer || (max1 <= growthbound && !sawnan1)) { *sigma = lsigma; shift = SLEFT; break; }
         s = -rsigma;
-        WK(1) = d[0] + s;
-        if (fabs(WK(1)) < pivmin) { WK(1) = -pivmin; sawnan2 = true; }
-        max2 = fabs(WK(1));
+        dp = d[0] + s;
+        if (fabs(dp) < pivmin) { dp = -pivmin; sawnan2 = true; }
+        WK(1) = dp;
+        max2 = fabs(dp);
         for (i = 1; i <= n - 1; ++i) {
-            WK(n + i) = ld[i - 1] / WK(i);
-            s = s * WK(n + i) * l[i - 1] - rsigma;
-            WK(i + 1) = d[i] + s;
-            if (fabs(WK(i + 1)) < pivmin) { WK(i + 1) = -pivmin; sawnan2 = true; }
-            max2 = fmax(max2, fabs(WK(i + 1)));
+            const double lp = ld[i - 1] / dp;
+            WK(n + i) = lp;
+            s = s * lp * l[i - 1] - rsigma;
+            dp = d[i] + s;
+            if (fabs(dp) < pivmin) { dp = -pivmin; sawnan2 = true; }
+            WK(i + 1) = dp;
+            max2 = fmax(max2, fabs(dp));
         }
         sawnan2 = sawnan2 || isnan(max2);
         if (forcer || (max2 <= growthbound && !sawnan2)) { *sigma = rsigma; shift = SRIGHT; break; }
@@ -1848,17 +1927,25 @@ SM_HD inline int dlarre_team(int tid, int nt, Shared& sh, int irange, int n, dou
 // ---------------------------------------------------------------------------------
 // dlarrv (odrrv.f:150-896) for the whole spectrum of every block (DOL = 1, DOU = M as
 // dstemr calls it).  The reference walks the representation tree breadth first and
-// treats the children of a node one after the other: an eigenvector (singleton) or a
-// new representation (cluster).  A child reads the gap on its left, which the
-// singleton before it may have widened when it finished (odrrv.f:856-866), and nothing
-// else that its siblings change.  Here the children of a node run side by side, one
-// per thread, each on the gap it would see if no sibling had changed anything; the
-// first thread then replays the node's sequence of gap updates with the results,
-// finds the children whose input gap differs from the one they ran with, and those
-// run again with the right one.  A child's result depends on nothing else, so when
-// no input differs every child holds the result of the sequential order, bit for
-// bit.  Usually one rerun suffices; the first child is always right, so the k-th is
-// right after at most k reruns.
+// treats the nodes of a level, and the children of each node (an eigenvector, i.e. a
+// singleton, or a new representation, i.e. a cluster), one after the other.  What one
+// of them reads from the ones before it is narrow:
+//   * a child reads the gap on its left, which the singleton before it may have
+//     widened when it finished (odrrv.f:856-866);
+//   * a node's refinement (dlarrb, odrrv.f:421-435) reads the gap on its left for its
+//     first eigenvalue, and only in the convergence test of the bisection, and its
+//     boundary updates (odrrv.f:440-449) read the final value and error of its left
+//     neighbour; everything else a node reads is as the level found it.
+// Here up to kVGroup nodes of a level are refined side by side (one thread per
+// eigenvalue), their children all run side by side (one thread per child), each on
+// the inputs it would see if nothing before it had changed them, and then the first
+// thread replays the level in the reference's order with the results: it recomputes
+// every input, finds the children and refinements whose input differs from the one
+// they ran with, and those run again.  A refinement, or a singleton, also records the
+// range of its input over which it would take the very same path (its certificate),
+// and is not rerun for an input in that range.  A result depends on nothing else, so
+// when no input differs every result is the sequential order's, bit for bit.  The
+// k-th item in the order is right after at most k passes; usually one rerun suffices.
 // ---------------------------------------------------------------------------------
 struct VNode {
     int ibegin, in, im, wbegin, wend;
@@ -1871,7 +1958,8 @@ struct VArr {
     double* z;
     int ldz;
     int *isuppz, *twist;
-    const double *snw, *sne, *sns, *sng;
+    const double *snw, *sne, *sns, *sng;   // each node's values before its children ran
+    const double *lw, *le;                 // values and errors as the level found them
 };
 
 // The singleton child k (block-relative) with left/right gap minimum `gap`
@@ -1977,6 +2065,20 @@ SM_HD inline bool sng_same_path(const VNode& v, int k, double g, const double* c
     return cert[0] < gaptol && gaptol <= cert[1] && tol * g < cert[2] && cert[3] <= tol * g;
 }
 
+// True when the refinement of a node's first eigenvalue (dlarrb1 with certificate
+// cert), which ran with left gap gran and right gap rg, would take the same path with
+// left gap g.  The left gap enters only the bisection's test width <= max(rtol1 *
+// min(lg, rg), rtol2 * tmp): the path is the same when rtol1 * min(g, rg) stays below
+// every width that went on and, unless something else ended it, reaches the width it
+// stopped at.
+SM_HD inline bool rb_same_path(const double* cert, double rtol1, double rg, double gran, double g) {
+    if (g == gran) return true;
+    if (cert[3] == 1.0) return true;
+    if (cert[3] != 0.0 || isnan(g)) return false;
+    const double h = rtol1 * fmn(g, rg);
+    return h < cert[0] && (cert[2] != 0.0 || cert[1] <= h);
+}
+
 // The cluster child f..l (block-relative) with left gap `lgap` (odrrv.f:498-611): its
 // end eigenvalues refined to full accuracy, a new representation by dlarrf stored in
 // Z's columns f and f+1 with the accumulated shift, shifted values and fudged errors.
@@ -1986,6 +2088,7 @@ SM_HD inline int cl_compute(const VNode& v, const VArr& a, const double* D0, con
     const double rqtol = 2.0 * kEps;
     const int wb = v.wbegin, in = v.in;
     for (int k = f; k <= l; ++k) {
+        a.w[wb + k - 2] = a.snw[wb + k - 2];
         a.wsh[wb + k - 2] = a.sns[wb + k - 2];
         a.werr[wb + k - 2] = a.sne[wb + k - 2];
     }
@@ -2016,15 +2119,18 @@ SM_HD inline int cl_compute(const VNode& v, const VArr& a, const double* D0, con
     return 0;
 }
 
-// The node's sequence of gap updates replayed with the children's current results
-// (one thread): the input each child should have run with, compared with the one it
-// ran with.  Returns the number of children to run again.
-SM_HD inline int node_replay(const VNode& v, const VArr& a, int oldfst, int oldlst, int nch,
-                             const int* ch, double* inp, int* dirty, const double* cert) {
-    const int wb = v.wbegin;
-    for (int j = imx(oldfst - 1, 1); j <= oldlst; ++j) a.wgap[wb + j - 2] = a.sng[wb + j - 2];
+// A node's sequence of gap updates replayed with its children's current results (one
+// thread): the input each child should have run with, compared with the one it ran
+// with.  lbg: the gap on the node's left as the node's boundary update left it.  The
+// node's children are entries oldfst-1 .. oldfst-2+nch of ch, inp, dirty, cert.
+// Returns the number of children to run again.
+SM_HD inline int node_replay(const VNode& v, const VArr& a, int oldfst, int oldlst, double lbg,
+                             int nch, const int* ch, double* inp, int* dirty, const double* cert) {
+    const int wb = v.wbegin, c0 = oldfst - 1;
+    if (oldfst > 1) a.wgap[wb + oldfst - 3] = lbg;
+    for (int j = oldfst; j <= oldlst; ++j) a.wgap[wb + j - 2] = a.sng[wb + j - 2];
     int ndirty = 0;
-    for (int c = 0; c < nch; ++c) {
+    for (int c = c0; c < c0 + nch; ++c) {
         const int f = ch[2 * c], l = ch[2 * c + 1];
         double in_true;
         if (f == l) {
@@ -2033,14 +2139,15 @@ SM_HD inline int node_replay(const VNode& v, const VArr& a, int oldfst, int oldl
             const double lg = (k == 1) ? kEps * fmx(fabs(left0), fabs(right0)) : a.wgap[x - 2];
             const double rg = (k == v.im) ? kEps * fmx(fabs(left0), fabs(right0)) : a.wgap[x - 1];
             in_true = fmn(lg, rg);
-            // odrrv.f:856-866: widen the gaps on both sides, left neighbour as finished,
-            // right neighbour as it was before this node's children ran
+            // odrrv.f:856-866: widen the gaps on both sides, the left neighbour as
+            // finished, the right one as it was before this node's children ran (a
+            // right neighbour outside the node: as the level found it)
             const double wx = a.w[x - 1], ex = a.werr[x - 1];
             if (k > 1) a.wgap[x - 2] = fmx(a.wgap[x - 2], wx - ex - a.w[x - 2] - a.werr[x - 2]);
             if (x < v.wend) {
                 const bool inside = (k + 1 <= oldlst);
-                const double wn = inside ? a.snw[x] : a.w[x];
-                const double en = inside ? a.sne[x] : a.werr[x];
+                const double wn = inside ? a.snw[x] : a.lw[x];
+                const double en = inside ? a.sne[x] : a.le[x];
                 a.wgap[x - 1] = fmx(a.sng[x - 1], wn - en - wx - ex);
             }
         } else {
@@ -2058,8 +2165,72 @@ SM_HD inline int node_replay(const VNode& v, const VArr& a, int oldfst, int oldl
     return ndirty;
 }
 
-// d, l, ld, lld: the block's current representation (the root from dlarre, later ones
-// restored from Z where the reference stores them).  wk: 4n + 2 doubles for thread 0.
+// A node's children (odrrv.f:453-485): a singleton for every eigenvalue whose right gap
+// is at least minrgp times its shifted value, a cluster for every run between them,
+// stored from entry oldfst-1 of ch; all of them to run.  Returns their number.
+SM_HD inline int node_children(int oldfst, int oldlst, const double* wg, const double* ws, int* ch,
+                               double* inp, int* dirty) {
+    const double minrgp = 1.0e-3;
+    int nch = 0, newfst = oldfst;
+    for (int j = oldfst; j <= oldlst; ++j) {
+        dirty[j - 1] = 0;
+        if (!(j == oldlst || wg[j - 1] >= minrgp * fabs(ws[j - 1]))) continue;
+        const int c = oldfst - 1 + nch;
+        ch[2 * c] = newfst;
+        ch[2 * c + 1] = j;
+        inp[c] = NAN;
+        ++nch;
+        newfst = j + 1;
+    }
+    return nch;
+}
+
+// A level's group of nodes replayed in the reference's order (one thread): for each
+// node the gap on its left as the node before it left it, its first eigenvalue's
+// refinement again if that gap would change it (then its children anew), its left
+// boundary update, then its children's replay.  Returns the number of children to run
+// again.
+SM_HD inline int grp_replay(const VNode& vb, const VArr& a, const Work& wv, int K, int depth,
+                            double rtol1, double rtol2) {
+    const int wb = vb.wbegin;
+    int nd = 0;
+    for (int g = 0; g < K; ++g) {
+        int* gi = wv.GI + 4 * g;
+        double* gd = wv.GD + 10 * g;
+        const int ofst = gi[0], olst = gi[1];
+        VNode v = vb;
+        v.sigma = gd[0];
+        double lbg = 0.0;
+        if (depth > 0 && ofst > 1) {
+            const int xf = wb + ofst - 1;   // the node's first eigenvalue (1-based)
+            const bool adj = g > 0 && wv.GI[4 * (g - 1) + 1] == ofst - 1;
+            const double gprev = adj ? a.wgap[xf - 2] : wv.LG[xf - 2];
+            if (!rb_same_path(gd + 5, rtol1, gd[4], gd[1], gprev)) {
+                const double* s = wv.RS + (long long)g * 4 * v.in;
+                double ws1 = gd[2], we1 = gd[3];
+                dlarrb1(v.in, s, s + 3 * v.in, ofst + a.indexw[wb - 1] - 1, rtol1, rtol2, gd[4],
+                        fmn(gprev, gd[4]), &ws1, &we1, v.pivmin, v.spdiam, v.in, gd + 5);
+                gd[1] = gprev;
+                wv.SNS[xf - 1] = ws1;
+                wv.SNE[xf - 1] = we1;
+                wv.SNG[xf - 1] = fmx(0.0, wv.SNS[xf] - wv.SNE[xf] - ws1 - we1);
+                wv.SNW[xf - 1] = ws1 + v.sigma;
+                a.w[xf - 1] = wv.SNW[xf - 1];
+                a.wsh[xf - 1] = ws1;
+                a.werr[xf - 1] = we1;
+                gi[2] = node_children(ofst, olst, wv.SNG + wb - 1, wv.SNS + wb - 1, wv.CH, wv.INP,
+                                      wv.DIRTY);
+            }
+            // odrrv.f:440-443 with the node's first value as the level found it
+            lbg = fmx(gprev, wv.LW[xf - 1] - wv.SNE[xf - 1] - a.w[xf - 2] - a.werr[xf - 2]);
+        }
+        nd += node_replay(v, a, ofst, olst, lbg, gi[2], wv.CH, wv.INP, wv.DIRTY, wv.CERT);
+    }
+    return nd;
+}
+
+// d, l, ld, lld: the block's root representation (from dlarre); deeper ones are read
+// from Z where the reference stores them, into one slot of wv.RS per node.
 SM_HD inline int dlarrv_team(int tid, int nt, Shared& sh, int n, double vl, double* d,
                              double* l, double* ld, double* lld, double pivmin,
                              const int* isplit, int m, double rtol1, double rtol2, double* w,
@@ -2067,28 +2238,31 @@ SM_HD inline int dlarrv_team(int tid, int nt, Shared& sh, int n, double vl, doub
                              const int* indexw, const double* gers, double* z, int ldz,
                              int* isuppz, int* twist, int* cl1, int* cl2, const Work& wv) {
 #define ZZ(i, j) z[(long long)((i) - 1) + (long long)((j) - 1) * (long long)ldz]
-    const double minrgp = 1.0e-3;
     for (long long idx = tid; idx < (long long)n * m; idx += nt) {
         const long long j = idx / n, i = idx - j * n;
         z[i + j * (long long)ldz] = 0.0;
     }
-    for (int i = tid; i < m; i += nt) twist[i] = 0;
+    for (int i = tid; i < m; i += nt) {
+        twist[i] = 0;
+        wv.DIRTY[i] = 0;
+    }
     if (tid == 0) sh.info = 0;
     SM_SYNC();
     VArr va;
     va.w = w; va.werr = werr; va.wgap = wgap; va.wsh = wsh; va.indexw = indexw; va.z = z;
     va.ldz = ldz; va.isuppz = isuppz; va.twist = twist;
     va.snw = wv.SNW; va.sne = wv.SNE; va.sns = wv.SNS; va.sng = wv.SNG;
+    va.lw = wv.LW; va.le = wv.LE;
+    double* wkt = wv.WKL + (long long)tid * (4 * n + 8);
     const int nblk = (m > 0) ? iblock[m - 1] : 0;
     int ibegin = 1, wbegin = 1;          // thread 0's running state
-    double sigma = 0.0;
     int ndepth = 0, parity = 1, nclus = 0, idone = 0;
     for (int jblk = 1; jblk <= nblk; ++jblk) {
         SM_SYNC();   // every thread has read the previous block's flags
         if (tid == 0) {
             sh.lv[0] = 0;   // block active
             const int iend = isplit[jblk - 1];
-            sigma = l[iend - 1];
+            const double sigma = l[iend - 1];
             int wend = wbegin - 1;
             while (wend < m && iblock[wend] == jblk) ++wend;
             if (wend < wbegin) {
@@ -2122,13 +2296,26 @@ SM_HD inline int dlarrv_team(int tid, int nt, Shared& sh, int n, double vl, doub
                     sh.lv[2] = im;
                     sh.lv[3] = wend;
                     sh.spdiam = gu - gl;
+                    sh.ibeg = ibegin;
+                    sh.wbeg = wbegin;
+                    sh.w[1] = sigma;
                 }
             }
         }
         SM_SYNC();
         if (!sh.lv[0]) continue;
-        const int in = sh.lv[1], im = sh.lv[2], wend = sh.lv[3];
-        const double spdiam = sh.spdiam;
+        const int in = sh.lv[1], im = sh.lv[2], ib = sh.ibeg;
+        VNode vb;
+        vb.ibegin = ib; vb.in = in; vb.im = im; vb.wbegin = sh.wbeg; vb.wend = sh.lv[3];
+        vb.sigma = 0.0; vb.spdiam = sh.spdiam; vb.pivmin = pivmin; vb.vl = vl;
+        const int wb = vb.wbegin;
+        const double rsigma = sh.w[1];
+        // the root representation's D*L and D*L*L
+        for (int j = ib + tid; j <= ib + in - 2; j += nt) {
+            const double tmp = d[j - 1] * l[j - 1];
+            ld[j - 1] = tmp;
+            lld[j - 1] = tmp * l[j - 1];
+        }
         for (;;) {
             if (tid == 0) {
                 sh.lv[4] = (idone < im) ? 1 : 0;
@@ -2138,121 +2325,162 @@ SM_HD inline int dlarrv_team(int tid, int nt, Shared& sh, int n, double vl, doub
                     sh.lv[6] = ndepth;
                     nclus = 0;
                     parity = 1 - parity;
+                    sh.lv[7] = parity;           // the level's nodes: cl1 if 0, else cl2
                 }
             }
             SM_SYNC();
             if (sh.info) return sh.info;
             if (!sh.lv[4]) break;
             const int oldncl = sh.lv[5], depth = sh.lv[6];
-            for (int ic = 1; ic <= oldncl; ++ic) {
-                // node setup (thread 0): restore its representation, D*L and D*L*L
+            const int* oldcl = (sh.lv[7] == 0) ? cl1 : cl2;
+            int* newcl = (sh.lv[7] == 0) ? cl2 : cl1;
+            for (int g0 = 1; g0 <= oldncl; g0 += kVGroup) {
+                const int K = imn(kVGroup, oldncl - g0 + 1);
+                // ---- the group's nodes, their shifts, and the values as the level found
+                // them around the group (thread 0)
                 if (tid == 0) {
-                    int* oldcl = (parity == 0) ? cl1 : cl2;
-                    const int oldfst = oldcl[2 * ic - 1], oldlst = oldcl[2 * ic];
-                    const int iend = ibegin + in - 1;
-                    if (ndepth > 0) {
-                        const int j = wbegin + oldfst - 1;
-                        for (int k = 0; k < in; ++k) d[ibegin - 1 + k] = ZZ(ibegin + k, j);
-                        for (int k = 0; k < in - 1; ++k) l[ibegin - 1 + k] = ZZ(ibegin + k, j + 1);
-                        sigma = ZZ(iend, j + 1);
-                        for (int k = 0; k < in; ++k) { ZZ(ibegin + k, j) = 0.0; ZZ(ibegin + k, j + 1) = 0.0; }
+                    int cnt = 0;
+                    for (int g = 0; g < K; ++g) {
+                        const int ofst = oldcl[2 * (g0 + g) - 1], olst = oldcl[2 * (g0 + g)];
+                        int* gi = wv.GI + 4 * g;
+                        double* gd = wv.GD + 10 * g;
+                        gi[0] = ofst;
+                        gi[1] = olst;
+                        gi[2] = 0;
+                        gd[0] = (depth > 0) ? ZZ(ib + in - 1, wb + ofst) : rsigma;
+                        const int xf = wb + ofst - 1;
+                        gd[1] = (ofst > 1) ? wgap[xf - 2] : 0.0;   // the left gap it is refined with
+                        gd[2] = wsh[xf - 1];
+                        gd[3] = werr[xf - 1];
+                        gd[4] = wgap[xf - 1];
+                        gd[5] = INFINITY; gd[6] = 0.0; gd[7] = 1.0; gd[8] = 1.0;
+                        for (int k = ofst; k <= olst; ++k) {
+                            wv.NODEOF[k - 1] = g;
+                            wv.ITEMS[cnt++] = k;
+                        }
                     }
-                    for (int j = ibegin; j <= iend - 1; ++j) {
-                        const double tmp = d[j - 1] * l[j - 1];
-                        ld[j - 1] = tmp;
-                        lld[j - 1] = tmp * l[j - 1];
+                    const int lo = imx(wb + oldcl[2 * g0 - 1] - 2, wb);
+                    const int hi = imn(wb + oldcl[2 * (g0 + K - 1)], vb.wend);
+                    for (int x = lo; x <= hi; ++x) {
+                        wv.LW[x - 1] = w[x - 1];
+                        wv.LE[x - 1] = werr[x - 1];
+                        wv.LG[x - 1] = wgap[x - 1];
                     }
-                    sh.i0 = indexw[wbegin - 1 + oldfst - 1];
-                    sh.i1 = indexw[wbegin - 1 + oldlst - 1];
-                    sh.offset = indexw[wbegin - 1] - 1;
-                    sh.ibeg = ibegin;
-                    sh.wbeg = wbegin;
-                    sh.lv[7] = oldfst * 65536 + oldlst;
+                    sh.iw[0] = cnt;
                 }
                 SM_SYNC();
+                const int nitems = sh.iw[0];
                 if (depth > 0) {
-                    const int b0 = sh.ibeg, wb0 = sh.wbeg;
-                    dlarrb_team(tid, nt, in, d + b0 - 1, lld + b0 - 1, sh.i0, sh.i1, rtol1, rtol2,
-                                sh.offset, wsh + wb0 - 1, wgap + wb0 - 1, werr + wb0 - 1, pivmin,
-                                spdiam, in);
+                    // ---- each node's representation from Z (odrrv.f:388-403) into its
+                    // slot, with D*L and D*L*L; Z's two columns cleared
+                    for (long long idx = tid; idx < (long long)K * in; idx += nt) {
+                        const int g = (int)(idx / in), k = (int)(idx - (long long)g * in);
+                        const int j = wb + wv.GI[4 * g] - 1;
+                        double* s = wv.RS + (long long)g * 4 * in;
+                        const double dd = ZZ(ib + k, j);
+                        const double ll = (k < in - 1) ? ZZ(ib + k, j + 1) : 0.0;
+                        const double tmp = dd * ll;
+                        s[k] = dd;
+                        s[in + k] = ll;
+                        s[2 * in + k] = tmp;
+                        s[3 * in + k] = tmp * ll;
+                        ZZ(ib + k, j) = 0.0;
+                        ZZ(ib + k, j + 1) = 0.0;
+                    }
+                    SM_SYNC();
+                    // ---- refinement of every node's eigenvalues (odrrv.f:421-435), one
+                    // thread each; the first of a node records its certificate
+                    const int offset = indexw[wb - 1] - 1;
+                    for (int t = tid; t < nitems; t += nt) {
+                        const int k = wv.ITEMS[t], g = wv.NODEOF[k - 1], ofst = wv.GI[4 * g];
+                        const double* s = wv.RS + (long long)g * 4 * in;
+                        const int x = wb + k - 1;
+                        const double rg = wgap[x - 1];
+                        const double lg = (k > 1) ? wgap[x - 2] : rg;
+                        const double gap0 = (k == ofst) ? rg : fmn(lg, rg);
+                        dlarrb1(in, s, s + 3 * in, k + offset, rtol1, rtol2, gap0, fmn(lg, rg),
+                                &wsh[x - 1], &werr[x - 1], pivmin, vb.spdiam, in,
+                                (k == ofst && k > 1) ? wv.GD + 10 * g + 5 : nullptr);
+                    }
+                    SM_SYNC();
+                    for (int t = tid; t < nitems; t += nt) {
+                        const int k = wv.ITEMS[t], g = wv.NODEOF[k - 1];
+                        if (k == wv.GI[4 * g]) continue;
+                        const int x = wb + k - 1;
+                        wgap[x - 2] = fmx(0.0, wsh[x - 1] - werr[x - 1] - wsh[x - 2] - werr[x - 2]);
+                    }
+                    SM_SYNC();
                 }
-                // the node's children (thread 0): odrrv.f:453-485 and the split into
-                // singletons and clusters; the node's values as they are before them
+                // ---- right boundary gaps, shifted values and children of every node
+                // (thread 0): odrrv.f:444-485; the left boundary waits for the replay
                 if (tid == 0) {
-                    const int oldfst = sh.lv[7] / 65536, oldlst = sh.lv[7] % 65536;
-                    double* W_ = w + wbegin - 1;      // 0-based within the block
-                    double* WE = werr + wbegin - 1;
-                    double* WG = wgap + wbegin - 1;
-                    double* WS = wsh + wbegin - 1;
-                    if (ndepth > 0) {
-                        if (oldfst > 1)
-                            WG[oldfst - 2] = fmx(WG[oldfst - 2], W_[oldfst - 1] - WE[oldfst - 1] -
-                                                                     W_[oldfst - 2] - WE[oldfst - 2]);
-                        if (wbegin + oldlst - 1 < wend)
-                            WG[oldlst - 1] = fmx(WG[oldlst - 1], W_[oldlst] - WE[oldlst] -
-                                                                     W_[oldlst - 1] - WE[oldlst - 1]);
-                        for (int j = oldfst; j <= oldlst; ++j) W_[j - 1] = WS[j - 1] + sigma;
+                    for (int g = 0; g < K; ++g) {
+                        int* gi = wv.GI + 4 * g;
+                        const int ofst = gi[0], olst = gi[1];
+                        const double sig = wv.GD[10 * g];
+                        double* WG = wgap + wb - 1;
+                        const double* LW_ = wv.LW + wb - 1;
+                        const double* LE_ = wv.LE + wb - 1;
+                        if (depth > 0) {
+                            if (wb + olst - 1 < vb.wend)
+                                WG[olst - 1] = fmx(WG[olst - 1], LW_[olst] - LE_[olst] - LW_[olst - 1] -
+                                                                     werr[wb + olst - 2]);
+                            for (int j = ofst; j <= olst; ++j) w[wb + j - 2] = wsh[wb + j - 2] + sig;
+                        }
+                        gi[2] = node_children(ofst, olst, WG, wsh + wb - 1, wv.CH, wv.INP, wv.DIRTY);
+                        for (int j = ofst; j <= olst; ++j) {
+                            const int x = wb + j - 1;
+                            wv.SNW[x - 1] = w[x - 1];
+                            wv.SNE[x - 1] = werr[x - 1];
+                            wv.SNS[x - 1] = wsh[x - 1];
+                            wv.SNG[x - 1] = wgap[x - 1];
+                        }
                     }
-                    int nch = 0, newfst = oldfst;
-                    for (int j = oldfst; j <= oldlst; ++j) {
-                        if (!(j == oldlst || WG[j - 1] >= minrgp * fabs(WS[j - 1]))) continue;
-                        wv.CH[2 * nch] = newfst;
-                        wv.CH[2 * nch + 1] = j;
-                        wv.INP[nch] = NAN;
-                        wv.DIRTY[nch] = 0;
-                        ++nch;
-                        newfst = j + 1;
-                    }
-                    for (int j = imx(oldfst - 1, 1); j <= oldlst; ++j) {
-                        const int x = wbegin + j - 1;
-                        wv.SNW[x - 1] = w[x - 1];
-                        wv.SNE[x - 1] = werr[x - 1];
-                        wv.SNS[x - 1] = wsh[x - 1];
-                        wv.SNG[x - 1] = wgap[x - 1];
-                    }
-                    sh.iw[0] = nch;
-                    sh.iw[1] = oldfst;
-                    sh.iw[2] = oldlst;
-                    sh.w[0] = sigma;
                 }
                 SM_SYNC();
-                VNode vn;
-                vn.ibegin = sh.ibeg; vn.in = in; vn.im = im; vn.wbegin = sh.wbeg; vn.wend = wend;
-                vn.sigma = sh.w[0]; vn.spdiam = spdiam; vn.pivmin = pivmin; vn.vl = vl;
-                const int nch = sh.iw[0], ofst = sh.iw[1], olst = sh.iw[2];
-                const double* D0 = d + vn.ibegin - 1;
-                const double* L0 = l + vn.ibegin - 1;
-                const double* LD0 = ld + vn.ibegin - 1;
-                const double* LLD0 = lld + vn.ibegin - 1;
-                double* wkt = wv.WKL + tid;
+                // ---- the children: replay in order (thread 0), run the ones whose input
+                // differs from the one they ran with (all threads)
                 for (;;) {
                     if (tid == 0) {
-                        sh.lv[4] = node_replay(vn, va, ofst, olst, nch, wv.CH, wv.INP, wv.DIRTY, wv.CERT);
+                        sh.lv[4] = grp_replay(vb, va, wv, K, depth, rtol1, rtol2);
                         if (sh.lv[4]) ++sh.passes;
                     }
                     SM_SYNC();
                     if (sh.lv[4] == 0) break;
-                    for (int c = tid; c < nch; c += nt) {
+                    for (int c = tid; c < im; c += nt) {
                         if (!wv.DIRTY[c]) continue;
                         wv.DIRTY[c] = 0;
+                        const int g = wv.NODEOF[c];
+                        VNode vn = vb;
+                        vn.sigma = wv.GD[10 * g];
+                        const double *D0, *L0, *LD0, *LLD0;
+                        if (depth > 0) {
+                            const double* s = wv.RS + (long long)g * 4 * in;
+                            D0 = s; L0 = s + in; LD0 = s + 2 * in; LLD0 = s + 3 * in;
+                        } else {
+                            D0 = d + ib - 1; L0 = l + ib - 1; LD0 = ld + ib - 1; LLD0 = lld + ib - 1;
+                        }
                         const int f = wv.CH[2 * c], lc = wv.CH[2 * c + 1];
-                        const int rc = (f == lc) ? sng_compute(vn, va, D0, L0, LD0, LLD0, f, wv.INP[c], wkt, wv.WKS,
-                                                               wv.CERT + 5 * c)
-                                                 : cl_compute(vn, va, D0, L0, LD0, LLD0, f, lc, wv.INP[c], wkt, wv.WKS);
+                        const int rc = (f == lc) ? sng_compute(vn, va, D0, L0, LD0, LLD0, f, wv.INP[c], wkt,
+                                                               1, wv.CERT + 5 * c)
+                                                 : cl_compute(vn, va, D0, L0, LD0, LLD0, f, lc, wv.INP[c],
+                                                              wkt, 1);
                         if (rc) sh.info = rc;
                     }
                     SM_SYNC();
                     if (sh.info) return sh.info;
                 }
                 if (tid == 0) {
-                    int* newcl = (parity == 0) ? cl2 : cl1;
-                    for (int c = 0; c < nch; ++c) {
-                        const int f = wv.CH[2 * c], lc = wv.CH[2 * c + 1];
-                        if (f == lc) ++idone;
-                        else {
-                            ++nclus;
-                            newcl[2 * nclus - 1] = f;
-                            newcl[2 * nclus] = lc;
+                    for (int g = 0; g < K; ++g) {
+                        const int ofst = wv.GI[4 * g], nch = wv.GI[4 * g + 2];
+                        for (int c = ofst - 1; c < ofst - 1 + nch; ++c) {
+                            const int f = wv.CH[2 * c], lc = wv.CH[2 * c + 1];
+                            if (f == lc) ++idone;
+                            else {
+                                ++nclus;
+                                newcl[2 * nclus - 1] = f;
+                                newcl[2 * nclus] = lc;
+                            }
                         }
                     }
                 }
@@ -2260,10 +2488,11 @@ SM_HD inline int dlarrv_team(int tid, int nt, Shared& sh, int n, double vl, doub
             }
             if (tid == 0) ++ndepth;
             SM_SYNC();
+            SM_MARK(16 + depth);
         }
         if (tid == 0) {
             ibegin = ibegin + in;
-            wbegin = wend + 1;
+            wbegin = vb.wend + 1;
         }
         SM_SYNC();
     }
@@ -2393,6 +2622,7 @@ SM_HD inline Result dstemr_team(int tid, int nt, Shared& sh, const Prob& p, cons
                             relsplit, p.W, wv.WERR, wv.WGAP, wv.IBLOCK, wv.INDEXW, wv.ISPLIT, wv.GERS,
                             wv.WK);
     res.tryrac = tryrac;
+    SM_MARK(1);
     if (iinfo != 0) { res.info = 10 + (iinfo < 0 ? -iinfo : iinfo); return res; }
     const int m = sh.m;
     const int nsplit = sh.nsplit;
@@ -2408,6 +2638,7 @@ SM_HD inline Result dstemr_team(int tid, int nt, Shared& sh, const Prob& p, cons
         for (int j = tid; j < m; j += nt) p.W[j] = p.W[j] + E[wv.ISPLIT[wv.IBLOCK[j] - 1] - 1];
         SM_SYNC();
     }
+    SM_MARK(2);
     if (tryrac) {   // odstmr.f:560-590: dlarrj block by block
         int ibegin = 1, wbegin = 1;
         const int nblk = (m > 0) ? wv.IBLOCK[m - 1] : 0;
@@ -2428,6 +2659,7 @@ SM_HD inline Result dstemr_team(int tid, int nt, Shared& sh, const Prob& p, cons
         for (int j = tid; j < m; j += nt) p.W[j] = (1.0 / scale) * p.W[j];
         SM_SYNC();
     }
+    SM_MARK(3);
     if (nsplit > 1) {   // odstmr.f:596-633
         if (!p.wantz) {
             if (tid == 0) {
