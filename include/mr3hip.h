@@ -158,6 +158,22 @@ int mr3_solve_batch(mr3_problem *probs, int nprob, void *stream);
 int mr3_solve_one(mr3_problem *prob, void *stream);
 
 /*
+ * mr3_solve_batch for problems of order n < 500 (the reference's dstemr route,
+ * SRC/mrrr.c:149-153) that allocates nothing, for callers that cannot allocate (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  No
+ * counterpart in the reference.  The work arrays live in a static device buffer of
+ * 64 MiB that each GPU the library runs on holds until the process ends, and the
+ * descriptors are staged in static host arrays; the caller passes the inputs and
+ * outputs as for mr3_solve_batch (stream NULL: the calling thread's default stream).
+ * Same results, bit for bit, and return codes as mr3_solve_batch, plus 3 -- nothing
+ * run, every info 3 -- when a problem has n >= 500, nprob exceeds 1024, or the
+ * problems' work arrays exceed the buffer (about 1.6 MB for n = 499, proportional to
+ * n below).  The calls of a process are serialised; calls that overlap on one GPU
+ * from several processes share the buffer and race.
+ */
+int mr3_solve_small_noalloc(mr3_problem *probs, int nprob, void *stream);
+
+/*
  * Serving: a request loop that admits problems as device memory and concurrency
  * slots free up, runs them on several streams at once and reports completion.
  * budget_bytes bounds the device memory of admitted problems (8 n m for Z plus the

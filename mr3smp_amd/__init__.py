@@ -97,6 +97,8 @@ def lib():
         L.mr3_solve_batch.argtypes = [ctypes.POINTER(Problem), ctypes.c_int, _P]
         L.mr3_solve_one.restype = ctypes.c_int
         L.mr3_solve_one.argtypes = [ctypes.POINTER(Problem), _P]
+        L.mr3_solve_small_noalloc.restype = ctypes.c_int
+        L.mr3_solve_small_noalloc.argtypes = [ctypes.POINTER(Problem), ctypes.c_int, _P]
         L.mr3_server_create.restype = _P
         L.mr3_server_create.argtypes = [ctypes.c_longlong, ctypes.c_int, ctypes.c_int]
         L.mr3_server_submit.restype = ctypes.c_longlong
@@ -307,6 +309,13 @@ def run_batch(arr, stream=None):
     it points at are read again, the outputs rewritten); returns the return code."""
     st = None if stream is None else ctypes.c_void_p(stream.cuda_stream)
     return lib().mr3_solve_batch(arr, len(arr), st)
+
+
+def run_batch_noalloc(arr, stream=None):
+    """One mr3_solve_small_noalloc call on a Problem array from prepare_batch (every
+    problem of order n < 500); returns the return code."""
+    st = None if stream is None else ctypes.c_void_p(stream.cuda_stream)
+    return lib().mr3_solve_small_noalloc(arr, len(arr), st)
 
 
 def prepare_batch(problems):

@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "../../include/mr3hip.h"
+#include "mr3_args.h"
 #include "mr3_internal.h"
 #include "mr3_staging.h"
 
@@ -37,33 +38,6 @@ static hipStream_t thread_stream() {
     return s;
 }
 
-static int upper(char c) { return (c >= 'a' && c <= 'z') ? c - 32 : c; }
-
-
-// Argument checks of mrrr.c:118-126.  Returns 0 if valid.
-static int check_args(int jz, int rg, int n, double vl, double vu, int il, int iu) {
-    if (!(jz == 'N' || jz == 'V' || jz == 'C')) return 1;
-    if (!(rg == 'A' || rg == 'V' || rg == 'I')) return 1;
-    if (n <= 0) return 1;
-    if (rg == 'V') { if (vu <= vl) return 1; }
-    else if (rg == 'I') { if (il < 1 || il > n || iu < il || iu > n) return 1; }
-    return 0;
-}
-
-static MatDesc to_desc(const mr3_problem& p) {
-    MatDesc d;
-    memset(&d, 0, sizeof(d));
-    d.Din = p.D; d.Ein = p.E; d.Wout = p.W; d.Z = p.Z; d.Zsupp = p.Zsupp;
-    d.vl = p.vl; d.vu = p.vu; d.n = p.n;
-    int jz = upper(p.jobz), rg = upper(p.range);
-    d.jobz = jz == 'V' ? kV : (jz == 'N' ? kN : kC);
-    d.range = rg == 'A' ? kAll : (rg == 'V' ? kVal : kInd);
-    d.il = p.il; d.iu = p.iu; d.tryrac = p.tryrac ? 1 : 0;
-    d.ldz = p.ldz > 0 ? p.ldz : p.n;
-    d.mcols = p.mcols;
-    return d;
-}
-
 }  // namespace mr3
 
 using namespace mr3;
@@ -78,20 +52,9 @@ extern "C" int mr3_solve_batch(mr3_problem* probs, int nprob, void* stream) {
     int worst = 0;
     for (int i = 0; i < nprob; ++i) {
         mr3_problem& p = probs[i];
-        p.m = 0;
-        p.info = check_args(upper(p.jobz), upper(p.range), p.n, p.vl, p.vu, p.il, p.iu);
-        p.vl_out = p.vl; p.vu_out = p.vu; p.il_out = p.il; p.iu_out = p.iu;
-        p.tryrac_out = p.tryrac;
-        if (p.info) { worst = std::max(worst, 1); continue; }
-        const int jz = upper(p.jobz), rg = upper(p.range);
-        if (jz == 'C' && rg != 'V') {   // mrrr.c:130-138
-            p.m = (rg == 'A') ? p.n : p.iu - p.il + 1;
-            continue;
-        }
-        if (jz == 'V' && (p.Z == nullptr || p.Zsupp == nullptr || (p.ldz > 0 && p.ldz < p.n) ||
-                          p.mcols < 1)) {
-            p.info = 1; worst = std::max(worst, 1); continue;
-        }
+        const int how = prepare_problem(p);
+        if (how > 0) { worst = std::max(worst, 1); continue; }
+        if (how < 0) continue;
         if (p.n < kSmallRouteBelow) { mdS.push_back(to_desc(p)); idxS.push_back(i); }
         else { mdL.push_back(to_desc(p)); idxL.push_back(i); }
     }
@@ -104,13 +67,7 @@ extern "C" int mr3_solve_batch(mr3_problem* probs, int nprob, void* stream) {
     auto fill = [&](const std::vector<int>& idx, const std::vector<MatState>& ms, int rc) {
         for (size_t j = 0; j < idx.size(); ++j) {
             mr3_problem& p = probs[idx[j]];
-            if (rc != 0) { p.info = 2; worst = 2; continue; }
-            const MatState& S = ms[j];
-            p.m = S.m;
-            p.info = (S.status == kOk) ? 0 : 2;
-            p.vl_out = S.vl; p.vu_out = S.vu; p.il_out = S.il; p.iu_out = S.iu;
-            p.tryrac_out = S.tryrac;
-            if ((p.jobz == 'V' || p.jobz == 'v') && p.m > p.mcols) p.info = 2;
+            take_state(p, rc == 0 ? ms[j] : MatState(), rc);
             if (p.info) worst = std::max(worst, p.info);
         }
     };

@@ -903,3 +903,4 @@ void get_tuning(mr3_tuning* t) {
 }
 
 }  // namespace mr3
+#include "mr3_noalloc.hip"
