@@ -27,11 +27,15 @@ sys.path.insert(0, os.path.join(REPO, "tests"))
 # FP64 operations per counted element, per kernel family (algorithmic work:
 # Sturm step sub/div/sub = 3, dlaneg add/div/mul/sub = 4, dlar1v 16 per element,
 # dlarrf sweep 5 per element).
+# The small-matrix route (k_small, n < 500, dstemr): dqds step 5 (add, divide, two
+# multiplies, subtract), dlaneg 4, Sturm step on T 3, dlar1v row 16, dlarrf element 5.
 FLOPS_PER_STEP = {"coarse": 3, "refine": 4, "endref": 4, "r1v": 16, "r1v_bisect": 4,
-                  "rrf": 5, "rrj": 3, "refine_cl": 4}
+                  "rrf": 5, "rrj": 3, "refine_cl": 4,
+                  "sm_dqds": 5, "sm_neg": 4, "sm_sturm": 3, "sm_r1v": 16, "sm_rrf": 5}
 KERNEL_STEPS = {"k_coarse": ["coarse"], "k_refine_root": ["refine"], "k_endref": ["endref"],
                 "k_s_iter": ["r1v"], "k_s_bisect": ["r1v_bisect"], "k_dlarrf": ["rrf"], "k_rrj": ["rrj"],
-                "k_refine_cl": ["refine_cl"]}
+                "k_refine_cl": ["refine_cl"],
+                "k_small": ["sm_dqds", "sm_neg", "sm_sturm", "sm_r1v", "sm_rrf"]}
 FP64_PEAK_TFLOPS = 78.6
 HBM_PEAK_GBS = 8000.0
 # The divide-bound ceiling: the IEEE FP64 division sequence the compiler emits is ten
@@ -45,7 +49,8 @@ DIVIDE_STEPS_PER_S = 256 * 4 * 2.4e9 * 64 / 52.0
 # divisions the reference's algorithm needs per counted step of each family (dlar1v: the
 # stationary and the progressive transform; the vector solve multiplies)
 DIVIDES_PER_STEP = {"coarse": 1, "refine": 1, "endref": 1, "r1v": 2, "r1v_bisect": 1,
-                    "rrf": 1, "rrj": 1, "refine_cl": 1}
+                    "rrf": 1, "rrj": 1, "refine_cl": 1,
+                    "sm_dqds": 1, "sm_neg": 1, "sm_sturm": 1, "sm_r1v": 2, "sm_rrf": 1}
 
 
 def log(*a):
@@ -81,8 +86,12 @@ def accuracy_sample(mats, keep, arr, stride):
 
 def cpu_baseline(mats, stride, threads):
     """The compiled reference (oracle/_ref) when present, else the oracle, on every
-    stride-th matrix, `threads` concurrent callers with PMR_NUM_THREADS=1 (the
-    reference's sequential dstemr route; it is reentrant)."""
+    stride-th matrix of the batched workload, run the two ways a CPU user would run it:
+    'callers' -- `threads` concurrent callers with PMR_NUM_THREADS=1 (every matrix takes
+    the reference's sequential dstemr route; the routine is reentrant); 'threads' -- one
+    matrix at a time with PMR_NUM_THREADS=threads (the reference's own routing: dstemr
+    below n = 500, the task-parallel route above).  value is the better of the two (the
+    oracle, being sequential, has the first only)."""
     from concurrent.futures import ThreadPoolExecutor
     from mrrr_ctypes import ORACLE_SO, REF_SO, call_mrrr, load
     if os.path.exists(REF_SO):
@@ -93,7 +102,6 @@ def cpu_baseline(mats, stride, threads):
         kind = "port"
     else:
         return None
-    os.environ["PMR_NUM_THREADS"] = "1"
     sample = mats[::stride]
     pairs = sum(len(D) for D, _, _ in sample)
 
@@ -102,15 +110,29 @@ def cpu_baseline(mats, stride, threads):
         r = call_mrrr(fn, "V", "A", D, E)
         return r["m"]
 
+    modes = {}
+    os.environ["PMR_NUM_THREADS"] = "1"
     t0 = time.perf_counter()
     with ThreadPoolExecutor(max_workers=threads) as ex:
         ms = list(ex.map(one, sample))
     dt = time.perf_counter() - t0
     assert sum(ms) == pairs
-    return dict(value=pairs / dt, unit="eigenpairs/s", cores=threads, kind=kind,
+    modes["callers"] = dict(value=pairs / dt, wall_s=dt,
+                            how=f"{threads} concurrent callers, PMR_NUM_THREADS=1")
+    if kind == "reference":
+        os.environ["PMR_NUM_THREADS"] = str(threads)
+        t0 = time.perf_counter()
+        ms = [one(t) for t in sample]
+        dt = time.perf_counter() - t0
+        assert sum(ms) == pairs
+        modes["threads"] = dict(value=pairs / dt, wall_s=dt,
+                                how=f"one matrix at a time, PMR_NUM_THREADS={threads}")
+    os.environ.pop("PMR_NUM_THREADS", None)
+    best = max(modes, key=lambda k: modes[k]["value"])
+    return dict(value=modes[best]["value"], unit="eigenpairs/s", cores=threads, kind=kind,
+                mode=best, modes=modes,
                 sample=f"every {stride}th matrix of the batched workload ({len(sample)} matrices, "
-                       f"{pairs} eigenpairs, jobz=V range=A), {threads} concurrent callers with "
-                       f"PMR_NUM_THREADS=1, wall {dt:.2f} s")
+                       f"{pairs} eigenpairs, jobz=V range=A); value: the better mode ({best})")
 
 
 def timed_steps(M, torch, arr, stream, steps, warmup, pairs):
@@ -165,12 +187,14 @@ def plain_path(M, torch, probs, stream, stride):
     return plain, len(sample)
 
 
-def roofline(prof, counted, prof_steps, traffic_key):
-    """Roofline of the dominant kernel: its algorithmic flops per step (from the step
-    run with the device work counters on) over its launches per step, against its
-    average launch time in the timed steps.  Returns (kernel name, roofline dict)."""
+def roofline(prof, counted, prof_steps, traffic_key, dom=None):
+    """Roofline of the dominant kernel (or of `dom`): its algorithmic flops per step
+    (from the step run with the device work counters on) over its launches per step,
+    against its average launch time in the timed steps.  Returns (kernel name,
+    roofline dict)."""
     kern = prof["kernels"]
-    dom = max(kern, key=lambda k: kern[k]["ms"])
+    if dom is None:
+        dom = max(kern, key=lambda k: kern[k]["ms"])
     steps = counted["steps"]
     launches_per_step = kern[dom]["launches"] / prof_steps
     assert counted["kernels"][dom]["launches"] == launches_per_step, "launch count differs between steps"
@@ -191,7 +215,7 @@ def roofline(prof, counted, prof_steps, traffic_key):
             traffic = rec.get(dom) if isinstance(rec, dict) else None
         except Exception:
             traffic = None
-    return dom, {"bound": "mfma", "engine": "fp64 vector ALU (no MFMA on this path)",
+    return dom, {"bound": "fp64_valu", "engine": "fp64 vector ALU (no MFMA on this path)",
                  "kernel": dom, "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
                  "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
                  "divide_ceiling": divide_ceiling,
@@ -200,34 +224,64 @@ def roofline(prof, counted, prof_steps, traffic_key):
                  "ms_per_launch": per_launch_s * 1e3}
 
 
-def serve_stream(M, count, rate, budget, inflight=3, batch=64):
+def serve_stream(M, count, rate, budget, inflight=3, batch=64, window=None):
+    """The served stream through the server, open loop (Poisson arrivals at `rate`
+    requests/s) or, with `window`, closed loop (at most `window` requests outstanding, a
+    new one submitted as soon as one is taken back: the server's capacity).  A waiter
+    thread takes the results back in submission order and keeps only latency and m, so
+    that host memory holds the outstanding requests only."""
+    import queue
+    import threading
     from mr3smp_amd import workloads as WL
-    reqs = WL.served_stream(count, rate)
+    reqs = WL.served_stream(count, rate if rate else 1.0)
     srv = M.Server(budget_bytes=budget, max_inflight=inflight, max_batch=batch)
+    q = queue.Queue()
+    got = []
+    slots = threading.Semaphore(window) if window else None
+
+    def waiter():
+        while True:
+            rid = q.get()
+            if rid is None:
+                return
+            o = srv.wait(rid)
+            got.append((o["t_done"] - o["t_submit"], o["m"], o["info"]))
+            del o
+            if slots:
+                slots.release()
+
+    th = threading.Thread(target=waiter)
+    th.start()
     t0 = time.perf_counter()
-    ids = []
-    for r in reqs:
-        while time.perf_counter() - t0 < r["t"]:
-            time.sleep(min(0.002, max(0.0, r["t"] - (time.perf_counter() - t0))))
-        kw = {k: r[k] for k in ("vl", "vu", "il", "iu") if k in r}
-        mc = (r["iu"] - r["il"] + 1) if r["range"] == "I" else None
-        ids.append(srv.submit(r["jobz"], r["range"], r["D"], r["E"], mcols=mc, **kw))
-    outs = [srv.wait(i) for i in ids]
+    try:
+        for r in reqs:
+            if slots:
+                slots.acquire()
+            else:
+                while time.perf_counter() - t0 < r["t"]:
+                    time.sleep(min(0.002, max(0.0, r["t"] - (time.perf_counter() - t0))))
+            kw = {k: r[k] for k in ("vl", "vu", "il", "iu") if k in r}
+            mc = (r["iu"] - r["il"] + 1) if r["range"] == "I" else None
+            q.put(srv.submit(r["jobz"], r["range"], r["D"], r["E"], mcols=mc, **kw))
+    finally:
+        q.put(None)
+        th.join()
     dt = time.perf_counter() - t0
     srv.close()
-    lat = np.array([o["t_done"] - o["t_submit"] for o in outs])
-    pairs = sum(o["m"] for o in outs)
-    assert all(o["info"] == 0 for o in outs)
-    return dict(requests=count, rate_per_s=rate, p50_ms=float(np.percentile(lat, 50) * 1e3),
-                p99_ms=float(np.percentile(lat, 99) * 1e3), eigenpairs_per_s=pairs / dt,
-                wall_s=dt)
+    lat = np.array([g[0] for g in got])
+    pairs = sum(g[1] for g in got)
+    assert len(got) == count and all(g[2] == 0 for g in got)
+    return dict(requests=count, rate_per_s=(count / dt) if window else rate,
+                p50_ms=float(np.percentile(lat, 50) * 1e3), p99_ms=float(np.percentile(lat, 99) * 1e3),
+                eigenpairs_per_s=pairs / dt, wall_s=dt,
+                arrivals=f"closed loop, {window} outstanding" if window else "Poisson")
 
 
 def serve_plain(M, count):
-    """The served stream's requests solved one after the other through the drop-in
-    mrrr() (host arrays in and out), for a like-for-like plain rate."""
+    """The first `count` requests of the served stream solved one after the other
+    through the drop-in mrrr() (host arrays in and out), for a like-for-like plain rate."""
     from mr3smp_amd import workloads as WL
-    reqs = WL.served_stream(count, 1e9)
+    reqs = WL.served_stream(count, 1.0)
     t0 = time.perf_counter()
     pairs = 0
     for r in reqs:
@@ -237,7 +291,7 @@ def serve_plain(M, count):
         assert o["info"] == 0
         pairs += o["m"]
     dt = time.perf_counter() - t0
-    return dict(eigenpairs_per_s=pairs / dt, wall_s=dt)
+    return dict(eigenpairs_per_s=pairs / dt, wall_s=dt, requests=count)
 
 
 def cpu_baseline_subset(mats, stride, threads):
@@ -359,7 +413,11 @@ def main():
     ap.add_argument("--cpu-stride", type=int, default=None,
                     help="the CPU baseline runs on every k-th matrix (default 16 on the mixed "
                          "workload, 32 on the subset workload)")
-    ap.add_argument("--serve-requests", type=int, default=192)
+    ap.add_argument("--serve-requests", type=int, default=2000)
+    ap.add_argument("--serve-window", type=int, default=48,
+                    help="requests outstanding in the closed-loop capacity run")
+    ap.add_argument("--serve-plain-requests", type=int, default=192,
+                    help="requests of the stream solved one by one through mrrr() for the plain rate")
     ap.add_argument("--serve-rate", type=float, default=None,
                     help="arrival rate of the served stream in requests/s (default: 0.8 x the "
                          "capacity measured with every request arriving at once)")
@@ -387,14 +445,8 @@ def main():
                           "n_gpus": args.gpus}))
         sys.exit(2)
 
-    # the solver runs a batch on the caller's stream and up to three side streams, and
-    # the served stream adds a stream per worker: give the process enough hardware
-    # queues that they do not share (HIP's default is 4; set before HIP starts)
-    try:
-        hwq = int(os.environ.get("GPU_MAX_HW_QUEUES", "4"))
-    except ValueError:
-        hwq = 4
-    os.environ["GPU_MAX_HW_QUEUES"] = str(max(hwq, 16))
+    # HIP's hardware queues per process (GPU_MAX_HW_QUEUES) are left as the environment
+    # has them (HIP's default is 4); the solver picks its number of side streams from it
     import torch
     import mr3smp_amd as M
     dev = torch.device("cuda:0")
@@ -439,6 +491,11 @@ def main():
     worst_r, worst_o = accuracy_sample(mats, keep, arr, 64)
 
     dom, roof = roofline(prof, counted, prof_steps, traffic_key="batched_mixed")
+    # the task-parallel pipeline's own dominant kernel beside it (k_small, the small
+    # matrices' route, runs concurrently with it)
+    roof_pipe = None
+    if dom == "k_small" and "k_s_iter" in prof["kernels"]:
+        roof_pipe = roofline(prof, counted, prof_steps, traffic_key="batched_mixed", dom="k_s_iter")[1]
     kern = prof["kernels"]
     steps = counted["steps"]
     total_flops = sum(steps.get(f, 0) * FLOPS_PER_STEP[f] for f in FLOPS_PER_STEP) * prof_steps
@@ -453,9 +510,9 @@ def main():
             cap = None
             rate = args.serve_rate
             if rate is None:
-                # capacity: the same requests, all arriving at once
-                cap = serve_stream(M, args.serve_requests, 1e9, 200 << 30,
-                                   args.serve_inflight, args.serve_batch)
+                # capacity: the same requests, a closed loop that keeps the server busy
+                cap = serve_stream(M, args.serve_requests, None, 200 << 30,
+                                   args.serve_inflight, args.serve_batch, window=args.serve_window)
                 rate = 0.8 * args.serve_requests / cap["wall_s"]
                 log(f"serve capacity: {cap}")
             serve = serve_stream(M, args.serve_requests, rate, 200 << 30, args.serve_inflight,
@@ -466,8 +523,8 @@ def main():
                                      "eigenpairs_per_s": cap["eigenpairs_per_s"],
                                      "p50_ms": cap["p50_ms"], "p99_ms": cap["p99_ms"]}
                 serve["rate_rule"] = "0.8 x capacity"
-                plain_s = serve_plain(M, args.serve_requests)
-                serve["plain_same_requests"] = plain_s
+                plain_s = serve_plain(M, args.serve_plain_requests)
+                serve["plain_first_requests"] = plain_s
                 serve["capacity_over_plain"] = cap["eigenpairs_per_s"] / plain_s["eigenpairs_per_s"]
             log(f"serve: {serve}")
         except Exception as e:  # reported, not fatal for the headline number
@@ -501,6 +558,7 @@ def main():
         "accuracy_sample": {"max_resid": worst_r, "max_orth": worst_o,
                             "units": "n*eps*||T|| and n*eps"},
         "roofline": roof,
+        "roofline_pipeline": roof_pipe,
         "z_writeback": {"bytes_per_step": z_bytes / prof_steps,
                         "GBps_over_normalize_kernel": (z_bytes / (norm_ms / 1e3) / 1e9) if norm_ms else None,
                         "peak_GBps": HBM_PEAK_GBS},

@@ -201,7 +201,13 @@ typedef struct mr3_request {
 mr3_server *mr3_server_create(long long budget_bytes, int max_inflight, int max_batch);
 /* Queues a request; returns its id (>= 0) or -1 if it can never fit the budget. */
 long long mr3_server_submit(mr3_server *s, mr3_request *req);
-/* Blocks until request id is complete; returns its info. */
+/* Blocks until request id is complete; returns its info (0, or 1 invalid argument, 2
+ * failure), also when it completed before the call, and -1 for an id that is unknown or
+ * was already waited for.  A failure of the group's device memory or copies fails every
+ * request of the group; a solver failure only its own request.  A range-V request
+ * without mcols reserves its interval's eigenvalue count plus 4 columns on the device
+ * (the count the reference's jobz "C" query returns) and is solved again with n
+ * columns if the interval holds more. */
 int mr3_server_wait(mr3_server *s, long long id);
 /* Blocks until every submitted request is complete. */
 void mr3_server_drain(mr3_server *s);

@@ -421,7 +421,9 @@ class Server:
         Zs = np.zeros(2 * max(mc, 1), dtype=np.int32) if wantz else None
         req.D, req.E, req.W = _ptr(D), _ptr(E), _ptr(W)
         req.Z, req.Zsupp = (_ptr(Z), _ptr(Zs)) if wantz else (None, None)
-        req.mcols = mc if wantz else 0
+        # 0: no caller limit (Z holds n columns), so that the server may size a range-V
+        # request by its interval's count
+        req.mcols = (0 if mcols is None else mc) if wantz else 0
         rid = self.L.mr3_server_submit(self.h, ctypes.byref(req))
         if rid < 0:
             raise ValueError("request can never fit the memory budget")

@@ -95,7 +95,10 @@ extern "C" int mrrr(char* jobz, char* range, int* np, double* D, double* E, doub
     hipStream_t st = thread_stream();
     if (!st) return 2;
     const bool wantz = (jz == 'V');
-    const int mcols = wantz ? ((rg == 'I') ? (*iu - *il + 1) : n) : 0;
+    // columns of the device Z: the index range, or for range V the count of the
+    // interval plus a margin (the solve is redone with n columns if it finds more)
+    int mcols = !wantz ? 0 : (rg == 'I') ? (*iu - *il + 1)
+                        : (rg == 'V') ? range_v_columns(n, D, E, *vl, *vu) : n;
     double *dD = nullptr, *dE = nullptr, *dW = nullptr, *dZ = nullptr;
     int* dZs = nullptr;
     int rc = 0;
@@ -111,20 +114,28 @@ extern "C" int mrrr(char* jobz, char* range, int* np, double* D, double* E, doub
     if (hipMallocAsync((void**)&dD, sizeof(double) * n, st) != hipSuccess) return fail(2);
     if (hipMallocAsync((void**)&dE, sizeof(double) * n, st) != hipSuccess) return fail(2);
     if (hipMallocAsync((void**)&dW, sizeof(double) * n, st) != hipSuccess) return fail(2);
-    if (wantz) {
-        if (hipMallocAsync((void**)&dZ, sizeof(double) * (size_t)n * mcols, st) != hipSuccess) return fail(2);
-        if (hipMallocAsync((void**)&dZs, sizeof(int) * 2 * (size_t)mcols, st) != hipSuccess) return fail(2);
-    }
     if (hipMemcpyAsync(dD, D, sizeof(double) * n, hipMemcpyHostToDevice, st) != hipSuccess) return fail(2);
     if (n > 1 && hipMemcpyAsync(dE, E, sizeof(double) * (n - 1), hipMemcpyHostToDevice, st) != hipSuccess)
         return fail(2);
     mr3_problem p;
-    memset(&p, 0, sizeof(p));
-    p.n = n; p.jobz = (char)jz; p.range = (char)rg;
-    p.vl = *vl; p.vu = *vu; p.il = *il; p.iu = *iu; p.tryrac = *tryrac;
-    p.ldz = n; p.mcols = mcols;
-    p.D = dD; p.E = dE; p.W = dW; p.Z = dZ; p.Zsupp = dZs;
-    rc = mr3_solve_batch(&p, 1, st);
+    for (int attempt = 0;; ++attempt) {
+        if (wantz) {
+            if (hipMallocAsync((void**)&dZ, sizeof(double) * (size_t)n * mcols, st) != hipSuccess) return fail(2);
+            if (hipMallocAsync((void**)&dZs, sizeof(int) * 2 * (size_t)mcols, st) != hipSuccess) return fail(2);
+        }
+        memset(&p, 0, sizeof(p));
+        p.n = n; p.jobz = (char)jz; p.range = (char)rg;
+        p.vl = *vl; p.vu = *vu; p.il = *il; p.iu = *iu; p.tryrac = *tryrac;
+        p.ldz = n; p.mcols = mcols;
+        p.D = dD; p.E = dE; p.W = dW; p.Z = dZ; p.Zsupp = dZs;
+        rc = mr3_solve_batch(&p, 1, st);
+        if (rc == 0 || !(wantz && rg == 'V' && attempt == 0 && p.m > mcols)) break;
+        (void)hipFreeAsync(dZ, st);
+        (void)hipFreeAsync(dZs, st);
+        dZ = nullptr;
+        dZs = nullptr;
+        mcols = n;
+    }
     if (rc != 0) return fail(rc == 1 ? 1 : 2);
     *m = p.m;
     *vl = p.vl_out; *vu = p.vu_out; *il = p.il_out; *iu = p.iu_out; *tryrac = p.tryrac_out;

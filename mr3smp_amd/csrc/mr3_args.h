@@ -55,6 +55,25 @@ inline MatDesc to_desc(const mr3_problem& p) {
     return d;
 }
 
+// Eigenvectors to reserve for a range-V problem: the count of eigenvalues in (vl, vu]
+// by the Sturm sequence of T (as dlarrc does with JOBT = 'T', odrrc.f:95-113, which is
+// how the reference's dstemr answers the count query) plus a margin of 4 for values
+// within rounding of vl or vu, at most n.  The solve reports info 2 if it finds more
+// (m > mcols), and mrrr() then solves again with n columns.  Host arrays.
+inline int range_v_columns(int n, const double* D, const double* E, double vl, double vu) {
+    double lp = D[0] - vl, rp = D[0] - vu;
+    int lc = (lp <= 0.0) ? 1 : 0, rc = (rp <= 0.0) ? 1 : 0;
+    for (int i = 1; i < n; ++i) {
+        const double t = E[i - 1] * E[i - 1];
+        lp = (D[i] - vl) - t / lp;
+        rp = (D[i] - vu) - t / rp;
+        if (lp <= 0.0) ++lc;
+        if (rp <= 0.0) ++rc;
+    }
+    const int c = rc - lc + 4;
+    return c < 1 ? 1 : (c > n ? n : c);
+}
+
 // A device result into the problem's scalar outputs (rc: the solve's return code).
 inline void take_state(mr3_problem& p, const MatState& S, int rc) {
     if (rc != 0) { p.info = 2; return; }

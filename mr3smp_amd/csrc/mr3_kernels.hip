@@ -43,7 +43,12 @@ enum StepKind : int {
     kStepTR1v = 13,      // wave time in the twisted factorisations (dlar1v to the twist)
     kStepTSolve = 14,    // wave time in the vector solves and column writes
     kStepRrfN = 15,      // block rows of the clusters given to dlarrf (one count per cluster)
-    kStepKinds = 16
+    kStepSmDqds = 16,    // small-matrix route: dqds steps, 5 flops
+    kStepSmNeg = 17,     //   dlaneg steps, 4 flops
+    kStepSmSturm = 18,   //   Sturm steps on T, 3 flops
+    kStepSmR1v = 19,     //   dlar1v rows, 16 flops
+    kStepSmRrf = 20,     //   dlarrf elements, 5 flops
+    kStepKinds = 21
 };
 __device__ unsigned long long g_steps[kStepKinds];
 __constant__ int g_count_steps;
@@ -2958,7 +2963,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
         const int bbeg = __builtin_amdgcn_readfirstlane(t.bbeg);
         const MatDesc& dsc = md[t.mat];
         int col = Zidx[t.slot];
-        if (col >= dsc.mcols) col = dsc.mcols - 1;   // never write past the caller's Z
+        // never write past the caller's Z; such a problem is reported failed (k_finalize)
+        if (col >= dsc.mcols) col = dsc.mcols - 1;
         ColOut co;
         co.zcol = zcol;
         co.wr0 = wr0;
@@ -3862,6 +3868,13 @@ __global__ __launch_bounds__(256) void k_finalize(const MatDesc* __restrict__ md
     const int off = dsc.off, m = S.m;
     const double inv = 1.0 / S.scale;
     const bool wantz = dsc.jobz == kV;
+    if (wantz && m > dsc.mcols) {
+        // more eigenvectors than the caller's Z holds: the problem fails (info 2) and
+        // nothing is moved, so that no access reaches past Z or Zsupp (the vector
+        // phase wrote the extra vectors into the last column, inside the buffer)
+        if (tid == 0) { S.status = kTooFew; S.done = 1; }
+        return;
+    }
     const GPtrW Wout = as_global_w(dsc.Wout);
     for (int s = tid; s < m; s += blockDim.x) {
         double w = W[off + s];

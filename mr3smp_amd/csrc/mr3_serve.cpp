@@ -29,6 +29,7 @@
 #include <vector>
 
 #include "../../include/mr3hip.h"
+#include "mr3_args.h"
 #include "mr3_staging.h"
 
 namespace mr3 {
@@ -41,9 +42,20 @@ using mr3::copy_out;
 using mr3::Staging;
 
 constexpr long long kRowBytes = 512;       // per-row workspace estimate of the pipeline
-long long footprint(const mr3_request& r) {
-    long long cols = 0;
-    if (r.jobz == 'V' || r.jobz == 'v') cols = r.mcols > 0 ? r.mcols : r.n;
+
+// Eigenvector columns a request takes on the device: the caller's mcols if given,
+// else iu - il + 1 for range I, the interval's count plus a margin for range V (the
+// reference's jobz = 'C' query, mrrr.c:130-138), n for range A.
+int device_columns(const mr3_request& r) {
+    if (!(r.jobz == 'V' || r.jobz == 'v')) return 0;
+    if (r.mcols > 0) return r.mcols;
+    const int rg = mr3::upper(r.range);
+    if (rg == 'I') return std::max(1, r.iu - r.il + 1);
+    if (rg == 'V' && r.n > 0 && r.vu > r.vl) return mr3::range_v_columns(r.n, r.D, r.E, r.vl, r.vu);
+    return std::max(1, r.n);
+}
+
+long long footprint(const mr3_request& r, int cols) {
     return 8LL * r.n * cols + kRowBytes * r.n + 64LL * r.n;
 }
 
@@ -51,6 +63,7 @@ struct Entry {
     long long id;
     mr3_request* req;
     long long bytes;
+    int cols;              // device eigenvector columns
 };
 
 }  // namespace
@@ -63,6 +76,7 @@ struct mr3_server {
     std::deque<Entry> pending;
     std::deque<std::vector<Entry>> groups;               // admitted, waiting for a worker
     std::unordered_map<long long, mr3_request*> live;   // submitted, not yet complete
+    std::unordered_map<long long, mr3_request*> reqs;   // submitted, not yet waited for
     std::mutex mu;
     std::condition_variable cv_sched, cv_done, cv_work;
     std::thread sched;
@@ -83,40 +97,61 @@ void mr3_server::run_group(std::vector<Entry>& grp, hipStream_t st, Staging& sg)
     auto take = [&](size_t b) { size_t o = bytes; bytes += (b + 255) & ~(size_t)255; return o; };
     for (size_t i = 0; i < grp.size(); ++i) {
         const mr3_request& r = *grp[i].req;
-        const bool wz = (r.jobz == 'V' || r.jobz == 'v');
-        const int mc = wz ? (r.mcols > 0 ? r.mcols : r.n) : 0;
+        const int mc = grp[i].cols;
         offs[5 * i + 0] = take(sizeof(double) * r.n);
         offs[5 * i + 1] = take(sizeof(double) * r.n);
         offs[5 * i + 2] = take(sizeof(double) * r.n);
         offs[5 * i + 3] = take(sizeof(double) * (size_t)r.n * mc);
         offs[5 * i + 4] = take(sizeof(int) * 2 * (size_t)(mc > 0 ? mc : 1));
     }
+    // transport: the group's allocation, copies and synchronisation; a failure there
+    // fails every request of the group, a solver failure only its own request
+    bool transport_ok = true;
     char* base = nullptr;
-    int rc = (hipMallocAsync((void**)&base, bytes ? bytes : 1, st) == hipSuccess) ? 0 : 2;
+    if (hipMallocAsync((void**)&base, bytes ? bytes : 1, st) != hipSuccess) {
+        base = nullptr;
+        transport_ok = false;
+    }
     std::vector<mr3_problem> probs(grp.size());
-    if (rc == 0) {
-        for (size_t i = 0; i < grp.size(); ++i) {
+    if (transport_ok) {
+        for (size_t i = 0; i < grp.size() && transport_ok; ++i) {
             const mr3_request& r = *grp[i].req;
-            const bool wz = (r.jobz == 'V' || r.jobz == 'v');
+            const bool wz = grp[i].cols > 0;
             mr3_problem& p = probs[i];
             memset(&p, 0, sizeof(p));
             p.n = r.n; p.jobz = r.jobz; p.range = r.range;
             p.vl = r.vl; p.vu = r.vu; p.il = r.il; p.iu = r.iu; p.tryrac = r.tryrac;
             p.ldz = r.n;
-            p.mcols = wz ? (r.mcols > 0 ? r.mcols : r.n) : 0;
+            p.mcols = grp[i].cols;
             p.D = (const double*)(base + offs[5 * i + 0]);
             p.E = (const double*)(base + offs[5 * i + 1]);
             p.W = (double*)(base + offs[5 * i + 2]);
             p.Z = wz ? (double*)(base + offs[5 * i + 3]) : nullptr;
             p.Zsupp = wz ? (int*)(base + offs[5 * i + 4]) : nullptr;
-            (void)hipMemcpyAsync((void*)p.D, r.D, sizeof(double) * r.n, hipMemcpyHostToDevice, st);
-            if (r.n > 1)
-                (void)hipMemcpyAsync((void*)p.E, r.E, sizeof(double) * (r.n - 1), hipMemcpyHostToDevice, st);
+            if (hipMemcpyAsync((void*)p.D, r.D, sizeof(double) * r.n, hipMemcpyHostToDevice, st) != hipSuccess)
+                transport_ok = false;
+            if (r.n > 1 && hipMemcpyAsync((void*)p.E, r.E, sizeof(double) * (r.n - 1), hipMemcpyHostToDevice,
+                                          st) != hipSuccess)
+                transport_ok = false;
         }
-        rc = mr3_solve_batch(probs.data(), (int)probs.size(), st);
+    }
+    if (transport_ok) {
+        (void)mr3_solve_batch(probs.data(), (int)probs.size(), st);
         for (size_t i = 0; i < grp.size(); ++i) {
             mr3_request& r = *grp[i].req;
-            const mr3_problem& p = probs[i];
+            mr3_problem& p = probs[i];
+            // a range-V request without mcols (its host Z holds n columns) whose
+            // interval held more than its count's margin runs again alone with n
+            // columns (mrrr() does the same); the retry's memory is outside the budget
+            char* big = nullptr;
+            if (p.info == 2 && p.Z && p.m > p.mcols && r.mcols <= 0 && mr3::upper(r.range) == 'V' &&
+                hipMallocAsync((void**)&big, sizeof(double) * (size_t)r.n * r.n + sizeof(int) * 2 * r.n + 256,
+                               st) == hipSuccess) {
+                p.mcols = r.n;
+                p.Z = (double*)big;
+                p.Zsupp = (int*)(big + (((sizeof(double) * (size_t)r.n * r.n) + 255) & ~(size_t)255));
+                (void)mr3_solve_batch(&p, 1, st);
+            }
             r.m = p.m;
             r.info = p.info;
             if (p.info == 0 && p.m > 0) {
@@ -124,16 +159,17 @@ void mr3_server::run_group(std::vector<Entry>& grp, hipStream_t st, Staging& sg)
                 if (r.W) c |= copy_out(r.W, p.W, sizeof(double) * p.m, st, sg);
                 if (p.Z && r.Z) c |= copy_out(r.Z, p.Z, sizeof(double) * (size_t)r.n * p.m, st, sg);
                 if (p.Zsupp && r.Zsupp) c |= copy_out(r.Zsupp, p.Zsupp, sizeof(int) * 2 * p.m, st, sg);
-                if (c) rc = 2;
+                if (c) r.info = 2;
             }
+            if (big) (void)hipFreeAsync(big, st);
         }
-        (void)hipFreeAsync(base, st);
-        if (hipStreamSynchronize(st) != hipSuccess) rc = 2;
     }
+    if (base) (void)hipFreeAsync(base, st);
+    if (hipStreamSynchronize(st) != hipSuccess) transport_ok = false;
     const double t1 = mr3_now();
     std::lock_guard<std::mutex> lk(mu);
     for (auto& e : grp) {
-        if (rc == 2 && e.req->info == 0) e.req->info = 2;
+        if (!transport_ok) e.req->info = 2;
         e.req->t_done = t1;
         in_use -= e.bytes;
         live.erase(e.id);
@@ -224,7 +260,8 @@ extern "C" mr3_server* mr3_server_create(long long budget_bytes, int max_infligh
 extern "C" long long mr3_server_submit(mr3_server* s, mr3_request* req) {
     Entry e;
     e.req = req;
-    e.bytes = footprint(*req);
+    e.cols = device_columns(*req);
+    e.bytes = footprint(*req, e.cols);
     req->t_submit = mr3_now();
     req->t_start = req->t_done = 0.0;
     req->m = 0;
@@ -234,17 +271,18 @@ extern "C" long long mr3_server_submit(mr3_server* s, mr3_request* req) {
     e.id = s->next_id++;
     s->pending.push_back(e);
     s->live[e.id] = req;
+    s->reqs[e.id] = req;
     s->cv_sched.notify_all();
     return e.id;
 }
 
 extern "C" int mr3_server_wait(mr3_server* s, long long id) {
     std::unique_lock<std::mutex> lk(s->mu);
-    mr3_request* r = nullptr;
-    auto it = s->live.find(id);
-    if (it == s->live.end()) return 0;
-    r = it->second;
+    auto it = s->reqs.find(id);
+    if (it == s->reqs.end()) return -1;   // unknown, rejected or already waited for
+    mr3_request* r = it->second;
     s->cv_done.wait(lk, [&] { return s->live.find(id) == s->live.end(); });
+    s->reqs.erase(id);
     return r->info;
 }
 

@@ -24,6 +24,12 @@
 #ifndef SM_MARK
 #define SM_MARK(k) ((void)0)
 #endif
+// SM_STEPS(kind, count): an optional hook that counts the route's algorithmic work, once
+// per call of an inner loop (kinds below); none by default.  A profiling build defines
+// it to add to its work counters.
+#ifndef SM_STEPS
+#define SM_STEPS(kind, count) ((void)0)
+#endif
 
 namespace mr3sm {
 
@@ -31,6 +37,9 @@ constexpr double kEps = DBL_EPSILON;   // odmch.f 'P'
 constexpr double kSafmin = DBL_MIN;    // odmch.f 'S'
 constexpr int kAll = 1, kVal = 2, kInd = 3;
 constexpr int kMaxExpand = 128;        // cap on bracket doublings
+// work kinds of SM_STEPS: dqds steps (5 flops: add, divide, two multiplies, subtract),
+// dlaneg steps (4), Sturm steps on T (3), dlar1v rows (16), dlarrf elements (5)
+enum StepKind : int { kCntDqds = 0, kCntNeg = 1, kCntSturm = 2, kCntR1v = 3, kCntRrf = 4 };
 
 SM_HD inline double fmn(double a, double b) { return b < a ? b : a; }
 SM_HD inline double fmx(double a, double b) { return b > a ? b : a; }
@@ -150,6 +159,7 @@ SM_HD inline void carve(Work& w, double* lds, char* base, int n, int wks) {
 // NaN; a NaN quotient persists to the block end, and the re-run differs from the fast
 // loop only at NaN quotients, which it replaces by 1, so replacing per step is the same.
 SM_HD inline int dlaneg(int n, const double* d, const double* lld, double sigma, int r) {
+    SM_STEPS(kCntNeg, n);
     int neg = 0;
     double t = -sigma;
     for (int j = 0; j < r - 1; ++j) {
@@ -174,6 +184,7 @@ SM_HD inline int dlaneg(int n, const double* d, const double* lld, double sigma,
 
 // Count of eigenvalues of T below x, dlaebz's convention (odebz.f:265-277, :396-415).
 SM_HD inline int count_T(int n, const double* d, const double* e2, double x, double pivmin) {
+    SM_STEPS(kCntSturm, n);
     double t = d[0] - x;
     if (fabs(t) < pivmin) t = -pivmin;
     int c = (t <= 0.0) ? 1 : 0;
@@ -188,6 +199,7 @@ SM_HD inline int count_T(int n, const double* d, const double* e2, double x, dou
 // dlaebz IJOB=2/3 step count (odebz.f:396-415): pivot <= pivmin counts, clamp to
 // -pivmin.
 SM_HD inline int count_T2(int n, const double* d, const double* e2, double x, double pivmin) {
+    SM_STEPS(kCntSturm, n);
     double t = d[0] - x;
     int c = 0;
     if (t <= pivmin) { c = 1; t = fmn(t, -pivmin); }
@@ -289,6 +301,7 @@ SM_HD inline void dlarrj1(int n, const double* d, const double* e2, int i, doubl
     double width = right - mid, tmp = fmx(fabs(left), fabs(right));
     if (width < rtol * tmp) return;
     auto cnt = [&](double s) {
+        SM_STEPS(kCntSturm, n);
         int c = 0;
         double dplus = d[0] - s;
         if (dplus < 0.0) ++c;
@@ -339,6 +352,7 @@ SM_HD inline void dlarrk(int n, int iw, double gl, double gu, const double* d, c
 // dlarrc with JOBT = 'T' (odrrc.f:95-113): counts of eigenvalues <= vl and <= vu.
 SM_HD inline void dlarrc_T(int n, double vl, double vu, const double* d, const double* e,
                            int* lcnt, int* rcnt) {
+    SM_STEPS(kCntSturm, 2 * n);
     double lp = d[0] - vl, rp = d[0] - vu;
     int lc = (lp <= 0.0) ? 1 : 0, rc = (rp <= 0.0) ? 1 : 0;
     for (int i = 1; i < n; ++i) {
@@ -548,6 +562,7 @@ SM_HD inline void dlasq4(int i0, int n0, const double* z, int pp, int n0in, doub
 SM_HD inline void dlasq5(int i0, int n0, double* z, int pp, double tau, double* dmin,
                          double* dmin1, double* dmin2, double* dn, double* dnm1, double* dnm2) {
     if ((n0 - i0 - 1) <= 0) return;
+    SM_STEPS(kCntDqds, n0 - i0 + 1);
     int j4 = 4 * i0 + pp - 3, j4p2;
     double emin = SMZ(j4 + 4), d = SMZ(j4) - tau, temp;
     *dmin = d;
@@ -615,6 +630,7 @@ SM_HD inline void dlasq5(int i0, int n0, double* z, int pp, double tau, double* 
 SM_HD inline void dlasq6(int i0, int n0, double* z, int pp, double* dmin, double* dmin1,
                          double* dmin2, double* dn, double* dnm1, double* dnm2) {
     if ((n0 - i0 - 1) <= 0) return;
+    SM_STEPS(kCntDqds, n0 - i0 + 1);
     int j4 = 4 * i0 + pp - 3, j4p2;
     double emin = SMZ(j4 + 4), d = SMZ(j4), temp;
     *dmin = d;
@@ -1043,6 +1059,7 @@ SM_HD inline void dlar1v(int n, int b1, int bn, double lambda, const double* __r
     int r1, r2, neg1, neg2, i;
     double s, dplus, dminus, tmp;
     bool sawnan1, sawnan2;
+    SM_STEPS(kCntR1v, bn - b1 + 1);
     if (*r == 0) { r1 = b1; r2 = bn; } else { r1 = *r; r2 = *r; }
     const int indlpl = 0, indumn = n, inds = 2 * n + 1, indp = 3 * n + 1;
     const double s0 = (b1 == 1) ? 0.0 : LLD_(b1 - 1);
@@ -1259,6 +1276,7 @@ SM_HD inline int dlarrf(int n, const double* __restrict__ d, const double* __res
         sawnan2 = false;
         ldelta = fmin(ldmax, ldelta);
         rdelta = fmin(rdmax, rdelta);
+        SM_STEPS(kCntRrf, n);
         s = -lsigma;
         double dp = d[0] + s;   // the running pivot lives in a register (see dlar1v)
         if (fabs(dp) < pivmin) { dp = -pivmin; sawnan1 = true; }
@@ -1275,6 +1293,7 @@ SM_HD inline int dlarrf(int n, const double* __restrict__ d, const double* __res
         }
         sawnan1 = sawnan1 || isnan(max1);
         if (forcer || (max1 <= growthbound && !sawnan1)) { *sigma = lsigma; shift = SLEFT; break; }
+        SM_STEPS(kCntRrf, n);
         s = -rsigma;
         dp = d[0] + s;
         if (fabs(dp) < pivmin) { dp = -pivmin; sawnan2 = true; }
