@@ -15,7 +15,7 @@
 #define SM_SYNC() __syncthreads()
 // the route's work into the library's step counters (they count only when profiling
 // asks for it; on the host side of a __host__ __device__ function nothing is counted)
-#ifdef __HIP_DEVICE_COMPILE__
+#if defined(__HIP_DEVICE_COMPILE__) && !defined(SM_STEPS)
 #define SM_STEPS(kind, count) ::mr3::add_steps(::mr3::kStepSmDqds + (kind), (unsigned long long)(count))
 #endif
 #include "mr3_small_core.h"

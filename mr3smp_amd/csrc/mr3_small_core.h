@@ -18,9 +18,11 @@
 #include <float.h>
 #include <math.h>
 
-// SM_MARK(k): an optional hook at the route's phase boundaries (k = 1 after dlarre,
-// 2 after dlarrv, 3 after dlarrj, 16 + depth after each level of the tree); none by
-// default.  A profiling build defines it to record a timestamp.
+// SM_MARK(k): an optional hook at the route's phase boundaries (k = 0 at the start,
+// 1 after dlarre, 2 after dlarrv, 3 after dlarrj, 4 before and 5 after each dqds call,
+// 16 + depth after each level of the tree, 32 + i inside dqds and 42 + i inside dlarrv:
+// the time since the previous mark belongs to part i); none by default.  A profiling build defines
+// it to record a timestamp.
 #ifndef SM_MARK
 #define SM_MARK(k) ((void)0)
 #endif
@@ -744,6 +746,7 @@ SM_HD inline void dlasq3(int i0, double* z, Qd& q) {
         n0 = n0 - 2;
     }
     if (pp == 2) pp = 0;
+    SM_MARK(33);   // deflation
     if (q.dmin <= 0.0 || n0 < n0in) {
         if (CBIAS * SMZ(4 * i0 + pp - 3) < SMZ(4 * n0 + pp - 3)) {
             const int ipn4 = 4 * (i0 + n0);
@@ -764,10 +767,13 @@ SM_HD inline void dlasq3(int i0, double* z, Qd& q) {
             q.dmin = -0.0;
         }
     }
+    SM_MARK(34);   // reversal
     dlasq4(i0, n0, z, pp, n0in, q.dmin, q.dmin1, q.dmin2, q.dn, q.dn1, q.dn2, &q.tau, &q.ttype, &q.g);
+    SM_MARK(35);   // shift
     bool use6 = false;
     for (;;) {
         dlasq5(i0, n0, z, pp, q.tau, &q.dmin, &q.dmin1, &q.dmin2, &q.dn, &q.dn1, &q.dn2);
+        SM_MARK(36);   // transform
         q.ndiv += n0 - i0 + 2;
         q.iter++;
         if (q.dmin >= 0.0 && q.dmin1 > 0.0) break;
@@ -792,12 +798,14 @@ SM_HD inline void dlasq3(int i0, double* z, Qd& q) {
         use6 = true;
         break;
     }
+    SM_MARK(37);   // retry decisions
     if (use6) {
         dlasq6(i0, n0, z, pp, &q.dmin, &q.dmin1, &q.dmin2, &q.dn, &q.dn1, &q.dn2);
         q.ndiv += n0 - i0 + 2;
         q.iter++;
         q.tau = 0.0;
     }
+    SM_MARK(38);   // unshifted transform
     if (q.tau < q.sigma) {
         q.desig = q.desig + q.tau;
         t = q.sigma + q.desig;
@@ -809,6 +817,7 @@ SM_HD inline void dlasq3(int i0, double* z, Qd& q) {
     q.sigma = t;
     q.n0 = n0;
     q.pp = pp;
+    SM_MARK(39);   // shift accumulation
 }
 
 // Sorts z[0..n) in decreasing order (dlasrt 'D' leaves the same values in the same
@@ -973,6 +982,7 @@ SM_HD inline int dlasq2(int n, double* z) {
             if (i0 > n0) { split_done = true; break; }
             q.n0 = n0;
             q.pp = pp;
+            SM_MARK(32);   // dlasq2's loop
             dlasq3(i0, z, q);
             n0 = q.n0;
             pp = 1 - q.pp;
@@ -1903,7 +1913,9 @@ SM_HD inline int dlarre_team(int tid, int nt, Shared& sh, int irange, int n, dou
                 }
                 wk[2 * in - 2] = fabs(d[iend - 1]);
                 wk[2 * in - 1] = 0.0;
+                SM_MARK(4);
                 int iinfo = dlasq2(in, wk);
+                SM_MARK(5);
                 if (iinfo != 0) sh.stop = -5;
                 else {
                     for (int i = 1; i <= in; ++i)
@@ -2267,6 +2279,7 @@ SM_HD inline int dlarrv_team(int tid, int nt, Shared& sh, int n, double vl, doub
     }
     if (tid == 0) sh.info = 0;
     SM_SYNC();
+    SM_MARK(42);   // parallel work
     VArr va;
     va.w = w; va.werr = werr; va.wgap = wgap; va.wsh = wsh; va.indexw = indexw; va.z = z;
     va.ldz = ldz; va.isuppz = isuppz; va.twist = twist;
@@ -2322,6 +2335,7 @@ SM_HD inline int dlarrv_team(int tid, int nt, Shared& sh, int n, double vl, doub
             }
         }
         SM_SYNC();
+        SM_MARK(43);   // one thread's work
         if (!sh.lv[0]) continue;
         const int in = sh.lv[1], im = sh.lv[2], ib = sh.ibeg;
         VNode vb;
@@ -2348,6 +2362,7 @@ SM_HD inline int dlarrv_team(int tid, int nt, Shared& sh, int n, double vl, doub
                 }
             }
             SM_SYNC();
+            SM_MARK(43);
             if (sh.info) return sh.info;
             if (!sh.lv[4]) break;
             const int oldncl = sh.lv[5], depth = sh.lv[6];
@@ -2388,6 +2403,7 @@ SM_HD inline int dlarrv_team(int tid, int nt, Shared& sh, int n, double vl, doub
                     sh.iw[0] = cnt;
                 }
                 SM_SYNC();
+                SM_MARK(47);   // a group's set-up
                 const int nitems = sh.iw[0];
                 if (depth > 0) {
                     // ---- each node's representation from Z (odrrv.f:388-403) into its
@@ -2407,6 +2423,7 @@ SM_HD inline int dlarrv_team(int tid, int nt, Shared& sh, int n, double vl, doub
                         ZZ(ib + k, j + 1) = 0.0;
                     }
                     SM_SYNC();
+                    SM_MARK(42);
                     // ---- refinement of every node's eigenvalues (odrrv.f:421-435), one
                     // thread each; the first of a node records its certificate
                     const int offset = indexw[wb - 1] - 1;
@@ -2422,6 +2439,7 @@ SM_HD inline int dlarrv_team(int tid, int nt, Shared& sh, int n, double vl, doub
                                 (k == ofst && k > 1) ? wv.GD + 10 * g + 5 : nullptr);
                     }
                     SM_SYNC();
+                    SM_MARK(44);   // refinement of the nodes' eigenvalues
                     for (int t = tid; t < nitems; t += nt) {
                         const int k = wv.ITEMS[t], g = wv.NODEOF[k - 1];
                         if (k == wv.GI[4 * g]) continue;
@@ -2429,6 +2447,7 @@ SM_HD inline int dlarrv_team(int tid, int nt, Shared& sh, int n, double vl, doub
                         wgap[x - 2] = fmx(0.0, wsh[x - 1] - werr[x - 1] - wsh[x - 2] - werr[x - 2]);
                     }
                     SM_SYNC();
+                    SM_MARK(42);
                 }
                 // ---- right boundary gaps, shifted values and children of every node
                 // (thread 0): odrrv.f:444-485; the left boundary waits for the replay
@@ -2457,6 +2476,7 @@ SM_HD inline int dlarrv_team(int tid, int nt, Shared& sh, int n, double vl, doub
                     }
                 }
                 SM_SYNC();
+                SM_MARK(43);
                 // ---- the children: replay in order (thread 0), run the ones whose input
                 // differs from the one they ran with (all threads)
                 for (;;) {
@@ -2465,6 +2485,7 @@ SM_HD inline int dlarrv_team(int tid, int nt, Shared& sh, int n, double vl, doub
                         if (sh.lv[4]) ++sh.passes;
                     }
                     SM_SYNC();
+                    SM_MARK(46);   // replay
                     if (sh.lv[4] == 0) break;
                     for (int c = tid; c < im; c += nt) {
                         if (!wv.DIRTY[c]) continue;
@@ -2487,6 +2508,7 @@ SM_HD inline int dlarrv_team(int tid, int nt, Shared& sh, int n, double vl, doub
                         if (rc) sh.info = rc;
                     }
                     SM_SYNC();
+                    SM_MARK(45);   // children
                     if (sh.info) return sh.info;
                 }
                 if (tid == 0) {
@@ -2504,6 +2526,7 @@ SM_HD inline int dlarrv_team(int tid, int nt, Shared& sh, int n, double vl, doub
                     }
                 }
                 SM_SYNC();
+                SM_MARK(43);
             }
             if (tid == 0) ++ndepth;
             SM_SYNC();
@@ -2549,6 +2572,7 @@ SM_HD inline Result dstemr_team(int tid, int nt, Shared& sh, const Prob& p, cons
     res.tryrac = p.tryrac;
     double* D = wv.D;
     double* E = wv.E;
+    SM_MARK(0);
     if (p.cnt) {   // odstmr.f:321-336 with NZC = -1 (mrrr.c:414-419)
         if (tid == 0) {
             int lc, rc;
