@@ -1,11 +1,16 @@
 // mr3_driver.hip -- host side of the MI355X MRRR solver: the batched device pipeline
 // and the drop-in mrrr() entry point (HPAC/mr3smp SRC/mrrr.c:87-296).
 //
-// A solve runs the kernels of mr3_kernels.hip on one stream.  The representation tree
-// is walked level by level: after the classify kernel of a level the host reads four
-// counters (singletons, clusters, eigen entries, arena size) and launches the next
-// level with exact grids.  That is one small device-to-host copy per tree level and
-// per batch, independent of the number of matrices in the batch.
+// A solve runs the kernels of mr3_kernels.hip on one stream (with side streams for the
+// singletons of each tree level).  Every launch is sized on the device: the kernels
+// read their item counts (blocks, clusters, eigen entries, singletons, wave tasks) from
+// device memory and walk them with grid-stride loops, and the block lists are built by
+// a kernel, so a grid is only a hint.  The representation tree is walked level by
+// level; the host waits for nothing but each level's counters (an event on a small
+// copy into pinned memory), which give the size of the level's child representation
+// slots, while the side streams go on.  A solve with a fixed number of levels and one
+// pool for all slots needs no count at all: that is what a captured graph replays
+// (mr3_graph_*).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -37,6 +42,13 @@ namespace mr3 {
         }                                                                               \
     } while (0)
 
+// A device failure of the solve (code 2), reported with its place under MR3_DEBUG.
+#define MR3_FAIL()                                                                      \
+    do {                                                                                \
+        if (getenv("MR3_DEBUG")) fprintf(stderr, "mr3: failure at %s:%d\n", __FILE__, __LINE__); \
+        return 2;                                                                       \
+    } while (0)
+
 static inline unsigned cdiv(long long a, long long b) { return (unsigned)((a + b - 1) / b); }
 
 // Launch-shape parameters of a solve (mr3_set_tuning).  The results do not depend on
@@ -61,17 +73,6 @@ static mr3_tuning tuning_snapshot() {
     return g_tune;
 }
 
-// Lanes per eigenvalue for the cluster refinements of a level with `items`
-// eigenvalues: 1 (one-lane kernels) unless a group of 4 or more lanes per eigenvalue
-// keeps the launch within the lane budget; then the largest such power of two up to a
-// wave.  Multisection with G lanes does (G - 1) / log2(G) times the sweeps of
-// one-point bisection in 1 / log2(G) of the sequential rounds.
-static int group_lanes(long long items, long long budget) {
-    if (items * 4 > budget) return 1;
-    int G = 4;
-    while (G < 64 && items * 2 * G <= budget) G *= 2;
-    return G;
-}
 
 // ---- profiling: HIP events around every launch (on the launch stream) and the
 // on-device step counters, accumulated over the solves run while enabled.
@@ -302,11 +303,13 @@ static size_t singleton_scratch_budget() {
     return std::min(b, cap);
 }
 
-// Pinned host copies of the level counters, reused across solves (allocating pinned
-// memory per solve would cost more than the solve of a small matrix).
+// Pinned host copies of the counters of every tree level of a solve, reused across
+// solves (allocating pinned memory per solve would cost more than the solve of a small
+// matrix).
+constexpr int kMaxLevelsPinned = 66;   // levels 0 .. 64 and the block count
 static std::mutex g_pin_mu;
 static std::vector<LevelCounters*> g_pin_free;
-static LevelCounters* pinned_counters_get() {
+static LevelCounters* pinned_levels_get() {
     {
         std::lock_guard<std::mutex> lk(g_pin_mu);
         if (!g_pin_free.empty()) {
@@ -316,28 +319,46 @@ static LevelCounters* pinned_counters_get() {
         }
     }
     LevelCounters* p = nullptr;
-    if (hipHostMalloc((void**)&p, sizeof(LevelCounters), hipHostMallocDefault) != hipSuccess)
+    if (hipHostMalloc((void**)&p, sizeof(LevelCounters) * kMaxLevelsPinned, hipHostMallocDefault) !=
+        hipSuccess)
         return nullptr;
     return p;
 }
-static void pinned_counters_put(LevelCounters* p) {
+static void pinned_levels_put(LevelCounters* p) {
     std::lock_guard<std::mutex> lk(g_pin_mu);
     g_pin_free.push_back(p);
 }
 
-static int env_int(const char* name, int dflt) {
-    const char* v = getenv(name);
-    return v ? atoi(v) : dflt;
-}
 
-// Runs the whole pipeline for the problems described by md (host copy); the device
-// outputs land in the caller's arrays named in md; states are copied back to ms.
-int run_pipeline(std::vector<MatDesc>& md, std::vector<MatState>& ms, hipStream_t st) {
+// Most tree levels a solve may have (the reference has no bound; a deeper tree is
+// reported as a device failure).
+constexpr int kMaxLevels = kMaxLevelsPinned - 2;
+
+// Grid of a grid-stride kernel for an item bound: enough workgroups for the items
+// (items_per_wg each), at most cap.
+static unsigned grid_for(long long items, int items_per_wg, unsigned cap) {
+    const long long g = (items + items_per_wg - 1) / items_per_wg;
+    return (unsigned)std::max<long long>(1, std::min<long long>(g, (long long)cap));
+}
+constexpr unsigned kGridCap = 8192;
+
+// One pass of the pipeline for the problems described by md (host copy).  Every launch
+// is sized on the device: the kernels read their item counts from device memory and
+// walk them with grid-stride loops, so the host never waits for the device inside the
+// solve.  It enqueues the cluster work of the tree levels up to kAhead levels ahead of
+// the last level whose counts it has seen (a level with no clusters ends the walk;
+// levels enqueued past it find nothing to do), and the singleton phase of a level once
+// it knows that the level has singletons.  When `fixed_levels` >= 0 it enqueues
+// exactly that many levels and every singleton phase, without looking at any count,
+// which is what a captured graph replays.  Returns 0, 2 (device failure) or 3 (the
+// representation pool was too small: *pool_need holds the doubles needed).
+static int pipeline_once(std::vector<MatDesc>& md, std::vector<MatState>& ms, hipStream_t st,
+                         size_t pool_cap, size_t* pool_need, int fixed_levels, int* levels_used) {
     configure_pool_once();
     const mr3_tuning tune = tuning_snapshot();
     const int nmat = (int)md.size();
     long long ntot = 0;
-    int nmax = 0;
+    int nmax = 1;
     bool any_v = false, any_n = false;
     // Matrices take their slots of the eigenvalue arrays in order of decreasing size, so
     // that the lanes of a wave, which take consecutive slots, mostly belong to matrices of
@@ -354,13 +375,15 @@ int run_pipeline(std::vector<MatDesc>& md, std::vector<MatState>& ms, hipStream_
         any_v |= (d.jobz == kV);
         any_n |= (d.jobz == kN);
     }
-    if (ntot > 0x7fffffff / 2) return 2;
+    if (ntot > 0x7fffffff / 2) MR3_FAIL();
     const int N = (int)ntot;
+    const bool fixed = fixed_levels >= 0;
     Alloc A(st);
     CallEvents ce;
     ce.on = g_prof.on.load() != 0;
     MatDesc* dmd = A.get<MatDesc>(nmat);
     MatState* dms = A.get<MatState>(nmat);
+    int* dorder = A.get<int>(nmat);
     double* D = A.get_swept(N);
     double* E = A.get_swept(N);
     double* E2 = A.get_swept(N);
@@ -383,69 +406,47 @@ int run_pipeline(std::vector<MatDesc>& md, std::vector<MatState>& ms, hipStream_
     int* isplit = A.get<int>(N);
     BlockRec* blk = A.get<BlockRec>(N);
     int* tcount = A.get<int>(N);
-    if (!A.ok) return 2;
+    int* plist = A.get<int>(N);    // blocks of the shared top of the coarse bisection
+    int* dbl = A.get<int>(N);      // blocks of the root representations
+    int* lcount = A.get<int>(2);   // their lengths
+    if (!A.ok) MR3_FAIL();
     MR3_CHECK(hipMemcpyAsync(dmd, md.data(), sizeof(MatDesc) * nmat, hipMemcpyHostToDevice, st));
+    MR3_CHECK(hipMemcpyAsync(dorder, order.data(), sizeof(int) * nmat, hipMemcpyHostToDevice, st));
 
     LAUNCH(k_prep, dim3(nmat), dim3(256), dmd, dms, D, E, E2, gers, Dcopy,
                        E2copy, isplit, blk, W, Werr, Wcoarse, iblock, Windex, smat);
-    MR3_CHECK(hipGetLastError());
-    ms.resize(nmat);
-    MR3_CHECK(hipMemcpyAsync(ms.data(), dms, sizeof(MatState) * nmat, hipMemcpyDeviceToHost, st));
-    MR3_CHECK(hipStreamSynchronize(st));
-    {
-        // blocks of the split, for the shared top of the coarse bisection
-        std::vector<int> plist;
-        for (int k : order) {
-            if (ms[k].done || ms[k].maxbl < 128) continue;
-            for (int j = 0; j < ms[k].nsplit; ++j) plist.push_back(md[k].off + j);
-        }
-        if (!plist.empty()) {
-            int* dpl = A.get<int>(plist.size());
-            if (!A.ok) return 2;
-            MR3_CHECK(hipMemcpyAsync(dpl, plist.data(), sizeof(int) * plist.size(),
-                                     hipMemcpyHostToDevice, st));
-            LAUNCH(k_coarse_tree, dim3((unsigned)plist.size()), dim3(256), dmd, dms, D, E2, blk,
-                   smat, dpl, (int)plist.size(), tcount);
-        }
-    }
+    LAUNCH(k_block_list, dim3(1), dim3(kListNT), dmd, dms, dorder, nmat, 0, plist, lcount);
+    LAUNCH(k_coarse_tree, dim3(grid_for(N, 1, 2048)), dim3(256), dmd, dms, D, E2, blk, smat, plist,
+           lcount, tcount);
     LAUNCH(k_coarse, dim3(cdiv(N, 256)), dim3(256), dmd, dms, D, E2, blk, smat,
                        iblock, Windex, tcount, W, Werr, Wcoarse, N);
     LAUNCH(k_discard, dim3(cdiv(nmat, 64)), dim3(64), dmd, dms, blk, W, Werr,
                        Wcoarse, iblock, Windex, nmat);
+    LAUNCH(k_block_list, dim3(1), dim3(kListNT), dmd, dms, dorder, nmat, 1, dbl, lcount + 1);
+    const int* nbl_p = lcount + 1;
+    LAUNCH(k_root, dim3(grid_for(N, 1, kGridCap)), dim3(64), dmd, dms, blk, smat, dbl, D, E, DLr,
+           DLLr, W, Werr, Wgap, nbl_p);
+    LAUNCH(k_refine_root, dim3(cdiv(N, 256)), dim3(256), dmd, dms, blk, smat,
+                       iblock, Windex, D, DLLr, W, Werr, Wgap, Wtmp, Werrtmp, N);
+    LAUNCH(k_gaps_root, dim3(cdiv(N, 256)), dim3(256), dmd, dms, blk, smat,
+                       iblock, Wtmp, Werrtmp, W, Werr, Wgap, N);
+    if (any_n)
+        LAUNCH(k_shift_n, dim3(cdiv(N, 256)), dim3(256), dmd, dms, blk, smat,
+                           iblock, W, N);
     MR3_CHECK(hipGetLastError());
-    ms.resize(nmat);
-    MR3_CHECK(hipMemcpyAsync(ms.data(), dms, sizeof(MatState) * nmat, hipMemcpyDeviceToHost, st));
-    MR3_CHECK(hipStreamSynchronize(st));
-    // block list for the root kernel: (matrix offset + block index) of every block
-    std::vector<int> blist;
-    int maxbl = 1;
-    for (int k : order) {
-        if (ms[k].done || ms[k].status != kOk) continue;
-        for (int j = 0; j < ms[k].nsplit; ++j) blist.push_back(md[k].off + j);
-        maxbl = std::max(maxbl, ms[k].maxbl);
-    }
-    const int nbl = (int)blist.size();
-    int* dbl = nullptr;
-    if (nbl > 0) {
-        dbl = A.get<int>(nbl);
-        if (!A.ok) return 2;
-        MR3_CHECK(hipMemcpyAsync(dbl, blist.data(), sizeof(int) * nbl, hipMemcpyHostToDevice, st));
-        LAUNCH(k_root, dim3(nbl), dim3(64), dmd, dms, blk, smat, dbl, D, E, DLr,
-                           DLLr, W, Werr, Wgap, nbl);
-        LAUNCH(k_refine_root, dim3(cdiv(N, 256)), dim3(256), dmd, dms, blk, smat,
-                           iblock, Windex, D, DLLr, W, Werr, Wgap, Wtmp, Werrtmp, N);
-        LAUNCH(k_gaps_root, dim3(cdiv(N, 256)), dim3(256), dmd, dms, blk, smat,
-                           iblock, Wtmp, Werrtmp, W, Werr, Wgap, N);
-        if (any_n)
-            LAUNCH(k_shift_n, dim3(cdiv(N, 256)), dim3(256), dmd, dms, blk, smat,
-                               iblock, W, N);
-        MR3_CHECK(hipGetLastError());
-    }
-    if (any_v && nbl > 0) {
+    int levels_done = 0;
+    size_t pool_used = 0;   // slots of the levels (eager walk)
+    int* pool_flag = nullptr;
+    unsigned long long* bump = nullptr;
+    // pinned copies of each level's counters (the host reads them without waiting)
+    LevelCounters* hpin = nullptr;
+    if (any_v) {
         LAUNCH(k_zindex, dim3(nmat), dim3(256), dmd, dms, blk, W, iblock, Windex,
                            Zidx);
         // task lists: singletons (<= N), clusters (<= N/2 + 1), eigen entries (<= N)
         const int capS = N, capC = N / 2 + 1, capE = N;
+        // wave tasks of a level: ceil(count / 64) per block, at most N blocks
+        const int capT = N / 64 + N + 1;
         // singleton lists of four consecutive levels: the side streams may still work on
         // levels L - 2 .. L while the main stream classifies level L + 1
         constexpr int kLB = 4;
@@ -458,9 +459,15 @@ int run_pipeline(std::vector<MatDesc>& md, std::vector<MatState>& ms, hipStream_
         ClTask* Cl[2] = {A.get<ClTask>(capC), A.get<ClTask>(capC)};
         int* es[2] = {A.get<int>(capE), A.get<int>(capE)};
         int* ec[2] = {A.get<int>(capE), A.get<int>(capE)};
-        LevelCounters* dcnt = A.get<LevelCounters>(1);
-        unsigned long long* bump = A.get<unsigned long long>(1);
-        const int capT = N / 64 + nbl + 1;
+        // counters of the level being worked on and of the next one (each level's
+        // classification writes the next level's)
+        LevelCounters* lc = A.get<LevelCounters>(2);
+        bump = A.get<unsigned long long>(1);
+        pool_flag = A.get<int>(1);
+        // fixed levels: one pool for the slots of all levels; otherwise each level gets
+        // slots of the size its counts ask for
+        double* pool = fixed ? A.get_swept(pool_cap) : nullptr;
+        double* spare = A.get_swept(4 * (size_t)nmax);
         // singleton iterations, one set per side stream: the first iteration's tasks,
         // per-singleton state, two generations of compacted lists (S entries per
         // block) with their tasks and counters, the bisection list
@@ -493,63 +500,63 @@ int run_pipeline(std::vector<MatDesc>& md, std::vector<MatState>& ms, hipStream_
             ss[k].DI = A.get<int>(N);
             ss[k].DIb = A.get<int>(N);
         }
-        if (!A.ok) return 2;
-        for (int k = 0; k < kLB; ++k)
-            MR3_CHECK(hipMemsetAsync(bcnt2[k], 0, sizeof(int) * (size_t)N, st));
-        for (int k = 0; k < nside; ++k)
-            MR3_CHECK(hipMemsetAsync(ss[k].sbc, 0, sizeof(int) * (size_t)N, st));
-        // eigenvector scratch: two checkpoint arrays of maxbl/kChunk + 2 carries per
-        // lane, 64 lanes per wave
+        if (!A.ok) MR3_FAIL();
+        // eigenvector scratch: two checkpoint arrays of nmax/kChunk + 2 carries per
+        // lane, 64 lanes per wave (nmax bounds every block)
+        const int maxbl = nmax;
         const size_t ck_per_lane = (size_t)2 * (maxbl / kChunk + 2);
         const size_t per_wave = ck_per_lane * 64 * sizeof(double);
         const size_t budget = singleton_scratch_budget() / nside;
         int max_waves = (int)std::max<size_t>(1, std::min<size_t>(budget / per_wave,
                                                                   (size_t)tune.singleton_waves));
-        const int want_waves = capT;
-        const int nwaves = std::min(max_waves, want_waves);
+        const int nwaves = std::min(max_waves, capT);
         for (int k = 0; k < nside; ++k) ss[k].scratch = A.get<double>((size_t)nwaves * ck_per_lane * 64);
         // the large-block tail kernel: S and P of up to kTailBigWG singletons at a time
         const int tstride = 2 * (maxbl + 2);
         const int tail_rows = (maxbl > kTailN) ? maxbl : kTailN;   // blocks the tail kernels take
         for (int k = 0; k < nside; ++k)
             ss[k].tscratch = (maxbl > kTailN) ? A.get<double>((size_t)kTailBigWG * tstride) : nullptr;
-        if (!A.ok) return 2;
-        LevelCounters hc;
-        auto reset = [&]() { return hipMemsetAsync(dcnt, 0, sizeof(LevelCounters), st); };
-        MR3_CHECK(reset());
+        hpin = pinned_levels_get();
+        if (!A.ok || !hpin) {
+            if (hpin) pinned_levels_put(hpin);
+            MR3_FAIL();
+        }
+        struct PinFree {
+            LevelCounters* p;
+            ~PinFree() { pinned_levels_put(p); }
+        } pin_free{hpin};
+        // cleared with kernels, not memsets: a captured graph replays them
+        for (int k = 0; k < kLB; ++k)
+            LAUNCH(k_fill0, dim3(grid_for(N, 256, 1024)), dim3(256), bcnt2[k], (long long)N);
+        for (int k = 0; k < nside; ++k)
+            LAUNCH(k_fill0, dim3(grid_for(N, 256, 1024)), dim3(256), ss[k].sbc, (long long)N);
+        LAUNCH(k_zero, dim3(1), dim3(64), (int*)bump, 2, (int*)lc,
+               (int)(2 * sizeof(LevelCounters) / sizeof(int)));
+        LAUNCH(k_zero, dim3(1), dim3(64), pool_flag, 1, (int*)nullptr, 0);
         TreeOut out;
-        out.S = Sl2[0]; out.C = Cl[0]; out.eig_slot = es[0]; out.eig_cl = ec[0]; out.cnt = dcnt;
+        out.S = Sl2[0]; out.C = Cl[0]; out.eig_slot = es[0]; out.eig_cl = ec[0]; out.cnt = lc;
         out.bcnt = bcnt2[0];
+        out.halt = pool_flag;
         out.capS = capS; out.capC = capC; out.capEig = capE;
-        LAUNCH(k_classify_root, dim3(nbl), dim3(64), dmd, dms, blk,
-                           smat, dbl, gers, D, E, DLr, DLLr, W, Werr, Wgap, Wsh, out, nbl);
+        LAUNCH(k_classify_root, dim3(grid_for(N, 1, kGridCap)), dim3(64), dmd, dms, blk,
+                           smat, dbl, gers, D, E, DLr, DLLr, W, Werr, Wgap, Wsh, out, nbl_p);
         MR3_CHECK(hipGetLastError());
-        MR3_CHECK(hipMemcpyAsync(&hc, dcnt, sizeof(hc), hipMemcpyDeviceToHost, st));
-        MR3_CHECK(hipStreamSynchronize(st));
-        if (hc.err) return 2;
-        int cur = 0;
-        const bool trace = getenv("MR3_TRACE_LEVELS") != nullptr;   // per-level timing to stderr
-        unsigned long long prev_rrf = 0, prev_rrfn = 0;
         hipStream_t s2p[kSide] = {};
         for (int k = 0; k < nside; ++k) {
             s2p[k] = side_stream(k);
-            if (!s2p[k]) return 2;
+            if (!s2p[k]) MR3_FAIL();
         }
-        // end of the singleton phase that last read list buffer k
+        // events: end of the singleton phase that last read list buffer k; the lists of
+        // level L ready on the main stream; level L's counts copied to pinned memory
+        struct EvSet {
+            std::vector<hipEvent_t> e;
+            ~EvSet() { for (hipEvent_t x : e) event_pool().put(x); }
+            hipEvent_t get() { e.push_back(event_pool().get()); return e.back(); }
+        } evs;
         hipEvent_t evS[kLB] = {};
-        for (int k = 0; k < kLB; ++k) MR3_CHECK(hipEventCreateWithFlags(&evS[k], hipEventDisableTiming));
-        struct EvFree {
-            hipEvent_t* e;
-            ~EvFree() { for (int k = 0; k < kLB; ++k) (void)hipEventDestroy(e[k]); }
-        } ev_free{evS};
-        // the next level's counts arrive in pinned memory, so that queueing their copy
-        // does not wait for the main stream
-        LevelCounters* hpin = pinned_counters_get();
-        if (!hpin) return 2;
-        struct PinFree {
-            LevelCounters* p;
-            ~PinFree() { pinned_counters_put(p); }
-        } pin_free{hpin};
+        bool have_evS[kLB] = {};
+        for (int k = 0; k < kLB; ++k) evS[k] = evs.get();
+        hipEvent_t evReady[kMaxLevels + 1] = {}, evCnt[kMaxLevels + 1] = {};
         struct SideSync {   // no buffer or event goes away while a side stream may use it
             const hipStream_t* s;
             ~SideSync() {
@@ -557,186 +564,216 @@ int run_pipeline(std::vector<MatDesc>& md, std::vector<MatState>& ms, hipStream_
                     if (s[k]) (void)hipStreamSynchronize(s[k]);
             }
         } side_sync{s2p};
-        bool have_evS[kLB] = {};
-        // the side streams start after everything so far on the main stream
-        hipEvent_t evStart = nullptr;
-        MR3_CHECK(hipEventCreateWithFlags(&evStart, hipEventDisableTiming));
-        MR3_CHECK(hipEventRecord(evStart, st));
-        for (int k = 0; k < nside; ++k) MR3_CHECK(hipStreamWaitEvent(s2p[k], evStart, 0));
-        (void)hipEventDestroy(evStart);
-        for (int level = 0;; ++level) {
-            const int nS = hc.nS, nC = hc.nC, nE = hc.nEig;
-            const long long arena_need = hc.arena;
-            const int p = level % nside;       // side stream and its buffer set
-            const int lb = level % kLB;        // this level's list buffer
-            const int lbn = (level + 1) % kLB; // the next level's
-            // 1. cluster work of this level and the next level's classification, on the
-            // main stream
-            if (nC > 0) {
-                if (level > 64) return 2;
-                double* arena = A.get_swept((size_t)arena_need);
-                if (!A.ok) return 2;
-                // the arena cursor and the level counters start at zero
-                LAUNCH(k_zero, dim3(1), dim3(64), (int*)bump, 2, (int*)dcnt,
-                       (int)(sizeof(LevelCounters) / sizeof(int)));
-                LAUNCH(k_arena_assign, dim3(cdiv(nC, 64)), dim3(64), Cl[cur], nC, arena, bump);
-                const int Ge = group_lanes(2LL * nC, tune.group_lanes);
-                if (Ge == 1)
-                    LAUNCH(k_endref, dim3(cdiv(2LL * nC, 64)), dim3(64), Cl[cur], nC, Windex, Wsh,
-                           Werr, dms, Wtmp, Werrtmp);
-                else
-                    LAUNCH(k_endref_grp, dim3(cdiv(2LL * nC, 64 / Ge)), dim3(64), Cl[cur], nC, Ge,
-                           Windex, Wsh, Werr, dms, Wtmp, Werrtmp);
-                // four lanes per cluster unless the clusters alone fill the device
-                if (nC > tune.rrf_group_max) {
-                    LAUNCH(k_dlarrf<1>, dim3(cdiv(nC, 64)), dim3(64), Cl[cur], nC, Wsh, Werr, Wgap,
-                           Wtmp, Werrtmp, dms);
-                } else {
-                    LAUNCH(k_dlarrf<4>, dim3(cdiv(nC, 16)), dim3(64), Cl[cur], nC, Wsh, Werr, Wgap,
-                           Wtmp, Werrtmp, dms);
-                    LAUNCH(k_rrf_store, dim3(cdiv(nC, 64)), dim3(64), Cl[cur], nC, dms);
-                }
-                LAUNCH(k_newrrr_fin, dim3(nC), dim3(64), Cl[cur], nC, Wsh, Werr);
-                const int Gr = group_lanes(nE, tune.group_lanes);
-                if (Gr == 1)
-                    LAUNCH(k_refine_cl, dim3(cdiv(nE, 64)), dim3(64), Cl[cur], es[cur], ec[cur], nE,
-                           Windex, Wsh, Werr, Wgap, dms, Wtmp, Werrtmp);
-                else
-                    LAUNCH(k_refine_cl_grp, dim3(cdiv(nE, 64 / Gr)), dim3(64), Cl[cur], es[cur],
-                           ec[cur], nE, Gr, Windex, Wsh, Werr, Wgap, dms, Wtmp, Werrtmp);
-                LAUNCH(k_gaps_cl, dim3(cdiv(nE, 64)), dim3(64), Cl[cur], es[cur], ec[cur], nE,
-                       Wtmp, Werrtmp, Wsh, Werr, Wgap, W);
-                // the next level's singleton list reuses the buffers of the phase four
-                // levels back, which a side stream may still be reading
-                if (have_evS[lbn]) MR3_CHECK(hipStreamWaitEvent(st, evS[lbn], 0));
-                const int nxt = cur ^ 1;
-                out.C = Cl[nxt]; out.eig_slot = es[nxt]; out.eig_cl = ec[nxt];
-                out.S = Sl2[lbn];
-                out.bcnt = bcnt2[lbn];
-                LAUNCH(k_classify, dim3(cdiv(nC, 64)), dim3(64), Cl[cur], nC, Wgap, Wsh, out);
-                MR3_CHECK(hipGetLastError());
-                MR3_CHECK(hipMemcpyAsync(hpin, dcnt, sizeof(LevelCounters), hipMemcpyDeviceToHost, st));
-                cur = nxt;
+        // level L's counts, copied to pinned memory once its lists are complete
+        auto publish = [&](int L, const LevelCounters* src) -> int {
+            evReady[L] = evs.get();
+            MR3_CHECK(hipEventRecord(evReady[L], st));
+            MR3_CHECK(hipMemcpyAsync(&hpin[L], src, sizeof(LevelCounters), hipMemcpyDeviceToHost, st));
+            evCnt[L] = evs.get();
+            MR3_CHECK(hipEventRecord(evCnt[L], st));
+            return 0;
+        };
+        // the number of blocks, ahead of level 0's counts (it sizes the planning launches)
+        LevelCounters& hblocks = hpin[kMaxLevelsPinned - 1];
+        MR3_CHECK(hipMemcpyAsync(&hblocks.nS, nbl_p, sizeof(int), hipMemcpyDeviceToHost, st));
+        if (publish(0, lc)) MR3_FAIL();
+        // what the host knows: the counts of levels 0 .. known - 1
+        int known = 0;
+        auto learn = [&](int upto, bool wait) -> int {   // reads the counts that arrived
+            while (known <= upto) {
+                if (wait) MR3_CHECK(hipEventSynchronize(evCnt[known]));
+                else if (hipEventQuery(evCnt[known]) != hipSuccess) break;
+                if (hpin[known].err) MR3_FAIL();
+                ++known;
             }
-            // 2. the singletons of this level on side stream p, concurrently with the
-            // main stream and with the other side stream's previous level; every launch
-            // sizes itself on the device, so the whole phase is queued without waiting
-            if (nS > 0) {
-                hipStream_t s2 = s2p[p];
-                SSet& q = ss[p];
-                STask* Sl = Sl2[lb];
-                int* bcnt = bcnt2[lb];
-                // (the level's lists, representations and eigenvalues were produced on
-                // the main stream before the host read this level's counts, which waited
-                // for it; the main stream's later work touches other slots)
-                hipEvent_t t0 = nullptr, t1 = nullptr;
-                if (trace) {
-                    (void)hipEventCreate(&t0);
-                    (void)hipEventCreate(&t1);
-                    (void)hipEventRecord(t0, s2);
-                }
-                // first iteration over the level's singleton lists (64 per task,
-                // longest blocks first)
-                LAUNCH_ON(s2, k_zero, dim3(1), dim3(64), (int*)q.scnt,
-                          (int)(sizeof(LevelCounters) / sizeof(int)), (int*)nullptr, 0);
-                LAUNCH_ON(s2, k_plan_count, dim3(cdiv(nbl, 64)), dim3(64), dbl, nbl, blk, bcnt, 64,
-                          q.scnt, q.cit[1]);
-                LAUNCH_ON(s2, k_plan_s, dim3(cdiv(nbl, 64)), dim3(64), dbl, nbl, smat, dmd, blk, bcnt,
-                          64, q.wtasks, capT, q.scnt);
-                const int g1 = nS / 64 + nbl + 1;   // bound on the tasks of any iteration
-                const int gw = std::min(nwaves, g1);
-                LAUNCH_ON(s2, k_s_iter, dim3(gw), dim3(64), q.wtasks, q.scnt, 1, (const int*)nullptr,
-                          Sl, dmd, dms, Windex, Zidx, W, Wsh, Werr, Wgap, q.ST, q.scratch, maxbl,
+            return 0;
+        };
+        // item bound of level L: the eigen entries of the latest level seen at or before it
+        auto bound = [&](int L) -> long long {
+            for (int j = std::min(L, known - 1); j >= 0; --j) return std::max(1, hpin[j].nEig + hpin[j].nS);
+            return N;
+        };
+        const long long gbud = tune.group_lanes;
+        const int rrf_max = tune.rrf_group_max;
+        // the cluster work of level L and the classification into level L + 1 (main stream)
+        auto main_level = [&](int L, double* arena, size_t arena_cap) -> int {
+            const int cur = L & 1, nxt = cur ^ 1;
+            const long long b = fixed ? N : bound(L);
+            const long long bc = std::min<long long>(capC, b / 2 + 1);
+            LevelCounters* lcur = lc + cur;
+            LevelCounters* lnxt = lc + nxt;
+            LAUNCH(k_zero, dim3(1), dim3(64), (int*)lnxt,
+                   (int)(sizeof(LevelCounters) / sizeof(int)), (int*)nullptr, 0);
+            if (!fixed)   // a slot of the level's own: the bump counter starts again
+                LAUNCH(k_zero, dim3(1), dim3(64), (int*)bump, 2, (int*)nullptr, 0);
+            LAUNCH(k_arena_assign, dim3(grid_for(bc, 64, kGridCap)), dim3(64), Cl[cur], &lcur->nC, arena,
+                   bump, (unsigned long long)arena_cap, spare, pool_flag);
+            LAUNCH(k_endref, dim3(grid_for(2 * bc, 64, kGridCap)), dim3(64), Cl[cur], &lcur->nC, Windex,
+                   Wsh, Werr, dms, Wtmp, Werrtmp, gbud);
+            LAUNCH(k_endref_grp, dim3(grid_for(std::min<long long>(gbud, 64 * 2 * bc), 64, kGridCap)),
+                   dim3(64), Cl[cur], &lcur->nC, gbud, Windex, Wsh, Werr, dms, Wtmp, Werrtmp);
+            // four lanes per cluster unless the clusters alone fill the device
+            LAUNCH(k_dlarrf<1>, dim3(grid_for(bc, 64, kGridCap)), dim3(64), Cl[cur], &lcur->nC,
+                   rrf_max, Wsh, Werr, Wgap, Wtmp, Werrtmp, dms);
+            LAUNCH(k_dlarrf<4>, dim3(grid_for(std::min<long long>(bc, rrf_max), 16, kGridCap)), dim3(64),
+                   Cl[cur], &lcur->nC, rrf_max, Wsh, Werr, Wgap, Wtmp, Werrtmp, dms);
+            LAUNCH(k_rrf_store, dim3(grid_for(std::min<long long>(bc, rrf_max), 64, kGridCap)), dim3(64),
+                   Cl[cur], &lcur->nC, rrf_max, dms);
+            LAUNCH(k_newrrr_fin, dim3(grid_for(bc, 1, kGridCap)), dim3(64), Cl[cur], &lcur->nC, Wsh,
+                   Werr);
+            LAUNCH(k_refine_cl, dim3(grid_for(b, 64, kGridCap)), dim3(64), Cl[cur], es[cur], ec[cur],
+                   &lcur->nEig, Windex, Wsh, Werr, Wgap, dms, Wtmp, Werrtmp, gbud);
+            LAUNCH(k_refine_cl_grp, dim3(grid_for(std::min<long long>(gbud, 64 * b), 64, kGridCap)),
+                   dim3(64), Cl[cur], es[cur], ec[cur], &lcur->nEig, gbud, Windex, Wsh, Werr, Wgap,
+                   dms, Wtmp, Werrtmp);
+            LAUNCH(k_gaps_cl, dim3(grid_for(b, 64, kGridCap)), dim3(64), Cl[cur], es[cur], ec[cur],
+                   &lcur->nEig, Wtmp, Werrtmp, Wsh, Werr, Wgap, W);
+            // the next level's singleton list reuses the buffers of the phase four
+            // levels back, which a side stream may still be reading
+            const int lbn = (L + 1) % kLB;
+            if (have_evS[lbn]) MR3_CHECK(hipStreamWaitEvent(st, evS[lbn], 0));
+            TreeOut o = out;
+            o.C = Cl[nxt]; o.eig_slot = es[nxt]; o.eig_cl = ec[nxt];
+            o.S = Sl2[lbn];
+            o.bcnt = bcnt2[lbn];
+            o.cnt = lnxt;
+            LAUNCH(k_classify, dim3(grid_for(bc, 64, kGridCap)), dim3(64), Cl[cur], &lcur->nC, Wgap,
+                   Wsh, o);
+            MR3_CHECK(hipGetLastError());
+            return publish(L + 1, lnxt);
+        };
+        // the singletons of level L on a side stream, concurrently with the main stream
+        // and with the other side streams' earlier levels; every launch sizes itself on
+        // the device
+        auto side_level = [&](int L) -> int {
+            const int p = L % nside;
+            hipStream_t s2 = s2p[p];
+            SSet& q = ss[p];
+            const int lb = L % kLB;
+            STask* Sl = Sl2[lb];
+            int* bcnt = bcnt2[lb];
+            // the level's lists, representations and eigenvalues are complete on the main
+            // stream when its counts are published; the main stream's later work touches
+            // other slots
+            MR3_CHECK(hipStreamWaitEvent(s2, evReady[L], 0));
+            // grids: from the level's counts when the host has them, else from bounds
+            const bool kn = !fixed && known > L;
+            const long long nS = kn ? hpin[L].nS : N;
+            const long long nb = kn ? hblocks.nS : N;
+            const int g1 = (int)std::min<long long>(capT, nS / 64 + nb + 1);   // wave tasks
+            const unsigned gT = grid_for(g1, 1, kGridCap);
+            const unsigned gw = (unsigned)std::min(nwaves, g1);
+            const unsigned gB = grid_for(nb, 64, 1024);
+            const unsigned gTail = (unsigned)std::min<long long>(nS, 1024);
+            const unsigned gTailB = (unsigned)std::min<long long>(nS, kTailBigWG);
+            // first iteration over the level's singleton lists (64 per task,
+            // longest blocks first)
+            LAUNCH_ON(s2, k_zero, dim3(1), dim3(64), (int*)q.scnt,
+                      (int)(sizeof(LevelCounters) / sizeof(int)), (int*)nullptr, 0);
+            LAUNCH_ON(s2, k_plan_count, dim3(gB), dim3(64), dbl, nbl_p, blk, bcnt, 64,
+                      q.scnt, q.cit[1]);
+            LAUNCH_ON(s2, k_plan_s, dim3(gB), dim3(64), dbl, nbl_p, smat, dmd, blk, bcnt,
+                      64, q.wtasks, capT, q.scnt);
+            LAUNCH_ON(s2, k_s_iter, dim3(gw), dim3(64), q.wtasks, q.scnt, 1, (const int*)nullptr,
+                      Sl, dmd, dms, Windex, Zidx, W, Wsh, Werr, Wgap, q.ST, q.scratch, maxbl,
+                      tune.tail_singletons, tail_rows);
+            // further iterations on the compacted singletons that go on (the
+            // reference's loop runs at most kMaxIterRQI dlar1v calls plus the
+            // best-residual step)
+            const WaveTask* ptasks = q.wtasks;
+            const LevelCounters* pcnt = q.scnt;
+            const int* psi = nullptr;
+            int pfirst = 1;
+            for (int it = 1; it <= kMaxIterRQI + 1; ++it) {
+                const int b = it & 1;
+                // q.cit[b] was cleared by the previous iteration's k_plan_count
+                LAUNCH_ON(s2, k_s_collect, dim3(gT), dim3(64), ptasks, pcnt, pfirst, psi, Sl,
+                          q.ST, q.sit[b], q.sbc, q.BI, q.DI, q.DIb, q.cit[b]);
+                LAUNCH_ON(s2, k_plan_count, dim3(gB), dim3(64), dbl, nbl_p, blk, q.sbc,
+                          64, q.cit[b], it < kMaxIterRQI + 1 ? q.cit[b ^ 1] : (LevelCounters*)nullptr);
+                LAUNCH_ON(s2, k_plan_s, dim3(gB), dim3(64), dbl, nbl_p, smat, dmd, blk,
+                          q.sbc, 64, q.tit[b], capT, q.cit[b]);
+                LAUNCH_ON(s2, k_s_bisect, dim3(std::min(gw, 2048u)), dim3(64), q.BI, q.cit[b], Sl,
+                          dms, Windex, q.ST);
+                LAUNCH_ON(s2, k_s_iter, dim3(gw), dim3(64), q.tit[b], q.cit[b], 0, q.sit[b], Sl,
+                          dmd, dms, Windex, Zidx, W, Wsh, Werr, Wgap, q.ST, q.scratch, maxbl,
                           tune.tail_singletons, tail_rows);
-                // further iterations on the compacted singletons that go on (the
-                // reference's loop runs at most kMaxIterRQI dlar1v calls plus the
-                // best-residual step)
-                const WaveTask* ptasks = q.wtasks;
-                const LevelCounters* pcnt = q.scnt;
-                const int* psi = nullptr;
-                int pfirst = 1;
-                for (int it = 1; it <= kMaxIterRQI + 1; ++it) {
-                    const int b = it & 1;
-                    // q.cit[b] was cleared by the previous iteration's k_plan_count
-                    LAUNCH_ON(s2, k_s_collect, dim3(g1), dim3(64), ptasks, pcnt, pfirst, psi, Sl,
-                              q.ST, q.sit[b], q.sbc, q.BI, q.DI, q.DIb, q.cit[b]);
-                    LAUNCH_ON(s2, k_plan_count, dim3(cdiv(nbl, 64)), dim3(64), dbl, nbl, blk, q.sbc,
-                              64, q.cit[b], it < kMaxIterRQI + 1 ? q.cit[b ^ 1] : (LevelCounters*)nullptr);
-                    LAUNCH_ON(s2, k_plan_s, dim3(cdiv(nbl, 64)), dim3(64), dbl, nbl, smat, dmd, blk,
-                              q.sbc, 64, q.tit[b], capT, q.cit[b]);
-                    LAUNCH_ON(s2, k_s_bisect, dim3(std::min(gw, 2048)), dim3(64), q.BI, q.cit[b], Sl,
-                              dms, Windex, q.ST);
-                    LAUNCH_ON(s2, k_s_iter, dim3(gw), dim3(64), q.tit[b], q.cit[b], 0, q.sit[b], Sl,
-                              dmd, dms, Windex, Zidx, W, Wsh, Werr, Wgap, q.ST, q.scratch, maxbl,
-                              tune.tail_singletons, tail_rows);
-                    LAUNCH_ON(s2, k_s_tail<false>, dim3(std::min(nS, 1024)), dim3(128), q.DI, q.cit[b],
-                              Sl, dmd, dms, Windex, Zidx, W, Wsh, Werr, Wgap, q.ST,
-                              kMaxIterRQI + 2 - it, tune.tail_singletons, (double*)nullptr, 0);
-                    if (maxbl > kTailN)
-                        LAUNCH_ON(s2, k_s_tail<true>, dim3(std::min(nS, kTailBigWG)), dim3(128), q.DIb,
-                                  q.cit[b], Sl, dmd, dms, Windex, Zidx, W, Wsh, Werr, Wgap, q.ST,
-                                  kMaxIterRQI + 2 - it, tune.tail_singletons, q.tscratch, tstride);
-                    ptasks = q.tit[b];
-                    pcnt = q.cit[b];
-                    psi = q.sit[b];
-                    pfirst = 0;
-                    if (trace) {   // the iteration's counts, waiting for it
-                        LevelCounters ic;
-                        (void)hipMemcpyAsync(&ic, q.cit[b], sizeof(ic), hipMemcpyDeviceToHost, s2);
-                        (void)hipStreamSynchronize(s2);
-                        float ms = 0.f;
-                        (void)hipEventRecord(t1, s2);
-                        (void)hipEventSynchronize(t1);
-                        (void)hipEventElapsedTime(&ms, t0, t1);
-                        fprintf(stderr, "mr3 level %d iteration %d: going on %d, bisected %d, "
-                                "lists %d, %.3f ms\n", level, it, ic.nS, ic.nEig, ic.nT, ms);
-                    }
-                }
-                LAUNCH_ON(s2, k_s_normalize, dim3(g1), dim3(64), q.wtasks, q.scnt, Sl, dmd, Zidx, q.ST);
-                MR3_CHECK(hipGetLastError());
-                MR3_CHECK(hipEventRecord(evS[lb], s2));
-                have_evS[lb] = true;
-                if (trace) {
-                    float ms = 0.f;
-                    (void)hipEventRecord(t1, s2);
-                    (void)hipEventSynchronize(t1);
-                    (void)hipEventElapsedTime(&ms, t0, t1);
-                    unsigned long long stv[kStepKinds] = {};
-                    (void)hipMemcpyFromSymbol(stv, HIP_SYMBOL(g_steps), sizeof(stv));
-                    fprintf(stderr,
-                            "mr3 level %d: singletons %d clusters %d  %.3f ms  "
-                            "dlarrf sweeps per cluster %.2f (with the work counters on)\n",
-                            level, nS, nC, ms,
-                            stv[kStepRrfN] > prev_rrfn
-                                ? (double)(stv[kStepRrf] - prev_rrf) / (double)(stv[kStepRrfN] - prev_rrfn)
-                                : 0.0);
-                    prev_rrf = stv[kStepRrf];
-                    prev_rrfn = stv[kStepRrfN];
-                    (void)hipEventDestroy(t0);
-                    (void)hipEventDestroy(t1);
-                }
+                LAUNCH_ON(s2, k_s_tail<false>, dim3(gTail), dim3(128), q.DI, q.cit[b],
+                          Sl, dmd, dms, Windex, Zidx, W, Wsh, Werr, Wgap, q.ST,
+                          kMaxIterRQI + 2 - it, tune.tail_singletons, (double*)nullptr, 0);
+                if (maxbl > kTailN)
+                    LAUNCH_ON(s2, k_s_tail<true>, dim3(gTailB), dim3(128), q.DIb,
+                              q.cit[b], Sl, dmd, dms, Windex, Zidx, W, Wsh, Werr, Wgap, q.ST,
+                              kMaxIterRQI + 2 - it, tune.tail_singletons, q.tscratch, tstride);
+                ptasks = q.tit[b];
+                pcnt = q.cit[b];
+                psi = q.sit[b];
+                pfirst = 0;
             }
-            if (nC == 0) break;
-            // 3. the next level's counts
-            MR3_CHECK(hipStreamSynchronize(st));
-            hc = *hpin;
-            if (hc.err) return 2;
+            LAUNCH_ON(s2, k_s_normalize, dim3(gT), dim3(64), q.wtasks, q.scnt, Sl, dmd, Zidx, q.ST);
+            MR3_CHECK(hipGetLastError());
+            MR3_CHECK(hipEventRecord(evS[lb], s2));
+            have_evS[lb] = true;
+            return 0;
+        };
+        int mainL = 0;
+        if (fixed) {
+            for (int L = 0; L < std::min(fixed_levels, kMaxLevels); ++L) {
+                if (main_level(L, pool, pool_cap)) MR3_FAIL();
+                if (side_level(L)) MR3_FAIL();
+            }
+            mainL = std::min(fixed_levels, kMaxLevels);
+        } else {
+            // level L's counts arrive (an event, not a stream wait: the side streams and
+            // the copy engines go on) before its work is enqueued: they size the slots
+            // of the level's child representations
+            for (int L = 0;; ++L) {
+                if (learn(L, true)) MR3_FAIL();
+                if (hpin[L].nS > 0 && side_level(L)) MR3_FAIL();
+                if (hpin[L].nC == 0) break;
+                if (L >= kMaxLevels) MR3_FAIL();   // deeper than any tree the solver allows
+                const size_t need = (size_t)hpin[L].arena;
+                double* arena = A.get_swept(need);
+                if (!A.ok) MR3_FAIL();
+                pool_used += need;
+                if (main_level(L, arena, need)) MR3_FAIL();
+                mainL = L + 1;
+            }
         }
         for (int k = 0; k < kLB; ++k)
             if (have_evS[k]) MR3_CHECK(hipStreamWaitEvent(st, evS[k], 0));
+        levels_done = mainL;
     }
-    if (nbl > 0) {
-        LAUNCH(k_rrj, dim3(cdiv(N, 256)), dim3(256), dmd, dms, blk, smat, iblock,
-                           Windex, Dcopy, E2copy, W, Werr, N);
-        // (tcount and isplit are free by now: the reordering's permutation and marks)
-        LAUNCH(k_finalize, dim3(nmat), dim3(256), dmd, dms, W, Zidx, blk, iblock,
-                           Windex, tcount, isplit, Wcoarse);
-        MR3_CHECK(hipGetLastError());
-    }
+    LAUNCH(k_rrj, dim3(cdiv(N, 256)), dim3(256), dmd, dms, blk, smat, iblock,
+                       Windex, Dcopy, E2copy, W, Werr, N);
+    // (tcount and isplit are free by now: the reordering's permutation and marks)
+    LAUNCH(k_finalize, dim3(nmat), dim3(256), dmd, dms, W, Zidx, blk, iblock,
+                       Windex, tcount, isplit, Wcoarse);
+    MR3_CHECK(hipGetLastError());
+    ms.resize(nmat);
     MR3_CHECK(hipMemcpyAsync(ms.data(), dms, sizeof(MatState) * nmat, hipMemcpyDeviceToHost, st));
+    int hflag = 0;
+    unsigned long long hbump = 0;
+    if (pool_flag) {
+        MR3_CHECK(hipMemcpyAsync(&hflag, pool_flag, sizeof(int), hipMemcpyDeviceToHost, st));
+        MR3_CHECK(hipMemcpyAsync(&hbump, bump, sizeof(hbump), hipMemcpyDeviceToHost, st));
+    }
     MR3_CHECK(hipStreamSynchronize(st));
     ce.collect();
+    if (levels_used) *levels_used = levels_done;
+    if (pool_need) *pool_need = fixed ? (size_t)hbump : pool_used;
+    if (hflag) return 3;
+    if (hpin)
+        for (int j = 0; j <= levels_done && j <= kMaxLevels; ++j)
+            if (hpin[j].err) MR3_FAIL();
     return 0;
+}
+
+// Runs the whole pipeline for the problems described by md (host copy); the device
+// outputs land in the caller's arrays named in md; states are copied back to ms.
+int run_pipeline(std::vector<MatDesc>& md, std::vector<MatState>& ms, hipStream_t st) {
+    const int rc = pipeline_once(md, ms, st, 0, nullptr, -1, nullptr);
+    return rc == 3 ? 2 : rc;   // (the eager walk sizes every level's slots: no overflow)
 }
 
 // ---------------------------------------------------------------------------------

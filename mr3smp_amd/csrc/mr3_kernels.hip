@@ -57,6 +57,19 @@ __device__ __forceinline__ void add_steps(int kind, unsigned long long v) {
     if (g_count_steps && v) atomicAdd(&g_steps[kind], v);
 }
 
+// Lanes per eigenvalue of the cluster refinements of a level with `items` eigenvalues:
+// 1 (the one-lane kernels) unless a group of 4 or more lanes per eigenvalue keeps the
+// launch within the lane budget; then the largest such power of two up to a wave.
+// Multisection with G lanes does (G - 1) / log2(G) times the sweeps of one-point
+// bisection in 1 / log2(G) of the sequential rounds.  The kernels of a level read
+// their item counts on the device and pick their variant with it.
+__host__ __device__ inline int group_lanes_hd(long long items, long long budget) {
+    if (items * 4 > budget) return 1;
+    int G = 4;
+    while (G < 64 && items * 2 * G <= budget) G *= 2;
+    return G;
+}
+
 
 // ---------------------------------------------------------------------------------
 // Workgroup helpers (256 threads)
@@ -617,7 +630,7 @@ __device__ __forceinline__ void coarse_bounds(const BlockRec& br, int nbl, doubl
 
 // Counts at the tree nodes 1 .. 2^depth - 1 of every block of at least 128 rows; one
 // workgroup per block, tcount[off + bbeg + t - 1] = count at node t.
-__global__ __launch_bounds__(256) void k_coarse_tree(const MatDesc* __restrict__ md,
+__device__ __forceinline__ void k_coarse_tree_one(int bx, const MatDesc* __restrict__ md,
                                                      const MatState* __restrict__ ms,
                                                      const double* __restrict__ D,
                                                      const double* __restrict__ E2,
@@ -625,7 +638,7 @@ __global__ __launch_bounds__(256) void k_coarse_tree(const MatDesc* __restrict__
                                                      const int* __restrict__ smat,
                                                      const int* __restrict__ blist, int nblist,
                                                      int* tcount) {
-    const int ib = blockIdx.x;
+    const int ib = bx;
     if (ib >= nblist) return;
     const int gb = blist[ib];
     const int k = smat[gb];
@@ -656,6 +669,21 @@ __global__ __launch_bounds__(256) void k_coarse_tree(const MatDesc* __restrict__
         ++sweeps;
     }
     add_steps(kStepCoarse, (unsigned long long)sweeps * nbl);
+}
+
+__global__ __launch_bounds__(256) void k_coarse_tree(const MatDesc* __restrict__ md,
+                                                     const MatState* __restrict__ ms,
+                                                     const double* __restrict__ D,
+                                                     const double* __restrict__ E2,
+                                                     const BlockRec* __restrict__ blk,
+                                                     const int* __restrict__ smat,
+                                                     const int* __restrict__ blist, const int* __restrict__ nblist_p,
+                                                     int* tcount) {
+    const int nblist = *nblist_p;
+    const int nblk_ = nblist;
+    for (int bx = blockIdx.x; bx < nblk_; bx += gridDim.x) {
+        k_coarse_tree_one(bx, md, ms, D, E2, blk, smat, blist, nblist, tcount);
+    }
 }
 
 __global__ __launch_bounds__(256) void k_coarse(const MatDesc* __restrict__ md,
@@ -710,6 +738,55 @@ __global__ __launch_bounds__(256) void k_coarse(const MatDesc* __restrict__ md,
     W[gidx] = w;
     Wcoarse[gidx] = w;
     Werr[gidx] = 0.5 * fabs(left - right);
+}
+
+// ---------------------------------------------------------------------------------
+// k_block_list: the (matrix offset + block index) of every block of the matrices that
+// take part in a phase, matrices in the order given (largest first), blocks in order;
+// *count receives the length.  One workgroup: each thread counts a run of matrices,
+// a workgroup scan places the runs.  which = 0: blocks for the shared top of the
+// coarse bisection (matrices not done with a block of 128 rows or more); 1: blocks of
+// the root representations (matrices not done whose status is ok).
+// ---------------------------------------------------------------------------------
+constexpr int kListNT = 1024;
+__global__ __launch_bounds__(kListNT) void k_block_list(const MatDesc* __restrict__ md,
+                                                        const MatState* __restrict__ ms,
+                                                        const int* __restrict__ order, int nmat,
+                                                        int which, int* list, int* count) {
+    __shared__ int part[kListNT];
+    const int t = threadIdx.x;
+    const int per = (nmat + kListNT - 1) / kListNT;
+    const int lo = min(nmat, t * per), hi = min(nmat, lo + per);
+    auto blocks = [&](int k) -> int {
+        const MatState& S = ms[k];
+        if (S.done) return 0;
+        if (which == 0) return S.maxbl >= 128 ? S.nsplit : 0;
+        return S.status == kOk ? S.nsplit : 0;
+    };
+    int mine = 0;
+    for (int j = lo; j < hi; ++j) mine += blocks(order[j]);
+    part[t] = mine;
+    __syncthreads();
+    for (int o = 1; o < kListNT; o <<= 1) {   // inclusive scan
+        const int v = (t >= o) ? part[t - o] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    int pos = part[t] - mine;
+    for (int j = lo; j < hi; ++j) {
+        const int k = order[j];
+        const int nb = blocks(k);
+        for (int b = 0; b < nb; ++b) list[pos++] = md[k].off + b;
+    }
+    if (t == kListNT - 1) *count = part[t];
+}
+
+// Fills n ints with zero, grid-stride (a kernel, not a memset, so that a captured
+// graph replays it like any other node).
+__global__ __launch_bounds__(256) void k_fill0(int* a, long long n) {
+    for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n; i += (long long)gridDim.x * 256)
+        a[i] = 0;
 }
 
 // ---------------------------------------------------------------------------------
@@ -791,13 +868,13 @@ __global__ void k_discard(const MatDesc* __restrict__ md, MatState* ms, BlockRec
 // (compute_root_rrr, mrrr_val.c:717-948) and the set-up of the refinement
 // (mrrr_val.c:1003-1011).  One 64-lane workgroup per block slot.
 // ---------------------------------------------------------------------------------
-__global__ __launch_bounds__(64) void k_root(const MatDesc* __restrict__ md, MatState* ms,
+__device__ __forceinline__ void k_root_one(int bx, const MatDesc* __restrict__ md, MatState* ms,
                                              BlockRec* blk, const int* __restrict__ smat,
                                              const int* __restrict__ blist, double* D, double* E,
                                              double* DLr, double* DLLr, double* W, double* Werr,
                                              double* Wgap, int nblist) {
-    if ((int)blockIdx.x >= nblist) return;
-    const int gb = blist[blockIdx.x];
+    if ((int)bx >= nblist) return;
+    const int gb = blist[bx];
     const int k = smat[gb];
     MatState& S = ms[k];
     const int off = md[k].off;
@@ -981,6 +1058,19 @@ __global__ __launch_bounds__(64) void k_root(const MatDesc* __restrict__ md, Mat
     }
 }
 
+__global__ __launch_bounds__(64) void k_root(const MatDesc* __restrict__ md, MatState* ms,
+                                             BlockRec* blk, const int* __restrict__ smat,
+                                             const int* __restrict__ blist, double* D, double* E,
+                                             double* DLr, double* DLLr, double* W, double* Werr,
+                                             double* Wgap, const int* __restrict__ nblist_p) {
+    const int nblist = *nblist_p;
+    const int nblk_ = nblist;
+    for (int bx = blockIdx.x; bx < nblk_; bx += gridDim.x) {
+        k_root_one(bx, md, ms, blk, smat, blist, D, E, DLr, DLLr, W, Werr, Wgap, nblist);
+        __syncthreads();
+    }
+}
+
 // ---------------------------------------------------------------------------------
 // k_refine_root: dlarrb on the root representation, one lane per eigenvalue
 // (refine_eigval_approx, mrrr_val.c:1075-1084; the whole block is one call, so its
@@ -1150,6 +1240,7 @@ struct TreeOut {
     int* eig_cl;            // next level: cluster of each eigen entry
     int* bcnt;              // singletons emitted per block in this level
     LevelCounters* cnt;
+    const int* halt;        // nonzero: the representation pool overflowed, emit nothing
     int capS, capC, capEig;
 };
 
@@ -1279,7 +1370,7 @@ __device__ __forceinline__ int wave_exscan64(int v, int* total) {
     return x - v;
 }
 
-__global__ __launch_bounds__(64) void k_classify_root(const MatDesc* __restrict__ md,
+__device__ __forceinline__ void k_classify_root_one(int bx, const MatDesc* __restrict__ md,
                                                       const MatState* __restrict__ ms,
                                                       const BlockRec* __restrict__ blk,
                                                       const int* __restrict__ smat,
@@ -1293,7 +1384,7 @@ __global__ __launch_bounds__(64) void k_classify_root(const MatDesc* __restrict_
                                                       const double* __restrict__ Wgap, double* Wsh,
                                                       TreeOut out, int nblist) {
     __shared__ int sh_base[3];
-    const int ib = blockIdx.x;
+    const int ib = bx;
     const int lane = threadIdx.x;
     if (ib >= nblist) return;
     const int gb = blist[ib];
@@ -1467,15 +1558,46 @@ __global__ __launch_bounds__(64) void k_classify_root(const MatDesc* __restrict_
     }
 }
 
+__global__ __launch_bounds__(64) void k_classify_root(const MatDesc* __restrict__ md,
+                                                      const MatState* __restrict__ ms,
+                                                      const BlockRec* __restrict__ blk,
+                                                      const int* __restrict__ smat,
+                                                      const int* __restrict__ blist,
+                                                      const double* __restrict__ gers,
+                                                      const double* __restrict__ D,
+                                                      const double* __restrict__ E,
+                                                      const double* __restrict__ DLr,
+                                                      const double* __restrict__ DLLr, double* W,
+                                                      const double* __restrict__ Werr,
+                                                      const double* __restrict__ Wgap, double* Wsh,
+                                                      TreeOut out, const int* __restrict__ nblist_p) {
+    const int nblist = *nblist_p;
+    const int nblk_ = nblist;
+    for (int bx = blockIdx.x; bx < nblk_; bx += gridDim.x) {
+        k_classify_root_one(bx, md, ms, blk, smat, blist, gers, D, E, DLr, DLLr, W, Werr, Wgap, Wsh, out, nblist);
+        __syncthreads();
+    }
+}
+
 // Deeper levels: one thread per cluster, after its eigenvalues were refined.
-__global__ __launch_bounds__(64) void k_classify(const ClTask* __restrict__ C, int nC,
+__device__ __forceinline__ void k_classify_one(int bx, const ClTask* __restrict__ C, int nC,
                                                  const double* __restrict__ Wgap,
                                                  const double* __restrict__ Wsh, TreeOut out) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= nC) return;
+    const int c = bx * blockDim.x + threadIdx.x;
+    if (c >= nC || *out.halt) return;
     const ClTask t = C[c];
     classify_cluster(t.cb, t.ce, t.depth, t.mat, t.gblk, t.wseg, t.bbeg, t.bsize, t.bl_spdiam,
                      t.lgap, t.cD, t.cL, t.cDL, t.cDLL, Wgap, Wsh, out);
+}
+
+__global__ __launch_bounds__(64) void k_classify(const ClTask* __restrict__ C, const int* __restrict__ nC_p,
+                                                 const double* __restrict__ Wgap,
+                                                 const double* __restrict__ Wsh, TreeOut out) {
+    const int nC = *nC_p;
+    const int nblk_ = (nC + 63) / 64;
+    for (int bx = blockIdx.x; bx < nblk_; bx += gridDim.x) {
+        k_classify_one(bx, C, nC, Wgap, Wsh, out);
+    }
 }
 
 // Cuts every block's singletons of this level into wave tasks of at most 64 and
@@ -1495,15 +1617,11 @@ __global__ __launch_bounds__(64) void k_zero(int* a, int na, int* b, int nb) {
 
 // zero_next (optional): the counters of the iteration after this one, which nothing
 // reads from here until then; cleared by the first wave.
-__global__ __launch_bounds__(64) void k_plan_count(const int* __restrict__ blist, int nblist,
+__device__ __forceinline__ void k_plan_count_one(int bx, const int* __restrict__ blist, int nblist,
                                                    const BlockRec* __restrict__ blk,
                                                    const int* __restrict__ bcnt, int tsize,
-                                                   LevelCounters* cnt, LevelCounters* zero_next) {
-    if (zero_next && blockIdx.x == 0) {
-        int* z = reinterpret_cast<int*>(zero_next);
-        for (int i = threadIdx.x; i < (int)(sizeof(LevelCounters) / sizeof(int)); i += 64) z[i] = 0;
-    }
-    const int ib = blockIdx.x * blockDim.x + threadIdx.x;
+                                                   LevelCounters* cnt) {
+    const int ib = bx * blockDim.x + threadIdx.x;
     if (ib >= nblist) return;
     const int gb = blist[ib];
     const int c = bcnt[gb];
@@ -1512,13 +1630,28 @@ __global__ __launch_bounds__(64) void k_plan_count(const int* __restrict__ blist
     atomicAdd(&cnt->cls[size_class(n)], (c + tsize - 1) / tsize);
 }
 
-__global__ __launch_bounds__(64) void k_plan_s(const int* __restrict__ blist, int nblist,
+__global__ __launch_bounds__(64) void k_plan_count(const int* __restrict__ blist, const int* __restrict__ nblist_p,
+                                                   const BlockRec* __restrict__ blk,
+                                                   const int* __restrict__ bcnt, int tsize,
+                                                   LevelCounters* cnt, LevelCounters* zero_next) {
+    if (zero_next && blockIdx.x == 0) {
+        int* z = reinterpret_cast<int*>(zero_next);
+        for (int i = threadIdx.x; i < (int)(sizeof(LevelCounters) / sizeof(int)); i += 64) z[i] = 0;
+    }
+    const int nblist = *nblist_p;
+    const int nblk_ = (nblist + 63) / 64;
+    for (int bx = blockIdx.x; bx < nblk_; bx += gridDim.x) {
+        k_plan_count_one(bx, blist, nblist, blk, bcnt, tsize, cnt);
+    }
+}
+
+__device__ __forceinline__ void k_plan_s_one(int bx, const int* __restrict__ blist, int nblist,
                                                const int* __restrict__ smat,
                                                const MatDesc* __restrict__ md,
                                                const BlockRec* __restrict__ blk, int* bcnt,
                                                int tsize, WaveTask* tasks, int capT,
                                                LevelCounters* cnt) {
-    const int ib = blockIdx.x * blockDim.x + threadIdx.x;
+    const int ib = bx * blockDim.x + threadIdx.x;
     if (ib >= nblist) return;
     const int gb = blist[ib];
     const int c = bcnt[gb];
@@ -1536,19 +1669,57 @@ __global__ __launch_bounds__(64) void k_plan_s(const int* __restrict__ blist, in
     for (int t = 0; t < k; ++t) tasks[base + t] = WaveTask{wseg + tsize * t, min(tsize, c - tsize * t)};
 }
 
-// Assigns each cluster of a level its arena slot (4 * bsize doubles).
-__global__ __launch_bounds__(64) void k_arena_assign(ClTask* C, int nC, double* arena,
-                                                      unsigned long long* bump) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(64) void k_plan_s(const int* __restrict__ blist, const int* __restrict__ nblist_p,
+                                               const int* __restrict__ smat,
+                                               const MatDesc* __restrict__ md,
+                                               const BlockRec* __restrict__ blk, int* bcnt,
+                                               int tsize, WaveTask* tasks, int capT,
+                                               LevelCounters* cnt) {
+    const int nblist = *nblist_p;
+    const int nblk_ = (nblist + 63) / 64;
+    for (int bx = blockIdx.x; bx < nblk_; bx += gridDim.x) {
+        k_plan_s_one(bx, blist, nblist, smat, md, blk, bcnt, tsize, tasks, capT, cnt);
+    }
+}
+
+// Assigns each cluster of a level its slot of 4 * bsize doubles in the solve's pool of
+// representations (the slots of all levels of the solve, handed out by one bump
+// counter).  A slot that does not fit sets the pool flag and takes the spare slot
+// instead (4 * nmax doubles), so that every write stays inside allocated memory: the
+// level's classification then emits nothing, the walk ends, and the caller repeats the
+// solve with a pool as large as the bump counter grew.
+__device__ __forceinline__ void k_arena_assign_one(int bx, ClTask* C, int nC, double* arena,
+                                                   unsigned long long* bump,
+                                                   unsigned long long cap, double* spare,
+                                                   int* flag) {
+    const int c = bx * blockDim.x + threadIdx.x;
     if (c >= nC) return;
     ClTask& t = C[c];
-    unsigned long long o = atomicAdd(bump, (unsigned long long)(4LL * t.bsize));
-    double* base = arena + o;
-    t.arena_off = (long long)o;
+    const unsigned long long need = (unsigned long long)(4LL * t.bsize);
+    const unsigned long long o = atomicAdd(bump, need);
+    double* base;
+    if (o + need > cap) {
+        atomicOr(flag, 1);
+        base = spare;
+        t.arena_off = -1;
+    } else {
+        base = arena + o;
+        t.arena_off = (long long)o;
+    }
     t.cD = base;
     t.cL = base + t.bsize;
     t.cDL = base + 2 * t.bsize;
     t.cDLL = base + 3 * t.bsize;
+}
+
+__global__ __launch_bounds__(64) void k_arena_assign(ClTask* C, const int* __restrict__ nC_p,
+                                                     double* arena, unsigned long long* bump,
+                                                     unsigned long long cap, double* spare,
+                                                     int* flag) {
+    const int nC = *nC_p;
+    const int nblk_ = (nC + 63) / 64;
+    for (int bx = blockIdx.x; bx < nblk_; bx += gridDim.x)
+        k_arena_assign_one(bx, C, nC, arena, bump, cap, spare, flag);
 }
 
 // ---------------------------------------------------------------------------------
@@ -1814,13 +1985,13 @@ __device__ __forceinline__ RrfEnds rrf_ends(const ClTask& t, const double* Wsh, 
 // (process_cluster.c:168-195), one lane per cluster end.  The two dlarrb calls of a
 // cluster run with a zero gap and do not depend on each other; results go to Wtmp
 // and are committed by k_dlarrf.
-__global__ __launch_bounds__(64) void k_endref(const ClTask* __restrict__ C, int nC,
+__device__ __forceinline__ void k_endref_one(int bx, const ClTask* __restrict__ C, int nC,
                                                const int* __restrict__ Windex,
                                                const double* __restrict__ Wsh,
                                                const double* __restrict__ Werr,
                                                const MatState* __restrict__ ms, double* Wtmp,
                                                double* Werrtmp) {
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    const int e = bx * blockDim.x + threadIdx.x;
     if (e >= 2 * nC) return;
     const ClTask t = C[e >> 1];
     const int slot = (e & 1) ? t.ce : t.cb;
@@ -1831,6 +2002,20 @@ __global__ __launch_bounds__(64) void k_endref(const ClTask* __restrict__ C, int
     add_steps(kStepEndref, (unsigned long long)sw * t.bsize);
     Wtmp[slot] = w;
     Werrtmp[slot] = we;
+}
+
+__global__ __launch_bounds__(64) void k_endref(const ClTask* __restrict__ C, const int* __restrict__ nC_p,
+                                               const int* __restrict__ Windex,
+                                               const double* __restrict__ Wsh,
+                                               const double* __restrict__ Werr,
+                                               const MatState* __restrict__ ms, double* Wtmp,
+                                               double* Werrtmp, long long budget) {
+    const int nC = *nC_p;
+    if (group_lanes_hd(2LL * nC, budget) != 1) return;   // k_endref_grp takes them
+    const int nblk_ = (2 * nC + 63) / 64;
+    for (int bx = blockIdx.x; bx < nblk_; bx += gridDim.x) {
+        k_endref_one(bx, C, nC, Windex, Wsh, Werr, ms, Wtmp, Werrtmp);
+    }
 }
 
 // The chosen factorisation L+ D+ L+^T = L D L^T - tau I of each lane's cluster,
@@ -1903,13 +2088,13 @@ __device__ __forceinline__ void rrf_store(int n, int nmax, P d, P l, P ld, doubl
 // With one lane per cluster the kernel also stores the chosen factorisation; with four,
 // k_rrf_store does.
 template <int G>
-__global__ __launch_bounds__(64) void k_dlarrf(ClTask* C, int nC, double* Wsh, double* Werr,
+__device__ __forceinline__ void k_dlarrf_one(int bx, ClTask* C, int nC, double* Wsh, double* Werr,
                                                double* Wgap, const double* __restrict__ Wtmp,
                                                const double* __restrict__ Werrtmp,
                                                const MatState* __restrict__ ms) {
     const int lane = threadIdx.x;
     const int sub = lane % G, gbase = lane - sub;
-    const int c = blockIdx.x * (64 / G) + lane / G;
+    const int c = bx * (64 / G) + lane / G;
     const bool valid = c < nC;
     ClTask& t = C[min(c, nC - 1)];
     const int cb = t.cb, ce = t.ce, clsize = ce - cb + 1, n = t.bsize;
@@ -1988,17 +2173,31 @@ __global__ __launch_bounds__(64) void k_dlarrf(ClTask* C, int nC, double* Wsh, d
     }
 }
 
+template <int G>
+__global__ __launch_bounds__(64) void k_dlarrf(ClTask* C, const int* __restrict__ nC_p, int rrf_max, double* Wsh, double* Werr,
+                                               double* Wgap, const double* __restrict__ Wtmp,
+                                               const double* __restrict__ Werrtmp,
+                                               const MatState* __restrict__ ms) {
+    const int nC = *nC_p;
+    if ((G == 1) != (nC > rrf_max)) return;   // one lane per cluster only past rrf_max
+    const int nblk_ = (nC + 64 / G - 1) / (64 / G);
+    for (int bx = blockIdx.x; bx < nblk_; bx += gridDim.x) {
+        k_dlarrf_one<G>(bx, C, nC, Wsh, Werr, Wgap, Wtmp, Werrtmp, ms);
+        __syncthreads();
+    }
+}
+
 // The chosen factorisation L+ D+ L+^T = L D L^T - tau I of every cluster, recomputed
 // with the operations of the accepted trial, into cD (D+) and cL (L+); one lane per
 // cluster, stored through LDS (rrf_store).
-__global__ __launch_bounds__(64) void k_rrf_store(const ClTask* __restrict__ C, int nC,
+__device__ __forceinline__ void k_rrf_store_one(int bx, const ClTask* __restrict__ C, int nC,
                                                   const MatState* __restrict__ ms) {
     __shared__ double bD[16][65], bL[16][65];
     __shared__ double* dD[64];
     __shared__ double* dL[64];
     __shared__ int nn[64];
     const int lane = threadIdx.x;
-    const int c = blockIdx.x * blockDim.x + lane;
+    const int c = bx * blockDim.x + lane;
     const bool valid = c < nC;
     // lanes past the last cluster sweep the last cluster's parent (nothing is stored
     // for them), so every lane reads valid memory
@@ -2020,11 +2219,22 @@ __global__ __launch_bounds__(64) void k_rrf_store(const ClTask* __restrict__ C, 
                   bL, dD, dL, nn);
 }
 
+__global__ __launch_bounds__(64) void k_rrf_store(const ClTask* __restrict__ C, const int* __restrict__ nC_p, int rrf_max,
+                                                  const MatState* __restrict__ ms) {
+    const int nC = *nC_p;
+    if (nC > rrf_max) return;   // k_dlarrf<1> stored them
+    const int nblk_ = (nC + 63) / 64;
+    for (int bx = blockIdx.x; bx < nblk_; bx += gridDim.x) {
+        k_rrf_store_one(bx, C, nC, ms);
+        __syncthreads();
+    }
+}
+
 // Accumulates the shift, forms DL and DLL from the chosen factorisation and moves
 // the cluster's eigenvalues to the new representation (process_cluster.c:206-232).
-__global__ __launch_bounds__(64) void k_newrrr_fin(const ClTask* __restrict__ C, int nC,
+__device__ __forceinline__ void k_newrrr_fin_one(int bx, const ClTask* __restrict__ C, int nC,
                                                    double* Wsh, double* Werr) {
-    const int c = blockIdx.x;
+    const int c = bx;
     if (c >= nC) return;
     const ClTask t = C[c];
     const int n = t.bsize;
@@ -2051,9 +2261,18 @@ __global__ __launch_bounds__(64) void k_newrrr_fin(const ClTask* __restrict__ C,
     }
 }
 
+__global__ __launch_bounds__(64) void k_newrrr_fin(const ClTask* __restrict__ C, const int* __restrict__ nC_p,
+                                                   double* Wsh, double* Werr) {
+    const int nC = *nC_p;
+    const int nblk_ = nC;
+    for (int bx = blockIdx.x; bx < nblk_; bx += gridDim.x) {
+        k_newrrr_fin_one(bx, C, nC, Wsh, Werr);
+    }
+}
+
 // Refinement of a cluster's eigenvalues against its new representation, one lane
 // per eigenvalue (process_cluster.c:414-418; the cluster is one dlarrb call).
-__global__ __launch_bounds__(64) void k_refine_cl(const ClTask* __restrict__ C,
+__device__ __forceinline__ void k_refine_cl_one(int bx, const ClTask* __restrict__ C,
                                                    const int* __restrict__ eig_slot,
                                                    const int* __restrict__ eig_cl, int nE,
                                                    const int* __restrict__ Windex,
@@ -2062,7 +2281,7 @@ __global__ __launch_bounds__(64) void k_refine_cl(const ClTask* __restrict__ C,
                                                    const double* __restrict__ Wgap,
                                                    const MatState* __restrict__ ms, double* Wtmp,
                                                    double* Werrtmp) {
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    const int e = bx * blockDim.x + threadIdx.x;
     if (e >= nE) return;
     const int slot = eig_slot[e];
     const ClTask t = C[eig_cl[e]];
@@ -2077,13 +2296,30 @@ __global__ __launch_bounds__(64) void k_refine_cl(const ClTask* __restrict__ C,
     Werrtmp[slot] = we;
 }
 
-__global__ __launch_bounds__(64) void k_gaps_cl(const ClTask* __restrict__ C,
+__global__ __launch_bounds__(64) void k_refine_cl(const ClTask* __restrict__ C,
+                                                   const int* __restrict__ eig_slot,
+                                                   const int* __restrict__ eig_cl, const int* __restrict__ nE_p,
+                                                   const int* __restrict__ Windex,
+                                                   const double* __restrict__ Wsh,
+                                                   const double* __restrict__ Werr,
+                                                   const double* __restrict__ Wgap,
+                                                   const MatState* __restrict__ ms, double* Wtmp,
+                                                   double* Werrtmp, long long budget) {
+    const int nE = *nE_p;
+    if (group_lanes_hd(nE, budget) != 1) return;   // k_refine_cl_grp takes them
+    const int nblk_ = (nE + 63) / 64;
+    for (int bx = blockIdx.x; bx < nblk_; bx += gridDim.x) {
+        k_refine_cl_one(bx, C, eig_slot, eig_cl, nE, Windex, Wsh, Werr, Wgap, ms, Wtmp, Werrtmp);
+    }
+}
+
+__device__ __forceinline__ void k_gaps_cl_one(int bx, const ClTask* __restrict__ C,
                                                  const int* __restrict__ eig_slot,
                                                  const int* __restrict__ eig_cl, int nE,
                                                  const double* __restrict__ Wtmp,
                                                  const double* __restrict__ Werrtmp, double* Wsh,
                                                  double* Werr, double* Wgap, double* W) {
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    const int e = bx * blockDim.x + threadIdx.x;
     if (e >= nE) return;
     const int slot = eig_slot[e];
     const ClTask t = C[eig_cl[e]];
@@ -2092,6 +2328,19 @@ __global__ __launch_bounds__(64) void k_gaps_cl(const ClTask* __restrict__ C,
     Werr[slot] = we;
     if (slot < t.ce) Wgap[slot] = fmax(0.0, Wtmp[slot + 1] - Werrtmp[slot + 1] - w - we);
     W[slot] = w + t.cL[t.bsize - 1];
+}
+
+__global__ __launch_bounds__(64) void k_gaps_cl(const ClTask* __restrict__ C,
+                                                 const int* __restrict__ eig_slot,
+                                                 const int* __restrict__ eig_cl, const int* __restrict__ nE_p,
+                                                 const double* __restrict__ Wtmp,
+                                                 const double* __restrict__ Werrtmp, double* Wsh,
+                                                 double* Werr, double* Wgap, double* W) {
+    const int nE = *nE_p;
+    const int nblk_ = (nE + 63) / 64;
+    for (int bx = blockIdx.x; bx < nblk_; bx += gridDim.x) {
+        k_gaps_cl_one(bx, C, eig_slot, eig_cl, nE, Wtmp, Werrtmp, Wsh, Werr, Wgap, W);
+    }
 }
 
 // ---------------------------------------------------------------------------------
@@ -3173,7 +3422,7 @@ __device__ __forceinline__ long long dlarrb_grp(int G, int sub, int gbase, bool 
 // per eigenvalue (G = 4 .. 64, chosen on the host from the level's counts): a lane
 // per eigenvalue would leave them a long chain of dependent sweeps on few waves, and
 // they gate the next level.  Same results as k_endref and k_refine_cl.
-__global__ __launch_bounds__(64) void k_endref_grp(const ClTask* __restrict__ C, int nC, int G,
+__device__ __forceinline__ void k_endref_grp_one(int bx, const ClTask* __restrict__ C, int nC, int G,
                                                    const int* __restrict__ Windex,
                                                    const double* __restrict__ Wsh,
                                                    const double* __restrict__ Werr,
@@ -3182,7 +3431,7 @@ __global__ __launch_bounds__(64) void k_endref_grp(const ClTask* __restrict__ C,
     const int lane = threadIdx.x;
     const int sub = lane % G, gbase = lane - sub;
     const int ne = 2 * nC;
-    const int e = blockIdx.x * (SW / G) + lane / G;
+    const int e = bx * (SW / G) + lane / G;
     const bool valid = e < ne;
     const int ee = min(e, ne - 1);
     const ClTask t = C[ee >> 1];
@@ -3207,7 +3456,22 @@ __global__ __launch_bounds__(64) void k_endref_grp(const ClTask* __restrict__ C,
     }
 }
 
-__global__ __launch_bounds__(64) void k_refine_cl_grp(const ClTask* __restrict__ C,
+__global__ __launch_bounds__(64) void k_endref_grp(const ClTask* __restrict__ C, const int* __restrict__ nC_p, long long budget,
+                                                   const int* __restrict__ Windex,
+                                                   const double* __restrict__ Wsh,
+                                                   const double* __restrict__ Werr,
+                                                   const MatState* __restrict__ ms, double* Wtmp,
+                                                   double* Werrtmp) {
+    const int nC = *nC_p;
+    const int G = group_lanes_hd(2LL * nC, budget);
+    if (G == 1) return;
+    const int nblk_ = (2 * nC + SW / G - 1) / (SW / G);
+    for (int bx = blockIdx.x; bx < nblk_; bx += gridDim.x) {
+        k_endref_grp_one(bx, C, nC, G, Windex, Wsh, Werr, ms, Wtmp, Werrtmp);
+    }
+}
+
+__device__ __forceinline__ void k_refine_cl_grp_one(int bx, const ClTask* __restrict__ C,
                                                       const int* __restrict__ eig_slot,
                                                       const int* __restrict__ eig_cl, int nE,
                                                       int G, const int* __restrict__ Windex,
@@ -3218,7 +3482,7 @@ __global__ __launch_bounds__(64) void k_refine_cl_grp(const ClTask* __restrict__
                                                       double* Wtmp, double* Werrtmp) {
     const int lane = threadIdx.x;
     const int sub = lane % G, gbase = lane - sub;
-    const int e = blockIdx.x * (SW / G) + lane / G;
+    const int e = bx * (SW / G) + lane / G;
     const bool valid = e < nE;
     const int ee = min(e, nE - 1);
     const int slot = eig_slot[ee];
@@ -3242,6 +3506,24 @@ __global__ __launch_bounds__(64) void k_refine_cl_grp(const ClTask* __restrict__
         add_steps(kStepRefineCl, (unsigned long long)sw * n);
         Wtmp[slot] = w;
         Werrtmp[slot] = we;
+    }
+}
+
+__global__ __launch_bounds__(64) void k_refine_cl_grp(const ClTask* __restrict__ C,
+                                                      const int* __restrict__ eig_slot,
+                                                      const int* __restrict__ eig_cl, const int* __restrict__ nE_p,
+                                                      long long budget, const int* __restrict__ Windex,
+                                                      const double* __restrict__ Wsh,
+                                                      const double* __restrict__ Werr,
+                                                      const double* __restrict__ Wgap,
+                                                      const MatState* __restrict__ ms,
+                                                      double* Wtmp, double* Werrtmp) {
+    const int nE = *nE_p;
+    const int G = group_lanes_hd(nE, budget);
+    if (G == 1) return;
+    const int nblk_ = (nE + SW / G - 1) / (SW / G);
+    for (int bx = blockIdx.x; bx < nblk_; bx += gridDim.x) {
+        k_refine_cl_grp_one(bx, C, eig_slot, eig_cl, nE, G, Windex, Wsh, Werr, Wgap, ms, Wtmp, Werrtmp);
     }
 }
 
@@ -3750,15 +4032,15 @@ __global__ __launch_bounds__(128) void k_s_tail(const int* __restrict__ DI,
 // rows, DIb for larger ones) and the ones that need the bisection fallback first.  One wave per
 // task of the previous iteration.  Counters: nS = singletons going on, nEig = of
 // them needing bisection.
-__global__ __launch_bounds__(64) void k_s_collect(const WaveTask* __restrict__ tasks,
+__device__ __forceinline__ void k_s_collect_one(int bx, const WaveTask* __restrict__ tasks,
                                                   const LevelCounters* __restrict__ pcnt,
                                                   int first, const int* __restrict__ SIp,
                                                   const STask* __restrict__ T,
                                                   const SState* __restrict__ ST, int* SIn,
                                                   int* bcnt, int* BI, int* DI, int* DIb,
                                                   LevelCounters* cnt) {
-    if ((int)blockIdx.x >= pcnt->nT) return;
-    const WaveTask wt = tasks[blockIdx.x];
+    if ((int)bx >= pcnt->nT) return;
+    const WaveTask wt = tasks[bx];
     const int lane = threadIdx.x;
     const bool valid = lane < wt.cnt;
     const int idx = valid ? (first ? wt.start + lane : SIp[wt.start + lane]) : 0;
@@ -3796,10 +4078,24 @@ __global__ __launch_bounds__(64) void k_s_collect(const WaveTask* __restrict__ t
     if (bis) BI[b_i + __popcll(m_i & below)] = idx;
 }
 
+__global__ __launch_bounds__(64) void k_s_collect(const WaveTask* __restrict__ tasks,
+                                                  const LevelCounters* __restrict__ pcnt,
+                                                  int first, const int* __restrict__ SIp,
+                                                  const STask* __restrict__ T,
+                                                  const SState* __restrict__ ST, int* SIn,
+                                                  int* bcnt, int* BI, int* DI, int* DIb,
+                                                  LevelCounters* cnt) {
+    const int nT_ = pcnt->nT;
+    const int nblk_ = nT_;
+    for (int bx = blockIdx.x; bx < nblk_; bx += gridDim.x) {
+        k_s_collect_one(bx, tasks, pcnt, first, SIp, T, ST, SIn, bcnt, BI, DI, DIb, cnt);
+    }
+}
+
 // Final vectors: Z(isupp) scaled by 1/||z|| (the last dlar1v's NRMINV), zeros in the
 // rest of the column (process_singleton.c:300-324).  One wave per task of the first
 // iteration, which lists every singleton of the level.
-__global__ __launch_bounds__(64) void k_s_normalize(const WaveTask* __restrict__ tasks,
+__device__ __forceinline__ void k_s_normalize_one(int bx, const WaveTask* __restrict__ tasks,
                                                     const LevelCounters* __restrict__ cnt,
                                                     const STask* __restrict__ T,
                                                     const MatDesc* __restrict__ md,
@@ -3808,8 +4104,8 @@ __global__ __launch_bounds__(64) void k_s_normalize(const WaveTask* __restrict__
     __shared__ double* zc_s[SW];
     __shared__ int lo_s[SW], hi_s[SW];
     __shared__ double sc_s[SW];
-    if ((int)blockIdx.x >= cnt->nT) return;
-    const WaveTask wt = tasks[blockIdx.x];
+    if ((int)bx >= cnt->nT) return;
+    const WaveTask wt = tasks[bx];
     const int lane = threadIdx.x;
     const int cntw = wt.cnt;
     const int idx = wt.start + min(lane, cntw - 1);
@@ -3842,6 +4138,20 @@ __global__ __launch_bounds__(64) void k_s_normalize(const WaveTask* __restrict__
                 if (rr < nfull) zc[rr] = v[u];
             }
         }
+    }
+}
+
+__global__ __launch_bounds__(64) void k_s_normalize(const WaveTask* __restrict__ tasks,
+                                                    const LevelCounters* __restrict__ cnt,
+                                                    const STask* __restrict__ T,
+                                                    const MatDesc* __restrict__ md,
+                                                    const int* __restrict__ Zidx,
+                                                    const SState* __restrict__ ST) {
+    const int nT_ = cnt->nT;
+    const int nblk_ = nT_;
+    for (int bx = blockIdx.x; bx < nblk_; bx += gridDim.x) {
+        k_s_normalize_one(bx, tasks, cnt, T, md, Zidx, ST);
+        __syncthreads();
     }
 }
 
