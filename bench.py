@@ -169,6 +169,29 @@ def timed_steps(M, torch, arr, stream, steps, warmup, pairs):
     return dt, prof, counted
 
 
+def graph_steps(M, torch, arr, stream, steps, pairs):
+    """The same batch captured once as one HIP graph (mr3_graph_create: the small
+    route and the pipeline with the tree's depth fixed, in memory the graph owns) and
+    replayed `steps` times, each replay waited for like a mr3_solve_batch call.  Returns
+    (seconds of the replays, seconds of the capture)."""
+    t0 = time.perf_counter()
+    g = M.BatchGraph(arr, stream=stream)
+    tcap = time.perf_counter() - t0
+    assert g.rc == 0, g.rc
+    assert g.launch(stream) == 0
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        rc = g.launch(stream)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    assert rc == 0 and sum(a.m for a in arr) == pairs
+    g.close()
+    log(f"graph: {dt / steps * 1e3:.1f} ms/step, {pairs * steps / dt:.4g} eigenpairs/s "
+        f"(capture {tcap:.2f} s)")
+    return dt, tcap
+
+
 def plain_path(M, torch, probs, stream, stride):
     """The plain path: one solve per matrix over every stride-th problem, same
     device-resident buffers.  Returns (eigenpairs/s, matrices solved)."""
@@ -375,7 +398,7 @@ def subset_main(args, torch, M, dev):
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": dt / args.steps * 1e3,
+        "ms_per_step": step_s / args.steps * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -484,8 +507,14 @@ def main():
         tuning_src = os.path.relpath(args.tuning_file, REPO)
     tuning = dict(M.get_tuning(), source=tuning_src)
     dt, prof, counted = timed_steps(M, torch, arr, stream, args.steps, args.warmup, pairs)
-    value = pairs * args.steps / dt
+    eager_value = pairs * args.steps / dt
     prof_steps = args.steps
+    # the same steps replayed from one captured graph; the headline is the faster form
+    gdt, gcap = graph_steps(M, torch, arr, stream, args.steps, pairs)
+    graph_value = pairs * args.steps / gdt
+    launch = "graph" if graph_value > eager_value else "eager"
+    value = max(graph_value, eager_value)
+    step_s = gdt if launch == "graph" else dt
     plain, nplain = plain_path(M, torch, probs, stream, args.plain_stride)
 
     worst_r, worst_o = accuracy_sample(mats, keep, arr, 64)
@@ -541,7 +570,7 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": dt / args.steps * 1e3,
+        "ms_per_step": step_s / args.steps * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -550,7 +579,11 @@ def main():
         "config": {"workload": f"batched mixed tridiagonals: {args.count} matrices, n log-uniform "
                                f"[32,2048], 50% U(-1,1) / 25% perturbed 1-2-1 / 25% glued W21+ "
                                f"(glue 1e-10), jobz=V range=A",
-                   "matrices": args.count, "eigenpairs_per_step": pairs},
+                   "matrices": args.count, "eigenpairs_per_step": pairs, "launch": launch},
+        "eager": {"eigenpairs_per_s": eager_value, "ms_per_step": dt / args.steps * 1e3,
+                  "how": "mr3_solve_batch per step"},
+        "graph": {"eigenpairs_per_s": graph_value, "ms_per_step": gdt / args.steps * 1e3,
+                  "capture_s": gcap, "how": "one captured hipGraph replayed per step (mr3_graph_launch)"},
         "plain_value": plain,
         "batched_over_plain": value / plain,
         "plain_sample": f"every {args.plain_stride}th matrix, one solve per matrix",

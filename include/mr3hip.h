@@ -174,6 +174,28 @@ int mr3_solve_one(mr3_problem *prob, void *stream);
 int mr3_solve_small_noalloc(mr3_problem *probs, int nprob, void *stream);
 
 /*
+ * A batched solve captured as one HIP graph, for a caller that solves the same
+ * problems (the same device arrays; D and E may change between launches) again and
+ * again: the small-matrix route and the task-parallel pipeline with a fixed number of
+ * tree levels and one pool for the child representations, in device memory the graph
+ * owns, launched with no host work inside.  No counterpart in the reference (the
+ * reference walks its representation tree with a pthreads work queue, mrrr_vec.c:
+ * 226-299, queue.c); this is the replayable form of mr3_solve_batch.
+ * mr3_graph_create solves the batch twice as mr3_solve_batch does (to learn the depth
+ * of its tree and the slots it needs; the problems' outputs are valid afterwards) and
+ * captures it; it returns what mr3_solve_batch returns, with *g NULL on failure.
+ * mr3_graph_launch replays it on stream (NULL: the library's per-thread stream),
+ * waits, and fills the problems' scalar outputs like mr3_solve_batch; it returns 4,
+ * with info 2 for the pipeline's problems, when their tree this time needs more
+ * levels or slots than the capture had (new D and E): solve them with
+ * mr3_solve_batch, or capture again.  probs must be the array given at creation.
+ */
+typedef struct mr3_graph mr3_graph;
+int mr3_graph_create(mr3_problem *probs, int nprob, void *stream, mr3_graph **g);
+int mr3_graph_launch(mr3_graph *g, mr3_problem *probs, void *stream);
+void mr3_graph_destroy(mr3_graph *g);
+
+/*
  * Serving: a request loop that admits problems as device memory and concurrency
  * slots free up, runs them on several streams at once and reports completion.
  * budget_bytes bounds the device memory of admitted problems (8 n m for Z plus the

@@ -99,6 +99,12 @@ def lib():
         L.mr3_solve_one.argtypes = [ctypes.POINTER(Problem), _P]
         L.mr3_solve_small_noalloc.restype = ctypes.c_int
         L.mr3_solve_small_noalloc.argtypes = [ctypes.POINTER(Problem), ctypes.c_int, _P]
+        L.mr3_graph_create.restype = ctypes.c_int
+        L.mr3_graph_create.argtypes = [ctypes.POINTER(Problem), ctypes.c_int, _P, ctypes.POINTER(_P)]
+        L.mr3_graph_launch.restype = ctypes.c_int
+        L.mr3_graph_launch.argtypes = [_P, ctypes.POINTER(Problem), _P]
+        L.mr3_graph_destroy.restype = None
+        L.mr3_graph_destroy.argtypes = [_P]
         L.mr3_server_create.restype = _P
         L.mr3_server_create.argtypes = [ctypes.c_longlong, ctypes.c_int, ctypes.c_int]
         L.mr3_server_submit.restype = ctypes.c_longlong
@@ -309,6 +315,35 @@ def run_batch(arr, stream=None):
     it points at are read again, the outputs rewritten); returns the return code."""
     st = None if stream is None else ctypes.c_void_p(stream.cuda_stream)
     return lib().mr3_solve_batch(arr, len(arr), st)
+
+
+class BatchGraph:
+    """A batched solve captured as one HIP graph (mr3_graph_create) over a Problem
+    array from prepare_batch; launch() replays it (mr3_graph_launch) and returns the
+    return code (4: the tree needed more levels or slots than the capture had)."""
+
+    def __init__(self, arr, stream=None):
+        self.arr = arr
+        self.h = ctypes.c_void_p(None)
+        st = None if stream is None else ctypes.c_void_p(stream.cuda_stream)
+        self.rc = lib().mr3_graph_create(arr, len(arr), st, ctypes.byref(self.h))
+        if not self.h.value:
+            raise RuntimeError(f"mr3_graph_create failed (rc {self.rc})")
+
+    def launch(self, stream=None):
+        st = None if stream is None else ctypes.c_void_p(stream.cuda_stream)
+        return lib().mr3_graph_launch(self.h, self.arr, st)
+
+    def close(self):
+        if self.h.value:
+            lib().mr3_graph_destroy(self.h)
+            self.h = ctypes.c_void_p(None)
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def run_batch_noalloc(arr, stream=None):

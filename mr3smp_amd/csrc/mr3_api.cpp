@@ -22,6 +22,13 @@ int run_pipeline(std::vector<MatDesc>& md, std::vector<MatState>& ms, hipStream_
 int small_begin(SmallJob& job, const std::vector<MatDesc>& md, hipStream_t st);
 int small_end(SmallJob& job, std::vector<MatState>& ms, hipStream_t st);
 constexpr int kSmallRouteBelow = 500;   // mrrr.c:149-153 (DSTEMR_IF_SMALLER)
+struct BatchGraph;
+BatchGraph* graph_new();
+void graph_delete(BatchGraph* g);
+int graph_create(std::vector<MatDesc>& mdS, std::vector<MatDesc>& mdL, hipStream_t st,
+                 BatchGraph* g);
+int graph_launch(BatchGraph* g, hipStream_t st, std::vector<MatState>& msS,
+                 std::vector<MatState>& msL);
 void profile_enable(int on);
 int set_tuning(const mr3_tuning* t);
 void get_tuning(mr3_tuning* t);
@@ -74,6 +81,81 @@ extern "C" int mr3_solve_batch(mr3_problem* probs, int nprob, void* stream) {
     fill(idxS, msS, rcE);
     fill(idxL, msL, rcL);
     return worst;
+}
+
+// The problems of a batch split by route: host-answered and invalid ones are done here
+// (their outputs set), n < 500 to the small route, the rest to the pipeline.
+static int split_batch(mr3_problem* probs, int nprob, std::vector<MatDesc>& mdS,
+                       std::vector<MatDesc>& mdL, std::vector<int>& idxS, std::vector<int>& idxL) {
+    int worst = 0;
+    for (int i = 0; i < nprob; ++i) {
+        mr3_problem& p = probs[i];
+        const int how = prepare_problem(p);
+        if (how > 0) { worst = std::max(worst, 1); continue; }
+        if (how < 0) continue;
+        if (p.n < kSmallRouteBelow) { mdS.push_back(to_desc(p)); idxS.push_back(i); }
+        else { mdL.push_back(to_desc(p)); idxL.push_back(i); }
+    }
+    return worst;
+}
+
+struct mr3_graph {
+    BatchGraph* g = nullptr;
+    int nprob = 0;
+    std::vector<int> idxS, idxL;
+};
+
+extern "C" int mr3_graph_create(mr3_problem* probs, int nprob, void* stream, mr3_graph** out) {
+    if (!out) return 1;
+    *out = nullptr;
+    if (nprob <= 0) return 1;
+    hipStream_t st = stream ? (hipStream_t)stream : thread_stream();
+    if (!st) return 2;
+    mr3_graph* h = new mr3_graph();
+    h->nprob = nprob;
+    std::vector<MatDesc> mdS, mdL;
+    int worst = split_batch(probs, nprob, mdS, mdL, h->idxS, h->idxL);
+    h->g = graph_new();
+    const int rc = graph_create(mdS, mdL, st, h->g);
+    if (rc != 0) {
+        graph_delete(h->g);
+        delete h;
+        for (int i = 0; i < nprob; ++i)
+            if (probs[i].info == 0) probs[i].info = 2;
+        return 2;
+    }
+    *out = h;
+    // the outputs of the last solve run by the creation (the capture itself runs nothing)
+    return mr3_graph_launch(h, probs, stream) == 0 ? worst : 2;
+}
+
+extern "C" int mr3_graph_launch(mr3_graph* h, mr3_problem* probs, void* stream) {
+    if (!h || !probs) return 1;
+    hipStream_t st = stream ? (hipStream_t)stream : thread_stream();
+    if (!st) return 2;
+    std::vector<MatDesc> mdS, mdL;
+    std::vector<int> idxS, idxL;
+    int worst = split_batch(probs, h->nprob, mdS, mdL, idxS, idxL);
+    if (idxS != h->idxS || idxL != h->idxL) return 1;   // not the captured problems
+    std::vector<MatState> msS, msL;
+    const int rc = graph_launch(h->g, st, msS, msL);
+    for (size_t j = 0; j < idxS.size(); ++j) {
+        mr3_problem& p = probs[idxS[j]];
+        take_state(p, rc == 2 ? MatState() : msS[j], rc == 2 ? 2 : 0);
+        if (p.info) worst = std::max(worst, p.info);
+    }
+    for (size_t j = 0; j < idxL.size(); ++j) {
+        mr3_problem& p = probs[idxL[j]];
+        take_state(p, rc == 0 ? msL[j] : MatState(), rc == 0 ? 0 : 2);
+        if (p.info) worst = std::max(worst, p.info);
+    }
+    return rc == 4 ? 4 : worst;
+}
+
+extern "C" void mr3_graph_destroy(mr3_graph* h) {
+    if (!h) return;
+    graph_delete(h->g);
+    delete h;
 }
 
 extern "C" int mr3_solve_one(mr3_problem* prob, void* stream) {

@@ -223,6 +223,11 @@ struct Alloc {
     size_t off = 0, need = 0;
     std::vector<void*> extra;   // buffers beyond the block
     bool ok = true;
+    bool borrowed = false;      // a block owned by a captured graph: no growth, no release
+    Alloc(hipStream_t s, char* base, size_t cap) : st(s), borrowed(true) {
+        ws.base = base;
+        ws.cap = cap;
+    }
     explicit Alloc(hipStream_t s) : st(s) {
         std::lock_guard<std::mutex> lk(g_ws_mu);
         if (!g_ws_free.empty()) {
@@ -245,7 +250,7 @@ struct Alloc {
             return (T*)p;
         }
         void* p = nullptr;
-        if (hipMallocAsync(&p, bytes, st) != hipSuccess) { ok = false; return nullptr; }
+        if (borrowed || hipMallocAsync(&p, bytes, st) != hipSuccess) { ok = false; return nullptr; }
         extra.push_back(p);
         return (T*)p;
     }
@@ -257,6 +262,7 @@ struct Alloc {
         return p ? p + kSweepPad : nullptr;
     }
     ~Alloc() {
+        if (borrowed) return;
         // every stream of the solve is idle once the main stream is: the side streams
         // were synchronised when the level loop's scope ended
         (void)hipStreamSynchronize(st);
@@ -342,6 +348,36 @@ static unsigned grid_for(long long items, int items_per_wg, unsigned cap) {
 }
 constexpr unsigned kGridCap = 8192;
 
+// Device memory and pinned host buffers that a captured solve owns: a captured graph
+// copies its descriptors from pinned memory and its results into pinned memory, and
+// works in a block nothing else uses.
+struct GraphWs {
+    char* base = nullptr;
+    size_t cap = 0;
+    MatDesc* hmd = nullptr;        // descriptors (the copy source of the graph)
+    int* horder = nullptr;
+    MatState* hms = nullptr;       // states after a replay
+    LevelCounters* hpin = nullptr; // level counters after a replay
+    int* hflag = nullptr;          // the representation pool's overflow flag
+    int nmat = 0;
+};
+
+// Options and results of one pass of the pipeline.
+struct PipeOpts {
+    int fixed_levels = -1;   // >= 0: exactly that many tree levels, slots from one pool
+    size_t pool_cap = 0;     // doubles of that pool
+    GraphWs* gw = nullptr;   // set while capturing: gw's block and buffers, no waits
+    int nwaves = 0;          // > 0: singleton waves to size for (a capture repeats a run)
+    // the counts of every level of an earlier solve of the same problems (a fixed walk
+    // sizes its grids with them; the grid-stride kernels are right with any grid)
+    std::vector<LevelCounters> hint;
+    int hint_blocks = 0;
+    size_t pool_need = 0;    // results: doubles of slots used, bytes of device memory,
+    size_t ws_need = 0;      //   levels walked, singleton waves sized for
+    int levels = 0;
+    int nwaves_used = 0;
+};
+
 // One pass of the pipeline for the problems described by md (host copy).  Every launch
 // is sized on the device: the kernels read their item counts from device memory and
 // walk them with grid-stride loops, so the host never waits for the device inside the
@@ -353,7 +389,11 @@ constexpr unsigned kGridCap = 8192;
 // which is what a captured graph replays.  Returns 0, 2 (device failure) or 3 (the
 // representation pool was too small: *pool_need holds the doubles needed).
 static int pipeline_once(std::vector<MatDesc>& md, std::vector<MatState>& ms, hipStream_t st,
-                         size_t pool_cap, size_t* pool_need, int fixed_levels, int* levels_used) {
+                         PipeOpts& o) {
+    const int fixed_levels = o.fixed_levels;
+    const size_t pool_cap = o.pool_cap;
+    GraphWs* const gw = o.gw;
+    const bool capturing = gw != nullptr;
     configure_pool_once();
     const mr3_tuning tune = tuning_snapshot();
     const int nmat = (int)md.size();
@@ -378,9 +418,10 @@ static int pipeline_once(std::vector<MatDesc>& md, std::vector<MatState>& ms, hi
     if (ntot > 0x7fffffff / 2) MR3_FAIL();
     const int N = (int)ntot;
     const bool fixed = fixed_levels >= 0;
-    Alloc A(st);
+    if (capturing && (!fixed || gw->nmat != nmat)) MR3_FAIL();
+    Alloc A = capturing ? Alloc(st, gw->base, gw->cap) : Alloc(st);
     CallEvents ce;
-    ce.on = g_prof.on.load() != 0;
+    ce.on = !capturing && g_prof.on.load() != 0;
     MatDesc* dmd = A.get<MatDesc>(nmat);
     MatState* dms = A.get<MatState>(nmat);
     int* dorder = A.get<int>(nmat);
@@ -410,8 +451,16 @@ static int pipeline_once(std::vector<MatDesc>& md, std::vector<MatState>& ms, hi
     int* dbl = A.get<int>(N);      // blocks of the root representations
     int* lcount = A.get<int>(2);   // their lengths
     if (!A.ok) MR3_FAIL();
-    MR3_CHECK(hipMemcpyAsync(dmd, md.data(), sizeof(MatDesc) * nmat, hipMemcpyHostToDevice, st));
-    MR3_CHECK(hipMemcpyAsync(dorder, order.data(), sizeof(int) * nmat, hipMemcpyHostToDevice, st));
+    const MatDesc* src_md = md.data();
+    const int* src_order = order.data();
+    if (capturing) {   // a graph copies from memory that lives as long as it does
+        memcpy(gw->hmd, md.data(), sizeof(MatDesc) * nmat);
+        memcpy(gw->horder, order.data(), sizeof(int) * nmat);
+        src_md = gw->hmd;
+        src_order = gw->horder;
+    }
+    MR3_CHECK(hipMemcpyAsync(dmd, src_md, sizeof(MatDesc) * nmat, hipMemcpyHostToDevice, st));
+    MR3_CHECK(hipMemcpyAsync(dorder, src_order, sizeof(int) * nmat, hipMemcpyHostToDevice, st));
 
     LAUNCH(k_prep, dim3(nmat), dim3(256), dmd, dms, D, E, E2, gers, Dcopy,
                        E2copy, isplit, blk, W, Werr, Wcoarse, iblock, Windex, smat);
@@ -485,7 +534,10 @@ static int pipeline_once(std::vector<MatDesc>& md, std::vector<MatState>& ms, hi
             double* scratch;
             double* tscratch; // S and P of the large-block tail kernel, per workgroup
         } ss[kSide] = {};
-        const int nside = side_count(tune.side_streams);
+        // a fixed walk (what a graph captures) takes at most two side streams: HIP runs
+        // a graph's independent branches on at most four streams at once, and the graph
+        // has the main stream and the small route's besides
+        const int nside = fixed ? std::min(2, side_count(tune.side_streams)) : side_count(tune.side_streams);
         for (int k = 0; k < nside; ++k) {
             ss[k].wtasks = A.get<WaveTask>(capT);
             ss[k].scnt = A.get<LevelCounters>(1);
@@ -509,22 +561,24 @@ static int pipeline_once(std::vector<MatDesc>& md, std::vector<MatState>& ms, hi
         const size_t budget = singleton_scratch_budget() / nside;
         int max_waves = (int)std::max<size_t>(1, std::min<size_t>(budget / per_wave,
                                                                   (size_t)tune.singleton_waves));
+        if (o.nwaves > 0) max_waves = o.nwaves;
         const int nwaves = std::min(max_waves, capT);
+        o.nwaves_used = nwaves;
         for (int k = 0; k < nside; ++k) ss[k].scratch = A.get<double>((size_t)nwaves * ck_per_lane * 64);
         // the large-block tail kernel: S and P of up to kTailBigWG singletons at a time
         const int tstride = 2 * (maxbl + 2);
         const int tail_rows = (maxbl > kTailN) ? maxbl : kTailN;   // blocks the tail kernels take
         for (int k = 0; k < nside; ++k)
             ss[k].tscratch = (maxbl > kTailN) ? A.get<double>((size_t)kTailBigWG * tstride) : nullptr;
-        hpin = pinned_levels_get();
+        hpin = capturing ? gw->hpin : pinned_levels_get();
         if (!A.ok || !hpin) {
-            if (hpin) pinned_levels_put(hpin);
+            if (hpin && !capturing) pinned_levels_put(hpin);
             MR3_FAIL();
         }
         struct PinFree {
             LevelCounters* p;
-            ~PinFree() { pinned_levels_put(p); }
-        } pin_free{hpin};
+            ~PinFree() { if (p) pinned_levels_put(p); }
+        } pin_free{capturing ? nullptr : hpin};
         // cleared with kernels, not memsets: a captured graph replays them
         for (int k = 0; k < kLB; ++k)
             LAUNCH(k_fill0, dim3(grid_for(N, 256, 1024)), dim3(256), bcnt2[k], (long long)N);
@@ -559,11 +613,13 @@ static int pipeline_once(std::vector<MatDesc>& md, std::vector<MatState>& ms, hi
         hipEvent_t evReady[kMaxLevels + 1] = {}, evCnt[kMaxLevels + 1] = {};
         struct SideSync {   // no buffer or event goes away while a side stream may use it
             const hipStream_t* s;
+            bool capturing;
             ~SideSync() {
+                if (capturing) return;   // (the capture joined them to the main stream)
                 for (int k = 0; k < kSide; ++k)
                     if (s[k]) (void)hipStreamSynchronize(s[k]);
             }
-        } side_sync{s2p};
+        } side_sync{s2p, capturing};
         // level L's counts, copied to pinned memory once its lists are complete
         auto publish = [&](int L, const LevelCounters* src) -> int {
             evReady[L] = evs.get();
@@ -588,8 +644,18 @@ static int pipeline_once(std::vector<MatDesc>& md, std::vector<MatState>& ms, hi
             }
             return 0;
         };
+        // the counts of level L when the host has them: this solve's, or in a fixed walk
+        // those of the earlier solve the caller passed
+        auto counts = [&](int L) -> const LevelCounters* {
+            if (!fixed) return known > L ? &hpin[L] : nullptr;
+            return L < (int)o.hint.size() ? &o.hint[L] : nullptr;
+        };
         // item bound of level L: the eigen entries of the latest level seen at or before it
         auto bound = [&](int L) -> long long {
+            if (fixed) {
+                const LevelCounters* c = counts(L);
+                return c ? std::max(1, c->nEig + c->nS) : N;
+            }
             for (int j = std::min(L, known - 1); j >= 0; --j) return std::max(1, hpin[j].nEig + hpin[j].nS);
             return N;
         };
@@ -598,7 +664,7 @@ static int pipeline_once(std::vector<MatDesc>& md, std::vector<MatState>& ms, hi
         // the cluster work of level L and the classification into level L + 1 (main stream)
         auto main_level = [&](int L, double* arena, size_t arena_cap) -> int {
             const int cur = L & 1, nxt = cur ^ 1;
-            const long long b = fixed ? N : bound(L);
+            const long long b = bound(L);
             const long long bc = std::min<long long>(capC, b / 2 + 1);
             LevelCounters* lcur = lc + cur;
             LevelCounters* lnxt = lc + nxt;
@@ -657,9 +723,9 @@ static int pipeline_once(std::vector<MatDesc>& md, std::vector<MatState>& ms, hi
             // other slots
             MR3_CHECK(hipStreamWaitEvent(s2, evReady[L], 0));
             // grids: from the level's counts when the host has them, else from bounds
-            const bool kn = !fixed && known > L;
-            const long long nS = kn ? hpin[L].nS : N;
-            const long long nb = kn ? hblocks.nS : N;
+            const LevelCounters* c = counts(L);
+            const long long nS = c ? c->nS : N;
+            const long long nb = !fixed ? (known > 0 ? hblocks.nS : N) : (o.hint_blocks > 0 ? o.hint_blocks : N);
             const int g1 = (int)std::min<long long>(capT, nS / 64 + nb + 1);   // wave tasks
             const unsigned gT = grid_for(g1, 1, kGridCap);
             const unsigned gw = (unsigned)std::min(nwaves, g1);
@@ -718,11 +784,14 @@ static int pipeline_once(std::vector<MatDesc>& md, std::vector<MatState>& ms, hi
         };
         int mainL = 0;
         if (fixed) {
-            for (int L = 0; L < std::min(fixed_levels, kMaxLevels); ++L) {
-                if (main_level(L, pool, pool_cap)) MR3_FAIL();
+            // the clusters of levels 0 .. fixed_levels - 1 and the singletons of levels
+            // 0 .. fixed_levels (the last level has singletons only)
+            const int nl = std::min(fixed_levels, kMaxLevels);
+            for (int L = 0; L <= nl; ++L) {
+                if (L < nl && main_level(L, pool, pool_cap)) MR3_FAIL();
                 if (side_level(L)) MR3_FAIL();
             }
-            mainL = std::min(fixed_levels, kMaxLevels);
+            mainL = nl;
         } else {
             // level L's counts arrive (an event, not a stream wait: the side streams and
             // the copy engines go on) before its work is enqueued: they size the slots
@@ -750,6 +819,14 @@ static int pipeline_once(std::vector<MatDesc>& md, std::vector<MatState>& ms, hi
     LAUNCH(k_finalize, dim3(nmat), dim3(256), dmd, dms, W, Zidx, blk, iblock,
                        Windex, tcount, isplit, Wcoarse);
     MR3_CHECK(hipGetLastError());
+    o.levels = levels_done;
+    o.ws_need = A.need;
+    if (capturing) {   // results into the graph's pinned buffers; the caller waits
+        MR3_CHECK(hipMemcpyAsync(gw->hms, dms, sizeof(MatState) * nmat, hipMemcpyDeviceToHost, st));
+        if (pool_flag)
+            MR3_CHECK(hipMemcpyAsync(gw->hflag, pool_flag, sizeof(int), hipMemcpyDeviceToHost, st));
+        return 0;
+    }
     ms.resize(nmat);
     MR3_CHECK(hipMemcpyAsync(ms.data(), dms, sizeof(MatState) * nmat, hipMemcpyDeviceToHost, st));
     int hflag = 0;
@@ -760,9 +837,14 @@ static int pipeline_once(std::vector<MatDesc>& md, std::vector<MatState>& ms, hi
     }
     MR3_CHECK(hipStreamSynchronize(st));
     ce.collect();
-    if (levels_used) *levels_used = levels_done;
-    if (pool_need) *pool_need = fixed ? (size_t)hbump : pool_used;
+    o.pool_need = fixed ? (size_t)hbump : pool_used;
+    if (hpin && !fixed) {
+        o.hint.assign(hpin, hpin + std::min(levels_done, kMaxLevels) + 1);
+        o.hint_blocks = hpin[kMaxLevelsPinned - 1].nS;
+    }
     if (hflag) return 3;
+    if (fixed && hpin && levels_done <= kMaxLevels && hpin[levels_done].nC > 0)
+        return 4;   // the tree is deeper than the levels walked
     if (hpin)
         for (int j = 0; j <= levels_done && j <= kMaxLevels; ++j)
             if (hpin[j].err) MR3_FAIL();
@@ -772,8 +854,9 @@ static int pipeline_once(std::vector<MatDesc>& md, std::vector<MatState>& ms, hi
 // Runs the whole pipeline for the problems described by md (host copy); the device
 // outputs land in the caller's arrays named in md; states are copied back to ms.
 int run_pipeline(std::vector<MatDesc>& md, std::vector<MatState>& ms, hipStream_t st) {
-    const int rc = pipeline_once(md, ms, st, 0, nullptr, -1, nullptr);
-    return rc == 3 ? 2 : rc;   // (the eager walk sizes every level's slots: no overflow)
+    PipeOpts o;
+    const int rc = pipeline_once(md, ms, st, o);
+    return rc == 0 ? 0 : 2;   // (the eager walk sizes every level's slots: no overflow)
 }
 
 // ---------------------------------------------------------------------------------
@@ -883,6 +966,189 @@ int small_end(SmallJob& job, std::vector<MatState>& ms, hipStream_t st) {
     J.ready = nullptr;
     return 0;
 }
+
+// ---------------------------------------------------------------------------------
+// Captured solves (mr3_graph_*): one hipGraph replays the whole batch, the small route
+// and the pipeline with a fixed number of tree levels, in memory the graph owns.
+// ---------------------------------------------------------------------------------
+struct SmallGraph {   // the small route's part of a captured batch
+    int nmat = 0, nmax = 1;
+    char* buf = nullptr;            // descriptors, states, list, offsets, slices (device)
+    MatDesc* hmd = nullptr;         // pinned copy sources and result destination
+    int* hlist = nullptr;
+    long long* hoff = nullptr;
+    MatState* hms = nullptr;
+    size_t bmd = 0, bms = 0, bl = 0, bo = 0;
+};
+
+static void small_graph_free(SmallGraph& g) {
+    if (g.buf) (void)hipFree(g.buf);
+    if (g.hmd) (void)hipHostFree(g.hmd);
+    if (g.hlist) (void)hipHostFree(g.hlist);
+    if (g.hoff) (void)hipHostFree(g.hoff);
+    if (g.hms) (void)hipHostFree(g.hms);
+    g = SmallGraph();
+}
+
+static int small_graph_alloc(SmallGraph& g, const std::vector<MatDesc>& md) {
+    const int nm = (int)md.size();
+    g.nmat = nm;
+    if (nm == 0) return 0;
+    std::vector<int> list(nm);
+    for (int i = 0; i < nm; ++i) list[i] = i;
+    std::stable_sort(list.begin(), list.end(), [&](int a, int b) { return md[a].n > md[b].n; });
+    auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    g.bmd = al(sizeof(MatDesc) * nm); g.bms = al(sizeof(MatState) * nm);
+    g.bl = al(sizeof(int) * nm); g.bo = al(sizeof(long long) * nm);
+    MR3_CHECK(hipHostMalloc((void**)&g.hmd, sizeof(MatDesc) * nm, hipHostMallocDefault));
+    MR3_CHECK(hipHostMalloc((void**)&g.hlist, sizeof(int) * nm, hipHostMallocDefault));
+    MR3_CHECK(hipHostMalloc((void**)&g.hoff, sizeof(long long) * nm, hipHostMallocDefault));
+    MR3_CHECK(hipHostMalloc((void**)&g.hms, sizeof(MatState) * nm, hipHostMallocDefault));
+    long long tot = 0;
+    for (int j = 0; j < nm; ++j) {
+        g.hlist[j] = list[j];
+        g.hoff[j] = tot;
+        tot += small_slice_bytes(md[list[j]].n);
+        g.nmax = std::max(g.nmax, md[list[j]].n);
+    }
+    memcpy(g.hmd, md.data(), sizeof(MatDesc) * nm);
+    MR3_CHECK(hipMalloc((void**)&g.buf, g.bmd + g.bms + g.bl + g.bo + (size_t)tot));
+    return 0;
+}
+
+// Enqueues the small route of a captured batch: a fork from st to the small-route
+// stream, copies from and to pinned memory, one launch; *join is recorded at its end
+// (st waits for it once the pipeline is enqueued, so that both run at the same time).
+static int small_graph_enqueue(SmallGraph& g, hipStream_t st, hipEvent_t* join_out) {
+    *join_out = nullptr;
+    if (g.nmat == 0) return 0;
+    hipStream_t s = small_stream();
+    if (!s) return 2;
+    const int nm = g.nmat;
+    MatDesc* dmd = (MatDesc*)g.buf;
+    MatState* dms = (MatState*)(g.buf + g.bmd);
+    int* dlist = (int*)(g.buf + g.bmd + g.bms);
+    long long* doff = (long long*)(g.buf + g.bmd + g.bms + g.bl);
+    char* slices = g.buf + g.bmd + g.bms + g.bl + g.bo;
+    hipEvent_t fork = event_pool().get(), join = event_pool().get();
+    MR3_CHECK(hipEventRecord(fork, st));
+    MR3_CHECK(hipStreamWaitEvent(s, fork, 0));
+    MR3_CHECK(hipMemcpyAsync(dmd, g.hmd, sizeof(MatDesc) * nm, hipMemcpyHostToDevice, s));
+    MR3_CHECK(hipMemcpyAsync(dlist, g.hlist, sizeof(int) * nm, hipMemcpyHostToDevice, s));
+    MR3_CHECK(hipMemcpyAsync(doff, g.hoff, sizeof(long long) * nm, hipMemcpyHostToDevice, s));
+    const size_t lds = sizeof(double) * (size_t)mr3sm::lds_doubles(g.nmax);
+    hipLaunchKernelGGL(k_small, dim3(nm), dim3(kSmallNT), lds, s, dmd, dms, dlist, doff, slices);
+    MR3_CHECK(hipGetLastError());
+    MR3_CHECK(hipMemcpyAsync(g.hms, dms, sizeof(MatState) * nm, hipMemcpyDeviceToHost, s));
+    MR3_CHECK(hipEventRecord(join, s));
+    event_pool().put(fork);
+    *join_out = join;
+    return 0;
+}
+
+struct BatchGraph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int nprob = 0;
+    std::vector<int> idxS, idxL;    // problems of the small route and of the pipeline
+    SmallGraph sg;
+    GraphWs gw;
+    int levels = 0;
+    ~BatchGraph() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        small_graph_free(sg);
+        if (gw.base) (void)hipFree(gw.base);
+        if (gw.hmd) (void)hipHostFree(gw.hmd);
+        if (gw.horder) (void)hipHostFree(gw.horder);
+        if (gw.hms) (void)hipHostFree(gw.hms);
+        if (gw.hpin) (void)hipHostFree(gw.hpin);
+        if (gw.hflag) (void)hipHostFree(gw.hflag);
+    }
+};
+
+// Captures the batch (mdS: the small route's problems, mdL: the pipeline's) as one
+// graph.  The pipeline's problems are solved once as usual to learn the depth of their
+// tree and the slots it needs, then once more with exactly that many levels and one
+// pool (this also sizes the device memory), then captured.
+int graph_create(std::vector<MatDesc>& mdS, std::vector<MatDesc>& mdL, hipStream_t st,
+                 BatchGraph* g) {
+    configure_pool_once();
+    if (small_graph_alloc(g->sg, mdS)) return 2;
+    PipeOpts ofix;
+    if (!mdL.empty()) {
+        std::vector<MatState> ms;
+        PipeOpts oe;
+        if (pipeline_once(mdL, ms, st, oe) != 0) return 2;
+        ofix.fixed_levels = oe.levels;
+        ofix.pool_cap = oe.pool_need + 1024;
+        ofix.hint = oe.hint;
+        ofix.hint_blocks = oe.hint_blocks;
+        if (pipeline_once(mdL, ms, st, ofix) != 0) return 2;
+        g->levels = oe.levels;
+        GraphWs& w = g->gw;
+        const int nm = (int)mdL.size();
+        w.nmat = nm;
+        w.cap = ofix.ws_need + ((size_t)1 << 20);
+        MR3_CHECK(hipMalloc((void**)&w.base, w.cap));
+        MR3_CHECK(hipHostMalloc((void**)&w.hmd, sizeof(MatDesc) * nm, hipHostMallocDefault));
+        MR3_CHECK(hipHostMalloc((void**)&w.horder, sizeof(int) * nm, hipHostMallocDefault));
+        MR3_CHECK(hipHostMalloc((void**)&w.hms, sizeof(MatState) * nm, hipHostMallocDefault));
+        MR3_CHECK(hipHostMalloc((void**)&w.hpin, sizeof(LevelCounters) * kMaxLevelsPinned,
+                                hipHostMallocDefault));
+        MR3_CHECK(hipHostMalloc((void**)&w.hflag, sizeof(int), hipHostMallocDefault));
+        *w.hflag = 0;
+    }
+    MR3_CHECK(hipStreamSynchronize(st));
+    MR3_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
+    hipEvent_t join = nullptr;
+    int rc = small_graph_enqueue(g->sg, st, &join);
+    if (rc == 0 && !mdL.empty()) {
+        std::vector<MatState> ms;
+        PipeOpts oc;
+        oc.fixed_levels = ofix.fixed_levels;
+        oc.pool_cap = ofix.pool_cap;
+        oc.nwaves = ofix.nwaves_used;
+        oc.hint = ofix.hint;
+        oc.hint_blocks = ofix.hint_blocks;
+        oc.gw = &g->gw;
+        rc = pipeline_once(mdL, ms, st, oc);
+    }
+    if (join) {
+        if (rc == 0 && hipStreamWaitEvent(st, join, 0) != hipSuccess) rc = 2;
+        event_pool().put(join);
+    }
+    hipGraph_t graph = nullptr;
+    const hipError_t ec = hipStreamEndCapture(st, &graph);
+    if (rc != 0 || ec != hipSuccess || !graph) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return 2;
+    }
+    g->graph = graph;
+    MR3_CHECK(hipGraphInstantiateWithFlags(&g->exec, graph, 0));
+    return 0;
+}
+
+// Replays a captured batch and reads its results: the states of the small route's and
+// the pipeline's problems; returns 4 if the pipeline's tree needed more levels or slots
+// than the capture had (its results are then not valid).
+int graph_launch(BatchGraph* g, hipStream_t st, std::vector<MatState>& msS,
+                 std::vector<MatState>& msL) {
+    MR3_CHECK(hipGraphLaunch(g->exec, st));
+    MR3_CHECK(hipStreamSynchronize(st));
+    msS.assign(g->sg.hms, g->sg.hms + g->sg.nmat);
+    msL.assign(g->gw.hms, g->gw.hms + g->gw.nmat);
+    if (g->gw.nmat > 0) {
+        if (*g->gw.hflag) return 4;
+        for (int j = 0; j <= g->levels && j <= kMaxLevels; ++j)
+            if (g->gw.hpin[j].err) return 2;
+        if (g->levels <= kMaxLevels && g->gw.hpin[g->levels].nC > 0) return 4;
+    }
+    return 0;
+}
+
+BatchGraph* graph_new() { return new BatchGraph(); }
+void graph_delete(BatchGraph* g) { delete g; }
 
 void profile_enable(int on) {
     const int mode = (on == 1 || on == 2) ? on : 0;
