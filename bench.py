@@ -31,11 +31,13 @@ sys.path.insert(0, os.path.join(REPO, "tests"))
 # multiplies, subtract), dlaneg 4, Sturm step on T 3, dlar1v row 16, dlarrf element 5.
 FLOPS_PER_STEP = {"coarse": 3, "refine": 4, "endref": 4, "r1v": 16, "r1v_bisect": 4,
                   "rrf": 5, "rrj": 3, "refine_cl": 4,
-                  "sm_dqds": 5, "sm_neg": 4, "sm_sturm": 3, "sm_r1v": 16, "sm_rrf": 5}
+                  "sm_dqds": 5, "sm_neg": 4, "sm_sturm": 3, "sm_r1v": 16, "sm_rrf": 5,
+                  "tail_r1v": 16, "tail_bisect": 4, "tailb_r1v": 16, "tailb_bisect": 4}
 KERNEL_STEPS = {"k_coarse": ["coarse"], "k_refine_root": ["refine"], "k_endref": ["endref"],
                 "k_s_iter": ["r1v"], "k_s_bisect": ["r1v_bisect"], "k_dlarrf": ["rrf"], "k_rrj": ["rrj"],
                 "k_refine_cl": ["refine_cl"],
-                "k_small": ["sm_dqds", "sm_neg", "sm_sturm", "sm_r1v", "sm_rrf"]}
+                "k_small": ["sm_dqds", "sm_neg", "sm_sturm", "sm_r1v", "sm_rrf"],
+                "k_s_tail<false>": ["tail_r1v", "tail_bisect"], "k_s_tail<true>": ["tailb_r1v", "tailb_bisect"]}
 FP64_PEAK_TFLOPS = 78.6
 HBM_PEAK_GBS = 8000.0
 # The divide-bound ceiling: the IEEE FP64 division sequence the compiler emits is ten
@@ -50,7 +52,8 @@ DIVIDE_STEPS_PER_S = 256 * 4 * 2.4e9 * 64 / 52.0
 # stationary and the progressive transform; the vector solve multiplies)
 DIVIDES_PER_STEP = {"coarse": 1, "refine": 1, "endref": 1, "r1v": 2, "r1v_bisect": 1,
                     "rrf": 1, "rrj": 1, "refine_cl": 1,
-                    "sm_dqds": 1, "sm_neg": 1, "sm_sturm": 1, "sm_r1v": 2, "sm_rrf": 1}
+                    "sm_dqds": 1, "sm_neg": 1, "sm_sturm": 1, "sm_r1v": 2, "sm_rrf": 1,
+                    "tail_r1v": 2, "tail_bisect": 1, "tailb_r1v": 2, "tailb_bisect": 1}
 
 
 def log(*a):
@@ -398,7 +401,7 @@ def subset_main(args, torch, M, dev):
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": step_s / args.steps * 1e3,
+        "ms_per_step": dt / args.steps * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,

@@ -1168,7 +1168,8 @@ int profile_report(char* buf, int len) {
                                             "r1v_bisect", "rrf", "rrj", "refine_cl",
                                             "r1v_wave", "bisect_wave", "task_ticks",
                                             "tasks", "t_bisect", "t_r1v", "t_solve", "rrf_rows",
-                                            "sm_dqds", "sm_neg", "sm_sturm", "sm_r1v", "sm_rrf"};
+                                            "sm_dqds", "sm_neg", "sm_sturm", "sm_r1v", "sm_rrf",
+                                            "tail_r1v", "tail_bisect", "tailb_r1v", "tailb_bisect"};
     std::string js = "{\"kernels\": {";
     std::lock_guard<std::mutex> lk(g_prof.mu);
     bool first = true;

@@ -48,7 +48,11 @@ enum StepKind : int {
     kStepSmSturm = 18,   //   Sturm steps on T, 3 flops
     kStepSmR1v = 19,     //   dlar1v rows, 16 flops
     kStepSmRrf = 20,     //   dlarrf elements, 5 flops
-    kStepKinds = 21
+    kStepTailR1v = 21,   // dlar1v elements of the tail kernel (S and P in LDS), 16 flops
+    kStepTailBis = 22,   //   its bisection-fallback dlaneg steps, 4 flops
+    kStepTailBR1v = 23,  // the same for the large-block tail kernel (S and P in device memory)
+    kStepTailBBis = 24,
+    kStepKinds = 25
 };
 __device__ unsigned long long g_steps[kStepKinds];
 __constant__ int g_count_steps;
@@ -3871,7 +3875,7 @@ __global__ __launch_bounds__(128) void k_s_tail(const int* __restrict__ DI,
                     int rounds = 0;
                     const long long sw = s_bisect_state(q, t, S, i_local, SW, lane, 0, true, &rounds, true);
                     if (lane == 0) {
-                        add_steps(kStepR1vBis, (unsigned long long)n * (unsigned long long)sw);
+                        add_steps(BIG ? kStepTailBBis : kStepTailBis, (unsigned long long)n * (unsigned long long)sw);
                         sq = q;
                     }
                 }
@@ -4017,7 +4021,7 @@ __global__ __launch_bounds__(128) void k_s_tail(const int* __restrict__ DI,
                 o.isupp1 = is1;
                 s_update(q, o, s2, lambda, usedBS, usedRQ, needBS, wantnc, state, i_local, tol, gap, dsc,
                          col, bbeg, t, n, W, Wsh, Werr, ST, idx);
-                add_steps(kStepR1v, (unsigned long long)n);
+                add_steps(BIG ? kStepTailBR1v : kStepTailR1v, (unsigned long long)n);
                 sq = q;
             }
             __syncthreads();

@@ -75,7 +75,18 @@ def test_small_route_mixed_workload_team64(small_cpu, oracle):
         _check(small_cpu, oracle, D, E, f"{i} {kind} n={len(D)}", np.random.default_rng(i), team=64)
 
 
-@pytest.mark.parametrize("team", [1, 64])
+def test_small_route_mixed_workload_team256(small_cpu, oracle):
+    """The largest small matrices of the mixed workload (n >= 400, every 40th) run by a
+    team of 256 threads: the work split of any team size gives the same bits."""
+    from mr3smp_amd import workloads as WL
+    mats = WL.batched_mixed(4096)
+    idx = [i for i, (D, _, _) in enumerate(mats) if 400 <= len(D) < 500][::40]
+    for i in idx:
+        D, E, kind = mats[i]
+        _check(small_cpu, oracle, D, E, f"{i} {kind} n={len(D)}", np.random.default_rng(i), team=256)
+
+
+@pytest.mark.parametrize("team", [1, 64, 256])
 @pytest.mark.parametrize("name", ["n1", "n2", "n3", "diag", "diag8", "diag16", "zerosD", "split",
                                   "tiny", "huge", "repeated", "relacc", "toeplitz499", "glued499"])
 def test_small_route_edge_cases(name, team, small_cpu, oracle):
