@@ -776,7 +776,7 @@ static int pipeline_once(std::vector<MatDesc>& md, std::vector<MatState>& ms, hi
                 psi = q.sit[b];
                 pfirst = 0;
             }
-            LAUNCH_ON(s2, k_s_normalize, dim3(gT), dim3(64), q.wtasks, q.scnt, Sl, dmd, Zidx, q.ST);
+            LAUNCH_ON(s2, k_s_normalize, dim3(gT), dim3(64), q.wtasks, q.scnt, Sl, dmd, dms, Zidx, q.ST);
             MR3_CHECK(hipGetLastError());
             MR3_CHECK(hipEventRecord(evS[lb], s2));
             have_evS[lb] = true;

@@ -1199,15 +1199,22 @@ __device__ __forceinline__ bool zkey_before(double la, int ia, int ba, double lb
 }
 
 __global__ __launch_bounds__(256) void k_zindex(const MatDesc* __restrict__ md,
-                                                const MatState* __restrict__ ms,
+                                                MatState* __restrict__ ms,
                                                 const BlockRec* __restrict__ blk,
                                                 const double* __restrict__ W,
                                                 const int* __restrict__ iblock,
                                                 const int* __restrict__ Windex, int* Zidx) {
     const int k = blockIdx.x;
-    const MatState& S = ms[k];
+    MatState& S = ms[k];
     if (S.done || S.status != kOk || md[k].jobz != kV) return;
     const int off = md[k].off, m = S.m, nsplit = S.nsplit;
+    if (m > md[k].mcols) {
+        // more eigenvectors than the caller's Z holds: the problem fails (info 2) here,
+        // before the tree walk gives it any task, so no column of Z or Zsupp is written
+        __syncthreads();   // every thread has read the status before it changes
+        if (threadIdx.x == 0) { S.status = kTooFew; S.done = 1; }
+        return;
+    }
     for (int s = threadIdx.x; s < m; s += blockDim.x) {
         if (nsplit == 1) { Zidx[off + s] = s; continue; }
         const int b = iblock[off + s] - 1;
@@ -3175,7 +3182,7 @@ __device__ __forceinline__ void s_update(SState& q, const R1v& o, bool s2, doubl
 
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_s_iter(
     const WaveTask* __restrict__ tasks, LevelCounters* cnt, int first, const int* __restrict__ SI,
-    const STask* __restrict__ T, const MatDesc* __restrict__ md, const MatState* __restrict__ ms,
+    const STask* __restrict__ T, const MatDesc* __restrict__ md, MatState* __restrict__ ms,
     const int* __restrict__ Windex, const int* __restrict__ Zidx, double* W, double* Wsh,
     double* Werr, const double* __restrict__ Wgap, SState* ST, double* scratch, int cap,
     int tailS, int tailN) {
@@ -3215,9 +3222,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
         if (leave_small && n <= tailN) continue;
         const int bbeg = __builtin_amdgcn_readfirstlane(t.bbeg);
         const MatDesc& dsc = md[t.mat];
-        int col = Zidx[t.slot];
-        // never write past the caller's Z; such a problem is reported failed (k_finalize)
-        if (col >= dsc.mcols) col = dsc.mcols - 1;
+        const int col = Zidx[t.slot];
+        if (__any(col >= dsc.mcols)) {
+            // no column past the caller's Z: k_zindex fails such a matrix before its
+            // tasks exist, and should one appear all the same, the wave writes nothing
+            // and fails every matrix it holds
+            ms[t.mat].status = kTooFew;
+            continue;
+        }
         ColOut co;
         co.zcol = zcol;
         co.wr0 = wr0;
@@ -3810,7 +3822,7 @@ __global__ __launch_bounds__(128) void k_s_tail(const int* __restrict__ DI,
                                                 LevelCounters* cnt,
                                                 const STask* __restrict__ T,
                                                 const MatDesc* __restrict__ md,
-                                                const MatState* __restrict__ ms,
+                                                MatState* __restrict__ ms,
                                                 const int* __restrict__ Windex,
                                                 const int* __restrict__ Zidx, double* W,
                                                 double* Wsh, double* Werr,
@@ -3850,8 +3862,12 @@ __global__ __launch_bounds__(128) void k_s_tail(const int* __restrict__ DI,
         if (n < 2) continue;   // uniform
         const MatDesc& dsc = md[t.mat];
         const MatState& S = ms[t.mat];
-        int col = Zidx[t.slot];
-        if (col >= dsc.mcols) col = dsc.mcols - 1;
+        const int col = Zidx[t.slot];
+        if (col >= dsc.mcols) {
+            // never past the caller's Z (k_zindex rules it out): write nothing, fail
+            if (tid == 0) ms[t.mat].status = kTooFew;
+            continue;
+        }
         const int bbeg = t.bbeg;
         const int i = t.slot - dsc.off;
         const int bl_begin = bbeg, bl_end = bbeg + n - 1;
@@ -4103,6 +4119,7 @@ __device__ __forceinline__ void k_s_normalize_one(int bx, const WaveTask* __rest
                                                     const LevelCounters* __restrict__ cnt,
                                                     const STask* __restrict__ T,
                                                     const MatDesc* __restrict__ md,
+                                                    MatState* __restrict__ ms,
                                                     const int* __restrict__ Zidx,
                                                     const SState* __restrict__ ST) {
     __shared__ double* zc_s[SW];
@@ -4115,8 +4132,12 @@ __device__ __forceinline__ void k_s_normalize_one(int bx, const WaveTask* __rest
     const int idx = wt.start + min(lane, cntw - 1);
     const STask t = T[idx];
     const MatDesc& dsc = md[t.mat];
-    int col = Zidx[t.slot];
-    if (col >= dsc.mcols) col = dsc.mcols - 1;
+    const int col = Zidx[t.slot];
+    if (__any(col >= dsc.mcols)) {
+        // never past the caller's Z (k_zindex rules it out): write nothing, fail
+        ms[t.mat].status = kTooFew;
+        return;
+    }
     const SState q = ST[idx];
     zc_s[lane] = dsc.Z + (size_t)col * dsc.ldz;
     lo_s[lane] = t.bbeg + q.is0 - 1;
@@ -4149,12 +4170,13 @@ __global__ __launch_bounds__(64) void k_s_normalize(const WaveTask* __restrict__
                                                     const LevelCounters* __restrict__ cnt,
                                                     const STask* __restrict__ T,
                                                     const MatDesc* __restrict__ md,
+                                                    MatState* __restrict__ ms,
                                                     const int* __restrict__ Zidx,
                                                     const SState* __restrict__ ST) {
     const int nT_ = cnt->nT;
     const int nblk_ = nT_;
     for (int bx = blockIdx.x; bx < nblk_; bx += gridDim.x) {
-        k_s_normalize_one(bx, tasks, cnt, T, md, Zidx, ST);
+        k_s_normalize_one(bx, tasks, cnt, T, md, ms, Zidx, ST);
         __syncthreads();
     }
 }
@@ -4183,9 +4205,8 @@ __global__ __launch_bounds__(256) void k_finalize(const MatDesc* __restrict__ md
     const double inv = 1.0 / S.scale;
     const bool wantz = dsc.jobz == kV;
     if (wantz && m > dsc.mcols) {
-        // more eigenvectors than the caller's Z holds: the problem fails (info 2) and
-        // nothing is moved, so that no access reaches past Z or Zsupp (the vector
-        // phase wrote the extra vectors into the last column, inside the buffer)
+        // more eigenvectors than the caller's Z holds (k_zindex has failed such a
+        // problem already): nothing is moved, so that no access reaches past Z or Zsupp
         if (tid == 0) { S.status = kTooFew; S.done = 1; }
         return;
     }

@@ -586,3 +586,47 @@ def test_gpu_server_reports_each_requests_own_info(M):
         assert srv.L.mr3_server_wait(srv.h, 10 ** 9) == -1      # unknown
     finally:
         srv.close()
+
+
+def test_gpu_too_few_columns_fail_without_writing(M):
+    """A problem whose caller's Z has fewer columns than it has eigenvalues (range A or
+    an index range wider than mcols) fails with info 2 and writes no column of Z or
+    Zsupp, on both routes (n < 500 and the task-parallel one); the other problems of
+    the batch are solved as if alone."""
+    import torch
+    from mr3smp_amd import workloads as WL
+    dev = torch.device("cuda:0")
+    rng = np.random.default_rng(91)
+    probs, bad = [], []
+    for n in (37, 300, 640, 1260):
+        D, E = (WL.glued_wilkinson if n % 21 == 0 else WL.uniform)(n, rng)
+        for kw, fails in ((dict(range="A", mcols=n // 2), True),
+                          (dict(range="I", il=3, iu=n // 3, mcols=n // 3 - 4), True),
+                          (dict(range="I", il=3, iu=n // 3, mcols=n // 3 - 2), False),
+                          (dict(range="A"), False)):
+            mc = kw.get("mcols", n)
+            # one spare column past mcols and spare Zsupp entries, all sentinels
+            Z = torch.full((mc + 1, n), 7.0, dtype=torch.float64, device=dev)
+            Zs = torch.full((2 * mc + 2,), -7, dtype=torch.int32, device=dev)
+            probs.append(dict(D=torch.tensor(D, device=dev), E=torch.tensor(E, device=dev),
+                              jobz="V", Z=Z, Zsupp=Zs, **kw))
+            bad.append(fails)
+    print(f"{len(probs)} problems, {sum(bad)} with too few columns", flush=True)
+    singles = [{k: v for k, v in p.items() if k not in ("Z", "Zsupp")} for p in probs]
+    rc, arr, keep = M.solve_batch(probs)
+    torch.cuda.synchronize()
+    assert rc == 2
+    for i, fails in enumerate(bad):
+        if fails:
+            assert arr[i].info == 2, f"problem {i}: info {arr[i].info}"
+            assert bool((keep[i]["Z"] == 7.0).all()), f"problem {i}: Z written"
+            assert bool((keep[i]["Zsupp"] == -7).all()), f"problem {i}: Zsupp written"
+            continue
+        assert arr[i].info == 0, f"problem {i}: info {arr[i].info}"
+        m = arr[i].m
+        assert bool((keep[i]["Z"][m:] == 7.0).all()), f"problem {i}: Z written past m"
+        rc1, a1, k1 = M.solve_batch([dict(singles[i])])
+        assert rc1 == 0 and a1[0].m == m
+        assert torch.equal(keep[i]["W"][:m], k1[0]["W"][:m]), f"W differs for problem {i}"
+        assert torch.equal(keep[i]["Z"][:m], k1[0]["Z"][:m]), f"Z differs for problem {i}"
+        assert torch.equal(keep[i]["Zsupp"][:2 * m], k1[0]["Zsupp"][:2 * m])
