@@ -560,6 +560,152 @@ SM_HD inline void dlasq4(int i0, int n0, const double* z, int pp, int n0in, doub
     *tau = s;
 }
 
+#if defined(__HIP_DEVICE_COMPILE__) && !defined(SM_NO_DQDS_ASM)
+// The steps of one dqds transform (the loop of odsq5.f with IEEE = .TRUE.) on one lane,
+// with the qd array in LDS.  Step k, with d the value left by step k - 1:
+//   sj = d + za;  Z(sj cell) = sj;  t = zb / sj;  d = d * t - tau;  dmin = fmn(dmin, d);
+//   zj = za * t;  Z(zj cell) = zj;  emin = fmn(zj, emin)
+// The same IEEE operations in the same order as the C++ loop (the division is the
+// correctly rounded scale / reciprocal / Newton / fmas / fixup sequence), so the
+// results are the same bits.  Only the issue order differs: the chain d -> sj -> t ->
+// d issues back to back, and each step's stores, its emin update and the dmin update of
+// the value before it go into the next step's division, where they cost no time.  The
+// loads of the next step are issued two steps' worth of work before their use.  Two
+// steps per turn of the loop, with their own registers, so that nothing is copied.
+//   PP      0 or 1: which half of the qd array the transform reads (odsq5.f pp)
+//   base    LDS byte address of the first step's sj cell (Z(4 i0 - 2 - pp))
+//   count   steps (>= 1)
+// The next step's za and zb sit 32 bytes past the current step's; the sj and zj cells
+// of a step are at base and base + 16.  Runs on one lane (its callers run it from
+// lane 0 of a wave alone), with the loop counter in an SGPR.
+#define SMQ_D "v[16:17]"
+#define SMQ_DM "v[18:19]"
+#define SMQ_EM "v[20:21]"
+#define SMQ_DD "v[38:39]"
+#define SMQ_R "v[14:15]"
+#define SMQ_NN "v[12:13]"
+#define SMQ_E "v[10:11]"
+#define SMQ_Q "v[8:9]"
+#define SMQ_T "v[6:7]"
+#define SMQ_DN "v[4:5]"
+#define SMQ_TAU "v[2:3]"
+// step core on the current registers X and the prefetch into Y; FULL adds the deferred
+// work of the previous step (its registers are Y's, written at BY)
+#define SMQ_STEP(ZAX, ZBX, SJX, ZJX, BX, ZAY, ZBY, SJY, ZJY, BY, NA, NB, FULL_W, FULL_E) \
+    "s_waitcnt lgkmcnt(0)\n"                                                           \
+    "v_add_f64 " SJX ", " SMQ_D ", " ZAX "\n"                                          \
+    "v_cmp_lt_f64_e64 %[m1], " SMQ_D ", " SMQ_DM "\n"                                  \
+    "v_div_scale_f64 " SMQ_DD ", %[m0], " SJX ", " SJX ", " ZBX "\n"                    \
+    "ds_read_b64 " ZAY ", " BX " offset:" NA "\n"                                      \
+    "ds_read_b64 " ZBY ", " BX " offset:" NB "\n"                                      \
+    "v_rcp_f64 " SMQ_R ", " SMQ_DD "\n"                                                \
+    "v_div_scale_f64 " SMQ_NN ", vcc, " ZBX ", " SJX ", " ZBX "\n"                      \
+    FULL_W                                                                             \
+    "v_fma_f64 " SMQ_E ", -" SMQ_DD ", " SMQ_R ", 1.0\n"                               \
+    "v_cndmask_b32_e64 v18, v18, v16, %[m1]\n"                                          \
+    "v_cndmask_b32_e64 v19, v19, v17, %[m1]\n"                                          \
+    "v_fma_f64 " SMQ_R ", " SMQ_R ", " SMQ_E ", " SMQ_R "\n"                           \
+    FULL_E                                                                             \
+    "v_fma_f64 " SMQ_E ", -" SMQ_DD ", " SMQ_R ", 1.0\n"                               \
+    "v_fma_f64 " SMQ_R ", " SMQ_R ", " SMQ_E ", " SMQ_R "\n"                           \
+    "v_mul_f64 " SMQ_Q ", " SMQ_NN ", " SMQ_R "\n"                                     \
+    "v_fma_f64 " SMQ_E ", -" SMQ_DD ", " SMQ_Q ", " SMQ_NN "\n"                        \
+    "v_div_fmas_f64 " SMQ_E ", " SMQ_E ", " SMQ_R ", " SMQ_Q "\n"                      \
+    "v_div_fixup_f64 " SMQ_T ", " SMQ_E ", " SJX ", " ZBX "\n"                         \
+    "v_mul_f64 " SMQ_DN ", " SMQ_D ", " SMQ_T "\n"                                     \
+    "v_mul_f64 " ZJX ", " ZAX ", " SMQ_T "\n"                                          \
+    "v_add_f64 " SMQ_D ", " SMQ_DN ", -" SMQ_TAU "\n"
+// deferred stores of the previous step and its emin (select: emin < zj ? emin : zj)
+#define SMQ_W(SJY, ZJY, BY, ZJLO, ZJHI)                                                \
+    "ds_write_b64 " BY ", " SJY "\n"                                                   \
+    "ds_write_b64 " BY ", " ZJY " offset:16\n"                                         \
+    "v_cmp_lt_f64_e64 %[m2], " SMQ_EM ", " ZJY "\n"
+#define SMQ_E2(BY, ZJLO, ZJHI)                                                         \
+    "v_cndmask_b32_e64 v20, " ZJLO ", v20, %[m2]\n"                                      \
+    "v_cndmask_b32_e64 v21, " ZJHI ", v21, %[m2]\n"                                      \
+    "v_add_u32 " BY ", 64, " BY "\n"
+#define SMQ_A_REGS "v[22:23]", "v[24:25]", "v[30:31]", "v[32:33]", "v1"
+#define SMQ_B_REGS "v[26:27]", "v[28:29]", "v[34:35]", "v[36:37]", "v0"
+#define SMQ_STEP_(...) SMQ_STEP(__VA_ARGS__)
+#define SMQ_ASM(CA, CB, NA, NB)                                                        \
+    "s_waitcnt lgkmcnt(0)\n"                                                           \
+    "v_mov_b64 " SMQ_D ", %[d]\n"                                                      \
+    "v_mov_b64 " SMQ_DM ", %[dm]\n"                                                    \
+    "v_mov_b64 " SMQ_EM ", %[em]\n"                                                    \
+    "v_mov_b64 " SMQ_TAU ", %[tau]\n"                                                  \
+    "v_mov_b32 v1, %[b]\n"                                                           \
+    "v_add_u32 v0, 32, v1\n"                                                       \
+    "ds_read_b64 v[22:23], v1 offset:" CA "\n"                                     \
+    "ds_read_b64 v[24:25], v1 offset:" CB "\n"                                     \
+    SMQ_STEP_(SMQ_A_REGS, SMQ_B_REGS, NA, NB, "", "")                                  \
+    "s_sub_u32 %[cnt], %[cnt], 1\n"                                                    \
+    "s_cmp_eq_u32 %[cnt], 0\n"                                                         \
+    "s_cbranch_scc1 .Lsmq_xa%=\n"                                                 \
+    ".Lsmq_top%=:\n"                                                              \
+    SMQ_STEP_(SMQ_B_REGS, SMQ_A_REGS, NA, NB,                                          \
+              SMQ_W("v[30:31]", "v[32:33]", "v1", "v32", "v33"),               \
+              SMQ_E2("v1", "v32", "v33"))                                          \
+    "s_sub_u32 %[cnt], %[cnt], 1\n"                                                    \
+    "s_cmp_eq_u32 %[cnt], 0\n"                                                         \
+    "s_cbranch_scc1 .Lsmq_xb%=\n"                                                 \
+    SMQ_STEP_(SMQ_A_REGS, SMQ_B_REGS, NA, NB,                                          \
+              SMQ_W("v[34:35]", "v[36:37]", "v0", "v36", "v37"),               \
+              SMQ_E2("v0", "v36", "v37"))                                          \
+    "s_sub_u32 %[cnt], %[cnt], 1\n"                                                    \
+    "s_cmp_lg_u32 %[cnt], 0\n"                                                         \
+    "s_cbranch_scc1 .Lsmq_top%=\n"                                                \
+    ".Lsmq_xa%=:\n"                                                               \
+    SMQ_W("v[30:31]", "v[32:33]", "v1", "v32", "v33")                          \
+    "s_nop 4\n"                                                                        \
+    "v_cndmask_b32_e64 v20, v32, v20, %[m2]\n"                                          \
+    "v_cndmask_b32_e64 v21, v33, v21, %[m2]\n"                                          \
+    "s_branch .Lsmq_end%=\n"                                                      \
+    ".Lsmq_xb%=:\n"                                                               \
+    SMQ_W("v[34:35]", "v[36:37]", "v0", "v36", "v37")                          \
+    "s_nop 4\n"                                                                        \
+    "v_cndmask_b32_e64 v20, v36, v20, %[m2]\n"                                          \
+    "v_cndmask_b32_e64 v21, v37, v21, %[m2]\n"                                          \
+    ".Lsmq_end%=:\n"                                                              \
+    "v_cmp_lt_f64_e64 %[m1], " SMQ_D ", " SMQ_DM "\n"                                  \
+    "s_nop 4\n"                                                                        \
+    "v_cndmask_b32_e64 v18, v18, v16, %[m1]\n"                                          \
+    "v_cndmask_b32_e64 v19, v19, v17, %[m1]\n"                                          \
+    "s_waitcnt lgkmcnt(0)\n"                                                           \
+    "v_mov_b64 %[d], " SMQ_D "\n"                                                      \
+    "v_mov_b64 %[dm], " SMQ_DM "\n"                                                    \
+    "v_mov_b64 %[em], " SMQ_EM "\n"
+
+template <int PP>
+__device__ __forceinline__ void dqds_steps_lds(unsigned base, int count, double tau,
+                                               double& d, double& dmin, double& emin) {
+    int cnt = __builtin_amdgcn_readfirstlane(count);
+    if (cnt < 1) return;
+    unsigned long long m0, m1, m2;
+    if (PP == 0)
+        asm volatile(SMQ_ASM("8", "24", "40", "56")
+                     : [d] "+v"(d), [dm] "+v"(dmin), [em] "+v"(emin), [cnt] "+s"(cnt),
+                       [m0] "=&s"(m0), [m1] "=&s"(m1), [m2] "=&s"(m2)
+                     : [tau] "v"(tau), [b] "v"(base)
+                     : "memory", "vcc", "scc", "v0", "v1", "v2", "v3", "v4", "v5",
+                       "v6", "v7", "v8", "v9", "v10", "v11", "v12", "v13",
+                       "v14", "v15", "v16", "v17", "v18", "v19", "v20", "v21",
+                       "v22", "v23", "v24", "v25", "v26", "v27", "v28", "v29",
+                       "v30", "v31", "v32", "v33", "v34", "v35", "v36", "v37",
+                       "v38", "v39");
+    else
+        asm volatile(SMQ_ASM("24", "40", "56", "72")
+                     : [d] "+v"(d), [dm] "+v"(dmin), [em] "+v"(emin), [cnt] "+s"(cnt),
+                       [m0] "=&s"(m0), [m1] "=&s"(m1), [m2] "=&s"(m2)
+                     : [tau] "v"(tau), [b] "v"(base)
+                     : "memory", "vcc", "scc", "v0", "v1", "v2", "v3", "v4", "v5",
+                       "v6", "v7", "v8", "v9", "v10", "v11", "v12", "v13",
+                       "v14", "v15", "v16", "v17", "v18", "v19", "v20", "v21",
+                       "v22", "v23", "v24", "v25", "v26", "v27", "v28", "v29",
+                       "v30", "v31", "v32", "v33", "v34", "v35", "v36", "v37",
+                       "v38", "v39");
+}
+#endif
+
 // One dqds transform with shift tau, IEEE form (odsq5.f).
 SM_HD inline void dlasq5(int i0, int n0, double* z, int pp, double tau, double* dmin,
                          double* dmin1, double* dmin2, double* dn, double* dnm1, double* dnm2) {
@@ -572,6 +718,16 @@ SM_HD inline void dlasq5(int i0, int n0, double* z, int pp, double tau, double* 
     // The loads of the next step are issued before this step's stores: they read
     // entries 4 beyond this step's, which its stores never touch, so the values are
     // the same and the divide chain does not wait for memory.
+#if defined(__HIP_DEVICE_COMPILE__) && !defined(SM_NO_DQDS_ASM)
+    if (__builtin_amdgcn_is_shared(z)) {
+        j4 = 4 * i0;
+        if (j4 <= 4 * (n0 - 3)) {
+            const unsigned base = (unsigned)(unsigned long long)(&SMZ(j4 - 2 - pp));
+            if (pp == 0) dqds_steps_lds<0>(base, n0 - i0 - 2, tau, d, *dmin, emin);
+            else dqds_steps_lds<1>(base, n0 - i0 - 2, tau, d, *dmin, emin);
+        }
+    } else
+#endif
     if (pp == 0) {
         j4 = 4 * i0;
         if (j4 <= 4 * (n0 - 3)) {
