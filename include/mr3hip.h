@@ -263,6 +263,9 @@ int mr3_profile_report(char *buf, int len);
  *   group_lanes      lane budget of the multisection cluster refinements, >= 4096
  *   rrf_group_max    levels with more clusters run dlarrf on one lane per cluster
  *   singleton_waves  most waves of one singleton launch, >= 64
+ *   small_first      matrices of the small route (n < 500) of at least this order start
+ *                    with the solve, the others once the large problems' root
+ *                    eigenvalues are refined; 0: all at once.  0..2^20
  * mr3_set_tuning returns 0, or 1 (and changes nothing) when a value is out of range.
  */
 typedef struct mr3_tuning {
@@ -271,6 +274,7 @@ typedef struct mr3_tuning {
     int group_lanes;
     int rrf_group_max;
     int singleton_waves;
+    int small_first;
 } mr3_tuning;
 void mr3_get_tuning(mr3_tuning *t);
 int mr3_set_tuning(const mr3_tuning *t);

@@ -45,7 +45,7 @@ class Tuning(ctypes.Structure):
     """Mirror of ``mr3_tuning`` (include/mr3hip.h)."""
     _fields_ = [("side_streams", ctypes.c_int), ("tail_singletons", ctypes.c_int),
                 ("group_lanes", ctypes.c_int), ("rrf_group_max", ctypes.c_int),
-                ("singleton_waves", ctypes.c_int)]
+                ("singleton_waves", ctypes.c_int), ("small_first", ctypes.c_int)]
 
 
 class Request(ctypes.Structure):

@@ -18,8 +18,9 @@
 #include "mr3_staging.h"
 
 namespace mr3 {
-int run_pipeline(std::vector<MatDesc>& md, std::vector<MatState>& ms, hipStream_t st);
-int small_begin(SmallJob& job, const std::vector<MatDesc>& md, hipStream_t st);
+int run_pipeline(std::vector<MatDesc>& md, std::vector<MatState>& ms, hipStream_t st,
+                 SmallJob* small);
+int small_begin(SmallJob& job, const std::vector<MatDesc>& md, hipStream_t st, bool defer);
 int small_end(SmallJob& job, std::vector<MatState>& ms, hipStream_t st);
 constexpr int kSmallRouteBelow = 500;   // mrrr.c:149-153 (DSTEMR_IF_SMALLER)
 struct BatchGraph;
@@ -67,9 +68,9 @@ extern "C" int mr3_solve_batch(mr3_problem* probs, int nprob, void* stream) {
     }
     if (mdS.empty() && mdL.empty()) return worst;
     SmallJob job;
-    const int rcS = small_begin(job, mdS, st);
+    const int rcS = small_begin(job, mdS, st, !mdL.empty());
     std::vector<MatState> msL, msS;
-    const int rcL = mdL.empty() ? 0 : run_pipeline(mdL, msL, st);
+    const int rcL = mdL.empty() ? 0 : run_pipeline(mdL, msL, st, rcS == 0 ? &job : nullptr);
     const int rcE = (rcS == 0) ? small_end(job, msS, st) : rcS;
     auto fill = [&](const std::vector<int>& idx, const std::vector<MatState>& ms, int rc) {
         for (size_t j = 0; j < idx.size(); ++j) {

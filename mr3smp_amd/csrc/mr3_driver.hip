@@ -43,6 +43,11 @@ namespace mr3 {
     } while (0)
 
 // A device failure of the solve (code 2), reported with its place under MR3_DEBUG.
+// Host timestamps of the eager tree walk (a profiling build defines the hook); none by
+// default.
+#ifndef MR3_HOST_MARK
+#define MR3_HOST_MARK(tag, level) ((void)0)
+#endif
 #define MR3_FAIL()                                                                      \
     do {                                                                                \
         if (getenv("MR3_DEBUG")) fprintf(stderr, "mr3: failure at %s:%d\n", __FILE__, __LINE__); \
@@ -61,8 +66,11 @@ static inline unsigned cdiv(long long a, long long b) { return (unsigned)((a + b
 //   group_lanes     lane budget of the group (multisection) cluster refinements
 //   rrf_group_max   levels with more clusters run dlarrf with one lane per cluster
 //   singleton_waves most waves of one singleton launch (also bounds its scratch)
+//   small_first     small-route matrices of at least this order start with the solve;
+//                   the smaller ones wait until the pipeline's root eigenvalues are
+//                   refined (0: all start at once)
 static std::mutex g_tune_mu;
-static mr3_tuning g_tune = {-1, 8192, 262144, 1 << 16, 8192};
+static mr3_tuning g_tune = {-1, 8192, 262144, 1 << 16, 8192, 0};
 static mr3_tuning tuning_snapshot() {
     std::lock_guard<std::mutex> lk(g_tune_mu);
     if (g_tune.side_streams < 0) {   // first use: the environment's overrides
@@ -363,6 +371,8 @@ struct GraphWs {
 };
 
 // Options and results of one pass of the pipeline.
+int small_release(SmallJob& job, hipStream_t after);
+
 struct PipeOpts {
     int fixed_levels = -1;   // >= 0: exactly that many tree levels, slots from one pool
     size_t pool_cap = 0;     // doubles of that pool
@@ -376,6 +386,7 @@ struct PipeOpts {
     size_t ws_need = 0;      //   levels walked, singleton waves sized for
     int levels = 0;
     int nwaves_used = 0;
+    SmallJob* small = nullptr;   // small-route work to start once the roots are refined
 };
 
 // One pass of the pipeline for the problems described by md (host copy).  Every launch
@@ -395,6 +406,7 @@ static int pipeline_once(std::vector<MatDesc>& md, std::vector<MatState>& ms, hi
     GraphWs* const gw = o.gw;
     const bool capturing = gw != nullptr;
     configure_pool_once();
+    MR3_HOST_MARK("begin", -1);
     const mr3_tuning tune = tuning_snapshot();
     const int nmat = (int)md.size();
     long long ntot = 0;
@@ -477,6 +489,10 @@ static int pipeline_once(std::vector<MatDesc>& md, std::vector<MatState>& ms, hi
            DLLr, W, Werr, Wgap, nbl_p);
     LAUNCH(k_refine_root, dim3(cdiv(N, 256)), dim3(256), dmd, dms, blk, smat,
                        iblock, Windex, D, DLLr, W, Werr, Wgap, Wtmp, Werrtmp, N);
+    // the small route's smaller matrices start now: the coarse bisection and the root
+    // refinement, which every later step of the pipeline waits for, have had the device
+    // to themselves (apart from the small route's largest matrices)
+    if (o.small && !capturing && small_release(*o.small, st)) MR3_FAIL();
     LAUNCH(k_gaps_root, dim3(cdiv(N, 256)), dim3(256), dmd, dms, blk, smat,
                        iblock, Wtmp, Werrtmp, W, Werr, Wgap, N);
     if (any_n)
@@ -783,6 +799,7 @@ static int pipeline_once(std::vector<MatDesc>& md, std::vector<MatState>& ms, hi
             return 0;
         };
         int mainL = 0;
+        MR3_HOST_MARK("walk", -1);
         if (fixed) {
             // the clusters of levels 0 .. fixed_levels - 1 and the singletons of levels
             // 0 .. fixed_levels (the last level has singletons only)
@@ -798,7 +815,9 @@ static int pipeline_once(std::vector<MatDesc>& md, std::vector<MatState>& ms, hi
             // of the level's child representations
             for (int L = 0;; ++L) {
                 if (learn(L, true)) MR3_FAIL();
+                MR3_HOST_MARK("learned", L);
                 if (hpin[L].nS > 0 && side_level(L)) MR3_FAIL();
+                MR3_HOST_MARK("side", L);
                 if (hpin[L].nC == 0) break;
                 if (L >= kMaxLevels) MR3_FAIL();   // deeper than any tree the solver allows
                 const size_t need = (size_t)hpin[L].arena;
@@ -806,6 +825,7 @@ static int pipeline_once(std::vector<MatDesc>& md, std::vector<MatState>& ms, hi
                 if (!A.ok) MR3_FAIL();
                 pool_used += need;
                 if (main_level(L, arena, need)) MR3_FAIL();
+                MR3_HOST_MARK("main", L);
                 mainL = L + 1;
             }
         }
@@ -853,54 +873,93 @@ static int pipeline_once(std::vector<MatDesc>& md, std::vector<MatState>& ms, hi
 
 // Runs the whole pipeline for the problems described by md (host copy); the device
 // outputs land in the caller's arrays named in md; states are copied back to ms.
-int run_pipeline(std::vector<MatDesc>& md, std::vector<MatState>& ms, hipStream_t st) {
+int run_pipeline(std::vector<MatDesc>& md, std::vector<MatState>& ms, hipStream_t st,
+                 SmallJob* small) {
     PipeOpts o;
+    o.small = small;
     const int rc = pipeline_once(md, ms, st, o);
     return rc == 0 ? 0 : 2;   // (the eager walk sizes every level's slots: no overflow)
 }
 
 // ---------------------------------------------------------------------------------
-// The small-matrix route (n < 500, the reference's dstemr): one k_small launch on a
-// stream of its own, started before the large problems' pipeline and collected after
-// it, so that both run on the device at the same time.
+// The small-matrix route (n < 500, the reference's dstemr): k_small on streams of its
+// own, started before the large problems' pipeline and collected after it, so that both
+// run on the device at the same time.  Its largest matrices (order >= small_first)
+// start at once: each is one long sequential chain on one lane that would otherwise end
+// last.  The others, many short solves that fill the device, start when the pipeline
+// has refined its root eigenvalues (small_release): before that the coarse bisection
+// and the root refinement, on which every later pipeline step waits, have the device.
 // ---------------------------------------------------------------------------------
-static hipStream_t small_stream() {
-    thread_local hipStream_t s = nullptr;
-    thread_local bool tried = false;
-    if (!tried) {
-        tried = true;
-        if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) s = nullptr;
+static hipStream_t small_stream(int which) {
+    thread_local hipStream_t s[2] = {nullptr, nullptr};
+    thread_local bool tried[2] = {false, false};
+    if (!tried[which]) {
+        tried[which] = true;
+        if (hipStreamCreateWithFlags(&s[which], hipStreamNonBlocking) != hipSuccess) s[which] = nullptr;
     }
-    return s;
+    return s[which];
 }
 
 struct SmallJob::Impl {
-    hipStream_t s = nullptr;
+    hipStream_t s = nullptr, s2 = nullptr;
     char* buf = nullptr;
     MatState* dms = nullptr;
-    hipEvent_t ready = nullptr;
+    hipEvent_t ready = nullptr;     // the descriptors are on the device
+    hipEvent_t done2 = nullptr;     // the deferred launch has ended
     int nmat = 0;
+    int nfirst = 0;                 // matrices launched at once (the largest)
+    bool pending = false;           // the others not launched yet
+    MatDesc* dmd = nullptr;
+    int* dlist = nullptr;
+    long long* doff = nullptr;
+    char* slices = nullptr;
     std::vector<int> list;
     std::vector<long long> off;
+    std::vector<int> nlist;         // order of the matrix of each list entry
     CallEvents ce;
     ~Impl() {   // a job that failed before small_end
         if (buf) {
             (void)hipStreamSynchronize(s);
+            if (s2) (void)hipStreamSynchronize(s2);
             (void)hipFree(buf);
         }
         if (ready) event_pool().put(ready);
+        if (done2) event_pool().put(done2);
     }
 };
 
 SmallJob::SmallJob() : p(new Impl) {}
 SmallJob::~SmallJob() { delete p; }
 
-int small_begin(SmallJob& job, const std::vector<MatDesc>& md, hipStream_t st) {
+// k_small on list entries [first, first + count) on stream s
+static int small_launch(SmallJob::Impl& J, hipStream_t s, int first, int count) {
+    if (count <= 0) return 0;
+    CallEvents& ce = J.ce;
+    const size_t lds = sizeof(double) * (size_t)mr3sm::lds_doubles(std::max(1, J.nlist[first]));
+    hipEvent_t e0_ = nullptr, e1_ = nullptr;
+    if (ce.on) {
+        e0_ = event_pool().get();
+        e1_ = event_pool().get();
+        (void)hipEventRecord(e0_, s);
+    }
+    hipLaunchKernelGGL(k_small, dim3(count), dim3(kSmallNT), lds, s, J.dmd, J.dms, J.dlist + first,
+                       J.doff + first, J.slices);
+    if (ce.on) {
+        (void)hipEventRecord(e1_, s);
+        ce.ev.push_back({"k_small", {e0_, e1_}});
+    }
+    MR3_CHECK(hipGetLastError());
+    return 0;
+}
+
+// defer: a pipeline runs beside the route and calls small_release when its roots are
+// refined; otherwise every matrix starts at once
+int small_begin(SmallJob& job, const std::vector<MatDesc>& md, hipStream_t st, bool defer) {
     SmallJob::Impl& J = *job.p;
     const int nm = (int)md.size();
     J.nmat = nm;
     if (nm == 0) return 0;
-    J.s = small_stream();
+    J.s = small_stream(0);
     if (!J.s) return 2;
     configure_pool_once();
     // largest first: the longest solves start first
@@ -908,12 +967,18 @@ int small_begin(SmallJob& job, const std::vector<MatDesc>& md, hipStream_t st) {
     for (int i = 0; i < nm; ++i) J.list[i] = i;
     std::stable_sort(J.list.begin(), J.list.end(), [&](int a, int b) { return md[a].n > md[b].n; });
     J.off.resize(nm);
+    J.nlist.resize(nm);
     long long tot = 0;
-    int nmax = 1;
     for (int j = 0; j < nm; ++j) {
         J.off[j] = tot;
+        J.nlist[j] = md[J.list[j]].n;
         tot += small_slice_bytes(md[J.list[j]].n);
-        nmax = std::max(nmax, md[J.list[j]].n);
+    }
+    const int first_n = tuning_snapshot().small_first;
+    J.nfirst = nm;
+    if (defer && first_n > 0) {
+        J.nfirst = 0;
+        while (J.nfirst < nm && J.nlist[J.nfirst] >= first_n) ++J.nfirst;
     }
     auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
     const size_t bmd = al(sizeof(MatDesc) * nm), bms = al(sizeof(MatState) * nm);
@@ -921,39 +986,47 @@ int small_begin(SmallJob& job, const std::vector<MatDesc>& md, hipStream_t st) {
     hipEvent_t e0 = event_pool().get();
     MR3_CHECK(hipEventRecord(e0, st));
     MR3_CHECK(hipStreamWaitEvent(J.s, e0, 0));
+    event_pool().put(e0);
     MR3_CHECK(hipMallocAsync((void**)&J.buf, bmd + bms + bl + bo + (size_t)tot, J.s));
-    MatDesc* dmd = (MatDesc*)J.buf;
+    J.dmd = (MatDesc*)J.buf;
     J.dms = (MatState*)(J.buf + bmd);
-    int* dlist = (int*)(J.buf + bmd + bms);
-    long long* doff = (long long*)(J.buf + bmd + bms + bl);
-    char* slices = J.buf + bmd + bms + bl + bo;
-    MR3_CHECK(hipMemcpyAsync(dmd, md.data(), sizeof(MatDesc) * nm, hipMemcpyHostToDevice, J.s));
-    MR3_CHECK(hipMemcpyAsync(dlist, J.list.data(), sizeof(int) * nm, hipMemcpyHostToDevice, J.s));
-    MR3_CHECK(hipMemcpyAsync(doff, J.off.data(), sizeof(long long) * nm, hipMemcpyHostToDevice, J.s));
-    J.ce.on = g_prof.on.load() != 0;
-    CallEvents& ce = J.ce;
-    const size_t lds = sizeof(double) * (size_t)mr3sm::lds_doubles(nmax);
-    hipEvent_t e0_ = nullptr, e1_ = nullptr;
-    if (ce.on) {
-        e0_ = event_pool().get();
-        e1_ = event_pool().get();
-        (void)hipEventRecord(e0_, J.s);
-    }
-    hipLaunchKernelGGL(k_small, dim3(nm), dim3(kSmallNT), lds, J.s, dmd, J.dms, dlist, doff, slices);
-    if (ce.on) {
-        (void)hipEventRecord(e1_, J.s);
-        ce.ev.push_back({"k_small", {e0_, e1_}});
-    }
-    MR3_CHECK(hipGetLastError());
+    J.dlist = (int*)(J.buf + bmd + bms);
+    J.doff = (long long*)(J.buf + bmd + bms + bl);
+    J.slices = J.buf + bmd + bms + bl + bo;
+    MR3_CHECK(hipMemcpyAsync(J.dmd, md.data(), sizeof(MatDesc) * nm, hipMemcpyHostToDevice, J.s));
+    MR3_CHECK(hipMemcpyAsync(J.dlist, J.list.data(), sizeof(int) * nm, hipMemcpyHostToDevice, J.s));
+    MR3_CHECK(hipMemcpyAsync(J.doff, J.off.data(), sizeof(long long) * nm, hipMemcpyHostToDevice, J.s));
     J.ready = event_pool().get();
     MR3_CHECK(hipEventRecord(J.ready, J.s));
-    event_pool().put(e0);
+    J.ce.on = g_prof.on.load() != 0;
+    if (small_launch(J, J.s, 0, J.nfirst)) return 2;
+    J.pending = J.nfirst < nm;
+    return 0;
+}
+
+// Starts the deferred matrices once the work enqueued on `after` so far is done.
+int small_release(SmallJob& job, hipStream_t after) {
+    SmallJob::Impl& J = *job.p;
+    if (!J.pending) return 0;
+    J.pending = false;
+    J.s2 = small_stream(1);
+    if (!J.s2) return 2;
+    hipEvent_t ea = event_pool().get();
+    MR3_CHECK(hipEventRecord(ea, after));
+    MR3_CHECK(hipStreamWaitEvent(J.s2, J.ready, 0));
+    MR3_CHECK(hipStreamWaitEvent(J.s2, ea, 0));
+    event_pool().put(ea);
+    if (small_launch(J, J.s2, J.nfirst, J.nmat - J.nfirst)) return 2;
+    J.done2 = event_pool().get();
+    MR3_CHECK(hipEventRecord(J.done2, J.s2));
     return 0;
 }
 
 int small_end(SmallJob& job, std::vector<MatState>& ms, hipStream_t st) {
     SmallJob::Impl& J = *job.p;
     if (J.nmat == 0) return 0;
+    if (J.pending && small_release(job, J.s)) return 2;   // no pipeline released them
+    if (J.done2) MR3_CHECK(hipStreamWaitEvent(J.s, J.done2, 0));
     ms.resize(J.nmat);
     MR3_CHECK(hipMemcpyAsync(ms.data(), J.dms, sizeof(MatState) * J.nmat, hipMemcpyDeviceToHost, J.s));
     MR3_CHECK(hipStreamSynchronize(J.s));
@@ -964,6 +1037,10 @@ int small_end(SmallJob& job, std::vector<MatState>& ms, hipStream_t st) {
     MR3_CHECK(hipStreamWaitEvent(st, J.ready, 0));
     event_pool().put(J.ready);
     J.ready = nullptr;
+    if (J.done2) {
+        event_pool().put(J.done2);
+        J.done2 = nullptr;
+    }
     return 0;
 }
 
@@ -1022,7 +1099,7 @@ static int small_graph_alloc(SmallGraph& g, const std::vector<MatDesc>& md) {
 static int small_graph_enqueue(SmallGraph& g, hipStream_t st, hipEvent_t* join_out) {
     *join_out = nullptr;
     if (g.nmat == 0) return 0;
-    hipStream_t s = small_stream();
+    hipStream_t s = small_stream(0);
     if (!s) return 2;
     const int nm = g.nmat;
     MatDesc* dmd = (MatDesc*)g.buf;
@@ -1197,6 +1274,7 @@ int set_tuning(const mr3_tuning* t) {
     if (t->side_streams < 0 || t->side_streams > kSide) return 1;
     if (t->tail_singletons < 0 || t->tail_singletons > (1 << 24)) return 1;
     if (t->group_lanes < 4096 || t->rrf_group_max < 0 || t->singleton_waves < 64) return 1;
+    if (t->small_first < 0 || t->small_first > (1 << 20)) return 1;
     (void)tuning_snapshot();   // environment defaults first, so that they do not override
     std::lock_guard<std::mutex> lk(g_tune_mu);
     g_tune = *t;

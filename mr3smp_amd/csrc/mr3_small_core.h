@@ -44,6 +44,15 @@ constexpr int kMaxExpand = 128;        // cap on bracket doublings
 enum StepKind : int { kCntDqds = 0, kCntNeg = 1, kCntSturm = 2, kCntR1v = 3, kCntRrf = 4 };
 
 SM_HD inline double fmn(double a, double b) { return b < a ? b : a; }
+// A value the device compiler may not look through: the first entries that a loop with
+// loads one step ahead takes in before it starts go through it, so that the compiler
+// does not merge them with the loop's own loads into one load at the top of each step
+// (which would make the step wait for memory again).  Nothing on the host.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define SM_OPAQUE(x) asm volatile("" : "+v"(x))
+#else
+#define SM_OPAQUE(x) ((void)0)
+#endif
 SM_HD inline double fmx(double a, double b) { return b > a ? b : a; }
 SM_HD inline int imn(int a, int b) { return a < b ? a : b; }
 SM_HD inline int imx(int a, int b) { return a > b ? a : b; }
@@ -164,20 +173,38 @@ SM_HD inline int dlaneg(int n, const double* d, const double* lld, double sigma,
     SM_STEPS(kCntNeg, n);
     int neg = 0;
     double t = -sigma;
-    for (int j = 0; j < r - 1; ++j) {
-        double dplus = d[j] + t;
-        if (dplus < 0.0) ++neg;
-        double tmp = t / dplus;
-        if (isnan(tmp)) tmp = 1.0;
-        t = tmp * lld[j] - sigma;
+    // each step loads the next step's entries before its own divide (the last step
+    // loads its own again), so that the chain through t does not wait for memory
+    if (r - 1 > 0) {
+        double dj = d[0], lj = lld[0];
+        SM_OPAQUE(dj); SM_OPAQUE(lj);
+        for (int j = 0; j < r - 1; ++j) {
+            const int jn = (j + 1 < r - 1) ? j + 1 : j;
+            const double dn = d[jn], ln = lld[jn];
+            double dplus = dj + t;
+            if (dplus < 0.0) ++neg;
+            double tmp = t / dplus;
+            if (isnan(tmp)) tmp = 1.0;
+            t = tmp * lj - sigma;
+            dj = dn;
+            lj = ln;
+        }
     }
     double p = d[n - 1] - sigma;
-    for (int j = n - 2; j >= r - 1; --j) {
-        double dminus = lld[j] + p;
-        if (dminus < 0.0) ++neg;
-        double tmp = p / dminus;
-        if (isnan(tmp)) tmp = 1.0;
-        p = tmp * d[j] - sigma;
+    if (n - 2 >= r - 1) {
+        double lj = lld[n - 2], dj = d[n - 2];
+        SM_OPAQUE(lj); SM_OPAQUE(dj);
+        for (int j = n - 2; j >= r - 1; --j) {
+            const int jn = (j - 1 >= r - 1) ? j - 1 : j;
+            const double ln = lld[jn], dn = d[jn];
+            double dminus = lj + p;
+            if (dminus < 0.0) ++neg;
+            double tmp = p / dminus;
+            if (isnan(tmp)) tmp = 1.0;
+            p = tmp * dj - sigma;
+            lj = ln;
+            dj = dn;
+        }
     }
     double gamma = (t + sigma) + p;
     if (gamma < 0.0) ++neg;
@@ -190,10 +217,19 @@ SM_HD inline int count_T(int n, const double* d, const double* e2, double x, dou
     double t = d[0] - x;
     if (fabs(t) < pivmin) t = -pivmin;
     int c = (t <= 0.0) ? 1 : 0;
-    for (int j = 1; j < n; ++j) {
-        t = d[j] - e2[j - 1] / t - x;
-        if (fabs(t) < pivmin) t = -pivmin;
-        if (t <= 0.0) ++c;
+    if (n > 1) {
+        // the next step's entries are loaded first (the last step loads its own again)
+        double dj = d[1], ej = e2[0];
+        SM_OPAQUE(dj); SM_OPAQUE(ej);
+        for (int j = 1; j < n; ++j) {
+            const int jn = (j < n - 1) ? j + 1 : j;
+            const double dn = d[jn], en = e2[jn - 1];
+            t = dj - ej / t - x;
+            if (fabs(t) < pivmin) t = -pivmin;
+            if (t <= 0.0) ++c;
+            dj = dn;
+            ej = en;
+        }
     }
     return c;
 }
@@ -205,9 +241,17 @@ SM_HD inline int count_T2(int n, const double* d, const double* e2, double x, do
     double t = d[0] - x;
     int c = 0;
     if (t <= pivmin) { c = 1; t = fmn(t, -pivmin); }
-    for (int j = 1; j < n; ++j) {
-        t = d[j] - e2[j - 1] / t - x;
-        if (t <= pivmin) { ++c; t = fmn(t, -pivmin); }
+    if (n > 1) {
+        double dj = d[1], ej = e2[0];
+        SM_OPAQUE(dj); SM_OPAQUE(ej);
+        for (int j = 1; j < n; ++j) {
+            const int jn = (j < n - 1) ? j + 1 : j;
+            const double dn = d[jn], en = e2[jn - 1];
+            t = dj - ej / t - x;
+            if (t <= pivmin) { ++c; t = fmn(t, -pivmin); }
+            dj = dn;
+            ej = en;
+        }
     }
     return c;
 }
@@ -307,9 +351,17 @@ SM_HD inline void dlarrj1(int n, const double* d, const double* e2, int i, doubl
         int c = 0;
         double dplus = d[0] - s;
         if (dplus < 0.0) ++c;
-        for (int j = 1; j < n; ++j) {
-            dplus = d[j] - s - e2[j - 1] / dplus;
-            if (dplus < 0.0) ++c;
+        if (n > 1) {
+            double dj = d[1], ej = e2[0];
+            SM_OPAQUE(dj); SM_OPAQUE(ej);
+            for (int j = 1; j < n; ++j) {
+                const int jn = (j < n - 1) ? j + 1 : j;
+                const double dn = d[jn], en = e2[jn - 1];
+                dplus = dj - s - ej / dplus;
+                if (dplus < 0.0) ++c;
+                dj = dn;
+                ej = en;
+            }
         }
         return c;
     };
@@ -1232,24 +1284,45 @@ SM_HD inline void dlar1v(int n, int b1, int bn, double lambda, const double* __r
     WK(inds + b1 - 1) = s0;
     neg1 = 0;
     s = s0 - lambda;
-    for (i = b1; i <= r1 - 1; ++i) {
-        dplus = D_(i) + s;
-        const double lpl = LD_(i) / dplus;
-        WK(indlpl + i) = lpl;
-        if (dplus < 0.0) ++neg1;
-        const double sv = s * lpl * L_(i);
-        WK(inds + i) = sv;
-        s = sv - lambda;
+    // The loops below load the next step's entries before the current step's divide
+    // (the last step loads its own again), so that the chain does not wait for memory;
+    // the arithmetic and its order are those of the plain loops.
+    if (b1 <= r1 - 1) {
+        double di = D_(b1), ldi = LD_(b1), li = L_(b1);
+        SM_OPAQUE(di); SM_OPAQUE(ldi); SM_OPAQUE(li);
+        for (i = b1; i <= r1 - 1; ++i) {
+            const int in = (i < r1 - 1) ? i + 1 : i;
+            const double dn = D_(in), ldn = LD_(in), ln = L_(in);
+            dplus = di + s;
+            const double lpl = ldi / dplus;
+            WK(indlpl + i) = lpl;
+            if (dplus < 0.0) ++neg1;
+            const double sv = s * lpl * li;
+            WK(inds + i) = sv;
+            s = sv - lambda;
+            di = dn;
+            ldi = ldn;
+            li = ln;
+        }
     }
     sawnan1 = isnan(s);
     if (!sawnan1) {
-        for (i = r1; i <= r2 - 1; ++i) {
-            dplus = D_(i) + s;
-            const double lpl = LD_(i) / dplus;
-            WK(indlpl + i) = lpl;
-            const double sv = s * lpl * L_(i);
-            WK(inds + i) = sv;
-            s = sv - lambda;
+        if (r1 <= r2 - 1) {
+            double di = D_(r1), ldi = LD_(r1), li = L_(r1);
+            SM_OPAQUE(di); SM_OPAQUE(ldi); SM_OPAQUE(li);
+            for (i = r1; i <= r2 - 1; ++i) {
+                const int in = (i < r2 - 1) ? i + 1 : i;
+                const double dn = D_(in), ldn = LD_(in), ln = L_(in);
+                dplus = di + s;
+                const double lpl = ldi / dplus;
+                WK(indlpl + i) = lpl;
+                const double sv = s * lpl * li;
+                WK(inds + i) = sv;
+                s = sv - lambda;
+                di = dn;
+                ldi = ldn;
+                li = ln;
+            }
         }
         sawnan1 = isnan(s);
     }
@@ -1282,13 +1355,22 @@ SM_HD inline void dlar1v(int n, int b1, int bn, double lambda, const double* __r
     const double p0 = D_(bn) - lambda;
     WK(indp + bn - 1) = p0;
     double pv = p0;
-    for (i = bn - 1; i >= r1; --i) {
-        dminus = LLD_(i) + pv;
-        tmp = D_(i) / dminus;
-        if (dminus < 0.0) ++neg2;
-        WK(indumn + i) = L_(i) * tmp;
-        pv = pv * tmp - lambda;
-        WK(indp + i - 1) = pv;
+    if (bn - 1 >= r1) {
+        double lldi = LLD_(bn - 1), di = D_(bn - 1), li = L_(bn - 1);
+        SM_OPAQUE(lldi); SM_OPAQUE(di); SM_OPAQUE(li);
+        for (i = bn - 1; i >= r1; --i) {
+            const int in = (i > r1) ? i - 1 : i;
+            const double lldn = LLD_(in), dn = D_(in), ln = L_(in);
+            dminus = lldi + pv;
+            tmp = di / dminus;
+            if (dminus < 0.0) ++neg2;
+            WK(indumn + i) = li * tmp;
+            pv = pv * tmp - lambda;
+            WK(indp + i - 1) = pv;
+            lldi = lldn;
+            di = dn;
+            li = ln;
+        }
     }
     tmp = pv;
     sawnan2 = isnan(tmp);
@@ -1311,10 +1393,18 @@ SM_HD inline void dlar1v(int n, int b1, int bn, double lambda, const double* __r
     *negcnt = wantnc ? neg1 + neg2 : -1;
     if (fabs(mg) == 0.0) mg = kEps * WK(inds + r1 - 1);
     int rr = r1;
-    for (i = r1; i <= r2 - 1; ++i) {
-        tmp = WK(inds + i) + WK(indp + i);
-        if (tmp == 0.0) tmp = kEps * WK(inds + i);
-        if (fabs(tmp) <= fabs(mg)) { mg = tmp; rr = i + 1; }
+    if (r1 <= r2 - 1) {
+        double si = WK(inds + r1), pi = WK(indp + r1);
+        SM_OPAQUE(si); SM_OPAQUE(pi);
+        for (i = r1; i <= r2 - 1; ++i) {
+            const int in = (i < r2 - 1) ? i + 1 : i;
+            const double sn = WK(inds + in), pn = WK(indp + in);
+            tmp = si + pi;
+            if (tmp == 0.0) tmp = kEps * si;
+            if (fabs(tmp) <= fabs(mg)) { mg = tmp; rr = i + 1; }
+            si = sn;
+            pi = pn;
+        }
     }
     *mingma = mg;
     *r = rr;
@@ -1323,9 +1413,17 @@ SM_HD inline void dlar1v(int n, int b1, int bn, double lambda, const double* __r
     double zz = 1.0, cmax = *cutmax, ncmin = *noncutmin;
     if (!sawnan1 && !sawnan2) {
         double zn = 1.0;   // Z(i + 1)
+        double wi = (rr - 1 >= b1) ? WK(indlpl + rr - 1) : 0.0;
+        SM_OPAQUE(wi);
+        double ldi = (rr - 1 >= b1) ? LD_(rr - 1) : 0.0;
+        SM_OPAQUE(ldi);
         for (i = rr - 1; i >= b1; --i) {
-            const double zi = -(WK(indlpl + i) * zn);
-            const double tv = (fabs(zi) + fabs(zn)) * fabs(LD_(i));
+            const int in = (i > b1) ? i - 1 : i;
+            const double wn = WK(indlpl + in), ldn = LD_(in);
+            const double zi = -(wi * zn);
+            const double tv = (fabs(zi) + fabs(zn)) * fabs(ldi);
+            wi = wn;
+            ldi = ldn;
             if (tv < gaptol) {
                 cmax = fmx(cmax, tv);
                 Z_(i) = 0.0;
@@ -1338,9 +1436,17 @@ SM_HD inline void dlar1v(int n, int b1, int bn, double lambda, const double* __r
             zn = zi;
         }
         double zc = 1.0;   // Z(i)
+        double wu = (rr <= bn - 1) ? WK(indumn + rr) : 0.0;
+        SM_OPAQUE(wu);
+        double ldu = (rr <= bn - 1) ? LD_(rr) : 0.0;
+        SM_OPAQUE(ldu);
         for (i = rr; i <= bn - 1; ++i) {
-            const double zi = -(WK(indumn + i) * zc);
-            const double tv = (fabs(zc) + fabs(zi)) * fabs(LD_(i));
+            const int in = (i < bn - 1) ? i + 1 : i;
+            const double wn = WK(indumn + in), ldn = LD_(in);
+            const double zi = -(wu * zc);
+            const double tv = (fabs(zc) + fabs(zi)) * fabs(ldu);
+            wu = wn;
+            ldu = ldn;
             if (tv < gaptol) {
                 cmax = fmx(cmax, tv);
                 Z_(i + 1) = 0.0;
@@ -1448,14 +1554,24 @@ SM_HD inline int dlarrf(int n, const double* __restrict__ d, const double* __res
         if (fabs(dp) < pivmin) { dp = -pivmin; sawnan1 = true; }
         DP(1) = dp;
         max1 = fabs(dp);
-        for (i = 1; i <= n - 1; ++i) {
-            const double lp = ld[i - 1] / dp;
-            LP(i) = lp;
-            s = s * lp * l[i - 1] - lsigma;
-            dp = d[i] + s;
-            if (fabs(dp) < pivmin) { dp = -pivmin; sawnan1 = true; }
-            DP(i + 1) = dp;
-            max1 = fmax(max1, fabs(dp));
+        // each step loads the next step's entries first (the last step its own again)
+        if (n > 1) {
+            double ldi = ld[0], li = l[0], di = d[1];
+            SM_OPAQUE(ldi); SM_OPAQUE(li); SM_OPAQUE(di);
+            for (i = 1; i <= n - 1; ++i) {
+                const int in = (i < n - 1) ? i + 1 : i;
+                const double ldn = ld[in - 1], ln = l[in - 1], dn = d[in];
+                const double lp = ldi / dp;
+                LP(i) = lp;
+                s = s * lp * li - lsigma;
+                dp = di + s;
+                if (fabs(dp) < pivmin) { dp = -pivmin; sawnan1 = true; }
+                DP(i + 1) = dp;
+                max1 = fmax(max1, fabs(dp));
+                ldi = ldn;
+                li = ln;
+                di = dn;
+            }
         }
         sawnan1 = sawnan1 || isnan(max1);
         if (forcer || (max1 <= growthbound && !sawnan1)) { *sigma = lsigma; shift = SLEFT; break; }
@@ -1465,14 +1581,23 @@ SM_HD inline int dlarrf(int n, const double* __restrict__ d, const double* __res
         if (fabs(dp) < pivmin) { dp = -pivmin; sawnan2 = true; }
         WK(1) = dp;
         max2 = fabs(dp);
-        for (i = 1; i <= n - 1; ++i) {
-            const double lp = ld[i - 1] / dp;
-            WK(n + i) = lp;
-            s = s * lp * l[i - 1] - rsigma;
-            dp = d[i] + s;
-            if (fabs(dp) < pivmin) { dp = -pivmin; sawnan2 = true; }
-            WK(i + 1) = dp;
-            max2 = fmax(max2, fabs(dp));
+        if (n > 1) {
+            double ldi = ld[0], li = l[0], di = d[1];
+            SM_OPAQUE(ldi); SM_OPAQUE(li); SM_OPAQUE(di);
+            for (i = 1; i <= n - 1; ++i) {
+                const int in = (i < n - 1) ? i + 1 : i;
+                const double ldn = ld[in - 1], ln = l[in - 1], dn = d[in];
+                const double lp = ldi / dp;
+                WK(n + i) = lp;
+                s = s * lp * li - rsigma;
+                dp = di + s;
+                if (fabs(dp) < pivmin) { dp = -pivmin; sawnan2 = true; }
+                WK(i + 1) = dp;
+                max2 = fmax(max2, fabs(dp));
+                ldi = ldn;
+                li = ln;
+                di = dn;
+            }
         }
         sawnan2 = sawnan2 || isnan(max2);
         if (forcer || (max2 <= growthbound && !sawnan2)) { *sigma = rsigma; shift = SRIGHT; break; }

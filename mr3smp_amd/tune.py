@@ -3,10 +3,11 @@
 The solver's results do not depend on these parameters, only its speed does: how many
 streams carry the singleton phases, when the latency kernel takes over the last
 singleton iterations, the lane budget of the multisection cluster refinements, the
-level size from which dlarrf runs on one lane per cluster, and the most waves of one
-singleton launch.  `autotune` times a prepared batch under candidate values, one
-parameter at a time (coordinate descent, repeated while it improves), keeps the
-fastest, and leaves it applied.  The tuned values are plain JSON (`save` / `load` /
+level size from which dlarrf runs on one lane per cluster, the most waves of one
+singleton launch, and the order from which the small route's matrices start with the
+solve rather than after the large problems' root refinement.  `autotune` times a
+prepared batch under candidate values, one parameter at a time (coordinate descent,
+repeated while it improves), keeps the fastest, and leaves it applied.  The tuned values are plain JSON (`save` / `load` /
 `apply`), so a search made once on a device and workload can be reused.
 
 No counterpart in the reference, whose only knob is PMR_NUM_THREADS.
@@ -23,6 +24,7 @@ SPACE = {
     "group_lanes": [262144, 131072, 524288],
     "rrf_group_max": [65536, 16384, 262144],
     "singleton_waves": [8192, 4096, 16384],
+    "small_first": [0, 350, 250],
 }
 
 

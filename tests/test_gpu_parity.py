@@ -493,9 +493,10 @@ def test_gpu_batch_equals_single(M):
 
 
 def test_gpu_side_stream_count_bitwise(M):
-    """The tree levels' singleton phases may run on one, two or three side streams, and
-    the other launch-shape parameters (mr3_set_tuning) move work between kernels; the
-    results are the same bit for bit (they only change what overlaps and where)."""
+    """The tree levels' singleton phases may run on one, two or three side streams, the
+    small route may start its smaller matrices late (small_first), and the other
+    launch-shape parameters (mr3_set_tuning) move work between kernels; the results are
+    the same bit for bit (they only change what overlaps and where)."""
     import torch
     from mr3smp_amd import workloads as WL
     dev = torch.device("cuda:0")
@@ -504,7 +505,8 @@ def test_gpu_side_stream_count_bitwise(M):
                 dict(side_streams=2, tail_singletons=0, group_lanes=4096, rrf_group_max=0,
                      singleton_waves=64),
                 dict(side_streams=3, tail_singletons=1 << 20, group_lanes=1 << 22,
-                     rrf_group_max=1 << 20)]
+                     rrf_group_max=1 << 20),
+                dict(side_streams=3, small_first=300), dict(side_streams=2, small_first=40)]
     outs = []
     before = M.get_tuning()
     try:
