@@ -452,6 +452,7 @@ def main():
     ap.add_argument("--serve-batch", type=int, default=16, help="most requests per batch")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-graph", action="store_true", help="skip the captured-graph form")
     ap.add_argument("--autotune", action="store_true",
                     help="search the launch-shape parameters on this workload before timing "
                          "(mr3smp_amd/tune.py) and write them to --tuning-file")
@@ -513,7 +514,10 @@ def main():
     eager_value = pairs * args.steps / dt
     prof_steps = args.steps
     # the same steps replayed from one captured graph; the headline is the faster form
-    gdt, gcap = graph_steps(M, torch, arr, stream, args.steps, pairs)
+    if args.no_graph:
+        gdt, gcap = float("inf"), None
+    else:
+        gdt, gcap = graph_steps(M, torch, arr, stream, args.steps, pairs)
     graph_value = pairs * args.steps / gdt
     launch = "graph" if graph_value > eager_value else "eager"
     value = max(graph_value, eager_value)
@@ -539,17 +543,23 @@ def main():
     serve = None
     if not args.no_serve:
         try:
+            # the server's device-memory budget: what this process has not taken yet (the
+            # batched workload's buffers and the library's cached workspaces stay
+            # allocated), with room left for the runtime's own scratch allocations
+            free_b, _ = torch.cuda.mem_get_info()
+            budget = int(min(200 << 30, 0.6 * free_b))
             cap = None
             rate = args.serve_rate
             if rate is None:
                 # capacity: the same requests, a closed loop that keeps the server busy
-                cap = serve_stream(M, args.serve_requests, None, 200 << 30,
+                cap = serve_stream(M, args.serve_requests, None, budget,
                                    args.serve_inflight, args.serve_batch, window=args.serve_window)
                 rate = 0.8 * args.serve_requests / cap["wall_s"]
                 log(f"serve capacity: {cap}")
-            serve = serve_stream(M, args.serve_requests, rate, 200 << 30, args.serve_inflight,
+            serve = serve_stream(M, args.serve_requests, rate, budget, args.serve_inflight,
                                  args.serve_batch)
-            serve["server"] = {"max_inflight": args.serve_inflight, "max_batch": args.serve_batch}
+            serve["server"] = {"max_inflight": args.serve_inflight, "max_batch": args.serve_batch,
+                               "budget_GiB": round(budget / 2**30, 1)}
             if cap is not None:
                 serve["capacity"] = {"requests_per_s": args.serve_requests / cap["wall_s"],
                                      "eigenpairs_per_s": cap["eigenpairs_per_s"],
@@ -585,8 +595,9 @@ def main():
                    "matrices": args.count, "eigenpairs_per_step": pairs, "launch": launch},
         "eager": {"eigenpairs_per_s": eager_value, "ms_per_step": dt / args.steps * 1e3,
                   "how": "mr3_solve_batch per step"},
-        "graph": {"eigenpairs_per_s": graph_value, "ms_per_step": gdt / args.steps * 1e3,
-                  "capture_s": gcap, "how": "one captured hipGraph replayed per step (mr3_graph_launch)"},
+        "graph": None if args.no_graph else {
+            "eigenpairs_per_s": graph_value, "ms_per_step": gdt / args.steps * 1e3,
+            "capture_s": gcap, "how": "one captured hipGraph replayed per step (mr3_graph_launch)"},
         "plain_value": plain,
         "batched_over_plain": value / plain,
         "plain_sample": f"every {args.plain_stride}th matrix, one solve per matrix",

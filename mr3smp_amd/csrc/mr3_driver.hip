@@ -217,13 +217,16 @@ void release_thread_streams() {
 // took about 12 ms per batch of 4096 matrices).  A solve that needs more than its
 // block takes the rest from the stream-ordered pool and, at its end, replaces the
 // block by one large enough for next time.  Concurrent solves (host threads, server
-// batches) each hold their own block.
+// batches) each hold their own block.  The list keeps at most kWsCacheBytes: a block
+// handed back beyond that is released (a server's workers each grow a block to the
+// largest solve they met, and sixteen such blocks had taken most of the device).
 struct Workspace {
     char* base = nullptr;
     size_t cap = 0;
 };
 static std::mutex g_ws_mu;
 static std::vector<Workspace> g_ws_free;
+constexpr size_t kWsCacheBytes = 64ull << 30;
 
 struct Alloc {
     hipStream_t st;
@@ -289,7 +292,13 @@ struct Alloc {
         }
         if (ws.base) {
             std::lock_guard<std::mutex> lk(g_ws_mu);
-            g_ws_free.push_back(ws);
+            size_t held = 0;
+            for (const Workspace& w : g_ws_free) held += w.cap;
+            if (held + ws.cap <= kWsCacheBytes) {
+                g_ws_free.push_back(ws);
+            } else {
+                (void)hipFreeAsync(ws.base, st);
+            }
         }
     }
 };
@@ -309,11 +318,16 @@ static void configure_pool_once() {
 
 // Scratch budget of the eigenvector kernel (per batch): a quarter of free memory,
 // at most 32 GiB.
+// Device memory for the singleton kernels' per-wave checkpoints, shared by the side
+// streams of a solve: a quarter of what is free, at most 6 GiB.  With blocks of 2048 rows
+// that is every wave the tuning asks for; a batch with blocks of 16384 rows gets about
+// 2000 waves per side stream (a wave takes tasks until none are left, so fewer waves
+// only lose concurrency), and concurrent solves of a server stay within the device.
 static size_t singleton_scratch_budget() {
     size_t fr = 0, tot = 0;
     if (hipMemGetInfo(&fr, &tot) != hipSuccess) return (size_t)1 << 30;
     size_t b = fr / 4;
-    const size_t cap = (size_t)32 << 30;
+    const size_t cap = (size_t)6 << 30;
     return std::min(b, cap);
 }
 
