@@ -217,16 +217,24 @@ void release_thread_streams() {
 // took about 12 ms per batch of 4096 matrices).  A solve that needs more than its
 // block takes the rest from the stream-ordered pool and, at its end, replaces the
 // block by one large enough for next time.  Concurrent solves (host threads, server
-// batches) each hold their own block.  The list keeps at most kWsCacheBytes: a block
-// handed back beyond that is released (a server's workers each grow a block to the
-// largest solve they met, and sixteen such blocks had taken most of the device).
+// batches) each hold their own block.  The list keeps at most 60% of the device's
+// memory: a block handed back beyond that is released (a server's workers each grow a
+// block to the largest solve they met).
 struct Workspace {
     char* base = nullptr;
     size_t cap = 0;
 };
 static std::mutex g_ws_mu;
 static std::vector<Workspace> g_ws_free;
-constexpr size_t kWsCacheBytes = 64ull << 30;
+static size_t ws_cache_bytes() {
+    static size_t cap = 0;
+    static std::once_flag once;
+    std::call_once(once, [] {
+        size_t fr = 0, tot = 0;
+        cap = (hipMemGetInfo(&fr, &tot) == hipSuccess) ? tot / 10 * 6 : ((size_t)64 << 30);
+    });
+    return cap;
+}
 
 struct Alloc {
     hipStream_t st;
@@ -294,7 +302,7 @@ struct Alloc {
             std::lock_guard<std::mutex> lk(g_ws_mu);
             size_t held = 0;
             for (const Workspace& w : g_ws_free) held += w.cap;
-            if (held + ws.cap <= kWsCacheBytes) {
+            if (held + ws.cap <= ws_cache_bytes()) {
                 g_ws_free.push_back(ws);
             } else {
                 (void)hipFreeAsync(ws.base, st);
@@ -318,17 +326,32 @@ static void configure_pool_once() {
 
 // Scratch budget of the eigenvector kernel (per batch): a quarter of free memory,
 // at most 32 GiB.
-// Device memory for the singleton kernels' per-wave checkpoints, shared by the side
-// streams of a solve: a quarter of what is free, at most 6 GiB.  With blocks of 2048 rows
-// that is every wave the tuning asks for; a batch with blocks of 16384 rows gets about
-// 2000 waves per side stream (a wave takes tasks until none are left, so fewer waves
-// only lose concurrency), and concurrent solves of a server stay within the device.
+// Device memory for the singleton kernels' per-wave checkpoints: a quarter of the free
+// memory, at most 32 GiB.  Within it a singleton launch gets at most the tuned number of
+// waves, and beyond 2 GiB of checkpoints per side stream no more than the device holds
+// at once (a wave takes tasks until none are left, so a wave that can only start after
+// others have ended finds none).  Without that bound a server's concurrent solves of
+// blocks of 16384 rows each took about 26 GiB and ran the device out of memory.
 static size_t singleton_scratch_budget() {
     size_t fr = 0, tot = 0;
     if (hipMemGetInfo(&fr, &tot) != hipSuccess) return (size_t)1 << 30;
     size_t b = fr / 4;
-    const size_t cap = (size_t)6 << 30;
+    const size_t cap = (size_t)32 << 30;
     return std::min(b, cap);
+}
+
+static int singleton_waves_resident() {
+    static std::once_flag once;
+    static int waves = 0;
+    std::call_once(once, [] {
+        int dev = 0, per_cu = 0;
+        hipDeviceProp_t pr;
+        if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&pr, dev) != hipSuccess) return;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_s_iter, 64, 0) != hipSuccess)
+            return;
+        waves = per_cu * pr.multiProcessorCount;
+    });
+    return waves > 0 ? waves : 2048;
 }
 
 // Pinned host copies of the counters of every tree level of a solve, reused across
@@ -589,8 +612,9 @@ static int pipeline_once(std::vector<MatDesc>& md, std::vector<MatState>& ms, hi
         const size_t ck_per_lane = (size_t)2 * (maxbl / kChunk + 2);
         const size_t per_wave = ck_per_lane * 64 * sizeof(double);
         const size_t budget = singleton_scratch_budget() / nside;
-        int max_waves = (int)std::max<size_t>(1, std::min<size_t>(budget / per_wave,
-                                                                  (size_t)tune.singleton_waves));
+        const size_t roomy = std::max<size_t>(singleton_waves_resident(), ((size_t)2 << 30) / per_wave);
+        int max_waves = (int)std::max<size_t>(
+            1, std::min(std::min<size_t>(budget / per_wave, (size_t)tune.singleton_waves), roomy));
         if (o.nwaves > 0) max_waves = o.nwaves;
         const int nwaves = std::min(max_waves, capT);
         o.nwaves_used = nwaves;
