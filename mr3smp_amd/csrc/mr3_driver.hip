@@ -448,7 +448,7 @@ static int pipeline_once(std::vector<MatDesc>& md, std::vector<MatState>& ms, hi
     const int nmat = (int)md.size();
     long long ntot = 0;
     int nmax = 1;
-    bool any_v = false, any_n = false;
+    bool any_v = false, any_n = false, any_part = false;
     // Matrices take their slots of the eigenvalue arrays in order of decreasing size, so
     // that the lanes of a wave, which take consecutive slots, mostly belong to matrices of
     // about the same size and sweep about the same number of rows.  Results do not
@@ -463,6 +463,7 @@ static int pipeline_once(std::vector<MatDesc>& md, std::vector<MatState>& ms, hi
         nmax = std::max(nmax, d.n);
         any_v |= (d.jobz == kV);
         any_n |= (d.jobz == kN);
+        any_part |= (d.range != kAll && d.jobz != kC);
     }
     if (ntot > 0x7fffffff / 2) MR3_FAIL();
     const int N = (int)ntot;
@@ -513,6 +514,7 @@ static int pipeline_once(std::vector<MatDesc>& md, std::vector<MatState>& ms, hi
 
     LAUNCH(k_prep, dim3(nmat), dim3(256), dmd, dms, D, E, E2, gers, Dcopy,
                        E2copy, isplit, blk, W, Werr, Wcoarse, iblock, Windex, smat);
+    if (any_part) LAUNCH(k_block_bracket, dim3(nmat), dim3(PREP_NT), dmd, dms, D, E2, blk);
     LAUNCH(k_block_list, dim3(1), dim3(kListNT), dmd, dms, dorder, nmat, 0, plist, lcount);
     LAUNCH(k_coarse_tree, dim3(grid_for(N, 1, 2048)), dim3(256), dmd, dms, D, E2, blk, smat, plist,
            lcount, tcount);
@@ -1284,7 +1286,8 @@ int profile_report(char* buf, int len) {
                                             "r1v_wave", "bisect_wave", "task_ticks",
                                             "tasks", "t_bisect", "t_r1v", "t_solve", "rrf_rows",
                                             "sm_dqds", "sm_neg", "sm_sturm", "sm_r1v", "sm_rrf",
-                                            "tail_r1v", "tail_bisect", "tailb_r1v", "tailb_bisect"};
+                                            "tail_r1v", "tail_bisect", "tailb_r1v", "tailb_bisect",
+                                            "fallbacks"};
     std::string js = "{\"kernels\": {";
     std::lock_guard<std::mutex> lk(g_prof.mu);
     bool first = true;

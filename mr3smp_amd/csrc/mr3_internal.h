@@ -69,6 +69,11 @@ struct BlockRec {           // one per unreduced block
     double glraw, guraw;    // unfudged Gershgorin bounds of the block
     double spdiam;          // spdiam returned by compute_root_rrr
     double sigma;           // root shift
+    double bl, bu;          // interval the block's coarse bisection starts from, and
+    int nl, nr;             //   the counts at its ends (dlarrd's block loop,
+                            //   odrrd.f:455-480): the fudged Gershgorin interval with
+                            //   counts 0 and nbl, cut to the block's own range-I bracket
+                            //   when only part of its spectrum is wanted
 };
 
 // A cluster of the representation tree at depth >= 1 (process_cluster.c cluster_t).

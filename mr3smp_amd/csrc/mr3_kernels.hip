@@ -52,7 +52,9 @@ enum StepKind : int {
     kStepTailBis = 22,   //   its bisection-fallback dlaneg steps, 4 flops
     kStepTailBR1v = 23,  // the same for the large-block tail kernel (S and P in device memory)
     kStepTailBBis = 24,
-    kStepKinds = 25
+    kStepFallbacks = 25, // singletons that fall back to bisection (process_singleton.c:186-204),
+                         // one count per fallback
+    kStepKinds = 26
 };
 __device__ unsigned long long g_steps[kStepKinds];
 __constant__ int g_count_steps;
@@ -222,6 +224,15 @@ __device__ void small_case(const MatDesc& d, MatState& S, const double* D, const
 // the path the sequential loop takes, with its updates and its stopping test step by
 // step.  Points, counts and results are the sequential loop's bit for bit, with about
 // a seventh of its sweeps one after the other.
+// The block's fudged Gershgorin interval as dlarrd's block loop forms it
+// (odrrd.f:337-341 with FUDGE = 2, the block's own norm).
+__device__ __forceinline__ void coarse_bounds(const BlockRec& br, int nbl, double pivmin,
+                                              double* left, double* right) {
+    const double tnormb = fmax(fabs(br.glraw), fabs(br.guraw));
+    *left = br.glraw - 2.0 * tnormb * kEps * nbl - 2.0 * pivmin;
+    *right = br.guraw + 2.0 * tnormb * kEps * nbl + 2.0 * pivmin;
+}
+
 constexpr int PREP_NT = 256;   // threads of k_prep
 constexpr int kL3 = 7;
 struct Ijob3 {
@@ -506,6 +517,9 @@ __global__ __launch_bounds__(PREP_NT) void k_prep(const MatDesc* __restrict__ md
             br.bbeg = ibgn; br.bend = iend; br.mat = k; br.sigma = 0.0;
             br.ilow = 1; br.iupp = 0;
             br.gl = br.gu = br.glraw = br.guraw = d[ibgn];
+            br.bl = br.bu = d[ibgn];
+            br.nl = 0;
+            br.nr = nbl;
             br.spdiam = 0.0;
             if (nbl == 1) {
                 int in = (irange == kAll) || (irange == kVal && d[ibgn] > wl && d[ibgn] <= wu) ||
@@ -523,6 +537,9 @@ __global__ __launch_bounds__(PREP_NT) void k_prep(const MatDesc* __restrict__ md
                 br.glraw = bgl; br.guraw = bgu;
                 br.gl = bgl - 2 * tnorm * DBL_EPSILON * nbl - 4 * pivmin;
                 br.gu = bgu + 2 * tnorm * DBL_EPSILON * nbl + 4 * pivmin;
+                coarse_bounds(br, nbl, pivmin, &br.bl, &br.bu);
+                br.nl = 0;
+                br.nr = nbl;
                 int ilow, iupp;
                 if (irange == kAll) { ilow = 1; iupp = nbl; }
                 else {
@@ -598,6 +615,79 @@ __global__ __launch_bounds__(PREP_NT) void k_prep(const MatDesc* __restrict__ md
 }
 
 // ---------------------------------------------------------------------------------
+// k_block_bracket: the range-I bracket of each block whose wanted index range is not
+// its whole spectrum.  mrrr_val hands such a block to dlarrd with range I and the
+// block's own index range ilow..iupp (mrrr_val.c:421, find_eigval_irreducible), and
+// dlarrd first brackets those indices by dlaebz IJOB = 3 on the block (odrrd.f:343-393,
+// from the block's Gershgorin interval fudged by FUDGE·tnorm·eps·n + FUDGE·2·pivmin, to
+// RELTOL = bsrtol), then bisects from the block loop's interval cut to that bracket
+// (odrrd.f:455-480), with the counts at its ends.  Those are the points the reference's
+// bisection visits, so the coarse eigenvalues are its bits.  One workgroup per matrix,
+// its blocks one after the other (ijob3_pair spreads each step over the workgroup).
+// ---------------------------------------------------------------------------------
+__global__ __launch_bounds__(PREP_NT) void k_block_bracket(const MatDesc* __restrict__ md,
+                                                           MatState* __restrict__ ms,
+                                                           const double* __restrict__ D,
+                                                           const double* __restrict__ E2,
+                                                           BlockRec* blk) {
+    __shared__ Ijob3 st3[2];
+    __shared__ int cnt3[2][1 << kL3];
+    __shared__ int sh_bad;
+    const int k = blockIdx.x, tid = threadIdx.x;
+    MatState& S = ms[k];
+    if (S.done || S.status != kOk || S.irange == kAll) return;
+    const int off = md[k].off, nsplit = S.nsplit;
+    const double pivmin = S.pivmin, bsrtol = S.bsrtol;
+    const double atoli = 2.0 * 2.0 * kSafmin + 2.0 * 2.0 * pivmin;
+    if (tid == 0) sh_bad = 0;
+    for (int jb = 0; jb < nsplit; ++jb) {
+        __syncthreads();
+        BlockRec br = blk[off + jb];
+        const int nbl = br.bend - br.bbeg + 1;
+        const int mbl = br.wend - br.wbeg + 1;
+        if (nbl == 1 || mbl <= 0 || (br.ilow == 1 && br.iupp == nbl)) continue;   // uniform
+        const double* d = D + off + br.bbeg;
+        const double* e2 = E2 + off + br.bbeg;
+        // odrrd.f:330-341 on the block alone
+        const double tnorm = fmax(fabs(br.glraw), fabs(br.guraw));
+        const double gl = br.glraw - 2.0 * tnorm * kEps * nbl - 2.0 * 2.0 * pivmin;
+        const double gu = br.guraw + 2.0 * tnorm * kEps * nbl + 2.0 * 2.0 * pivmin;
+        const int itmax = (int)((log(tnorm + pivmin) - log(pivmin)) / log(2.0)) + 2;
+        const int nval[2] = {br.ilow - 1, br.iupp};
+        if (tid < 2) {
+            st3[tid].a1 = gl; st3[tid].a2 = gu; st3[tid].c = tid == 0 ? gl : gu;
+            st3[tid].n1 = -1; st3[tid].n2 = nbl + 1; st3[tid].it = 0; st3[tid].done = itmax <= 0;
+        }
+        ijob3_pair(nbl, d, e2, pivmin, atoli, bsrtol, itmax, nval, st3, cnt3);
+        if (tid == 0) {
+            const double wl = st3[0].a1, wu = st3[1].a2;
+            const int nwl = st3[0].n1, nwu = st3[1].n2;
+            double bl = br.bl, bu = br.bu;   // the block loop's interval (odrrd.f:455-462)
+            bool bad = nwl < 0 || nwl >= nbl || nwu < 1 || nwu > nbl;   // odrrd.f:384-390
+            if (!bad) {
+                if (bu < wl) bad = true;      // no wanted eigenvalue here after all
+                bl = fmax(bl, wl);
+                bu = fmin(bu, wu);
+                if (bl >= bu) bad = true;
+            }
+            if (bad) {
+                // the reference's dlarrd would return an error or fewer values than
+                // mrrr_val counted: the problem fails (info 2)
+                sh_bad = 1;
+            } else {
+                br.bl = bl;
+                br.bu = bu;
+                br.nl = count_T_ijob1(nbl, d, e2, bl, pivmin);   // IJOB = 1 at both ends
+                br.nr = count_T_ijob1(nbl, d, e2, bu, pivmin);
+                blk[off + jb] = br;
+            }
+        }
+    }
+    __syncthreads();
+    if (tid == 0 && sh_bad) { S.status = kNoRoot; S.done = 1; }
+}
+
+// ---------------------------------------------------------------------------------
 // k_coarse: eigenvalue approximations by bisection on T, one lane per wanted
 // eigenvalue (dlarrd/dlaebz, odrrd.f:415-574, to RELTOL = bsrtol).
 //
@@ -625,13 +715,6 @@ __device__ __forceinline__ int coarse_tree_depth(int nbl, int mbl) {
 
 // Initial interval of a block's coarse bisection: dlarrd called on the block alone
 // (mrrr_val.c:1291-1295) with its own fudged bounds (odrrd.f:515-519, FUDGE = 2).
-__device__ __forceinline__ void coarse_bounds(const BlockRec& br, int nbl, double pivmin,
-                                              double* left, double* right) {
-    const double tnormb = fmax(fabs(br.glraw), fabs(br.guraw));
-    *left = br.glraw - 2.0 * tnormb * kEps * nbl - 2.0 * pivmin;
-    *right = br.guraw + 2.0 * tnormb * kEps * nbl + 2.0 * pivmin;
-}
-
 // Counts at the tree nodes 1 .. 2^depth - 1 of every block of at least 128 rows; one
 // workgroup per block, tcount[off + bbeg + t - 1] = count at node t.
 __device__ __forceinline__ void k_coarse_tree_one(int bx, const MatDesc* __restrict__ md,
@@ -658,7 +741,8 @@ __device__ __forceinline__ void k_coarse_tree_one(int bx, const MatDesc* __restr
     const double* d = D + off + br.bbeg;
     const double* e2 = E2 + off + br.bbeg;
     double left0, right0;
-    coarse_bounds(br, nbl, pivmin, &left0, &right0);
+    left0 = br.bl;
+    right0 = br.bu;
     int sweeps = 0;
     for (int t = threadIdx.x + 1; t < (1 << depth); t += blockDim.x) {
         const int dt = 31 - __clz(t);
@@ -715,14 +799,13 @@ __global__ __launch_bounds__(256) void k_coarse(const MatDesc* __restrict__ md,
     const double* e2 = E2 + off + br.bbeg;
     const int i = Windex[gidx];
     // the block's own bounds and tolerances (odrrd.f:337-341)
-    double left, right;
-    coarse_bounds(br, nbl, pivmin, &left, &right);
+    double left = br.bl, right = br.bu;
     const double atoli = 2.0 * 2.0 * kSafmin + 2.0 * 2.0 * pivmin;
     const double rtoli = S.bsrtol;
     const int itmax = (int)((log(right - left + pivmin) - log(pivmin)) / log(2.0)) + 2;
     const int depth = coarse_tree_depth(nbl, br.wend - br.wbeg + 1);
     const int* tc = tcount + off + br.bbeg - 1;   // tc[t]: count at tree node t
-    int nl = 0, nr = nbl, sweeps = 0, t = 1;
+    int nl = br.nl, nr = br.nr, sweeps = 0, t = 1;
     for (int it = 0; it < itmax; ++it) {
         double tmp1 = fabs(right - left), tmp2 = fmax(fabs(right), fabs(left));
         if (tmp1 < fmax(fmax(atoli, pivmin), rtoli * tmp2)) break;
@@ -3892,6 +3975,7 @@ __global__ __launch_bounds__(128) void k_s_tail(const int* __restrict__ DI,
                     const long long sw = s_bisect_state(q, t, S, i_local, SW, lane, 0, true, &rounds, true);
                     if (lane == 0) {
                         add_steps(BIG ? kStepTailBBis : kStepTailBis, (unsigned long long)n * (unsigned long long)sw);
+                        add_steps(kStepFallbacks, 1);
                         sq = q;
                     }
                 }
@@ -4086,7 +4170,10 @@ __device__ __forceinline__ void k_s_collect_one(int bx, const WaveTask* __restri
         atomicAdd(&cnt->nS, __popcll(m_on));
         if (m_s) b_s = atomicAdd(&cnt->nD, __popcll(m_s));
         if (m_b) b_b = atomicAdd(&cnt->nDb, __popcll(m_b));
-        if (m_i) b_i = atomicAdd(&cnt->nEig, __popcll(m_i));
+        if (m_i) {
+            b_i = atomicAdd(&cnt->nEig, __popcll(m_i));
+            add_steps(kStepFallbacks, (unsigned long long)__popcll(m_i));
+        }
     }
     b_on = __shfl(b_on, lead);
     b_s = __shfl(b_s, lead);

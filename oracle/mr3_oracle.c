@@ -18,6 +18,16 @@
 #include <stdlib.h>
 #include <string.h>
 
+/* Singletons that fell back to bisection (process_singleton.c:186-204) since the last
+ * oracle_fallbacks(1) call: a statistic that tests compare with the device's count. */
+static long long o_fallbacks = 0;
+long long oracle_fallbacks(int reset)
+{
+    const long long v = o_fallbacks;
+    if (reset) o_fallbacks = 0;
+    return v;
+}
+
 #define EPS     DBL_EPSILON      /* odmch.f 'P' = EPSILON(0d0)            */
 #define SAFMIN  DBL_MIN          /* odmch.f 'S' and 'U' = TINY(0d0)       */
 
@@ -1533,6 +1543,7 @@ static void o_process_s_task(o_vec *v, o_task *sng)
         memset(&Z[zind * ldz], 0, sizeof(double) * (size_t)n);
         for (int k = 1; k <= MAXITER_S; k++) {
             if (needBS) {
+                ++o_fallbacks;
                 usedBS = 1;
                 int itmp = r, offset = Windex[i] - 1;
                 double tmp = Wgap[i];

@@ -43,6 +43,8 @@ int oracle_dstemr(char jobz, char range, int n, double *D, double *E, double vl,
                   int nzc, int *isuppz, int *tryrac);
 
 /* Kernel-level entry points (1-based index arguments as in the Fortran). */
+/* singletons that fell back to bisection since the last reset (a test statistic) */
+long long oracle_fallbacks(int reset);
 int    oracle_dlaneg(int n, const double *d, const double *lld, double sigma,
                      double pivmin, int r);                         /* odneg.f  */
 int    oracle_sturm_count_T(int n, const double *d, const double *e2,

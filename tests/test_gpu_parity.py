@@ -632,3 +632,117 @@ def test_gpu_too_few_columns_fail_without_writing(M):
         assert torch.equal(keep[i]["W"][:m], k1[0]["W"][:m]), f"W differs for problem {i}"
         assert torch.equal(keep[i]["Z"][:m], k1[0]["Z"][:m]), f"Z differs for problem {i}"
         assert torch.equal(keep[i]["Zsupp"][:2 * m], k1[0]["Zsupp"][:2 * m])
+
+
+def test_gpu_group_bisection_walk_on_glued_7_blocks(oracle, M):
+    """The multisection walk that the group refinements and the singleton bisection share
+    (dlarrb_grp: a tree of 2^d - 1 midpoints per sweep, then the reference's one-point
+    test walked down it) gives the bits of one-point bisection: matrices of n >= 500
+    made of glued 7-row Wilkinson blocks (clusters of every size, down to single 7-row
+    blocks when the glue is zero) are solved with the cluster refinements on one lane per
+    eigenvalue (a lane budget of 4096) and on groups of up to 64 lanes (2^22); W, Z and
+    Zsupp must be the same bit for bit, and within the oracle's bars."""
+    import torch
+    dev = torch.device("cuda:0")
+    rng = np.random.default_rng(707)
+    w7 = np.abs(3.0 - np.arange(7.0))
+    mats = []
+    for n, glue in ((504, 1e-10), (707, 1e-8), (1001, 0.0), (560, 1e-6)):
+        nb = n // 7
+        D = np.tile(w7, nb) * (1.0 + 1e-12 * rng.standard_normal(nb * 7))
+        E = np.ones(nb * 7)
+        E[6::7] = glue * rng.uniform(0.5, 1.0, nb)
+        E[-1] = 0.0
+        mats.append((D, E))
+    before = M.get_tuning()
+    outs = []
+    try:
+        for lanes in (4096, 1 << 22):
+            M.set_tuning(**dict(before, group_lanes=lanes, rrf_group_max=0 if lanes == 4096 else 1 << 20))
+            print("group_lanes", lanes, flush=True)
+            probs = [dict(D=torch.tensor(D, device=dev), E=torch.tensor(E, device=dev)) for D, E in mats]
+            rc, arr, keep = M.solve_batch(probs)
+            torch.cuda.synchronize()
+            assert rc == 0
+            outs.append((arr, keep))
+    finally:
+        M.set_tuning(**before)
+    (a0, k0), (a1, k1) = outs
+    for i, (D, E) in enumerate(mats):
+        m = a0[i].m
+        assert a1[i].m == m
+        assert torch.equal(k0[i]["W"][:m], k1[i]["W"][:m]), f"matrix {i}: W differs"
+        assert torch.equal(k0[i]["Z"][:m], k1[i]["Z"][:m]), f"matrix {i}: Z differs"
+        assert torch.equal(k0[i]["Zsupp"][:2 * m], k1[i]["Zsupp"][:2 * m])
+        g = dict(info=a0[i].info, m=m, W=k0[i]["W"][:m].cpu().numpy(),
+                 Z=k0[i]["Z"][:m].cpu().numpy().T, Zsupp=k0[i]["Zsupp"][:2 * m].cpu().numpy())
+        o = call_mrrr(oracle[1], "V", "A", D, E)
+        check_pair(D, E, g, o, f"glued7 n={len(D)}")
+
+
+def test_gpu_bisection_fallbacks_match_oracle(oracle, oracle_lib, M):
+    """Range I and V on the task-parallel route give the oracle's eigenvalues bit for
+    bit, and the singletons that fall back to bisection (process_singleton.c:186-204)
+    are the reference's: on N(0,1) tridiagonals of the subset workload's kind (an index
+    window of 10% of the spectrum, and a value window), W equals the oracle's byte for
+    byte and the device's count of fallbacks (work counter `fallbacks`) equals the
+    oracle's on the same matrices.  Z and Zsupp equal the oracle's too, except where a
+    support end differs by entries below 1e-15 (one column of the 1160 here: the cut
+    test of odr1v.f:300-330 on values of about 1e-20)."""
+    import ctypes
+    import torch
+    from mr3smp_amd import workloads as WL
+    dev = torch.device("cuda:0")
+    rng = np.random.default_rng(4242)
+    calls = []
+    for n in (4096, 5200, 1200):
+        D, E = WL.normal(n, rng)
+        lo = int(rng.integers(1, n - n // 10))
+        calls.append((D, E, "I", dict(il=lo, iu=lo + (n + 9) // 10 - 1, mcols=(n + 9) // 10)))
+        med = float(np.median(D))
+        calls.append((D, E, "V", dict(vl=med - 0.25, vu=med + 0.2)))
+    oracle_lib.oracle_fallbacks.restype = ctypes.c_longlong
+    oracle_lib.oracle_fallbacks.argtypes = [ctypes.c_int]
+    oracle_lib.oracle_fallbacks(1)
+    refs = []
+    for D, E, rg, kw in calls:
+        o = call_mrrr(oracle[1], "V", rg, D, E, **kw)
+        assert o["info"] == 0
+        refs.append(o)
+    want = int(oracle_lib.oracle_fallbacks(1))
+    probs = []
+    for (D, E, rg, kw), o in zip(calls, refs):
+        mc = kw.get("mcols", o["m"] + 4)
+        q = {k: v for k, v in kw.items() if k != "mcols"}
+        probs.append(dict(D=torch.tensor(D, device=dev), E=torch.tensor(E, device=dev), jobz="V",
+                          range=rg, mcols=mc, **q))
+    M.profile_reset()
+    M.profile_enable(True, counters=True)
+    try:
+        rc, arr, keep = M.solve_batch(probs)
+        torch.cuda.synchronize()
+        rep = M.profile_report()
+    finally:
+        M.profile_enable(False)
+        M.profile_reset()
+    assert rc == 0
+    cut_only = 0
+    for i, o in enumerate(refs):
+        m = arr[i].m
+        assert m == o["m"], f"call {i}: m {m} vs {o['m']}"
+        W = keep[i]["W"][:m].cpu().numpy()
+        Z = keep[i]["Z"][:m].cpu().numpy().T
+        Zs = keep[i]["Zsupp"][:2 * m].cpu().numpy()
+        Zo = np.asarray(o["Z"])
+        assert np.array_equal(W.view(np.int64), o["W"][:m].view(np.int64)), f"call {i}: W bits differ"
+        for j in np.where(np.any(Z.view(np.int64) != Zo.view(np.int64), axis=0))[0]:
+            rows = Z[:, j].view(np.int64) != Zo[:, j].view(np.int64)
+            assert np.abs(Z[rows, j]).max() < 1e-15 and np.abs(Zo[rows, j]).max() < 1e-15, \
+                f"call {i} column {j}: Z differs beyond a support cut"
+            cut_only += 1
+            Zs[2 * j:2 * j + 2] = o["Zsupp"][2 * j:2 * j + 2]
+        assert np.array_equal(Zs, o["Zsupp"]), f"call {i}: Zsupp differs"
+    assert cut_only <= 2
+    got = int(rep["steps"]["fallbacks"])
+    print(f"bisection fallbacks: device {got}, oracle {want}", flush=True)
+    assert got == want
