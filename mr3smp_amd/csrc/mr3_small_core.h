@@ -82,6 +82,9 @@ struct Prob {
     int* Zsupp;
 };
 
+struct DqRec;
+struct StageState;
+
 struct Work {
     double *D, *E;        // D, E / root L (overwritten as in the reference)
     double *LD, *LLD;     // D*L, D*L*L of the current representation
@@ -105,10 +108,35 @@ struct Work {
     // eigenvalues to refine
     double *RS, *LW, *LE, *LG, *GD;
     int *GI, *NODEOF, *ITEMS;
+    // the staged form (dstemr_stage_a, the blocks' dqds one lane each, dstemr_stage_b):
+    // each block's qd array at 4 (ib - 1) (4n + 8 doubles), D and E kept across the
+    // stages (2n), the deferred blocks and the state between the stages
+    double *QD, *DSAVE;
+    DqRec* DQR;
+    StageState* STG;
 };
 
 struct Result {
     int m, info, tryrac;
+};
+
+// A block whose dqds (odrre.f:797-804) runs later, by itself: where its qd array is and
+// what dlarre needs to take the eigenvalues back (odrre.f:805-840).  info: dlasq2's.
+struct DqRec {
+    int ib, in, mb, indl, indu, we, mbefore, jblk;
+    int info, pad;
+    double sigma, sgndef;
+};
+
+// What the staged form keeps from its first stage for its second.
+struct StageState {
+    int fin;              // the first stage's result is final
+    int ndq;              // blocks whose dqds is deferred
+    int tryrac, relsplit, dlarre_info, pad;
+    int m, nsplit;        // dlarre's sh.m and sh.nsplit
+    double pivmin;        //   and sh.pivmin
+    double scale, tnrm, rtol1, rtol2, wl, wu;
+    Result res;
 };
 
 // Doubles of LDS and bytes of device memory that carve() hands out for a matrix of
@@ -116,9 +144,11 @@ struct Result {
 constexpr int kVGroup = 32;   // nodes of a tree level handled side by side (dlarrv)
 SM_HD inline long long lds_doubles(int n) { return 8LL * n + 8; }
 SM_HD inline long long slice_bytes(int n, int wks) {
-    const long long dbl = 20LL * n + 10LL * kVGroup + 4LL * kVGroup * n + (long long)wks * (4LL * n + 8);
-    const long long b = dbl * (long long)sizeof(double) +
-                        (13LL * n + 10 + 4LL * kVGroup) * (long long)sizeof(int);
+    const long long dbl = 20LL * n + 10LL * kVGroup + 4LL * kVGroup * n + (long long)wks * (4LL * n + 8) +
+                          6LL * n + 8;
+    const long long ints = 13LL * n + 10 + 4LL * kVGroup;
+    const long long b = dbl * (long long)sizeof(double) + ((ints * (long long)sizeof(int) + 7) & ~7LL) +
+                        (long long)n * (long long)sizeof(DqRec) + (long long)sizeof(StageState);
     return (b + 255) & ~255LL;
 }
 
@@ -149,7 +179,9 @@ SM_HD inline void carve(Work& w, double* lds, char* base, int n, int wks) {
     w.RS = w.GD + 10 * kVGroup;       // kVGroup 4n
     w.WKS = wks;
     w.WKL = w.RS + 4LL * kVGroup * n; // wks (4n + 8)
-    w.ISPLIT = (int*)(w.WKL + (long long)wks * (4 * n + 8));
+    w.QD = w.WKL + (long long)wks * (4 * n + 8);   // 4n + 8
+    w.DSAVE = w.QD + 4 * n + 8;                       // 2n
+    w.ISPLIT = (int*)(w.DSAVE + 2 * n);
     w.IBLOCK = w.ISPLIT + n;
     w.INDEXW = w.IBLOCK + n;
     w.TWIST = w.INDEXW + n;
@@ -160,6 +192,9 @@ SM_HD inline void carve(Work& w, double* lds, char* base, int n, int wks) {
     w.NODEOF = w.DIRTY + n;
     w.ITEMS = w.NODEOF + n;
     w.GI = w.ITEMS + n;               // 4 kVGroup
+    const long long ibytes = ((13LL * n + 10 + 4LL * kVGroup) * (long long)sizeof(int) + 7) & ~7LL;
+    w.DQR = (DqRec*)((char*)w.ISPLIT + ibytes);   // n
+    w.STG = (StageState*)(w.DQR + n);
 }
 
 // ---------------------------------------------------------------------------------
@@ -1891,11 +1926,15 @@ SM_HD inline int dlarrd_team(int tid, int nt, Shared& sh, int irange, int n, dou
 // stream) and its eigenvalues: dqds for range A (dlasq2), otherwise dlarrd's
 // approximations refined by dlarrb.  Returns INFO; m in sh.
 // ---------------------------------------------------------------------------------
+// dqr non-null: the staged form.  A block's dqds is not run here; its qd array goes to
+// qd + 4 (ib - 1), a record to dqr[(*ndq)++], and the running counts move on as if it
+// had succeeded; dlarre_dqds_finish takes the results back once the blocks' dqds ran.
 SM_HD inline int dlarre_team(int tid, int nt, Shared& sh, int irange, int n, double* vl,
                              double* vu, int il, int iu, double* d, double* e, double* e2,
                              double rtol1, double rtol2, int relsplit, double* w,
                              double* werr, double* wgap, int* iblock, int* indexw,
-                             int* isplit, double* gers, double* wk) {
+                             int* isplit, double* gers, double* wk, DqRec* dqr = nullptr,
+                             double* qd = nullptr, int* ndq = nullptr) {
     const int MAXTRY = 6;
     const double FAC = 0.5, FUDGE = 2.0, PERT = 8.0, MAXGROWTH = 64.0, HNDRD = 100.0;
     const double rtl = sqrt(kEps), bsrtol = sqrt(kEps);
@@ -2180,7 +2219,26 @@ SM_HD inline int dlarre_team(int tid, int nt, Shared& sh, int irange, int n, dou
                 wbegin = we + 1;
             }
         } else {
-            if (tid == 0) {
+            if (tid == 0 && dqr) {
+                const int ib = sh.iw[3], in = sh.iw[4], mb = sh.iw[5], we = sh.wbeg;
+                const int iend = ib + in - 1;
+                double* z = qd + 4 * (ib - 1);
+                int j = ib;
+                for (int i = 1; i <= in - 1; ++i) {
+                    z[2 * i - 2] = fabs(d[j - 1]);
+                    z[2 * i - 1] = e[j - 1] * e[j - 1] * z[2 * i - 2];
+                    ++j;
+                }
+                z[2 * in - 2] = fabs(d[iend - 1]);
+                z[2 * in - 1] = 0.0;
+                DqRec& r = dqr[(*ndq)++];
+                r.ib = ib; r.in = in; r.mb = mb; r.indl = sh.i0; r.indu = sh.i1; r.we = we;
+                r.mbefore = m; r.jblk = jblk; r.info = 0; r.pad = 0;
+                r.sigma = sh.w[0]; r.sgndef = sh.w[1];
+                m += sh.i1 - sh.i0 + 1;
+                ibegin = iend + 1;
+                wbegin = we + 1;
+            } else if (tid == 0) {
                 const int ib = sh.iw[3], in = sh.iw[4], mb = sh.iw[5], we = sh.wbeg;
                 const int indl = sh.i0, indu = sh.i1;
                 const double sigma = sh.w[0], sgndef = sh.w[1];
@@ -2235,6 +2293,50 @@ SM_HD inline int dlarre_team(int tid, int nt, Shared& sh, int irange, int n, dou
     return 0;
 }
 
+
+// The end of dlarre's dqds branch (odrre.f:805-840) for the blocks the staged form
+// deferred, in block order, once their dqds ran (dlasq2's INFO in each record): the
+// checks, the eigenvalues unshifted into w in the block's order, iblock, indexw, werr
+// and wgap.  Returns the stop code the inline form would have returned (-5, -6) or 0.
+SM_HD inline int dlarre_dqds_finish(int tid, Shared& sh, const DqRec* dqr, int ndq,
+                                    const double* qd, double vu, double* w, double* werr,
+                                    double* wgap, int* iblock, int* indexw) {
+    if (tid == 0) {
+        sh.stop = 0;
+        for (int q = 0; q < ndq && !sh.stop; ++q) {
+            const DqRec& r = dqr[q];
+            const double* wk = qd + 4 * (r.ib - 1);
+            const int in = r.in, mb = r.mb;
+            if (r.info != 0) { sh.stop = -5; break; }
+            for (int i = 1; i <= in; ++i)
+                if (wk[i - 1] < 0.0) sh.stop = -6;
+            if (sh.stop) break;
+            const double rtol = log((double)in) * 4.0 * kEps;
+            int m = r.mbefore;
+            if (r.sgndef > 0.0) {
+                for (int i = r.indl; i <= r.indu; ++i) {
+                    ++m;
+                    w[m - 1] = wk[in - i];
+                    iblock[m - 1] = r.jblk;
+                    indexw[m - 1] = i;
+                }
+            } else {
+                for (int i = r.indl; i <= r.indu; ++i) {
+                    ++m;
+                    w[m - 1] = -wk[i - 1];
+                    iblock[m - 1] = r.jblk;
+                    indexw[m - 1] = i;
+                }
+            }
+            for (int i = m - mb + 1; i <= m; ++i) werr[i - 1] = rtol * fabs(w[i - 1]);
+            for (int i = m - mb + 1; i <= m - 1; ++i)
+                wgap[i - 1] = fmx(0.0, w[i] - werr[i] - (w[i - 1] + werr[i - 1]));
+            wgap[m - 1] = fmx(0.0, (vu - r.sigma) - (w[m - 1] + werr[m - 1]));
+        }
+    }
+    SM_SYNC();
+    return sh.stop;
+}
 
 // ---------------------------------------------------------------------------------
 // dlarrv (odrrv.f:150-896) for the whole spectrum of every block (DOL = 1, DOU = M as
@@ -2845,9 +2947,13 @@ SM_HD inline int dlarrr(int n, const double* d, const double* e) {
 // for jobz C).  VL, VU, IL, IU are inputs only: dstemr leaves the caller's copies as
 // they were.  D and E are read from the caller and worked on in wv.D, wv.E.
 // ---------------------------------------------------------------------------------
-SM_HD inline Result dstemr_team(int tid, int nt, Shared& sh, const Prob& p, const Work& wv) {
+// The first stage of dstemr: everything up to and including dlarre.  Returns true
+// when res is final (a count query, n <= 2, a failure); otherwise the state the second
+// stage needs is in st (thread 0 writes it).  staged: the blocks' dqds is deferred
+// (dlarre_team with wv.QD and wv.DQR; st.ndq records).
+SM_HD inline bool dstemr_stage_a(int tid, int nt, Shared& sh, const Prob& p, const Work& wv,
+                                 bool staged, Result& res, StageState& st) {
     const int n = p.n;
-    Result res;
     res.m = 0;
     res.info = 0;
     res.tryrac = p.tryrac;
@@ -2862,7 +2968,7 @@ SM_HD inline Result dstemr_team(int tid, int nt, Shared& sh, const Prob& p, cons
         }
         SM_SYNC();
         res.m = sh.m;
-        return res;
+        return true;
     }
     for (int i = tid; i < n; i += nt) {
         D[i] = p.Din[i];
@@ -2904,7 +3010,7 @@ SM_HD inline Result dstemr_team(int tid, int nt, Shared& sh, const Prob& p, cons
         }
         SM_SYNC();
         res.m = sh.m;
-        return res;
+        return true;
     }
     // scaling (odstmr.f:452-469), relative-accuracy test (:479-493)
     const double smlnum = kSafmin / kEps, bignum = 1.0 / smlnum;
@@ -2942,11 +3048,45 @@ SM_HD inline Result dstemr_team(int tid, int nt, Shared& sh, const Prob& p, cons
     double rtol1, rtol2;
     if (!p.wantz) { rtol1 = 4.0 * kEps; rtol2 = 4.0 * kEps; }
     else { rtol1 = sqrt(kEps); rtol2 = fmx(sqrt(kEps) * 5.0e-3, 4.0 * kEps); }
+    int ndq = 0;
     int iinfo = dlarre_team(tid, nt, sh, p.range, n, &wl, &wu, iil, iiu, D, E, wv.E2, rtol1, rtol2,
                             relsplit, p.W, wv.WERR, wv.WGAP, wv.IBLOCK, wv.INDEXW, wv.ISPLIT, wv.GERS,
-                            wv.WK);
+                            wv.WK, staged ? wv.DQR : nullptr, wv.QD, &ndq);
     res.tryrac = tryrac;
     SM_MARK(1);
+    // every thread writes the same values (thread 0 alone knows ndq)
+    if (tid == 0) sh.iw[6] = ndq;
+    SM_SYNC();
+    st.fin = 0; st.ndq = sh.iw[6]; st.tryrac = tryrac; st.relsplit = relsplit;
+    st.dlarre_info = iinfo; st.pad = 0;
+    st.m = sh.m; st.nsplit = sh.nsplit; st.pivmin = sh.pivmin;
+    st.scale = scale; st.tnrm = tnrm; st.rtol1 = rtol1; st.rtol2 = rtol2; st.wl = wl; st.wu = wu;
+    st.res = res;
+    SM_SYNC();
+    return false;
+}
+
+// The second stage: the deferred blocks' dqds results taken back, then dlarrv, dlarrj,
+// unscaling and the final ordering (odstmr.f:500-647).  res and st as the first stage
+// left them; sh.m, sh.nsplit and sh.pivmin as dlarre left them.
+SM_HD inline Result dstemr_stage_b(int tid, int nt, Shared& sh, const Prob& p, const Work& wv,
+                                   Result res, const StageState& st) {
+    const int n = p.n;
+    double* D = wv.D;
+    double* E = wv.E;
+    const double scale = st.scale, tnrm = st.tnrm, rtol1 = st.rtol1, rtol2 = st.rtol2;
+    const double wl = st.wl;
+    const int tryrac = st.tryrac;
+    int iinfo = st.dlarre_info;
+    if (tid == 0) {   // dlarre's results kept in sh (the stages may be separate launches)
+        sh.m = st.m;
+        sh.nsplit = st.nsplit;
+        sh.pivmin = st.pivmin;
+    }
+    SM_SYNC();
+    if (iinfo == 0 && st.ndq > 0)
+        iinfo = dlarre_dqds_finish(tid, sh, wv.DQR, st.ndq, wv.QD, st.wu, p.W, wv.WERR, wv.WGAP,
+                                   wv.IBLOCK, wv.INDEXW);
     if (iinfo != 0) { res.info = 10 + (iinfo < 0 ? -iinfo : iinfo); return res; }
     const int m = sh.m;
     const int nsplit = sh.nsplit;
@@ -3024,4 +3164,12 @@ SM_HD inline Result dstemr_team(int tid, int nt, Shared& sh, const Prob& p, cons
     return res;
 }
 
+
+// dstemr in one piece: both stages, dqds inline.
+SM_HD inline Result dstemr_team(int tid, int nt, Shared& sh, const Prob& p, const Work& wv) {
+    Result res;
+    StageState st;
+    if (dstemr_stage_a(tid, nt, sh, p, wv, false, res, st)) return res;
+    return dstemr_stage_b(tid, nt, sh, p, wv, res, st);
+}
 }  // namespace mr3sm

@@ -64,14 +64,55 @@ struct TeamArg {
     Result r;
 };
 
+// the stage a team thread runs: 0 the whole solve, 1 the first stage of the staged
+// form, 2 its second stage
+static int g_stage = 0;
+static int g_staged = 0;
+
 static void* team_main(void* a) {
     TeamArg* t = (TeamArg*)a;
     t_tid = t->tid;
-    t->r = dstemr_team(t->tid, t->nt, *t->sh, *t->p, *t->w);
+    if (g_stage == 0) {
+        t->r = dstemr_team(t->tid, t->nt, *t->sh, *t->p, *t->w);
+    } else if (g_stage == 1) {
+        if (dstemr_stage_a(t->tid, t->nt, *t->sh, *t->p, *t->w, true, t->r, *t->w->STG))
+            t->w->STG->fin = 1;
+        if (t->w->STG->fin) t->w->STG->res = t->r;
+    } else {
+        t->r = dstemr_stage_b(t->tid, t->nt, *t->sh, *t->p, *t->w, t->w->STG->res, *t->w->STG);
+    }
     return nullptr;
 }
 
+// runs one stage on the team (g_team threads, or the calling thread alone)
+static Result run_team(int stage, Shared& sh, const Prob& p, const Work& w) {
+    const int nt = g_team;
+    g_stage = stage;
+    Result r;
+    if (nt == 1) {
+        TeamArg a;
+        a.tid = 0; a.nt = 1; a.sh = &sh; a.p = &p; a.w = &w;
+        team_main(&a);
+        r = a.r;
+    } else {
+        g_arrived = 0;
+        TeamArg* args = new TeamArg[nt];
+        pthread_t* th = new pthread_t[nt];
+        for (int t = 0; t < nt; ++t) {
+            args[t].tid = t; args[t].nt = nt; args[t].sh = &sh; args[t].p = &p; args[t].w = &w;
+            pthread_create(&th[t], nullptr, team_main, &args[t]);
+        }
+        for (int t = 0; t < nt; ++t) pthread_join(th[t], nullptr);
+        r = args[0].r;
+        delete[] th;
+        delete[] args;
+    }
+    return r;
+}
+
 extern "C" void small_route_set_team(int nt) { g_team = nt < 1 ? 1 : (nt > 1024 ? 1024 : nt); }
+// staged != 0: run the two-stage form with every deferred dqds between the stages
+extern "C" void small_route_set_staged(int staged) { g_staged = staged; }
 
 // Same arguments as mrrr() for the matrices it sends to dstemr (n < 500).
 extern "C" int small_route_mrrr(char* jobz, char* range, int* np, double* D, double* E,
@@ -106,20 +147,28 @@ extern "C" int small_route_mrrr(char* jobz, char* range, int* np, double* D, dou
     Shared sh;
     memset(&sh, 0, sizeof(sh));
     Result r;
-    if (nt == 1) {
-        r = dstemr_team(0, 1, sh, p, w);
+    if (!g_staged) {
+        r = run_team(0, sh, p, w);
     } else {
-        g_arrived = 0;
-        TeamArg* args = new TeamArg[nt];
-        pthread_t* th = new pthread_t[nt];
-        for (int t = 0; t < nt; ++t) {
-            args[t].tid = t; args[t].nt = nt; args[t].sh = &sh; args[t].p = &p; args[t].w = &w;
-            pthread_create(&th[t], nullptr, team_main, &args[t]);
+        // the staged form as the device runs it: the first stage, D and E kept, the
+        // team's LDS and shared record lost (poisoned), every deferred block's dqds on
+        // its own, then the second stage with D and E back
+        run_team(1, sh, p, w);
+        if (w.STG->fin) {
+            r = w.STG->res;
+        } else {
+            memcpy(w.DSAVE, w.D, sizeof(double) * n);
+            memcpy(w.DSAVE + n, w.E, sizeof(double) * n);
+            memset(lds, pat, (size_t)lds_doubles(n) * sizeof(double));
+            memset(&sh, pat, sizeof(sh));
+            for (int q = 0; q < w.STG->ndq; ++q) {
+                DqRec& rec = w.DQR[q];
+                rec.info = dlasq2(rec.in, w.QD + 4 * (rec.ib - 1));
+            }
+            memcpy(w.D, w.DSAVE, sizeof(double) * n);
+            memcpy(w.E, w.DSAVE + n, sizeof(double) * n);
+            r = run_team(2, sh, p, w);
         }
-        for (int t = 0; t < nt; ++t) pthread_join(th[t], nullptr);
-        r = args[0].r;
-        delete[] th;
-        delete[] args;
     }
     if (getenv("SMALL_ROUTE_PASSES")) fprintf(stderr, "n=%d passes=%d\n", n, sh.passes);
     free(lds);

@@ -40,9 +40,10 @@ def _calls(D, E, rng):
             ("V", "A", dict(tryrac=0)))
 
 
-def _check(small_cpu, oracle, D, E, label, rng, team=1):
+def _check(small_cpu, oracle, D, E, label, rng, team=1, staged=0):
     lib, fn = small_cpu
     lib.small_route_set_team(team)
+    lib.small_route_set_staged(staged)
     for jobz, rg, kw in _calls(D, E, rng):
         g = call_mrrr(fn, jobz, rg, D, E, **kw)
         o = call_mrrr(oracle[1], jobz, rg, D, E, **kw)
@@ -86,10 +87,25 @@ def test_small_route_mixed_workload_team256(small_cpu, oracle):
         _check(small_cpu, oracle, D, E, f"{i} {kind} n={len(D)}", np.random.default_rng(i), team=256)
 
 
+def test_small_route_staged_team64(small_cpu, oracle):
+    """The two-stage form: the first stage defers every block's dqds (dlarre's
+    root-finding of the block's representation), the team's LDS is lost, each deferred
+    dqds runs on its own and the second stage resumes from the saved state -- the same
+    bits as the one-launch form, on every 48th small matrix of the mixed workload."""
+    from mr3smp_amd import workloads as WL
+    mats = WL.batched_mixed(4096)
+    idx = [i for i, (D, _, _) in enumerate(mats) if len(D) < 500][7::48]
+    for i in idx:
+        D, E, kind = mats[i]
+        _check(small_cpu, oracle, D, E, f"{i} {kind} n={len(D)}", np.random.default_rng(i),
+               team=64, staged=1)
+
+
+@pytest.mark.parametrize("staged", [0, 1])
 @pytest.mark.parametrize("team", [1, 64, 256])
 @pytest.mark.parametrize("name", ["n1", "n2", "n3", "diag", "diag8", "diag16", "zerosD", "split",
                                   "tiny", "huge", "repeated", "relacc", "toeplitz499", "glued499"])
-def test_small_route_edge_cases(name, team, small_cpu, oracle):
+def test_small_route_edge_cases(name, team, staged, small_cpu, oracle):
     from mr3smp_amd import workloads as WL
     rng = np.random.default_rng(5)
     cases = {
@@ -111,4 +127,4 @@ def test_small_route_edge_cases(name, team, small_cpu, oracle):
     D, E = cases[name]
     E = E.copy()
     E[-1] = 0.0
-    _check(small_cpu, oracle, D, E, name, np.random.default_rng(len(D)), team=team)
+    _check(small_cpu, oracle, D, E, name, np.random.default_rng(len(D)), team=team, staged=staged)
